@@ -1,0 +1,204 @@
+/*
+ * mdsim_capi.h - C ABI of the MI355X (gfx950) Brownian/LLG step for magnetic
+ * nanoparticle suspensions.
+ *
+ * The reference program has no plugin or FFI layer: its step is a set of free
+ * C++ functions over Eigen arrays and a global configuration singleton
+ * (src/simulation.h:28-66, src/interactions.h:27-72, src/configuration.h:98).
+ * Each entry point below replaces one of them; the comment on each names the
+ * reference function (file:line) it stands in for.
+ *
+ * Conventions
+ *  - Every call returns an mds_status and never exits the process.
+ *  - Arrays of N x 3 doubles are column-major, exactly as Eigen's ArrayXXd
+ *    stores them in the reference: x[0..N), then y[0..N), then z[0..N).
+ *  - Host pointers stay caller-owned and are read/written only during the call.
+ *    Device memory is owned by the context.
+ *  - One host thread per context; contexts on one device are independent.
+ */
+#ifndef MDSIM_CAPI_H_
+#define MDSIM_CAPI_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MDS_OK = 0,
+    MDS_EINVAL = -1,      /* bad argument or state */
+    MDS_ENOMEM = -2,      /* device or host allocation failed */
+    MDS_EDEVICE = -3,     /* HIP runtime error / no device */
+    MDS_ECONFIG = -4,     /* configuration file missing or malformed */
+    MDS_ENOCONV = -5,     /* step rejected at deltaTmin (the reference loops forever here) */
+    MDS_EOVERLAP = -6,    /* overlap correction moved a particle further than the contact search allows */
+    MDS_ENONFINITE = -7,  /* NaN/Inf found in the state */
+    MDS_ECAPACITY = -8    /* a fixed device capacity was exceeded */
+} mds_status;
+
+/* Solver ids of [adaptive_timestepping_solver] SOLVER (src/configuration.h:8-10). */
+enum { MDS_HEUN_EULER = 0, MDS_BOGACKI_SHAMPINE = 1, MDS_DORMAND_PRINCE = 2 };
+/* Arithmetic of the pair and k-space phase terms. */
+enum { MDS_PREC_FP64 = 0, MDS_PREC_MIXED = 1 };
+
+/*
+ * Every key of the reference's configuration.ini (src/configuration.ini:1-79,
+ * read by Configuration::Configuration, src/configuration.cpp:6-86) with the
+ * same meaning, followed by the derived constants that constructor computes,
+ * followed by keys this library adds (all optional; a value <= 0 / -1 keeps
+ * the reference behaviour).
+ */
+typedef struct mds_config {
+    /* [initial_configuration] */
+    int32_t init_config_rot, init_config_trans, read_from_file;
+    /* [size_distribution] */
+    int32_t size_dist;
+    /* [mobilization] */
+    int32_t enable_mobilization;
+    /* [thermal_fluctuations] */
+    int32_t enable_therm_field, enable_therm_torque, enable_therm_force;
+    int64_t seed;
+    /* [interactions] */
+    int32_t enable_coating_pot;
+    /* [adaptive_timestepping_solver] */
+    int32_t solver;
+    /* [particle_properties] */
+    int32_t np_dim;
+    double mag_damp, temp, sat_mag, anis_en;
+    /* [assemble_properties] */
+    double r_mag_mean, d_shell, vol_frac;
+    /* [steric_repulsion], [vdw_attraction], [electrostat_repulsion] */
+    double dens_surf_mol, hamaker, z_valency, epsilon_r, gamma_0, conc;
+    /* [time_settings], [external_magnetic_field_t] */
+    double t_mag, t_relax, delta_t_init, mag_flux_dens;
+    /* [solver_settings], [long_range_interactions], [short_range_interactions], [output_settings] */
+    double abs_tol, delta_t_min, delta_t_max, err_tol_ewald, err_tol_sr, np_coords;
+    /* [physical_constants], [faster_calculation] */
+    double my_pi, kB, gyro_mr, avogadro, epsilon_0, el, fac_ewald;
+
+    /* derived exactly as src/configuration.cpp:39,64,71-84 */
+    int64_t numb_part;          /* npDim^3 unless numbPart is given */
+    double mu0, eff_len_surf_mol, vel_mm_const, anis_const, ster_rep_fac, gamma_el, epsilon,
+        kappa, electrostat_rep_fac, ir_pi, vis, sigma, t_end;
+
+    /* added keys (section [mi355x]) */
+    int64_t numb_part_override; /* numbPart: any N, not only npDim^3 */
+    double viscosity;           /* viscosity: Pa s, overrides the T-derived value */
+    double r_mag_median;        /* rMagMedian: lognormal median core radius */
+    double sigma_log;           /* sigmaLog: lognormal sigma of ln(r) */
+    int32_t fixed_dt;           /* fixedDt: 1 = Heun step at deltaTinit, controller bypassed */
+    int32_t precision;          /* precision: MDS_PREC_FP64 or MDS_PREC_MIXED */
+    double ewald_alpha_l;       /* ewaldAlphaL: alpha * L override */
+    double ewald_rc_frac;       /* ewaldRc: r_c / L override */
+    int32_t ewald_nc;           /* ewaldNc: k-space cut-off override */
+} mds_config;
+
+/* Parse an ini file with the reference's keys (Configuration::Configuration,
+ * src/configuration.cpp:6-86; inih rules of src/INIReader.h). Missing keys keep
+ * the reference's default of -1/false. Returns MDS_ECONFIG on a missing or
+ * malformed file instead of printing and continuing. */
+mds_status mds_config_from_ini(const char* path, mds_config* out);
+/* Same, from ini text in memory. */
+mds_status mds_config_from_string(const char* text, mds_config* out);
+/* Recompute the derived constants after fields were edited. */
+void mds_config_derive(mds_config* cfg);
+
+/* Per-particle properties from core radii: InitParticleProps,
+ * src/simulation.cpp:118-214 (radii are inputs because the reference draws
+ * them from its own RNG). Outputs may be NULL. */
+mds_status mds_particle_props(const mds_config* cfg, int64_t n, const double* r_mag, double* r_hydr,
+                              double* vol_mag, double* mag_mom, double* zeta_rot, double* zeta_trans);
+/* Box edge from sum of hydrodynamic volumes (src/simulation.cpp:56). */
+double mds_box_length(const mds_config* cfg, int64_t n, const double* r_hydr);
+/* Short-range cut-off: InitCutOffSR, src/interaction.cpp:20-57. */
+mds_status mds_cutoff_sr(const mds_config* cfg, int64_t n, const double* r_mag, const double* r_hydr,
+                         const double* mag_mom, double* r_cut_sr);
+/* Closed-form rms error models, FunRealError / FunRezError
+ * (src/interaction.cpp:330-347), for alpha = x (1/m) or k_c = x. */
+double mds_ewald_real_error(const mds_config* cfg, int64_t n, double M, double f_al, double l_box,
+                            double r_cut, double x);
+double mds_ewald_rez_error(const mds_config* cfg, int64_t n, double M, double f_al, double l_box,
+                           double alpha, double x);
+/* alpha and k_c for a given r_c by the reference's root scans
+ * (src/interaction.cpp:192-257); k_c is returned unrounded. */
+mds_status mds_ewald_params_for_rc(const mds_config* cfg, int64_t n, const double* mag_mom,
+                                   const double* r_hydr, double l_box, double r_cut, double* alpha,
+                                   double* kc);
+
+/* ----------------------------------------------------------------------- */
+/* Device context                                                          */
+/* ----------------------------------------------------------------------- */
+typedef struct mds_ctx mds_ctx;
+
+/* Create a context on HIP device `device` (one HIP stream per context). */
+mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out);
+void mds_destroy(mds_ctx* ctx);
+const char* mds_last_error(const mds_ctx* ctx);
+/* Number of HIP devices visible (0 when none); no context needed. */
+int mds_device_count(void);
+
+/* Particle radii and box: InitParticleProps (src/simulation.cpp:118-214) and
+ * the box edge (56), plus InitCutOffSR (src/interaction.cpp:20-57). l_box <= 0
+ * derives it from the volume fraction as the reference does. */
+mds_status mds_set_particles(mds_ctx* ctx, int64_t n, const double* r_mag, const double* r_hydr,
+                             double l_box);
+/* Ewald parameters: InitEwaldConst (src/interaction.cpp:353-374), with alpha,
+ * r_c and n_c given explicitly. */
+mds_status mds_set_ewald(mds_ctx* ctx, double alpha, double r_cut, int32_t nc);
+/* Read back box edge, r_c(SR), alpha, r_c, n_c. */
+mds_status mds_get_params(const mds_ctx* ctx, double* l_box, double* r_cut_sr, double* alpha,
+                          double* r_cut, int32_t* nc);
+/* State: coordsTrans.pos, coordsRot.posMm, coordsRot.posEa (src/typedefs.h:37-47). */
+mds_status mds_set_state(mds_ctx* ctx, const double* pos, const double* pos_mm, const double* pos_ea);
+mds_status mds_get_state(mds_ctx* ctx, double* pos, double* pos_mm, double* pos_ea);
+
+/* Thermal draws for the next step, as DefThermFluct leaves them
+ * (src/simulation.cpp:563-603: already scaled by their sigma). Any NULL
+ * channel is zero. mds_use_philox switches back to in-kernel Philox draws. */
+mds_status mds_set_noise(mds_ctx* ctx, const double* therm_torque, const double* therm_field,
+                         const double* therm_force);
+mds_status mds_use_philox(mds_ctx* ctx, uint64_t seed, uint64_t counter);
+
+/* CompInteractions (src/simulation.cpp:628-680) at time t: moves overlapping
+ * pairs, returns force (incl. thermal force), demag field, pressure sums
+ * (Pxy, Pxz, Pyz) and the corrected positions. Outputs may be NULL. */
+mds_status mds_comp_interactions(mds_ctx* ctx, double* force, double* field, double* press,
+                                 double* pos_after);
+/* CompDemagField (src/simulation.cpp:688-697) for stage moments pos_mm,
+ * reusing the pair geometry of the last mds_comp_interactions. */
+mds_status mds_comp_demag_field(mds_ctx* ctx, const double* pos_mm, double* field);
+/* RKCoeffs (src/simulation.cpp:876-902) for the given orientations, demag
+ * field and the current thermal draws; t selects the external field. */
+mds_status mds_rk_coeffs(mds_ctx* ctx, double t, const double* pos_mm, const double* pos_ea,
+                         const double* field, double* k_mm, double* k_ea);
+
+/* One iteration of RunSimulation's loop (src/simulation.cpp:466-520):
+ * DefExtField, DefThermFluct, EvalSinCos, CompInteractions, IntegrationRot
+ * (with AdjustTimeStep), IntegrationTrans, ApplyBoundaryCondArr, <m_z>.
+ * t and dt are updated as the reference updates tInt and deltaT. */
+mds_status mds_step(mds_ctx* ctx, double* t, double* dt, double* dt_used, double* mz);
+/* `steps` iterations; with fixedDt they are enqueued back to back on the
+ * context's stream (captured once into a HIP graph and replayed). mz_series
+ * (may be NULL) receives <m_z> after each step. */
+mds_status mds_run(mds_ctx* ctx, int64_t steps, double* t, double* dt, double* mz_series);
+/* Wait for queued work. */
+mds_status mds_synchronize(mds_ctx* ctx);
+
+/* Kernel timing by HIP events on the context's stream. Enabled contexts
+ * record an event pair around each launch; mds_kernel_stats returns, per
+ * kernel class id (0..count-1), launches and total milliseconds. */
+mds_status mds_enable_timing(mds_ctx* ctx, int32_t on);
+int32_t mds_kernel_class_count(void);
+const char* mds_kernel_class_name(int32_t id);
+mds_status mds_kernel_stats(mds_ctx* ctx, int64_t* launches, double* total_ms);
+/* Work counters of the last evaluation: unique pairs inside r_c, candidate
+ * pairs visited, k-vectors in use, overlap events. */
+mds_status mds_work_counts(mds_ctx* ctx, int64_t* pairs, int64_t* candidates, int64_t* kvecs,
+                           int64_t* overlap_events);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDSIM_CAPI_H_ */
