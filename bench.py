@@ -1,0 +1,174 @@
+#!/usr/bin/env python
+"""Particle-steps/s of the Brownian/LLG step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): N = 100,000
+magnetite particles, lognormal core radius (median 10 nm, sigma_ln = 0.25)
+plus a 5 nm shell, volume fraction 10 % in a periodic cube, 50 mT along z,
+Gilbert damping 0.05, T = 300 K, water (1e-3 Pa s), Ewald tuned to 1e-6 with
+the reference's error model, fixed dt = 1e-12 s Heun steps, in-kernel Philox
+thermal noise. Inputs are resident in HBM before the timed region.
+
+One JSON line on stdout; progress on stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic flop counts per unit of work (exp, erfc, sqrt, div count 1)
+FLOP_PAIR_FORCE = 94     # real-space pair, force + field + pressure
+FLOP_PAIR_FIELD = 62     # real-space pair, field only
+FLOP_K_BUILD = 21        # S(k) build per (k, particle)
+FLOP_K_SCATTER_F = 38    # force + field scatter per (k, particle)
+FLOP_K_SCATTER_B = 22    # field-only scatter per (k, particle)
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_context(mds, workload, precision, rc_frac, seed):
+    cfg, r, rh, L, pos, mm, ea = workload.system("c3", seed=seed, precision=precision)
+    mu = cfg.sat_mag * 4.0 / 3.0 * cfg.my_pi * r ** 3
+    alpha, rc, nc = workload.ewald_for(cfg, mu, rh, L, rc_frac)
+    ctx = mds.Context(cfg)
+    ctx.set_particles(r, rh, L)
+    ctx.set_ewald(alpha, rc, nc)
+    ctx.set_state(pos, mm, ea)
+    ctx.use_philox(seed, 0)
+    sysinfo = {"n": int(cfg.numb_part), "L": L, "alpha": alpha, "rc": rc, "nc": nc, "cfg": cfg, "r": r, "rh": rh,
+               "pos": pos, "mm": mm, "ea": ea, "mu": mu}
+    return ctx, sysinfo
+
+
+def time_path(ctx, steps, warmup):
+    t, dt = 0.0, 1e-12
+    t, dt, _ = ctx.run(warmup, t, dt)
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    t, dt, _ = ctx.run(steps, t, dt)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    stats = ctx.kernel_stats()
+    counts = ctx.work_counts()
+    ctx.enable_timing(False)
+    return t1 - t0, stats, counts
+
+
+def roofline(stats, counts, n, nk, steps):
+    """Dominant kernel class, its algorithmic flops per launch over its
+    event-timed average duration."""
+    lf, _ = stats["pairs_force"]
+    lb, _ = stats["pairs_field"]
+    # each unique pair is evaluated once from each of its particles
+    pairs_per_launch = counts["pair_evals"] / 2 / max(1, lf + lb)
+    per_launch = {
+        "pairs_force": FLOP_PAIR_FORCE * pairs_per_launch,
+        "pairs_field": FLOP_PAIR_FIELD * pairs_per_launch,
+        "kbuild": FLOP_K_BUILD * nk * n,
+        "kscatter_force": FLOP_K_SCATTER_F * nk * n,
+        "kscatter_field": FLOP_K_SCATTER_B * nk * n,
+    }
+    dom = max(per_launch, key=lambda k: stats[k][1])
+    launches, total_ms = stats[dom]
+    avg_s = total_ms / 1e3 / max(1, launches)
+    achieved = per_launch[dom] / avg_s / 1e12
+    breakdown = {k: {"launches": v[0], "ms_per_step": v[1] / steps} for k, v in stats.items() if v[0]}
+    return {"kernel": dom, "bound": "mfma", "compute_unit": "FP64 vector ALU (no dense contraction: MFMA unused; "
+            "MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+            "avg_launch_ms": avg_s * 1e3, "flop_per_launch": per_launch[dom],
+            "unique_pairs_per_launch": pairs_per_launch, "kvecs": nk}, breakdown
+
+
+def cpu_baseline(sysinfo, threads):
+    """One Heun step of the same N = 1e5 system with the CPU oracle (the
+    reference's algorithm restated: O(N^2) pair scan, k-space two-pass sums)."""
+    import oracle
+    from tests.refcfg import derived, parse_ini
+    from md_sim_wolfschwenger_amd import workload
+    cfg = sysinfo["cfg"]
+    n = sysinfo["n"]
+    c = derived(parse_ini(workload.config_text("c3")))
+    c["fixed_dt"] = 1
+    r, rh = sysinfo["r"], sysinfo["rh"]
+    props = {"r_mag": r, "r_hydr": rh, "vol_mag": 4 / 3 * cfg.my_pi * r ** 3, "mag_mom": sysinfo["mu"],
+             "zeta_rot": 8 * cfg.my_pi * cfg.vis * rh ** 3, "zeta_trans": 6 * cfg.my_pi * cfg.vis * rh}
+    s = oracle.System(n, sysinfo["L"], sysinfo["alpha"], sysinfo["rc"], 2 * r.max(), sysinfo["nc"], props, c,
+                      threads=threads)
+    s.p.fixed_dt = 1
+    rng = np.random.default_rng(0)
+    z = rng.standard_normal((n, 3)) * 1e-12
+    t0 = time.perf_counter()
+    out = s.step(sysinfo["pos"], sysinfo["mm"], sysinfo["ea"], 0.0, 1e-12, z, z * 1e-10, z)
+    t1 = time.perf_counter()
+    assert out["status"] == 0
+    return n / (t1 - t0), t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rc-frac", type=float, default=0.3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
+              "(the path is single-GPU; multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    import md_sim_wolfschwenger_amd as mds
+    from md_sim_wolfschwenger_amd import workload
+
+    log("building the N=1e5 workload")
+    ctx, info = make_context(mds, workload, "fp64", args.rc_frac, args.seed)
+    n, nk = info["n"], None
+    params = ctx.params()
+    nk = ctx.work_counts()["kvecs"]
+    log(f"L={info['L']:.4e} alpha*L={info['alpha'] * info['L']:.2f} rc/L={args.rc_frac} nc={info['nc']} nk={nk}")
+    el, stats, counts = time_path(ctx, args.steps, args.warmup)
+    value = n * args.steps / el
+    roof, breakdown = roofline(stats, counts, n, nk, args.steps)
+    log(f"fp64: {value:.4e} particle-steps/s, {el / args.steps * 1e3:.3f} ms/step")
+    for k, v in breakdown.items():
+        log(f"  {k:16s} {v['ms_per_step']:8.3f} ms/step ({v['launches']} launches)")
+    ctx.close()
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline: one oracle Heun step at N={n} on {args.cpu_threads} threads")
+        cv, cs = cpu_baseline(info, args.cpu_threads)
+        cpu = {"value": cv, "unit": "particle-steps/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"one fixed-dt Heun step of the same N={n} system ({cs:.1f} s), CPU oracle "
+                         f"(the reference's O(N^2) pair scan and two-pass k sums restated in C++, OpenMP)"}
+    line = {
+        "metric": "particle-steps/s at N=1e5 mixed vs FP64; p50/p99 ensemble-request latency",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "C3: N=100000 lognormal core (median 10 nm radius, sigma_ln 0.25) + 5 nm shell, "
+                               "phi=0.10, B=50 mT z, alpha_G=0.05, T=300 K, eta=1e-3 Pa s, Ewald 1e-6, "
+                               "fixed dt=1e-12 s Heun, Philox noise",
+                   "n": n, "precision": "fp64", "rc_over_L": args.rc_frac, "alpha_L": info["alpha"] * info["L"],
+                   "nc": info["nc"], "kvecs": nk},
+        "fp64_value": value, "mixed_value": None,
+        "roofline": roof, "kernel_ms_per_step": breakdown, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -103,6 +103,8 @@ mds_status mds_config_from_ini(const char* path, mds_config* out);
 mds_status mds_config_from_string(const char* text, mds_config* out);
 /* Recompute the derived constants after fields were edited. */
 void mds_config_derive(mds_config* cfg);
+/* sizeof(mds_config), for FFI bindings to check their struct layout. */
+int64_t mds_config_sizeof(void);
 
 /* Per-particle properties from core radii: InitParticleProps,
  * src/simulation.cpp:118-214 (radii are inputs because the reference draws

@@ -1,0 +1,266 @@
+"""ctypes binding of libmdsim.so (include/mdsim_capi.h).
+
+The product is the C ABI; this module is the thin Python mirror of it that
+the tests and bench.py drive. Arrays cross the boundary as (N, 3) float64
+numpy arrays here and as column-major N x 3 buffers in C, the layout of the
+reference's Eigen arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdsim.so")
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+STATUS = {0: "MDS_OK", -1: "MDS_EINVAL", -2: "MDS_ENOMEM", -3: "MDS_EDEVICE", -4: "MDS_ECONFIG",
+          -5: "MDS_ENOCONV", -6: "MDS_EOVERLAP", -7: "MDS_ENONFINITE", -8: "MDS_ECAPACITY"}
+HEUN_EULER, BOGACKI_SHAMPINE, DORMAND_PRINCE = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class MdsError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    """Mirror of mds_config (field order and types as in the header)."""
+    _fields_ = [
+        ("init_config_rot", C.c_int32), ("init_config_trans", C.c_int32), ("read_from_file", C.c_int32),
+        ("size_dist", C.c_int32), ("enable_mobilization", C.c_int32),
+        ("enable_therm_field", C.c_int32), ("enable_therm_torque", C.c_int32), ("enable_therm_force", C.c_int32),
+        ("seed", C.c_int64), ("enable_coating_pot", C.c_int32), ("solver", C.c_int32), ("np_dim", C.c_int32),
+        ("mag_damp", C.c_double), ("temp", C.c_double), ("sat_mag", C.c_double), ("anis_en", C.c_double),
+        ("r_mag_mean", C.c_double), ("d_shell", C.c_double), ("vol_frac", C.c_double),
+        ("dens_surf_mol", C.c_double), ("hamaker", C.c_double), ("z_valency", C.c_double),
+        ("epsilon_r", C.c_double), ("gamma_0", C.c_double), ("conc", C.c_double),
+        ("t_mag", C.c_double), ("t_relax", C.c_double), ("delta_t_init", C.c_double),
+        ("mag_flux_dens", C.c_double),
+        ("abs_tol", C.c_double), ("delta_t_min", C.c_double), ("delta_t_max", C.c_double),
+        ("err_tol_ewald", C.c_double), ("err_tol_sr", C.c_double), ("np_coords", C.c_double),
+        ("my_pi", C.c_double), ("kB", C.c_double), ("gyro_mr", C.c_double), ("avogadro", C.c_double),
+        ("epsilon_0", C.c_double), ("el", C.c_double), ("fac_ewald", C.c_double),
+        ("numb_part", C.c_int64),
+        ("mu0", C.c_double), ("eff_len_surf_mol", C.c_double), ("vel_mm_const", C.c_double),
+        ("anis_const", C.c_double), ("ster_rep_fac", C.c_double), ("gamma_el", C.c_double),
+        ("epsilon", C.c_double), ("kappa", C.c_double), ("electrostat_rep_fac", C.c_double),
+        ("ir_pi", C.c_double), ("vis", C.c_double), ("sigma", C.c_double), ("t_end", C.c_double),
+        ("numb_part_override", C.c_int64), ("viscosity", C.c_double), ("r_mag_median", C.c_double),
+        ("sigma_log", C.c_double), ("fixed_dt", C.c_int32), ("precision", C.c_int32),
+        ("ewald_alpha_l", C.c_double), ("ewald_rc_frac", C.c_double), ("ewald_nc", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# (name, restype, argtypes) of every exported entry point of the header
+SIGNATURES = [
+    ("mds_config_from_ini", C.c_int, [C.c_char_p, C.POINTER(Config)]),
+    ("mds_config_from_string", C.c_int, [C.c_char_p, C.POINTER(Config)]),
+    ("mds_config_derive", None, [C.POINTER(Config)]),
+    ("mds_config_sizeof", C.c_int64, []),
+    ("mds_particle_props", C.c_int, [C.POINTER(Config), C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]),
+    ("mds_box_length", C.c_double, [C.POINTER(Config), C.c_int64, _dp]),
+    ("mds_cutoff_sr", C.c_int, [C.POINTER(Config), C.c_int64, _dp, _dp, _dp, _dp]),
+    ("mds_ewald_real_error", C.c_double, [C.POINTER(Config), C.c_int64, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_double]),
+    ("mds_ewald_rez_error", C.c_double, [C.POINTER(Config), C.c_int64, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_double]),
+    ("mds_ewald_params_for_rc", C.c_int, [C.POINTER(Config), C.c_int64, _dp, _dp, C.c_double, C.c_double, _dp,
+                                          _dp]),
+    ("mds_create", C.c_int, [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]),
+    ("mds_destroy", None, [C.c_void_p]),
+    ("mds_last_error", C.c_char_p, [C.c_void_p]),
+    ("mds_device_count", C.c_int, []),
+    ("mds_set_particles", C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, C.c_double]),
+    ("mds_set_ewald", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32]),
+    ("mds_get_params", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)]),
+    ("mds_set_state", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("mds_get_state", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("mds_set_noise", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("mds_use_philox", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
+    ("mds_comp_interactions", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    ("mds_comp_demag_field", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("mds_rk_coeffs", C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp]),
+    ("mds_step", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    ("mds_run", C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, _dp]),
+    ("mds_synchronize", C.c_int, [C.c_void_p]),
+    ("mds_enable_timing", C.c_int, [C.c_void_p, C.c_int32]),
+    ("mds_kernel_class_count", C.c_int32, []),
+    ("mds_kernel_class_name", C.c_char_p, [C.c_int32]),
+    ("mds_kernel_stats", C.c_int, [C.c_void_p, _i64p, _dp]),
+    ("mds_work_counts", C.c_int, [C.c_void_p, _i64p, _i64p, _i64p, _i64p]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libmdsim.so; raises if it is not built (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run `make -C {HERE}` or __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        if L.mds_config_sizeof() != C.sizeof(Config):
+            raise ImportError("mds_config layout mismatch between libmdsim.so and capi.Config")
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def cm(a) -> np.ndarray:
+    """(N,3) -> contiguous column-major flat buffer."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).T).reshape(-1)
+
+
+def rm(flat, n) -> np.ndarray:
+    return flat.reshape(3, n).T.copy()
+
+
+def config_from_string(text: str) -> Config:
+    c = Config()
+    st = lib().mds_config_from_string(text.encode(), C.byref(c))
+    if st:
+        raise MdsError(st, "configuration text")
+    return c
+
+
+def config_from_ini(path: str) -> Config:
+    c = Config()
+    st = lib().mds_config_from_ini(path.encode(), C.byref(c))
+    if st:
+        raise MdsError(st, path)
+    return c
+
+
+def device_count() -> int:
+    return lib().mds_device_count()
+
+
+class Context:
+    """One device context (mds_ctx)."""
+
+    def __init__(self, cfg: Config, device: int = 0):
+        self._L = lib()
+        self.cfg = cfg
+        h = C.c_void_p()
+        st = self._L.mds_create(C.byref(cfg), device, C.byref(h))
+        if st:
+            raise MdsError(st, "mds_create")
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            self._L.mds_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st, what):
+        if st:
+            raise MdsError(st, f"{what}: {self._L.mds_last_error(self.h).decode()}")
+
+    def set_particles(self, r_mag, r_hydr=None, l_box=0.0):
+        r_mag = np.ascontiguousarray(r_mag, dtype=np.float64)
+        rh = None if r_hydr is None else np.ascontiguousarray(r_hydr, dtype=np.float64)
+        self._chk(self._L.mds_set_particles(self.h, len(r_mag), _p(r_mag), _p(rh), float(l_box)), "set_particles")
+        self.n = len(r_mag)
+
+    def set_ewald(self, alpha, r_cut, nc):
+        self._chk(self._L.mds_set_ewald(self.h, float(alpha), float(r_cut), int(nc)), "set_ewald")
+
+    def params(self):
+        v = [C.c_double() for _ in range(4)]
+        nc = C.c_int32()
+        self._chk(self._L.mds_get_params(self.h, *[C.byref(x) for x in v], C.byref(nc)), "get_params")
+        return {"l_box": v[0].value, "r_cut_sr": v[1].value, "alpha": v[2].value, "r_cut": v[3].value,
+                "nc": nc.value}
+
+    def set_state(self, pos, mm, ea):
+        a, b, c = cm(pos), cm(mm), cm(ea)
+        self._chk(self._L.mds_set_state(self.h, _p(a), _p(b), _p(c)), "set_state")
+
+    def get_state(self):
+        n = self.n
+        a, b, c = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+        self._chk(self._L.mds_get_state(self.h, _p(a), _p(b), _p(c)), "get_state")
+        return rm(a, n), rm(b, n), rm(c, n)
+
+    def set_noise(self, n_t=None, n_b=None, n_f=None):
+        arrs = [None if x is None else cm(x) for x in (n_t, n_b, n_f)]
+        self._chk(self._L.mds_set_noise(self.h, *[_p(x) for x in arrs]), "set_noise")
+
+    def use_philox(self, seed, counter=0):
+        self._chk(self._L.mds_use_philox(self.h, int(seed), int(counter)), "use_philox")
+
+    def comp_interactions(self):
+        n = self.n
+        F, B, P, X = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3), np.zeros(3 * n)
+        self._chk(self._L.mds_comp_interactions(self.h, _p(F), _p(B), _p(P), _p(X)), "comp_interactions")
+        return rm(F, n), rm(B, n), P, rm(X, n)
+
+    def comp_demag_field(self, mm):
+        n = self.n
+        m = cm(mm)
+        B = np.zeros(3 * n)
+        self._chk(self._L.mds_comp_demag_field(self.h, _p(m), _p(B)), "comp_demag_field")
+        return rm(B, n)
+
+    def rk_coeffs(self, t, mm, ea, field):
+        n = self.n
+        m, e, b = cm(mm), cm(ea), cm(field)
+        km, ke = np.zeros(3 * n), np.zeros(3 * n)
+        self._chk(self._L.mds_rk_coeffs(self.h, float(t), _p(m), _p(e), _p(b), _p(km), _p(ke)), "rk_coeffs")
+        return rm(km, n), rm(ke, n)
+
+    def step(self, t, dt, want_mz=True):
+        tt, dd, du, mz = (C.c_double(t), C.c_double(dt), C.c_double(), C.c_double())
+        self._chk(self._L.mds_step(self.h, C.byref(tt), C.byref(dd), C.byref(du),
+                                   C.byref(mz) if want_mz else None), "step")
+        return tt.value, dd.value, du.value, (mz.value if want_mz else None)
+
+    def run(self, steps, t, dt, want_mz=False):
+        tt, dd = C.c_double(t), C.c_double(dt)
+        mz = np.zeros(steps) if want_mz else None
+        self._chk(self._L.mds_run(self.h, int(steps), C.byref(tt), C.byref(dd), _p(mz)), "run")
+        return tt.value, dd.value, mz
+
+    def synchronize(self):
+        self._chk(self._L.mds_synchronize(self.h), "synchronize")
+
+    def enable_timing(self, on=True):
+        self._chk(self._L.mds_enable_timing(self.h, 1 if on else 0), "enable_timing")
+
+    def kernel_stats(self):
+        k = self._L.mds_kernel_class_count()
+        la = np.zeros(k, np.int64)
+        ms = np.zeros(k)
+        self._chk(self._L.mds_kernel_stats(self.h, la.ctypes.data_as(_i64p), _p(ms)), "kernel_stats")
+        return {self._L.mds_kernel_class_name(i).decode(): (int(la[i]), float(ms[i])) for i in range(k)}
+
+    def work_counts(self):
+        v = [C.c_int64() for _ in range(4)]
+        self._chk(self._L.mds_work_counts(self.h, *[C.byref(x) for x in v]), "work_counts")
+        return {"pair_evals": v[0].value, "candidates": v[1].value, "kvecs": v[2].value,
+                "overlap_events": v[3].value}

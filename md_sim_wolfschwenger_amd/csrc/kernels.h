@@ -1,0 +1,134 @@
+// kernels.h - launchers of the gfx950 kernels (implemented in kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mds {
+
+// One k-space row: all n = (nx, ny, nz) with the same (ny, nz), nx in
+// [-nxmax, nxmax] (nx in [1, nxmax] for the ny = nz = 0 row). koff is the
+// index of the row's first k-vector.
+struct KRow {
+    int32_t ny, nz, nxmax, koff;
+};
+
+// A segment of a k-space row for the structure-factor build: |nx| in
+// [alo, ahi] (both signs unless the row is ny = nz = 0), at most KSEG values.
+constexpr int KSEG = 8;
+struct KSeg {
+    int32_t ny, nz, alo, ahi, koff, nxmax;
+};
+
+// Per-step geometry of the real-space sum (cell list + overlap versions).
+struct Geometry {
+    int64_t n;
+    double L;
+    // main cell grid
+    int32_t ncell;        // cells per dimension
+    int32_t reach;        // stencil half-width in cells
+    double cell;          // cell edge
+    const int32_t* cell_start;   // [ncell^3]
+    const int32_t* cell_end;     // [ncell^3]
+    const int32_t* sorted_idx;   // [n] original index of sorted slot
+    const double4* rec_pos;      // [n] (x, y, z, rMag) in sorted order, step-start positions
+    const int32_t* work;         // [nwork] packed (cell << 8) | chunk
+    const int32_t* nwork;        // device scalar
+    // overlap-correction versions
+    const int32_t* ev_head;      // [n] first event of particle or -1
+    const int32_t* ev_next;      // [cap]
+    const uint64_t* ev_key;      // [cap] lex key lo*n+hi of the pair that moved it
+    const double* ev_pos;        // [cap*3] position after that event
+    const int32_t* n_events;     // device scalar
+    const double* max_disp;      // device scalar
+};
+
+struct PairOut {
+    double* force;   // [3n] SoA, written (not accumulated); may be null (field only)
+    double* field;   // [3n] SoA, written
+    double* press_partial;  // [nwaves*3]
+};
+
+struct PairParams {
+    double alpha, alpha2, alpha4, ir_pi, fac, rc, rc_sr;
+    // short-range (coating) constants
+    int32_t coating;
+    double eff_len, ster_fac, el_fac, kappa, hamaker;
+    const double* r_mag;   // original order
+};
+
+// Launch a kernel and check the launch.
+hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* pos0, double* pos_cur,
+                          double L, int32_t ncell, double cell, int32_t* key_main, int32_t fncell, double fcell,
+                          int32_t* key_fine);
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v);
+hipError_t launch_fill_f64(hipStream_t s, double* p, int64_t count, double v);
+hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
+                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp);
+hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_idx, const double* pos0,
+                                const double* r_mag, double4* rec_pos, int32_t* inv_perm);
+hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
+                             int32_t* work, int32_t* nwork, int32_t cap, int32_t* err);
+// contact search + sequential overlap resolution
+hipError_t launch_contacts(hipStream_t s, int64_t n, double L, const double* pos0, const double* r_mag,
+                           int32_t fncell, double fcell, const int32_t* fstart, const int32_t* fend,
+                           const int32_t* fsorted, double search_extra, int32_t* cand_count, int32_t* cand_off,
+                           int32_t* cand_j, int32_t cand_cap, int32_t* scan_tmp, int32_t* err);
+hipError_t launch_resolve(hipStream_t s, int64_t n, double L, double rc, const double* r_mag, const int32_t* cand_off,
+                          const int32_t* cand_j, double* pos_cur, const double* pos0, int32_t* ev_head,
+                          int32_t* ev_tail, int32_t* ev_next, uint64_t* ev_key, double* ev_pos, int32_t ev_cap,
+                          int32_t* n_events, double* max_disp, double disp_limit, int32_t* err);
+hipError_t launch_clear_events(hipStream_t s, int64_t n, int32_t* ev_head, int32_t* ev_tail);
+
+hipError_t launch_moments(hipStream_t s, int64_t n, const double* mm, const double* mag_mom, const int32_t* inv_perm,
+                          double* mom, double4* rec_mom, double* msum_partial, int32_t nblk);
+hipError_t launch_reduce_sum3(hipStream_t s, const double* partial, int32_t count, double* out);
+
+hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
+                        const PairParams& pp, const PairOut& out, int32_t max_work, int64_t* cnt_pairs);
+
+// k-space
+hipError_t launch_sincos_tables(hipStream_t s, int64_t n, int32_t nc, double L, double two_pi, const double* pos0,
+                                double* tab);
+hipError_t launch_kbuild(hipStream_t s, int32_t precision, int64_t n, int32_t nc, int32_t nseg, const KSeg* segs,
+                         const double* tab, const double* mom, int32_t chunk, int32_t nchunks, int32_t nk,
+                         double* partial);
+hipError_t launch_kreduce(hipStream_t s, int32_t nk, int32_t nchunks, const double* partial, double* sk);
+hipError_t launch_kscatter(hipStream_t s, bool force, int32_t precision, int64_t n, int32_t nc, int32_t nrows,
+                           const KRow* rows, const double* ek, const double* sk, const double* tab,
+                           const double* mom, const double* msum, double c_force, double c_flux, double c_self,
+                           double c_surf, const double* add_force, double* force_out, double* field);
+
+// integrator
+struct StageParams {
+    int64_t n;
+    double h;            // step
+    double bext_z;
+    double anis_const, vel_mm_const, mag_damp;
+    int32_t mobil;
+    const double* c1;    // velEaConst1
+    const double* c2;    // velEaConst2
+    const double* mag_mom;
+    const int32_t* inv_perm;
+};
+// k_s = RHS(y_s); then the next stage point (coef of length s+1) or, when
+// last, the error norm partials and the normalised accepted state.
+hipError_t launch_stage(hipStream_t s, const StageParams& sp, int32_t stage, int32_t stages,
+                        const double* mm0, const double* ea0, const double* ymm, const double* yea,
+                        const double* field, const double* n_t, const double* n_b, double* kmm, double* kea,
+                        const double* a_next, const double* b_acc, const double* e_err, double* ymm_out,
+                        double* yea_out, double* mom, double4* rec_mom, double* msum_partial, double* err_partial,
+                        double* new_mm, double* new_ea, int32_t nblk);
+hipError_t launch_reduce_max(hipStream_t s, const double* partial, int32_t count, double scale, double* out);
+hipError_t launch_translate(hipStream_t s, int64_t n, double h, double L, const double* pos_cur, const double* force,
+                            const double* zeta_trans, double* pos, const double* mm, double* mz_partial, int32_t nblk);
+hipError_t launch_reduce_mean(hipStream_t s, const double* partial, int32_t count, double inv_n, double* out,
+                              int32_t* counter, double* series);
+hipError_t launch_noise(hipStream_t s, int64_t n, uint64_t seed, const uint64_t* step_counter, uint64_t step_host,
+                        double dt, const double* zeta_rot, const double* vol_mag, const double* zeta_trans,
+                        double kT, double alpha_g, double gyro, double sat_mag, int32_t en_t, int32_t en_b,
+                        int32_t en_f, double* n_t, double* n_b, double* n_f);
+hipError_t launch_scale(hipStream_t s, double* p, int64_t count, double f);
+hipError_t launch_step_count(hipStream_t s, uint64_t* counter);
+
+}  // namespace mds

@@ -1,0 +1,1224 @@
+// kernels.hip - gfx950 kernels of the Brownian/LLG step.
+//
+// Layout: every per-particle vector quantity is SoA (x[0..n), y[0..n),
+// z[0..n)), the column-major layout of the reference's Eigen N x 3 arrays,
+// so each component read or write by a wavefront is one coalesced 512 B line.
+// Kernels are written for 64-lane wavefronts.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace mds {
+
+namespace {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ double min_image(double v, double L) {
+    // ApplyBoundaryCondVec (src/interaction.cpp:145-155): one shift each way
+    if (v > L / 2) v -= L;
+    if (v < -L / 2) v += L;
+    return v;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+
+__device__ __forceinline__ int mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
+    return __longlong_as_double(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int cell_coord(double x, double L, double cell, int ncell) {
+    int c = (int)floor((x + L / 2) / cell);
+    return c < 0 ? 0 : (c >= ncell ? ncell - 1 : c);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// ------------------------------------------------------------------ fills
+__global__ void k_fill_i32(int32_t* p, int64_t count, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_f64(double* p, int64_t count, double v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_scale(double* p, int64_t count, double f) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] *= f;
+}
+
+int grid_for(int64_t n, int block, int cap = 65535 * 4) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return (int)(g > cap ? cap : g);
+}
+
+// ------------------------------------------------------------------ step setup
+// pos0 <- pos (step-start positions: EvalSinCos and the pair scan read these),
+// pos_cur <- pos (positions the overlap correction moves), cell keys.
+__global__ void k_prepare(int64_t n, const double* __restrict__ pos, double* __restrict__ pos0,
+                          double* __restrict__ pos_cur, double L, int32_t ncell, double cell,
+                          int32_t* __restrict__ key_main, int32_t fncell, double fcell, int32_t* __restrict__ key_fine) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
+    pos0[i] = x;
+    pos0[n + i] = y;
+    pos0[2 * n + i] = z;
+    pos_cur[i] = x;
+    pos_cur[n + i] = y;
+    pos_cur[2 * n + i] = z;
+    key_main[i] = (cell_coord(z, L, cell, ncell) * ncell + cell_coord(y, L, cell, ncell)) * ncell +
+                  cell_coord(x, L, cell, ncell);
+    key_fine[i] = (cell_coord(z, L, fcell, fncell) * fncell + cell_coord(y, L, fcell, fncell)) * fncell +
+                  cell_coord(x, L, fcell, fncell);
+}
+
+__global__ void k_count(int64_t n, const int32_t* __restrict__ key, int32_t* __restrict__ count) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&count[key[i]], 1);
+}
+
+// Exclusive scan of `count` (one block of 1024 threads); writes start, end
+// (= start + count) and a cursor copy of start. total (if not null) gets the sum.
+__global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ count, int32_t m, int32_t* __restrict__ start,
+                                               int32_t* __restrict__ end, int32_t* __restrict__ cursor,
+                                               int32_t* __restrict__ total) {
+    __shared__ int32_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (m + 1023) / 1024;
+    const int64_t b = t * per, e = (b + per < m) ? b + per : m;
+    int32_t s = 0;
+    for (int64_t i = b; i < e; ++i) s += count[i];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int32_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int32_t run = part[t] - s;
+    for (int64_t i = b; i < e; ++i) {
+        const int32_t c = count[i];
+        start[i] = run;
+        if (end) end[i] = run + c;
+        if (cursor) cursor[i] = run;
+        run += c;
+    }
+    if (total && t == 1023) *total = part[1023];
+}
+
+__global__ void k_place(int64_t n, const int32_t* __restrict__ key, int32_t* __restrict__ cursor,
+                        int32_t* __restrict__ sorted_idx) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) sorted_idx[atomicAdd(&cursor[key[i]], 1)] = (int32_t)i;
+}
+
+__global__ void k_records(int64_t n, const int32_t* __restrict__ sorted_idx, const double* __restrict__ pos0,
+                          const double* __restrict__ r_mag, double4* __restrict__ rec, int32_t* __restrict__ inv_perm) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int32_t i = sorted_idx[k];
+    rec[k] = make_double4(pos0[i], pos0[n + i], pos0[2 * n + i], r_mag[i]);
+    inv_perm[i] = (int32_t)k;
+}
+
+// Work items of the pair kernel: one per 64 particles of a cell.
+__global__ __launch_bounds__(1024) void k_work(int32_t ncells, const int32_t* __restrict__ start,
+                                               const int32_t* __restrict__ end, int32_t* __restrict__ work,
+                                               int32_t* __restrict__ nwork, int32_t cap, int32_t* __restrict__ err) {
+    __shared__ int32_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (ncells + 1023) / 1024;
+    const int64_t b = t * per, e = (b + per < ncells) ? b + per : ncells;
+    int32_t s = 0;
+    for (int64_t c = b; c < e; ++c) s += (end[c] - start[c] + WAVE - 1) / WAVE;
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int32_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int32_t run = part[t] - s;
+    for (int64_t c = b; c < e; ++c) {
+        const int32_t nch = (end[c] - start[c] + WAVE - 1) / WAVE;
+        if (nch > 256) {
+            atomicOr(err, 1);
+            continue;
+        }
+        for (int32_t k = 0; k < nch; ++k)
+            if (run + k < cap) work[run + k] = ((int32_t)c << 8) | k;
+        run += nch;
+    }
+    if (t == 1023) {
+        *nwork = part[1023] < cap ? part[1023] : cap;
+        if (part[1023] > cap) atomicOr(err, 2);
+    }
+}
+
+// ------------------------------------------------------------------ contacts
+// Candidate pairs (i < j) whose step-start distance is below the contact
+// distance r_i + r_j + 1e-10 plus a search margin. Pass 0 counts, pass 1
+// writes each particle's partners in ascending j (so the candidate list is in
+// the reference's lexicographic (i, j) order).
+template <int PASS>
+__global__ void k_contacts(int64_t n, double L, const double* __restrict__ pos0, const double* __restrict__ r_mag,
+                           int32_t fncell, double fcell, const int32_t* __restrict__ fstart,
+                           const int32_t* __restrict__ fend, const int32_t* __restrict__ fsorted, double extra,
+                           int32_t* __restrict__ cand_count, const int32_t* __restrict__ cand_off,
+                           int32_t* __restrict__ cand_i, int32_t* __restrict__ cand_j, int32_t cap,
+                           int32_t* __restrict__ err) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = pos0[i], yi = pos0[n + i], zi = pos0[2 * n + i], ri = r_mag[i];
+    const int cx = cell_coord(xi, L, fcell, fncell), cy = cell_coord(yi, L, fcell, fncell),
+              cz = cell_coord(zi, L, fcell, fncell);
+    const int reach = fncell >= 3 ? 1 : 0;
+    constexpr int MAXC = 32;
+    int32_t list[PASS ? MAXC : 1];
+    int cnt = 0;
+    for (int dz = -reach; dz <= reach; ++dz)
+        for (int dy = -reach; dy <= reach; ++dy)
+            for (int dx = -reach; dx <= reach; ++dx) {
+                const int c = (((cz + dz + fncell) % fncell) * fncell + (cy + dy + fncell) % fncell) * fncell +
+                              (cx + dx + fncell) % fncell;
+                for (int32_t s = fstart[c]; s < fend[c]; ++s) {
+                    const int32_t j = fsorted[s];
+                    if (j <= i) continue;
+                    const double vx = min_image(pos0[j] - xi, L), vy = min_image(pos0[n + j] - yi, L),
+                                 vz = min_image(pos0[2 * n + j] - zi, L);
+                    const double lim = ri + r_mag[j] + 1e-10 + extra;
+                    if (vx * vx + vy * vy + vz * vz < lim * lim) {
+                        if (PASS) {
+                            if (cnt < MAXC) list[cnt] = j;
+                        }
+                        ++cnt;
+                    }
+                }
+            }
+    if (!PASS) {
+        cand_count[i] = cnt;
+        return;
+    }
+    if (cnt > MAXC) {
+        atomicOr(err, 4);
+        cnt = MAXC;
+    }
+    // ascending j
+    for (int a = 1; a < cnt; ++a) {
+        const int32_t v = list[a];
+        int b = a - 1;
+        while (b >= 0 && list[b] > v) {
+            list[b + 1] = list[b];
+            --b;
+        }
+        list[b + 1] = v;
+    }
+    const int32_t off = cand_off[i];
+    for (int a = 0; a < cnt; ++a) {
+        if (off + a < cap) {
+            cand_i[off + a] = (int32_t)i;
+            cand_j[off + a] = list[a];
+        } else {
+            atomicOr(err, 8);
+        }
+    }
+}
+
+// Sequential overlap correction of CompInteractVecs (src/interaction.cpp:120-129)
+// over the candidate pairs in lexicographic order: a pair closer than contact
+// moves both particles half the overlap along the pre-move axis, and later
+// pairs see the moved positions. Every move is recorded as a version
+// (pair key, new position) of that particle, so the pair kernels can give each
+// pair the positions the reference's scan would have used.
+__global__ void k_resolve(int64_t n, double L, double rc, const double* __restrict__ r_mag,
+                          const int32_t* __restrict__ ncand, const int32_t* __restrict__ cand_i,
+                          const int32_t* __restrict__ cand_j, double* __restrict__ cur, const double* __restrict__ pos0,
+                          int32_t* __restrict__ ev_head, int32_t* __restrict__ ev_tail, int32_t* __restrict__ ev_next,
+                          uint64_t* __restrict__ ev_key, double* __restrict__ ev_pos, int32_t ev_cap,
+                          int32_t* __restrict__ n_events, double* __restrict__ max_disp, double disp_limit,
+                          int32_t* __restrict__ err) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    // candidates beyond the list capacity were never written (flagged by k_contacts)
+    const int32_t total = min(*ncand, ev_cap / 2);
+    int32_t nev = 0;
+    double md = 0;
+    for (int32_t k = 0; k < total; ++k) {
+        const int32_t i = cand_i[k], j = cand_j[k];
+        double v[3];
+        for (int d = 0; d < 3; ++d) v[d] = min_image(cur[d * n + j] - cur[d * n + i], L);
+        const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (!(r < rc)) continue;
+        const double dCont = r_mag[i] + r_mag[j] + 1e-10;
+        if (!(r < dCont)) continue;
+        const double dd = dCont - r;
+        const uint64_t key = (uint64_t)i * (uint64_t)n + (uint64_t)j;
+        const int32_t pp[2] = {i, j};
+        for (int d = 0; d < 3; ++d) {
+            const double u = v[d] / r;
+            cur[d * n + i] -= 0.5 * dd * u;
+            cur[d * n + j] += 0.5 * dd * u;
+        }
+        for (int s = 0; s < 2; ++s) {
+            const int32_t p = pp[s];
+            if (nev >= ev_cap) {
+                atomicOr(err, 16);
+                break;
+            }
+            const int32_t e = nev++;
+            ev_key[e] = key;
+            double dsq = 0;
+            for (int d = 0; d < 3; ++d) {
+                ev_pos[3 * (int64_t)e + d] = cur[d * n + p];
+                const double dl = cur[d * n + p] - pos0[d * n + p];
+                dsq += dl * dl;
+            }
+            md = fmax(md, sqrt(dsq));
+            ev_next[e] = -1;
+            if (ev_tail[p] < 0)
+                ev_head[p] = e;
+            else
+                ev_next[ev_tail[p]] = e;
+            ev_tail[p] = e;
+        }
+    }
+    *n_events = nev;
+    *max_disp = md;
+    if (md > disp_limit) atomicOr(err, 32);
+}
+
+// ------------------------------------------------------------------ moments
+template <int BS>
+__device__ __forceinline__ void block_sum3(double a, double b, double c, double* out3) {
+    __shared__ double sh[3][BS / WAVE];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int w = threadIdx.x / WAVE;
+    if (lane_id() == 0) {
+        sh[0][w] = a;
+        sh[1][w] = b;
+        sh[2][w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int k = 0; k < BS / WAVE; ++k) {
+            s0 += sh[0][k];
+            s1 += sh[1][k];
+            s2 += sh[2][k];
+        }
+        out3[0] = s0;
+        out3[1] = s1;
+        out3[2] = s2;
+    }
+}
+
+// magMomVec = posMm .* magMom and its column sums (src/simulation.cpp:638-639).
+__global__ __launch_bounds__(256) void k_moments(int64_t n, const double* __restrict__ mm,
+                                                 const double* __restrict__ mag_mom, const int32_t* __restrict__ inv_perm,
+                                                 double* __restrict__ mom, double4* __restrict__ rec_mom,
+                                                 double* __restrict__ partial) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double mx = 0, my = 0, mz = 0;
+    if (i < n) {
+        const double a = mag_mom[i];
+        mx = mm[i] * a;
+        my = mm[n + i] * a;
+        mz = mm[2 * n + i] * a;
+        mom[i] = mx;
+        mom[n + i] = my;
+        mom[2 * n + i] = mz;
+        rec_mom[inv_perm[i]] = make_double4(mx, my, mz, 0.0);
+    }
+    block_sum3<256>(mx, my, mz, partial + 3 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_sum3(const double* __restrict__ partial, int32_t count,
+                                                     double* __restrict__ out) {
+    double a = 0, b = 0, c = 0;
+    for (int k = threadIdx.x; k < count; k += blockDim.x) {
+        a += partial[3 * k];
+        b += partial[3 * k + 1];
+        c += partial[3 * k + 2];
+    }
+    block_sum3<256>(a, b, c, out);
+}
+
+// ------------------------------------------------------------------ real space
+// Real-space Ewald pair terms (CompDipInteractEwaldR, src/interaction.cpp:383-422;
+// field only: CompDipFieldEwaldR, 555-582), pressure (EvalPressTensLR,
+// src/postprocessing.cpp:195-200) and, with the coating potential, the
+// short-range forces (src/interaction.cpp:766-832).
+//
+// One wavefront owns up to 64 particles of one cell (one per lane) and visits
+// the stencil cells. Each candidate j is tested by all lanes (a cheap distance
+// test); lanes that hit push (lane, j) into a wavefront-private LDS queue, and
+// every 64 queued pairs are evaluated densely, one pair per lane, so the
+// expensive erfc/exp path never runs with idle lanes. Results are added into
+// the owning lane's LDS accumulator. Every pair is evaluated from its lower
+// original index (the reference's i < j orientation), so both of its
+// particles see bit-identical geometry.
+struct IWave {
+    int32_t q[128];
+    double acc[6][WAVE];
+    double4 ipos[WAVE];
+    double4 imom[WAVE];
+    int32_t iorig[WAVE];
+};
+
+__device__ __forceinline__ void version_at(const Geometry& g, int32_t p, uint64_t key, double& x, double& y, double& z) {
+    int32_t e = g.ev_head[p];
+    while (e >= 0 && g.ev_key[e] < key) {
+        x = g.ev_pos[3 * (int64_t)e];
+        y = g.ev_pos[3 * (int64_t)e + 1];
+        z = g.ev_pos[3 * (int64_t)e + 2];
+        e = g.ev_next[e];
+    }
+}
+__device__ __forceinline__ bool is_event(const Geometry& g, int32_t lo, uint64_t key) {
+    for (int32_t e = g.ev_head[lo]; e >= 0; e = g.ev_next[e]) {
+        if (g.ev_key[e] == key) return true;
+        if (g.ev_key[e] > key) break;
+    }
+    return false;
+}
+
+template <bool FORCE>
+__device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __restrict__ rec_mom, const PairParams& pp,
+                                           IWave& W, int32_t entry, bool active, bool moved_any, double press[3],
+                                           int64_t& npairs) {
+    const int il = entry & 63;
+    const int32_t j = entry >> 6;
+    double Fi[3] = {0, 0, 0}, Bi[3] = {0, 0, 0};
+    if (active) {
+        const double4 pi = W.ipos[il];
+        const double4 mi4 = W.imom[il];
+        const int32_t oi = W.iorig[il];
+        const double4 pj = g.rec_pos[j];
+        const double4 mj4 = rec_mom[j];
+        const int32_t oj = g.sorted_idx[j];
+        const bool i_lo = oi < oj;
+        const int32_t lo = i_lo ? oi : oj, hi = i_lo ? oj : oi;
+        double xl = i_lo ? pi.x : pj.x, yl = i_lo ? pi.y : pj.y, zl = i_lo ? pi.z : pj.z;
+        double xh = i_lo ? pj.x : pi.x, yh = i_lo ? pj.y : pi.y, zh = i_lo ? pj.z : pi.z;
+        bool event = false;
+        if (moved_any) {
+            const uint64_t key = (uint64_t)lo * (uint64_t)g.n + (uint64_t)hi;
+            version_at(g, lo, key, xl, yl, zl);
+            version_at(g, hi, key, xh, yh, zh);
+            event = is_event(g, lo, key);
+        }
+        const double L = g.L;
+        const double vx = min_image(xh - xl, L), vy = min_image(yh - yl, L), vz = min_image(zh - zl, L);
+        const double r0 = sqrt(vx * vx + vy * vy + vz * vz);
+        if (r0 < pp.rc) {
+            ++npairs;
+            const double ri = pi.w, rj = pj.w;
+            const double r = event ? ri + rj + 1e-10 : r0;
+            const double sgn = i_lo ? 1.0 : -1.0;  // u from i to j
+            const double ux = sgn * (vx / r0), uy = sgn * (vy / r0), uz = sgn * (vz / r0);
+            const double r2 = r * r;
+            const double t = 2.0 * pp.alpha * exp(-pp.alpha2 * r2) * pp.ir_pi / r2;
+            const double a1 = erfc(pp.alpha * r) / (r2 * r) + t;
+            const double a2 = 3.0 * a1 / r2 + 2.0 * pp.alpha2 * t;
+            const double di = mi4.x * ux + mi4.y * uy + mi4.z * uz;
+            const double dj = mj4.x * ux + mj4.y * uy + mj4.z * uz;
+            const double f = pp.fac;
+            Bi[0] = f * (a2 * dj * ux * r2 - a1 * mj4.x);
+            Bi[1] = f * (a2 * dj * uy * r2 - a1 * mj4.y);
+            Bi[2] = f * (a2 * dj * uz * r2 - a1 * mj4.z);
+            if (FORCE) {
+                const double a3 = 5.0 * a2 / r2 + 4.0 * pp.alpha4 * t;
+                const double mimj = mi4.x * mj4.x + mi4.y * mj4.y + mi4.z * mj4.z;
+                const double c0 = (a2 * mimj - a3 * di * dj * r2) * r;
+                const double c1 = a2 * r;
+                // force on j from the pair (applied -F to i)
+                const double Fx = f * (c0 * ux + c1 * (dj * mi4.x + di * mj4.x));
+                const double Fy = f * (c0 * uy + c1 * (dj * mi4.y + di * mj4.y));
+                const double Fz = f * (c0 * uz + c1 * (dj * mi4.z + di * mj4.z));
+                Fi[0] = -Fx;
+                Fi[1] = -Fy;
+                Fi[2] = -Fz;
+                if (i_lo) {
+                    press[0] += r * ux * Fy;
+                    press[1] += r * ux * Fz;
+                    press[2] += r * uy * Fz;
+                }
+                if (pp.coating && r0 < pp.rc_sr) {
+                    // short-range forces along u(lo->hi); steric uses r_lo
+                    const double rl = i_lo ? ri : rj, rh = i_lo ? rj : ri;
+                    const double dDelta = 2 * (rl + pp.eff_len);
+                    double s = 0;
+                    if (r <= dDelta) s = (2 * rh / (rl + rh)) * pp.ster_fac * rl * rl * log(dDelta / r);
+                    s += (rl * rh / (rl + rh)) * pp.el_fac * exp(-pp.kappa * (r - rl - rh));
+                    const double q1 = (r - rl - rh) * (r + rl + rh);
+                    const double q2 = r * r - (rl - rh) * (rl - rh);
+                    s += -pp.hamaker * r * pow(4 * rl * rh, 3.0) / (6 * (q1 * q1) * (q2 * q2));
+                    // u(lo->hi) = sgn * u(i->j); force on hi is +s u(lo->hi)
+                    const double ulx = sgn * ux, uly = sgn * uy, ulz = sgn * uz;
+                    Fi[0] -= sgn * s * ulx;
+                    Fi[1] -= sgn * s * uly;
+                    Fi[2] -= sgn * s * ulz;
+                    if (i_lo) {
+                        press[0] += r * ulx * (s * uly);
+                        press[1] += r * ulx * (s * ulz);
+                        press[2] += r * uly * (s * ulz);
+                    }
+                }
+            }
+        }
+        atomicAdd(&W.acc[3][il], Bi[0]);
+        atomicAdd(&W.acc[4][il], Bi[1]);
+        atomicAdd(&W.acc[5][il], Bi[2]);
+        if (FORCE) {
+            atomicAdd(&W.acc[0][il], Fi[0]);
+            atomicAdd(&W.acc[1][il], Fi[1]);
+            atomicAdd(&W.acc[2][il], Fi[2]);
+        }
+    }
+}
+
+template <bool FORCE>
+__global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
+                                               PairOut out, unsigned long long* __restrict__ cnt_pairs) {
+    __shared__ IWave waves[4];
+    const int wv = threadIdx.x / WAVE;
+    const int lane = lane_id();
+    IWave& W = waves[wv];
+    const int32_t w = blockIdx.x * 4 + wv;
+    const int32_t nwork = *g.nwork;
+    if (w >= nwork) return;
+    const int32_t item = g.work[w];
+    const int32_t cell = item >> 8, chunk = item & 255;
+    const int32_t s0 = g.cell_start[cell] + chunk * WAVE;
+    const int32_t e0 = min(s0 + WAVE, g.cell_end[cell]);
+    const int32_t k = s0 + lane;
+    const bool vi = k < e0;
+    double4 pi = make_double4(0, 0, 0, 0);
+    if (vi) {
+        pi = g.rec_pos[k];
+        W.ipos[lane] = pi;
+        W.imom[lane] = rec_mom[k];
+        W.iorig[lane] = g.sorted_idx[k];
+    }
+    for (int d = 0; d < 6; ++d) W.acc[d][lane] = 0.0;
+    const bool moved_any = *g.n_events > 0;
+    const double reach_r = pp.rc * (1.0 + 1e-12) + (moved_any ? 2.0 * *g.max_disp : 0.0);
+    const double rr2 = reach_r * reach_r;
+    const double L = g.L;
+    const int nc = g.ncell;
+    const int cx = cell % nc, cy = (cell / nc) % nc, cz = cell / (nc * nc);
+    const int R = nc >= 2 * g.reach + 1 ? g.reach : 0;
+    double press[3] = {0, 0, 0};
+    int64_t npairs = 0;
+    int qn = 0;
+    wave_sync();
+    for (int dz = -R; dz <= R; ++dz)
+        for (int dy = -R; dy <= R; ++dy)
+            for (int dx = -R; dx <= R; ++dx) {
+                const int c = (((cz + dz + nc) % nc) * nc + (cy + dy + nc) % nc) * nc + (cx + dx + nc) % nc;
+                const int32_t cs = g.cell_start[c], ce = g.cell_end[c];
+                for (int32_t jb = cs; jb < ce; jb += WAVE) {
+                    const int32_t jl = jb + lane;
+                    double4 pj = make_double4(0, 0, 0, 0);
+                    if (jl < ce) pj = g.rec_pos[jl];
+                    const int cnt = min(WAVE, ce - jb);
+                    for (int t = 0; t < cnt; ++t) {
+                        const double xj = readlane_d(pj.x, t), yj = readlane_d(pj.y, t), zj = readlane_d(pj.z, t);
+                        const double ddx = min_image(xj - pi.x, L), ddy = min_image(yj - pi.y, L),
+                                     ddz = min_image(zj - pi.z, L);
+                        const bool hit = vi && (jb + t != k) && (ddx * ddx + ddy * ddy + ddz * ddz < rr2);
+                        const uint64_t m = __ballot(hit);
+                        if (m) {
+                            if (hit) W.q[qn + mbcnt(m)] = ((jb + t) << 6) | lane;
+                            qn += __popcll(m);
+                            if (qn >= WAVE) {
+                                wave_sync();
+                                pair_dense<FORCE>(g, rec_mom, pp, W, W.q[lane], true, moved_any, press, npairs);
+                                wave_sync();
+                                const int rest = qn - WAVE;
+                                int32_t moved = lane < rest ? W.q[WAVE + lane] : 0;
+                                wave_sync();
+                                if (lane < rest) W.q[lane] = moved;
+                                qn = rest;
+                                wave_sync();
+                            }
+                        }
+                    }
+                }
+            }
+    wave_sync();
+    if (qn > 0) pair_dense<FORCE>(g, rec_mom, pp, W, lane < qn ? W.q[lane] : 0, lane < qn, moved_any, press, npairs);
+    wave_sync();
+    if (vi) {
+        const int32_t o = W.iorig[lane];
+        const int64_t n = g.n;
+        out.field[o] = W.acc[3][lane];
+        out.field[n + o] = W.acc[4][lane];
+        out.field[2 * n + o] = W.acc[5][lane];
+        if (FORCE) {
+            out.force[o] = W.acc[0][lane];
+            out.force[n + o] = W.acc[1][lane];
+            out.force[2 * n + o] = W.acc[2][lane];
+        }
+    }
+    if (FORCE) {
+        const double p0 = wave_sum(press[0]), p1 = wave_sum(press[1]), p2 = wave_sum(press[2]);
+        if (lane == 0) {
+            out.press_partial[3 * (int64_t)w] = p0;
+            out.press_partial[3 * (int64_t)w + 1] = p1;
+            out.press_partial[3 * (int64_t)w + 2] = p2;
+        }
+    }
+    if (cnt_pairs) {
+        const int64_t np = wave_sum(npairs);
+        if (lane == 0) atomicAdd(cnt_pairs, (unsigned long long)np);
+    }
+}
+
+// ------------------------------------------------------------------ k space
+// cos/sin(2 pi n x_d / L), n = 0..nc, by the Rapaport recurrence of EvalSinCos
+// (src/interaction.cpp:521-548). Layout per particle: [d][n][cos, sin].
+__global__ void k_sincos_tab(int64_t n, int32_t nc, double two_pi_over_L, const double* __restrict__ pos0,
+                             double* __restrict__ tab) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t stride = 3 * (int64_t)(nc + 1) * 2;
+    double* t = tab + i * stride;
+    for (int d = 0; d < 3; ++d) {
+        const double tt = two_pi_over_L * pos0[d * n + i];
+        double* td = t + d * (nc + 1) * 2;
+        double s1, c1;
+        sincos(tt, &s1, &c1);
+        td[0] = 1;
+        td[1] = 0;
+        if (nc >= 1) {
+            td[2] = c1;
+            td[3] = s1;
+        }
+        const double u = 2.0 * c1;
+        double cm2 = 1, sm2 = 0, cm1 = c1, sm1 = s1;
+        for (int k = 2; k <= nc; ++k) {
+            const double ck = u * cm1 - cm2, sk = u * sm1 - sm2;
+            td[2 * k] = ck;
+            td[2 * k + 1] = sk;
+            cm2 = cm1;
+            sm2 = sm1;
+            cm1 = ck;
+            sm1 = sk;
+        }
+    }
+}
+
+// Structure factors S(k) = sum_j (n . m_j) e^{i k.r_j} (the first pass of
+// CompDipInteractEwaldF, src/interaction.cpp:459-483), over the half space of
+// n vectors. One lane per row segment (ny, nz, |nx| in [alo, ahi]); the
+// particles of a chunk are walked in step by the whole wavefront (broadcast
+// loads), and the segment's nx values are accumulated in registers through
+// the recurrence e^{i(a+1)x} = e^{iax} e^{ix}. Each chunk writes its partial
+// sums; k_kreduce adds them in a fixed order.
+template <int SEG>
+__global__ __launch_bounds__(64) void k_kbuild(int64_t n, int32_t nc, int32_t nseg, const KSeg* __restrict__ segs,
+                                               const double* __restrict__ tab, const double* __restrict__ mom,
+                                               int32_t chunk, int32_t nk, double* __restrict__ partial) {
+    const int r = blockIdx.x * WAVE + threadIdx.x;
+    const bool vr = r < nseg;
+    const KSeg sg = vr ? segs[r] : KSeg{0, 0, 1, 0, 0, 0};
+    const int ay = sg.ny < 0 ? -sg.ny : sg.ny;
+    const double ysg = sg.ny < 0 ? -1.0 : 1.0;
+    const bool half = (sg.ny == 0 && sg.nz == 0);
+    const int64_t stride = 3 * (int64_t)(nc + 1) * 2;
+    double pc_[SEG], ps_[SEG], mc_[SEG], ms_[SEG];
+#pragma unroll
+    for (int a = 0; a < SEG; ++a) pc_[a] = ps_[a] = mc_[a] = ms_[a] = 0.0;
+    const int64_t jb = (int64_t)blockIdx.y * chunk;
+    const int64_t je = jb + chunk < n ? jb + chunk : n;
+    for (int64_t j = jb; j < je; ++j) {
+        const double* t = tab + j * stride;
+        const double ex = t[2], sx = t[3];
+        const double* ty = t + (nc + 1) * 2;
+        const double* tz = t + 2 * (nc + 1) * 2;
+        const double yc = ty[2 * ay], ys = ysg * ty[2 * ay + 1];
+        const double zc = tz[2 * sg.nz], zs = tz[2 * sg.nz + 1];
+        const double wc = yc * zc - ys * zs, ws = ys * zc + yc * zs;
+        const double mx = mom[j], my = mom[n + j], mz = mom[2 * n + j];
+        const double A = sg.ny * my + sg.nz * mz;
+        double xc = t[2 * sg.alo], xs = t[2 * sg.alo + 1];
+#pragma unroll
+        for (int q = 0; q < SEG; ++q) {
+            const int a = sg.alo + q;
+            if (a <= sg.ahi) {
+                const double p1 = xc * wc, p2 = xs * ws, p3 = xc * ws, p4 = xs * wc;
+                const double qp = a * mx + A, qm = -a * mx + A;
+                pc_[q] += qp * (p1 - p2);
+                ps_[q] += qp * (p3 + p4);
+                mc_[q] += qm * (p1 + p2);
+                ms_[q] += qm * (p3 - p4);
+            }
+            const double nxc = xc * ex - xs * sx, nxs = xs * ex + xc * sx;
+            xc = nxc;
+            xs = nxs;
+        }
+    }
+    if (!vr) return;
+    double* out = partial + (int64_t)blockIdx.y * nk * 2;
+#pragma unroll
+    for (int q = 0; q < SEG; ++q) {
+        const int a = sg.alo + q;
+        if (a > sg.ahi) break;
+        const int kp = half ? sg.koff + a - 1 : sg.koff + a + sg.nxmax;
+        out[2 * kp] = pc_[q];
+        out[2 * kp + 1] = ps_[q];
+        if (a >= 1 && !half) {
+            const int km = sg.koff - a + sg.nxmax;
+            out[2 * km] = mc_[q];
+            out[2 * km + 1] = ms_[q];
+        }
+    }
+}
+
+__global__ void k_kreduce(int32_t nk, int32_t nchunks, const double* __restrict__ partial, double* __restrict__ sk) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    double c = 0, s = 0;
+    for (int ch = 0; ch < nchunks; ++ch) {
+        c += partial[((int64_t)ch * nk + k) * 2];
+        s += partial[((int64_t)ch * nk + k) * 2 + 1];
+    }
+    sk[2 * k] = c;
+    sk[2 * k + 1] = s;
+}
+
+// Second pass of CompDipInteractEwaldF / CompDipFieldEwaldF
+// (src/interaction.cpp:485-504, 644-662) plus the self and vacuum-surface
+// terms (510-513): one lane per particle, all k in row order; the particle's
+// phases advance by the recurrences in y (row to row) and x (within a row).
+template <bool FORCE>
+__global__ __launch_bounds__(256) void k_kscatter(int64_t n, int32_t nc, int32_t nrows, const KRow* __restrict__ rows,
+                                                  const double* __restrict__ ek, const double* __restrict__ sk,
+                                                  const double* __restrict__ tab, const double* __restrict__ mom,
+                                                  const double* __restrict__ msum, double c_force, double c_flux,
+                                                  double c_self, double c_surf, const double* __restrict__ add_force,
+                                                  double* __restrict__ force, double* __restrict__ field) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t stride = 3 * (int64_t)(nc + 1) * 2;
+    const double* t = tab + i * stride;
+    const double* ty = t + (nc + 1) * 2;
+    const double* tz = t + 2 * (nc + 1) * 2;
+    const double ex = t[2], sx = t[3], ey = ty[2], sy = ty[3];
+    const double mx = mom[i], my = mom[n + i], mz = mom[2 * n + i];
+    double F0 = 0, F1 = 0, F2 = 0, B0 = 0, B1 = 0, B2 = 0;
+    double yc = 1, ys = 0, zc = 1, zs = 0;
+    int cur_nz = -1;
+    for (int r = 0; r < nrows; ++r) {
+        const KRow row = rows[r];
+        if (row.nz != cur_nz) {
+            cur_nz = row.nz;
+            zc = tz[2 * row.nz];
+            zs = tz[2 * row.nz + 1];
+            const int ay = row.ny < 0 ? -row.ny : row.ny;
+            yc = ty[2 * ay];
+            ys = row.ny < 0 ? -ty[2 * ay + 1] : ty[2 * ay + 1];
+        } else {
+            const double nyc = yc * ey - ys * sy, nys = ys * ey + yc * sy;
+            yc = nyc;
+            ys = nys;
+        }
+        const double wc = yc * zc - ys * zs, ws = ys * zc + yc * zs;
+        const double A = row.ny * my + row.nz * mz;
+        const bool half = (row.ny == 0 && row.nz == 0);
+        double xc = 1.0, xs = 0.0;
+        for (int a = 0; a <= row.nxmax; ++a) {
+            const double p1 = xc * wc, p2 = xs * ws, p3 = xc * ws, p4 = xs * wc;
+            for (int sgn = 1; sgn >= -1; sgn -= 2) {
+                if (a == 0 && sgn < 0) break;
+                if (half && (a == 0 || sgn < 0)) continue;
+                const int nx = sgn * a;
+                const double pc = sgn > 0 ? p1 - p2 : p1 + p2;
+                const double ps = sgn > 0 ? p3 + p4 : p3 - p4;
+                const int k = half ? row.koff + nx - 1 : row.koff + nx + row.nxmax;
+                const double sc = sk[2 * k], ss = sk[2 * k + 1], e = ek[k];
+                const double b = c_flux * e * (sc * pc + ss * ps);
+                B0 -= b * nx;
+                B1 -= b * row.ny;
+                B2 -= b * row.nz;
+                if (FORCE) {
+                    const double q = nx * mx + A;
+                    const double s = c_force * e * q * (sc * ps - ss * pc);
+                    F0 += s * nx;
+                    F1 += s * row.ny;
+                    F2 += s * row.nz;
+                }
+            }
+            const double nxc = xc * ex - xs * sx, nxs = xs * ex + xc * sx;
+            xc = nxc;
+            xs = nxs;
+        }
+    }
+    B0 += c_self * mx - c_surf * msum[0];
+    B1 += c_self * my - c_surf * msum[1];
+    B2 += c_self * mz - c_surf * msum[2];
+    field[i] += B0;
+    field[n + i] += B1;
+    field[2 * n + i] += B2;
+    if (FORCE) {
+        force[i] += F0 + (add_force ? add_force[i] : 0.0);
+        force[n + i] += F1 + (add_force ? add_force[n + i] : 0.0);
+        force[2 * n + i] += F2 + (add_force ? add_force[2 * n + i] : 0.0);
+    }
+}
+
+// ------------------------------------------------------------------ integrator
+struct Coef {
+    double v[8];
+};
+
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double r[3]) {
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// One Runge-Kutta stage of IntegrationRot (src/simulation.cpp:707-868): the
+// right-hand side RKCoeffs (876-902) at the stage point, then either the next
+// stage point y + h sum_r a_r k_r (with its moments for the next field
+// evaluation) or, at the last stage, the error norm |h sum_r e_r kMm_r| and
+// the accepted, renormalised state y + h sum_r b_r k_r (746-747, 866-867).
+__global__ __launch_bounds__(256) void k_stage(StageParams sp, int32_t stage, int32_t stages,
+                                               const double* __restrict__ mm0, const double* __restrict__ ea0,
+                                               const double* __restrict__ ymm, const double* __restrict__ yea,
+                                               const double* __restrict__ field, const double* __restrict__ n_t,
+                                               const double* __restrict__ n_b, double* __restrict__ kmm,
+                                               double* __restrict__ kea, Coef a_next, Coef b_acc, Coef e_err,
+                                               double* __restrict__ ymm_out, double* __restrict__ yea_out,
+                                               double* __restrict__ mom, double4* __restrict__ rec_mom,
+                                               double* __restrict__ msum_partial, double* __restrict__ err_partial,
+                                               double* __restrict__ new_mm, double* __restrict__ new_ea) {
+    const int64_t n = sp.n;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool v = i < n;
+    double mo[3] = {0, 0, 0};
+    double errn = 0;
+    if (v) {
+        double m[3], e[3], B[3], T[3], Bt[3];
+        for (int d = 0; d < 3; ++d) {
+            m[d] = ymm[d * n + i];
+            e[d] = yea[d * n + i];
+            B[d] = field[d * n + i];
+            T[d] = n_t[d * n + i];
+            Bt[d] = n_b[d * n + i];
+        }
+        const double dt = m[0] * e[0] + m[1] * e[1] + m[2] * e[2];
+        double Be[3];
+        for (int d = 0; d < 3; ++d) Be[d] = (d == 2 ? sp.bext_z : 0.0) + B[d] + sp.anis_const * (e[d] * dt) + Bt[d];
+        double ke[3] = {0, 0, 0};
+        if (sp.mobil) {
+            const double c1 = sp.c1[i], c2 = sp.c2[i];
+            double Te[3];
+            cross3(T, e, Te);
+            for (int d = 0; d < 3; ++d) ke[d] = (m[d] - e[d] * dt) * (c1 * dt) + Te[d] * c2;
+        }
+        double b2[3], b3[3], km[3];
+        cross3(m, Be, b2);
+        cross3(m, b2, b3);
+        for (int d = 0; d < 3; ++d) km[d] = -sp.vel_mm_const * (b2[d] + sp.mag_damp * b3[d]);
+        const int64_t so = (int64_t)stage * 3 * n;
+        for (int d = 0; d < 3; ++d) {
+            kmm[so + d * n + i] = km[d];
+            kea[so + d * n + i] = ke[d];
+        }
+        if (stage + 1 < stages) {
+            for (int d = 0; d < 3; ++d) {
+                double sm = a_next.v[stage] * km[d], se = a_next.v[stage] * ke[d];
+                for (int r = 0; r < stage; ++r) {
+                    sm += a_next.v[r] * kmm[(int64_t)r * 3 * n + d * n + i];
+                    se += a_next.v[r] * kea[(int64_t)r * 3 * n + d * n + i];
+                }
+                const double ym = mm0[d * n + i] + sp.h * sm;
+                ymm_out[d * n + i] = ym;
+                yea_out[d * n + i] = ea0[d * n + i] + sp.h * se;
+                mo[d] = ym * sp.mag_mom[i];
+                mom[d * n + i] = mo[d];
+            }
+            rec_mom[sp.inv_perm[i]] = make_double4(mo[0], mo[1], mo[2], 0.0);
+        } else {
+            double nm[3], ne[3], ev[3];
+            for (int d = 0; d < 3; ++d) {
+                double sm = b_acc.v[stage] * km[d], se = b_acc.v[stage] * ke[d], er = e_err.v[stage] * km[d];
+                for (int r = 0; r < stage; ++r) {
+                    const double kr = kmm[(int64_t)r * 3 * n + d * n + i];
+                    sm += b_acc.v[r] * kr;
+                    er += e_err.v[r] * kr;
+                    se += b_acc.v[r] * kea[(int64_t)r * 3 * n + d * n + i];
+                }
+                nm[d] = mm0[d * n + i] + sp.h * sm;
+                ne[d] = ea0[d * n + i] + sp.h * se;
+                ev[d] = er;
+            }
+            errn = sqrt(ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2]);
+            const double lm = sqrt(nm[0] * nm[0] + nm[1] * nm[1] + nm[2] * nm[2]);
+            const double le = sqrt(ne[0] * ne[0] + ne[1] * ne[1] + ne[2] * ne[2]);
+            for (int d = 0; d < 3; ++d) {
+                new_mm[d * n + i] = nm[d] / lm;
+                new_ea[d * n + i] = ne[d] / le;
+            }
+        }
+    }
+    if (stage + 1 < stages) {
+        block_sum3<256>(mo[0], mo[1], mo[2], msum_partial + 3 * blockIdx.x);
+    } else {
+        __shared__ double sh[256 / WAVE];
+        errn = wave_max(errn);
+        if (lane_id() == 0) sh[threadIdx.x / WAVE] = errn;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0;
+            for (int k = 0; k < 256 / WAVE; ++k) m = fmax(m, sh[k]);
+            err_partial[blockIdx.x] = m;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_max(const double* __restrict__ partial, int32_t count, double scale,
+                                                    double* __restrict__ out) {
+    __shared__ double sh[256 / WAVE];
+    double m = 0;
+    for (int k = threadIdx.x; k < count; k += blockDim.x) m = fmax(m, partial[k]);
+    m = wave_max(m);
+    if (lane_id() == 0) sh[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int k = 0; k < 256 / WAVE; ++k) r = fmax(r, sh[k]);
+        *out = r * scale;
+    }
+}
+
+// IntegrationTrans (src/simulation.cpp:959-962) + ApplyBoundaryCondArr
+// (610-620), and the <m_z> partial sums (515).
+__global__ __launch_bounds__(256) void k_translate(int64_t n, double h, double L, const double* __restrict__ pos_cur,
+                                                   const double* __restrict__ force,
+                                                   const double* __restrict__ zeta_trans, double* __restrict__ pos,
+                                                   const double* __restrict__ mm, double* __restrict__ mz_partial) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double mz = 0;
+    if (i < n) {
+        const double zt = zeta_trans[i];
+        for (int d = 0; d < 3; ++d) {
+            double x = pos_cur[d * n + i] + h * (force[d * n + i] / zt);
+            if (x > L / 2) x -= L;
+            if (x < -L / 2) x += L;
+            pos[d * n + i] = x;
+        }
+        mz = mm[2 * n + i];
+    }
+    block_sum3<256>(mz, 0.0, 0.0, mz_partial + 3 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_mean(const double* __restrict__ partial, int32_t count, double inv_n,
+                                                     double* __restrict__ out, int32_t* __restrict__ counter,
+                                                     double* __restrict__ series) {
+    double a = 0;
+    for (int k = threadIdx.x; k < count; k += blockDim.x) a += partial[3 * k];
+    __shared__ double r3[3];
+    block_sum3<256>(a, 0.0, 0.0, r3);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double v = r3[0] * inv_n;
+        *out = v;
+        if (series) {
+            series[*counter] = v;
+            *counter += 1;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ noise
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11) keyed by the seed, counter (particle, step, call);
+// Box-Muller turns each pair of 53-bit uniforms into two standard normals.
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = (uint32_t)p1;
+        c[2] = n2;
+        c[3] = (uint32_t)p0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_noise(int64_t n, uint64_t seed, const uint64_t* __restrict__ step_counter,
+                                               uint64_t step_host, double dt, const double* __restrict__ zeta_rot,
+                                               const double* __restrict__ vol_mag, const double* __restrict__ zeta_trans,
+                                               double kT, double alpha_g, double gyro, double sat_mag, int32_t en_t,
+                                               int32_t en_b, int32_t en_f, double* __restrict__ n_t,
+                                               double* __restrict__ n_b, double* __restrict__ n_f) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t step = step_counter ? *step_counter : step_host;
+    double z[10];
+    for (int call = 0; call < 5; ++call) {
+        uint32_t c[4] = {(uint32_t)i, (uint32_t)(i >> 32) ^ ((uint32_t)call << 24), (uint32_t)step,
+                         (uint32_t)(step >> 32)};
+        philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const uint64_t a = ((uint64_t)c[0] << 32) | c[1], b = ((uint64_t)c[2] << 32) | c[3];
+        const double u1 = ((a >> 11) + 1) * 0x1.0p-53;  // (0, 1]
+        const double u2 = (b >> 11) * 0x1.0p-53;        // [0, 1)
+        const double rr = sqrt(-2.0 * log(u1));
+        double s, co;
+        sincospi(2.0 * u2, &s, &co);
+        z[2 * call] = rr * co;
+        z[2 * call + 1] = rr * s;
+    }
+    // DefThermFluct, src/simulation.cpp:565-602: torque, field, force
+    const double sT = en_t ? sqrt((2.0 * kT / dt) * zeta_rot[i]) : 0.0;
+    const double sB = en_b ? sqrt(2.0 * kT * alpha_g / (gyro * sat_mag * dt) * (1.0 / vol_mag[i])) : 0.0;
+    const double sF = en_f ? sqrt((2.0 * kT / dt) * zeta_trans[i]) : 0.0;
+    for (int d = 0; d < 3; ++d) {
+        n_t[d * n + i] = z[d] * sT;
+        n_b[d * n + i] = z[3 + d] * sB;
+        n_f[d * n + i] = z[6 + d] * sF;
+    }
+}
+
+__global__ void k_step_count(uint64_t* c) {
+    if (threadIdx.x == 0) *c += 1;
+}
+
+__global__ void k_clear_events(int64_t n, int32_t* __restrict__ head, int32_t* __restrict__ tail) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) {
+        head[i] = -1;
+        tail[i] = -1;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+#define LAUNCH_CHECK() return hipGetLastError()
+
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, p, count, v);
+    LAUNCH_CHECK();
+}
+hipError_t launch_fill_f64(hipStream_t s, double* p, int64_t count, double v) {
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, p, count, v);
+    LAUNCH_CHECK();
+}
+hipError_t launch_scale(hipStream_t s, double* p, int64_t count, double f) {
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, p, count, f);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* pos0, double* pos_cur, double L,
+                          int32_t ncell, double cell, int32_t* key_main, int32_t fncell, double fcell,
+                          int32_t* key_fine) {
+    hipLaunchKernelGGL(k_prepare, dim3(grid_for(n, 256)), dim3(256), 0, s, n, pos, pos0, pos_cur, L, ncell, cell,
+                       key_main, fncell, fcell, key_fine);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
+                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp) {
+    (void)scan_tmp;
+    hipError_t e = launch_fill_i32(s, count, ncells, 0);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_count, dim3(grid_for(n, 256)), dim3(256), 0, s, n, key, count);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, count, ncells, start, end, cursor, (int32_t*)nullptr);
+    hipLaunchKernelGGL(k_place, dim3(grid_for(n, 256)), dim3(256), 0, s, n, key, cursor, sorted_idx);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_idx, const double* pos0,
+                                const double* r_mag, double4* rec_pos, int32_t* inv_perm) {
+    hipLaunchKernelGGL(k_records, dim3(grid_for(n, 256)), dim3(256), 0, s, n, sorted_idx, pos0, r_mag, rec_pos,
+                       inv_perm);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end, int32_t* work,
+                             int32_t* nwork, int32_t cap, int32_t* err) {
+    hipLaunchKernelGGL(k_work, dim3(1), dim3(1024), 0, s, ncells, start, end, work, nwork, cap, err);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_contacts(hipStream_t s, int64_t n, double L, const double* pos0, const double* r_mag,
+                           int32_t fncell, double fcell, const int32_t* fstart, const int32_t* fend,
+                           const int32_t* fsorted, double search_extra, int32_t* cand_count, int32_t* cand_off,
+                           int32_t* cand_j, int32_t cand_cap, int32_t* scan_tmp, int32_t* err) {
+    // cand_off has n+1 slots: offsets then the total at [n]; cand_j holds
+    // 2*cand_cap ints: partner j then owner i.
+    int32_t* cand_i = cand_j + cand_cap;
+    hipLaunchKernelGGL(k_contacts<0>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, L, pos0, r_mag, fncell, fcell,
+                       fstart, fend, fsorted, search_extra, cand_count, (const int32_t*)nullptr, (int32_t*)nullptr,
+                       (int32_t*)nullptr, cand_cap, err);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, cand_count, (int32_t)n, cand_off, (int32_t*)nullptr,
+                       (int32_t*)nullptr, cand_off + n);
+    hipLaunchKernelGGL(k_contacts<1>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, L, pos0, r_mag, fncell, fcell,
+                       fstart, fend, fsorted, search_extra, cand_count, cand_off, cand_i, cand_j, cand_cap, err);
+    (void)scan_tmp;
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_resolve(hipStream_t s, int64_t n, double L, double rc, const double* r_mag, const int32_t* cand_off,
+                          const int32_t* cand_j, double* pos_cur, const double* pos0, int32_t* ev_head,
+                          int32_t* ev_tail, int32_t* ev_next, uint64_t* ev_key, double* ev_pos, int32_t ev_cap,
+                          int32_t* n_events, double* max_disp, double disp_limit, int32_t* err) {
+    // the candidate total sits at cand_off[n]; its capacity is ev_cap / 2
+    const int32_t cand_cap = ev_cap / 2;
+    hipLaunchKernelGGL(k_resolve, dim3(1), dim3(64), 0, s, n, L, rc, r_mag, cand_off + n, cand_j + cand_cap, cand_j,
+                       pos_cur, pos0, ev_head, ev_tail, ev_next, ev_key, ev_pos, ev_cap, n_events, max_disp,
+                       disp_limit, err);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_clear_events(hipStream_t s, int64_t n, int32_t* ev_head, int32_t* ev_tail) {
+    hipLaunchKernelGGL(k_clear_events, dim3(grid_for(n, 256)), dim3(256), 0, s, n, ev_head, ev_tail);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_moments(hipStream_t s, int64_t n, const double* mm, const double* mag_mom, const int32_t* inv_perm,
+                          double* mom, double4* rec_mom, double* msum_partial, int32_t nblk) {
+    hipLaunchKernelGGL(k_moments, dim3(nblk), dim3(256), 0, s, n, mm, mag_mom, inv_perm, mom, rec_mom, msum_partial);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_reduce_sum3(hipStream_t s, const double* partial, int32_t count, double* out) {
+    hipLaunchKernelGGL(k_reduce_sum3, dim3(1), dim3(256), 0, s, partial, count, out);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
+                        const PairParams& pp, const PairOut& out, int32_t max_work, int64_t* cnt_pairs) {
+    (void)precision;
+    const int blocks = (max_work + 3) / 4;
+    if (blocks <= 0) return hipSuccess;
+    if (force)
+        hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out,
+                           (unsigned long long*)cnt_pairs);
+    else
+        hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out,
+                           (unsigned long long*)cnt_pairs);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_sincos_tables(hipStream_t s, int64_t n, int32_t nc, double L, double two_pi, const double* pos0,
+                                double* tab) {
+    hipLaunchKernelGGL(k_sincos_tab, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nc, two_pi / L, pos0, tab);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_kbuild(hipStream_t s, int32_t precision, int64_t n, int32_t nc, int32_t nseg, const KSeg* segs,
+                         const double* tab, const double* mom, int32_t chunk, int32_t nchunks, int32_t nk,
+                         double* partial) {
+    (void)precision;
+    dim3 grid((nseg + WAVE - 1) / WAVE, nchunks);
+    hipLaunchKernelGGL(k_kbuild<KSEG>, grid, dim3(WAVE), 0, s, n, nc, nseg, segs, tab, mom, chunk, nk, partial);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_kreduce(hipStream_t s, int32_t nk, int32_t nchunks, const double* partial, double* sk) {
+    hipLaunchKernelGGL(k_kreduce, dim3((nk + 255) / 256), dim3(256), 0, s, nk, nchunks, partial, sk);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_kscatter(hipStream_t s, bool force, int32_t precision, int64_t n, int32_t nc, int32_t nrows,
+                           const KRow* rows, const double* ek, const double* sk, const double* tab, const double* mom,
+                           const double* msum, double c_force, double c_flux, double c_self, double c_surf,
+                           const double* add_force, double* force_out, double* field) {
+    (void)precision;
+    if (force)
+        hipLaunchKernelGGL(k_kscatter<true>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nc, nrows, rows, ek, sk, tab,
+                           mom, msum, c_force, c_flux, c_self, c_surf, add_force, force_out, field);
+    else
+        hipLaunchKernelGGL(k_kscatter<false>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nc, nrows, rows, ek, sk,
+                           tab, mom, msum, c_force, c_flux, c_self, c_surf, add_force, force_out, field);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_stage(hipStream_t s, const StageParams& sp, int32_t stage, int32_t stages, const double* mm0,
+                        const double* ea0, const double* ymm, const double* yea, const double* field,
+                        const double* n_t, const double* n_b, double* kmm, double* kea, const double* a_next,
+                        const double* b_acc, const double* e_err, double* ymm_out, double* yea_out, double* mom,
+                        double4* rec_mom, double* msum_partial, double* err_partial, double* new_mm, double* new_ea,
+                        int32_t nblk) {
+    Coef a{}, b{}, e{};
+    for (int k = 0; k < 8; ++k) {
+        a.v[k] = a_next ? a_next[k] : 0.0;
+        b.v[k] = b_acc ? b_acc[k] : 0.0;
+        e.v[k] = e_err ? e_err[k] : 0.0;
+    }
+    hipLaunchKernelGGL(k_stage, dim3(nblk), dim3(256), 0, s, sp, stage, stages, mm0, ea0, ymm, yea, field, n_t, n_b,
+                       kmm, kea, a, b, e, ymm_out, yea_out, mom, rec_mom, msum_partial, err_partial, new_mm, new_ea);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_reduce_max(hipStream_t s, const double* partial, int32_t count, double scale, double* out) {
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, s, partial, count, scale, out);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_translate(hipStream_t s, int64_t n, double h, double L, const double* pos_cur, const double* force,
+                            const double* zeta_trans, double* pos, const double* mm, double* mz_partial, int32_t nblk) {
+    hipLaunchKernelGGL(k_translate, dim3(nblk), dim3(256), 0, s, n, h, L, pos_cur, force, zeta_trans, pos, mm,
+                       mz_partial);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_reduce_mean(hipStream_t s, const double* partial, int32_t count, double inv_n, double* out,
+                              int32_t* counter, double* series) {
+    hipLaunchKernelGGL(k_reduce_mean, dim3(1), dim3(256), 0, s, partial, count, inv_n, out, counter, series);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_noise(hipStream_t s, int64_t n, uint64_t seed, const uint64_t* step_counter, uint64_t step_host,
+                        double dt, const double* zeta_rot, const double* vol_mag, const double* zeta_trans, double kT,
+                        double alpha_g, double gyro, double sat_mag, int32_t en_t, int32_t en_b, int32_t en_f,
+                        double* n_t, double* n_b, double* n_f) {
+    hipLaunchKernelGGL(k_noise, dim3(grid_for(n, 256)), dim3(256), 0, s, n, seed, step_counter, step_host, dt,
+                       zeta_rot, vol_mag, zeta_trans, kT, alpha_g, gyro, sat_mag, en_t, en_b, en_f, n_t, n_b, n_f);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_step_count(hipStream_t s, uint64_t* counter) {
+    hipLaunchKernelGGL(k_step_count, dim3(1), dim3(64), 0, s, counter);
+    LAUNCH_CHECK();
+}
+
+}  // namespace mds

