@@ -149,7 +149,7 @@ struct mds_ctx {
     DevBuf<double> kmm, kea, pt[2], new_mm, new_ea;
     // cell lists
     int32_t ncell = 1, reach = 2, fncell = 1;
-    double cell = 0, fcell = 0, margin = 0;
+    double cell = 0, fcell = 0, margin = 0, margin_scale = 0.1;
     int32_t work_cap = 0, cand_cap = 0;
     DevBuf<int32_t> key_main, key_fine, cnt_main, start_main, end_main, cur_main, sorted_main, inv_perm, work, nwork;
     DevBuf<int32_t> cnt_fine, start_fine, end_fine, cur_fine, sorted_fine;
@@ -264,7 +264,8 @@ mds_status plan_grids(mds_ctx* c) {
     std::vector<double> rm(n);
     HIPCHK(hipMemcpy(rm.data(), c->r_mag.p, n * sizeof(double), hipMemcpyDeviceToHost));
     const double dmax = 2 * *std::max_element(rm.begin(), rm.end()) + 1e-10;
-    c->margin = 0.1 * dmax;
+    if (c->margin_scale <= 0) c->margin_scale = 0.1;
+    c->margin = c->margin_scale * dmax;
     const double cmin = (c->rc * (1 + 1e-9) + c->margin) / c->reach;
     int32_t nc = (int32_t)std::floor(c->L / cmin);
     if (nc < 2 * c->reach + 1) nc = 1;
@@ -395,7 +396,7 @@ PairParams pair_params(mds_ctx* c) {
 
 // EvalSinCos + the pair geometry of CompInteractVecs for the current
 // positions: cell lists, contact candidates, sequential overlap correction.
-mds_status prepare_geometry(mds_ctx* c) {
+mds_status prepare_geometry_once(mds_ctx* c) {
     const int64_t n = c->n;
     hipStream_t s = c->stream;
     {
@@ -426,6 +427,23 @@ mds_status prepare_geometry(mds_ctx* c) {
     }
     c->geometry_ready = true;
     return MDS_OK;
+}
+
+// With `check`, the overlap correction's displacement bound is verified
+// before use: a move larger than half the contact-search margin could have
+// hidden a contact, so the search is redone with a 4x wider margin.
+mds_status prepare_geometry(mds_ctx* c, bool check) {
+    for (int attempt = 0;; ++attempt) {
+        HIPCHK(launch_fill_i32(c->stream, c->err_flag.p, 1, 0));
+        mds_status st = prepare_geometry_once(c);
+        if (st != MDS_OK || !check) return st;
+        int32_t f = 0;
+        HIPCHK(hipMemcpyAsync(&f, c->err_flag.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (!(f & 32) || c->margin_scale >= 2.0 || attempt > 4) return MDS_OK;
+        c->margin_scale = std::min(2.0, c->margin_scale * 4);
+        if ((st = plan_grids(c)) != MDS_OK) return st;
+    }
 }
 
 // Real-space + k-space Ewald (+ short range) for the moments in c->mom /
@@ -521,7 +539,7 @@ mds_status step_impl(mds_ctx* c, double* t, double* dt, double* dt_used, double*
                             c->cfg.enable_therm_torque, c->cfg.enable_therm_field, c->cfg.enable_therm_force,
                             c->n_t.p, c->n_b.p, c->n_f.p));
     }
-    mds_status st = prepare_geometry(c);
+    mds_status st = prepare_geometry(c, sync);
     if (st != MDS_OK) return st;
     if ((st = moments_of(c, c->mm.p)) != MDS_OK) return st;
     if ((st = evaluate(c, true)) != MDS_OK) return st;
@@ -795,8 +813,7 @@ mds_status mds_comp_interactions(mds_ctx* c, double* force, double* field, doubl
     mds_status st = check_ready(c);
     if (st != MDS_OK) return st;
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(launch_fill_i32(c->stream, c->err_flag.p, 1, 0));
-    if ((st = prepare_geometry(c)) != MDS_OK) return st;
+    if ((st = prepare_geometry(c, true)) != MDS_OK) return st;
     if ((st = moments_of(c, c->mm.p)) != MDS_OK) return st;
     if ((st = evaluate(c, true)) != MDS_OK) return st;
     if ((st = read_flags(c)) != MDS_OK) return st;
@@ -866,7 +883,7 @@ mds_status mds_run(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz_
     for (int64_t k = 0; k < steps; ++k) {
         double mz = 0;
         const bool want = mz_series != nullptr;
-        st = step_impl(c, t, dt, nullptr, want ? &mz : nullptr, false);
+        st = step_impl(c, t, dt, nullptr, want ? &mz : nullptr, k == 0);
         if (st != MDS_OK) return st;
         if (want) mz_series[k] = mz;
     }

@@ -20,7 +20,7 @@ def test_comp_interactions_matches_reference(case):
     F, B, P, pos = s.comp_interactions(d["s0_pos_in"], d["s0_mm_in"], d["s0_nF"])
     assert rel_err(F, d["s0_force"], f_al) < TOL
     assert rel_err(B, d["s0_field"], b_al) < TOL
-    assert rel_err(P, d["s0_press"]) < TOL
+    assert rel_err(P, d["s0_press"], f_al * 2 * float(d["rHydr"].min())) < TOL
     assert rel_err(pos, d["s0_pos_mid"]) < 1e-14
     pl = s.pairs(d["s0_pos_in"])
     assert len(pl["i"]) == int(d["s0_pairs"][0])
