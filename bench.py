@@ -30,16 +30,22 @@ FLOP_K_BUILD = 21        # S(k) build per (k, particle)
 FLOP_K_SCATTER_F = 38    # force + field scatter per (k, particle)
 FLOP_K_SCATTER_B = 22    # field-only scatter per (k, particle)
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_context(mds, workload, precision, rc_frac, seed):
-    cfg, r, rh, L, pos, mm, ea = workload.system("c3", seed=seed, precision=precision)
-    mu = cfg.sat_mag * 4.0 / 3.0 * cfg.my_pi * r ** 3
-    alpha, rc, nc = workload.ewald_for(cfg, mu, rh, L, rc_frac)
+def make_context(mds, workload, precision, rc_frac, seed, info=None):
+    if info is None:
+        cfg, r, rh, L, pos, mm, ea = workload.system("c3", seed=seed)
+        mu = cfg.sat_mag * 4.0 / 3.0 * cfg.my_pi * r ** 3
+        alpha, rc, nc = workload.ewald_for(cfg, mu, rh, L, rc_frac)
+    else:
+        cfg, r, rh, L, pos, mm, ea = (info[k] for k in ("cfg", "r", "rh", "L", "pos", "mm", "ea"))
+        mu, alpha, rc, nc = info["mu"], info["alpha"], info["rc"], info["nc"]
+    cfg = mds.config_from_string(workload.config_text("c3", seed=seed, precision=precision))
     ctx = mds.Context(cfg)
     ctx.set_particles(r, rh, L)
     ctx.set_ewald(alpha, rc, nc)
@@ -66,7 +72,30 @@ def time_path(ctx, steps, warmup):
     return t1 - t0, stats, counts
 
 
-def roofline(stats, counts, n, nk, steps):
+def tune(ctx, info, args):
+    """Pick the cell reach of the pair search by timing one step of each."""
+    if args.reach > 0:
+        ctx.set_tiling(args.reach)
+    else:
+        best = None
+        for reach in (2, 3, 4):
+            ctx.set_tiling(reach)
+            ctx.set_state(info["pos"], info["mm"], info["ea"])
+            ctx.run(1, 0.0, 1e-12)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.run(1, 0.0, 1e-12)
+            ctx.synchronize()
+            el = time.perf_counter() - t0
+            log(f"  tiling reach={reach} {ctx.tiling()} {el * 1e3:.2f} ms/step")
+            if best is None or el < best[0]:
+                best = (el, reach)
+        ctx.set_tiling(best[1])
+    ctx.set_state(info["pos"], info["mm"], info["ea"])
+    ctx.use_philox(args.seed, 0)
+
+
+def roofline(stats, counts, n, nk, steps, prec):
     """Dominant kernel class, its algorithmic flops per launch over its
     event-timed average duration."""
     lf, _ = stats["pairs_force"]
@@ -85,9 +114,11 @@ def roofline(stats, counts, n, nk, steps):
     avg_s = total_ms / 1e3 / max(1, launches)
     achieved = per_launch[dom] / avg_s / 1e12
     breakdown = {k: {"launches": v[0], "ms_per_step": v[1] / steps} for k, v in stats.items() if v[0]}
-    return {"kernel": dom, "bound": "mfma", "compute_unit": "FP64 vector ALU (no dense contraction: MFMA unused; "
-            "MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved,
-            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+    peak = PEAK_FP64_TFLOPS if prec == "fp64" else PEAK_FP32_TFLOPS
+    unit = ("FP64 vector ALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)" if prec == "fp64" else
+            "FP32 vector ALU (packed FP32 disabled; 157.3 TFLOP/s is the packed peak)")
+    return {"kernel": dom, "bound": "mfma", "compute_unit": unit + "; no dense contraction, MFMA unused",
+            "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
             "avg_launch_ms": avg_s * 1e3, "flop_per_launch": per_launch[dom],
             "unique_pairs_per_launch": pairs_per_launch, "kvecs": nk}, breakdown
 
@@ -123,6 +154,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rc-frac", type=float, default=0.3)
+    ap.add_argument("--reach", type=int, default=0, help="cell reach of the pair search; 0 = autotune")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -136,17 +168,44 @@ def main():
 
     log("building the N=1e5 workload")
     ctx, info = make_context(mds, workload, "fp64", args.rc_frac, args.seed)
-    n, nk = info["n"], None
-    params = ctx.params()
+    n = info["n"]
     nk = ctx.work_counts()["kvecs"]
     log(f"L={info['L']:.4e} alpha*L={info['alpha'] * info['L']:.2f} rc/L={args.rc_frac} nc={info['nc']} nk={nk}")
-    el, stats, counts = time_path(ctx, args.steps, args.warmup)
-    value = n * args.steps / el
-    roof, breakdown = roofline(stats, counts, n, nk, args.steps)
-    log(f"fp64: {value:.4e} particle-steps/s, {el / args.steps * 1e3:.3f} ms/step")
-    for k, v in breakdown.items():
-        log(f"  {k:16s} {v['ms_per_step']:8.3f} ms/step ({v['launches']} launches)")
+    results = {}
+    for prec in ("fp64", "mixed"):
+        if prec == "mixed":
+            ctx.close()
+            ctx, _ = make_context(mds, workload, "mixed", args.rc_frac, args.seed, info)
+        tune(ctx, info, args)
+        tiling = ctx.tiling()
+        el, stats, counts = time_path(ctx, args.steps, args.warmup)
+        value = n * args.steps / el
+        roof, breakdown = roofline(stats, counts, n, nk, args.steps, prec)
+        log(f"{prec}: {value:.4e} particle-steps/s, {el / args.steps * 1e3:.3f} ms/step, tiling {tiling}")
+        for k, v in breakdown.items():
+            log(f"  {k:16s} {v['ms_per_step']:8.3f} ms/step ({v['launches']} launches)")
+        # one CompInteractions of the bench's initial state, for the error of
+        # the mixed path against the FP64 path (itself pinned to the oracle)
+        ctx.set_state(info["pos"], info["mm"], info["ea"])
+        ctx.set_noise(None, None, None)
+        F, B, _, _ = ctx.comp_interactions()
+        results[prec] = {"value": value, "el": el, "roof": roof, "breakdown": breakdown, "tiling": tiling,
+                         "F": F, "B": B}
     ctx.close()
+    cfg = info["cfg"]
+    m_min, rh_min = float(info["mu"].min()), float(info["rh"].min())
+    f_al = 3 * cfg.mu0 * m_min ** 2 / (2 * cfg.my_pi * (2 * rh_min) ** 4)
+    b_al = cfg.mu0 * m_min / (2 * cfg.my_pi * (2 * rh_min) ** 3)
+    mom = info["mm"] * info["mu"][:, None]
+    rel = lambda a, b, fl: float(np.abs(a - b).max() / max(np.abs(b).max(), fl))
+    Bf, Bm, Ff, Fm = results["fp64"]["B"], results["mixed"]["B"], results["fp64"]["F"], results["mixed"]["F"]
+    mixed_err = {"field": rel(Bm, Bf, b_al), "force": rel(Fm, Ff, f_al),
+                 "torque": rel(np.cross(mom, Bm), np.cross(mom, Bf), 0.0),
+                 "norm": "max|mixed - fp64| / max(max|fp64|, contact value), over all particles and components"}
+    log(f"mixed vs fp64 error: {mixed_err}")
+    value = results["mixed"]["value"]
+    el = results["mixed"]["el"]
+    roof, breakdown = results["mixed"]["roof"], results["mixed"]["breakdown"]
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline: one oracle Heun step at N={n} on {args.cpu_threads} threads")
@@ -158,14 +217,17 @@ def main():
         "metric": "particle-steps/s at N=1e5 mixed vs FP64; p50/p99 ensemble-request latency",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "mixed (f32 real-space pair terms, f64 geometry/accumulation/k-space)", "data": "synthetic",
         "config": {"workload": "C3: N=100000 lognormal core (median 10 nm radius, sigma_ln 0.25) + 5 nm shell, "
                                "phi=0.10, B=50 mT z, alpha_G=0.05, T=300 K, eta=1e-3 Pa s, Ewald 1e-6, "
                                "fixed dt=1e-12 s Heun, Philox noise",
-                   "n": n, "precision": "fp64", "rc_over_L": args.rc_frac, "alpha_L": info["alpha"] * info["L"],
-                   "nc": info["nc"], "kvecs": nk},
-        "fp64_value": value, "mixed_value": None,
-        "roofline": roof, "kernel_ms_per_step": breakdown, "cpu_baseline": cpu,
+                   "n": n, "precision": "mixed", "rc_over_L": args.rc_frac, "alpha_L": info["alpha"] * info["L"],
+                   "nc": info["nc"], "kvecs": nk, "tiling": results["mixed"]["tiling"]},
+        "fp64_value": results["fp64"]["value"], "mixed_value": results["mixed"]["value"],
+        "mixed_over_fp64": results["mixed"]["value"] / results["fp64"]["value"],
+        "mixed_err_vs_fp64": mixed_err,
+        "roofline": roof, "roofline_fp64": results["fp64"]["roof"], "kernel_ms_per_step": breakdown,
+        "kernel_ms_per_step_fp64": results["fp64"]["breakdown"], "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

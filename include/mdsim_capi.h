@@ -148,6 +148,12 @@ mds_status mds_set_particles(mds_ctx* ctx, int64_t n, const double* r_mag, const
 /* Ewald parameters: InitEwaldConst (src/interaction.cpp:353-374), with alpha,
  * r_c and n_c given explicitly. */
 mds_status mds_set_ewald(mds_ctx* ctx, double alpha, double r_cut, int32_t nc);
+/* Tiling of the real-space pair search (no counterpart in the reference,
+ * whose pair scan is O(N^2), src/interaction.cpp:94-134): neighbour cells of
+ * edge >= (r_c + margin) / cell_reach, searched over a culled stencil of
+ * +-cell_reach cells. Default 3; chosen by the project's autotuner. */
+mds_status mds_set_tiling(mds_ctx* ctx, int32_t cell_reach);
+mds_status mds_get_tiling(const mds_ctx* ctx, int32_t* cell_reach, int32_t* ncell, int32_t* nstencil);
 /* Read back box edge, r_c(SR), alpha, r_c, n_c. */
 mds_status mds_get_params(const mds_ctx* ctx, double* l_box, double* r_cut_sr, double* alpha,
                           double* r_cut, int32_t* nc);

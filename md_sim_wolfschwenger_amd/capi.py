@@ -82,6 +82,8 @@ SIGNATURES = [
     ("mds_device_count", C.c_int, []),
     ("mds_set_particles", C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, C.c_double]),
     ("mds_set_ewald", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32]),
+    ("mds_set_tiling", C.c_int, [C.c_void_p, C.c_int32]),
+    ("mds_get_tiling", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("mds_get_params", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)]),
     ("mds_set_state", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     ("mds_get_state", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
@@ -189,6 +191,14 @@ class Context:
 
     def set_ewald(self, alpha, r_cut, nc):
         self._chk(self._L.mds_set_ewald(self.h, float(alpha), float(r_cut), int(nc)), "set_ewald")
+
+    def set_tiling(self, cell_reach):
+        self._chk(self._L.mds_set_tiling(self.h, int(cell_reach)), "set_tiling")
+
+    def tiling(self):
+        v = [C.c_int32() for _ in range(3)]
+        self._chk(self._L.mds_get_tiling(self.h, *[C.byref(x) for x in v]), "get_tiling")
+        return {"cell_reach": v[0].value, "ncell": v[1].value, "nstencil": v[2].value}
 
     def params(self):
         v = [C.c_double() for _ in range(4)]

@@ -145,15 +145,18 @@ struct mds_ctx {
     // noise, outputs
     DevBuf<double> n_t, n_b, n_f, force, field, mom;
     DevBuf<double4> rec_pos, rec_mom;
+    DevBuf<float4> rec_pos32;
+    double mref = 1.0;  // largest magnetic moment (unit of the FP32 pair path)
     // integrator
     DevBuf<double> kmm, kea, pt[2], new_mm, new_ea;
     // cell lists
-    int32_t ncell = 1, reach = 2, fncell = 1;
+    int32_t ncell = 1, reach = 3, fncell = 1;
     double cell = 0, fcell = 0, margin = 0, margin_scale = 0.1;
-    int32_t work_cap = 0, cand_cap = 0;
+    int32_t work_cap = 0, cand_cap = 0, nsten = 0;
+    DevBuf<int4> stencil;
     DevBuf<int32_t> key_main, key_fine, cnt_main, start_main, end_main, cur_main, sorted_main, inv_perm, work, nwork;
     DevBuf<int32_t> cnt_fine, start_fine, end_fine, cur_fine, sorted_fine;
-    DevBuf<int32_t> cand_count, cand_off, cand_j, ev_head, ev_tail, ev_next, n_events, err_flag;
+    DevBuf<int32_t> cand_count, cand_off, cand_j, ev_head, ev_tail, ev_next, n_events, err_flag, moved_list, n_moved;
     DevBuf<uint64_t> ev_key;
     DevBuf<double> ev_pos, max_disp;
     // k space
@@ -278,6 +281,23 @@ mds_status plan_grids(mds_ctx* c) {
     fn = std::min(fn, 128);
     c->fncell = fn;
     c->fcell = c->L / fn;
+    // stencil of neighbour-cell offsets whose nearest points lie within the
+    // candidate radius (corner cells of the cube of +-reach are dropped)
+    {
+        std::vector<int4> st;
+        const int R = nc == 1 ? 0 : c->reach;
+        const double lim = c->rc * (1 + 1e-9) + c->margin;
+        for (int dz = -R; dz <= R; ++dz)
+            for (int dy = -R; dy <= R; ++dy)
+                for (int dx = -R; dx <= R; ++dx) {
+                    auto gap = [&](int o) { return std::max(std::abs(o) - 1, 0) * c->cell; };
+                    const double d2 = gap(dx) * gap(dx) + gap(dy) * gap(dy) + gap(dz) * gap(dz);
+                    if (d2 < lim * lim) st.push_back(make_int4(dx, dy, dz, 0));
+                }
+        c->nsten = (int32_t)st.size();
+        HIPCHK(c->stencil.ensure(st.size()));
+        HIPCHK(hipMemcpy(c->stencil.p, st.data(), st.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
     const int64_t ncm = (int64_t)nc * nc * nc, ncf = (int64_t)fn * fn * fn;
     c->work_cap = (int32_t)((n + 63) / 64 + ncm);
     c->cand_cap = (int32_t)(2 * n + 4096);
@@ -294,7 +314,8 @@ mds_status plan_grids(mds_ctx* c) {
     HIPCHK(c->ev_next.ensure(2 * (size_t)c->cand_cap));
     HIPCHK(c->ev_key.ensure(2 * (size_t)c->cand_cap));
     HIPCHK(c->ev_pos.ensure(6 * (size_t)c->cand_cap));
-    HIPCHK(c->press_partial.ensure(3 * (size_t)c->work_cap));
+    HIPCHK(c->press_partial.ensure(3 * ((size_t)c->work_cap + 1)));
+    HIPCHK(c->moved_list.ensure(2 * (size_t)c->cand_cap));
     return MDS_OK;
 }
 
@@ -372,6 +393,10 @@ Geometry geometry(mds_ctx* c) {
     g.ev_pos = c->ev_pos.p;
     g.n_events = c->n_events.p;
     g.max_disp = c->max_disp.p;
+    g.moved_list = c->moved_list.p;
+    g.n_moved = c->n_moved.p;
+    g.inv_perm = c->inv_perm.p;
+    g.rec_pos32 = c->rec_pos32.p;
     return g;
 }
 
@@ -391,6 +416,21 @@ PairParams pair_params(mds_ctx* c) {
     pp.kappa = c->cfg.kappa;
     pp.hamaker = c->cfg.hamaker;
     pp.r_mag = c->r_mag.p;
+    const double f = c->cfg.fac_ewald;
+    if (c->cfg.precision == MDS_PREC_MIXED) {
+        pp.ell = c->rc;
+        pp.inv_ell = 1.0 / c->rc;
+        pp.inv_mref = 1.0 / c->mref;
+        pp.bscale = f * c->mref / std::pow(pp.ell, 3.0);
+        pp.fscale = f * c->mref * c->mref / std::pow(pp.ell, 4.0);
+        pp.pscale = pp.ell * pp.fscale;
+    } else {
+        pp.ell = pp.inv_ell = pp.inv_mref = 1.0;
+        pp.bscale = pp.fscale = pp.pscale = f;
+    }
+    pp.al_s = (float)(c->alpha * pp.ell);
+    pp.al2_s = pp.al_s * pp.al_s;
+    pp.al4_s = pp.al2_s * pp.al2_s;
     return pp;
 }
 
@@ -405,7 +445,8 @@ mds_status prepare_geometry_once(mds_ctx* c) {
                               c->fncell, c->fcell, c->key_fine.p));
         HIPCHK(launch_cell_sort(s, n, c->key_main.p, c->ncell * c->ncell * c->ncell, c->cnt_main.p, c->start_main.p,
                                 c->end_main.p, c->cur_main.p, c->sorted_main.p, nullptr));
-        HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->r_mag.p, c->rec_pos.p, c->inv_perm.p));
+        HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->r_mag.p, c->rec_pos.p, c->rec_pos32.p,
+                                    c->inv_perm.p));
         HIPCHK(launch_build_work(s, c->ncell * c->ncell * c->ncell, c->start_main.p, c->end_main.p, c->work.p,
                                  c->nwork.p, c->work_cap, c->err_flag.p));
     }
@@ -419,7 +460,8 @@ mds_status prepare_geometry_once(mds_ctx* c) {
                                c->cand_j.p, c->cand_cap, nullptr, c->err_flag.p));
         HIPCHK(launch_resolve(s, n, c->L, c->rc, c->r_mag.p, c->cand_off.p, c->cand_j.p, c->pos_cur.p, c->pos0.p,
                               c->ev_head.p, c->ev_tail.p, c->ev_next.p, c->ev_key.p, c->ev_pos.p, 2 * c->cand_cap,
-                              c->n_events.p, c->max_disp.p, 0.5 * c->margin, c->err_flag.p));
+                              c->n_events.p, c->max_disp.p, 0.5 * c->margin, c->moved_list.p, c->n_moved.p,
+                              c->err_flag.p));
     }
     {
         mds_ctx::Span sp(c, KC_SINCOS);
@@ -458,8 +500,11 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
     PairOut po{force_pass ? c->force.p : nullptr, c->field.p, c->press_partial.p};
     {
         mds_ctx::Span sp(c, force_pass ? KC_PAIRS_FORCE : KC_PAIRS_FIELD);
-        HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap,
-                            c->count_work ? (int64_t*)c->pair_counter.p : nullptr));
+        int64_t* cnt = c->count_work ? (int64_t*)c->pair_counter.p : nullptr;
+        if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + 3 * (size_t)c->work_cap, 3, 0.0));
+        HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p,
+                            c->nsten, cnt));
+        HIPCHK(launch_pairs_moved(s, force_pass, g, c->rec_mom.p, pp, po, c->stencil.p, c->nsten, c->work_cap, cnt));
     }
     if (c->nk > 0) {
         {
@@ -637,7 +682,7 @@ mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out) {
         delete c;
         return MDS_ECONFIG;
     }
-    if (cfg->precision != MDS_PREC_FP64) {
+    if (cfg->precision != MDS_PREC_FP64 && cfg->precision != MDS_PREC_MIXED) {
         delete c;
         return MDS_EINVAL;
     }
@@ -648,7 +693,7 @@ mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out) {
     if (c->step_counter.ensure(1) != hipSuccess || c->pair_counter.ensure(1) != hipSuccess ||
         c->series_counter.ensure(1) != hipSuccess || c->scal.ensure(8) != hipSuccess ||
         c->msum.ensure(3) != hipSuccess || c->n_events.ensure(1) != hipSuccess || c->err_flag.ensure(1) != hipSuccess ||
-        c->max_disp.ensure(1) != hipSuccess || c->nwork.ensure(1) != hipSuccess) {
+        c->max_disp.ensure(1) != hipSuccess || c->nwork.ensure(1) != hipSuccess || c->n_moved.ensure(1) != hipSuccess) {
         delete c;
         return MDS_ENOMEM;
     }
@@ -658,6 +703,7 @@ mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out) {
     (void)hipMemset(c->err_flag.p, 0, sizeof(int32_t));
     (void)hipMemset(c->n_events.p, 0, sizeof(int32_t));
     (void)hipMemset(c->max_disp.p, 0, sizeof(double));
+    (void)hipMemset(c->n_moved.p, 0, sizeof(int32_t));
     *out = c;
     return MDS_OK;
 }
@@ -705,6 +751,8 @@ mds_status mds_set_particles(mds_ctx* c, int64_t n, const double* r_mag, const d
     HIPCHK(c->kmm.ensure(7 * n3));
     HIPCHK(c->kea.ensure(7 * n3));
     HIPCHK(c->rec_pos.ensure(n));
+    HIPCHK(c->rec_pos32.ensure(n));
+    c->mref = *std::max_element(mu.begin(), mu.end());
     HIPCHK(c->rec_mom.ensure(n));
     for (auto* b : {&c->key_main, &c->key_fine, &c->sorted_main, &c->sorted_fine, &c->inv_perm, &c->cand_count,
                     &c->ev_head, &c->ev_tail})
@@ -743,6 +791,22 @@ mds_status mds_set_ewald(mds_ctx* c, double alpha, double r_cut, int32_t nc) {
     if ((st = plan_kspace(c)) != MDS_OK) return st;
     c->ewald_set = true;
     c->geometry_ready = false;
+    return MDS_OK;
+}
+
+mds_status mds_set_tiling(mds_ctx* c, int32_t cell_reach) {
+    if (!c || cell_reach < 1 || cell_reach > 8) return fail(c, MDS_EINVAL, "cell reach must be 1..8");
+    c->reach = cell_reach;
+    c->geometry_ready = false;
+    if (c->ewald_set) return plan_grids(c);
+    return MDS_OK;
+}
+
+mds_status mds_get_tiling(const mds_ctx* c, int32_t* cell_reach, int32_t* ncell, int32_t* nstencil) {
+    if (!c) return MDS_EINVAL;
+    if (cell_reach) *cell_reach = c->reach;
+    if (ncell) *ncell = c->ncell;
+    if (nstencil) *nstencil = c->nsten;
     return MDS_OK;
 }
 
@@ -829,6 +893,9 @@ mds_status mds_comp_interactions(mds_ctx* c, double* force, double* field, doubl
         press[0] = press[1] = press[2] = 0;
         for (int32_t w = 0; w < nw; ++w)
             for (int d = 0; d < 3; ++d) press[d] += pp[3 * (size_t)w + d];
+        double pm[3];
+        HIPCHK(hipMemcpy(pm, c->press_partial.p + 3 * (size_t)c->work_cap, sizeof(pm), hipMemcpyDeviceToHost));
+        for (int d = 0; d < 3; ++d) press[d] += pm[d];
     }
     return MDS_OK;
 }

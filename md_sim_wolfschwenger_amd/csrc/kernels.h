@@ -41,6 +41,10 @@ struct Geometry {
     const double* ev_pos;        // [cap*3] position after that event
     const int32_t* n_events;     // device scalar
     const double* max_disp;      // device scalar
+    const int32_t* moved_list;   // original indices of moved particles
+    const int32_t* n_moved;      // device scalar
+    const int32_t* inv_perm;     // sorted slot of an original index
+    const float4* rec_pos32;     // FP32 copy of rec_pos (candidate search of the mixed path)
 };
 
 struct PairOut {
@@ -51,6 +55,11 @@ struct PairOut {
 
 struct PairParams {
     double alpha, alpha2, alpha4, ir_pi, fac, rc, rc_sr;
+    // FP32 path: lengths in units of ell, moments in units of mref; the sums
+    // are rescaled by bscale (field), fscale (force), pscale (pressure) when
+    // written (FP64 path: ell = mref = 1, scales = facEwald)
+    double ell, inv_ell, inv_mref, bscale, fscale, pscale;
+    float al_s, al2_s, al4_s;
     // short-range (coating) constants
     int32_t coating;
     double eff_len, ster_fac, el_fac, kappa, hamaker;
@@ -66,7 +75,7 @@ hipError_t launch_fill_f64(hipStream_t s, double* p, int64_t count, double v);
 hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
                             int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp);
 hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_idx, const double* pos0,
-                                const double* r_mag, double4* rec_pos, int32_t* inv_perm);
+                                const double* r_mag, double4* rec_pos, float4* rec_pos32, int32_t* inv_perm);
 hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
                              int32_t* work, int32_t* nwork, int32_t cap, int32_t* err);
 // contact search + sequential overlap resolution
@@ -77,7 +86,8 @@ hipError_t launch_contacts(hipStream_t s, int64_t n, double L, const double* pos
 hipError_t launch_resolve(hipStream_t s, int64_t n, double L, double rc, const double* r_mag, const int32_t* cand_off,
                           const int32_t* cand_j, double* pos_cur, const double* pos0, int32_t* ev_head,
                           int32_t* ev_tail, int32_t* ev_next, uint64_t* ev_key, double* ev_pos, int32_t ev_cap,
-                          int32_t* n_events, double* max_disp, double disp_limit, int32_t* err);
+                          int32_t* n_events, double* max_disp, double disp_limit, int32_t* moved_list,
+                          int32_t* n_moved, int32_t* err);
 hipError_t launch_clear_events(hipStream_t s, int64_t n, int32_t* ev_head, int32_t* ev_tail);
 
 hipError_t launch_moments(hipStream_t s, int64_t n, const double* mm, const double* mag_mom, const int32_t* inv_perm,
@@ -85,7 +95,12 @@ hipError_t launch_moments(hipStream_t s, int64_t n, const double* mm, const doub
 hipError_t launch_reduce_sum3(hipStream_t s, const double* partial, int32_t count, double* out);
 
 hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
-                        const PairParams& pp, const PairOut& out, int32_t max_work, int64_t* cnt_pairs);
+                        const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
+                        int32_t nsten, int64_t* cnt_pairs);
+// Pairs with an endpoint moved by the overlap correction (after launch_pairs).
+hipError_t launch_pairs_moved(hipStream_t s, bool force, const Geometry& g, const double4* rec_mom,
+                              const PairParams& pp, const PairOut& out, const int4* stencil, int32_t nsten,
+                              int32_t press_off, int64_t* cnt_pairs);
 
 // k-space
 hipError_t launch_sincos_tables(hipStream_t s, int64_t n, int32_t nc, double L, double two_pi, const double* pos0,

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "erfcx_coef.h"
 #include "kernels.h"
 
 namespace mds {
@@ -141,11 +142,14 @@ __global__ void k_place(int64_t n, const int32_t* __restrict__ key, int32_t* __r
 }
 
 __global__ void k_records(int64_t n, const int32_t* __restrict__ sorted_idx, const double* __restrict__ pos0,
-                          const double* __restrict__ r_mag, double4* __restrict__ rec, int32_t* __restrict__ inv_perm) {
+                          const double* __restrict__ r_mag, double4* __restrict__ rec, float4* __restrict__ rec32,
+                          int32_t* __restrict__ inv_perm) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int32_t i = sorted_idx[k];
-    rec[k] = make_double4(pos0[i], pos0[n + i], pos0[2 * n + i], r_mag[i]);
+    const double4 r = make_double4(pos0[i], pos0[n + i], pos0[2 * n + i], r_mag[i]);
+    rec[k] = r;
+    rec32[k] = make_float4((float)r.x, (float)r.y, (float)r.z, (float)r.w);
     inv_perm[i] = (int32_t)k;
 }
 
@@ -265,11 +269,11 @@ __global__ void k_resolve(int64_t n, double L, double rc, const double* __restri
                           int32_t* __restrict__ ev_head, int32_t* __restrict__ ev_tail, int32_t* __restrict__ ev_next,
                           uint64_t* __restrict__ ev_key, double* __restrict__ ev_pos, int32_t ev_cap,
                           int32_t* __restrict__ n_events, double* __restrict__ max_disp, double disp_limit,
-                          int32_t* __restrict__ err) {
+                          int32_t* __restrict__ moved_list, int32_t* __restrict__ n_moved, int32_t* __restrict__ err) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     // candidates beyond the list capacity were never written (flagged by k_contacts)
     const int32_t total = min(*ncand, ev_cap / 2);
-    int32_t nev = 0;
+    int32_t nev = 0, nmv = 0;
     double md = 0;
     for (int32_t k = 0; k < total; ++k) {
         const int32_t i = cand_i[k], j = cand_j[k];
@@ -303,14 +307,16 @@ __global__ void k_resolve(int64_t n, double L, double rc, const double* __restri
             }
             md = fmax(md, sqrt(dsq));
             ev_next[e] = -1;
-            if (ev_tail[p] < 0)
+            if (ev_tail[p] < 0) {
                 ev_head[p] = e;
-            else
+                moved_list[nmv++] = p;
+            } else
                 ev_next[ev_tail[p]] = e;
             ev_tail[p] = e;
         }
     }
     *n_events = nev;
+    *n_moved = nmv;
     *max_disp = md;
     if (md > disp_limit) atomicOr(err, 32);
 }
@@ -395,128 +401,192 @@ struct IWave {
     int32_t iorig[WAVE];
 };
 
-__device__ __forceinline__ void version_at(const Geometry& g, int32_t p, uint64_t key, double& x, double& y, double& z) {
-    int32_t e = g.ev_head[p];
+// Positions of lo and hi as the reference's sequential scan sees them when it
+// reaches pair (lo, hi): the last overlap move of each with a smaller
+// lexicographic key; and whether this pair itself was moved.
+__device__ __forceinline__ void version_pair(const Geometry& g, int32_t lo, int32_t hi, double& xl, double& yl,
+                                             double& zl, double& xh, double& yh, double& zh, int& event) {
+    const uint64_t key = (uint64_t)lo * (uint64_t)g.n + (uint64_t)hi;
+    int32_t e = g.ev_head[lo];
     while (e >= 0 && g.ev_key[e] < key) {
-        x = g.ev_pos[3 * (int64_t)e];
-        y = g.ev_pos[3 * (int64_t)e + 1];
-        z = g.ev_pos[3 * (int64_t)e + 2];
+        xl = g.ev_pos[3 * (int64_t)e];
+        yl = g.ev_pos[3 * (int64_t)e + 1];
+        zl = g.ev_pos[3 * (int64_t)e + 2];
+        e = g.ev_next[e];
+    }
+    event = (e >= 0 && g.ev_key[e] == key) ? 1 : 0;
+    e = g.ev_head[hi];
+    while (e >= 0 && g.ev_key[e] < key) {
+        xh = g.ev_pos[3 * (int64_t)e];
+        yh = g.ev_pos[3 * (int64_t)e + 1];
+        zh = g.ev_pos[3 * (int64_t)e + 2];
         e = g.ev_next[e];
     }
 }
-__device__ __forceinline__ bool is_event(const Geometry& g, int32_t lo, uint64_t key) {
-    for (int32_t e = g.ev_head[lo]; e >= 0; e = g.ev_next[e]) {
-        if (g.ev_key[e] == key) return true;
-        if (g.ev_key[e] > key) break;
-    }
-    return false;
+
+__device__ __forceinline__ double erfcx64(double x) {
+    const double t = 1.0 - 2.0 * kErfcxK64 / (x + kErfcxK64);
+    double p = kErfcx64[kErfcxDeg64];
+#pragma unroll
+    for (int k = kErfcxDeg64 - 1; k >= 0; --k) p = fma(p, t, kErfcx64[k]);
+    return p;
+}
+__device__ __forceinline__ float erfcx32(float x) {
+    const float t = 1.0f - 2.0f * kErfcxK32 * __frcp_rn(x + kErfcxK32);
+    float p = kErfcx32[kErfcxDeg32];
+#pragma unroll
+    for (int k = kErfcxDeg32 - 1; k >= 0; --k) p = fmaf(p, t, kErfcx32[k]);
+    return p;
 }
 
-template <bool FORCE>
+// Terms of one pair (CompDipInteractEwaldR, src/interaction.cpp:396-417):
+// field on i from j, and (FORCE) the force on j (applied -F to i), from the
+// separation r, its reciprocal, the unit vector u from i to j and both
+// moments. FP64 path in SI units without the facEwald factor; FP32 path in
+// units of ell and mref (see PairParams). erfc(alpha r) is
+// exp(-(alpha r)^2) erfcx(alpha r), sharing the Gaussian of the t term.
+template <bool FORCE, bool MIXED>
+__device__ __forceinline__ void pair_terms(const PairParams& pp, double r, double ir, double ux, double uy, double uz,
+                                           const double4& mi, const double4& mj, double B[3], double F[3]) {
+    if (!MIXED) {
+        const double ir2 = ir * ir;
+        const double x = pp.alpha * r;
+        const double gau = exp(-x * x);
+        const double t = 2.0 * pp.alpha * gau * pp.ir_pi * ir2;
+        const double a1 = gau * erfcx64(x) * (ir2 * ir) + t;
+        const double a2 = 3.0 * a1 * ir2 + 2.0 * pp.alpha2 * t;
+        const double dj = mj.x * ux + mj.y * uy + mj.z * uz;
+        const double g2 = a2 * dj * r * r;
+        B[0] = g2 * ux - a1 * mj.x;
+        B[1] = g2 * uy - a1 * mj.y;
+        B[2] = g2 * uz - a1 * mj.z;
+        if (FORCE) {
+            const double di = mi.x * ux + mi.y * uy + mi.z * uz;
+            const double a3 = 5.0 * a2 * ir2 + 4.0 * pp.alpha4 * t;
+            const double mimj = mi.x * mj.x + mi.y * mj.y + mi.z * mj.z;
+            const double c0 = (a2 * mimj - a3 * di * dj * r * r) * r;
+            const double c1 = a2 * r;
+            F[0] = c0 * ux + c1 * (dj * mi.x + di * mj.x);
+            F[1] = c0 * uy + c1 * (dj * mi.y + di * mj.y);
+            F[2] = c0 * uz + c1 * (dj * mi.z + di * mj.z);
+        }
+    } else {
+        const float rs = (float)(r * pp.inv_ell);
+        const float irs = (float)(ir * pp.ell);
+        const float irs2 = irs * irs;
+        const float fx = (float)ux, fy = (float)uy, fz = (float)uz;
+        const float mjx = (float)(mj.x * pp.inv_mref), mjy = (float)(mj.y * pp.inv_mref),
+                    mjz = (float)(mj.z * pp.inv_mref);
+        const float x = pp.al_s * rs;
+        const float gau = __expf(-x * x);
+        const float t = 2.0f * pp.al_s * gau * 0.5641895835477563f * irs2;
+        const float a1 = gau * erfcx32(x) * (irs2 * irs) + t;
+        const float a2 = 3.0f * a1 * irs2 + 2.0f * pp.al2_s * t;
+        const float dj = mjx * fx + mjy * fy + mjz * fz;
+        const float g2 = a2 * dj * rs * rs;
+        B[0] = (double)(g2 * fx - a1 * mjx);
+        B[1] = (double)(g2 * fy - a1 * mjy);
+        B[2] = (double)(g2 * fz - a1 * mjz);
+        if (FORCE) {
+            const float mix = (float)(mi.x * pp.inv_mref), miy = (float)(mi.y * pp.inv_mref),
+                        miz = (float)(mi.z * pp.inv_mref);
+            const float di = mix * fx + miy * fy + miz * fz;
+            const float a3 = 5.0f * a2 * irs2 + 4.0f * pp.al4_s * t;
+            const float mimj = mix * mjx + miy * mjy + miz * mjz;
+            const float c0 = (a2 * mimj - a3 * di * dj * rs * rs) * rs;
+            const float c1 = a2 * rs;
+            F[0] = (double)(c0 * fx + c1 * (dj * mix + di * mjx));
+            F[1] = (double)(c0 * fy + c1 * (dj * miy + di * mjy));
+            F[2] = (double)(c0 * fz + c1 * (dj * miz + di * mjz));
+        }
+    }
+}
+
+// Short-range forces (CompSterRep, CompElectrostatRep, CompForceVdW,
+// src/interaction.cpp:766-832) of a pair at stored length r, as the
+// magnitude along u(lo -> hi) of the force on hi; the steric shell diameter
+// uses r_lo only, as the reference does.
+__device__ __forceinline__ double short_range(const PairParams& pp, double r, double rl, double rh) {
+    const double dDelta = 2 * (rl + pp.eff_len);
+    double sf = 0;
+    if (r <= dDelta) sf = (2 * rh / (rl + rh)) * pp.ster_fac * rl * rl * log(dDelta / r);
+    sf += (rl * rh / (rl + rh)) * pp.el_fac * exp(-pp.kappa * (r - rl - rh));
+    const double q1 = (r - rl - rh) * (r + rl + rh);
+    const double q2 = r * r - (rl - rh) * (rl - rh);
+    sf += -pp.hamaker * r * pow(4 * rl * rh, 3.0) / (6 * (q1 * q1) * (q2 * q2));
+    return sf;
+}
+
+// One queued pair (lane il of the wave, sorted index j). Pairs with an
+// endpoint moved by the overlap correction are left to k_pairs_moved.
+template <bool FORCE, bool COATING, bool MIXED>
 __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __restrict__ rec_mom, const PairParams& pp,
                                            IWave& W, int32_t entry, bool active, bool moved_any, double press[3],
                                            int64_t& npairs) {
+    if (!active) return;
     const int il = entry & 63;
     const int32_t j = entry >> 6;
-    double Fi[3] = {0, 0, 0}, Bi[3] = {0, 0, 0};
-    if (active) {
-        const double4 pi = W.ipos[il];
-        const double4 mi4 = W.imom[il];
-        const int32_t oi = W.iorig[il];
-        const double4 pj = g.rec_pos[j];
-        const double4 mj4 = rec_mom[j];
-        const int32_t oj = g.sorted_idx[j];
-        const bool i_lo = oi < oj;
-        const int32_t lo = i_lo ? oi : oj, hi = i_lo ? oj : oi;
-        double xl = i_lo ? pi.x : pj.x, yl = i_lo ? pi.y : pj.y, zl = i_lo ? pi.z : pj.z;
-        double xh = i_lo ? pj.x : pi.x, yh = i_lo ? pj.y : pi.y, zh = i_lo ? pj.z : pi.z;
-        bool event = false;
-        if (moved_any) {
-            const uint64_t key = (uint64_t)lo * (uint64_t)g.n + (uint64_t)hi;
-            version_at(g, lo, key, xl, yl, zl);
-            version_at(g, hi, key, xh, yh, zh);
-            event = is_event(g, lo, key);
+    const int32_t oi = W.iorig[il];
+    const int32_t oj = g.sorted_idx[j];
+    if (moved_any && (g.ev_head[oi] >= 0 || g.ev_head[oj] >= 0)) return;
+    const double4 pi = W.ipos[il];
+    const double4 pj = g.rec_pos[j];
+    const bool i_lo = oi < oj;
+    // canonical i < j orientation of CompInteractVecs (src/interaction.cpp:99-104)
+    const double L = g.L;
+    const double vx = min_image(i_lo ? pj.x - pi.x : pi.x - pj.x, L);
+    const double vy = min_image(i_lo ? pj.y - pi.y : pi.y - pj.y, L);
+    const double vz = min_image(i_lo ? pj.z - pi.z : pi.z - pj.z, L);
+    const double r = sqrt(vx * vx + vy * vy + vz * vz);
+    if (!(r < pp.rc)) return;
+    ++npairs;
+    const double ir = 1.0 / r;
+    const double sir = i_lo ? ir : -ir;  // u from i to j
+    const double ux = vx * sir, uy = vy * sir, uz = vz * sir;
+    double B[3], F[3];
+    pair_terms<FORCE, MIXED>(pp, r, ir, ux, uy, uz, W.imom[il], rec_mom[j], B, F);
+    atomicAdd(&W.acc[3][il], B[0]);
+    atomicAdd(&W.acc[4][il], B[1]);
+    atomicAdd(&W.acc[5][il], B[2]);
+    if (FORCE) {
+        if (i_lo) {
+            const double rr = MIXED ? r * pp.inv_ell : r;
+            press[0] += rr * ux * F[1];
+            press[1] += rr * ux * F[2];
+            press[2] += rr * uy * F[2];
         }
-        const double L = g.L;
-        const double vx = min_image(xh - xl, L), vy = min_image(yh - yl, L), vz = min_image(zh - zl, L);
-        const double r0 = sqrt(vx * vx + vy * vy + vz * vz);
-        if (r0 < pp.rc) {
-            ++npairs;
-            const double ri = pi.w, rj = pj.w;
-            const double r = event ? ri + rj + 1e-10 : r0;
-            const double sgn = i_lo ? 1.0 : -1.0;  // u from i to j
-            const double ux = sgn * (vx / r0), uy = sgn * (vy / r0), uz = sgn * (vz / r0);
-            const double r2 = r * r;
-            const double t = 2.0 * pp.alpha * exp(-pp.alpha2 * r2) * pp.ir_pi / r2;
-            const double a1 = erfc(pp.alpha * r) / (r2 * r) + t;
-            const double a2 = 3.0 * a1 / r2 + 2.0 * pp.alpha2 * t;
-            const double di = mi4.x * ux + mi4.y * uy + mi4.z * uz;
-            const double dj = mj4.x * ux + mj4.y * uy + mj4.z * uz;
-            const double f = pp.fac;
-            Bi[0] = f * (a2 * dj * ux * r2 - a1 * mj4.x);
-            Bi[1] = f * (a2 * dj * uy * r2 - a1 * mj4.y);
-            Bi[2] = f * (a2 * dj * uz * r2 - a1 * mj4.z);
-            if (FORCE) {
-                const double a3 = 5.0 * a2 / r2 + 4.0 * pp.alpha4 * t;
-                const double mimj = mi4.x * mj4.x + mi4.y * mj4.y + mi4.z * mj4.z;
-                const double c0 = (a2 * mimj - a3 * di * dj * r2) * r;
-                const double c1 = a2 * r;
-                // force on j from the pair (applied -F to i)
-                const double Fx = f * (c0 * ux + c1 * (dj * mi4.x + di * mj4.x));
-                const double Fy = f * (c0 * uy + c1 * (dj * mi4.y + di * mj4.y));
-                const double Fz = f * (c0 * uz + c1 * (dj * mi4.z + di * mj4.z));
-                Fi[0] = -Fx;
-                Fi[1] = -Fy;
-                Fi[2] = -Fz;
-                if (i_lo) {
-                    press[0] += r * ux * Fy;
-                    press[1] += r * ux * Fz;
-                    press[2] += r * uy * Fz;
-                }
-                if (pp.coating && r0 < pp.rc_sr) {
-                    // short-range forces along u(lo->hi); steric uses r_lo
-                    const double rl = i_lo ? ri : rj, rh = i_lo ? rj : ri;
-                    const double dDelta = 2 * (rl + pp.eff_len);
-                    double s = 0;
-                    if (r <= dDelta) s = (2 * rh / (rl + rh)) * pp.ster_fac * rl * rl * log(dDelta / r);
-                    s += (rl * rh / (rl + rh)) * pp.el_fac * exp(-pp.kappa * (r - rl - rh));
-                    const double q1 = (r - rl - rh) * (r + rl + rh);
-                    const double q2 = r * r - (rl - rh) * (rl - rh);
-                    s += -pp.hamaker * r * pow(4 * rl * rh, 3.0) / (6 * (q1 * q1) * (q2 * q2));
-                    // u(lo->hi) = sgn * u(i->j); force on hi is +s u(lo->hi)
-                    const double ulx = sgn * ux, uly = sgn * uy, ulz = sgn * uz;
-                    Fi[0] -= sgn * s * ulx;
-                    Fi[1] -= sgn * s * uly;
-                    Fi[2] -= sgn * s * ulz;
-                    if (i_lo) {
-                        press[0] += r * ulx * (s * uly);
-                        press[1] += r * ulx * (s * ulz);
-                        press[2] += r * uly * (s * ulz);
-                    }
-                }
+        if (COATING && r < pp.rc_sr) {
+            const double sf = short_range(pp, r, i_lo ? pi.w : pj.w, i_lo ? pj.w : pi.w) / pp.fscale;
+            F[0] += sf * ux;
+            F[1] += sf * uy;
+            F[2] += sf * uz;
+            if (i_lo) {
+                const double rr = r * pp.inv_ell;
+                press[0] += rr * ux * (sf * uy);
+                press[1] += rr * ux * (sf * uz);
+                press[2] += rr * uy * (sf * uz);
             }
         }
-        atomicAdd(&W.acc[3][il], Bi[0]);
-        atomicAdd(&W.acc[4][il], Bi[1]);
-        atomicAdd(&W.acc[5][il], Bi[2]);
-        if (FORCE) {
-            atomicAdd(&W.acc[0][il], Fi[0]);
-            atomicAdd(&W.acc[1][il], Fi[1]);
-            atomicAdd(&W.acc[2][il], Fi[2]);
-        }
+        atomicAdd(&W.acc[0][il], -F[0]);
+        atomicAdd(&W.acc[1][il], -F[1]);
+        atomicAdd(&W.acc[2][il], -F[2]);
     }
 }
 
-template <bool FORCE>
+template <bool FORCE, bool COATING, bool MIXED>
 __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
-                                               PairOut out, unsigned long long* __restrict__ cnt_pairs) {
+                                               PairOut out, const int4* __restrict__ stencil, int32_t nsten,
+                                               unsigned long long* __restrict__ cnt_pairs) {
+    // One block per chunk of up to 64 particles of one cell; its 4 wavefronts
+    // split the stencil between them and merge their sums in LDS at the end.
     __shared__ IWave waves[4];
+    __shared__ double wpress[4][3];
     const int wv = threadIdx.x / WAVE;
     const int lane = lane_id();
     IWave& W = waves[wv];
-    const int32_t w = blockIdx.x * 4 + wv;
+    const int32_t w = blockIdx.x;
     const int32_t nwork = *g.nwork;
-    if (w >= nwork) return;
+    if (w >= nwork) return;  // uniform over the block
     const int32_t item = g.work[w];
     const int32_t cell = item >> 8, chunk = item & 255;
     const int32_t s0 = g.cell_start[cell] + chunk * WAVE;
@@ -532,76 +602,224 @@ __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __rest
     }
     for (int d = 0; d < 6; ++d) W.acc[d][lane] = 0.0;
     const bool moved_any = *g.n_events > 0;
-    const double reach_r = pp.rc * (1.0 + 1e-12) + (moved_any ? 2.0 * *g.max_disp : 0.0);
-    const double rr2 = reach_r * reach_r;
     const double L = g.L;
     const int nc = g.ncell;
+    const bool allpairs = nc == 1;
     const int cx = cell % nc, cy = (cell / nc) % nc, cz = cell / (nc * nc);
-    const int R = nc >= 2 * g.reach + 1 ? g.reach : 0;
+    // candidate radius: r_c with a relative slack covering the rounding of
+    // the (FP32 on the mixed path) distance test; the exact r < r_c test is
+    // made in FP64 on the canonical pair
+    const double rr2 = (pp.rc * (MIXED ? 1.0 + 1e-5 : 1.0 + 1e-12)) * (pp.rc * (MIXED ? 1.0 + 1e-5 : 1.0 + 1e-12));
+    const float rr2f = (float)rr2;
+    const float pxf = (float)pi.x, pyf = (float)pi.y, pzf = (float)pi.z;
     double press[3] = {0, 0, 0};
     int64_t npairs = 0;
     int qn = 0;
     wave_sync();
-    for (int dz = -R; dz <= R; ++dz)
-        for (int dy = -R; dy <= R; ++dy)
-            for (int dx = -R; dx <= R; ++dx) {
-                const int c = (((cz + dz + nc) % nc) * nc + (cy + dy + nc) % nc) * nc + (cx + dx + nc) % nc;
-                const int32_t cs = g.cell_start[c], ce = g.cell_end[c];
-                for (int32_t jb = cs; jb < ce; jb += WAVE) {
-                    const int32_t jl = jb + lane;
-                    double4 pj = make_double4(0, 0, 0, 0);
-                    if (jl < ce) pj = g.rec_pos[jl];
-                    const int cnt = min(WAVE, ce - jb);
-                    for (int t = 0; t < cnt; ++t) {
-                        const double xj = readlane_d(pj.x, t), yj = readlane_d(pj.y, t), zj = readlane_d(pj.z, t);
-                        const double ddx = min_image(xj - pi.x, L), ddy = min_image(yj - pi.y, L),
-                                     ddz = min_image(zj - pi.z, L);
-                        const bool hit = vi && (jb + t != k) && (ddx * ddx + ddy * ddy + ddz * ddz < rr2);
-                        const uint64_t m = __ballot(hit);
-                        if (m) {
-                            if (hit) W.q[qn + mbcnt(m)] = ((jb + t) << 6) | lane;
-                            qn += __popcll(m);
-                            if (qn >= WAVE) {
-                                wave_sync();
-                                pair_dense<FORCE>(g, rec_mom, pp, W, W.q[lane], true, moved_any, press, npairs);
-                                wave_sync();
-                                const int rest = qn - WAVE;
-                                int32_t moved = lane < rest ? W.q[WAVE + lane] : 0;
-                                wave_sync();
-                                if (lane < rest) W.q[lane] = moved;
-                                qn = rest;
-                                wave_sync();
-                            }
-                        }
+    for (int si = wv; si < nsten; si += 4) {
+        const int4 o = stencil[si];
+        int jx = cx + o.x, jy = cy + o.y, jz = cz + o.z;
+        // periodic image of the neighbour cell nearest this cell
+        double sx = 0, sy = 0, sz = 0;
+        if (jx < 0) { jx += nc; sx = -L; } else if (jx >= nc) { jx -= nc; sx = L; }
+        if (jy < 0) { jy += nc; sy = -L; } else if (jy >= nc) { jy -= nc; sy = L; }
+        if (jz < 0) { jz += nc; sz = -L; } else if (jz >= nc) { jz -= nc; sz = L; }
+        const int c = (jz * nc + jy) * nc + jx;
+        const int32_t cs = g.cell_start[c], ce = g.cell_end[c];
+        for (int32_t jb = cs; jb < ce; jb += WAVE) {
+            const int32_t jl = jb + lane;
+            // j's image nearest this cell (exact: every j within r_c of an i
+            // of this cell lies in exactly one stencil cell)
+            double qx = 0, qy = 0, qz = 0;
+            float qxf = 0, qyf = 0, qzf = 0;
+            if (MIXED) {
+                if (jl < ce) {
+                    const float4 pj = g.rec_pos32[jl];
+                    qxf = pj.x + (float)sx;
+                    qyf = pj.y + (float)sy;
+                    qzf = pj.z + (float)sz;
+                }
+            } else {
+                if (jl < ce) {
+                    const double4 pj = g.rec_pos[jl];
+                    qx = pj.x + sx;
+                    qy = pj.y + sy;
+                    qz = pj.z + sz;
+                }
+            }
+            const int cnt = min(WAVE, ce - jb);
+            for (int t = 0; t < cnt; ++t) {
+                bool inside;
+                if (MIXED && !allpairs) {
+                    const float dx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qxf), t)) - pxf;
+                    const float dy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qyf), t)) - pyf;
+                    const float dz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qzf), t)) - pzf;
+                    inside = dx * dx + dy * dy + dz * dz < rr2f;
+                } else if (MIXED) {
+                    double dx = (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qxf), t)) - pi.x;
+                    double dy = (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qyf), t)) - pi.y;
+                    double dz = (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qzf), t)) - pi.z;
+                    dx = min_image(dx, L);
+                    dy = min_image(dy, L);
+                    dz = min_image(dz, L);
+                    inside = dx * dx + dy * dy + dz * dz < rr2;
+                } else {
+                    double dx = readlane_d(qx, t) - pi.x;
+                    double dy = readlane_d(qy, t) - pi.y;
+                    double dz = readlane_d(qz, t) - pi.z;
+                    if (allpairs) {
+                        dx = min_image(dx, L);
+                        dy = min_image(dy, L);
+                        dz = min_image(dz, L);
+                    }
+                    inside = dx * dx + dy * dy + dz * dz < rr2;
+                }
+                const bool hit = vi && (jb + t != k) && inside;
+                const uint64_t m = __ballot(hit);
+                if (m) {
+                    if (hit) W.q[qn + mbcnt(m)] = ((jb + t) << 6) | lane;
+                    qn += __popcll(m);
+                    if (qn >= WAVE) {
+                        wave_sync();
+                        pair_dense<FORCE, COATING, MIXED>(g, rec_mom, pp, W, W.q[lane], true, moved_any, press,
+                                                          npairs);
+                        wave_sync();
+                        const int rest = qn - WAVE;
+                        int32_t moved = lane < rest ? W.q[WAVE + lane] : 0;
+                        wave_sync();
+                        if (lane < rest) W.q[lane] = moved;
+                        qn = rest;
+                        wave_sync();
                     }
                 }
             }
-    wave_sync();
-    if (qn > 0) pair_dense<FORCE>(g, rec_mom, pp, W, lane < qn ? W.q[lane] : 0, lane < qn, moved_any, press, npairs);
-    wave_sync();
-    if (vi) {
-        const int32_t o = W.iorig[lane];
-        const int64_t n = g.n;
-        out.field[o] = W.acc[3][lane];
-        out.field[n + o] = W.acc[4][lane];
-        out.field[2 * n + o] = W.acc[5][lane];
-        if (FORCE) {
-            out.force[o] = W.acc[0][lane];
-            out.force[n + o] = W.acc[1][lane];
-            out.force[2 * n + o] = W.acc[2][lane];
         }
     }
+    wave_sync();
+    if (qn > 0)
+        pair_dense<FORCE, COATING, MIXED>(g, rec_mom, pp, W, lane < qn ? W.q[lane] : 0, lane < qn, moved_any, press,
+                                          npairs);
     if (FORCE) {
         const double p0 = wave_sum(press[0]), p1 = wave_sum(press[1]), p2 = wave_sum(press[2]);
         if (lane == 0) {
-            out.press_partial[3 * (int64_t)w] = p0;
-            out.press_partial[3 * (int64_t)w + 1] = p1;
-            out.press_partial[3 * (int64_t)w + 2] = p2;
+            wpress[wv][0] = p0;
+            wpress[wv][1] = p1;
+            wpress[wv][2] = p2;
         }
     }
     if (cnt_pairs) {
         const int64_t np = wave_sum(npairs);
         if (lane == 0) atomicAdd(cnt_pairs, (unsigned long long)np);
+    }
+    __syncthreads();
+    if (wv == 0) {
+        if (vi) {
+            const int32_t o = W.iorig[lane];
+            const int64_t n = g.n;
+            double a[6];
+            for (int d = 0; d < 6; ++d)
+                a[d] = (waves[0].acc[d][lane] + waves[1].acc[d][lane]) + (waves[2].acc[d][lane] + waves[3].acc[d][lane]);
+            out.field[o] = a[3] * pp.bscale;
+            out.field[n + o] = a[4] * pp.bscale;
+            out.field[2 * n + o] = a[5] * pp.bscale;
+            if (FORCE) {
+                out.force[o] = a[0] * pp.fscale;
+                out.force[n + o] = a[1] * pp.fscale;
+                out.force[2 * n + o] = a[2] * pp.fscale;
+            }
+        }
+        if (FORCE && lane < 3)
+            out.press_partial[3 * (int64_t)w + lane] =
+                ((wpress[0][lane] + wpress[1][lane]) + (wpress[2][lane] + wpress[3][lane])) * pp.pscale;
+    }
+}
+
+// Pairs with at least one endpoint moved by the overlap correction, in FP64
+// with the positions the reference's sequential scan gives each pair
+// (CompInteractVecs, src/interaction.cpp:99-129): one wavefront per moved
+// particle p, lanes over the stencil's particles; a pair of two moved
+// particles is done by the lower index's wavefront. Results are added with
+// atomics after k_pairs has written its sums.
+template <bool FORCE, bool COATING>
+__global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
+                                                    PairOut out, const int4* __restrict__ stencil, int32_t nsten,
+                                                    int32_t press_off, unsigned long long* __restrict__ cnt_pairs) {
+    const int32_t nmv = *g.n_moved;
+    const int lane = lane_id();
+    const int64_t n = g.n;
+    const double L = g.L;
+    const int nc = g.ncell;
+    for (int32_t mi = blockIdx.x; mi < nmv; mi += gridDim.x) {
+        const int32_t p = g.moved_list[mi];
+        const int32_t kp = g.inv_perm[p];
+        const double4 pp0 = g.rec_pos[kp];
+        const double4 mp = rec_mom[kp];
+        const int cx = cell_coord(pp0.x, L, g.cell, nc), cy = cell_coord(pp0.y, L, g.cell, nc),
+                  cz = cell_coord(pp0.z, L, g.cell, nc);
+        double press[3] = {0, 0, 0};
+        int64_t npairs = 0;
+        for (int si = 0; si < nsten; ++si) {
+            const int4 o = stencil[si];
+            const int c = (((cz + o.z + nc) % nc) * nc + (cy + o.y + nc) % nc) * nc + (cx + o.x + nc) % nc;
+            const int32_t cs = g.cell_start[c], ce = g.cell_end[c];
+            for (int32_t j = cs + lane; j < ce; j += WAVE) {
+                if (j == kp) continue;
+                const int32_t oj = g.sorted_idx[j];
+                if (g.ev_head[oj] >= 0 && oj < p) continue;  // done by oj's wavefront
+                const double4 pj = g.rec_pos[j];
+                const bool p_lo = p < oj;
+                const int32_t lo = p_lo ? p : oj, hi = p_lo ? oj : p;
+                double xl = p_lo ? pp0.x : pj.x, yl = p_lo ? pp0.y : pj.y, zl = p_lo ? pp0.z : pj.z;
+                double xh = p_lo ? pj.x : pp0.x, yh = p_lo ? pj.y : pp0.y, zh = p_lo ? pj.z : pp0.z;
+                int event = 0;
+                version_pair(g, lo, hi, xl, yl, zl, xh, yh, zh, event);
+                const double vx = min_image(xh - xl, L), vy = min_image(yh - yl, L), vz = min_image(zh - zl, L);
+                const double r0 = sqrt(vx * vx + vy * vy + vz * vz);
+                if (!(r0 < pp.rc)) continue;
+                ++npairs;
+                const double r = event ? pp0.w + pj.w + 1e-10 : r0;
+                const double ir0 = 1.0 / r0;
+                const double ir = event ? 1.0 / r : ir0;
+                // u from lo to hi, taken from the pre-correction separation
+                const double ux = vx * ir0, uy = vy * ir0, uz = vz * ir0;
+                const double4 mlo = p_lo ? mp : rec_mom[j], mhi = p_lo ? rec_mom[j] : mp;
+                double Bl[3], Fh[3], Bh[3], dummy[3];
+                // field on lo from hi and force on hi (u from lo to hi) ...
+                pair_terms<FORCE, false>(pp, r, ir, ux, uy, uz, mlo, mhi, Bl, Fh);
+                // ... and field on hi from lo (u from hi to lo)
+                pair_terms<false, false>(pp, r, ir, -ux, -uy, -uz, mhi, mlo, Bh, dummy);
+                const double f = pp.fac;
+                for (int d = 0; d < 3; ++d) {
+                    atomicAdd(&out.field[d * n + lo], f * Bl[d]);
+                    atomicAdd(&out.field[d * n + hi], f * Bh[d]);
+                }
+                if (FORCE) {
+                    double sf = 0;
+                    if (COATING && r0 < pp.rc_sr) sf = short_range(pp, r, p_lo ? pp0.w : pj.w, p_lo ? pj.w : pp0.w);
+                    const double u3[3] = {ux, uy, uz};
+                    for (int d = 0; d < 3; ++d) {
+                        const double Fd = f * Fh[d] + sf * u3[d];
+                        atomicAdd(&out.force[d * n + lo], -Fd);
+                        atomicAdd(&out.force[d * n + hi], Fd);
+                    }
+                    press[0] += r * ux * (f * Fh[1] + sf * uy);
+                    press[1] += r * ux * (f * Fh[2] + sf * uz);
+                    press[2] += r * uy * (f * Fh[2] + sf * uz);
+                }
+            }
+        }
+        if (FORCE) {
+            const double p0 = wave_sum(press[0]), p1 = wave_sum(press[1]), p2 = wave_sum(press[2]);
+            if (lane == 0) {
+                atomicAdd(&out.press_partial[3 * (int64_t)press_off], p0);
+                atomicAdd(&out.press_partial[3 * (int64_t)press_off + 1], p1);
+                atomicAdd(&out.press_partial[3 * (int64_t)press_off + 2], p2);
+            }
+        }
+        if (cnt_pairs) {
+            const int64_t np = wave_sum(npairs);
+            if (lane == 0) atomicAdd(cnt_pairs, (unsigned long long)(2 * np));
+        }
     }
 }
 
@@ -1066,9 +1284,9 @@ hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_
 }
 
 hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_idx, const double* pos0,
-                                const double* r_mag, double4* rec_pos, int32_t* inv_perm) {
+                                const double* r_mag, double4* rec_pos, float4* rec_pos32, int32_t* inv_perm) {
     hipLaunchKernelGGL(k_records, dim3(grid_for(n, 256)), dim3(256), 0, s, n, sorted_idx, pos0, r_mag, rec_pos,
-                       inv_perm);
+                       rec_pos32, inv_perm);
     LAUNCH_CHECK();
 }
 
@@ -1099,12 +1317,13 @@ hipError_t launch_contacts(hipStream_t s, int64_t n, double L, const double* pos
 hipError_t launch_resolve(hipStream_t s, int64_t n, double L, double rc, const double* r_mag, const int32_t* cand_off,
                           const int32_t* cand_j, double* pos_cur, const double* pos0, int32_t* ev_head,
                           int32_t* ev_tail, int32_t* ev_next, uint64_t* ev_key, double* ev_pos, int32_t ev_cap,
-                          int32_t* n_events, double* max_disp, double disp_limit, int32_t* err) {
+                          int32_t* n_events, double* max_disp, double disp_limit, int32_t* moved_list,
+                          int32_t* n_moved, int32_t* err) {
     // the candidate total sits at cand_off[n]; its capacity is ev_cap / 2
     const int32_t cand_cap = ev_cap / 2;
     hipLaunchKernelGGL(k_resolve, dim3(1), dim3(64), 0, s, n, L, rc, r_mag, cand_off + n, cand_j + cand_cap, cand_j,
                        pos_cur, pos0, ev_head, ev_tail, ev_next, ev_key, ev_pos, ev_cap, n_events, max_disp,
-                       disp_limit, err);
+                       disp_limit, moved_list, n_moved, err);
     LAUNCH_CHECK();
 }
 
@@ -1125,16 +1344,39 @@ hipError_t launch_reduce_sum3(hipStream_t s, const double* partial, int32_t coun
 }
 
 hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
-                        const PairParams& pp, const PairOut& out, int32_t max_work, int64_t* cnt_pairs) {
-    (void)precision;
-    const int blocks = (max_work + 3) / 4;
+                        const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
+                        int32_t nsten, int64_t* cnt_pairs) {
+    const int blocks = max_work;
     if (blocks <= 0) return hipSuccess;
-    if (force)
-        hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out,
-                           (unsigned long long*)cnt_pairs);
+    unsigned long long* cp = (unsigned long long*)cnt_pairs;
+    const bool mixed = precision == 1;
+#define KP(F, C, M) \
+    hipLaunchKernelGGL((k_pairs<F, C, M>), dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out, stencil, nsten, cp)
+    if (force && pp.coating) {
+        if (mixed) KP(true, true, true); else KP(true, true, false);
+    } else if (force) {
+        if (mixed) KP(true, false, true); else KP(true, false, false);
+    } else {
+        if (mixed) KP(false, false, true); else KP(false, false, false);
+    }
+#undef KP
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_pairs_moved(hipStream_t s, bool force, const Geometry& g, const double4* rec_mom,
+                              const PairParams& pp, const PairOut& out, const int4* stencil, int32_t nsten,
+                              int32_t press_off, int64_t* cnt_pairs) {
+    unsigned long long* cp = (unsigned long long*)cnt_pairs;
+    const int blocks = 512;
+    if (force && pp.coating)
+        hipLaunchKernelGGL((k_pairs_moved<true, true>), dim3(blocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
+                           nsten, press_off, cp);
+    else if (force)
+        hipLaunchKernelGGL((k_pairs_moved<true, false>), dim3(blocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
+                           nsten, press_off, cp);
     else
-        hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out,
-                           (unsigned long long*)cnt_pairs);
+        hipLaunchKernelGGL((k_pairs_moved<false, false>), dim3(blocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
+                           nsten, press_off, cp);
     LAUNCH_CHECK();
 }
 

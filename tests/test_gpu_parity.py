@@ -169,3 +169,54 @@ def test_run_fixed_dt_matches_step_by_step():
         assert mz[k] == pytest.approx(out["mz"], rel=1e-11)
     assert rel_err(a[1], mm) < TOL and rel_err(a[2], ea) < TOL and rel_err(a[0], pos) < 1e-13
     assert t == pytest.approx(5e-12)
+
+
+MIXED_TOL = 1e-5
+
+
+@pytest.mark.parametrize("case", ["heun_n1000", "bs_lognormal_n125", "overlap_n64", "coating_n27"])
+def test_mixed_precision_interactions_vs_reference(case):
+    """FP32 pair terms (FP64 geometry and accumulation): field, force and
+    torque within 1e-5 of the reference (contact-value floors)."""
+    d = load(case)
+    cfg = mds.config_from_string(d["ini_text"])
+    cfg.precision = mds.PREC_MIXED
+    ctx = mds.Context(cfg)
+    ctx.set_particles(d["rMag"], d["rHydr"], float(d["lBox"][0]))
+    ctx.set_ewald(float(d["alpha"][0]), float(d["rCutLR"][0]), int(d["nc"][0]))
+    ctx.set_state(d["s0_pos_in"], d["s0_mm_in"], d["s0_ea_in"])
+    ctx.set_noise(d["s0_nT"], d["s0_nB"], None)
+    F, B, P, X = ctx.comp_interactions()
+    f_al, b_al, t_al = contact_scales(d)
+    Fref = d["s0_force"] - d["s0_nF"]
+    assert rel_err(B, d["s0_field"], b_al) < MIXED_TOL
+    assert rel_err(F, Fref, f_al) < MIXED_TOL
+    mom = d["s0_mm_in"] * d["magMom"][:, None]
+    assert rel_err(np.cross(mom, B), np.cross(mom, d["s0_field"]), t_al) < MIXED_TOL
+    assert rel_err(X, d["s0_pos_mid"]) < 1e-14
+
+
+def test_mixed_precision_cell_list_vs_oracle():
+    import oracle
+    from refcfg import derived, parse_ini
+    n = 4000
+    cfg, text, r, rh, L, pos, ea, alpha, rc, nc = _random_system(n, 0.10, 0.25, 77, 4)
+    cfg.precision = mds.PREC_MIXED
+    ctx = mds.Context(cfg)
+    ctx.set_particles(r, rh, L)
+    ctx.set_ewald(alpha, rc, nc)
+    ctx.set_state(pos, ea, ea)
+    ctx.set_noise(None, None, None)
+    F, B, P, X = ctx.comp_interactions()
+    c = derived(parse_ini(text))
+    mu = cfg.sat_mag * 4 / 3 * cfg.my_pi * r ** 3
+    props = {"r_mag": r, "r_hydr": rh, "vol_mag": 4 / 3 * cfg.my_pi * r ** 3, "mag_mom": mu,
+             "zeta_rot": 8 * cfg.my_pi * cfg.vis * rh ** 3, "zeta_trans": 6 * cfg.my_pi * cfg.vis * rh}
+    s = oracle.System(n, L, alpha, rc, 2 * r.max(), nc, props, c, threads=8)
+    Fo, Bo, Po, Xo = s.comp_interactions(pos, ea, None)
+    m = mu.min()
+    b_al = cfg.mu0 * m / (2 * cfg.my_pi * (2 * rh.min()) ** 3)
+    f_al = 3 * cfg.mu0 * m * m / (2 * cfg.my_pi * (2 * rh.min()) ** 4)
+    assert rel_err(B, Bo, b_al) < MIXED_TOL
+    assert rel_err(F, Fo, f_al) < MIXED_TOL
+    assert rel_err(X, Xo) < 1e-14
