@@ -73,26 +73,20 @@ def time_path(ctx, steps, warmup):
 
 
 def tune(ctx, info, args):
-    """Pick the cell reach of the pair search by timing one step of each."""
-    if args.reach > 0:
+    """The library's autotuner: r_c (alpha, n_c from the reference's error
+    model at errTolEwald) and the cell reach, timed on the device."""
+    ctx.set_state(info["pos"], info["mm"], info["ea"])
+    if args.reach > 0 and args.rc_frac > 0:
         ctx.set_tiling(args.reach)
+        res = None
     else:
-        best = None
-        for reach in (2, 3, 4):
-            ctx.set_tiling(reach)
-            ctx.set_state(info["pos"], info["mm"], info["ea"])
-            ctx.run(1, 0.0, 1e-12)
-            ctx.synchronize()
-            t0 = time.perf_counter()
-            ctx.run(1, 0.0, 1e-12)
-            ctx.synchronize()
-            el = time.perf_counter() - t0
-            log(f"  tiling reach={reach} {ctx.tiling()} {el * 1e3:.2f} ms/step")
-            if best is None or el < best[0]:
-                best = (el, reach)
-        ctx.set_tiling(best[1])
+        rcs = (args.rc_frac,) if args.rc_frac > 0 else (0.2, 0.225, 0.25, 0.275, 0.3, 0.35)
+        reaches = (args.reach,) if args.reach > 0 else (2, 3, 4)
+        res = ctx.autotune(rcs, reaches, reps=2)
+        log(f"  autotune: {res} -> {ctx.params()} {ctx.tiling()}")
     ctx.set_state(info["pos"], info["mm"], info["ea"])
     ctx.use_philox(args.seed, 0)
+    return res
 
 
 def roofline(stats, counts, n, nk, steps, prec):
@@ -153,7 +147,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--rc-frac", type=float, default=0.3)
+    ap.add_argument("--rc-frac", type=float, default=0.0, help="r_c / L; 0 = autotune")
     ap.add_argument("--reach", type=int, default=0, help="cell reach of the pair search; 0 = autotune")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -167,7 +161,7 @@ def main():
     from md_sim_wolfschwenger_amd import workload
 
     log("building the N=1e5 workload")
-    ctx, info = make_context(mds, workload, "fp64", args.rc_frac, args.seed)
+    ctx, info = make_context(mds, workload, "fp64", args.rc_frac if args.rc_frac > 0 else 0.3, args.seed)
     n = info["n"]
     nk = ctx.work_counts()["kvecs"]
     log(f"L={info['L']:.4e} alpha*L={info['alpha'] * info['L']:.2f} rc/L={args.rc_frac} nc={info['nc']} nk={nk}")
@@ -176,8 +170,9 @@ def main():
         if prec == "mixed":
             ctx.close()
             ctx, _ = make_context(mds, workload, "mixed", args.rc_frac, args.seed, info)
-        tune(ctx, info, args)
-        tiling = ctx.tiling()
+        tuned = tune(ctx, info, args)
+        tiling = dict(ctx.tiling(), **ctx.params())
+        nk = ctx.work_counts()["kvecs"]
         el, stats, counts = time_path(ctx, args.steps, args.warmup)
         value = n * args.steps / el
         roof, breakdown = roofline(stats, counts, n, nk, args.steps, prec)
@@ -190,8 +185,11 @@ def main():
         ctx.set_noise(None, None, None)
         F, B, _, _ = ctx.comp_interactions()
         results[prec] = {"value": value, "el": el, "roof": roof, "breakdown": breakdown, "tiling": tiling,
-                         "F": F, "B": B}
+                         "F": F, "B": B, "nk": nk}
     ctx.close()
+    # the CPU baseline runs the FP64 path's tuned Ewald parameters
+    p64 = results["fp64"]["tiling"]
+    info.update(alpha=p64["alpha"], rc=p64["r_cut"], nc=p64["nc"])
     cfg = info["cfg"]
     m_min, rh_min = float(info["mu"].min()), float(info["rh"].min())
     f_al = 3 * cfg.mu0 * m_min ** 2 / (2 * cfg.my_pi * (2 * rh_min) ** 4)
@@ -221,8 +219,8 @@ def main():
         "config": {"workload": "C3: N=100000 lognormal core (median 10 nm radius, sigma_ln 0.25) + 5 nm shell, "
                                "phi=0.10, B=50 mT z, alpha_G=0.05, T=300 K, eta=1e-3 Pa s, Ewald 1e-6, "
                                "fixed dt=1e-12 s Heun, Philox noise",
-                   "n": n, "precision": "mixed", "rc_over_L": args.rc_frac, "alpha_L": info["alpha"] * info["L"],
-                   "nc": info["nc"], "kvecs": nk, "tiling": results["mixed"]["tiling"]},
+                   "n": n, "precision": "mixed", "ewald_and_tiling": results["mixed"]["tiling"],
+                   "kvecs": results["mixed"]["nk"], "fp64_ewald_and_tiling": results["fp64"]["tiling"]},
         "fp64_value": results["fp64"]["value"], "mixed_value": results["mixed"]["value"],
         "mixed_over_fp64": results["mixed"]["value"] / results["fp64"]["value"],
         "mixed_err_vs_fp64": mixed_err,

@@ -154,6 +154,16 @@ mds_status mds_set_ewald(mds_ctx* ctx, double alpha, double r_cut, int32_t nc);
  * +-cell_reach cells. Default 3; chosen by the project's autotuner. */
 mds_status mds_set_tiling(mds_ctx* ctx, int32_t cell_reach);
 mds_status mds_get_tiling(const mds_ctx* ctx, int32_t* cell_reach, int32_t* ncell, int32_t* nstencil);
+/* Autotuner, in place of CompEwaldParams's wall-clock choice among 30 r_c in
+ * [0.2, 0.5] L (src/interaction.cpp:164-324): for each r_c = rc_fracs[i] * L,
+ * alpha and k_c come from the reference's error model (so accuracy is the
+ * configured errTolEwald throughout) and each cell reach in `reaches` is
+ * timed on the device over one force + one field evaluation of the current
+ * state (`reps` timed repetitions after one warm-up). The fastest candidate
+ * is installed (mds_set_ewald + mds_set_tiling) and returned. */
+mds_status mds_autotune(mds_ctx* ctx, const double* rc_fracs, int32_t n_rc, const int32_t* reaches,
+                        int32_t n_reach, int32_t reps, double* best_rc_frac, int32_t* best_reach,
+                        double* best_ms);
 /* Read back box edge, r_c(SR), alpha, r_c, n_c. */
 mds_status mds_get_params(const mds_ctx* ctx, double* l_box, double* r_cut_sr, double* alpha,
                           double* r_cut, int32_t* nc);

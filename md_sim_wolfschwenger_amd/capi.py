@@ -84,6 +84,8 @@ SIGNATURES = [
     ("mds_set_ewald", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32]),
     ("mds_set_tiling", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_get_tiling", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("mds_autotune", C.c_int, [C.c_void_p, _dp, C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32, _dp,
+                               C.POINTER(C.c_int32), _dp]),
     ("mds_get_params", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)]),
     ("mds_set_state", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     ("mds_get_state", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
@@ -199,6 +201,15 @@ class Context:
         v = [C.c_int32() for _ in range(3)]
         self._chk(self._L.mds_get_tiling(self.h, *[C.byref(x) for x in v]), "get_tiling")
         return {"cell_reach": v[0].value, "ncell": v[1].value, "nstencil": v[2].value}
+
+    def autotune(self, rc_fracs=(0.2, 0.225, 0.25, 0.275, 0.3), reaches=(2, 3, 4), reps=2):
+        """Time each (r_c, cell reach) candidate on the device; install the fastest."""
+        rcs = np.ascontiguousarray(rc_fracs, dtype=np.float64)
+        rea = np.ascontiguousarray(reaches, dtype=np.int32)
+        brc, bre, bms = C.c_double(), C.c_int32(), C.c_double()
+        self._chk(self._L.mds_autotune(self.h, _p(rcs), len(rcs), rea.ctypes.data_as(C.POINTER(C.c_int32)), len(rea),
+                                       int(reps), C.byref(brc), C.byref(bre), C.byref(bms)), "autotune")
+        return {"rc_frac": brc.value, "reach": bre.value, "eval_ms": bms.value}
 
     def params(self):
         v = [C.c_double() for _ in range(4)]

@@ -147,6 +147,7 @@ struct mds_ctx {
     DevBuf<double4> rec_pos, rec_mom;
     DevBuf<float4> rec_pos32;
     double mref = 1.0;  // largest magnetic moment (unit of the FP32 pair path)
+    std::vector<double> h_mag_mom, h_r_hydr;
     // integrator
     DevBuf<double> kmm, kea, pt[2], new_mm, new_ea;
     // cell lists
@@ -753,6 +754,8 @@ mds_status mds_set_particles(mds_ctx* c, int64_t n, const double* r_mag, const d
     HIPCHK(c->rec_pos.ensure(n));
     HIPCHK(c->rec_pos32.ensure(n));
     c->mref = *std::max_element(mu.begin(), mu.end());
+    c->h_mag_mom = mu;
+    c->h_r_hydr = rh;
     HIPCHK(c->rec_mom.ensure(n));
     for (auto* b : {&c->key_main, &c->key_fine, &c->sorted_main, &c->sorted_fine, &c->inv_perm, &c->cand_count,
                     &c->ev_head, &c->ev_tail})
@@ -807,6 +810,75 @@ mds_status mds_get_tiling(const mds_ctx* c, int32_t* cell_reach, int32_t* ncell,
     if (cell_reach) *cell_reach = c->reach;
     if (ncell) *ncell = c->ncell;
     if (nstencil) *nstencil = c->nsten;
+    return MDS_OK;
+}
+
+mds_status mds_autotune(mds_ctx* c, const double* rc_fracs, int32_t n_rc, const int32_t* reaches, int32_t n_reach,
+                        int32_t reps, double* best_rc_frac, int32_t* best_reach, double* best_ms) {
+    // CompEwaldParams (src/interaction.cpp:164-324) picks r_c in [0.2, 0.5] L
+    // by timing the field sums on the CPU; here every candidate (r_c, cell
+    // reach) is timed on the device over one force and one field evaluation,
+    // with alpha and k_c from the same error model, so each candidate has the
+    // accuracy the configuration asks for.
+    if (!c || c->n <= 0 || !c->state_set || !rc_fracs || n_rc <= 0 || !reaches || n_reach <= 0)
+        return fail(c, MDS_EINVAL, "autotune needs particles, state and candidates");
+    HIPCHK(hipSetDevice(c->device));
+    if (reps < 1) reps = 1;
+    double best = 1e300, brc = -1;
+    int32_t breach = c->reach;
+    double b_alpha = 0;
+    int32_t b_nc = 0;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const bool timing = c->timing;
+    c->timing = false;
+    mds_status st = MDS_OK;
+    for (int a = 0; a < n_rc && st == MDS_OK; ++a) {
+        const double rc = rc_fracs[a] * c->L;
+        double alpha = 0, kc = 0;
+        if (mds_ewald_params_for_rc(&c->cfg, c->n, c->h_mag_mom.data(), c->h_r_hydr.data(), c->L, rc, &alpha, &kc) !=
+            MDS_OK)
+            continue;
+        const int32_t nc = (int32_t)std::floor(kc + 0.5);  // src/interaction.cpp:304
+        if (nc > 64 || rc > 0.5 * c->L) continue;
+        if ((st = mds_set_ewald(c, alpha, rc, nc)) != MDS_OK) break;
+        for (int b = 0; b < n_reach && st == MDS_OK; ++b) {
+            c->reach = reaches[b];
+            if ((st = plan_grids(c)) != MDS_OK) break;
+            if ((st = prepare_geometry(c, true)) != MDS_OK) break;
+            float best_ms_here = 1e30f;
+            for (int r = 0; r < reps + 1 && st == MDS_OK; ++r) {
+                HIPCHK(hipEventRecord(e0, c->stream));
+                if ((st = moments_of(c, c->mm.p)) != MDS_OK) break;
+                if ((st = evaluate(c, true)) != MDS_OK) break;
+                if ((st = evaluate(c, false)) != MDS_OK) break;
+                HIPCHK(hipEventRecord(e1, c->stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) best_ms_here = std::min(best_ms_here, ms);  // the first run warms up
+            }
+            if (best_ms_here < best) {
+                best = best_ms_here;
+                brc = rc_fracs[a];
+                breach = reaches[b];
+                b_alpha = alpha;
+                b_nc = nc;
+            }
+        }
+    }
+    c->timing = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != MDS_OK) return st;
+    if (brc < 0) return fail(c, MDS_ENOCONV, "no admissible Ewald candidate");
+    c->reach = breach;
+    if ((st = mds_set_ewald(c, b_alpha, brc * c->L, b_nc)) != MDS_OK) return st;
+    if (best_rc_frac) *best_rc_frac = brc;
+    if (best_reach) *best_reach = breach;
+    if (best_ms) *best_ms = best;
+    c->geometry_ready = false;
     return MDS_OK;
 }
 

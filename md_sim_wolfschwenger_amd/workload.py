@@ -182,13 +182,24 @@ def ewald_for(cfg: capi.Config, mag_mom: np.ndarray, r_hydr: np.ndarray, L: floa
 
 
 def system(name: str, seed: int = 1, noise: bool = True, fixed: bool = True, precision: str = "fp64",
-           n: int | None = None):
-    """(cfg, r_mag, r_hydr, L, pos, mm, ea) of a benchmark configuration."""
+           n: int | None = None, cache_dir: str | None = None):
+    """(cfg, r_mag, r_hydr, L, pos, mm, ea) of a benchmark configuration.
+    With cache_dir (or $MDS_WORKLOAD_CACHE) the generated arrays are kept in
+    an .npz there and reused by later calls with the same name/seed/N."""
+    import os
     cfg = capi.config_from_string(config_text(name, seed, noise, fixed, precision, n))
-    rng = np.random.default_rng(seed)
     N = int(cfg.numb_part)
+    cache_dir = cache_dir or os.environ.get("MDS_WORKLOAD_CACHE")
+    path = os.path.join(cache_dir, f"{name}_{seed}_{N}.npz") if cache_dir else None
+    if path and os.path.exists(path):
+        z = np.load(path)
+        return cfg, z["r"], z["rh"], float(z["L"]), z["pos"], z["ea"].copy(), z["ea"]
+    rng = np.random.default_rng(seed)
     r, rh = radii(cfg, N, rng)
     L = box_length(cfg, rh)
     pos = place(rh, L, rng)
     ea = orientations(N, rng, cfg.my_pi)
+    if path:
+        os.makedirs(cache_dir, exist_ok=True)
+        np.savez(path, r=r, rh=rh, L=L, pos=pos, ea=ea)
     return cfg, r, rh, L, pos, ea.copy(), ea
