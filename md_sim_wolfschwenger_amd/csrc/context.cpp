@@ -364,7 +364,7 @@ mds_status plan_kspace(mds_ctx* c) {
     HIPCHK(c->rows.ensure(std::max<size_t>(1, c->h_rows.size())));
     HIPCHK(c->segs.ensure(std::max<size_t>(1, c->h_segs.size())));
     HIPCHK(c->ek.ensure(std::max(1, c->nk)));
-    HIPCHK(c->sk.ensure(2 * (size_t)std::max(1, c->nk)));
+    HIPCHK(c->sk.ensure(4 * (size_t)std::max(1, c->nk)));
     HIPCHK(c->kpartial.ensure(2 * (size_t)std::max(1, c->nk) * c->nchunks));
     HIPCHK(c->tab.ensure((size_t)c->n * 3 * (nc + 1) * 2));
     if (!c->h_rows.empty())
@@ -502,6 +502,7 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
     {
         mds_ctx::Span sp(c, force_pass ? KC_PAIRS_FORCE : KC_PAIRS_FIELD);
         int64_t* cnt = c->count_work ? (int64_t*)c->pair_counter.p : nullptr;
+
         if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + 3 * (size_t)c->work_cap, 3, 0.0));
         HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p,
                             c->nsten, cnt));

@@ -395,7 +395,7 @@ __global__ __launch_bounds__(256) void k_reduce_sum3(const double* __restrict__ 
 // particles see bit-identical geometry.
 struct IWave {
     int32_t q[128];
-    double acc[6][WAVE];
+    double acc[6][WAVE];   // force, field of the wave's particles
     double4 ipos[WAVE];
     double4 imom[WAVE];
     int32_t iorig[WAVE];
@@ -445,9 +445,10 @@ __device__ __forceinline__ float erfcx32(float x) {
 // moments. FP64 path in SI units without the facEwald factor; FP32 path in
 // units of ell and mref (see PairParams). erfc(alpha r) is
 // exp(-(alpha r)^2) erfcx(alpha r), sharing the Gaussian of the t term.
-template <bool FORCE, bool MIXED>
+template <bool FORCE, bool MIXED, bool BOTH = false>
 __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, double ir, double ux, double uy, double uz,
-                                           const double4& mi, const double4& mj, double B[3], double F[3]) {
+                                           const double4& mi, const double4& mj, double B[3], double F[3],
+                                           double* Bj = nullptr) {
     if (!MIXED) {
         const double ir2 = ir * ir;
         const double x = pp.alpha * r;
@@ -460,8 +461,14 @@ __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, doubl
         B[0] = g2 * ux - a1 * mj.x;
         B[1] = g2 * uy - a1 * mj.y;
         B[2] = g2 * uz - a1 * mj.z;
+        const double di = mi.x * ux + mi.y * uy + mi.z * uz;
+        if (BOTH) {
+            const double h2 = a2 * di * r * r;
+            Bj[0] = h2 * ux - a1 * mi.x;
+            Bj[1] = h2 * uy - a1 * mi.y;
+            Bj[2] = h2 * uz - a1 * mi.z;
+        }
         if (FORCE) {
-            const double di = mi.x * ux + mi.y * uy + mi.z * uz;
             const double a3 = 5.0 * a2 * ir2 + 4.0 * pp.alpha4 * t;
             const double mimj = mi.x * mj.x + mi.y * mj.y + mi.z * mj.z;
             const double c0 = (a2 * mimj - a3 * di * dj * r * r) * r;
@@ -487,10 +494,16 @@ __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, doubl
         B[0] = (double)(g2 * fx - a1 * mjx);
         B[1] = (double)(g2 * fy - a1 * mjy);
         B[2] = (double)(g2 * fz - a1 * mjz);
+        const float mix = (float)(mi.x * pp.inv_mref), miy = (float)(mi.y * pp.inv_mref),
+                    miz = (float)(mi.z * pp.inv_mref);
+        const float di = mix * fx + miy * fy + miz * fz;
+        if (BOTH) {
+            const float h2 = a2 * di * rs * rs;
+            Bj[0] = (double)(h2 * fx - a1 * mix);
+            Bj[1] = (double)(h2 * fy - a1 * miy);
+            Bj[2] = (double)(h2 * fz - a1 * miz);
+        }
         if (FORCE) {
-            const float mix = (float)(mi.x * pp.inv_mref), miy = (float)(mi.y * pp.inv_mref),
-                        miz = (float)(mi.z * pp.inv_mref);
-            const float di = mix * fx + miy * fy + miz * fz;
             const float a3 = 5.0f * a2 * irs2 + 4.0f * pp.al4_s * t;
             const float mimj = mix * mjx + miy * mjy + miz * mjz;
             const float c0 = (a2 * mimj - a3 * di * dj * rs * rs) * rs;
@@ -607,10 +620,10 @@ __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __rest
     const bool allpairs = nc == 1;
     const int cx = cell % nc, cy = (cell / nc) % nc, cz = cell / (nc * nc);
     // candidate radius: r_c with a relative slack covering the rounding of
-    // the (FP32 on the mixed path) distance test; the exact r < r_c test is
-    // made in FP64 on the canonical pair
-    const double rr2 = (pp.rc * (MIXED ? 1.0 + 1e-5 : 1.0 + 1e-12)) * (pp.rc * (MIXED ? 1.0 + 1e-5 : 1.0 + 1e-12));
-    const float rr2f = (float)rr2;
+    // the FP32 distance test
+    const float rcf = (float)(pp.rc * (1.0 + 1e-5));
+    const float rr2f = rcf * rcf;
+    const float Lf = (float)L;
     const float pxf = (float)pi.x, pyf = (float)pi.y, pzf = (float)pi.z;
     double press[3] = {0, 0, 0};
     int64_t npairs = 0;
@@ -629,51 +642,27 @@ __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __rest
         for (int32_t jb = cs; jb < ce; jb += WAVE) {
             const int32_t jl = jb + lane;
             // j's image nearest this cell (exact: every j within r_c of an i
-            // of this cell lies in exactly one stencil cell)
-            double qx = 0, qy = 0, qz = 0;
-            float qxf = 0, qyf = 0, qzf = 0;
-            if (MIXED) {
-                if (jl < ce) {
-                    const float4 pj = g.rec_pos32[jl];
-                    qxf = pj.x + (float)sx;
-                    qyf = pj.y + (float)sy;
-                    qzf = pj.z + (float)sz;
-                }
-            } else {
-                if (jl < ce) {
-                    const double4 pj = g.rec_pos[jl];
-                    qx = pj.x + sx;
-                    qy = pj.y + sy;
-                    qz = pj.z + sz;
-                }
+            // of this cell lies in exactly one stencil cell), in FP32 for the
+            // distance test; the exact r < r_c decision is made in FP64 on the
+            // canonical pair
+            float qx = 0, qy = 0, qz = 0;
+            if (jl < ce) {
+                const float4 pj = g.rec_pos32[jl];
+                qx = pj.x + (float)sx;
+                qy = pj.y + (float)sy;
+                qz = pj.z + (float)sz;
             }
             const int cnt = min(WAVE, ce - jb);
             for (int t = 0; t < cnt; ++t) {
-                bool inside;
-                if (MIXED && !allpairs) {
-                    const float dx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qxf), t)) - pxf;
-                    const float dy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qyf), t)) - pyf;
-                    const float dz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qzf), t)) - pzf;
-                    inside = dx * dx + dy * dy + dz * dz < rr2f;
-                } else if (MIXED) {
-                    double dx = (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qxf), t)) - pi.x;
-                    double dy = (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qyf), t)) - pi.y;
-                    double dz = (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qzf), t)) - pi.z;
-                    dx = min_image(dx, L);
-                    dy = min_image(dy, L);
-                    dz = min_image(dz, L);
-                    inside = dx * dx + dy * dy + dz * dz < rr2;
-                } else {
-                    double dx = readlane_d(qx, t) - pi.x;
-                    double dy = readlane_d(qy, t) - pi.y;
-                    double dz = readlane_d(qz, t) - pi.z;
-                    if (allpairs) {
-                        dx = min_image(dx, L);
-                        dy = min_image(dy, L);
-                        dz = min_image(dz, L);
-                    }
-                    inside = dx * dx + dy * dy + dz * dz < rr2;
+                float dx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qx), t)) - pxf;
+                float dy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qy), t)) - pyf;
+                float dz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qz), t)) - pzf;
+                if (allpairs) {
+                    dx = dx > Lf / 2 ? dx - Lf : (dx < -Lf / 2 ? dx + Lf : dx);
+                    dy = dy > Lf / 2 ? dy - Lf : (dy < -Lf / 2 ? dy + Lf : dy);
+                    dz = dz > Lf / 2 ? dz - Lf : (dz < -Lf / 2 ? dz + Lf : dz);
                 }
+                const bool inside = dx * dx + dy * dy + dz * dz < rr2f;
                 const bool hit = vi && (jb + t != k) && inside;
                 const uint64_t m = __ballot(hit);
                 if (m) {
