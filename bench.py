@@ -117,6 +117,62 @@ def roofline(stats, counts, n, nk, steps, prec):
             "unique_pairs_per_launch": pairs_per_launch, "kvecs": nk}, breakdown
 
 
+def mix64(z):
+    """splitmix64 finaliser: replica seeds from (master seed, replica)."""
+    m = (1 << 64) - 1
+    z = (z + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def ensemble(mds, workload, precision, replicas, steps, warmup, master=20261203):
+    """Batched ensemble: `replicas` independent suspensions in one context
+    (every kernel launch covers all of them), N uniform in [500, 20000],
+    volume fraction uniform in [1 %, 10 %], applied field on a 0-100 mT grid,
+    Philox seed hash(master, replica); monodisperse 10 nm cores + 5 nm shell,
+    alpha_G 0.1, fixed dt 1e-12 s; r_c / L and the cell reach autotuned on the
+    whole batch; steps replayed from a HIP graph."""
+    rng = np.random.default_rng(master)
+    ns = rng.integers(500, 20001, replicas)
+    phis = rng.uniform(0.01, 0.10, replicas)
+    fields = np.linspace(0.0, 0.1, replicas)
+    cfg = mds.config_from_string(workload.config_text("c1", seed=master, precision=precision))
+    ctx = mds.Context(cfg)
+    t0 = time.perf_counter()
+    systems = []
+    for k in range(replicas):
+        n = int(ns[k])
+        r = np.full(n, cfg.r_mag_mean)
+        rh = r + cfg.d_shell
+        L = (np.sum(4.0 / 3.0 * cfg.my_pi * rh ** 3) / phis[k]) ** (1.0 / 3.0)
+        sd = mix64(master * 1000003 + k)
+        pos = mds.init_positions(cfg, rh, L, sd, periodic=True)
+        mm, ea = mds.init_orientations(cfg, n, sd ^ 0x5bd1e995)
+        systems.append(dict(r_mag=r, r_hydr=rh, l_box=L, field_z=float(fields[k]), seed=sd, pos=pos, mm=mm, ea=ea))
+    ctx.add_replicas(systems)
+    t_init = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tuned = ctx.autotune((0.2, 0.25, 0.3, 0.35), (0, 2, 3), reps=1)
+    t_tune = time.perf_counter() - t0
+    ctx.use_philox(0, 0)
+    ctx.set_graph(True)
+    ctx.run(warmup, 0.0, 1e-12)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(steps, 0.0, 1e-12)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    mz = ctx.replica_mz()
+    total = int(ns.sum())
+    out = {"value": total * steps / el, "unit": "particle-steps/s", "replicas": replicas, "particles": total,
+           "steps": steps, "warmup": warmup, "ms_per_step": el / steps * 1e3, "precision": precision,
+           "autotune": tuned, "init_s": t_init, "tune_s": t_tune, "device_gib": ctx.device_bytes() / 2 ** 30,
+           "mz_first_last": [float(mz[0]), float(mz[-1])], "graph": True}
+    ctx.close()
+    return out
+
+
 def cpu_baseline(sysinfo, threads):
     """One Heun step of the same N = 1e5 system with the CPU oracle (the
     reference's algorithm restated: O(N^2) pair scan, k-space two-pass sums)."""
@@ -152,6 +208,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--replicas", type=int, default=256, help="replicas of the batched-ensemble leg (0 = skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
@@ -204,6 +262,12 @@ def main():
     value = results["mixed"]["value"]
     el = results["mixed"]["el"]
     roof, breakdown = results["mixed"]["roof"], results["mixed"]["breakdown"]
+    ens = {}
+    if args.replicas > 0:
+        for prec in ("fp64", "mixed"):
+            log(f"batched ensemble: {args.replicas} replicas, {prec}")
+            ens[prec] = ensemble(mds, workload, prec, args.replicas, args.ensemble_steps, 2)
+            log(f"  {ens[prec]}")
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline: one oracle Heun step at N={n} on {args.cpu_threads} threads")
@@ -226,6 +290,7 @@ def main():
         "mixed_err_vs_fp64": mixed_err,
         "roofline": roof, "roofline_fp64": results["fp64"]["roof"], "kernel_ms_per_step": breakdown,
         "kernel_ms_per_step_fp64": results["fp64"]["breakdown"], "cpu_baseline": cpu,
+        "ensemble": ens or None,
     }
     print(json.dumps(line), flush=True)
 

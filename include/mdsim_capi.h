@@ -128,6 +128,28 @@ mds_status mds_ewald_params_for_rc(const mds_config* cfg, int64_t n, const doubl
                                    const double* r_hydr, double l_box, double r_cut, double* alpha,
                                    double* kc);
 
+/* Initial suspension: InitParticleProps' radii (src/simulation.cpp:132-214)
+ * and InitCoordsFromSketch (src/simulation.cpp:220-351), drawn from this
+ * library's seeded xoshiro256** stream instead of EigenRand's Mersenne
+ * twister (same distributions, different numbers).
+ * Radii: rMagMean for SIZE_DIST 0; for SIZE_DIST 1 the reference's lognormal
+ * of mean rMagMean and standard deviation sigma; the added key sigmaLog > 0
+ * selects a lognormal of median rMagMedian and sigma sigmaLog instead. */
+mds_status mds_init_radii(const mds_config* cfg, int64_t n, uint64_t seed, double* r_mag);
+/* Positions (N x 3): INIT_CONFIG_TRANS 2 is the npDim^3 grid; 1 is random
+ * sequential addition in index order with the reference's rule (candidates in
+ * [-(L/2 - r_i), L/2 - r_i)^3, rejected while nearer than r_i + r_j to an
+ * earlier particle), found through a cell list. periodic != 0 draws over the
+ * whole box and measures by minimum image, so no pair overlaps through the
+ * faces either. MDS_ECAPACITY when 1e8 draws do not place all particles. */
+mds_status mds_init_positions(const mds_config* cfg, int64_t n, const double* r_hydr, double l_box,
+                              uint64_t seed, int32_t periodic, double* pos);
+/* Easy axes uniform on the sphere by the reference's sampler (azimuth 2 pi u,
+ * elevation 2 asin sqrt u) and moments per INIT_CONFIG_ROT (1: along z,
+ * 2: on the easy axes, 3: easy axes and moments along z). */
+mds_status mds_init_orientations(const mds_config* cfg, int64_t n, uint64_t seed, double* pos_mm,
+                                 double* pos_ea);
+
 /* ----------------------------------------------------------------------- */
 /* Device context                                                          */
 /* ----------------------------------------------------------------------- */
@@ -151,7 +173,9 @@ mds_status mds_set_ewald(mds_ctx* ctx, double alpha, double r_cut, int32_t nc);
 /* Tiling of the real-space pair search (no counterpart in the reference,
  * whose pair scan is O(N^2), src/interaction.cpp:94-134): neighbour cells of
  * edge >= (r_c + margin) / cell_reach, searched over a culled stencil of
- * +-cell_reach cells. Default 3; chosen by the project's autotuner. */
+ * +-cell_reach cells. Default 3; chosen by the project's autotuner. 0 picks
+ * per replica the largest reach (4..1) whose cells hold >= 32 particles on
+ * average, so sparse replicas get larger cells. */
 mds_status mds_set_tiling(mds_ctx* ctx, int32_t cell_reach);
 mds_status mds_get_tiling(const mds_ctx* ctx, int32_t* cell_reach, int32_t* ncell, int32_t* nstencil);
 /* Autotuner, in place of CompEwaldParams's wall-clock choice among 30 r_c in
@@ -160,7 +184,12 @@ mds_status mds_get_tiling(const mds_ctx* ctx, int32_t* cell_reach, int32_t* ncel
  * configured errTolEwald throughout) and each cell reach in `reaches` is
  * timed on the device over one force + one field evaluation of the current
  * state (`reps` timed repetitions after one warm-up). The fastest candidate
- * is installed (mds_set_ewald + mds_set_tiling) and returned. */
+ * is installed (mds_set_ewald + mds_set_tiling) and returned. With several
+ * replicas, rc_fracs[i] is r_c / L for a replica of the mean size and each
+ * replica gets rc_fracs[i] * (N_mean / N_r)^(1/6), clamped to [0.1, 0.5] (the
+ * real-space/k-space cost balance at fixed accuracy scales so), with its own
+ * alpha and k_c from its own error model. A reach of 0 lets each replica
+ * pick its own (see mds_set_tiling). */
 mds_status mds_autotune(mds_ctx* ctx, const double* rc_fracs, int32_t n_rc, const int32_t* reaches,
                         int32_t n_reach, int32_t reps, double* best_rc_frac, int32_t* best_reach,
                         double* best_ms);
@@ -214,6 +243,59 @@ mds_status mds_kernel_stats(mds_ctx* ctx, int64_t* launches, double* total_ms);
  * pairs visited, k-vectors in use, overlap events. */
 mds_status mds_work_counts(mds_ctx* ctx, int64_t* pairs, int64_t* candidates, int64_t* kvecs,
                            int64_t* overlap_events);
+
+/* ----------------------------------------------------------------------- */
+/* Replicas: several independent systems in one context                    */
+/* ----------------------------------------------------------------------- */
+/* No counterpart in the reference, which runs one system per process
+ * (src/simulation.cpp:12-16 keeps it in static globals). A context holds an
+ * ordered list of replicas; their particles are concatenated in every array
+ * the other entry points take or return (replica r owns rows
+ * [offset_r, offset_r + n_r) of each column), and every kernel launch covers
+ * all of them. Each replica has its own box, Ewald parameters, applied field
+ * (magFluxDens, switched off at tMag of its own clock), Philox seed, step
+ * counter and time. mds_set_particles makes a context of one replica.
+ * Several replicas need fixedDt (the adaptive controller's error norm would
+ * couple them). */
+
+/* Append a replica: radii as mds_set_particles (r_hydr may be NULL), box
+ * edge (<= 0: from volFrac), field_z in T, Philox seed, and optionally its
+ * state (all three N x 3 arrays or none). Its clock and step counter start at
+ * 0; its Ewald parameters must be set before the next step
+ * (mds_replica_set_ewald, mds_set_ewald or mds_autotune). Device state of the
+ * other replicas is kept on the device. */
+mds_status mds_add_replica(mds_ctx* ctx, int64_t n, const double* r_mag, const double* r_hydr, double l_box,
+                           double field_z, uint64_t seed, const double* pos, const double* pos_mm,
+                           const double* pos_ea, int32_t* index);
+/* Append `count` replicas at once (one re-layout of the device arrays):
+ * n[count] sizes; r_mag, r_hydr (may be NULL) over all added particles in
+ * order; l_box, field_z, seeds per replica (NULL: box from volFrac, the
+ * configured magFluxDens, seed 0); pos, pos_mm, pos_ea (all or none) N x 3
+ * column-major over all added particles. *first_index gets the first new
+ * index. */
+mds_status mds_add_replicas(mds_ctx* ctx, int32_t count, const int64_t* n, const double* r_mag,
+                            const double* r_hydr, const double* l_box, const double* field_z,
+                            const uint64_t* seeds, const double* pos, const double* pos_mm,
+                            const double* pos_ea, int32_t* first_index);
+/* Remove replica `index`; later replicas move down by one index. */
+mds_status mds_remove_replica(mds_ctx* ctx, int32_t index);
+int32_t mds_replica_count(const mds_ctx* ctx);
+/* InitEwaldConst (src/interaction.cpp:353-374) for one replica. */
+mds_status mds_replica_set_ewald(mds_ctx* ctx, int32_t index, double alpha, double r_cut, int32_t nc);
+/* Size, row offset, box, Ewald parameters, clock and accepted steps. Any
+ * output may be NULL. */
+mds_status mds_replica_info(mds_ctx* ctx, int32_t index, int64_t* n, int64_t* offset, double* l_box,
+                            double* alpha, double* r_cut, int32_t* nc, double* t, uint64_t* steps);
+/* State of one replica (N_r x 3, column-major). */
+mds_status mds_replica_get_state(mds_ctx* ctx, int32_t index, double* pos, double* pos_mm, double* pos_ea);
+/* <m_z> of every replica after the last step (src/simulation.cpp:513). With
+ * several replicas, mds_run's mz_series is steps x replicas, row-major. */
+mds_status mds_replica_mz(mds_ctx* ctx, double* mz);
+/* With on != 0, mds_run (fixedDt) captures one step into a HIP graph and
+ * replays it (re-captured after any change of replicas, parameters or dt). */
+mds_status mds_set_graph(mds_ctx* ctx, int32_t on);
+/* Device memory held by the context, in bytes. */
+int64_t mds_device_bytes(const mds_ctx* ctx);
 
 #ifdef __cplusplus
 }

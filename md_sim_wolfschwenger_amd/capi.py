@@ -76,6 +76,9 @@ SIGNATURES = [
                                          C.c_double, C.c_double]),
     ("mds_ewald_params_for_rc", C.c_int, [C.POINTER(Config), C.c_int64, _dp, _dp, C.c_double, C.c_double, _dp,
                                           _dp]),
+    ("mds_init_radii", C.c_int, [C.POINTER(Config), C.c_int64, C.c_uint64, _dp]),
+    ("mds_init_positions", C.c_int, [C.POINTER(Config), C.c_int64, _dp, C.c_double, C.c_uint64, C.c_int32, _dp]),
+    ("mds_init_orientations", C.c_int, [C.POINTER(Config), C.c_int64, C.c_uint64, _dp, _dp]),
     ("mds_create", C.c_int, [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]),
     ("mds_destroy", None, [C.c_void_p]),
     ("mds_last_error", C.c_char_p, [C.c_void_p]),
@@ -102,6 +105,19 @@ SIGNATURES = [
     ("mds_kernel_class_name", C.c_char_p, [C.c_int32]),
     ("mds_kernel_stats", C.c_int, [C.c_void_p, _i64p, _dp]),
     ("mds_work_counts", C.c_int, [C.c_void_p, _i64p, _i64p, _i64p, _i64p]),
+    ("mds_add_replica", C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, C.c_double, C.c_double, C.c_uint64, _dp, _dp,
+                                  _dp, C.POINTER(C.c_int32)]),
+    ("mds_add_replicas", C.c_int, [C.c_void_p, C.c_int32, _i64p, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint64), _dp,
+                                   _dp, _dp, C.POINTER(C.c_int32)]),
+    ("mds_remove_replica", C.c_int, [C.c_void_p, C.c_int32]),
+    ("mds_replica_count", C.c_int32, [C.c_void_p]),
+    ("mds_replica_set_ewald", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_int32]),
+    ("mds_replica_info", C.c_int, [C.c_void_p, C.c_int32, _i64p, _i64p, _dp, _dp, _dp, C.POINTER(C.c_int32), _dp,
+                                   C.POINTER(C.c_uint64)]),
+    ("mds_replica_get_state", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp]),
+    ("mds_replica_mz", C.c_int, [C.c_void_p, _dp]),
+    ("mds_set_graph", C.c_int, [C.c_void_p, C.c_int32]),
+    ("mds_device_bytes", C.c_int64, [C.c_void_p]),
 ]
 
 _lib = None
@@ -151,6 +167,36 @@ def config_from_ini(path: str) -> Config:
     if st:
         raise MdsError(st, path)
     return c
+
+
+def init_radii(cfg: Config, n: int, seed: int) -> np.ndarray:
+    """Core radii of n particles (InitParticleProps' size distributions)."""
+    r = np.zeros(n)
+    st = lib().mds_init_radii(C.byref(cfg), n, int(seed) & 0xFFFFFFFFFFFFFFFF, _p(r))
+    if st:
+        raise MdsError(st, "init_radii")
+    return r
+
+
+def init_positions(cfg: Config, r_hydr, l_box: float, seed: int, periodic: bool = True) -> np.ndarray:
+    """(N, 3) initial positions (InitCoordsFromSketch's translational part)."""
+    rh = np.ascontiguousarray(r_hydr, dtype=np.float64)
+    n = len(rh)
+    p = np.zeros(3 * n)
+    st = lib().mds_init_positions(C.byref(cfg), n, _p(rh), float(l_box), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                  1 if periodic else 0, _p(p))
+    if st:
+        raise MdsError(st, "init_positions")
+    return rm(p, n)
+
+
+def init_orientations(cfg: Config, n: int, seed: int):
+    """(pos_mm, pos_ea), each (N, 3) (InitCoordsFromSketch's rotational part)."""
+    mm, ea = np.zeros(3 * n), np.zeros(3 * n)
+    st = lib().mds_init_orientations(C.byref(cfg), n, int(seed) & 0xFFFFFFFFFFFFFFFF, _p(mm), _p(ea))
+    if st:
+        raise MdsError(st, "init_orientations")
+    return rm(mm, n), rm(ea, n)
 
 
 def device_count() -> int:
@@ -262,9 +308,14 @@ class Context:
         return tt.value, dd.value, du.value, (mz.value if want_mz else None)
 
     def run(self, steps, t, dt, want_mz=False):
+        """`steps` steps; with want_mz the <m_z> series (steps,) for one
+        replica, (steps, replicas) for several."""
         tt, dd = C.c_double(t), C.c_double(dt)
-        mz = np.zeros(steps) if want_mz else None
+        R = max(1, self.replica_count())
+        mz = np.zeros(steps * R) if want_mz else None
         self._chk(self._L.mds_run(self.h, int(steps), C.byref(tt), C.byref(dd), _p(mz)), "run")
+        if mz is not None and R > 1:
+            mz = mz.reshape(steps, R)
         return tt.value, dd.value, mz
 
     def synchronize(self):
@@ -285,3 +336,79 @@ class Context:
         self._chk(self._L.mds_work_counts(self.h, *[C.byref(x) for x in v]), "work_counts")
         return {"pair_evals": v[0].value, "candidates": v[1].value, "kvecs": v[2].value,
                 "overlap_events": v[3].value}
+
+    # ------------------------------------------------------------ replicas
+    def add_replica(self, r_mag, r_hydr=None, l_box=0.0, field_z=0.0, seed=0, pos=None, mm=None, ea=None):
+        """Append a replica; returns its index."""
+        r_mag = np.ascontiguousarray(r_mag, dtype=np.float64)
+        rh = None if r_hydr is None else np.ascontiguousarray(r_hydr, dtype=np.float64)
+        st = [None if x is None else cm(x) for x in (pos, mm, ea)]
+        idx = C.c_int32()
+        self._chk(self._L.mds_add_replica(self.h, len(r_mag), _p(r_mag), _p(rh), float(l_box), float(field_z),
+                                          int(seed) & 0xFFFFFFFFFFFFFFFF, *[_p(x) for x in st], C.byref(idx)),
+                  "add_replica")
+        self.n += len(r_mag)
+        return idx.value
+
+    def add_replicas(self, systems):
+        """Append several replicas in one call; `systems` is a list of dicts
+        with r_mag and optionally r_hydr, l_box, field_z, seed, pos, mm, ea.
+        Returns the first new index."""
+        ns = np.array([len(d["r_mag"]) for d in systems], np.int64)
+        cat = lambda key: np.ascontiguousarray(np.concatenate([np.asarray(d[key], np.float64) for d in systems]))
+        r = cat("r_mag")
+        rh = cat("r_hydr") if all(d.get("r_hydr") is not None for d in systems) else None
+        lb = np.array([float(d.get("l_box", 0.0)) for d in systems])
+        fz = np.array([float(d.get("field_z", self.cfg.mag_flux_dens)) for d in systems])
+        sd = np.array([int(d.get("seed", 0)) & 0xFFFFFFFFFFFFFFFF for d in systems], np.uint64)
+        have = all(d.get("pos") is not None for d in systems)
+        st = [cm(cat(k)) if have else None for k in ("pos", "mm", "ea")]
+        first = C.c_int32()
+        self._chk(self._L.mds_add_replicas(self.h, len(systems), ns.ctypes.data_as(_i64p), _p(r), _p(rh), _p(lb),
+                                           _p(fz), sd.ctypes.data_as(C.POINTER(C.c_uint64)), *[_p(x) for x in st],
+                                           C.byref(first)), "add_replicas")
+        self.n += int(ns.sum())
+        return first.value
+
+    def remove_replica(self, index):
+        n = self.replica_info(index)["n"]
+        self._chk(self._L.mds_remove_replica(self.h, int(index)), "remove_replica")
+        self.n -= n
+
+    def replica_count(self):
+        return int(self._L.mds_replica_count(self.h))
+
+    def replica_set_ewald(self, index, alpha, r_cut, nc):
+        self._chk(self._L.mds_replica_set_ewald(self.h, int(index), float(alpha), float(r_cut), int(nc)),
+                  "replica_set_ewald")
+
+    def replica_info(self, index, clock=False):
+        n, off = C.c_int64(), C.c_int64()
+        L, a, rc, t = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        nc = C.c_int32()
+        steps = C.c_uint64()
+        self._chk(self._L.mds_replica_info(self.h, int(index), C.byref(n), C.byref(off), C.byref(L), C.byref(a),
+                                           C.byref(rc), C.byref(nc), C.byref(t) if clock else None,
+                                           C.byref(steps) if clock else None), "replica_info")
+        d = {"n": n.value, "offset": off.value, "l_box": L.value, "alpha": a.value, "r_cut": rc.value,
+             "nc": nc.value}
+        if clock:
+            d.update(t=t.value, steps=steps.value)
+        return d
+
+    def replica_state(self, index):
+        n = self.replica_info(index)["n"]
+        a, b, c = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+        self._chk(self._L.mds_replica_get_state(self.h, int(index), _p(a), _p(b), _p(c)), "replica_get_state")
+        return rm(a, n), rm(b, n), rm(c, n)
+
+    def replica_mz(self):
+        mz = np.zeros(self.replica_count())
+        self._chk(self._L.mds_replica_mz(self.h, _p(mz)), "replica_mz")
+        return mz
+
+    def set_graph(self, on=True):
+        self._chk(self._L.mds_set_graph(self.h, 1 if on else 0), "set_graph")
+
+    def device_bytes(self):
+        return int(self._L.mds_device_bytes(self.h))

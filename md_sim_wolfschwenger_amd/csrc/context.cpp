@@ -3,6 +3,11 @@
 // Each entry point maps to one reference function (see include/mdsim_capi.h).
 // The step follows RunSimulation's loop body (src/simulation.cpp:466-520);
 // the adaptive controller is a restatement of AdjustTimeStep (912-951).
+//
+// A context holds one or more replicas: independent periodic systems whose
+// particles are concatenated in every device array (replica r owns
+// [off_r, off_r + n_r)). Every kernel launch covers all replicas; the kernels
+// find a particle's, cell's or k-vector's replica in a small table (Rep).
 
 #include <hip/hip_runtime.h>
 
@@ -34,6 +39,7 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // capacity >= count; contents are not kept
     hipError_t ensure(size_t count) {
         if (count <= n && p) return hipSuccess;
         release();
@@ -43,6 +49,44 @@ struct DevBuf {
         return e;
     }
 };
+
+// A run of particles [src, src + len) of the old layout that goes to
+// [dst, dst + len) of the new one.
+struct Seg {
+    int64_t src, dst, len;
+};
+
+// Rebuild a column-major array of `cols` columns (old column stride
+// old_n, new stride new_n) from segments of the old one; rows not covered
+// by a segment are left for the caller to fill. Stream-ordered.
+template <typename T>
+hipError_t repack(DevBuf<T>& b, int cols, int64_t old_n, int64_t new_n, const std::vector<Seg>& segs,
+                  hipStream_t s) {
+    T* np = nullptr;
+    const size_t need = std::max<size_t>(1, (size_t)cols * (size_t)new_n);
+    hipError_t e = hipMalloc((void**)&np, need * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (b.p && old_n > 0)
+        for (int d = 0; d < cols; ++d)
+            for (const Seg& g : segs)
+                if (g.len > 0) {
+                    e = hipMemcpyAsync(np + (size_t)d * new_n + g.dst, b.p + (size_t)d * old_n + g.src,
+                                       (size_t)g.len * sizeof(T), hipMemcpyDeviceToDevice, s);
+                    if (e != hipSuccess) {
+                        (void)hipFree(np);
+                        return e;
+                    }
+                }
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipFree(np);
+        return e;
+    }
+    b.release();
+    b.p = np;
+    b.n = need;
+    return hipSuccess;
+}
 
 // Kernel classes timed by HIP events.
 enum KClass {
@@ -121,6 +165,27 @@ Tableau make_tableau(int solver) {
 
 int blocks_for(int64_t n, int bs = 256) { return (int)std::max<int64_t>(1, (n + bs - 1) / bs); }
 
+// splitmix64 finaliser (Steele, Lea, Flood, OOPSLA 2014): replica seeds
+uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Host-side description of a replica (what the device Rep table is planned from).
+struct RepMeta {
+    int64_t n = 0, off = 0;
+    double L = 0, rc_sr = 0, alpha = 0, rc = 0, field_z = 0, rmax = 0;
+    double mref = 1;  // largest magnetic moment (unit of the FP32 pair path)
+    int32_t nc = 0;
+    bool ewald_set = false;
+    uint64_t seed = 0;
+    double margin_scale = 0.1;
+};
+
+constexpr int32_t kSeriesCap = 1024;  // <m_z> rows buffered on the device per run chunk
+
 }  // namespace
 
 struct mds_ctx {
@@ -129,32 +194,38 @@ struct mds_ctx {
     hipStream_t stream = nullptr;
     std::string err;
 
-    int64_t n = 0;
-    double L = 0, rc_sr = 0, alpha = 0, rc = 0;
-    int32_t nc = 0;
-    bool ewald_set = false;
+    // replicas
+    std::vector<RepMeta> meta;
+    std::vector<Rep> h_rep;
+    int64_t n = 0;  // all particles
+    int32_t nrep = 0;
     bool state_set = false;
+    bool planned = false;
     bool geometry_ready = false;
     bool philox = false;
-    uint64_t seed = 0;
+    int32_t reach = 3;
+    // host copies of per-particle constants (planning, error model)
+    std::vector<double> h_r_mag, h_mag_mom, h_r_hydr;
 
-    // per-particle constants
+    // per-particle constants and state (kept across replica changes)
     DevBuf<double> r_mag, mag_mom, zeta_rot, zeta_trans, vol_mag, c1, c2;
-    // state
-    DevBuf<double> pos, pos0, pos_cur, mm, ea;
-    // noise, outputs
-    DevBuf<double> n_t, n_b, n_f, force, field, mom;
+    DevBuf<double> pos, mm, ea, n_t, n_b, n_f;
+    // per-step scratch
+    DevBuf<double> pos0, pos_cur, force, field, mom;
     DevBuf<double4> rec_pos, rec_mom;
     DevBuf<float4> rec_pos32;
-    double mref = 1.0;  // largest magnetic moment (unit of the FP32 pair path)
-    std::vector<double> h_mag_mom, h_r_hydr;
+    DevBuf<int32_t> rep_of;
+    // replica table and per-replica device values
+    DevBuf<Rep> reps;
+    DevBuf<double> rep_t, mz_out, msum;
+    DevBuf<uint64_t> rep_step;
     // integrator
     DevBuf<double> kmm, kea, pt[2], new_mm, new_ea;
     // cell lists
-    int32_t ncell = 1, reach = 3, fncell = 1;
-    double cell = 0, fcell = 0, margin = 0, margin_scale = 0.1;
-    int32_t work_cap = 0, cand_cap = 0, nsten = 0;
+    int32_t work_cap = 0, cand_cap = 0, cells_main = 0, cells_fine = 0;
+    std::vector<int4> h_sten;
     DevBuf<int4> stencil;
+    DevBuf<int32_t> cell_rep;
     DevBuf<int32_t> key_main, key_fine, cnt_main, start_main, end_main, cur_main, sorted_main, inv_perm, work, nwork;
     DevBuf<int32_t> cnt_fine, start_fine, end_fine, cur_fine, sorted_fine;
     DevBuf<int32_t> cand_count, cand_off, cand_j, ev_head, ev_tail, ev_next, n_events, err_flag, moved_list, n_moved;
@@ -163,16 +234,19 @@ struct mds_ctx {
     // k space
     std::vector<KRow> h_rows;
     std::vector<KSeg> h_segs;
-    int32_t nk = 0, nchunks = 1, kchunk = 256;
+    int32_t nk_all = 0, kb_items = 0;
     DevBuf<KRow> rows;
     DevBuf<KSeg> segs;
     DevBuf<double> ek, sk, kpartial, tab;
-    double c_force = 0, c_flux = 0, c_self = 0, c_surf = 0;
     // reductions
-    DevBuf<double> msum_partial, msum, press_partial, err_partial, scal;  // scal: [0]=err [1]=mz [2..4]=press
-    DevBuf<uint64_t> step_counter, pair_counter;
+    DevBuf<double> press_partial, err_partial, scal;  // scal[0] = error norm
+    DevBuf<uint64_t> pair_counter;
     DevBuf<int32_t> series_counter;
     DevBuf<double> series;
+    // HIP graph of one fixed-dt step
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    double graph_h = 0;
 
     // timing
     bool timing = false;
@@ -181,7 +255,7 @@ struct mds_ctx {
     int64_t launches[KC_COUNT] = {};
     double total_ms[KC_COUNT] = {};
     bool count_work = false;
-    int64_t last_pairs = 0, last_events = 0;
+    int64_t last_events = 0;
 
     hipEvent_t get_event() {
         if (!pool.empty()) {
@@ -224,8 +298,13 @@ struct mds_ctx {
         }
         pending.clear();
     }
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
     ~mds_ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
+        drop_graph();
         for (auto& p : pending) {
             (void)hipEventDestroy(p.second.first);
             (void)hipEventDestroy(p.second.second);
@@ -259,77 +338,86 @@ mds_status download(mds_ctx* c, double* dst, const double* src, size_t count) {
     HIPCHK(hipStreamSynchronize(c->stream));
     return MDS_OK;
 }
-
-// Cell grids: the main grid's stencil of +-reach cells must cover the
-// candidate radius r_c + margin (the margin covers positions moved by the
-// overlap correction); the fine grid covers the contact search radius.
-mds_status plan_grids(mds_ctx* c) {
-    const int64_t n = c->n;
-    std::vector<double> rm(n);
-    HIPCHK(hipMemcpy(rm.data(), c->r_mag.p, n * sizeof(double), hipMemcpyDeviceToHost));
-    const double dmax = 2 * *std::max_element(rm.begin(), rm.end()) + 1e-10;
-    if (c->margin_scale <= 0) c->margin_scale = 0.1;
-    c->margin = c->margin_scale * dmax;
-    const double cmin = (c->rc * (1 + 1e-9) + c->margin) / c->reach;
-    int32_t nc = (int32_t)std::floor(c->L / cmin);
-    if (nc < 2 * c->reach + 1) nc = 1;
-    nc = std::min(nc, 96);
-    c->ncell = nc;
-    c->cell = c->L / nc;
-    const double fmin = dmax + c->margin;
-    int32_t fn = (int32_t)std::floor(c->L / fmin);
-    if (fn < 3) fn = 1;
-    fn = std::min(fn, 128);
-    c->fncell = fn;
-    c->fcell = c->L / fn;
-    // stencil of neighbour-cell offsets whose nearest points lie within the
-    // candidate radius (corner cells of the cube of +-reach are dropped)
-    {
-        std::vector<int4> st;
-        const int R = nc == 1 ? 0 : c->reach;
-        const double lim = c->rc * (1 + 1e-9) + c->margin;
-        for (int dz = -R; dz <= R; ++dz)
-            for (int dy = -R; dy <= R; ++dy)
-                for (int dx = -R; dx <= R; ++dx) {
-                    auto gap = [&](int o) { return std::max(std::abs(o) - 1, 0) * c->cell; };
-                    const double d2 = gap(dx) * gap(dx) + gap(dy) * gap(dy) + gap(dz) * gap(dz);
-                    if (d2 < lim * lim) st.push_back(make_int4(dx, dy, dz, 0));
-                }
-        c->nsten = (int32_t)st.size();
-        HIPCHK(c->stencil.ensure(st.size()));
-        HIPCHK(hipMemcpy(c->stencil.p, st.data(), st.size() * sizeof(int4), hipMemcpyHostToDevice));
-    }
-    const int64_t ncm = (int64_t)nc * nc * nc, ncf = (int64_t)fn * fn * fn;
-    c->work_cap = (int32_t)((n + 63) / 64 + ncm);
-    c->cand_cap = (int32_t)(2 * n + 4096);
-    HIPCHK(c->cnt_main.ensure(ncm));
-    HIPCHK(c->start_main.ensure(ncm));
-    HIPCHK(c->end_main.ensure(ncm));
-    HIPCHK(c->cur_main.ensure(ncm));
-    HIPCHK(c->work.ensure(c->work_cap));
-    HIPCHK(c->cnt_fine.ensure(ncf));
-    HIPCHK(c->start_fine.ensure(ncf));
-    HIPCHK(c->end_fine.ensure(ncf));
-    HIPCHK(c->cur_fine.ensure(ncf));
-    HIPCHK(c->cand_j.ensure(2 * (size_t)c->cand_cap));
-    HIPCHK(c->ev_next.ensure(2 * (size_t)c->cand_cap));
-    HIPCHK(c->ev_key.ensure(2 * (size_t)c->cand_cap));
-    HIPCHK(c->ev_pos.ensure(6 * (size_t)c->cand_cap));
-    HIPCHK(c->press_partial.ensure(3 * ((size_t)c->work_cap + 1)));
-    HIPCHK(c->moved_list.ensure(2 * (size_t)c->cand_cap));
+// column-major N x 3 slice [off, off + m) of an array with stride n
+mds_status upload_cols(mds_ctx* c, double* dst, int64_t n, int64_t off, int64_t m, const double* src) {
+    for (int d = 0; d < 3; ++d)
+        HIPCHK(hipMemcpyAsync(dst + (size_t)d * n + off, src + (size_t)d * m, (size_t)m * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+    return MDS_OK;
+}
+mds_status download_cols(mds_ctx* c, double* dst, const double* src, int64_t n, int64_t off, int64_t m) {
+    for (int d = 0; d < 3; ++d)
+        HIPCHK(hipMemcpyAsync(dst + (size_t)d * m, src + (size_t)d * n + off, (size_t)m * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return MDS_OK;
 }
 
-// k-space rows of the half space n_z > 0, or n_z = 0 and (n_y > 0, or n_y = 0
-// and n_x > 0), |n|^2 <= n_c^2; with the reference's full n_z = 0 plane at
-// weight 1/2 (src/interaction.cpp:438-457) this is the same sum.
-mds_status plan_kspace(mds_ctx* c) {
-    const int nc = c->nc;
+// ---------------------------------------------------------------- planning
+// Cell grids of replica r: the main grid's stencil of +-reach cells must
+// cover the candidate radius r_c + margin (the margin covers positions moved
+// by the overlap correction); the fine grid covers the contact search radius.
+// Main-grid cells per dimension for a stencil half-width `reach`.
+int32_t main_cells(const RepMeta& m, double margin, int reach) {
+    const double cmin = (m.rc * (1 + 1e-9) + margin) / reach;
+    int32_t nc = (int32_t)std::floor(m.L / cmin);
+    if (nc < 2 * reach + 1) nc = 1;
+    return std::min(nc, 96);
+}
+
+// Cell reach 0 picks one per replica: the largest of 4..1 whose cells hold
+// at least 32 particles on average (one i-chunk of the pair kernel is up to 64
+// particles of one cell, so emptier cells leave lanes idle), else 1.
+int replica_reach(const mds_ctx* c, const RepMeta& m, double margin) {
+    if (c->reach > 0) return c->reach;
+    for (int r = 4; r >= 1; --r) {
+        const double nc = main_cells(m, margin, r);
+        if ((double)m.n / (nc * nc * nc) >= 32.0) return r;
+    }
+    return 1;
+}
+
+void plan_grid(mds_ctx* c, const RepMeta& m, Rep& R) {
+    const double dmax = 2 * m.rmax + 1e-10;
+    R.margin = m.margin_scale * dmax;
+    const int reach = replica_reach(c, m, R.margin);
+    const int32_t nc = main_cells(m, R.margin, reach);
+    R.ncell = nc;
+    R.cell = m.L / nc;
+    // contact-search grid: cells of at least the search diameter, about two
+    // per particle at most
+    const double fmin = dmax + R.margin;
+    int32_t fn = (int32_t)std::floor(m.L / fmin);
+    fn = std::min<int32_t>(fn, std::max<int32_t>(3, (int32_t)std::ceil(std::cbrt(2.0 * (double)m.n))));
+    if (fn < 3) fn = 1;
+    fn = std::min(fn, 128);
+    R.fncell = fn;
+    R.fcell = m.L / fn;
+    // stencil of neighbour-cell offsets whose nearest points lie within the
+    // candidate radius (corner cells of the cube of +-reach are dropped)
+    const int H = nc == 1 ? 0 : reach;
+    const double lim = m.rc * (1 + 1e-9) + R.margin;
+    R.sten_off = (int32_t)c->h_sten.size();
+    for (int dz = -H; dz <= H; ++dz)
+        for (int dy = -H; dy <= H; ++dy)
+            for (int dx = -H; dx <= H; ++dx) {
+                auto gap = [&](int o) { return std::max(std::abs(o) - 1, 0) * R.cell; };
+                const double d2 = gap(dx) * gap(dx) + gap(dy) * gap(dy) + gap(dz) * gap(dz);
+                if (d2 < lim * lim) c->h_sten.push_back(make_int4(dx, dy, dz, 0));
+            }
+    R.nsten = (int32_t)c->h_sten.size() - R.sten_off;
+}
+
+// k-space rows of replica r over the half space n_z > 0, or n_z = 0 and
+// (n_y > 0, or n_y = 0 and n_x > 0), |n|^2 <= n_c^2; with the reference's
+// full n_z = 0 plane at weight 1/2 (src/interaction.cpp:438-457) this is the
+// same sum.
+void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R, std::vector<double>& ek) {
+    const int nc = m.nc;
     const int64_t nc2 = (int64_t)nc * nc;
-    c->h_rows.clear();
-    c->h_segs.clear();
-    std::vector<double> ek;
-    const double w = std::pow(c->cfg.my_pi / (c->L * c->alpha), 2.0);
+    R.row_off = (int32_t)c->h_rows.size();
+    R.seg_off = (int32_t)c->h_segs.size();
+    const double w = std::pow(c->cfg.my_pi / (m.L * m.alpha), 2.0);
     int32_t koff = 0;
     for (int nz = 0; nz <= nc; ++nz)
         for (int ny = (nz == 0 ? 0 : -nc); ny <= nc; ++ny) {
@@ -341,7 +429,8 @@ mds_status plan_kspace(mds_ctx* c) {
             if (half && nxmax == 0) continue;
             c->h_rows.push_back({ny, nz, nxmax, koff});
             const int a0 = half ? 1 : 0;
-            for (int a = a0; a <= nxmax; a += KSEG) c->h_segs.push_back({ny, nz, a, std::min(nxmax, a + KSEG - 1), koff, nxmax});
+            for (int a = a0; a <= nxmax; a += KSEG)
+                c->h_segs.push_back({ny, nz, a, std::min(nxmax, a + KSEG - 1), koff, nxmax});
             const int lo = half ? 1 : -nxmax;
             for (int nx = lo; nx <= nxmax; ++nx) {
                 const double len = (double)nx * nx + (double)r2;
@@ -349,39 +438,184 @@ mds_status plan_kspace(mds_ctx* c) {
             }
             koff += half ? nxmax : 2 * nxmax + 1;
         }
-    c->nk = koff;
-    const double pi = c->cfg.my_pi, f = c->cfg.fac_ewald, L = c->L;
+    R.nrows = (int32_t)c->h_rows.size() - R.row_off;
+    R.nseg = (int32_t)c->h_segs.size() - R.seg_off;
+    R.nk = koff;
+    R.nc = nc;
+    const double pi = c->cfg.my_pi, f = c->cfg.fac_ewald, L = m.L;
     // InitEwaldConst, src/interaction.cpp:366-369
-    c->c_flux = 4.0 * pi * f / std::pow(L, 3.0);
-    c->c_force = 8.0 * pi * pi * f / std::pow(L, 4.0);
-    c->c_surf = f * 4.0 * pi / (3 * std::pow(L, 3.0));
-    c->c_self = f * 4.0 * std::pow(c->alpha, 3.0) / (3.0 * std::sqrt(pi));
-    const int64_t nsegw = ((int64_t)c->h_segs.size() + 63) / 64;
-    int64_t chunk = (c->n * std::max<int64_t>(1, nsegw) + 4095) / 4096;
+    R.c_flux = 4.0 * pi * f / std::pow(L, 3.0);
+    R.c_force = 8.0 * pi * pi * f / std::pow(L, 4.0);
+    R.c_surf = f * 4.0 * pi / (3 * std::pow(L, 3.0));
+    R.c_self = f * 4.0 * std::pow(m.alpha, 3.0) / (3.0 * std::sqrt(pi));
+    R.two_pi_over_L = 2 * pi / L;
+    R.nsegw = (R.nseg + 63) / 64;
+    int64_t chunk = (m.n * std::max<int64_t>(1, R.nsegw) + 4095) / 4096;
     chunk = std::max<int64_t>(32, std::min<int64_t>(1024, chunk));
-    c->kchunk = (int32_t)chunk;
-    c->nchunks = (int32_t)((c->n + chunk - 1) / chunk);
+    R.kchunk = (int32_t)chunk;
+    R.nchunks = (int32_t)((m.n + chunk - 1) / chunk);
+}
+
+// Ewald constants and FP32 units of replica r.
+void plan_ewald(mds_ctx* c, const RepMeta& m, Rep& R) {
+    R.alpha = m.alpha;
+    R.alpha2 = m.alpha * m.alpha;
+    R.alpha4 = R.alpha2 * R.alpha2;
+    R.rc = m.rc;
+    R.rc_sr = m.rc_sr;
+    const double f = c->cfg.fac_ewald;
+    if (c->cfg.precision == MDS_PREC_MIXED) {
+        R.ell = m.rc;
+        R.inv_ell = 1.0 / m.rc;
+        R.inv_mref = 1.0 / m.mref;
+        R.bscale = f * m.mref / std::pow(R.ell, 3.0);
+        R.fscale = f * m.mref * m.mref / std::pow(R.ell, 4.0);
+        R.pscale = R.ell * R.fscale;
+    } else {
+        R.ell = R.inv_ell = R.inv_mref = 1.0;
+        R.bscale = R.fscale = R.pscale = f;
+    }
+    R.al_s = (float)(m.alpha * R.ell);
+    R.al2_s = R.al_s * R.al_s;
+    R.al4_s = R.al2_s * R.al2_s;
+}
+
+// Plan grids, stencils and k space of every replica, size the scratch
+// buffers and upload the tables.
+mds_status plan_all(mds_ctx* c) {
+    if (c->nrep <= 0 || c->n <= 0) return fail(c, MDS_EINVAL, "no particles");
+    for (const RepMeta& m : c->meta)
+        if (!m.ewald_set) return fail(c, MDS_EINVAL, "Ewald parameters not set");
+    c->drop_graph();
+    c->h_rep.assign(c->nrep, Rep{});
+    c->h_sten.clear();
+    c->h_rows.clear();
+    c->h_segs.clear();
+    std::vector<double> ek;
+    std::vector<int32_t> cell_rep, rep_of((size_t)c->n);
+    int64_t cells = 0, fcells = 0, work_cap = 0, tab = 0, part = 0, kb = 0, kall = 0;
+    for (int r = 0; r < c->nrep; ++r) {
+        const RepMeta& m = c->meta[r];
+        Rep& R = c->h_rep[r];
+        R.off = m.off;
+        R.n = m.n;
+        R.L = m.L;
+        R.field_z = m.field_z;
+        R.seed = m.seed;
+        plan_grid(c, m, R);
+        plan_ewald(c, m, R);
+        plan_kspace(c, m, R, ek);
+        const int64_t ncm = (int64_t)R.ncell * R.ncell * R.ncell, ncf = (int64_t)R.fncell * R.fncell * R.fncell;
+        R.cell_off = (int32_t)cells;
+        R.fcell_off = (int32_t)fcells;
+        cells += ncm;
+        fcells += ncf;
+        cell_rep.insert(cell_rep.end(), (size_t)ncm, r);
+        work_cap += (m.n + 63) / 64 + ncm;
+        R.tab_off = tab;
+        tab += m.n * 3 * (int64_t)(R.nc + 1) * 2;
+        R.k_off = (int32_t)kall;
+        kall += R.nk;
+        R.part_off = part;
+        part += 2 * (int64_t)R.nk * R.nchunks;
+        R.kb_off = (int32_t)kb;
+        kb += (R.nk > 0) ? (int64_t)R.nsegw * R.nchunks : 0;
+        std::fill(rep_of.begin() + m.off, rep_of.begin() + m.off + m.n, r);
+    }
+    // work items pack (cell << 8) | chunk into an int32
+    if (cells >= (1 << 23) || fcells >= (1 << 30) || work_cap >= (1 << 30) || kb >= (1 << 30))
+        return fail(c, MDS_ECAPACITY, "too many cells or k-space work items in one context");
+    c->cells_main = (int32_t)cells;
+    c->cells_fine = (int32_t)fcells;
+    c->work_cap = (int32_t)work_cap;
+    c->cand_cap = (int32_t)std::min<int64_t>(2 * c->n + 4096, (int64_t)1 << 29);
+    c->nk_all = (int32_t)kall;
+    c->kb_items = (int32_t)kb;
+    HIPCHK(c->cnt_main.ensure(cells));
+    HIPCHK(c->start_main.ensure(cells));
+    HIPCHK(c->end_main.ensure(cells));
+    HIPCHK(c->cur_main.ensure(cells));
+    HIPCHK(c->cell_rep.ensure(cells));
+    HIPCHK(c->work.ensure(work_cap));
+    HIPCHK(c->cnt_fine.ensure(fcells));
+    HIPCHK(c->start_fine.ensure(fcells));
+    HIPCHK(c->end_fine.ensure(fcells));
+    HIPCHK(c->cur_fine.ensure(fcells));
+    HIPCHK(c->cand_j.ensure(2 * (size_t)c->cand_cap));
+    HIPCHK(c->ev_next.ensure(2 * (size_t)c->cand_cap));
+    HIPCHK(c->ev_key.ensure(2 * (size_t)c->cand_cap));
+    HIPCHK(c->ev_pos.ensure(6 * (size_t)c->cand_cap));
+    HIPCHK(c->moved_list.ensure(2 * (size_t)c->cand_cap));
+    HIPCHK(c->press_partial.ensure(3 * ((size_t)work_cap + c->nrep)));
+    HIPCHK(c->stencil.ensure(std::max<size_t>(1, c->h_sten.size())));
     HIPCHK(c->rows.ensure(std::max<size_t>(1, c->h_rows.size())));
     HIPCHK(c->segs.ensure(std::max<size_t>(1, c->h_segs.size())));
-    HIPCHK(c->ek.ensure(std::max(1, c->nk)));
-    HIPCHK(c->sk.ensure(4 * (size_t)std::max(1, c->nk)));
-    HIPCHK(c->kpartial.ensure(2 * (size_t)std::max(1, c->nk) * c->nchunks));
-    HIPCHK(c->tab.ensure((size_t)c->n * 3 * (nc + 1) * 2));
+    HIPCHK(c->ek.ensure(std::max<int64_t>(1, kall)));
+    HIPCHK(c->sk.ensure(2 * (size_t)std::max<int64_t>(1, kall)));
+    HIPCHK(c->kpartial.ensure(std::max<int64_t>(1, part)));
+    HIPCHK(c->tab.ensure(std::max<int64_t>(1, tab)));
+    HIPCHK(c->reps.ensure(c->nrep));
+    HIPCHK(c->rep_of.ensure(c->n));
+    HIPCHK(c->msum.ensure(3 * (size_t)c->nrep));
+    HIPCHK(c->mz_out.ensure(c->nrep));
+    HIPCHK(c->series.ensure((size_t)kSeriesCap * c->nrep));
+    hipStream_t s = c->stream;
+    HIPCHK(hipMemcpyAsync(c->reps.p, c->h_rep.data(), c->nrep * sizeof(Rep), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->rep_of.p, rep_of.data(), rep_of.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->cell_rep.p, cell_rep.data(), cell_rep.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                          s));
+    if (!c->h_sten.empty())
+        HIPCHK(hipMemcpyAsync(c->stencil.p, c->h_sten.data(), c->h_sten.size() * sizeof(int4), hipMemcpyHostToDevice,
+                              s));
     if (!c->h_rows.empty())
-        HIPCHK(hipMemcpy(c->rows.p, c->h_rows.data(), c->h_rows.size() * sizeof(KRow), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(c->rows.p, c->h_rows.data(), c->h_rows.size() * sizeof(KRow), hipMemcpyHostToDevice, s));
     if (!c->h_segs.empty())
-        HIPCHK(hipMemcpy(c->segs.p, c->h_segs.data(), c->h_segs.size() * sizeof(KSeg), hipMemcpyHostToDevice));
-    if (!ek.empty()) HIPCHK(hipMemcpy(c->ek.p, ek.data(), ek.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(c->segs.p, c->h_segs.data(), c->h_segs.size() * sizeof(KSeg), hipMemcpyHostToDevice, s));
+    if (!ek.empty()) HIPCHK(hipMemcpyAsync(c->ek.p, ek.data(), ek.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->planned = true;
+    c->geometry_ready = false;
     return MDS_OK;
+}
+
+// Per-particle scratch of n particles (contents not kept).
+mds_status size_scratch(mds_ctx* c) {
+    const int64_t n = c->n;
+    const size_t n3 = 3 * (size_t)n;
+    for (auto* b : {&c->pos0, &c->pos_cur, &c->force, &c->field, &c->mom, &c->new_mm, &c->new_ea}) HIPCHK(b->ensure(n3));
+    HIPCHK(c->pt[0].ensure(2 * n3));
+    HIPCHK(c->pt[1].ensure(2 * n3));
+    const int stages = make_tableau(c->cfg.fixed_dt ? MDS_HEUN_EULER : c->cfg.solver).stages;
+    HIPCHK(c->kmm.ensure((size_t)stages * n3));
+    HIPCHK(c->kea.ensure((size_t)stages * n3));
+    HIPCHK(c->rec_pos.ensure(n));
+    HIPCHK(c->rec_pos32.ensure(n));
+    HIPCHK(c->rec_mom.ensure(n));
+    for (auto* b : {&c->key_main, &c->key_fine, &c->sorted_main, &c->sorted_fine, &c->inv_perm, &c->cand_count,
+                    &c->ev_head, &c->ev_tail})
+        HIPCHK(b->ensure(n));
+    HIPCHK(c->cand_off.ensure(n + 1));
+    HIPCHK(c->err_partial.ensure(blocks_for(n)));
+    return MDS_OK;
+}
+
+mds_status ensure_planned(mds_ctx* c) {
+    if (c->planned) return MDS_OK;
+    mds_status st = size_scratch(c);
+    if (st != MDS_OK) return st;
+    return plan_all(c);
 }
 
 Geometry geometry(mds_ctx* c) {
     Geometry g{};
     g.n = c->n;
-    g.L = c->L;
-    g.ncell = c->ncell;
+    g.L = c->meta.empty() ? 0 : c->meta[0].L;
+    g.reps = c->reps.p;
+    g.rep_of = c->rep_of.p;
+    g.cell_rep = c->cell_rep.p;
+    g.ncell = c->h_rep.empty() ? 1 : c->h_rep[0].ncell;
     g.reach = c->reach;
-    g.cell = c->cell;
+    g.cell = c->h_rep.empty() ? 0 : c->h_rep[0].cell;
     g.cell_start = c->start_main.p;
     g.cell_end = c->end_main.p;
     g.sorted_idx = c->sorted_main.p;
@@ -401,15 +635,11 @@ Geometry geometry(mds_ctx* c) {
     return g;
 }
 
+// Constants shared by all replicas (the per-replica ones come from Rep).
 PairParams pair_params(mds_ctx* c) {
     PairParams pp{};
-    pp.alpha = c->alpha;
-    pp.alpha2 = c->alpha * c->alpha;
-    pp.alpha4 = pp.alpha2 * pp.alpha2;
     pp.ir_pi = c->cfg.ir_pi;
     pp.fac = c->cfg.fac_ewald;
-    pp.rc = c->rc;
-    pp.rc_sr = c->rc_sr;
     pp.coating = c->cfg.enable_coating_pot;
     pp.eff_len = c->cfg.eff_len_surf_mol;
     pp.ster_fac = c->cfg.ster_rep_fac;
@@ -417,21 +647,6 @@ PairParams pair_params(mds_ctx* c) {
     pp.kappa = c->cfg.kappa;
     pp.hamaker = c->cfg.hamaker;
     pp.r_mag = c->r_mag.p;
-    const double f = c->cfg.fac_ewald;
-    if (c->cfg.precision == MDS_PREC_MIXED) {
-        pp.ell = c->rc;
-        pp.inv_ell = 1.0 / c->rc;
-        pp.inv_mref = 1.0 / c->mref;
-        pp.bscale = f * c->mref / std::pow(pp.ell, 3.0);
-        pp.fscale = f * c->mref * c->mref / std::pow(pp.ell, 4.0);
-        pp.pscale = pp.ell * pp.fscale;
-    } else {
-        pp.ell = pp.inv_ell = pp.inv_mref = 1.0;
-        pp.bscale = pp.fscale = pp.pscale = f;
-    }
-    pp.al_s = (float)(c->alpha * pp.ell);
-    pp.al2_s = pp.al_s * pp.al_s;
-    pp.al4_s = pp.al2_s * pp.al2_s;
     return pp;
 }
 
@@ -442,57 +657,63 @@ mds_status prepare_geometry_once(mds_ctx* c) {
     hipStream_t s = c->stream;
     {
         mds_ctx::Span sp(c, KC_SETUP);
-        HIPCHK(launch_prepare(s, n, c->pos.p, c->pos0.p, c->pos_cur.p, c->L, c->ncell, c->cell, c->key_main.p,
-                              c->fncell, c->fcell, c->key_fine.p));
-        HIPCHK(launch_cell_sort(s, n, c->key_main.p, c->ncell * c->ncell * c->ncell, c->cnt_main.p, c->start_main.p,
-                                c->end_main.p, c->cur_main.p, c->sorted_main.p, nullptr));
+        HIPCHK(launch_prepare(s, n, c->pos.p, c->pos0.p, c->pos_cur.p, c->reps.p, c->rep_of.p, c->key_main.p,
+                              c->key_fine.p));
+        HIPCHK(launch_cell_sort(s, n, c->key_main.p, c->cells_main, c->cnt_main.p, c->start_main.p, c->end_main.p,
+                                c->cur_main.p, c->sorted_main.p));
         HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->r_mag.p, c->rec_pos.p, c->rec_pos32.p,
                                     c->inv_perm.p));
-        HIPCHK(launch_build_work(s, c->ncell * c->ncell * c->ncell, c->start_main.p, c->end_main.p, c->work.p,
-                                 c->nwork.p, c->work_cap, c->err_flag.p));
+        HIPCHK(launch_build_work(s, c->cells_main, c->start_main.p, c->end_main.p, c->work.p, c->nwork.p,
+                                 c->work_cap, c->err_flag.p));
     }
     {
         mds_ctx::Span sp(c, KC_CONTACTS);
-        HIPCHK(launch_cell_sort(s, n, c->key_fine.p, c->fncell * c->fncell * c->fncell, c->cnt_fine.p,
-                                c->start_fine.p, c->end_fine.p, c->cur_fine.p, c->sorted_fine.p, nullptr));
-        HIPCHK(launch_clear_events(s, n, c->ev_head.p, c->ev_tail.p));
-        HIPCHK(launch_contacts(s, n, c->L, c->pos0.p, c->r_mag.p, c->fncell, c->fcell, c->start_fine.p,
-                               c->end_fine.p, c->sorted_fine.p, c->margin, c->cand_count.p, c->cand_off.p,
-                               c->cand_j.p, c->cand_cap, nullptr, c->err_flag.p));
-        HIPCHK(launch_resolve(s, n, c->L, c->rc, c->r_mag.p, c->cand_off.p, c->cand_j.p, c->pos_cur.p, c->pos0.p,
-                              c->ev_head.p, c->ev_tail.p, c->ev_next.p, c->ev_key.p, c->ev_pos.p, 2 * c->cand_cap,
-                              c->n_events.p, c->max_disp.p, 0.5 * c->margin, c->moved_list.p, c->n_moved.p,
+        HIPCHK(launch_cell_sort(s, n, c->key_fine.p, c->cells_fine, c->cnt_fine.p, c->start_fine.p, c->end_fine.p,
+                                c->cur_fine.p, c->sorted_fine.p));
+        HIPCHK(launch_clear_events(s, n, c->ev_head.p, c->ev_tail.p, c->n_events.p, c->n_moved.p, c->max_disp.p));
+        HIPCHK(launch_contacts(s, n, c->reps.p, c->rep_of.p, c->pos0.p, c->r_mag.p, c->start_fine.p, c->end_fine.p,
+                               c->sorted_fine.p, c->cand_count.p, c->cand_off.p, c->cand_j.p, c->cand_cap,
+                               c->err_flag.p));
+        // displacement bound: half the smallest contact-search margin
+        double dl = 1e300;
+        for (const Rep& R : c->h_rep) dl = std::min(dl, 0.5 * R.margin);
+        HIPCHK(launch_resolve(s, c->nrep, n, c->reps.p, c->r_mag.p, c->cand_off.p, c->cand_j.p, c->pos_cur.p,
+                              c->pos0.p, c->ev_head.p, c->ev_tail.p, c->ev_next.p, c->ev_key.p, c->ev_pos.p,
+                              2 * c->cand_cap, c->n_events.p, c->max_disp.p, dl, c->moved_list.p, c->n_moved.p,
                               c->err_flag.p));
     }
     {
         mds_ctx::Span sp(c, KC_SINCOS);
-        HIPCHK(launch_sincos_tables(s, n, c->nc, c->L, 2 * c->cfg.my_pi, c->pos0.p, c->tab.p));
+        HIPCHK(launch_sincos_tables(s, n, c->reps.p, c->rep_of.p, c->pos0.p, c->tab.p));
     }
     c->geometry_ready = true;
     return MDS_OK;
 }
 
-// With `check`, the overlap correction's displacement bound is verified
-// before use: a move larger than half the contact-search margin could have
-// hidden a contact, so the search is redone with a 4x wider margin.
+// With `check`, the error flags are cleared first and the overlap
+// correction's displacement bound is verified before use: a move larger than
+// half the contact-search margin could have hidden a contact, so the search
+// is redone with a 4x wider margin. Without it, flags accumulate until read.
 mds_status prepare_geometry(mds_ctx* c, bool check) {
     for (int attempt = 0;; ++attempt) {
-        HIPCHK(launch_fill_i32(c->stream, c->err_flag.p, 1, 0));
+        if (check) HIPCHK(launch_fill_i32(c->stream, c->err_flag.p, 1, 0));
         mds_status st = prepare_geometry_once(c);
         if (st != MDS_OK || !check) return st;
         int32_t f = 0;
         HIPCHK(hipMemcpyAsync(&f, c->err_flag.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        if (!(f & 32) || c->margin_scale >= 2.0 || attempt > 4) return MDS_OK;
-        c->margin_scale = std::min(2.0, c->margin_scale * 4);
-        if ((st = plan_grids(c)) != MDS_OK) return st;
+        double ms = 0;
+        for (const RepMeta& m : c->meta) ms = std::max(ms, m.margin_scale);
+        if (!(f & 32) || ms >= 2.0 || attempt > 4) return MDS_OK;
+        for (RepMeta& m : c->meta) m.margin_scale = std::min(2.0, m.margin_scale * 4);
+        if ((st = plan_all(c)) != MDS_OK) return st;
     }
 }
 
 // Real-space + k-space Ewald (+ short range) for the moments in c->mom /
-// c->rec_mom (their sum in c->msum). force_pass: CompDipInteractEwaldR/F
-// with forces, pressure and the thermal force added; else the field-only
-// CompDipFieldEwaldR/F of CompDemagField.
+// c->rec_mom (their per-replica sums in c->msum). force_pass:
+// CompDipInteractEwaldR/F with forces, pressure and the thermal force added;
+// else the field-only CompDipFieldEwaldR/F of CompDemagField.
 mds_status evaluate(mds_ctx* c, bool force_pass) {
     const int64_t n = c->n;
     hipStream_t s = c->stream;
@@ -502,46 +723,39 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
     {
         mds_ctx::Span sp(c, force_pass ? KC_PAIRS_FORCE : KC_PAIRS_FIELD);
         int64_t* cnt = c->count_work ? (int64_t*)c->pair_counter.p : nullptr;
-
-        if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + 3 * (size_t)c->work_cap, 3, 0.0));
-        HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p,
-                            c->nsten, cnt));
-        HIPCHK(launch_pairs_moved(s, force_pass, g, c->rec_mom.p, pp, po, c->stencil.p, c->nsten, c->work_cap, cnt));
+        if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + 3 * (size_t)c->work_cap, 3 * c->nrep, 0.0));
+        HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p, cnt));
+        HIPCHK(launch_pairs_moved(s, force_pass, g, c->rec_mom.p, pp, po, c->stencil.p, c->work_cap, cnt));
     }
-    if (c->nk > 0) {
+    if (c->nk_all > 0) {
         {
             mds_ctx::Span sp(c, KC_KBUILD);
-            HIPCHK(launch_kbuild(s, c->cfg.precision, n, c->nc, (int32_t)c->h_segs.size(), c->segs.p, c->tab.p,
-                                 c->mom.p, c->kchunk, c->nchunks, c->nk, c->kpartial.p));
+            HIPCHK(launch_kbuild(s, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p, c->kpartial.p));
         }
         {
             mds_ctx::Span sp(c, KC_KREDUCE);
-            HIPCHK(launch_kreduce(s, c->nk, c->nchunks, c->kpartial.p, c->sk.p));
+            HIPCHK(launch_kreduce(s, c->nk_all, c->reps.p, c->nrep, c->kpartial.p, c->sk.p));
         }
     }
     {
         mds_ctx::Span sp(c, force_pass ? KC_KSCATTER_FORCE : KC_KSCATTER_FIELD);
-        HIPCHK(launch_kscatter(s, force_pass, c->cfg.precision, n, c->nc, (int32_t)c->h_rows.size(), c->rows.p,
-                               c->ek.p, c->sk.p, c->tab.p, c->mom.p, c->msum.p, c->c_force, c->c_flux, c->c_self,
-                               c->c_surf, force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
+        HIPCHK(launch_kscatter(s, force_pass, n, c->reps.p, c->rep_of.p, c->rows.p, c->ek.p, c->sk.p, c->tab.p,
+                               c->mom.p, c->msum.p, force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
     }
     return MDS_OK;
 }
 
 mds_status moments_of(mds_ctx* c, const double* mm_dev) {
     mds_ctx::Span sp(c, KC_MOMENTS);
-    const int nb = blocks_for(c->n);
-    HIPCHK(launch_moments(c->stream, c->n, mm_dev, c->mag_mom.p, c->inv_perm.p, c->mom.p, c->rec_mom.p,
-                          c->msum_partial.p, nb));
-    HIPCHK(launch_reduce_sum3(c->stream, c->msum_partial.p, nb, c->msum.p));
+    HIPCHK(launch_moments(c->stream, c->n, mm_dev, c->mag_mom.p, c->inv_perm.p, c->mom.p, c->rec_mom.p));
+    HIPCHK(launch_rep_sum3(c->stream, c->reps.p, c->nrep, c->n, c->mom.p, c->msum.p));
     return MDS_OK;
 }
 
-StageParams stage_params(mds_ctx* c, double h, double bext) {
+StageParams stage_params(mds_ctx* c, double h) {
     StageParams sp{};
     sp.n = c->n;
     sp.h = h;
-    sp.bext_z = bext;
     sp.anis_const = c->cfg.anis_const;
     sp.vel_mm_const = c->cfg.vel_mm_const;
     sp.mag_damp = c->cfg.mag_damp;
@@ -550,19 +764,26 @@ StageParams stage_params(mds_ctx* c, double h, double bext) {
     sp.c2 = c->c2.p;
     sp.mag_mom = c->mag_mom.p;
     sp.inv_perm = c->inv_perm.p;
+    sp.rep_t = c->rep_t.p;
+    sp.t_mag = c->cfg.t_mag;
+    sp.reps = c->reps.p;
+    sp.rep_of = c->rep_of.p;
     return sp;
 }
 
 mds_status check_ready(mds_ctx* c) {
     if (!c) return MDS_EINVAL;
     if (c->n <= 0) return fail(c, MDS_EINVAL, "particles not set");
-    if (!c->ewald_set) return fail(c, MDS_EINVAL, "Ewald parameters not set");
+    for (const RepMeta& m : c->meta)
+        if (!m.ewald_set) return fail(c, MDS_EINVAL, "Ewald parameters not set");
     if (!c->state_set) return fail(c, MDS_EINVAL, "state not set");
-    return MDS_OK;
+    HIPCHK(hipSetDevice(c->device));
+    return ensure_planned(c);
 }
 
 mds_status read_flags(mds_ctx* c) {
     int32_t f = 0, nev = 0;
+    if (!c->err_flag.p) return MDS_OK;
     HIPCHK(hipMemcpyAsync(&f, c->err_flag.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(&nev, c->n_events.p, sizeof(nev), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -572,48 +793,94 @@ mds_status read_flags(mds_ctx* c) {
     return MDS_OK;
 }
 
-// One RunSimulation iteration. Stages run back to back on the stream; the
-// adaptive controller reads the error norm (one sync per attempt).
-mds_status step_impl(mds_ctx* c, double* t, double* dt, double* dt_used, double* mz_out, bool sync) {
+// The device work of one step of size h after the force evaluation's inputs
+// are in place: noise, geometry, force pass, RK stages (one tableau pass, no
+// controller), translation, <m_z>, counters. Used for fixed-dt steps and for
+// the graph capture.
+mds_status fixed_step_body(mds_ctx* c, double h, bool check) {
     const int64_t n = c->n;
     const size_t n3 = 3 * (size_t)n;
     hipStream_t s = c->stream;
-    const double bext = (*t < c->cfg.t_mag) ? c->cfg.mag_flux_dens : 0.0;  // DefExtField
     if (c->philox) {
         mds_ctx::Span sp(c, KC_NOISE);
-        HIPCHK(launch_noise(s, n, c->seed, c->step_counter.p, 0, *dt, c->zeta_rot.p, c->vol_mag.p, c->zeta_trans.p,
-                            c->cfg.kB * c->cfg.temp, c->cfg.mag_damp, c->cfg.gyro_mr, c->cfg.sat_mag,
-                            c->cfg.enable_therm_torque, c->cfg.enable_therm_field, c->cfg.enable_therm_force,
-                            c->n_t.p, c->n_b.p, c->n_f.p));
+        HIPCHK(launch_noise(s, n, c->reps.p, c->rep_of.p, c->rep_step.p, h, c->zeta_rot.p, c->vol_mag.p,
+                            c->zeta_trans.p, c->cfg.kB * c->cfg.temp, c->cfg.mag_damp, c->cfg.gyro_mr,
+                            c->cfg.sat_mag, c->cfg.enable_therm_torque, c->cfg.enable_therm_field,
+                            c->cfg.enable_therm_force, c->n_t.p, c->n_b.p, c->n_f.p));
     }
-    mds_status st = prepare_geometry(c, sync);
+    mds_status st = prepare_geometry(c, check);
+    if (st != MDS_OK) return st;
+    if ((st = moments_of(c, c->mm.p)) != MDS_OK) return st;
+    if ((st = evaluate(c, true)) != MDS_OK) return st;
+    const Tableau tb = make_tableau(MDS_HEUN_EULER);
+    const int nb = blocks_for(n);
+    const StageParams sp = stage_params(c, h);
+    for (int sgi = 0; sgi < tb.stages; ++sgi) {
+        const double* ym = sgi == 0 ? c->mm.p : c->pt[(sgi - 1) & 1].p;
+        const double* ye = sgi == 0 ? c->ea.p : c->pt[(sgi - 1) & 1].p + n3;
+        double* yo = c->pt[sgi & 1].p;
+        if (sgi > 0) {
+            HIPCHK(launch_rep_sum3(s, c->reps.p, c->nrep, n, c->mom.p, c->msum.p));
+            if ((st = evaluate(c, false)) != MDS_OK) return st;
+        }
+        mds_ctx::Span spn(c, KC_STAGE);
+        const bool last = sgi + 1 == tb.stages;
+        HIPCHK(launch_stage(s, sp, sgi, tb.stages, c->mm.p, c->ea.p, ym, ye, c->field.p, c->n_t.p, c->n_b.p, c->kmm.p,
+                            c->kea.p, last ? nullptr : tb.a[sgi + 1], tb.b, tb.e, yo, yo + n3, c->mom.p, c->rec_mom.p,
+                            c->err_partial.p, c->new_mm.p, c->new_ea.p, nb));
+    }
+    {
+        mds_ctx::Span sp2(c, KC_TRANSLATE);
+        HIPCHK(launch_translate(s, n, c->reps.p, c->rep_of.p, h, c->pos_cur.p, c->force.p, c->zeta_trans.p, c->pos.p,
+                                c->new_mm.p, c->new_ea.p, c->mm.p, c->ea.p));
+        HIPCHK(launch_rep_mean_z(s, c->reps.p, c->nrep, n, c->mm.p, c->mz_out.p, c->series_counter.p, c->series.p,
+                                 kSeriesCap));
+        HIPCHK(launch_step_count(s, c->nrep, c->rep_step.p, c->rep_t.p, h, c->series_counter.p));
+    }
+    return MDS_OK;
+}
+
+// One RunSimulation iteration with the adaptive controller (single
+// replica). Stages run back to back on the stream; the controller reads the
+// error norm (one sync per attempt).
+mds_status adaptive_step(mds_ctx* c, double* t, double* dt, double* dt_used) {
+    const int64_t n = c->n;
+    const size_t n3 = 3 * (size_t)n;
+    hipStream_t s = c->stream;
+    if (c->philox) {
+        mds_ctx::Span sp(c, KC_NOISE);
+        HIPCHK(launch_noise(s, n, c->reps.p, c->rep_of.p, c->rep_step.p, *dt, c->zeta_rot.p, c->vol_mag.p,
+                            c->zeta_trans.p, c->cfg.kB * c->cfg.temp, c->cfg.mag_damp, c->cfg.gyro_mr,
+                            c->cfg.sat_mag, c->cfg.enable_therm_torque, c->cfg.enable_therm_field,
+                            c->cfg.enable_therm_force, c->n_t.p, c->n_b.p, c->n_f.p));
+    }
+    mds_status st = prepare_geometry(c, true);
     if (st != MDS_OK) return st;
     if ((st = moments_of(c, c->mm.p)) != MDS_OK) return st;
     if ((st = evaluate(c, true)) != MDS_OK) return st;
 
-    const Tableau tb = make_tableau(c->cfg.fixed_dt ? MDS_HEUN_EULER : c->cfg.solver);
+    const Tableau tb = make_tableau(c->cfg.solver);
     const int nb = blocks_for(n);
     bool no_failed = true;
     double h = *dt;
     int tries = 0;
     for (;;) {
         h = *dt;
-        const StageParams sp = stage_params(c, h, bext);
+        const StageParams sp = stage_params(c, h);
         for (int sgi = 0; sgi < tb.stages; ++sgi) {
             const double* ym = sgi == 0 ? c->mm.p : c->pt[(sgi - 1) & 1].p;
             const double* ye = sgi == 0 ? c->ea.p : c->pt[(sgi - 1) & 1].p + n3;
             double* yo = c->pt[sgi & 1].p;
             if (sgi > 0) {
-                HIPCHK(launch_reduce_sum3(s, c->msum_partial.p, nb, c->msum.p));
+                HIPCHK(launch_rep_sum3(s, c->reps.p, c->nrep, n, c->mom.p, c->msum.p));
                 if ((st = evaluate(c, false)) != MDS_OK) return st;
             }
             mds_ctx::Span spn(c, KC_STAGE);
             const bool last = sgi + 1 == tb.stages;
             HIPCHK(launch_stage(s, sp, sgi, tb.stages, c->mm.p, c->ea.p, ym, ye, c->field.p, c->n_t.p, c->n_b.p,
                                 c->kmm.p, c->kea.p, last ? nullptr : tb.a[sgi + 1], tb.b, tb.e, yo, yo + n3, c->mom.p,
-                                c->rec_mom.p, c->msum_partial.p, c->err_partial.p, c->new_mm.p, c->new_ea.p, nb));
+                                c->rec_mom.p, c->err_partial.p, c->new_mm.p, c->new_ea.p, nb));
         }
-        if (c->cfg.fixed_dt) break;
         // AdjustTimeStep (src/simulation.cpp:912-951)
         double err = 0;
         HIPCHK(launch_reduce_max(s, c->err_partial.p, nb, h, c->scal.p));
@@ -643,21 +910,239 @@ mds_status step_impl(mds_ctx* c, double* t, double* dt, double* dt_used, double*
     }
     {
         mds_ctx::Span sp(c, KC_TRANSLATE);
-        // accept + renormalised state -> mm/ea; IntegrationTrans + wrap; <m_z>
-        HIPCHK(launch_translate(s, n, h, c->L, c->pos_cur.p, c->force.p, c->zeta_trans.p, c->pos.p, c->new_mm.p,
-                                c->msum_partial.p, nb));
-        HIPCHK(hipMemcpyAsync(c->mm.p, c->new_mm.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(c->ea.p, c->new_ea.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHK(launch_reduce_mean(s, c->msum_partial.p, nb, 1.0 / (double)n, c->scal.p + 1, c->series_counter.p,
-                                  nullptr));
-        HIPCHK(launch_step_count(s, c->step_counter.p));
+        // accepted, renormalised state -> mm/ea; IntegrationTrans + wrap; <m_z>
+        HIPCHK(launch_translate(s, n, c->reps.p, c->rep_of.p, h, c->pos_cur.p, c->force.p, c->zeta_trans.p, c->pos.p,
+                                c->new_mm.p, c->new_ea.p, c->mm.p, c->ea.p));
+        HIPCHK(launch_rep_mean_z(s, c->reps.p, c->nrep, n, c->mm.p, c->mz_out.p, c->series_counter.p, c->series.p,
+                                 kSeriesCap));
+        HIPCHK(launch_step_count(s, c->nrep, c->rep_step.p, c->rep_t.p, h, c->series_counter.p));
     }
     *t += h;
     if (dt_used) *dt_used = h;
-    if (sync || mz_out) {
-        if ((st = read_flags(c)) != MDS_OK) return st;
-        if (mz_out) HIPCHK(hipMemcpy(mz_out, c->scal.p + 1, sizeof(double), hipMemcpyDeviceToHost));
+    return MDS_OK;
+}
+
+// Capture one fixed-dt step (no host synchronisation inside) into a graph.
+mds_status capture_step(mds_ctx* c, double h) {
+    c->drop_graph();
+    const bool timing = c->timing;
+    c->timing = false;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        c->timing = timing;
+        HIPCHK(e);
     }
+    mds_status st = fixed_step_body(c, h, false);
+    e = hipStreamEndCapture(c->stream, &graph);
+    c->timing = timing;
+    if (st != MDS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    HIPCHK(e);
+    e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        c->gexec = nullptr;
+        HIPCHK(e);
+    }
+    c->graph_h = h;
+    return MDS_OK;
+}
+
+mds_status set_rep_time(mds_ctx* c, int r, double t) {
+    HIPCHK(hipMemcpyAsync(c->rep_t.p + r, &t, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MDS_OK;
+}
+
+// Per-particle constants of m particles (InitParticleProps,
+// src/simulation.cpp:118-214; velEaConst1/2 at 209-210).
+struct Props {
+    std::vector<double> rh, vm, mu, zr, zt, c1, c2;
+};
+mds_status particle_props(mds_ctx* c, int64_t m, const double* r_mag, const double* r_hydr, Props& P) {
+    P.rh.resize(m);
+    P.vm.resize(m);
+    P.mu.resize(m);
+    P.zr.resize(m);
+    P.zt.resize(m);
+    P.c1.resize(m);
+    P.c2.resize(m);
+    mds_status st = mds_particle_props(&c->cfg, m, r_mag, P.rh.data(), P.vm.data(), P.mu.data(), P.zr.data(),
+                                       P.zt.data());
+    if (st != MDS_OK) return fail(c, st, "particle properties");
+    if (r_hydr) {
+        // hydrodynamic radii given: friction coefficients follow them
+        for (int64_t i = 0; i < m; ++i) {
+            P.rh[i] = r_hydr[i];
+            P.zr[i] = 8.0 * c->cfg.my_pi * c->cfg.vis * std::pow(P.rh[i], 3.0);
+            P.zt[i] = 6.0 * c->cfg.my_pi * c->cfg.vis * P.rh[i];
+        }
+    }
+    for (int64_t i = 0; i < m; ++i) {
+        P.c1[i] = 2.0 * c->cfg.anis_en * (P.vm[i] / P.zr[i]);
+        P.c2[i] = 1.0 / P.zr[i];
+    }
+    return MDS_OK;
+}
+
+// Rebuild the kept per-particle arrays for a new particle layout.
+mds_status relayout(mds_ctx* c, int64_t old_n, int64_t new_n, const std::vector<Seg>& segs) {
+    hipStream_t s = c->stream;
+    for (auto* b : {&c->r_mag, &c->mag_mom, &c->zeta_rot, &c->zeta_trans, &c->vol_mag, &c->c1, &c->c2})
+        HIPCHK(repack(*b, 1, old_n, new_n, segs, s));
+    for (auto* b : {&c->pos, &c->mm, &c->ea}) HIPCHK(repack(*b, 3, old_n, new_n, segs, s));
+    // injected noise does not survive a change of layout
+    for (auto* b : {&c->n_t, &c->n_b, &c->n_f}) {
+        HIPCHK(b->ensure(3 * (size_t)new_n));
+        HIPCHK(launch_fill_f64(s, b->p, 3 * new_n, 0.0));
+    }
+    return MDS_OK;
+}
+
+// Per-replica device values (time, step counter) for a new replica list.
+mds_status relayout_rep_values(mds_ctx* c, const std::vector<int>& keep, int added, uint64_t counter) {
+    const int old_nrep = (int)(c->rep_t.p ? c->rep_t.n : 0);
+    std::vector<double> t_old(old_nrep);
+    std::vector<uint64_t> s_old(old_nrep);
+    if (old_nrep > 0) {
+        HIPCHK(hipMemcpyAsync(t_old.data(), c->rep_t.p, old_nrep * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(s_old.data(), c->rep_step.p, old_nrep * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    std::vector<double> t_new;
+    std::vector<uint64_t> s_new;
+    for (int k : keep) {
+        t_new.push_back(t_old[k]);
+        s_new.push_back(s_old[k]);
+    }
+    for (int a = 0; a < added; ++a) {
+        t_new.push_back(0.0);
+        s_new.push_back(counter);
+    }
+    const size_t m = t_new.size();
+    c->rep_t.release();
+    c->rep_step.release();
+    // exact size: the allocation size records the replica count
+    HIPCHK(hipMalloc((void**)&c->rep_t.p, std::max<size_t>(1, m) * sizeof(double)));
+    c->rep_t.n = m;
+    HIPCHK(hipMalloc((void**)&c->rep_step.p, std::max<size_t>(1, m) * sizeof(uint64_t)));
+    c->rep_step.n = m;
+    if (m > 0) {
+        HIPCHK(hipMemcpy(c->rep_t.p, t_new.data(), m * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->rep_step.p, s_new.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice));
+    }
+    return MDS_OK;
+}
+
+void refresh_offsets(mds_ctx* c) {
+    int64_t off = 0;
+    for (RepMeta& m : c->meta) {
+        m.off = off;
+        off += m.n;
+    }
+    c->n = off;
+    c->nrep = (int32_t)c->meta.size();
+    c->planned = false;
+    c->geometry_ready = false;
+    c->drop_graph();
+}
+
+// Append a replica of m particles (state optional).
+// Append `count` replicas of ns[k] particles. Per-particle inputs cover all
+// added particles in order (column-major over them for the N x 3 state);
+// per-replica inputs have `count` entries (NULL: box from volFrac, the
+// configured field, seed 0).
+mds_status add_replicas(mds_ctx* c, int32_t count, const int64_t* ns, const double* r_mag, const double* r_hydr,
+                        const double* l_box, const double* field_z, const uint64_t* seeds, const double* pos,
+                        const double* pos_mm, const double* pos_ea, int32_t* first) {
+    if (count <= 0 || !ns || !r_mag) return fail(c, MDS_EINVAL, "bad particle arguments");
+    int64_t m = 0;
+    for (int32_t k = 0; k < count; ++k) {
+        if (ns[k] <= 0) return fail(c, MDS_EINVAL, "empty replica");
+        m += ns[k];
+    }
+    if (c->n + m > ((int64_t)1 << 25)) return fail(c, MDS_EINVAL, "at most 2^25 particles per context");
+    if (c->nrep + count > (1 << 20)) return fail(c, MDS_EINVAL, "too many replicas");
+    for (int64_t i = 0; i < m; ++i)
+        if (!(r_mag[i] > 0) || !std::isfinite(r_mag[i])) return fail(c, MDS_EINVAL, "radii must be positive");
+    if (pos || pos_mm || pos_ea) {
+        if (!(pos && pos_mm && pos_ea)) return fail(c, MDS_EINVAL, "state needs pos, pos_mm and pos_ea");
+        for (int64_t k = 0; k < 3 * m; ++k)
+            if (!std::isfinite(pos[k]) || !std::isfinite(pos_mm[k]) || !std::isfinite(pos_ea[k]))
+                return fail(c, MDS_ENONFINITE, "non-finite state");
+    }
+    HIPCHK(hipSetDevice(c->device));
+    Props P;
+    mds_status st = particle_props(c, m, r_mag, r_hydr, P);
+    if (st != MDS_OK) return st;
+    std::vector<RepMeta> added;
+    int64_t o = 0;
+    for (int32_t k = 0; k < count; ++k) {
+        const int64_t nk = ns[k];
+        RepMeta rm;
+        rm.n = nk;
+        rm.L = (l_box && l_box[k] > 0) ? l_box[k] : mds_box_length(&c->cfg, nk, P.rh.data() + o);
+        rm.field_z = field_z ? field_z[k] : c->cfg.mag_flux_dens;
+        rm.seed = seeds ? seeds[k] : 0;
+        rm.rmax = *std::max_element(r_mag + o, r_mag + o + nk);
+        rm.mref = *std::max_element(P.mu.begin() + o, P.mu.begin() + o + nk);
+        if (!(rm.mref > 0)) rm.mref = 1.0;
+        st = mds_cutoff_sr(&c->cfg, nk, r_mag + o, P.rh.data() + o, P.mu.data() + o, &rm.rc_sr);
+        if (st != MDS_OK) return fail(c, st, "short-range cut-off (InitCutOffSR)");
+        added.push_back(rm);
+        o += nk;
+    }
+    const int64_t old_n = c->n, new_n = old_n + m;
+    std::vector<Seg> segs{{0, 0, old_n}};
+    if ((st = relayout(c, old_n, new_n, segs)) != MDS_OK) return st;
+    std::vector<int> keep;
+    for (int r = 0; r < c->nrep; ++r) keep.push_back(r);
+    if ((st = relayout_rep_values(c, keep, count, 0)) != MDS_OK) return st;
+    const double* cols[7] = {r_mag, P.mu.data(), P.zr.data(), P.zt.data(), P.vm.data(), P.c1.data(), P.c2.data()};
+    DevBuf<double>* dst[7] = {&c->r_mag, &c->mag_mom, &c->zeta_rot, &c->zeta_trans, &c->vol_mag, &c->c1, &c->c2};
+    for (int k = 0; k < 7; ++k)
+        HIPCHK(hipMemcpyAsync(dst[k]->p + old_n, cols[k], m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (pos) {
+        if ((st = upload_cols(c, c->pos.p, new_n, old_n, m, pos)) != MDS_OK) return st;
+        if ((st = upload_cols(c, c->mm.p, new_n, old_n, m, pos_mm)) != MDS_OK) return st;
+        if ((st = upload_cols(c, c->ea.p, new_n, old_n, m, pos_ea)) != MDS_OK) return st;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->h_r_mag.insert(c->h_r_mag.end(), r_mag, r_mag + m);
+    c->h_mag_mom.insert(c->h_mag_mom.end(), P.mu.begin(), P.mu.end());
+    c->h_r_hydr.insert(c->h_r_hydr.end(), P.rh.begin(), P.rh.end());
+    if (first) *first = c->nrep;
+    c->meta.insert(c->meta.end(), added.begin(), added.end());
+    refresh_offsets(c);
+    return MDS_OK;
+}
+
+// r_c / L of replica m for the autotuner's candidate f: f itself for one
+// replica; with several, f is the value for a replica of the mean size and
+// each replica's is scaled by (N_mean / N)^(1/6), the optimum of a cost
+// ~ N rho r_c^3 (real space) + N (L / r_c)^3 (k space) at fixed accuracy,
+// kept in [0.1, 0.5].
+double rc_frac_of(const mds_ctx* c, double f, const RepMeta& m) {
+    if (c->nrep <= 1) return f;
+    const double mean = (double)c->n / (double)c->nrep;
+    return std::min(0.5, std::max(0.1, f * std::pow(mean / (double)m.n, 1.0 / 6.0)));
+}
+
+mds_status set_rep_ewald(mds_ctx* c, int r, double alpha, double r_cut, int32_t nc) {
+    RepMeta& m = c->meta[r];
+    if (!(alpha > 0) || !(r_cut > 0) || nc < 0 || nc > 64) return fail(c, MDS_EINVAL, "bad Ewald parameters");
+    if (r_cut > 0.5 * m.L * (1 + 1e-12)) return fail(c, MDS_EINVAL, "r_c above L/2 breaks the minimum image");
+    m.alpha = alpha;
+    m.rc = r_cut;
+    m.nc = nc;
+    m.ewald_set = true;
+    c->planned = false;
+    c->geometry_ready = false;
+    c->drop_graph();
     return MDS_OK;
 }
 
@@ -677,29 +1162,21 @@ mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out) {
     int cnt = 0;
     if (hipGetDeviceCount(&cnt) != hipSuccess || device < 0 || device >= cnt) return MDS_EDEVICE;
     if (hipSetDevice(device) != hipSuccess) return MDS_EDEVICE;
+    if (cfg->solver < 0 || cfg->solver > 2) return MDS_ECONFIG;
+    if (cfg->precision != MDS_PREC_FP64 && cfg->precision != MDS_PREC_MIXED) return MDS_EINVAL;
     mds_ctx* c = new mds_ctx();
     c->cfg = *cfg;
     c->device = device;
-    if (cfg->solver < 0 || cfg->solver > 2) {
-        delete c;
-        return MDS_ECONFIG;
-    }
-    if (cfg->precision != MDS_PREC_FP64 && cfg->precision != MDS_PREC_MIXED) {
-        delete c;
-        return MDS_EINVAL;
-    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return MDS_EDEVICE;
     }
-    if (c->step_counter.ensure(1) != hipSuccess || c->pair_counter.ensure(1) != hipSuccess ||
-        c->series_counter.ensure(1) != hipSuccess || c->scal.ensure(8) != hipSuccess ||
-        c->msum.ensure(3) != hipSuccess || c->n_events.ensure(1) != hipSuccess || c->err_flag.ensure(1) != hipSuccess ||
+    if (c->pair_counter.ensure(1) != hipSuccess || c->series_counter.ensure(1) != hipSuccess ||
+        c->scal.ensure(8) != hipSuccess || c->n_events.ensure(1) != hipSuccess || c->err_flag.ensure(1) != hipSuccess ||
         c->max_disp.ensure(1) != hipSuccess || c->nwork.ensure(1) != hipSuccess || c->n_moved.ensure(1) != hipSuccess) {
         delete c;
         return MDS_ENOMEM;
     }
-    (void)hipMemset(c->step_counter.p, 0, sizeof(uint64_t));
     (void)hipMemset(c->pair_counter.p, 0, sizeof(uint64_t));
     (void)hipMemset(c->series_counter.p, 0, sizeof(int32_t));
     (void)hipMemset(c->err_flag.p, 0, sizeof(int32_t));
@@ -718,99 +1195,53 @@ mds_status mds_set_particles(mds_ctx* c, int64_t n, const double* r_mag, const d
     if (!c || n <= 0 || !r_mag) return fail(c, MDS_EINVAL, "bad particle arguments");
     if (n > (int64_t)1 << 25) return fail(c, MDS_EINVAL, "at most 2^25 particles per context");
     HIPCHK(hipSetDevice(c->device));
-    std::vector<double> rh(n), vm(n), mu(n), zr(n), zt(n), c1(n), c2(n);
-    mds_status st = mds_particle_props(&c->cfg, n, r_mag, rh.data(), vm.data(), mu.data(), zr.data(), zt.data());
-    if (st != MDS_OK) return fail(c, st, "particle properties");
-    if (r_hydr) {
-        // hydrodynamic radii given: friction coefficients follow them
-        for (int64_t i = 0; i < n; ++i) {
-            rh[i] = r_hydr[i];
-            zr[i] = 8.0 * c->cfg.my_pi * c->cfg.vis * std::pow(rh[i], 3.0);
-            zt[i] = 6.0 * c->cfg.my_pi * c->cfg.vis * rh[i];
-        }
-    }
-    for (int64_t i = 0; i < n; ++i) {
-        c1[i] = 2.0 * c->cfg.anis_en * (vm[i] / zr[i]);  // velEaConst1, src/simulation.cpp:209
-        c2[i] = 1.0 / zr[i];                               // velEaConst2, 210
-    }
-    c->n = n;
-    c->L = l_box > 0 ? l_box : mds_box_length(&c->cfg, n, rh.data());
-    st = mds_cutoff_sr(&c->cfg, n, r_mag, rh.data(), mu.data(), &c->rc_sr);
-    if (st != MDS_OK) return fail(c, st, "short-range cut-off (InitCutOffSR)");
-    const size_t n3 = 3 * (size_t)n;
-    HIPCHK(c->r_mag.ensure(n));
-    HIPCHK(c->mag_mom.ensure(n));
-    HIPCHK(c->zeta_rot.ensure(n));
-    HIPCHK(c->zeta_trans.ensure(n));
-    HIPCHK(c->vol_mag.ensure(n));
-    HIPCHK(c->c1.ensure(n));
-    HIPCHK(c->c2.ensure(n));
-    for (auto* b : {&c->pos, &c->pos0, &c->pos_cur, &c->mm, &c->ea, &c->n_t, &c->n_b, &c->n_f, &c->force, &c->field,
-                    &c->mom, &c->new_mm, &c->new_ea})
-        HIPCHK(b->ensure(n3));
-    HIPCHK(c->pt[0].ensure(2 * n3));
-    HIPCHK(c->pt[1].ensure(2 * n3));
-    HIPCHK(c->kmm.ensure(7 * n3));
-    HIPCHK(c->kea.ensure(7 * n3));
-    HIPCHK(c->rec_pos.ensure(n));
-    HIPCHK(c->rec_pos32.ensure(n));
-    c->mref = *std::max_element(mu.begin(), mu.end());
-    c->h_mag_mom = mu;
-    c->h_r_hydr = rh;
-    HIPCHK(c->rec_mom.ensure(n));
-    for (auto* b : {&c->key_main, &c->key_fine, &c->sorted_main, &c->sorted_fine, &c->inv_perm, &c->cand_count,
-                    &c->ev_head, &c->ev_tail})
-        HIPCHK(b->ensure(n));
-    HIPCHK(c->cand_off.ensure(n + 1));
-    const int nb = blocks_for(n);
-    HIPCHK(c->msum_partial.ensure(3 * (size_t)nb));
-    HIPCHK(c->err_partial.ensure(nb));
-    HIPCHK(hipMemcpy(c->r_mag.p, r_mag, n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->mag_mom.p, mu.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->zeta_rot.p, zr.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->zeta_trans.p, zt.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->vol_mag.p, vm.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->c1.p, c1.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->c2.p, c2.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(launch_fill_f64(c->stream, c->n_t.p, n3, 0.0));
-    HIPCHK(launch_fill_f64(c->stream, c->n_b.p, n3, 0.0));
-    HIPCHK(launch_fill_f64(c->stream, c->n_f.p, n3, 0.0));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    c->ewald_set = false;
+    // a single replica: drop any previous ones
+    const uint64_t seed = c->meta.empty() ? 0 : c->meta[0].seed;
+    c->meta.clear();
+    c->h_r_mag.clear();
+    c->h_mag_mom.clear();
+    c->h_r_hydr.clear();
+    c->rep_t.release();
+    c->rep_step.release();
+    c->n = 0;
+    c->nrep = 0;
+    const double lb = l_box;
+    mds_status st = add_replicas(c, 1, &n, r_mag, r_hydr, &lb, nullptr, &seed, nullptr, nullptr, nullptr, nullptr);
+    if (st != MDS_OK) return st;
     c->state_set = false;
-    c->geometry_ready = false;
     return MDS_OK;
 }
 
 mds_status mds_set_ewald(mds_ctx* c, double alpha, double r_cut, int32_t nc) {
     if (!c || c->n <= 0) return fail(c, MDS_EINVAL, "set particles first");
-    if (!(alpha > 0) || !(r_cut > 0) || nc < 0 || nc > 64) return fail(c, MDS_EINVAL, "bad Ewald parameters");
-    if (r_cut > 0.5 * c->L * (1 + 1e-12)) return fail(c, MDS_EINVAL, "r_c above L/2 breaks the minimum image");
+    for (int r = 0; r < c->nrep; ++r) {
+        mds_status st = set_rep_ewald(c, r, alpha, r_cut, nc);
+        if (st != MDS_OK) return st;
+    }
     HIPCHK(hipSetDevice(c->device));
-    c->alpha = alpha;
-    c->rc = r_cut;
-    c->nc = nc;
-    mds_status st = plan_grids(c);
-    if (st != MDS_OK) return st;
-    if ((st = plan_kspace(c)) != MDS_OK) return st;
-    c->ewald_set = true;
-    c->geometry_ready = false;
-    return MDS_OK;
+    return ensure_planned(c);
 }
 
 mds_status mds_set_tiling(mds_ctx* c, int32_t cell_reach) {
-    if (!c || cell_reach < 1 || cell_reach > 8) return fail(c, MDS_EINVAL, "cell reach must be 1..8");
+    if (!c || cell_reach < 0 || cell_reach > 8) return fail(c, MDS_EINVAL, "cell reach must be 0..8");
     c->reach = cell_reach;
+    c->planned = false;
     c->geometry_ready = false;
-    if (c->ewald_set) return plan_grids(c);
+    c->drop_graph();
     return MDS_OK;
 }
 
 mds_status mds_get_tiling(const mds_ctx* c, int32_t* cell_reach, int32_t* ncell, int32_t* nstencil) {
     if (!c) return MDS_EINVAL;
     if (cell_reach) *cell_reach = c->reach;
-    if (ncell) *ncell = c->ncell;
-    if (nstencil) *nstencil = c->nsten;
+    if (!c->planned) {
+        // planned lazily: report the last plan if any
+        if (ncell) *ncell = c->h_rep.empty() ? 0 : c->h_rep[0].ncell;
+        if (nstencil) *nstencil = c->h_rep.empty() ? 0 : c->h_rep[0].nsten;
+        return MDS_OK;
+    }
+    if (ncell) *ncell = c->h_rep.empty() ? 0 : c->h_rep[0].ncell;
+    if (nstencil) *nstencil = c->h_rep.empty() ? 0 : c->h_rep[0].nsten;
     return MDS_OK;
 }
 
@@ -819,16 +1250,16 @@ mds_status mds_autotune(mds_ctx* c, const double* rc_fracs, int32_t n_rc, const 
     // CompEwaldParams (src/interaction.cpp:164-324) picks r_c in [0.2, 0.5] L
     // by timing the field sums on the CPU; here every candidate (r_c, cell
     // reach) is timed on the device over one force and one field evaluation,
-    // with alpha and k_c from the same error model, so each candidate has the
-    // accuracy the configuration asks for.
+    // with alpha and k_c of every replica from the same error model, so each
+    // candidate has the accuracy the configuration asks for.
     if (!c || c->n <= 0 || !c->state_set || !rc_fracs || n_rc <= 0 || !reaches || n_reach <= 0)
         return fail(c, MDS_EINVAL, "autotune needs particles, state and candidates");
     HIPCHK(hipSetDevice(c->device));
     if (reps < 1) reps = 1;
     double best = 1e300, brc = -1;
     int32_t breach = c->reach;
-    double b_alpha = 0;
-    int32_t b_nc = 0;
+    std::vector<double> b_alpha(c->nrep), cur_alpha(c->nrep);
+    std::vector<int32_t> b_nc(c->nrep), cur_nc(c->nrep);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -836,17 +1267,28 @@ mds_status mds_autotune(mds_ctx* c, const double* rc_fracs, int32_t n_rc, const 
     c->timing = false;
     mds_status st = MDS_OK;
     for (int a = 0; a < n_rc && st == MDS_OK; ++a) {
-        const double rc = rc_fracs[a] * c->L;
-        double alpha = 0, kc = 0;
-        if (mds_ewald_params_for_rc(&c->cfg, c->n, c->h_mag_mom.data(), c->h_r_hydr.data(), c->L, rc, &alpha, &kc) !=
-            MDS_OK)
-            continue;
-        const int32_t nc = (int32_t)std::floor(kc + 0.5);  // src/interaction.cpp:304
-        if (nc > 64 || rc > 0.5 * c->L) continue;
-        if ((st = mds_set_ewald(c, alpha, rc, nc)) != MDS_OK) break;
+        bool ok = true;
+        for (int r = 0; r < c->nrep && ok; ++r) {
+            const RepMeta& m = c->meta[r];
+            const double rc = rc_frac_of(c, rc_fracs[a], m) * m.L;
+            double alpha = 0, kc = 0;
+            if (mds_ewald_params_for_rc(&c->cfg, m.n, c->h_mag_mom.data() + m.off, c->h_r_hydr.data() + m.off, m.L,
+                                        rc, &alpha, &kc) != MDS_OK) {
+                ok = false;
+                break;
+            }
+            const int32_t nc = (int32_t)std::floor(kc + 0.5);  // src/interaction.cpp:304
+            if (nc > 64 || rc > 0.5 * m.L) ok = false;
+            cur_alpha[r] = alpha;
+            cur_nc[r] = nc;
+        }
+        if (!ok) continue;
+        for (int r = 0; r < c->nrep && st == MDS_OK; ++r)
+            st = set_rep_ewald(c, r, cur_alpha[r], rc_frac_of(c, rc_fracs[a], c->meta[r]) * c->meta[r].L, cur_nc[r]);
         for (int b = 0; b < n_reach && st == MDS_OK; ++b) {
             c->reach = reaches[b];
-            if ((st = plan_grids(c)) != MDS_OK) break;
+            c->planned = false;
+            if ((st = ensure_planned(c)) != MDS_OK) break;
             if ((st = prepare_geometry(c, true)) != MDS_OK) break;
             float best_ms_here = 1e30f;
             for (int r = 0; r < reps + 1 && st == MDS_OK; ++r) {
@@ -864,8 +1306,8 @@ mds_status mds_autotune(mds_ctx* c, const double* rc_fracs, int32_t n_rc, const 
                 best = best_ms_here;
                 brc = rc_fracs[a];
                 breach = reaches[b];
-                b_alpha = alpha;
-                b_nc = nc;
+                b_alpha = cur_alpha;
+                b_nc = cur_nc;
             }
         }
     }
@@ -875,22 +1317,25 @@ mds_status mds_autotune(mds_ctx* c, const double* rc_fracs, int32_t n_rc, const 
     if (st != MDS_OK) return st;
     if (brc < 0) return fail(c, MDS_ENOCONV, "no admissible Ewald candidate");
     c->reach = breach;
-    if ((st = mds_set_ewald(c, b_alpha, brc * c->L, b_nc)) != MDS_OK) return st;
+    for (int r = 0; r < c->nrep; ++r)
+        if ((st = set_rep_ewald(c, r, b_alpha[r], rc_frac_of(c, brc, c->meta[r]) * c->meta[r].L, b_nc[r])) != MDS_OK)
+            return st;
+    if ((st = ensure_planned(c)) != MDS_OK) return st;
     if (best_rc_frac) *best_rc_frac = brc;
     if (best_reach) *best_reach = breach;
     if (best_ms) *best_ms = best;
-    c->geometry_ready = false;
     return MDS_OK;
 }
 
 mds_status mds_get_params(const mds_ctx* c, double* l_box, double* r_cut_sr, double* alpha, double* r_cut,
                           int32_t* nc) {
-    if (!c) return MDS_EINVAL;
-    if (l_box) *l_box = c->L;
-    if (r_cut_sr) *r_cut_sr = c->rc_sr;
-    if (alpha) *alpha = c->alpha;
-    if (r_cut) *r_cut = c->rc;
-    if (nc) *nc = c->nc;
+    if (!c || c->meta.empty()) return MDS_EINVAL;
+    const RepMeta& m = c->meta[0];
+    if (l_box) *l_box = m.L;
+    if (r_cut_sr) *r_cut_sr = m.rc_sr;
+    if (alpha) *alpha = m.alpha;
+    if (r_cut) *r_cut = m.rc;
+    if (nc) *nc = m.nc;
     return MDS_OK;
 }
 
@@ -900,6 +1345,7 @@ mds_status mds_set_state(mds_ctx* c, const double* pos, const double* pos_mm, co
     for (size_t k = 0; k < n3; ++k)
         if (!std::isfinite(pos[k]) || !std::isfinite(pos_mm[k]) || !std::isfinite(pos_ea[k]))
             return fail(c, MDS_ENONFINITE, "non-finite state");
+    HIPCHK(hipSetDevice(c->device));
     mds_status st;
     if ((st = upload(c, c->pos.p, pos, n3)) != MDS_OK) return st;
     if ((st = upload(c, c->mm.p, pos_mm, n3)) != MDS_OK) return st;
@@ -912,6 +1358,7 @@ mds_status mds_set_state(mds_ctx* c, const double* pos, const double* pos_mm, co
 
 mds_status mds_get_state(mds_ctx* c, double* pos, double* pos_mm, double* pos_ea) {
     if (!c || c->n <= 0) return MDS_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
     const size_t n3 = 3 * (size_t)c->n;
     mds_status st;
     if (pos && (st = download(c, pos, c->pos.p, n3)) != MDS_OK) return st;
@@ -922,6 +1369,7 @@ mds_status mds_get_state(mds_ctx* c, double* pos, double* pos_mm, double* pos_ea
 
 mds_status mds_set_noise(mds_ctx* c, const double* tt, const double* tb, const double* tf) {
     if (!c || c->n <= 0) return MDS_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
     const size_t n3 = 3 * (size_t)c->n;
     mds_status st;
     double* dst[3] = {c->n_t.p, c->n_b.p, c->n_f.p};
@@ -935,21 +1383,27 @@ mds_status mds_set_noise(mds_ctx* c, const double* tt, const double* tb, const d
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->philox = false;
+    c->drop_graph();
     return MDS_OK;
 }
 
 mds_status mds_use_philox(mds_ctx* c, uint64_t seed, uint64_t counter) {
-    if (!c) return MDS_EINVAL;
+    if (!c || c->nrep <= 0) return fail(c, MDS_EINVAL, "set particles first");
+    HIPCHK(hipSetDevice(c->device));
     c->philox = true;
-    c->seed = seed;
-    HIPCHK(hipMemcpy(c->step_counter.p, &counter, sizeof(counter), hipMemcpyHostToDevice));
+    if (c->nrep == 1) {
+        c->meta[0].seed = seed;
+        c->planned = false;
+    }
+    std::vector<uint64_t> v(c->nrep, counter);
+    HIPCHK(hipMemcpy(c->rep_step.p, v.data(), v.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    c->drop_graph();
     return MDS_OK;
 }
 
 mds_status mds_comp_interactions(mds_ctx* c, double* force, double* field, double* press, double* pos_after) {
     mds_status st = check_ready(c);
     if (st != MDS_OK) return st;
-    HIPCHK(hipSetDevice(c->device));
     if ((st = prepare_geometry(c, true)) != MDS_OK) return st;
     if ((st = moments_of(c, c->mm.p)) != MDS_OK) return st;
     if ((st = evaluate(c, true)) != MDS_OK) return st;
@@ -959,6 +1413,7 @@ mds_status mds_comp_interactions(mds_ctx* c, double* force, double* field, doubl
     if (field && (st = download(c, field, c->field.p, n3)) != MDS_OK) return st;
     if (pos_after && (st = download(c, pos_after, c->pos_cur.p, n3)) != MDS_OK) return st;
     if (press) {
+        // pressure sums over all replicas (one replica: the reference's)
         std::vector<double> pp(3 * (size_t)c->work_cap);
         int32_t nw = 0;
         HIPCHK(hipMemcpy(&nw, c->nwork.p, sizeof(nw), hipMemcpyDeviceToHost));
@@ -966,9 +1421,11 @@ mds_status mds_comp_interactions(mds_ctx* c, double* force, double* field, doubl
         press[0] = press[1] = press[2] = 0;
         for (int32_t w = 0; w < nw; ++w)
             for (int d = 0; d < 3; ++d) press[d] += pp[3 * (size_t)w + d];
-        double pm[3];
-        HIPCHK(hipMemcpy(pm, c->press_partial.p + 3 * (size_t)c->work_cap, sizeof(pm), hipMemcpyDeviceToHost));
-        for (int d = 0; d < 3; ++d) press[d] += pm[d];
+        std::vector<double> pm(3 * (size_t)c->nrep);
+        HIPCHK(hipMemcpy(pm.data(), c->press_partial.p + 3 * (size_t)c->work_cap, pm.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+        for (int32_t r = 0; r < c->nrep; ++r)
+            for (int d = 0; d < 3; ++d) press[d] += pm[3 * (size_t)r + d];
     }
     return MDS_OK;
 }
@@ -988,20 +1445,34 @@ mds_status mds_comp_demag_field(mds_ctx* c, const double* pos_mm, double* field)
 mds_status mds_rk_coeffs(mds_ctx* c, double t, const double* pos_mm, const double* pos_ea, const double* field,
                          double* k_mm, double* k_ea) {
     if (!c || c->n <= 0 || !pos_mm || !pos_ea || !field) return MDS_EINVAL;
+    if (c->nrep != 1) return fail(c, MDS_EINVAL, "mds_rk_coeffs takes a single replica");
+    HIPCHK(hipSetDevice(c->device));
+    mds_status st = size_scratch(c);
+    if (st != MDS_OK) return st;
+    if (!c->planned) {
+        // the stage kernel needs the replica table (field, time), not the plan
+        Rep R{};
+        R.off = 0;
+        R.n = c->n;
+        R.L = c->meta[0].L;
+        R.field_z = c->meta[0].field_z;
+        HIPCHK(c->reps.ensure(1));
+        HIPCHK(c->rep_of.ensure(c->n));
+        HIPCHK(hipMemcpy(c->reps.p, &R, sizeof(Rep), hipMemcpyHostToDevice));
+        HIPCHK(launch_fill_i32(c->stream, c->rep_of.p, c->n, 0));
+    }
     const int64_t n = c->n;
     const size_t n3 = 3 * (size_t)n;
-    mds_status st;
+    if ((st = set_rep_time(c, 0, t)) != MDS_OK) return st;
     if ((st = upload(c, c->pt[0].p, pos_mm, n3)) != MDS_OK) return st;
     if ((st = upload(c, c->pt[0].p + n3, pos_ea, n3)) != MDS_OK) return st;
     if ((st = upload(c, c->field.p, field, n3)) != MDS_OK) return st;
-    const double bext = (t < c->cfg.t_mag) ? c->cfg.mag_flux_dens : 0.0;
-    const StageParams sp = stage_params(c, 0.0, bext);
+    const StageParams sp = stage_params(c, 0.0);
     const double zero[8] = {0};
     // a stage of a one-stage tableau: k only (the accepted-state outputs go to scratch)
     HIPCHK(launch_stage(c->stream, sp, 0, 1, c->pt[0].p, c->pt[0].p + n3, c->pt[0].p, c->pt[0].p + n3, c->field.p,
                         c->n_t.p, c->n_b.p, c->kmm.p, c->kea.p, nullptr, zero, zero, c->pt[1].p, c->pt[1].p + n3,
-                        c->mom.p, c->rec_mom.p, c->msum_partial.p, c->err_partial.p, c->new_mm.p, c->new_ea.p,
-                        blocks_for(n)));
+                        c->mom.p, c->rec_mom.p, c->err_partial.p, c->new_mm.p, c->new_ea.p, blocks_for(n)));
     if (k_mm && (st = download(c, k_mm, c->kmm.p, n3)) != MDS_OK) return st;
     if (k_ea && (st = download(c, k_ea, c->kea.p, n3)) != MDS_OK) return st;
     return MDS_OK;
@@ -1011,22 +1482,62 @@ mds_status mds_step(mds_ctx* c, double* t, double* dt, double* dt_used, double* 
     mds_status st = check_ready(c);
     if (st != MDS_OK) return st;
     if (!t || !dt) return MDS_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
-    return step_impl(c, t, dt, dt_used, mz, true);
+    if (c->nrep == 1 && (st = set_rep_time(c, 0, *t)) != MDS_OK) return st;
+    if (c->cfg.fixed_dt) {
+        HIPCHK(launch_fill_i32(c->stream, c->series_counter.p, 1, 0));
+        if ((st = fixed_step_body(c, *dt, true)) != MDS_OK) return st;
+        *t += *dt;
+        if (dt_used) *dt_used = *dt;
+    } else {
+        if (c->nrep != 1) return fail(c, MDS_EINVAL, "adaptive time stepping takes a single replica");
+        if ((st = adaptive_step(c, t, dt, dt_used)) != MDS_OK) return st;
+    }
+    if ((st = read_flags(c)) != MDS_OK) return st;
+    if (mz) HIPCHK(hipMemcpy(mz, c->mz_out.p, sizeof(double), hipMemcpyDeviceToHost));
+    return MDS_OK;
 }
 
 mds_status mds_run(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz_series) {
     mds_status st = check_ready(c);
     if (st != MDS_OK) return st;
     if (!t || !dt || steps < 0) return MDS_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
-    for (int64_t k = 0; k < steps; ++k) {
-        double mz = 0;
-        const bool want = mz_series != nullptr;
-        st = step_impl(c, t, dt, nullptr, want ? &mz : nullptr, k == 0);
-        if (st != MDS_OK) return st;
-        if (want) mz_series[k] = mz;
+    if (steps == 0) return MDS_OK;
+    if (c->nrep == 1 && (st = set_rep_time(c, 0, *t)) != MDS_OK) return st;
+    const int32_t R = c->nrep;
+    if (!c->cfg.fixed_dt) {
+        if (R != 1) return fail(c, MDS_EINVAL, "adaptive time stepping takes a single replica");
+        for (int64_t k = 0; k < steps; ++k) {
+            if ((st = adaptive_step(c, t, dt, nullptr)) != MDS_OK) return st;
+            if (mz_series) HIPCHK(hipMemcpy(mz_series + k, c->mz_out.p, sizeof(double), hipMemcpyDeviceToHost));
+        }
+        return read_flags(c);
     }
+    const double h = *dt;
+    // steps in chunks of the device <m_z> buffer; the first step checks the
+    // overlap margin (and clears the flags), later ones replay a graph when enabled
+    int64_t done = 0;
+    while (done < steps) {
+        const int64_t m = std::min<int64_t>(kSeriesCap, steps - done);
+        HIPCHK(launch_fill_i32(c->stream, c->series_counter.p, 1, 0));
+        for (int64_t k = 0; k < m; ++k) {
+            const bool first = done + k == 0;
+            if (first || !c->use_graph) {
+                if ((st = fixed_step_body(c, h, first)) != MDS_OK) return st;
+            } else {
+                if (!c->gexec || c->graph_h != h) {
+                    if ((st = capture_step(c, h)) != MDS_OK) return st;
+                }
+                HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+            }
+        }
+        if (mz_series) {
+            HIPCHK(hipMemcpyAsync(mz_series + done * R, c->series.p, (size_t)m * R * sizeof(double),
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        done += m;
+    }
+    for (int64_t k = 0; k < steps; ++k) *t += h;  // as tInt accumulates (src/simulation.cpp:518)
     return read_flags(c);
 }
 
@@ -1046,6 +1557,7 @@ mds_status mds_enable_timing(mds_ctx* c, int32_t on) {
         c->total_ms[k] = 0;
     }
     if (c->pair_counter.p) HIPCHK(hipMemset(c->pair_counter.p, 0, sizeof(uint64_t)));
+    c->drop_graph();
     return MDS_OK;
 }
 
@@ -1070,9 +1582,132 @@ mds_status mds_work_counts(mds_ctx* c, int64_t* pairs, int64_t* candidates, int6
     if (c->n_events.p) HIPCHK(hipMemcpy(&nev, c->n_events.p, sizeof(nev), hipMemcpyDeviceToHost));
     if (pairs) *pairs = (int64_t)p;
     if (candidates) *candidates = 0;
-    if (kvecs) *kvecs = c->nk;
+    if (kvecs) *kvecs = c->nk_all;
     if (overlap_events) *overlap_events = nev / 2;
     return MDS_OK;
+}
+
+// ------------------------------------------------------------------ replicas
+mds_status mds_add_replica(mds_ctx* c, int64_t n, const double* r_mag, const double* r_hydr, double l_box,
+                           double field_z, uint64_t seed, const double* pos, const double* pos_mm,
+                           const double* pos_ea, int32_t* index) {
+    if (!c) return MDS_EINVAL;
+    const bool had_state = c->state_set || c->nrep == 0;
+    mds_status st = add_replicas(c, 1, &n, r_mag, r_hydr, &l_box, &field_z, &seed, pos, pos_mm, pos_ea, index);
+    if (st != MDS_OK) return st;
+    c->state_set = had_state && pos != nullptr;
+    return MDS_OK;
+}
+
+mds_status mds_add_replicas(mds_ctx* c, int32_t count, const int64_t* n, const double* r_mag, const double* r_hydr,
+                            const double* l_box, const double* field_z, const uint64_t* seeds, const double* pos,
+                            const double* pos_mm, const double* pos_ea, int32_t* first_index) {
+    if (!c) return MDS_EINVAL;
+    const bool had_state = c->state_set || c->nrep == 0;
+    mds_status st = add_replicas(c, count, n, r_mag, r_hydr, l_box, field_z, seeds, pos, pos_mm, pos_ea, first_index);
+    if (st != MDS_OK) return st;
+    c->state_set = had_state && pos != nullptr;
+    return MDS_OK;
+}
+
+mds_status mds_remove_replica(mds_ctx* c, int32_t index) {
+    if (!c || index < 0 || index >= c->nrep) return fail(c, MDS_EINVAL, "no such replica");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const RepMeta gone = c->meta[index];
+    const int64_t old_n = c->n, new_n = old_n - gone.n;
+    std::vector<Seg> segs{{0, 0, gone.off}, {gone.off + gone.n, gone.off, old_n - gone.off - gone.n}};
+    mds_status st = relayout(c, old_n, new_n, segs);
+    if (st != MDS_OK) return st;
+    std::vector<int> keep;
+    for (int r = 0; r < c->nrep; ++r)
+        if (r != index) keep.push_back(r);
+    if ((st = relayout_rep_values(c, keep, 0, 0)) != MDS_OK) return st;
+    auto cut = [&](std::vector<double>& v) { v.erase(v.begin() + gone.off, v.begin() + gone.off + gone.n); };
+    cut(c->h_r_mag);
+    cut(c->h_mag_mom);
+    cut(c->h_r_hydr);
+    c->meta.erase(c->meta.begin() + index);
+    refresh_offsets(c);
+    if (c->nrep == 0) c->state_set = false;
+    return MDS_OK;
+}
+
+int32_t mds_replica_count(const mds_ctx* c) { return c ? c->nrep : 0; }
+
+mds_status mds_replica_set_ewald(mds_ctx* c, int32_t index, double alpha, double r_cut, int32_t nc) {
+    if (!c || index < 0 || index >= c->nrep) return fail(c, MDS_EINVAL, "no such replica");
+    return set_rep_ewald(c, index, alpha, r_cut, nc);
+}
+
+mds_status mds_replica_info(mds_ctx* c, int32_t index, int64_t* n, int64_t* offset, double* l_box, double* alpha,
+                            double* r_cut, int32_t* nc, double* t, uint64_t* steps) {
+    if (!c || index < 0 || index >= c->nrep) return fail(c, MDS_EINVAL, "no such replica");
+    const RepMeta& m = c->meta[index];
+    if (n) *n = m.n;
+    if (offset) *offset = m.off;
+    if (l_box) *l_box = m.L;
+    if (alpha) *alpha = m.alpha;
+    if (r_cut) *r_cut = m.rc;
+    if (nc) *nc = m.nc;
+    if (t || steps) {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (t) HIPCHK(hipMemcpy(t, c->rep_t.p + index, sizeof(double), hipMemcpyDeviceToHost));
+        if (steps) HIPCHK(hipMemcpy(steps, c->rep_step.p + index, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    }
+    return MDS_OK;
+}
+
+mds_status mds_replica_get_state(mds_ctx* c, int32_t index, double* pos, double* pos_mm, double* pos_ea) {
+    if (!c || index < 0 || index >= c->nrep) return fail(c, MDS_EINVAL, "no such replica");
+    HIPCHK(hipSetDevice(c->device));
+    const RepMeta& m = c->meta[index];
+    mds_status st;
+    if (pos && (st = download_cols(c, pos, c->pos.p, c->n, m.off, m.n)) != MDS_OK) return st;
+    if (pos_mm && (st = download_cols(c, pos_mm, c->mm.p, c->n, m.off, m.n)) != MDS_OK) return st;
+    if (pos_ea && (st = download_cols(c, pos_ea, c->ea.p, c->n, m.off, m.n)) != MDS_OK) return st;
+    return MDS_OK;
+}
+
+mds_status mds_replica_mz(mds_ctx* c, double* mz) {
+    if (!c || !mz || c->nrep <= 0) return MDS_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->mz_out.p) return fail(c, MDS_EINVAL, "no step taken");
+    HIPCHK(hipMemcpyAsync(mz, c->mz_out.p, c->nrep * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MDS_OK;
+}
+
+mds_status mds_set_graph(mds_ctx* c, int32_t on) {
+    if (!c) return MDS_EINVAL;
+    c->use_graph = on != 0;
+    c->drop_graph();
+    return MDS_OK;
+}
+
+int64_t mds_device_bytes(const mds_ctx* c) {
+    if (!c) return 0;
+    int64_t b = 0;
+    auto add = [&](size_t cnt, size_t sz) { b += (int64_t)(cnt * sz); };
+    for (auto* d : {&c->r_mag, &c->mag_mom, &c->zeta_rot, &c->zeta_trans, &c->vol_mag, &c->c1, &c->c2, &c->pos, &c->mm,
+                    &c->ea, &c->n_t, &c->n_b, &c->n_f, &c->pos0, &c->pos_cur, &c->force, &c->field, &c->mom, &c->kmm,
+                    &c->kea, &c->pt[0], &c->pt[1], &c->new_mm, &c->new_ea, &c->ev_pos, &c->ek, &c->sk, &c->kpartial,
+                    &c->tab, &c->press_partial, &c->series, &c->msum, &c->mz_out, &c->rep_t, &c->err_partial})
+        add(d->n, sizeof(double));
+    for (auto* d : {&c->rep_of, &c->cell_rep, &c->key_main, &c->key_fine, &c->cnt_main, &c->start_main, &c->end_main,
+                    &c->cur_main, &c->sorted_main, &c->inv_perm, &c->work, &c->cnt_fine, &c->start_fine, &c->end_fine,
+                    &c->cur_fine, &c->sorted_fine, &c->cand_count, &c->cand_off, &c->cand_j, &c->ev_head, &c->ev_tail,
+                    &c->ev_next, &c->moved_list})
+        add(d->n, sizeof(int32_t));
+    add(c->rec_pos.n + c->rec_mom.n, sizeof(double4));
+    add(c->rec_pos32.n, sizeof(float4));
+    add(c->ev_key.n + c->rep_step.n, sizeof(uint64_t));
+    add(c->stencil.n, sizeof(int4));
+    add(c->rows.n, sizeof(KRow));
+    add(c->segs.n, sizeof(KSeg));
+    add(c->reps.n, sizeof(Rep));
+    return b;
 }
 
 }  // extern "C"

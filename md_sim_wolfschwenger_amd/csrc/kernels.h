@@ -20,10 +20,31 @@ struct KSeg {
     int32_t ny, nz, alo, ahi, koff, nxmax;
 };
 
+// Parameters of one replica (an independent periodic system) in a context
+// that holds several; particles, cells, k vectors and tables of all replicas
+// are concatenated and every kernel finds its replica through this table.
+struct Rep {
+    int64_t off, n;              // particle range [off, off + n)
+    double L, cell, fcell, margin;
+    int32_t ncell, fncell, cell_off, fcell_off;  // grids; global cell id = cell_off + local
+    int32_t sten_off, nsten;     // slice of the stencil list
+    double alpha, alpha2, alpha4, rc, rc_sr;
+    double ell, inv_ell, inv_mref, bscale, fscale, pscale;  // units of the FP32 pair path
+    float al_s, al2_s, al4_s, pad_;
+    int32_t nc, row_off, nrows, seg_off, nseg, k_off, nk, kchunk, nchunks, kb_off, nsegw, pad2_;
+    int64_t tab_off, part_off;   // offsets (doubles) into the sin/cos tables and S(k) partials
+    double c_force, c_flux, c_self, c_surf, two_pi_over_L;
+    double field_z;              // magFluxDens of this replica
+    uint64_t seed;               // Philox key of this replica
+};
+
 // Per-step geometry of the real-space sum (cell list + overlap versions).
 struct Geometry {
-    int64_t n;
-    double L;
+    int64_t n;                   // all particles of all replicas
+    double L;                    // set per replica inside the kernels
+    const Rep* reps;
+    const int32_t* rep_of;       // [n] replica of a particle
+    const int32_t* cell_rep;     // [cells] replica of a main-grid cell
     // main cell grid
     int32_t ncell;        // cells per dimension
     int32_t reach;        // stencil half-width in cells
@@ -66,84 +87,94 @@ struct PairParams {
     const double* r_mag;   // original order
 };
 
-// Launch a kernel and check the launch.
-hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* pos0, double* pos_cur,
-                          double L, int32_t ncell, double cell, int32_t* key_main, int32_t fncell, double fcell,
-                          int32_t* key_fine);
+// Launchers: each launches its kernel(s) on stream s and returns the launch status.
 hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v);
 hipError_t launch_fill_f64(hipStream_t s, double* p, int64_t count, double v);
+hipError_t launch_scale(hipStream_t s, double* p, int64_t count, double f);
+
+// step setup: step-start positions and cell keys (global cell ids)
+hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* pos0, double* pos_cur, const Rep* reps,
+                          const int32_t* rep_of, int32_t* key_main, int32_t* key_fine);
 hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
-                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp);
+                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx);
 hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_idx, const double* pos0,
                                 const double* r_mag, double4* rec_pos, float4* rec_pos32, int32_t* inv_perm);
 hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
                              int32_t* work, int32_t* nwork, int32_t cap, int32_t* err);
 // contact search + sequential overlap resolution
-hipError_t launch_contacts(hipStream_t s, int64_t n, double L, const double* pos0, const double* r_mag,
-                           int32_t fncell, double fcell, const int32_t* fstart, const int32_t* fend,
-                           const int32_t* fsorted, double search_extra, int32_t* cand_count, int32_t* cand_off,
-                           int32_t* cand_j, int32_t cand_cap, int32_t* scan_tmp, int32_t* err);
-hipError_t launch_resolve(hipStream_t s, int64_t n, double L, double rc, const double* r_mag, const int32_t* cand_off,
-                          const int32_t* cand_j, double* pos_cur, const double* pos0, int32_t* ev_head,
-                          int32_t* ev_tail, int32_t* ev_next, uint64_t* ev_key, double* ev_pos, int32_t ev_cap,
-                          int32_t* n_events, double* max_disp, double disp_limit, int32_t* moved_list,
+hipError_t launch_contacts(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
+                           const double* r_mag, const int32_t* fstart, const int32_t* fend, const int32_t* fsorted,
+                           int32_t* cand_count, int32_t* cand_off, int32_t* cand_j, int32_t cand_cap, int32_t* err);
+// one thread per replica walks its candidates in order
+hipError_t launch_resolve(hipStream_t s, int32_t nrep, int64_t n, const Rep* reps, const double* r_mag,
+                          const int32_t* cand_off, const int32_t* cand_j, double* pos_cur, const double* pos0,
+                          int32_t* ev_head, int32_t* ev_tail, int32_t* ev_next, uint64_t* ev_key, double* ev_pos,
+                          int32_t ev_cap, int32_t* n_events, double* max_disp, double disp_limit, int32_t* moved_list,
                           int32_t* n_moved, int32_t* err);
-hipError_t launch_clear_events(hipStream_t s, int64_t n, int32_t* ev_head, int32_t* ev_tail);
+// clears the per-particle event lists and the event/moved counters
+hipError_t launch_clear_events(hipStream_t s, int64_t n, int32_t* ev_head, int32_t* ev_tail, int32_t* n_events,
+                               int32_t* n_moved, double* max_disp);
 
+// moments m_i = mm_i * mu_i (original and sorted order); per-replica sums
 hipError_t launch_moments(hipStream_t s, int64_t n, const double* mm, const double* mag_mom, const int32_t* inv_perm,
-                          double* mom, double4* rec_mom, double* msum_partial, int32_t nblk);
-hipError_t launch_reduce_sum3(hipStream_t s, const double* partial, int32_t count, double* out);
+                          double* mom, double4* rec_mom);
+hipError_t launch_rep_sum3(hipStream_t s, const Rep* reps, int32_t nrep, int64_t n, const double* v, double* out);
+hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, int64_t n, const double* v, double* out,
+                             const int32_t* counter, double* series, int32_t series_cap);
 
+// real space: one block per work item (max_work blocks, surplus blocks exit)
 hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
                         const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
-                        int32_t nsten, int64_t* cnt_pairs);
-// Pairs with an endpoint moved by the overlap correction (after launch_pairs).
+                        int64_t* cnt_pairs);
+// Pairs with an endpoint moved by the overlap correction (after launch_pairs);
+// pressure goes to press_partial slot press_off + replica.
 hipError_t launch_pairs_moved(hipStream_t s, bool force, const Geometry& g, const double4* rec_mom,
-                              const PairParams& pp, const PairOut& out, const int4* stencil, int32_t nsten,
-                              int32_t press_off, int64_t* cnt_pairs);
+                              const PairParams& pp, const PairOut& out, const int4* stencil, int32_t press_off,
+                              int64_t* cnt_pairs);
 
-// k-space
-hipError_t launch_sincos_tables(hipStream_t s, int64_t n, int32_t nc, double L, double two_pi, const double* pos0,
+// k space
+hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
                                 double* tab);
-hipError_t launch_kbuild(hipStream_t s, int32_t precision, int64_t n, int32_t nc, int32_t nseg, const KSeg* segs,
-                         const double* tab, const double* mom, int32_t chunk, int32_t nchunks, int32_t nk,
-                         double* partial);
-hipError_t launch_kreduce(hipStream_t s, int32_t nk, int32_t nchunks, const double* partial, double* sk);
-hipError_t launch_kscatter(hipStream_t s, bool force, int32_t precision, int64_t n, int32_t nc, int32_t nrows,
-                           const KRow* rows, const double* ek, const double* sk, const double* tab,
-                           const double* mom, const double* msum, double c_force, double c_flux, double c_self,
-                           double c_surf, const double* add_force, double* force_out, double* field);
+hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t kb_items, const KSeg* segs,
+                         const double* tab, const double* mom, double* partial);
+hipError_t launch_kreduce(hipStream_t s, int32_t nk_all, const Rep* reps, int32_t nrep, const double* partial,
+                          double* sk);
+hipError_t launch_kscatter(hipStream_t s, bool force, int64_t n, const Rep* reps, const int32_t* rep_of,
+                           const KRow* rows, const double* ek, const double* sk, const double* tab, const double* mom,
+                           const double* msum, const double* add_force, double* force_out, double* field);
 
 // integrator
 struct StageParams {
     int64_t n;
     double h;            // step
-    double bext_z;
     double anis_const, vel_mm_const, mag_damp;
     int32_t mobil;
     const double* c1;    // velEaConst1
     const double* c2;    // velEaConst2
     const double* mag_mom;
     const int32_t* inv_perm;
+    // applied field: replica r has field_z while its time rep_t[r] < t_mag
+    const double* rep_t;
+    double t_mag;
+    const Rep* reps;
+    const int32_t* rep_of;
 };
 // k_s = RHS(y_s); then the next stage point (coef of length s+1) or, when
 // last, the error norm partials and the normalised accepted state.
-hipError_t launch_stage(hipStream_t s, const StageParams& sp, int32_t stage, int32_t stages,
-                        const double* mm0, const double* ea0, const double* ymm, const double* yea,
-                        const double* field, const double* n_t, const double* n_b, double* kmm, double* kea,
-                        const double* a_next, const double* b_acc, const double* e_err, double* ymm_out,
-                        double* yea_out, double* mom, double4* rec_mom, double* msum_partial, double* err_partial,
-                        double* new_mm, double* new_ea, int32_t nblk);
+hipError_t launch_stage(hipStream_t s, const StageParams& sp, int32_t stage, int32_t stages, const double* mm0,
+                        const double* ea0, const double* ymm, const double* yea, const double* field,
+                        const double* n_t, const double* n_b, double* kmm, double* kea, const double* a_next,
+                        const double* b_acc, const double* e_err, double* ymm_out, double* yea_out, double* mom,
+                        double4* rec_mom, double* err_partial, double* new_mm, double* new_ea, int32_t nblk);
 hipError_t launch_reduce_max(hipStream_t s, const double* partial, int32_t count, double scale, double* out);
-hipError_t launch_translate(hipStream_t s, int64_t n, double h, double L, const double* pos_cur, const double* force,
-                            const double* zeta_trans, double* pos, const double* mm, double* mz_partial, int32_t nblk);
-hipError_t launch_reduce_mean(hipStream_t s, const double* partial, int32_t count, double inv_n, double* out,
-                              int32_t* counter, double* series);
-hipError_t launch_noise(hipStream_t s, int64_t n, uint64_t seed, const uint64_t* step_counter, uint64_t step_host,
+hipError_t launch_translate(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, double h,
+                            const double* pos_cur, const double* force, const double* zeta_trans, double* pos,
+                            const double* new_mm, const double* new_ea, double* mm, double* ea);
+hipError_t launch_noise(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const uint64_t* rep_step,
                         double dt, const double* zeta_rot, const double* vol_mag, const double* zeta_trans,
                         double kT, double alpha_g, double gyro, double sat_mag, int32_t en_t, int32_t en_b,
                         int32_t en_f, double* n_t, double* n_b, double* n_f);
-hipError_t launch_scale(hipStream_t s, double* p, int64_t count, double f);
-hipError_t launch_step_count(hipStream_t s, uint64_t* counter);
+hipError_t launch_step_count(hipStream_t s, int32_t nrep, uint64_t* rep_step, double* rep_t, double h,
+                             int32_t* series_counter);
 
 }  // namespace mds

@@ -81,12 +81,14 @@ int grid_for(int64_t n, int block, int cap = 65535 * 4) {
 
 // ------------------------------------------------------------------ step setup
 // pos0 <- pos (step-start positions: EvalSinCos and the pair scan read these),
-// pos_cur <- pos (positions the overlap correction moves), cell keys.
+// pos_cur <- pos (positions the overlap correction moves), cell keys in the
+// replica's main and fine grids (global cell ids).
 __global__ void k_prepare(int64_t n, const double* __restrict__ pos, double* __restrict__ pos0,
-                          double* __restrict__ pos_cur, double L, int32_t ncell, double cell,
-                          int32_t* __restrict__ key_main, int32_t fncell, double fcell, int32_t* __restrict__ key_fine) {
+                          double* __restrict__ pos_cur, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
+                          int32_t* __restrict__ key_main, int32_t* __restrict__ key_fine) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const Rep& R = reps[rep_of[i]];
     const double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
     pos0[i] = x;
     pos0[n + i] = y;
@@ -94,10 +96,12 @@ __global__ void k_prepare(int64_t n, const double* __restrict__ pos, double* __r
     pos_cur[i] = x;
     pos_cur[n + i] = y;
     pos_cur[2 * n + i] = z;
-    key_main[i] = (cell_coord(z, L, cell, ncell) * ncell + cell_coord(y, L, cell, ncell)) * ncell +
-                  cell_coord(x, L, cell, ncell);
-    key_fine[i] = (cell_coord(z, L, fcell, fncell) * fncell + cell_coord(y, L, fcell, fncell)) * fncell +
-                  cell_coord(x, L, fcell, fncell);
+    const double L = R.L;
+    const int nc = R.ncell, fn = R.fncell;
+    key_main[i] = R.cell_off + (cell_coord(z, L, R.cell, nc) * nc + cell_coord(y, L, R.cell, nc)) * nc +
+                  cell_coord(x, L, R.cell, nc);
+    key_fine[i] = R.fcell_off + (cell_coord(z, L, R.fcell, fn) * fn + cell_coord(y, L, R.fcell, fn)) * fn +
+                  cell_coord(x, L, R.fcell, fn);
 }
 
 __global__ void k_count(int64_t n, const int32_t* __restrict__ key, int32_t* __restrict__ count) {
@@ -194,14 +198,17 @@ __global__ __launch_bounds__(1024) void k_work(int32_t ncells, const int32_t* __
 // writes each particle's partners in ascending j (so the candidate list is in
 // the reference's lexicographic (i, j) order).
 template <int PASS>
-__global__ void k_contacts(int64_t n, double L, const double* __restrict__ pos0, const double* __restrict__ r_mag,
-                           int32_t fncell, double fcell, const int32_t* __restrict__ fstart,
-                           const int32_t* __restrict__ fend, const int32_t* __restrict__ fsorted, double extra,
-                           int32_t* __restrict__ cand_count, const int32_t* __restrict__ cand_off,
-                           int32_t* __restrict__ cand_i, int32_t* __restrict__ cand_j, int32_t cap,
-                           int32_t* __restrict__ err) {
+__global__ void k_contacts(int64_t n, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
+                           const double* __restrict__ pos0, const double* __restrict__ r_mag,
+                           const int32_t* __restrict__ fstart, const int32_t* __restrict__ fend,
+                           const int32_t* __restrict__ fsorted, int32_t* __restrict__ cand_count,
+                           const int32_t* __restrict__ cand_off, int32_t* __restrict__ cand_i,
+                           int32_t* __restrict__ cand_j, int32_t cap, int32_t* __restrict__ err) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const Rep& R = reps[rep_of[i]];
+    const double L = R.L, fcell = R.fcell, extra = R.margin;
+    const int32_t fncell = R.fncell, foff = R.fcell_off;
     const double xi = pos0[i], yi = pos0[n + i], zi = pos0[2 * n + i], ri = r_mag[i];
     const int cx = cell_coord(xi, L, fcell, fncell), cy = cell_coord(yi, L, fcell, fncell),
               cz = cell_coord(zi, L, fcell, fncell);
@@ -212,7 +219,7 @@ __global__ void k_contacts(int64_t n, double L, const double* __restrict__ pos0,
     for (int dz = -reach; dz <= reach; ++dz)
         for (int dy = -reach; dy <= reach; ++dy)
             for (int dx = -reach; dx <= reach; ++dx) {
-                const int c = (((cz + dz + fncell) % fncell) * fncell + (cy + dy + fncell) % fncell) * fncell +
+                const int c = foff + (((cz + dz + fncell) % fncell) * fncell + (cy + dy + fncell) % fncell) * fncell +
                               (cx + dx + fncell) % fncell;
                 for (int32_t s = fstart[c]; s < fend[c]; ++s) {
                     const int32_t j = fsorted[s];
@@ -262,20 +269,24 @@ __global__ void k_contacts(int64_t n, double L, const double* __restrict__ pos0,
 // moves both particles half the overlap along the pre-move axis, and later
 // pairs see the moved positions. Every move is recorded as a version
 // (pair key, new position) of that particle, so the pair kernels can give each
-// pair the positions the reference's scan would have used.
-__global__ void k_resolve(int64_t n, double L, double rc, const double* __restrict__ r_mag,
-                          const int32_t* __restrict__ ncand, const int32_t* __restrict__ cand_i,
+// pair the positions the reference's scan would have used. The scan is
+// sequential within a replica; one thread per replica.
+__global__ void k_resolve(int32_t nrep, int64_t n, const Rep* __restrict__ reps, const double* __restrict__ r_mag,
+                          const int32_t* __restrict__ cand_off, const int32_t* __restrict__ cand_i,
                           const int32_t* __restrict__ cand_j, double* __restrict__ cur, const double* __restrict__ pos0,
                           int32_t* __restrict__ ev_head, int32_t* __restrict__ ev_tail, int32_t* __restrict__ ev_next,
                           uint64_t* __restrict__ ev_key, double* __restrict__ ev_pos, int32_t ev_cap,
                           int32_t* __restrict__ n_events, double* __restrict__ max_disp, double disp_limit,
                           int32_t* __restrict__ moved_list, int32_t* __restrict__ n_moved, int32_t* __restrict__ err) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int32_t rid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rid >= nrep) return;
+    const Rep& R = reps[rid];
     // candidates beyond the list capacity were never written (flagged by k_contacts)
-    const int32_t total = min(*ncand, ev_cap / 2);
-    int32_t nev = 0, nmv = 0;
+    const int32_t cap = ev_cap / 2;
+    const int32_t kb = min(cand_off[R.off], cap), ke = min(cand_off[R.off + R.n], cap);
+    const double L = R.L, rc = R.rc;
     double md = 0;
-    for (int32_t k = 0; k < total; ++k) {
+    for (int32_t k = kb; k < ke; ++k) {
         const int32_t i = cand_i[k], j = cand_j[k];
         double v[3];
         for (int d = 0; d < 3; ++d) v[d] = min_image(cur[d * n + j] - cur[d * n + i], L);
@@ -293,11 +304,11 @@ __global__ void k_resolve(int64_t n, double L, double rc, const double* __restri
         }
         for (int s = 0; s < 2; ++s) {
             const int32_t p = pp[s];
-            if (nev >= ev_cap) {
+            const int32_t e = atomicAdd(n_events, 1);
+            if (e >= ev_cap) {
                 atomicOr(err, 16);
                 break;
             }
-            const int32_t e = nev++;
             ev_key[e] = key;
             double dsq = 0;
             for (int d = 0; d < 3; ++d) {
@@ -309,15 +320,14 @@ __global__ void k_resolve(int64_t n, double L, double rc, const double* __restri
             ev_next[e] = -1;
             if (ev_tail[p] < 0) {
                 ev_head[p] = e;
-                moved_list[nmv++] = p;
+                moved_list[atomicAdd(n_moved, 1)] = p;
             } else
                 ev_next[ev_tail[p]] = e;
             ev_tail[p] = e;
         }
     }
-    *n_events = nev;
-    *n_moved = nmv;
-    *max_disp = md;
+    // non-negative doubles order as their bit patterns
+    atomicMax((unsigned long long*)max_disp, (unsigned long long)__double_as_longlong(md));
     if (md > disp_limit) atomicOr(err, 32);
 }
 
@@ -348,35 +358,54 @@ __device__ __forceinline__ void block_sum3(double a, double b, double c, double*
     }
 }
 
-// magMomVec = posMm .* magMom and its column sums (src/simulation.cpp:638-639).
+// magMomVec = posMm .* magMom (src/simulation.cpp:638), in original and
+// cell-sorted order.
 __global__ __launch_bounds__(256) void k_moments(int64_t n, const double* __restrict__ mm,
                                                  const double* __restrict__ mag_mom, const int32_t* __restrict__ inv_perm,
-                                                 double* __restrict__ mom, double4* __restrict__ rec_mom,
-                                                 double* __restrict__ partial) {
+                                                 double* __restrict__ mom, double4* __restrict__ rec_mom) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    double mx = 0, my = 0, mz = 0;
-    if (i < n) {
-        const double a = mag_mom[i];
-        mx = mm[i] * a;
-        my = mm[n + i] * a;
-        mz = mm[2 * n + i] * a;
-        mom[i] = mx;
-        mom[n + i] = my;
-        mom[2 * n + i] = mz;
-        rec_mom[inv_perm[i]] = make_double4(mx, my, mz, 0.0);
-    }
-    block_sum3<256>(mx, my, mz, partial + 3 * blockIdx.x);
+    if (i >= n) return;
+    const double a = mag_mom[i];
+    const double mx = mm[i] * a, my = mm[n + i] * a, mz = mm[2 * n + i] * a;
+    mom[i] = mx;
+    mom[n + i] = my;
+    mom[2 * n + i] = mz;
+    rec_mom[inv_perm[i]] = make_double4(mx, my, mz, 0.0);
 }
 
-__global__ __launch_bounds__(256) void k_reduce_sum3(const double* __restrict__ partial, int32_t count,
-                                                     double* __restrict__ out) {
+// Column sums of an SoA N x 3 array per replica (magMomSum,
+// src/simulation.cpp:639): one block per replica, fixed summation order.
+__global__ __launch_bounds__(256) void k_rep_sum3(const Rep* __restrict__ reps, int64_t n, const double* __restrict__ v,
+                                                  double* __restrict__ out) {
+    const Rep& R = reps[blockIdx.x];
     double a = 0, b = 0, c = 0;
-    for (int k = threadIdx.x; k < count; k += blockDim.x) {
-        a += partial[3 * k];
-        b += partial[3 * k + 1];
-        c += partial[3 * k + 2];
+    for (int64_t i = R.off + threadIdx.x; i < R.off + R.n; i += blockDim.x) {
+        a += v[i];
+        b += v[n + i];
+        c += v[2 * n + i];
     }
-    block_sum3<256>(a, b, c, out);
+    block_sum3<256>(a, b, c, out + 3 * blockIdx.x);
+}
+
+// Mean of the z component per replica (<m_z>, src/simulation.cpp:515); with
+// a series buffer, also stored at the running step index.
+__global__ __launch_bounds__(256) void k_rep_mean_z(const Rep* __restrict__ reps, int64_t n, const double* __restrict__ v,
+                                                    double* __restrict__ out, const int32_t* __restrict__ counter,
+                                                    double* __restrict__ series, int32_t series_cap, int32_t nrep) {
+    const Rep& R = reps[blockIdx.x];
+    double a = 0;
+    for (int64_t i = R.off + threadIdx.x; i < R.off + R.n; i += blockDim.x) a += v[2 * n + i];
+    __shared__ double r3[3];
+    block_sum3<256>(a, 0.0, 0.0, r3);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = r3[0] / (double)R.n;
+        out[blockIdx.x] = m;
+        if (series) {
+            const int32_t k = *counter;
+            if (k < series_cap) series[(int64_t)k * nrep + blockIdx.x] = m;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ real space
@@ -530,6 +559,24 @@ __device__ __forceinline__ double short_range(const PairParams& pp, double r, do
     return sf;
 }
 
+// The replica's Ewald constants and FP32 units in a PairParams copy.
+__device__ __forceinline__ void apply_rep(PairParams& p, const Rep& R) {
+    p.alpha = R.alpha;
+    p.alpha2 = R.alpha2;
+    p.alpha4 = R.alpha4;
+    p.rc = R.rc;
+    p.rc_sr = R.rc_sr;
+    p.ell = R.ell;
+    p.inv_ell = R.inv_ell;
+    p.inv_mref = R.inv_mref;
+    p.bscale = R.bscale;
+    p.fscale = R.fscale;
+    p.pscale = R.pscale;
+    p.al_s = R.al_s;
+    p.al2_s = R.al2_s;
+    p.al4_s = R.al4_s;
+}
+
 // One queued pair (lane il of the wave, sorted index j). Pairs with an
 // endpoint moved by the overlap correction are left to k_pairs_moved.
 template <bool FORCE, bool COATING, bool MIXED>
@@ -588,7 +635,7 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
 
 template <bool FORCE, bool COATING, bool MIXED>
 __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
-                                               PairOut out, const int4* __restrict__ stencil, int32_t nsten,
+                                               PairOut out, const int4* __restrict__ stencil_all,
                                                unsigned long long* __restrict__ cnt_pairs) {
     // One block per chunk of up to 64 particles of one cell; its 4 wavefronts
     // split the stencil between them and merge their sums in LDS at the end.
@@ -602,6 +649,12 @@ __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __rest
     if (w >= nwork) return;  // uniform over the block
     const int32_t item = g.work[w];
     const int32_t cell = item >> 8, chunk = item & 255;
+    // the replica of this cell: its box, grid, stencil and Ewald constants
+    const Rep& R = g.reps[g.cell_rep[cell]];
+    g.L = R.L;
+    apply_rep(pp, R);
+    const int4* stencil = stencil_all + R.sten_off;
+    const int32_t nsten = R.nsten;
     const int32_t s0 = g.cell_start[cell] + chunk * WAVE;
     const int32_t e0 = min(s0 + WAVE, g.cell_end[cell]);
     const int32_t k = s0 + lane;
@@ -616,9 +669,10 @@ __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __rest
     for (int d = 0; d < 6; ++d) W.acc[d][lane] = 0.0;
     const bool moved_any = *g.n_events > 0;
     const double L = g.L;
-    const int nc = g.ncell;
+    const int nc = R.ncell;
     const bool allpairs = nc == 1;
-    const int cx = cell % nc, cy = (cell / nc) % nc, cz = cell / (nc * nc);
+    const int lc = cell - R.cell_off;
+    const int cx = lc % nc, cy = (lc / nc) % nc, cz = lc / (nc * nc);
     // candidate radius: r_c with a relative slack covering the rounding of
     // the FP32 distance test
     const float rcf = (float)(pp.rc * (1.0 + 1e-5));
@@ -637,7 +691,7 @@ __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __rest
         if (jx < 0) { jx += nc; sx = -L; } else if (jx >= nc) { jx -= nc; sx = L; }
         if (jy < 0) { jy += nc; sy = -L; } else if (jy >= nc) { jy -= nc; sy = L; }
         if (jz < 0) { jz += nc; sz = -L; } else if (jz >= nc) { jz -= nc; sz = L; }
-        const int c = (jz * nc + jy) * nc + jx;
+        const int c = R.cell_off + (jz * nc + jy) * nc + jx;
         const int32_t cs = g.cell_start[c], ce = g.cell_end[c];
         for (int32_t jb = cs; jb < ce; jb += WAVE) {
             const int32_t jl = jb + lane;
@@ -731,25 +785,30 @@ __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __rest
 // atomics after k_pairs has written its sums.
 template <bool FORCE, bool COATING>
 __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
-                                                    PairOut out, const int4* __restrict__ stencil, int32_t nsten,
+                                                    PairOut out, const int4* __restrict__ stencil_all,
                                                     int32_t press_off, unsigned long long* __restrict__ cnt_pairs) {
     const int32_t nmv = *g.n_moved;
     const int lane = lane_id();
     const int64_t n = g.n;
-    const double L = g.L;
-    const int nc = g.ncell;
     for (int32_t mi = blockIdx.x; mi < nmv; mi += gridDim.x) {
         const int32_t p = g.moved_list[mi];
+        const Rep& R = g.reps[g.rep_of[p]];
+        g.L = R.L;
+        apply_rep(pp, R);
+        const double L = R.L;
+        const int nc = R.ncell;
+        const int4* stencil = stencil_all + R.sten_off;
+        const int32_t nsten = R.nsten;
         const int32_t kp = g.inv_perm[p];
         const double4 pp0 = g.rec_pos[kp];
         const double4 mp = rec_mom[kp];
-        const int cx = cell_coord(pp0.x, L, g.cell, nc), cy = cell_coord(pp0.y, L, g.cell, nc),
-                  cz = cell_coord(pp0.z, L, g.cell, nc);
+        const int cx = cell_coord(pp0.x, L, R.cell, nc), cy = cell_coord(pp0.y, L, R.cell, nc),
+                  cz = cell_coord(pp0.z, L, R.cell, nc);
         double press[3] = {0, 0, 0};
         int64_t npairs = 0;
         for (int si = 0; si < nsten; ++si) {
             const int4 o = stencil[si];
-            const int c = (((cz + o.z + nc) % nc) * nc + (cy + o.y + nc) % nc) * nc + (cx + o.x + nc) % nc;
+            const int c = R.cell_off + (((cz + o.z + nc) % nc) * nc + (cy + o.y + nc) % nc) * nc + (cx + o.x + nc) % nc;
             const int32_t cs = g.cell_start[c], ce = g.cell_end[c];
             for (int32_t j = cs + lane; j < ce; j += WAVE) {
                 if (j == kp) continue;
@@ -800,9 +859,10 @@ __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* _
         if (FORCE) {
             const double p0 = wave_sum(press[0]), p1 = wave_sum(press[1]), p2 = wave_sum(press[2]);
             if (lane == 0) {
-                atomicAdd(&out.press_partial[3 * (int64_t)press_off], p0);
-                atomicAdd(&out.press_partial[3 * (int64_t)press_off + 1], p1);
-                atomicAdd(&out.press_partial[3 * (int64_t)press_off + 2], p2);
+                const int64_t slot = 3 * ((int64_t)press_off + g.rep_of[p]);
+                atomicAdd(&out.press_partial[slot], p0);
+                atomicAdd(&out.press_partial[slot + 1], p1);
+                atomicAdd(&out.press_partial[slot + 2], p2);
             }
         }
         if (cnt_pairs) {
@@ -815,14 +875,16 @@ __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* _
 // ------------------------------------------------------------------ k space
 // cos/sin(2 pi n x_d / L), n = 0..nc, by the Rapaport recurrence of EvalSinCos
 // (src/interaction.cpp:521-548). Layout per particle: [d][n][cos, sin].
-__global__ void k_sincos_tab(int64_t n, int32_t nc, double two_pi_over_L, const double* __restrict__ pos0,
-                             double* __restrict__ tab) {
+__global__ void k_sincos_tab(int64_t n, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
+                             const double* __restrict__ pos0, double* __restrict__ tab_all) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const Rep& R = reps[rep_of[i]];
+    const int32_t nc = R.nc;
     const int64_t stride = 3 * (int64_t)(nc + 1) * 2;
-    double* t = tab + i * stride;
+    double* t = tab_all + R.tab_off + (i - R.off) * stride;
     for (int d = 0; d < 3; ++d) {
-        const double tt = two_pi_over_L * pos0[d * n + i];
+        const double tt = R.two_pi_over_L * pos0[d * n + i];
         double* td = t + d * (nc + 1) * 2;
         double s1, c1;
         sincos(tt, &s1, &c1);
@@ -853,11 +915,29 @@ __global__ void k_sincos_tab(int64_t n, int32_t nc, double two_pi_over_L, const 
 // loads), and the segment's nx values are accumulated in registers through
 // the recurrence e^{i(a+1)x} = e^{iax} e^{ix}. Each chunk writes its partial
 // sums; k_kreduce adds them in a fixed order.
+// Replica of work item b: the last replica whose first item is <= b.
+__device__ __forceinline__ int find_rep(const Rep* __restrict__ reps, int32_t nrep, int64_t b, bool by_k) {
+    int lo = 0, hi = nrep - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((by_k ? reps[mid].k_off : reps[mid].kb_off) <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 template <int SEG>
-__global__ __launch_bounds__(64) void k_kbuild(int64_t n, int32_t nc, int32_t nseg, const KSeg* __restrict__ segs,
-                                               const double* __restrict__ tab, const double* __restrict__ mom,
-                                               int32_t chunk, int32_t nk, double* __restrict__ partial) {
-    const int r = blockIdx.x * WAVE + threadIdx.x;
+__global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restrict__ reps, int32_t nrep,
+                                               const KSeg* __restrict__ segs_all, const double* __restrict__ tab_all,
+                                               const double* __restrict__ mom, double* __restrict__ partial_all) {
+    // work item = (replica, wavefront of row segments, particle chunk)
+    const Rep& R = reps[find_rep(reps, nrep, blockIdx.x, false)];
+    const int32_t li = (int32_t)blockIdx.x - R.kb_off;
+    const int32_t segw = li % R.nsegw, ch = li / R.nsegw;
+    const int32_t nc = R.nc, nseg = R.nseg, chunk = R.kchunk, nk = R.nk;
+    const KSeg* segs = segs_all + R.seg_off;
+    const double* tab = tab_all + R.tab_off;
+    const int64_t n = n_all;
+    const int r = segw * WAVE + threadIdx.x;
     const bool vr = r < nseg;
     const KSeg sg = vr ? segs[r] : KSeg{0, 0, 1, 0, 0, 0};
     const int ay = sg.ny < 0 ? -sg.ny : sg.ny;
@@ -867,10 +947,11 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n, int32_t nc, int32_t ns
     double pc_[SEG], ps_[SEG], mc_[SEG], ms_[SEG];
 #pragma unroll
     for (int a = 0; a < SEG; ++a) pc_[a] = ps_[a] = mc_[a] = ms_[a] = 0.0;
-    const int64_t jb = (int64_t)blockIdx.y * chunk;
-    const int64_t je = jb + chunk < n ? jb + chunk : n;
-    for (int64_t j = jb; j < je; ++j) {
-        const double* t = tab + j * stride;
+    const int64_t jb = (int64_t)ch * chunk;
+    const int64_t je = jb + chunk < R.n ? jb + chunk : R.n;
+    for (int64_t jr = jb; jr < je; ++jr) {
+        const double* t = tab + jr * stride;
+        const int64_t j = R.off + jr;
         const double ex = t[2], sx = t[3];
         const double* ty = t + (nc + 1) * 2;
         const double* tz = t + 2 * (nc + 1) * 2;
@@ -897,7 +978,7 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n, int32_t nc, int32_t ns
         }
     }
     if (!vr) return;
-    double* out = partial + (int64_t)blockIdx.y * nk * 2;
+    double* out = partial_all + R.part_off + (int64_t)ch * nk * 2;
 #pragma unroll
     for (int q = 0; q < SEG; ++q) {
         const int a = sg.alo + q;
@@ -913,33 +994,38 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n, int32_t nc, int32_t ns
     }
 }
 
-__global__ void k_kreduce(int32_t nk, int32_t nchunks, const double* __restrict__ partial, double* __restrict__ sk) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nk) return;
+__global__ void k_kreduce(int32_t nk_all, const Rep* __restrict__ reps, int32_t nrep,
+                          const double* __restrict__ partial_all, double* __restrict__ sk) {
+    const int kg = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kg >= nk_all) return;
+    const Rep& R = reps[find_rep(reps, nrep, kg, true)];
+    const int k = kg - R.k_off, nk = R.nk;
+    const double* partial = partial_all + R.part_off;
     double c = 0, s = 0;
-    for (int ch = 0; ch < nchunks; ++ch) {
+    for (int ch = 0; ch < R.nchunks; ++ch) {
         c += partial[((int64_t)ch * nk + k) * 2];
         s += partial[((int64_t)ch * nk + k) * 2 + 1];
     }
-    sk[2 * k] = c;
-    sk[2 * k + 1] = s;
+    sk[2 * kg] = c;
+    sk[2 * kg + 1] = s;
 }
 
-// Second pass of CompDipInteractEwaldF / CompDipFieldEwaldF
-// (src/interaction.cpp:485-504, 644-662) plus the self and vacuum-surface
-// terms (510-513): one lane per particle, all k in row order; the particle's
-// phases advance by the recurrences in y (row to row) and x (within a row).
+// One particle of the k-space scatter (k_kscatter) in replica R.
 template <bool FORCE>
-__global__ __launch_bounds__(256) void k_kscatter(int64_t n, int32_t nc, int32_t nrows, const KRow* __restrict__ rows,
-                                                  const double* __restrict__ ek, const double* __restrict__ sk,
-                                                  const double* __restrict__ tab, const double* __restrict__ mom,
-                                                  const double* __restrict__ msum, double c_force, double c_flux,
-                                                  double c_self, double c_surf, const double* __restrict__ add_force,
-                                                  double* __restrict__ force, double* __restrict__ field) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void kscatter_one(int64_t n, int64_t i, int32_t rid, const Rep& R,
+                                             const KRow* __restrict__ rows_all, const double* __restrict__ ek_all,
+                                             const double* __restrict__ sk_all, const double* __restrict__ tab_all,
+                                             const double* __restrict__ mom, const double* __restrict__ msum_all,
+                                             const double* __restrict__ add_force, double* __restrict__ force,
+                                             double* __restrict__ field) {
+    const int32_t nc = R.nc, nrows = R.nrows;
+    const KRow* rows = rows_all + R.row_off;
+    const double* ek = ek_all + R.k_off;
+    const double* sk = sk_all + 2 * (int64_t)R.k_off;
+    const double* msum = msum_all + 3 * rid;
+    const double c_force = R.c_force, c_flux = R.c_flux, c_self = R.c_self, c_surf = R.c_surf;
     const int64_t stride = 3 * (int64_t)(nc + 1) * 2;
-    const double* t = tab + i * stride;
+    const double* t = tab_all + R.tab_off + (i - R.off) * stride;
     const double* ty = t + (nc + 1) * 2;
     const double* tz = t + 2 * (nc + 1) * 2;
     const double ex = t[2], sx = t[3], ey = ty[2], sy = ty[3];
@@ -1005,6 +1091,34 @@ __global__ __launch_bounds__(256) void k_kscatter(int64_t n, int32_t nc, int32_t
     }
 }
 
+// Second pass of CompDipInteractEwaldF / CompDipFieldEwaldF
+// (src/interaction.cpp:485-504, 644-662) plus the self and vacuum-surface
+// terms (510-513): one lane per particle, all k in row order; the particle's
+// phases advance by the recurrences in y (row to row) and x (within a row).
+// A wavefront whose particles share one replica (all but those straddling a
+// replica boundary) reads the k tables through scalar loads.
+template <bool FORCE>
+__global__ __launch_bounds__(256) void k_kscatter(int64_t n, const Rep* __restrict__ reps,
+                                                  const int32_t* __restrict__ rep_of, const KRow* __restrict__ rows_all,
+                                                  const double* __restrict__ ek_all, const double* __restrict__ sk_all,
+                                                  const double* __restrict__ tab_all, const double* __restrict__ mom,
+                                                  const double* __restrict__ msum_all,
+                                                  const double* __restrict__ add_force, double* __restrict__ force,
+                                                  double* __restrict__ field) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t ic = i < n ? i : n - 1;
+    const int32_t rid = rep_of[ic];
+    const int32_t rid0 = __builtin_amdgcn_readfirstlane(rid);
+    const bool uniform = __ballot(rid != rid0) == 0;
+    if (i >= n) return;
+    if (uniform)
+        kscatter_one<FORCE>(n, i, rid0, reps[rid0], rows_all, ek_all, sk_all, tab_all, mom, msum_all, add_force, force,
+                            field);
+    else
+        kscatter_one<FORCE>(n, i, rid, reps[rid], rows_all, ek_all, sk_all, tab_all, mom, msum_all, add_force, force,
+                            field);
+}
+
 // ------------------------------------------------------------------ integrator
 struct Coef {
     double v[8];
@@ -1029,14 +1143,17 @@ __global__ __launch_bounds__(256) void k_stage(StageParams sp, int32_t stage, in
                                                double* __restrict__ kea, Coef a_next, Coef b_acc, Coef e_err,
                                                double* __restrict__ ymm_out, double* __restrict__ yea_out,
                                                double* __restrict__ mom, double4* __restrict__ rec_mom,
-                                               double* __restrict__ msum_partial, double* __restrict__ err_partial,
-                                               double* __restrict__ new_mm, double* __restrict__ new_ea) {
+                                               double* __restrict__ err_partial, double* __restrict__ new_mm,
+                                               double* __restrict__ new_ea) {
     const int64_t n = sp.n;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool v = i < n;
     double mo[3] = {0, 0, 0};
     double errn = 0;
     if (v) {
+        // DefExtField (src/simulation.cpp:546-556) at the step's start time
+        const int32_t ri = sp.rep_of[i];
+        const double bext = (sp.rep_t[ri] < sp.t_mag) ? sp.reps[ri].field_z : 0.0;
         double m[3], e[3], B[3], T[3], Bt[3];
         for (int d = 0; d < 3; ++d) {
             m[d] = ymm[d * n + i];
@@ -1047,7 +1164,7 @@ __global__ __launch_bounds__(256) void k_stage(StageParams sp, int32_t stage, in
         }
         const double dt = m[0] * e[0] + m[1] * e[1] + m[2] * e[2];
         double Be[3];
-        for (int d = 0; d < 3; ++d) Be[d] = (d == 2 ? sp.bext_z : 0.0) + B[d] + sp.anis_const * (e[d] * dt) + Bt[d];
+        for (int d = 0; d < 3; ++d) Be[d] = (d == 2 ? bext : 0.0) + B[d] + sp.anis_const * (e[d] * dt) + Bt[d];
         double ke[3] = {0, 0, 0};
         if (sp.mobil) {
             const double c1 = sp.c1[i], c2 = sp.c2[i];
@@ -1101,9 +1218,8 @@ __global__ __launch_bounds__(256) void k_stage(StageParams sp, int32_t stage, in
             }
         }
     }
-    if (stage + 1 < stages) {
-        block_sum3<256>(mo[0], mo[1], mo[2], msum_partial + 3 * blockIdx.x);
-    } else {
+    (void)mo;
+    if (stage + 1 == stages) {
         __shared__ double sh[256 / WAVE];
         errn = wave_max(errn);
         if (lane_id() == 0) sh[threadIdx.x / WAVE] = errn;
@@ -1132,41 +1248,24 @@ __global__ __launch_bounds__(256) void k_reduce_max(const double* __restrict__ p
 }
 
 // IntegrationTrans (src/simulation.cpp:959-962) + ApplyBoundaryCondArr
-// (610-620), and the <m_z> partial sums (515).
-__global__ __launch_bounds__(256) void k_translate(int64_t n, double h, double L, const double* __restrict__ pos_cur,
-                                                   const double* __restrict__ force,
+// (610-620), and the accepted orientations into the state.
+__global__ __launch_bounds__(256) void k_translate(int64_t n, const Rep* __restrict__ reps,
+                                                   const int32_t* __restrict__ rep_of, double h,
+                                                   const double* __restrict__ pos_cur, const double* __restrict__ force,
                                                    const double* __restrict__ zeta_trans, double* __restrict__ pos,
-                                                   const double* __restrict__ mm, double* __restrict__ mz_partial) {
+                                                   const double* __restrict__ new_mm, const double* __restrict__ new_ea,
+                                                   double* __restrict__ mm, double* __restrict__ ea) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    double mz = 0;
-    if (i < n) {
-        const double zt = zeta_trans[i];
-        for (int d = 0; d < 3; ++d) {
-            double x = pos_cur[d * n + i] + h * (force[d * n + i] / zt);
-            if (x > L / 2) x -= L;
-            if (x < -L / 2) x += L;
-            pos[d * n + i] = x;
-        }
-        mz = mm[2 * n + i];
-    }
-    block_sum3<256>(mz, 0.0, 0.0, mz_partial + 3 * blockIdx.x);
-}
-
-__global__ __launch_bounds__(256) void k_reduce_mean(const double* __restrict__ partial, int32_t count, double inv_n,
-                                                     double* __restrict__ out, int32_t* __restrict__ counter,
-                                                     double* __restrict__ series) {
-    double a = 0;
-    for (int k = threadIdx.x; k < count; k += blockDim.x) a += partial[3 * k];
-    __shared__ double r3[3];
-    block_sum3<256>(a, 0.0, 0.0, r3);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double v = r3[0] * inv_n;
-        *out = v;
-        if (series) {
-            series[*counter] = v;
-            *counter += 1;
-        }
+    if (i >= n) return;
+    const double L = reps[rep_of[i]].L;
+    const double zt = zeta_trans[i];
+    for (int d = 0; d < 3; ++d) {
+        double x = pos_cur[d * n + i] + h * (force[d * n + i] / zt);
+        if (x > L / 2) x -= L;
+        if (x < -L / 2) x += L;
+        pos[d * n + i] = x;
+        mm[d * n + i] = new_mm[d * n + i];
+        ea[d * n + i] = new_ea[d * n + i];
     }
 }
 
@@ -1189,18 +1288,25 @@ __device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) 
     }
 }
 
-__global__ __launch_bounds__(256) void k_noise(int64_t n, uint64_t seed, const uint64_t* __restrict__ step_counter,
-                                               uint64_t step_host, double dt, const double* __restrict__ zeta_rot,
+__global__ __launch_bounds__(256) void k_noise(int64_t n, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
+                                               const uint64_t* __restrict__ rep_step, double dt,
+                                               const double* __restrict__ zeta_rot,
                                                const double* __restrict__ vol_mag, const double* __restrict__ zeta_trans,
                                                double kT, double alpha_g, double gyro, double sat_mag, int32_t en_t,
                                                int32_t en_b, int32_t en_f, double* __restrict__ n_t,
                                                double* __restrict__ n_b, double* __restrict__ n_f) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint64_t step = step_counter ? *step_counter : step_host;
+    // counter (index within the replica, draw, replica's step): a replica's
+    // stream does not depend on the other replicas of the context
+    const int32_t ri = rep_of[i];
+    const Rep& R = reps[ri];
+    const uint64_t step = rep_step[ri];
+    const uint64_t seed = R.seed;
+    const int64_t li = i - R.off;
     double z[10];
     for (int call = 0; call < 5; ++call) {
-        uint32_t c[4] = {(uint32_t)i, (uint32_t)(i >> 32) ^ ((uint32_t)call << 24), (uint32_t)step,
+        uint32_t c[4] = {(uint32_t)li, (uint32_t)(li >> 32) ^ ((uint32_t)call << 24), (uint32_t)step,
                          (uint32_t)(step >> 32)};
         philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
         const uint64_t a = ((uint64_t)c[0] << 32) | c[1], b = ((uint64_t)c[2] << 32) | c[3];
@@ -1223,15 +1329,29 @@ __global__ __launch_bounds__(256) void k_noise(int64_t n, uint64_t seed, const u
     }
 }
 
-__global__ void k_step_count(uint64_t* c) {
-    if (threadIdx.x == 0) *c += 1;
+// Advance every replica's step counter and time by one accepted step h.
+__global__ void k_step_count(int32_t nrep, uint64_t* __restrict__ rep_step, double* __restrict__ rep_t, double h,
+                             int32_t* __restrict__ series_counter) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrep) {
+        rep_step[r] += 1;
+        rep_t[r] += h;
+    }
+    if (r == 0 && series_counter) *series_counter += 1;
 }
 
-__global__ void k_clear_events(int64_t n, int32_t* __restrict__ head, int32_t* __restrict__ tail) {
+__global__ void k_clear_events(int64_t n, int32_t* __restrict__ head, int32_t* __restrict__ tail,
+                               int32_t* __restrict__ n_events, int32_t* __restrict__ n_moved,
+                               double* __restrict__ max_disp) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) {
         head[i] = -1;
         tail[i] = -1;
+    }
+    if (i == 0) {
+        *n_events = 0;
+        *n_moved = 0;
+        *max_disp = 0.0;
     }
 }
 
@@ -1253,17 +1373,15 @@ hipError_t launch_scale(hipStream_t s, double* p, int64_t count, double f) {
     LAUNCH_CHECK();
 }
 
-hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* pos0, double* pos_cur, double L,
-                          int32_t ncell, double cell, int32_t* key_main, int32_t fncell, double fcell,
-                          int32_t* key_fine) {
-    hipLaunchKernelGGL(k_prepare, dim3(grid_for(n, 256)), dim3(256), 0, s, n, pos, pos0, pos_cur, L, ncell, cell,
-                       key_main, fncell, fcell, key_fine);
+hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* pos0, double* pos_cur, const Rep* reps,
+                          const int32_t* rep_of, int32_t* key_main, int32_t* key_fine) {
+    hipLaunchKernelGGL(k_prepare, dim3(grid_for(n, 256)), dim3(256), 0, s, n, pos, pos0, pos_cur, reps, rep_of,
+                       key_main, key_fine);
     LAUNCH_CHECK();
 }
 
 hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
-                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp) {
-    (void)scan_tmp;
+                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx) {
     hipError_t e = launch_fill_i32(s, count, ncells, 0);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_count, dim3(grid_for(n, 256)), dim3(256), 0, s, n, key, count);
@@ -1285,62 +1403,69 @@ hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start
     LAUNCH_CHECK();
 }
 
-hipError_t launch_contacts(hipStream_t s, int64_t n, double L, const double* pos0, const double* r_mag,
-                           int32_t fncell, double fcell, const int32_t* fstart, const int32_t* fend,
-                           const int32_t* fsorted, double search_extra, int32_t* cand_count, int32_t* cand_off,
-                           int32_t* cand_j, int32_t cand_cap, int32_t* scan_tmp, int32_t* err) {
+hipError_t launch_contacts(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
+                           const double* r_mag, const int32_t* fstart, const int32_t* fend, const int32_t* fsorted,
+                           int32_t* cand_count, int32_t* cand_off, int32_t* cand_j, int32_t cand_cap, int32_t* err) {
     // cand_off has n+1 slots: offsets then the total at [n]; cand_j holds
     // 2*cand_cap ints: partner j then owner i.
     int32_t* cand_i = cand_j + cand_cap;
-    hipLaunchKernelGGL(k_contacts<0>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, L, pos0, r_mag, fncell, fcell,
-                       fstart, fend, fsorted, search_extra, cand_count, (const int32_t*)nullptr, (int32_t*)nullptr,
-                       (int32_t*)nullptr, cand_cap, err);
+    hipLaunchKernelGGL(k_contacts<0>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart,
+                       fend, fsorted, cand_count, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
+                       cand_cap, err);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, cand_count, (int32_t)n, cand_off, (int32_t*)nullptr,
                        (int32_t*)nullptr, cand_off + n);
-    hipLaunchKernelGGL(k_contacts<1>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, L, pos0, r_mag, fncell, fcell,
-                       fstart, fend, fsorted, search_extra, cand_count, cand_off, cand_i, cand_j, cand_cap, err);
-    (void)scan_tmp;
+    hipLaunchKernelGGL(k_contacts<1>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart,
+                       fend, fsorted, cand_count, cand_off, cand_i, cand_j, cand_cap, err);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_resolve(hipStream_t s, int64_t n, double L, double rc, const double* r_mag, const int32_t* cand_off,
-                          const int32_t* cand_j, double* pos_cur, const double* pos0, int32_t* ev_head,
-                          int32_t* ev_tail, int32_t* ev_next, uint64_t* ev_key, double* ev_pos, int32_t ev_cap,
-                          int32_t* n_events, double* max_disp, double disp_limit, int32_t* moved_list,
+hipError_t launch_resolve(hipStream_t s, int32_t nrep, int64_t n, const Rep* reps, const double* r_mag,
+                          const int32_t* cand_off, const int32_t* cand_j, double* pos_cur, const double* pos0,
+                          int32_t* ev_head, int32_t* ev_tail, int32_t* ev_next, uint64_t* ev_key, double* ev_pos,
+                          int32_t ev_cap, int32_t* n_events, double* max_disp, double disp_limit, int32_t* moved_list,
                           int32_t* n_moved, int32_t* err) {
-    // the candidate total sits at cand_off[n]; its capacity is ev_cap / 2
+    // candidate offsets per particle (total at cand_off[n]); cand_j holds
+    // partners then owners, each of capacity ev_cap / 2
     const int32_t cand_cap = ev_cap / 2;
-    hipLaunchKernelGGL(k_resolve, dim3(1), dim3(64), 0, s, n, L, rc, r_mag, cand_off + n, cand_j + cand_cap, cand_j,
-                       pos_cur, pos0, ev_head, ev_tail, ev_next, ev_key, ev_pos, ev_cap, n_events, max_disp,
-                       disp_limit, moved_list, n_moved, err);
+    hipLaunchKernelGGL(k_resolve, dim3((nrep + 63) / 64), dim3(64), 0, s, nrep, n, reps, r_mag, cand_off,
+                       cand_j + cand_cap, cand_j, pos_cur, pos0, ev_head, ev_tail, ev_next, ev_key, ev_pos, ev_cap,
+                       n_events, max_disp, disp_limit, moved_list, n_moved, err);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_clear_events(hipStream_t s, int64_t n, int32_t* ev_head, int32_t* ev_tail) {
-    hipLaunchKernelGGL(k_clear_events, dim3(grid_for(n, 256)), dim3(256), 0, s, n, ev_head, ev_tail);
+hipError_t launch_clear_events(hipStream_t s, int64_t n, int32_t* ev_head, int32_t* ev_tail, int32_t* n_events,
+                               int32_t* n_moved, double* max_disp) {
+    hipLaunchKernelGGL(k_clear_events, dim3(grid_for(n, 256)), dim3(256), 0, s, n, ev_head, ev_tail, n_events,
+                       n_moved, max_disp);
     LAUNCH_CHECK();
 }
 
 hipError_t launch_moments(hipStream_t s, int64_t n, const double* mm, const double* mag_mom, const int32_t* inv_perm,
-                          double* mom, double4* rec_mom, double* msum_partial, int32_t nblk) {
-    hipLaunchKernelGGL(k_moments, dim3(nblk), dim3(256), 0, s, n, mm, mag_mom, inv_perm, mom, rec_mom, msum_partial);
+                          double* mom, double4* rec_mom) {
+    hipLaunchKernelGGL(k_moments, dim3(grid_for(n, 256)), dim3(256), 0, s, n, mm, mag_mom, inv_perm, mom, rec_mom);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_reduce_sum3(hipStream_t s, const double* partial, int32_t count, double* out) {
-    hipLaunchKernelGGL(k_reduce_sum3, dim3(1), dim3(256), 0, s, partial, count, out);
+hipError_t launch_rep_sum3(hipStream_t s, const Rep* reps, int32_t nrep, int64_t n, const double* v, double* out) {
+    hipLaunchKernelGGL(k_rep_sum3, dim3(nrep), dim3(256), 0, s, reps, n, v, out);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, int64_t n, const double* v, double* out,
+                             const int32_t* counter, double* series, int32_t series_cap) {
+    hipLaunchKernelGGL(k_rep_mean_z, dim3(nrep), dim3(256), 0, s, reps, n, v, out, counter, series, series_cap, nrep);
     LAUNCH_CHECK();
 }
 
 hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
                         const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
-                        int32_t nsten, int64_t* cnt_pairs) {
+                        int64_t* cnt_pairs) {
     const int blocks = max_work;
     if (blocks <= 0) return hipSuccess;
     unsigned long long* cp = (unsigned long long*)cnt_pairs;
     const bool mixed = precision == 1;
 #define KP(F, C, M) \
-    hipLaunchKernelGGL((k_pairs<F, C, M>), dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out, stencil, nsten, cp)
+    hipLaunchKernelGGL((k_pairs<F, C, M>), dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out, stencil, cp)
     if (force && pp.coating) {
         if (mixed) KP(true, true, true); else KP(true, true, false);
     } else if (force) {
@@ -1353,53 +1478,51 @@ hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geom
 }
 
 hipError_t launch_pairs_moved(hipStream_t s, bool force, const Geometry& g, const double4* rec_mom,
-                              const PairParams& pp, const PairOut& out, const int4* stencil, int32_t nsten,
-                              int32_t press_off, int64_t* cnt_pairs) {
+                              const PairParams& pp, const PairOut& out, const int4* stencil, int32_t press_off,
+                              int64_t* cnt_pairs) {
     unsigned long long* cp = (unsigned long long*)cnt_pairs;
     const int blocks = 512;
     if (force && pp.coating)
         hipLaunchKernelGGL((k_pairs_moved<true, true>), dim3(blocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
-                           nsten, press_off, cp);
+                           press_off, cp);
     else if (force)
         hipLaunchKernelGGL((k_pairs_moved<true, false>), dim3(blocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
-                           nsten, press_off, cp);
+                           press_off, cp);
     else
         hipLaunchKernelGGL((k_pairs_moved<false, false>), dim3(blocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
-                           nsten, press_off, cp);
+                           press_off, cp);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_sincos_tables(hipStream_t s, int64_t n, int32_t nc, double L, double two_pi, const double* pos0,
+hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
                                 double* tab) {
-    hipLaunchKernelGGL(k_sincos_tab, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nc, two_pi / L, pos0, tab);
+    hipLaunchKernelGGL(k_sincos_tab, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, pos0, tab);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_kbuild(hipStream_t s, int32_t precision, int64_t n, int32_t nc, int32_t nseg, const KSeg* segs,
-                         const double* tab, const double* mom, int32_t chunk, int32_t nchunks, int32_t nk,
-                         double* partial) {
-    (void)precision;
-    dim3 grid((nseg + WAVE - 1) / WAVE, nchunks);
-    hipLaunchKernelGGL(k_kbuild<KSEG>, grid, dim3(WAVE), 0, s, n, nc, nseg, segs, tab, mom, chunk, nk, partial);
+hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t kb_items, const KSeg* segs,
+                         const double* tab, const double* mom, double* partial) {
+    if (kb_items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kbuild<KSEG>, dim3(kb_items), dim3(WAVE), 0, s, n, reps, nrep, segs, tab, mom, partial);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_kreduce(hipStream_t s, int32_t nk, int32_t nchunks, const double* partial, double* sk) {
-    hipLaunchKernelGGL(k_kreduce, dim3((nk + 255) / 256), dim3(256), 0, s, nk, nchunks, partial, sk);
+hipError_t launch_kreduce(hipStream_t s, int32_t nk_all, const Rep* reps, int32_t nrep, const double* partial,
+                          double* sk) {
+    if (nk_all <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kreduce, dim3((nk_all + 255) / 256), dim3(256), 0, s, nk_all, reps, nrep, partial, sk);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_kscatter(hipStream_t s, bool force, int32_t precision, int64_t n, int32_t nc, int32_t nrows,
+hipError_t launch_kscatter(hipStream_t s, bool force, int64_t n, const Rep* reps, const int32_t* rep_of,
                            const KRow* rows, const double* ek, const double* sk, const double* tab, const double* mom,
-                           const double* msum, double c_force, double c_flux, double c_self, double c_surf,
-                           const double* add_force, double* force_out, double* field) {
-    (void)precision;
+                           const double* msum, const double* add_force, double* force_out, double* field) {
     if (force)
-        hipLaunchKernelGGL(k_kscatter<true>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nc, nrows, rows, ek, sk, tab,
-                           mom, msum, c_force, c_flux, c_self, c_surf, add_force, force_out, field);
+        hipLaunchKernelGGL(k_kscatter<true>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, rows, ek, sk, tab,
+                           mom, msum, add_force, force_out, field);
     else
-        hipLaunchKernelGGL(k_kscatter<false>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nc, nrows, rows, ek, sk,
-                           tab, mom, msum, c_force, c_flux, c_self, c_surf, add_force, force_out, field);
+        hipLaunchKernelGGL(k_kscatter<false>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, rows, ek, sk,
+                           tab, mom, msum, add_force, force_out, field);
     LAUNCH_CHECK();
 }
 
@@ -1407,8 +1530,7 @@ hipError_t launch_stage(hipStream_t s, const StageParams& sp, int32_t stage, int
                         const double* ea0, const double* ymm, const double* yea, const double* field,
                         const double* n_t, const double* n_b, double* kmm, double* kea, const double* a_next,
                         const double* b_acc, const double* e_err, double* ymm_out, double* yea_out, double* mom,
-                        double4* rec_mom, double* msum_partial, double* err_partial, double* new_mm, double* new_ea,
-                        int32_t nblk) {
+                        double4* rec_mom, double* err_partial, double* new_mm, double* new_ea, int32_t nblk) {
     Coef a{}, b{}, e{};
     for (int k = 0; k < 8; ++k) {
         a.v[k] = a_next ? a_next[k] : 0.0;
@@ -1416,7 +1538,7 @@ hipError_t launch_stage(hipStream_t s, const StageParams& sp, int32_t stage, int
         e.v[k] = e_err ? e_err[k] : 0.0;
     }
     hipLaunchKernelGGL(k_stage, dim3(nblk), dim3(256), 0, s, sp, stage, stages, mm0, ea0, ymm, yea, field, n_t, n_b,
-                       kmm, kea, a, b, e, ymm_out, yea_out, mom, rec_mom, msum_partial, err_partial, new_mm, new_ea);
+                       kmm, kea, a, b, e, ymm_out, yea_out, mom, rec_mom, err_partial, new_mm, new_ea);
     LAUNCH_CHECK();
 }
 
@@ -1425,30 +1547,27 @@ hipError_t launch_reduce_max(hipStream_t s, const double* partial, int32_t count
     LAUNCH_CHECK();
 }
 
-hipError_t launch_translate(hipStream_t s, int64_t n, double h, double L, const double* pos_cur, const double* force,
-                            const double* zeta_trans, double* pos, const double* mm, double* mz_partial, int32_t nblk) {
-    hipLaunchKernelGGL(k_translate, dim3(nblk), dim3(256), 0, s, n, h, L, pos_cur, force, zeta_trans, pos, mm,
-                       mz_partial);
+hipError_t launch_translate(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, double h,
+                            const double* pos_cur, const double* force, const double* zeta_trans, double* pos,
+                            const double* new_mm, const double* new_ea, double* mm, double* ea) {
+    hipLaunchKernelGGL(k_translate, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, h, pos_cur, force,
+                       zeta_trans, pos, new_mm, new_ea, mm, ea);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_reduce_mean(hipStream_t s, const double* partial, int32_t count, double inv_n, double* out,
-                              int32_t* counter, double* series) {
-    hipLaunchKernelGGL(k_reduce_mean, dim3(1), dim3(256), 0, s, partial, count, inv_n, out, counter, series);
-    LAUNCH_CHECK();
-}
-
-hipError_t launch_noise(hipStream_t s, int64_t n, uint64_t seed, const uint64_t* step_counter, uint64_t step_host,
-                        double dt, const double* zeta_rot, const double* vol_mag, const double* zeta_trans, double kT,
-                        double alpha_g, double gyro, double sat_mag, int32_t en_t, int32_t en_b, int32_t en_f,
-                        double* n_t, double* n_b, double* n_f) {
-    hipLaunchKernelGGL(k_noise, dim3(grid_for(n, 256)), dim3(256), 0, s, n, seed, step_counter, step_host, dt,
+hipError_t launch_noise(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const uint64_t* rep_step,
+                        double dt, const double* zeta_rot, const double* vol_mag, const double* zeta_trans,
+                        double kT, double alpha_g, double gyro, double sat_mag, int32_t en_t, int32_t en_b,
+                        int32_t en_f, double* n_t, double* n_b, double* n_f) {
+    hipLaunchKernelGGL(k_noise, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, rep_step, dt,
                        zeta_rot, vol_mag, zeta_trans, kT, alpha_g, gyro, sat_mag, en_t, en_b, en_f, n_t, n_b, n_f);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_step_count(hipStream_t s, uint64_t* counter) {
-    hipLaunchKernelGGL(k_step_count, dim3(1), dim3(64), 0, s, counter);
+hipError_t launch_step_count(hipStream_t s, int32_t nrep, uint64_t* rep_step, double* rep_t, double h,
+                             int32_t* series_counter) {
+    hipLaunchKernelGGL(k_step_count, dim3((nrep + 255) / 256), dim3(256), 0, s, nrep, rep_step, rep_t, h,
+                       series_counter);
     LAUNCH_CHECK();
 }
 

@@ -120,44 +120,13 @@ def box_length(cfg: capi.Config, r_hydr: np.ndarray) -> float:
     return (np.sum(4.0 / 3.0 * cfg.my_pi * r_hydr ** 3) / cfg.vol_frac) ** (1.0 / 3.0)
 
 
-def place(r_hydr: np.ndarray, L: float, rng: np.random.Generator, max_tries: int = 10000) -> np.ndarray:
+def place(r_hydr: np.ndarray, L: float, rng: np.random.Generator) -> np.ndarray:
     """Random sequential addition of spheres of radii r_hydr into the periodic
-    box [-L/2, L/2)^3 (no two hydrodynamic spheres overlap)."""
-    n = len(r_hydr)
-    rmax = float(r_hydr.max())
-    ncell = max(1, int(L // (2 * rmax)))
-    cell = L / ncell
-    grid: dict = {}
-    pos = np.zeros((n, 3))
-    order = np.argsort(-r_hydr)  # big ones first
-    for i in order:
-        ri = r_hydr[i]
-        for _ in range(max_tries):
-            p = (rng.random(3) - 0.5) * L
-            c = tuple(((p + L / 2) // cell).astype(int) % ncell)
-            ok = True
-            for dx in (-1, 0, 1):
-                for dy in (-1, 0, 1):
-                    for dz in (-1, 0, 1):
-                        for j in grid.get(((c[0] + dx) % ncell, (c[1] + dy) % ncell, (c[2] + dz) % ncell), ()):
-                            d = pos[j] - p
-                            d -= L * np.round(d / L)
-                            if d @ d < (ri + r_hydr[j]) ** 2:
-                                ok = False
-                                break
-                        if not ok:
-                            break
-                    if not ok:
-                        break
-                if not ok:
-                    break
-            if ok:
-                pos[i] = p
-                grid.setdefault(c, []).append(i)
-                break
-        else:
-            raise RuntimeError("random placement failed; volume fraction too high")
-    return pos
+    box [-L/2, L/2)^3 (no two hydrodynamic spheres overlap, minimum image):
+    the library's cell-list placement, seeded from rng."""
+    seed = int(rng.integers(0, 2 ** 63))
+    cfg = capi.config_from_string("[initial_configuration]\nINIT_CONFIG_TRANS = 1\n")
+    return capi.init_positions(cfg, r_hydr, L, seed, periodic=True)
 
 
 def orientations(n: int, rng: np.random.Generator, my_pi: float) -> np.ndarray:
