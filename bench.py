@@ -173,6 +173,46 @@ def ensemble(mds, workload, precision, replicas, steps, warmup, master=20261203)
     return out
 
 
+def serving_leg(precision, n_req, steps_scale, budget_gib, load=0.8, seed=20261203):
+    """Request stream through the continuous-batching loop: N log-uniform in
+    [1e3, 2e5], steps log-uniform in [1e3, 1e5] x steps_scale, field uniform
+    in [0, 100] mT, volume fraction uniform in [1 %, 10 %], own seeds. First
+    all requests at t = 0 (saturated throughput), then the same requests with
+    Poisson arrivals at `load` x the saturated request rate; latency of each
+    request against its isolated-solve time (alone in its own context)."""
+    import copy
+    from md_sim_wolfschwenger_amd import serving
+    rng = np.random.default_rng(seed)
+    base = serving.poisson_stream(n_req, 0.0, rng, steps_range=(1e3 * steps_scale, 1e5 * steps_scale))
+    kw = dict(precision=precision, hbm_budget=budget_gib * 2 ** 30)
+    srv = serving.LaneServer(**kw)
+    sat = srv.serve(copy.deepcopy(base))
+    srv.close()
+    work = sum(q.n * q.steps for q in base)
+    rate = load * sat["throughput"] / (work / n_req)
+    reqs = copy.deepcopy(base)
+    t = 0.0
+    for q in reqs:
+        t += rng.exponential(1.0 / rate)
+        q.arrival = t
+    srv = serving.LaneServer(**kw)
+    ld = srv.serve(reqs)
+    srv.close()
+    iso = np.array([serving.isolated_seconds({"precision": precision}, q) for q in reqs])
+    lat = np.array([q.latency for q in reqs])
+    ratio = lat / iso
+    return {"requests": n_req, "steps_range": [1e3 * steps_scale, 1e5 * steps_scale], "n_range": [1e3, 2e5],
+            "hbm_budget_gib": budget_gib, "precision": precision,
+            "saturated": {"throughput": sat["throughput"], "wall_s": sat["wall_s"], "p50_latency_s":
+                          sat["p50_latency_s"], "p99_latency_s": sat["p99_latency_s"]},
+            "lanes": {"bounds_n": list(srv.bounds), "requests": [l["requests"] for l in ld["lanes"]]},
+            "load": load, "arrival_rate_per_s": rate, "throughput": ld["throughput"], "wall_s": ld["wall_s"],
+            "p50_latency_s": ld["p50_latency_s"], "p99_latency_s": ld["p99_latency_s"],
+            "isolated_p50_s": float(np.percentile(iso, 50)), "isolated_p99_s": float(np.percentile(iso, 99)),
+            "latency_over_isolated_p50": float(np.percentile(ratio, 50)),
+            "latency_over_isolated_p99": float(np.percentile(ratio, 99))}
+
+
 def cpu_baseline(sysinfo, threads):
     """One Heun step of the same N = 1e5 system with the CPU oracle (the
     reference's algorithm restated: O(N^2) pair scan, k-space two-pass sums)."""
@@ -198,6 +238,28 @@ def cpu_baseline(sysinfo, threads):
     return n / (t1 - t0), t1 - t0
 
 
+def copy_calibration(mib):
+    """Device-to-device copies of `mib` MiB (the HIP runtime's blit kernel):
+    the reference reading for scaling the FETCH_SIZE / WRITE_SIZE counters of
+    a profiler run to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    size = int(mib) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(size)):
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(size))
+    hip.hipDeviceSynchronize()
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(size), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": size, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,11 +272,20 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--replicas", type=int, default=256, help="replicas of the batched-ensemble leg (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--requests", type=int, default=24, help="requests of the serving leg (0 = skip)")
+    ap.add_argument("--steps-scale", type=float, default=0.01,
+                    help="serving steps are log-uniform in [1e3, 1e5] times this")
+    ap.add_argument("--hbm-budget-gib", type=float, default=32.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run device-to-device copies of this size (profiler counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
               "(the path is single-GPU; multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
+        return
     import md_sim_wolfschwenger_amd as mds
     from md_sim_wolfschwenger_amd import workload
 
@@ -268,6 +339,12 @@ def main():
             log(f"batched ensemble: {args.replicas} replicas, {prec}")
             ens[prec] = ensemble(mds, workload, prec, args.replicas, args.ensemble_steps, 2)
             log(f"  {ens[prec]}")
+    serv = {}
+    if args.requests > 0:
+        for prec in ("fp64", "mixed"):
+            log(f"serving: {args.requests} requests, {prec}")
+            serv[prec] = serving_leg(prec, args.requests, args.steps_scale, args.hbm_budget_gib)
+            log(f"  {serv[prec]}")
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline: one oracle Heun step at N={n} on {args.cpu_threads} threads")
@@ -291,6 +368,7 @@ def main():
         "roofline": roof, "roofline_fp64": results["fp64"]["roof"], "kernel_ms_per_step": breakdown,
         "kernel_ms_per_step_fp64": results["fp64"]["breakdown"], "cpu_baseline": cpu,
         "ensemble": ens or None,
+        "serving": serv or None,
     }
     print(json.dumps(line), flush=True)
 

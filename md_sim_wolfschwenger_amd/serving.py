@@ -1,0 +1,330 @@
+"""Continuous-batching request loop for ensemble solves on one device.
+
+Each request is an independent suspension (N particles, volume fraction,
+applied field, Philox seed, number of fixed-dt steps). Active requests are
+replicas of one device context (libmdsim.so), so every kernel launch
+advances all of them; their state stays resident in HBM from admission to
+completion. The loop runs in chunks of steps replayed from a HIP graph; at
+each chunk boundary finished requests are retired (their state is copied
+out) and arrived requests are admitted while the estimated device footprint
+of the active set stays within the HBM budget. A request's result does not
+depend on what else runs beside it: its Ewald parameters, cell grid, clock
+and Philox counter are its own.
+
+The reference has no counterpart (one system per process,
+src/simulation.cpp:12-16, 440-520).
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi, workload
+
+
+@dataclass
+class Request:
+    rid: int
+    n: int
+    steps: int
+    field_z: float          # T, along z
+    phi: float              # volume fraction
+    seed: int
+    arrival: float = 0.0    # s after the loop starts
+    # filled by the loop
+    admitted: float = math.nan
+    finished: float = math.nan
+    mz: float = math.nan
+    state: tuple | None = None
+    ewald: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def rc_frac_for(n: int, scale: float) -> float:
+    """r_c / L of a system of n particles: `scale` at n = 1e5 (the autotuned
+    optimum there), times (1e5 / n)^(1/6) (the real-space/k-space cost
+    balance at fixed accuracy), kept in [0.1, 0.5]."""
+    return min(0.5, max(0.1, scale * (1e5 / n) ** (1.0 / 6.0)))
+
+
+def footprint_bytes(n: int, nc: int) -> int:
+    """Device bytes a replica of n particles with k-space cut-off nc adds to
+    a context (per-particle arrays, sin/cos tables, S(k) partials, cells),
+    with 25 % headroom."""
+    per_particle = 900 + 48 * (nc + 1)
+    nk = int(2.2 * nc ** 3) + 1
+    nseg = max(1, nk // 8)
+    nsegw = (nseg + 63) // 64
+    chunk = min(1024, max(32, (n * nsegw + 4095) // 4096))
+    nchunks = (n + chunk - 1) // chunk
+    kspace = nk * 8 * (3 + 2 * nchunks)
+    cells = 64 * n
+    return int(1.25 * (n * per_particle + kspace + cells))
+
+
+class Budget:
+    """Device bytes shared by the lanes of a server (thread-safe)."""
+
+    def __init__(self, total: float):
+        import threading
+        self.total = total
+        self.used = 0
+        self.lock = threading.Lock()
+
+    def reserve(self, nbytes: int, force: bool = False) -> bool:
+        with self.lock:
+            if force or self.used + nbytes <= self.total:
+                self.used += nbytes
+                return True
+            return False
+
+    def release(self, nbytes: int):
+        with self.lock:
+            self.used -= nbytes
+
+
+class Server:
+    """Serve `Request`s on one device with continuous batching (one lane:
+    one device context, one HIP stream)."""
+
+    def __init__(self, precision: str = "fp64", hbm_budget: float = 64 * 2 ** 30, chunk_seconds: float = 0.25,
+                 rc_scale: float = 0.225, dt: float = 1e-12, graph: bool = True, base: str = "c1", device: int = 0,
+                 budget: Budget | None = None):
+        self.cfg = capi.config_from_string(workload.config_text(base, precision=precision))
+        self.precision = precision
+        self.hbm_budget = hbm_budget
+        self.budget = budget
+        self.chunk_seconds = chunk_seconds
+        self.rc_scale = rc_scale
+        self.dt = dt
+        self.ctx = capi.Context(self.cfg, device)
+        self.ctx.set_tiling(0)
+        self.graph = graph
+        self.ctx.set_graph(graph)
+        self.active: list[Request] = []
+        self.done_steps: dict[int, int] = {}
+        self.step_time = None  # s per step of the active set, per particle
+        self.philox_on = False
+
+    def close(self):
+        self.ctx.close()
+
+    # ------------------------------------------------------------ requests
+    def params_of(self, req: Request) -> dict:
+        """Radii (monodisperse cores + shell of the base configuration), box
+        from the volume fraction and Ewald parameters from the reference's
+        error model at rc_frac_for(n)."""
+        if not req.ewald:
+            cfg = self.cfg
+            r = np.full(req.n, cfg.r_mag_mean)
+            rh = r + cfg.d_shell
+            L = (np.sum(4.0 / 3.0 * cfg.my_pi * rh ** 3) / req.phi) ** (1.0 / 3.0)
+            mu = cfg.sat_mag * 4.0 / 3.0 * cfg.my_pi * r ** 3
+            alpha, rc, nc = workload.ewald_for(cfg, mu, rh, L, rc_frac_for(req.n, self.rc_scale))
+            req.ewald = {"alpha": alpha, "r_cut": rc, "nc": nc, "l_box": L}
+        return req.ewald
+
+    def system_of(self, req: Request) -> dict:
+        """Initial suspension of a request: positions by random sequential
+        addition, orientations uniform on the sphere, seeded by the request."""
+        cfg = self.cfg
+        p = self.params_of(req)
+        r = np.full(req.n, cfg.r_mag_mean)
+        rh = r + cfg.d_shell
+        pos = capi.init_positions(cfg, rh, p["l_box"], req.seed, periodic=True)
+        mm, ea = capi.init_orientations(cfg, req.n, req.seed ^ 0x5BD1E995)
+        return dict(r_mag=r, r_hydr=rh, l_box=p["l_box"], field_z=req.field_z, seed=req.seed, pos=pos, mm=mm,
+                    ea=ea)
+
+    def footprint(self, req: Request) -> int:
+        return footprint_bytes(req.n, self.params_of(req)["nc"])
+
+    def admit(self, reqs: list[Request], now: float):
+        if not reqs:
+            return
+        systems = [self.system_of(q) for q in reqs]
+        first = self.ctx.add_replicas(systems)
+        for k, q in enumerate(reqs):
+            p = self.params_of(q)
+            self.ctx.replica_set_ewald(first + k, p["alpha"], p["r_cut"], p["nc"])
+            q.admitted = now
+            self.done_steps[q.rid] = 0
+            self.active.append(q)
+        if not self.philox_on:
+            # switch to in-kernel draws once: later replicas join with their
+            # own seed and a step counter of 0, without resetting the others
+            self.ctx.use_philox(reqs[0].seed, 0)
+            self.philox_on = True
+
+    def retire(self, now: float):
+        keep = []
+        for idx in reversed(range(len(self.active))):
+            q = self.active[idx]
+            if self.done_steps[q.rid] >= q.steps:
+                q.state = self.ctx.replica_state(idx)
+                q.mz = float(np.mean(q.state[1][:, 2]))
+                q.finished = now
+                self.ctx.remove_replica(idx)
+                if self.budget is not None:
+                    self.budget.release(self.footprint(q))
+            else:
+                keep.append(q)
+        self.active = list(reversed(keep))
+
+    # ------------------------------------------------------------ loop
+    def serve(self, requests: list[Request], clock=time.perf_counter, log=None, t0: float | None = None) -> dict:
+        """Run until every request is finished. Arrivals are wall-clock
+        offsets from t0 (default: now). Returns throughput and latency
+        statistics."""
+        pending = sorted(requests, key=lambda q: q.arrival)
+        t0 = clock() if t0 is None else t0
+        total_ps = 0
+        chunks = 0
+        used = 0
+        while pending or self.active:
+            now = clock() - t0
+            # admit arrived requests while they fit
+            batch = []
+            used = sum(self.footprint(q) for q in self.active)
+            while pending and pending[0].arrival <= now:
+                f = self.footprint(pending[0])
+                alone = not (self.active or batch)
+                if self.budget is not None:
+                    # a lane with nothing active may always take one request
+                    if not self.budget.reserve(f, force=alone):
+                        break
+                elif used + f > self.hbm_budget and not alone:
+                    break
+                used += f
+                batch.append(pending.pop(0))
+            if batch:
+                self.admit(batch, now)
+            if not self.active:
+                # idle until the next arrival (or until another lane frees memory)
+                wait = pending[0].arrival - (clock() - t0)
+                time.sleep(wait if wait > 0 else 0.002)
+                continue
+            # a chunk: no request beyond its step count, about chunk_seconds long
+            remaining = min(q.steps - self.done_steps[q.rid] for q in self.active)
+            n_act = sum(q.n for q in self.active)
+            if self.step_time:
+                s = max(1, int(self.chunk_seconds / (self.step_time * n_act)))
+            else:
+                s = 2
+            s = min(s, remaining)
+            c0 = clock()
+            # the clock argument only matters for a single replica (several keep theirs on the device)
+            self.ctx.run(s, self.done_steps[self.active[0].rid] * self.dt, self.dt)
+            self.ctx.synchronize()
+            el = clock() - c0
+            self.step_time = el / (s * n_act)
+            total_ps += s * n_act
+            chunks += 1
+            for q in self.active:
+                self.done_steps[q.rid] += s
+            self.retire(clock() - t0)
+            if log:
+                log(f"t={clock() - t0:8.3f}s chunk {chunks}: {s} steps x {len(self.active)} left, "
+                    f"{n_act} particles, {el / s * 1e3:.2f} ms/step, queue {len(pending)}")
+        wall = clock() - t0
+        return summary(requests, wall, total_ps, chunks)
+
+
+def summary(requests: list[Request], wall: float, particle_steps: int, chunks: int) -> dict:
+    lat = np.array([q.latency for q in requests]) if requests else np.zeros(1)
+    return {"requests": len(requests), "wall_s": wall, "particle_steps": particle_steps,
+            "throughput": particle_steps / wall if wall > 0 else 0.0, "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "chunks": chunks}
+
+
+class LaneServer:
+    """Requests routed by size to lanes (one device context, HIP stream and
+    host thread each), all sharing one HBM budget. Within a lane requests are
+    batched continuously; lanes run concurrently on the device, so a small
+    request does not advance at the step rate of the largest ones. Chunks are
+    shorter in lanes of smaller requests (admission happens at chunk
+    boundaries)."""
+
+    def __init__(self, precision: str = "fp64", hbm_budget: float = 64 * 2 ** 30,
+                 bounds: tuple = (6e3, 4e4), chunk_seconds: tuple = (0.01, 0.04, 0.2), **kw):
+        assert len(chunk_seconds) == len(bounds) + 1
+        self.bounds = bounds
+        self.budget = Budget(hbm_budget)
+        self.lanes = [Server(precision=precision, hbm_budget=hbm_budget, chunk_seconds=cs, budget=self.budget, **kw)
+                      for cs in chunk_seconds]
+
+    def lane_of(self, req: Request) -> int:
+        k = 0
+        while k < len(self.bounds) and req.n >= self.bounds[k]:
+            k += 1
+        return k
+
+    def close(self):
+        for s in self.lanes:
+            s.close()
+
+    def serve(self, requests: list[Request], clock=time.perf_counter, log=None) -> dict:
+        import threading
+        parts = [[] for _ in self.lanes]
+        for q in requests:
+            parts[self.lane_of(q)].append(q)
+        out = [None] * len(self.lanes)
+        errs = []
+        t0 = clock()
+
+        def work(k):
+            try:
+                out[k] = self.lanes[k].serve(parts[k], clock, log, t0) if parts[k] else summary([], 0.0, 0, 0)
+            except Exception as e:  # surfaced below
+                errs.append(e)
+
+        th = [threading.Thread(target=work, args=(k,)) for k in range(len(self.lanes))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        if errs:
+            raise errs[0]
+        wall = clock() - t0
+        res = summary(requests, wall, sum(o["particle_steps"] for o in out), sum(o["chunks"] for o in out))
+        res["lanes"] = [{"requests": len(p), "chunks": o["chunks"]} for p, o in zip(parts, out)]
+        return res
+
+
+def isolated_seconds(server_kwargs: dict, req: Request, probe_steps: int = 5) -> float:
+    """Wall time of `req` solved alone (its own context, same settings):
+    probe_steps timed after one warm-up step, scaled to req.steps."""
+    s = Server(**server_kwargs)
+    try:
+        q = Request(req.rid, req.n, probe_steps + 1, req.field_z, req.phi, req.seed)
+        s.admit([q], 0.0)
+        s.ctx.run(1, 0.0, s.dt)
+        s.ctx.synchronize()
+        t0 = time.perf_counter()
+        s.ctx.run(probe_steps, 0.0, s.dt)
+        s.ctx.synchronize()
+        per_step = (time.perf_counter() - t0) / probe_steps
+    finally:
+        s.close()
+    return per_step * req.steps
+
+
+def poisson_stream(n_requests: int, rate: float, rng: np.random.Generator, n_range=(1e3, 2e5),
+                   steps_range=(1e3, 1e5), field_range=(0.0, 0.1), phi_range=(0.01, 0.10)) -> list[Request]:
+    """Requests with Poisson arrivals at `rate` per second, N and steps
+    log-uniform, field and volume fraction uniform, per-request seeds."""
+    t = 0.0
+    out = []
+    for k in range(n_requests):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        steps = int(round(math.exp(rng.uniform(math.log(steps_range[0]), math.log(steps_range[1])))))
+        out.append(Request(k, n, steps, float(rng.uniform(*field_range)), float(rng.uniform(*phi_range)),
+                           int(rng.integers(1, 2 ** 62)), t))
+    return out

@@ -1,0 +1,85 @@
+"""Continuous-batching request loop: each request's result is the same when
+it is served in a mix (admitted and retired at chunk boundaries, beside
+other requests, under an HBM budget that forces queueing) as when it is
+solved alone. Tolerance 1e-12 relative (only the order of atomics in the
+overlap-pair kernel and cell grids can differ)."""
+import copy
+
+import numpy as np
+import pytest
+
+from md_sim_wolfschwenger_amd import serving
+from refcfg import rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def requests():
+    # (N, steps, field T, phi, seed, arrival s)
+    spec = [(800, 9, 0.02, 0.03, 11, 0.0), (1500, 5, 0.0, 0.08, 22, 0.0), (600, 7, 0.09, 0.05, 33, 0.02),
+            (1100, 4, 0.05, 0.10, 44, 0.04)]
+    return [serving.Request(k, n, st, f, phi, sd, arr) for k, (n, st, f, phi, sd, arr) in enumerate(spec)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_each_result_same_alone_and_in_mix(precision):
+    print(f"serving mix {precision}", flush=True)
+    reqs = requests()
+    srv = serving.Server(precision=precision, chunk_seconds=0.0)
+    # a budget below any three footprints: at most two requests run together
+    fp = sorted(srv.footprint(q) for q in reqs)
+    srv.hbm_budget = fp[0] + fp[1] + fp[2] - 1
+    stats = srv.serve(reqs)
+    srv.close()
+    assert stats["requests"] == 4 and stats["chunks"] >= 3
+    # at most two requests were ever active together
+    events = sorted([(q.admitted, 1) for q in reqs] + [(q.finished, -1) for q in reqs], key=lambda e: (e[0], e[1]))
+    act = peak = 0
+    for _, d in events:
+        act += d
+        peak = max(peak, act)
+    assert peak <= 2
+    for q in reqs:
+        alone = copy.deepcopy(q)
+        alone.state = None
+        alone.ewald = {}
+        solo = serving.Server(precision=precision, chunk_seconds=10.0)
+        solo.serve([alone])
+        solo.close()
+        assert alone.ewald == q.ewald
+        for a, b in zip(q.state, alone.state):
+            assert rel_err(a, b) < TOL, q.rid
+        assert q.mz == pytest.approx(alone.mz, rel=1e-11, abs=1e-14)
+        assert q.finished >= q.arrival
+
+
+def test_lanes_each_result_same_alone():
+    """Size-class lanes (one context, stream and host thread each) sharing
+    one HBM budget: results equal the alone solves."""
+    print("serving lanes", flush=True)
+    reqs = requests()
+    ls = serving.LaneServer(precision="fp64", bounds=(700, 1200), chunk_seconds=(0.0, 0.0, 0.0))
+    stats = ls.serve(reqs)
+    ls.close()
+    assert [l["requests"] for l in stats["lanes"]] == [1, 2, 1]
+    for q in reqs:
+        alone = copy.deepcopy(q)
+        alone.state = None
+        alone.ewald = {}
+        solo = serving.Server(precision="fp64", chunk_seconds=10.0)
+        solo.serve([alone])
+        solo.close()
+        for a, b in zip(q.state, alone.state):
+            assert rel_err(a, b) < TOL, q.rid
+
+
+def test_footprint_estimate_covers_device_bytes():
+    srv = serving.Server(precision="fp64")
+    reqs = requests()
+    srv.admit(reqs, 0.0)
+    srv.ctx.run(1, 0.0, 1e-12)
+    used = srv.ctx.device_bytes()
+    est = sum(srv.footprint(q) for q in reqs)
+    srv.close()
+    assert est >= used * 0.9, (est, used)
