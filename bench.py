@@ -89,6 +89,33 @@ def tune(ctx, info, args):
     return res
 
 
+# kernel class -> kernel symbol (short form) per path, for the committed
+# rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+SYMBOL = {
+    ("pairs_force", "fp64"): "k_pairs<true,false,false>", ("pairs_force", "mixed"): "k_pairs<true,false,true>",
+    ("pairs_field", "fp64"): "k_pairs<false,false,false>", ("pairs_field", "mixed"): "k_pairs<false,false,true>",
+    ("kbuild", "fp64"): "k_kbuild<8>", ("kbuild", "mixed"): "k_kbuild<8>",
+    ("kscatter_force", "fp64"): "k_kscatter<true>", ("kscatter_force", "mixed"): "k_kscatter<true>",
+    ("kscatter_field", "fp64"): "k_kscatter<false>", ("kscatter_field", "mixed"): "k_kscatter<false>",
+}
+PROFILE = os.path.join(ROOT, "profiles", "c3_kernels_mi355x.md")
+
+
+def profiled_traffic(cls, prec):
+    """HBM bytes per dispatch (read + written) of a kernel class from the
+    committed profiler summary (counters scaled by a copy-kernel reading), or
+    None when the summary does not list it."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == SYMBOL.get((cls, prec)):
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def roofline(stats, counts, n, nk, steps, prec):
     """Dominant kernel class, its algorithmic flops per launch over its
     event-timed average duration."""
@@ -112,7 +139,8 @@ def roofline(stats, counts, n, nk, steps, prec):
     unit = ("FP64 vector ALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)" if prec == "fp64" else
             "FP32 vector ALU (packed FP32 disabled; 157.3 TFLOP/s is the packed peak)")
     return {"kernel": dom, "bound": "mfma", "compute_unit": unit + "; no dense contraction, MFMA unused",
-            "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+            "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": profiled_traffic(dom, prec), "traffic_source": os.path.relpath(PROFILE, ROOT),
             "avg_launch_ms": avg_s * 1e3, "flop_per_launch": per_launch[dom],
             "unique_pairs_per_launch": pairs_per_launch, "kvecs": nk}, breakdown
 
