@@ -213,7 +213,7 @@ struct mds_ctx {
     // per-step scratch
     DevBuf<double> pos0, pos_cur, force, field, mom;
     DevBuf<double4> rec_pos, rec_mom;
-    DevBuf<float4> rec_pos32;
+    DevBuf<float4> rec_pos32, rec_mom32;
     DevBuf<int32_t> rep_of;
     // replica table and per-replica device values
     DevBuf<Rep> reps;
@@ -234,12 +234,19 @@ struct mds_ctx {
     // k space
     std::vector<KRow> h_rows;
     std::vector<KSeg> h_segs;
+    std::vector<KPack> h_pack;
     int32_t nk_all = 0, kb_items = 0;
     DevBuf<KRow> rows;
     DevBuf<KSeg> segs;
-    DevBuf<double> ek, sk, kpartial, tab;
+    DevBuf<KPack> pack_desc;
+    DevBuf<double4> pack;
+    DevBuf<double> kpartial, tab;
     // reductions
     DevBuf<double> press_partial, err_partial, scal;  // scal[0] = error norm
+    DevBuf<int32_t> scan_tmp;
+    std::vector<int2> h_chunks;  // reduction chunks (replica, first particle)
+    DevBuf<int2> red_chunks;
+    DevBuf<double> red_part;
     DevBuf<uint64_t> pair_counter;
     DevBuf<int32_t> series_counter;
     DevBuf<double> series;
@@ -412,12 +419,15 @@ void plan_grid(mds_ctx* c, const RepMeta& m, Rep& R) {
 // (n_y > 0, or n_y = 0 and n_x > 0), |n|^2 <= n_c^2; with the reference's
 // full n_z = 0 plane at weight 1/2 (src/interaction.cpp:438-457) this is the
 // same sum.
-void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R, std::vector<double>& ek) {
+void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
     const int nc = m.nc;
     const int64_t nc2 = (int64_t)nc * nc;
     R.row_off = (int32_t)c->h_rows.size();
     R.seg_off = (int32_t)c->h_segs.size();
+    R.pk_off = (int32_t)c->h_pack.size();
     const double w = std::pow(c->cfg.my_pi / (m.L * m.alpha), 2.0);
+    // e(k) = 2 exp(-w n^2) / n^2 (src/interaction.cpp:450-452)
+    auto ek = [&](int64_t len) { return len > 0 ? 2.0 * std::exp(-w * (double)len) / (double)len : 0.0; };
     int32_t koff = 0;
     for (int nz = 0; nz <= nc; ++nz)
         for (int ny = (nz == 0 ? 0 : -nc); ny <= nc; ++ny) {
@@ -427,19 +437,28 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R, std::vector<double>& ek) 
             while ((int64_t)(nxmax + 1) * (nxmax + 1) + r2 <= nc2) ++nxmax;
             const bool half = (ny == 0 && nz == 0);
             if (half && nxmax == 0) continue;
-            c->h_rows.push_back({ny, nz, nxmax, koff});
+            c->h_rows.push_back({ny, nz, nxmax, (int32_t)c->h_pack.size() - R.pk_off});
             const int a0 = half ? 1 : 0;
             for (int a = a0; a <= nxmax; a += KSEG)
                 c->h_segs.push_back({ny, nz, a, std::min(nxmax, a + KSEG - 1), koff, nxmax});
-            const int lo = half ? 1 : -nxmax;
-            for (int nx = lo; nx <= nxmax; ++nx) {
-                const double len = (double)nx * nx + (double)r2;
-                ek.push_back(2.0 * std::exp(-w * len) / len);
+            // scatter table: |nx| = a pairs (+a, ny, nz) with (-a, ny, nz)
+            for (int a = 0; a <= nxmax; ++a) {
+                KPack d;
+                if (half) {
+                    d.kp = a >= 1 ? koff + a - 1 : -1;
+                    d.km = -1;
+                } else {
+                    d.kp = koff + a + nxmax;
+                    d.km = a >= 1 ? koff - a + nxmax : -1;
+                }
+                d.e = d.kp >= 0 ? ek((int64_t)a * a + r2) : 0.0;
+                c->h_pack.push_back(d);
             }
             koff += half ? nxmax : 2 * nxmax + 1;
         }
     R.nrows = (int32_t)c->h_rows.size() - R.row_off;
     R.nseg = (int32_t)c->h_segs.size() - R.seg_off;
+    R.npk = (int32_t)c->h_pack.size() - R.pk_off;
     R.nk = koff;
     R.nc = nc;
     const double pi = c->cfg.my_pi, f = c->cfg.fac_ewald, L = m.L;
@@ -491,8 +510,9 @@ mds_status plan_all(mds_ctx* c) {
     c->h_sten.clear();
     c->h_rows.clear();
     c->h_segs.clear();
-    std::vector<double> ek;
+    c->h_pack.clear();
     std::vector<int32_t> cell_rep, rep_of((size_t)c->n);
+    c->h_chunks.clear();
     int64_t cells = 0, fcells = 0, work_cap = 0, tab = 0, part = 0, kb = 0, kall = 0;
     for (int r = 0; r < c->nrep; ++r) {
         const RepMeta& m = c->meta[r];
@@ -504,7 +524,7 @@ mds_status plan_all(mds_ctx* c) {
         R.seed = m.seed;
         plan_grid(c, m, R);
         plan_ewald(c, m, R);
-        plan_kspace(c, m, R, ek);
+        plan_kspace(c, m, R);
         const int64_t ncm = (int64_t)R.ncell * R.ncell * R.ncell, ncf = (int64_t)R.fncell * R.fncell * R.fncell;
         R.cell_off = (int32_t)cells;
         R.fcell_off = (int32_t)fcells;
@@ -521,6 +541,9 @@ mds_status plan_all(mds_ctx* c) {
         R.kb_off = (int32_t)kb;
         kb += (R.nk > 0) ? (int64_t)R.nsegw * R.nchunks : 0;
         std::fill(rep_of.begin() + m.off, rep_of.begin() + m.off + m.n, r);
+        R.red_off = (int32_t)c->h_chunks.size();
+        for (int64_t b = m.off; b < m.off + m.n; b += RED_CHUNK_PARTICLES) c->h_chunks.push_back(make_int2(r, (int)b));
+        R.nred = (int32_t)c->h_chunks.size() - R.red_off;
     }
     // work items pack (cell << 8) | chunk into an int32
     if (cells >= (1 << 23) || fcells >= (1 << 30) || work_cap >= (1 << 30) || kb >= (1 << 30))
@@ -550,8 +573,8 @@ mds_status plan_all(mds_ctx* c) {
     HIPCHK(c->stencil.ensure(std::max<size_t>(1, c->h_sten.size())));
     HIPCHK(c->rows.ensure(std::max<size_t>(1, c->h_rows.size())));
     HIPCHK(c->segs.ensure(std::max<size_t>(1, c->h_segs.size())));
-    HIPCHK(c->ek.ensure(std::max<int64_t>(1, kall)));
-    HIPCHK(c->sk.ensure(2 * (size_t)std::max<int64_t>(1, kall)));
+    HIPCHK(c->pack_desc.ensure(std::max<size_t>(1, c->h_pack.size())));
+    HIPCHK(c->pack.ensure(c->h_pack.size() + 1));  // + 1: the scatter loop reads one entry ahead
     HIPCHK(c->kpartial.ensure(std::max<int64_t>(1, part)));
     HIPCHK(c->tab.ensure(std::max<int64_t>(1, tab)));
     HIPCHK(c->reps.ensure(c->nrep));
@@ -559,11 +582,16 @@ mds_status plan_all(mds_ctx* c) {
     HIPCHK(c->msum.ensure(3 * (size_t)c->nrep));
     HIPCHK(c->mz_out.ensure(c->nrep));
     HIPCHK(c->series.ensure((size_t)kSeriesCap * c->nrep));
+    HIPCHK(c->scan_tmp.ensure(scan_tiles(std::max<int64_t>({cells, fcells, c->n})) + 1));
+    HIPCHK(c->red_chunks.ensure(c->h_chunks.size()));
+    HIPCHK(c->red_part.ensure(3 * c->h_chunks.size()));
     hipStream_t s = c->stream;
     HIPCHK(hipMemcpyAsync(c->reps.p, c->h_rep.data(), c->nrep * sizeof(Rep), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->rep_of.p, rep_of.data(), rep_of.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->cell_rep.p, cell_rep.data(), cell_rep.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                           s));
+    HIPCHK(hipMemcpyAsync(c->red_chunks.p, c->h_chunks.data(), c->h_chunks.size() * sizeof(int2),
+                          hipMemcpyHostToDevice, s));
     if (!c->h_sten.empty())
         HIPCHK(hipMemcpyAsync(c->stencil.p, c->h_sten.data(), c->h_sten.size() * sizeof(int4), hipMemcpyHostToDevice,
                               s));
@@ -571,7 +599,9 @@ mds_status plan_all(mds_ctx* c) {
         HIPCHK(hipMemcpyAsync(c->rows.p, c->h_rows.data(), c->h_rows.size() * sizeof(KRow), hipMemcpyHostToDevice, s));
     if (!c->h_segs.empty())
         HIPCHK(hipMemcpyAsync(c->segs.p, c->h_segs.data(), c->h_segs.size() * sizeof(KSeg), hipMemcpyHostToDevice, s));
-    if (!ek.empty()) HIPCHK(hipMemcpyAsync(c->ek.p, ek.data(), ek.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    if (!c->h_pack.empty())
+        HIPCHK(hipMemcpyAsync(c->pack_desc.p, c->h_pack.data(), c->h_pack.size() * sizeof(KPack),
+                              hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
     c->planned = true;
     c->geometry_ready = false;
@@ -591,6 +621,7 @@ mds_status size_scratch(mds_ctx* c) {
     HIPCHK(c->rec_pos.ensure(n));
     HIPCHK(c->rec_pos32.ensure(n));
     HIPCHK(c->rec_mom.ensure(n));
+    HIPCHK(c->rec_mom32.ensure(n));
     for (auto* b : {&c->key_main, &c->key_fine, &c->sorted_main, &c->sorted_fine, &c->inv_perm, &c->cand_count,
                     &c->ev_head, &c->ev_tail})
         HIPCHK(b->ensure(n));
@@ -632,6 +663,7 @@ Geometry geometry(mds_ctx* c) {
     g.n_moved = c->n_moved.p;
     g.inv_perm = c->inv_perm.p;
     g.rec_pos32 = c->rec_pos32.p;
+    g.rec_mom32 = c->rec_mom32.p;
     return g;
 }
 
@@ -660,7 +692,7 @@ mds_status prepare_geometry_once(mds_ctx* c) {
         HIPCHK(launch_prepare(s, n, c->pos.p, c->pos0.p, c->pos_cur.p, c->reps.p, c->rep_of.p, c->key_main.p,
                               c->key_fine.p));
         HIPCHK(launch_cell_sort(s, n, c->key_main.p, c->cells_main, c->cnt_main.p, c->start_main.p, c->end_main.p,
-                                c->cur_main.p, c->sorted_main.p));
+                                c->cur_main.p, c->sorted_main.p, c->scan_tmp.p));
         HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->r_mag.p, c->rec_pos.p, c->rec_pos32.p,
                                     c->inv_perm.p));
         HIPCHK(launch_build_work(s, c->cells_main, c->start_main.p, c->end_main.p, c->work.p, c->nwork.p,
@@ -669,11 +701,11 @@ mds_status prepare_geometry_once(mds_ctx* c) {
     {
         mds_ctx::Span sp(c, KC_CONTACTS);
         HIPCHK(launch_cell_sort(s, n, c->key_fine.p, c->cells_fine, c->cnt_fine.p, c->start_fine.p, c->end_fine.p,
-                                c->cur_fine.p, c->sorted_fine.p));
+                                c->cur_fine.p, c->sorted_fine.p, c->scan_tmp.p));
         HIPCHK(launch_clear_events(s, n, c->ev_head.p, c->ev_tail.p, c->n_events.p, c->n_moved.p, c->max_disp.p));
         HIPCHK(launch_contacts(s, n, c->reps.p, c->rep_of.p, c->pos0.p, c->r_mag.p, c->start_fine.p, c->end_fine.p,
                                c->sorted_fine.p, c->cand_count.p, c->cand_off.p, c->cand_j.p, c->cand_cap,
-                               c->err_flag.p));
+                               c->err_flag.p, c->scan_tmp.p));
         // displacement bound: half the smallest contact-search margin
         double dl = 1e300;
         for (const Rep& R : c->h_rep) dl = std::min(dl, 0.5 * R.margin);
@@ -734,12 +766,13 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
         }
         {
             mds_ctx::Span sp(c, KC_KREDUCE);
-            HIPCHK(launch_kreduce(s, c->nk_all, c->reps.p, c->nrep, c->kpartial.p, c->sk.p));
+            HIPCHK(launch_kpack(s, (int32_t)c->h_pack.size(), c->reps.p, c->nrep, c->pack_desc.p, c->kpartial.p,
+                                c->pack.p));
         }
     }
     {
         mds_ctx::Span sp(c, force_pass ? KC_KSCATTER_FORCE : KC_KSCATTER_FIELD);
-        HIPCHK(launch_kscatter(s, force_pass, n, c->reps.p, c->rep_of.p, c->rows.p, c->ek.p, c->sk.p, c->tab.p,
+        HIPCHK(launch_kscatter(s, force_pass, n, c->reps.p, c->rep_of.p, c->rows.p, c->pack.p, c->tab.p,
                                c->mom.p, c->msum.p, force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
     }
     return MDS_OK;
@@ -747,8 +780,10 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
 
 mds_status moments_of(mds_ctx* c, const double* mm_dev) {
     mds_ctx::Span sp(c, KC_MOMENTS);
-    HIPCHK(launch_moments(c->stream, c->n, mm_dev, c->mag_mom.p, c->inv_perm.p, c->mom.p, c->rec_mom.p));
-    HIPCHK(launch_rep_sum3(c->stream, c->reps.p, c->nrep, c->n, c->mom.p, c->msum.p));
+    HIPCHK(launch_moments(c->stream, c->n, mm_dev, c->mag_mom.p, c->inv_perm.p, c->reps.p, c->rep_of.p, c->mom.p,
+                          c->rec_mom.p, c->rec_mom32.p));
+    HIPCHK(launch_rep_sum3(c->stream, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), c->n,
+                           c->mom.p, c->red_part.p, c->msum.p));
     return MDS_OK;
 }
 
@@ -768,6 +803,7 @@ StageParams stage_params(mds_ctx* c, double h) {
     sp.t_mag = c->cfg.t_mag;
     sp.reps = c->reps.p;
     sp.rep_of = c->rep_of.p;
+    sp.rec_mom32 = c->rec_mom32.p;
     return sp;
 }
 
@@ -820,7 +856,8 @@ mds_status fixed_step_body(mds_ctx* c, double h, bool check) {
         const double* ye = sgi == 0 ? c->ea.p : c->pt[(sgi - 1) & 1].p + n3;
         double* yo = c->pt[sgi & 1].p;
         if (sgi > 0) {
-            HIPCHK(launch_rep_sum3(s, c->reps.p, c->nrep, n, c->mom.p, c->msum.p));
+            HIPCHK(launch_rep_sum3(s, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), n, c->mom.p,
+                                   c->red_part.p, c->msum.p));
             if ((st = evaluate(c, false)) != MDS_OK) return st;
         }
         mds_ctx::Span spn(c, KC_STAGE);
@@ -833,8 +870,8 @@ mds_status fixed_step_body(mds_ctx* c, double h, bool check) {
         mds_ctx::Span sp2(c, KC_TRANSLATE);
         HIPCHK(launch_translate(s, n, c->reps.p, c->rep_of.p, h, c->pos_cur.p, c->force.p, c->zeta_trans.p, c->pos.p,
                                 c->new_mm.p, c->new_ea.p, c->mm.p, c->ea.p));
-        HIPCHK(launch_rep_mean_z(s, c->reps.p, c->nrep, n, c->mm.p, c->mz_out.p, c->series_counter.p, c->series.p,
-                                 kSeriesCap));
+        HIPCHK(launch_rep_mean_z(s, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), n, c->mm.p,
+                                 c->red_part.p, c->mz_out.p, c->series_counter.p, c->series.p, kSeriesCap));
         HIPCHK(launch_step_count(s, c->nrep, c->rep_step.p, c->rep_t.p, h, c->series_counter.p));
     }
     return MDS_OK;
@@ -872,7 +909,8 @@ mds_status adaptive_step(mds_ctx* c, double* t, double* dt, double* dt_used) {
             const double* ye = sgi == 0 ? c->ea.p : c->pt[(sgi - 1) & 1].p + n3;
             double* yo = c->pt[sgi & 1].p;
             if (sgi > 0) {
-                HIPCHK(launch_rep_sum3(s, c->reps.p, c->nrep, n, c->mom.p, c->msum.p));
+                HIPCHK(launch_rep_sum3(s, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), n, c->mom.p,
+                                   c->red_part.p, c->msum.p));
                 if ((st = evaluate(c, false)) != MDS_OK) return st;
             }
             mds_ctx::Span spn(c, KC_STAGE);
@@ -913,8 +951,8 @@ mds_status adaptive_step(mds_ctx* c, double* t, double* dt, double* dt_used) {
         // accepted, renormalised state -> mm/ea; IntegrationTrans + wrap; <m_z>
         HIPCHK(launch_translate(s, n, c->reps.p, c->rep_of.p, h, c->pos_cur.p, c->force.p, c->zeta_trans.p, c->pos.p,
                                 c->new_mm.p, c->new_ea.p, c->mm.p, c->ea.p));
-        HIPCHK(launch_rep_mean_z(s, c->reps.p, c->nrep, n, c->mm.p, c->mz_out.p, c->series_counter.p, c->series.p,
-                                 kSeriesCap));
+        HIPCHK(launch_rep_mean_z(s, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), n, c->mm.p,
+                                 c->red_part.p, c->mz_out.p, c->series_counter.p, c->series.p, kSeriesCap));
         HIPCHK(launch_step_count(s, c->nrep, c->rep_step.p, c->rep_t.p, h, c->series_counter.p));
     }
     *t += h;
@@ -1692,7 +1730,7 @@ int64_t mds_device_bytes(const mds_ctx* c) {
     auto add = [&](size_t cnt, size_t sz) { b += (int64_t)(cnt * sz); };
     for (auto* d : {&c->r_mag, &c->mag_mom, &c->zeta_rot, &c->zeta_trans, &c->vol_mag, &c->c1, &c->c2, &c->pos, &c->mm,
                     &c->ea, &c->n_t, &c->n_b, &c->n_f, &c->pos0, &c->pos_cur, &c->force, &c->field, &c->mom, &c->kmm,
-                    &c->kea, &c->pt[0], &c->pt[1], &c->new_mm, &c->new_ea, &c->ev_pos, &c->ek, &c->sk, &c->kpartial,
+                    &c->kea, &c->pt[0], &c->pt[1], &c->new_mm, &c->new_ea, &c->ev_pos, &c->kpartial,
                     &c->tab, &c->press_partial, &c->series, &c->msum, &c->mz_out, &c->rep_t, &c->err_partial})
         add(d->n, sizeof(double));
     for (auto* d : {&c->rep_of, &c->cell_rep, &c->key_main, &c->key_fine, &c->cnt_main, &c->start_main, &c->end_main,
@@ -1701,11 +1739,16 @@ int64_t mds_device_bytes(const mds_ctx* c) {
                     &c->ev_next, &c->moved_list})
         add(d->n, sizeof(int32_t));
     add(c->rec_pos.n + c->rec_mom.n, sizeof(double4));
-    add(c->rec_pos32.n, sizeof(float4));
+    add(c->rec_pos32.n + c->rec_mom32.n, sizeof(float4));
     add(c->ev_key.n + c->rep_step.n, sizeof(uint64_t));
     add(c->stencil.n, sizeof(int4));
     add(c->rows.n, sizeof(KRow));
     add(c->segs.n, sizeof(KSeg));
+    add(c->pack_desc.n, sizeof(KPack));
+    add(c->scan_tmp.n, sizeof(int32_t));
+    add(c->red_chunks.n, sizeof(int2));
+    add(c->red_part.n, sizeof(double));
+    add(c->pack.n, sizeof(double4));
     add(c->reps.n, sizeof(Rep));
     return b;
 }

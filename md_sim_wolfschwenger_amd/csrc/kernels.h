@@ -7,10 +7,21 @@
 namespace mds {
 
 // One k-space row: all n = (nx, ny, nz) with the same (ny, nz), nx in
-// [-nxmax, nxmax] (nx in [1, nxmax] for the ny = nz = 0 row). koff is the
-// index of the row's first k-vector.
+// [-nxmax, nxmax] (nx in [1, nxmax] for the ny = nz = 0 row). poff is the
+// index of the row's first entry in the packed scatter table (one entry per
+// |nx| = 0..nxmax).
 struct KRow {
-    int32_t ny, nz, nxmax, koff;
+    int32_t ny, nz, nxmax, poff;
+};
+
+// Packed scatter table entry for the k pair n = (+a, ny, nz), (-a, ny, nz):
+// the indices of both in the S(k) arrays (-1: absent, as for a = 0's mirror
+// or the ny = nz = 0 row's negative side) and e(k) = 2 exp(-w n^2) / n^2
+// (the same for both). k_kpack turns it into (U+ + U-, U+ - U-, V+ + V-,
+// V+ - V-) with U = e Re S, V = e Im S.
+struct KPack {
+    int32_t kp, km;
+    double e;
 };
 
 // A segment of a k-space row for the structure-factor build: |nx| in
@@ -32,6 +43,8 @@ struct Rep {
     double ell, inv_ell, inv_mref, bscale, fscale, pscale;  // units of the FP32 pair path
     float al_s, al2_s, al4_s, pad_;
     int32_t nc, row_off, nrows, seg_off, nseg, k_off, nk, kchunk, nchunks, kb_off, nsegw, pad2_;
+    int32_t pk_off, npk;         // slice of the packed scatter table
+    int32_t red_off, nred;       // slice of the reduction chunk table
     int64_t tab_off, part_off;   // offsets (doubles) into the sin/cos tables and S(k) partials
     double c_force, c_flux, c_self, c_surf, two_pi_over_L;
     double field_z;              // magFluxDens of this replica
@@ -65,7 +78,8 @@ struct Geometry {
     const int32_t* moved_list;   // original indices of moved particles
     const int32_t* n_moved;      // device scalar
     const int32_t* inv_perm;     // sorted slot of an original index
-    const float4* rec_pos32;     // FP32 copy of rec_pos (candidate search of the mixed path)
+    const float4* rec_pos32;     // FP32 copy of rec_pos (candidate search)
+    const float4* rec_mom32;     // FP32 moments / mref of the replica, sorted order (mixed path)
 };
 
 struct PairOut {
@@ -95,8 +109,10 @@ hipError_t launch_scale(hipStream_t s, double* p, int64_t count, double f);
 // step setup: step-start positions and cell keys (global cell ids)
 hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* pos0, double* pos_cur, const Rep* reps,
                           const int32_t* rep_of, int32_t* key_main, int32_t* key_fine);
+// scan_tmp of scan_tiles(cells) ints
+int32_t scan_tiles(int64_t m);
 hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
-                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx);
+                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp);
 hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_idx, const double* pos0,
                                 const double* r_mag, double4* rec_pos, float4* rec_pos32, int32_t* inv_perm);
 hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
@@ -104,7 +120,8 @@ hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start
 // contact search + sequential overlap resolution
 hipError_t launch_contacts(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
                            const double* r_mag, const int32_t* fstart, const int32_t* fend, const int32_t* fsorted,
-                           int32_t* cand_count, int32_t* cand_off, int32_t* cand_j, int32_t cand_cap, int32_t* err);
+                           int32_t* cand_count, int32_t* cand_off, int32_t* cand_j, int32_t cand_cap, int32_t* err,
+                           int32_t* scan_tmp);
 // one thread per replica walks its candidates in order
 hipError_t launch_resolve(hipStream_t s, int32_t nrep, int64_t n, const Rep* reps, const double* r_mag,
                           const int32_t* cand_off, const int32_t* cand_j, double* pos_cur, const double* pos0,
@@ -117,10 +134,15 @@ hipError_t launch_clear_events(hipStream_t s, int64_t n, int32_t* ev_head, int32
 
 // moments m_i = mm_i * mu_i (original and sorted order); per-replica sums
 hipError_t launch_moments(hipStream_t s, int64_t n, const double* mm, const double* mag_mom, const int32_t* inv_perm,
-                          double* mom, double4* rec_mom);
-hipError_t launch_rep_sum3(hipStream_t s, const Rep* reps, int32_t nrep, int64_t n, const double* v, double* out);
-hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, int64_t n, const double* v, double* out,
-                             const int32_t* counter, double* series, int32_t series_cap);
+                          const Rep* reps, const int32_t* rep_of, double* mom, double4* rec_mom, float4* rec_mom32);
+// chunks: (replica, first particle) of RED_CHUNK-particle pieces that never
+// straddle a replica; part: 3 doubles per chunk
+constexpr int RED_CHUNK_PARTICLES = 2048;
+hipError_t launch_rep_sum3(hipStream_t s, const Rep* reps, int32_t nrep, const int2* chunks, int32_t nch, int64_t n,
+                           const double* v, double* part, double* out);
+hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, const int2* chunks, int32_t nch, int64_t n,
+                             const double* v, double* part, double* out, const int32_t* counter, double* series,
+                             int32_t series_cap);
 
 // real space: one block per work item (max_work blocks, surplus blocks exit)
 hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
@@ -137,10 +159,11 @@ hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const
                                 double* tab);
 hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t kb_items, const KSeg* segs,
                          const double* tab, const double* mom, double* partial);
-hipError_t launch_kreduce(hipStream_t s, int32_t nk_all, const Rep* reps, int32_t nrep, const double* partial,
-                          double* sk);
+// S(k) partials -> packed scatter table (fixed summation order)
+hipError_t launch_kpack(hipStream_t s, int32_t npk_all, const Rep* reps, int32_t nrep, const KPack* desc,
+                        const double* partial, double4* pack);
 hipError_t launch_kscatter(hipStream_t s, bool force, int64_t n, const Rep* reps, const int32_t* rep_of,
-                           const KRow* rows, const double* ek, const double* sk, const double* tab, const double* mom,
+                           const KRow* rows, const double4* pack, const double* tab, const double* mom,
                            const double* msum, const double* add_force, double* force_out, double* field);
 
 // integrator
@@ -158,6 +181,7 @@ struct StageParams {
     double t_mag;
     const Rep* reps;
     const int32_t* rep_of;
+    float4* rec_mom32;   // FP32 copy of the stage-point moments (mixed pair path)
 };
 // k_s = RHS(y_s); then the next stage point (coef of length s+1) or, when
 // last, the error norm partials and the normalised accepted state.

@@ -109,17 +109,48 @@ __global__ void k_count(int64_t n, const int32_t* __restrict__ key, int32_t* __r
     if (i < n) atomicAdd(&count[key[i]], 1);
 }
 
-// Exclusive scan of `count` (one block of 1024 threads); writes start, end
-// (= start + count) and a cursor copy of start. total (if not null) gets the sum.
-__global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ count, int32_t m, int32_t* __restrict__ start,
-                                               int32_t* __restrict__ end, int32_t* __restrict__ cursor,
-                                               int32_t* __restrict__ total) {
+// Exclusive scan of `count` in three passes over tiles of SCAN_TILE entries:
+// tile sums, a scan of the tile sums (one block), and the scan within each
+// tile plus its offset. Writes start, end (= start + count, if not null) and
+// a cursor copy of start (if not null); total (if not null) gets the sum.
+constexpr int SCAN_TILE = 2048;  // 256 threads x 8 entries
+
+__device__ __forceinline__ int32_t block_excl_scan256(int32_t v, int32_t* sh, int32_t& block_total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const int32_t u = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += u;
+        __syncthreads();
+    }
+    block_total = sh[255];
+    const int32_t r = sh[t] - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums(const int32_t* __restrict__ count, int32_t m,
+                                                   int32_t* __restrict__ tile_sum) {
+    __shared__ int32_t sh[256];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * 8;
+    int32_t v = 0;
+    for (int k = 0; k < 8; ++k)
+        if (base + k < m) v += count[base + k];
+    int32_t tot;
+    block_excl_scan256(v, sh, tot);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(int32_t* __restrict__ tile_sum, int32_t nt,
+                                                   int32_t* __restrict__ total) {
     __shared__ int32_t part[1024];
     const int t = threadIdx.x;
-    const int64_t per = (m + 1023) / 1024;
-    const int64_t b = t * per, e = (b + per < m) ? b + per : m;
+    const int64_t per = (nt + 1023) / 1024;
+    const int64_t b = t * per, e = (b + per < nt) ? b + per : nt;
     int32_t s = 0;
-    for (int64_t i = b; i < e; ++i) s += count[i];
+    for (int64_t i = b; i < e; ++i) s += tile_sum[i];
     part[t] = s;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
@@ -130,13 +161,35 @@ __global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ count
     }
     int32_t run = part[t] - s;
     for (int64_t i = b; i < e; ++i) {
-        const int32_t c = count[i];
-        start[i] = run;
-        if (end) end[i] = run + c;
-        if (cursor) cursor[i] = run;
+        const int32_t c = tile_sum[i];
+        tile_sum[i] = run;
         run += c;
     }
     if (total && t == 1023) *total = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const int32_t* __restrict__ count, int32_t m,
+                                                    const int32_t* __restrict__ tile_off, int32_t* __restrict__ start,
+                                                    int32_t* __restrict__ end, int32_t* __restrict__ cursor) {
+    __shared__ int32_t sh[256];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * 8;
+    int32_t c[8];
+    int32_t v = 0;
+    for (int k = 0; k < 8; ++k) {
+        c[k] = base + k < m ? count[base + k] : 0;
+        v += c[k];
+    }
+    int32_t tot;
+    int32_t run = tile_off[blockIdx.x] + block_excl_scan256(v, sh, tot);
+    for (int k = 0; k < 8; ++k) {
+        const int64_t i = base + k;
+        if (i < m) {
+            start[i] = run;
+            if (end) end[i] = run + c[k];
+            if (cursor) cursor[i] = run;
+        }
+        run += c[k];
+    }
 }
 
 __global__ void k_place(int64_t n, const int32_t* __restrict__ key, int32_t* __restrict__ cursor,
@@ -362,7 +415,9 @@ __device__ __forceinline__ void block_sum3(double a, double b, double c, double*
 // cell-sorted order.
 __global__ __launch_bounds__(256) void k_moments(int64_t n, const double* __restrict__ mm,
                                                  const double* __restrict__ mag_mom, const int32_t* __restrict__ inv_perm,
-                                                 double* __restrict__ mom, double4* __restrict__ rec_mom) {
+                                                 const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
+                                                 double* __restrict__ mom, double4* __restrict__ rec_mom,
+                                                 float4* __restrict__ rec_mom32) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double a = mag_mom[i];
@@ -370,41 +425,66 @@ __global__ __launch_bounds__(256) void k_moments(int64_t n, const double* __rest
     mom[i] = mx;
     mom[n + i] = my;
     mom[2 * n + i] = mz;
-    rec_mom[inv_perm[i]] = make_double4(mx, my, mz, 0.0);
+    const int32_t k = inv_perm[i];
+    rec_mom[k] = make_double4(mx, my, mz, 0.0);
+    // FP32 copy in units of the replica's largest moment (mixed pair path)
+    const double s = reps[rep_of[i]].inv_mref;
+    rec_mom32[k] = make_float4((float)(mx * s), (float)(my * s), (float)(mz * s), 0.0f);
 }
 
-// Column sums of an SoA N x 3 array per replica (magMomSum,
-// src/simulation.cpp:639): one block per replica, fixed summation order.
-__global__ __launch_bounds__(256) void k_rep_sum3(const Rep* __restrict__ reps, int64_t n, const double* __restrict__ v,
-                                                  double* __restrict__ out) {
-    const Rep& R = reps[blockIdx.x];
-    double a = 0, b = 0, c = 0;
-    for (int64_t i = R.off + threadIdx.x; i < R.off + R.n; i += blockDim.x) {
-        a += v[i];
-        b += v[n + i];
-        c += v[2 * n + i];
-    }
-    block_sum3<256>(a, b, c, out + 3 * blockIdx.x);
-}
+// Per-replica sums in a fixed order: chunks of RED_CHUNK particles (never
+// straddling a replica) are summed by one block each, then each replica adds
+// its chunks' sums in order. Columns col0 .. col0 + ncol - 1 of an SoA
+// N x 3 array.
+constexpr int RED_CHUNK = RED_CHUNK_PARTICLES;
 
-// Mean of the z component per replica (<m_z>, src/simulation.cpp:515); with
-// a series buffer, also stored at the running step index.
-__global__ __launch_bounds__(256) void k_rep_mean_z(const Rep* __restrict__ reps, int64_t n, const double* __restrict__ v,
-                                                    double* __restrict__ out, const int32_t* __restrict__ counter,
-                                                    double* __restrict__ series, int32_t series_cap, int32_t nrep) {
-    const Rep& R = reps[blockIdx.x];
-    double a = 0;
-    for (int64_t i = R.off + threadIdx.x; i < R.off + R.n; i += blockDim.x) a += v[2 * n + i];
+__global__ __launch_bounds__(256) void k_red_part(const int2* __restrict__ chunks, const Rep* __restrict__ reps,
+                                                  int64_t n, const double* __restrict__ v, int32_t col0, int32_t ncol,
+                                                  double* __restrict__ part) {
+    const int2 ch = chunks[blockIdx.x];
+    const Rep& R = reps[ch.x];
+    const int64_t b = ch.y, e = min((int64_t)ch.y + RED_CHUNK, R.off + R.n);
+    double a[3] = {0, 0, 0};
+    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x)
+        for (int c = 0; c < ncol; ++c) a[c] += v[(int64_t)(col0 + c) * n + i];
     __shared__ double r3[3];
-    block_sum3<256>(a, 0.0, 0.0, r3);
+    block_sum3<256>(a[0], a[1], a[2], r3);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const double m = r3[0] / (double)R.n;
-        out[blockIdx.x] = m;
-        if (series) {
-            const int32_t k = *counter;
-            if (k < series_cap) series[(int64_t)k * nrep + blockIdx.x] = m;
-        }
+    if (threadIdx.x < 3) part[3 * (int64_t)blockIdx.x + threadIdx.x] = r3[threadIdx.x];
+}
+
+// magMomSum (src/simulation.cpp:639) per replica
+__global__ void k_rep_sum3(const Rep* __restrict__ reps, int32_t nrep, const double* __restrict__ part,
+                           double* __restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrep) return;
+    const Rep& R = reps[r];
+    double a = 0, b = 0, c = 0;
+    for (int k = R.red_off; k < R.red_off + R.nred; ++k) {
+        a += part[3 * (int64_t)k];
+        b += part[3 * (int64_t)k + 1];
+        c += part[3 * (int64_t)k + 2];
+    }
+    out[3 * r] = a;
+    out[3 * r + 1] = b;
+    out[3 * r + 2] = c;
+}
+
+// <m_z> per replica (src/simulation.cpp:515); with a series buffer, also
+// stored at the running step index.
+__global__ void k_rep_mean_z(const Rep* __restrict__ reps, int32_t nrep, const double* __restrict__ part,
+                             double* __restrict__ out, const int32_t* __restrict__ counter,
+                             double* __restrict__ series, int32_t series_cap) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrep) return;
+    const Rep& R = reps[r];
+    double a = 0;
+    for (int k = R.red_off; k < R.red_off + R.nred; ++k) a += part[3 * (int64_t)k];
+    const double m = a / (double)R.n;
+    out[r] = m;
+    if (series) {
+        const int32_t k = *counter;
+        if (k < series_cap) series[(int64_t)k * nrep + r] = m;
     }
 }
 
@@ -468,16 +548,44 @@ __device__ __forceinline__ float erfcx32(float x) {
     return p;
 }
 
+// FP32 terms of one pair in units of ell (lengths) and mref (moments): field
+// on i from j and (FORCE) the force on j, u the unit vector from i to j.
+template <bool FORCE>
+__device__ __forceinline__ void pair_terms32(const PairParams& pp, float rs, float irs, float fx, float fy, float fz,
+                                             float mix, float miy, float miz, float mjx, float mjy, float mjz,
+                                             float B[3], float F[3]) {
+    const float irs2 = irs * irs;
+    const float x = pp.al_s * rs;
+    const float gau = __expf(-x * x);
+    const float t = 2.0f * pp.al_s * gau * 0.5641895835477563f * irs2;
+    const float a1 = gau * erfcx32(x) * (irs2 * irs) + t;
+    const float a2 = 3.0f * a1 * irs2 + 2.0f * pp.al2_s * t;
+    const float dj = mjx * fx + mjy * fy + mjz * fz;
+    const float g2 = a2 * dj * rs * rs;
+    B[0] = g2 * fx - a1 * mjx;
+    B[1] = g2 * fy - a1 * mjy;
+    B[2] = g2 * fz - a1 * mjz;
+    if (FORCE) {
+        const float di = mix * fx + miy * fy + miz * fz;
+        const float a3 = 5.0f * a2 * irs2 + 4.0f * pp.al4_s * t;
+        const float mimj = mix * mjx + miy * mjy + miz * mjz;
+        const float c0 = (a2 * mimj - a3 * di * dj * rs * rs) * rs;
+        const float c1 = a2 * rs;
+        F[0] = c0 * fx + c1 * (dj * mix + di * mjx);
+        F[1] = c0 * fy + c1 * (dj * miy + di * mjy);
+        F[2] = c0 * fz + c1 * (dj * miz + di * mjz);
+    }
+}
+
 // Terms of one pair (CompDipInteractEwaldR, src/interaction.cpp:396-417):
 // field on i from j, and (FORCE) the force on j (applied -F to i), from the
 // separation r, its reciprocal, the unit vector u from i to j and both
 // moments. FP64 path in SI units without the facEwald factor; FP32 path in
 // units of ell and mref (see PairParams). erfc(alpha r) is
 // exp(-(alpha r)^2) erfcx(alpha r), sharing the Gaussian of the t term.
-template <bool FORCE, bool MIXED, bool BOTH = false>
+template <bool FORCE, bool MIXED>
 __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, double ir, double ux, double uy, double uz,
-                                           const double4& mi, const double4& mj, double B[3], double F[3],
-                                           double* Bj = nullptr) {
+                                           const double4& mi, const double4& mj, double B[3], double F[3]) {
     if (!MIXED) {
         const double ir2 = ir * ir;
         const double x = pp.alpha * r;
@@ -490,14 +598,8 @@ __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, doubl
         B[0] = g2 * ux - a1 * mj.x;
         B[1] = g2 * uy - a1 * mj.y;
         B[2] = g2 * uz - a1 * mj.z;
-        const double di = mi.x * ux + mi.y * uy + mi.z * uz;
-        if (BOTH) {
-            const double h2 = a2 * di * r * r;
-            Bj[0] = h2 * ux - a1 * mi.x;
-            Bj[1] = h2 * uy - a1 * mi.y;
-            Bj[2] = h2 * uz - a1 * mi.z;
-        }
         if (FORCE) {
+            const double di = mi.x * ux + mi.y * uy + mi.z * uz;
             const double a3 = 5.0 * a2 * ir2 + 4.0 * pp.alpha4 * t;
             const double mimj = mi.x * mj.x + mi.y * mj.y + mi.z * mj.z;
             const double c0 = (a2 * mimj - a3 * di * dj * r * r) * r;
@@ -507,39 +609,14 @@ __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, doubl
             F[2] = c0 * uz + c1 * (dj * mi.z + di * mj.z);
         }
     } else {
-        const float rs = (float)(r * pp.inv_ell);
-        const float irs = (float)(ir * pp.ell);
-        const float irs2 = irs * irs;
-        const float fx = (float)ux, fy = (float)uy, fz = (float)uz;
-        const float mjx = (float)(mj.x * pp.inv_mref), mjy = (float)(mj.y * pp.inv_mref),
-                    mjz = (float)(mj.z * pp.inv_mref);
-        const float x = pp.al_s * rs;
-        const float gau = __expf(-x * x);
-        const float t = 2.0f * pp.al_s * gau * 0.5641895835477563f * irs2;
-        const float a1 = gau * erfcx32(x) * (irs2 * irs) + t;
-        const float a2 = 3.0f * a1 * irs2 + 2.0f * pp.al2_s * t;
-        const float dj = mjx * fx + mjy * fy + mjz * fz;
-        const float g2 = a2 * dj * rs * rs;
-        B[0] = (double)(g2 * fx - a1 * mjx);
-        B[1] = (double)(g2 * fy - a1 * mjy);
-        B[2] = (double)(g2 * fz - a1 * mjz);
-        const float mix = (float)(mi.x * pp.inv_mref), miy = (float)(mi.y * pp.inv_mref),
-                    miz = (float)(mi.z * pp.inv_mref);
-        const float di = mix * fx + miy * fy + miz * fz;
-        if (BOTH) {
-            const float h2 = a2 * di * rs * rs;
-            Bj[0] = (double)(h2 * fx - a1 * mix);
-            Bj[1] = (double)(h2 * fy - a1 * miy);
-            Bj[2] = (double)(h2 * fz - a1 * miz);
-        }
-        if (FORCE) {
-            const float a3 = 5.0f * a2 * irs2 + 4.0f * pp.al4_s * t;
-            const float mimj = mix * mjx + miy * mjy + miz * mjz;
-            const float c0 = (a2 * mimj - a3 * di * dj * rs * rs) * rs;
-            const float c1 = a2 * rs;
-            F[0] = (double)(c0 * fx + c1 * (dj * mix + di * mjx));
-            F[1] = (double)(c0 * fy + c1 * (dj * miy + di * mjy));
-            F[2] = (double)(c0 * fz + c1 * (dj * miz + di * mjz));
+        const double s = pp.inv_mref;
+        float b[3], f[3];
+        pair_terms32<FORCE>(pp, (float)(r * pp.inv_ell), (float)(ir * pp.ell), (float)ux, (float)uy, (float)uz,
+                            (float)(mi.x * s), (float)(mi.y * s), (float)(mi.z * s), (float)(mj.x * s),
+                            (float)(mj.y * s), (float)(mj.z * s), b, f);
+        for (int d = 0; d < 3; ++d) {
+            B[d] = b[d];
+            if (FORCE) F[d] = f[d];
         }
     }
 }
@@ -592,6 +669,38 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
     const double4 pi = W.ipos[il];
     const double4 pj = g.rec_pos[j];
     const bool i_lo = oi < oj;
+    if (MIXED && !COATING) {
+        // separation in FP64 (positions are absolute), everything after it in
+        // FP32 in units of ell = r_c, moments in units of mref (FP32 copies)
+        const double L = g.L;
+        const float vx = (float)(min_image(pj.x - pi.x, L) * pp.inv_ell);
+        const float vy = (float)(min_image(pj.y - pi.y, L) * pp.inv_ell);
+        const float vz = (float)(min_image(pj.z - pi.z, L) * pp.inv_ell);
+        const float r2 = vx * vx + vy * vy + vz * vz;
+        if (!(r2 < 1.0f)) return;
+        ++npairs;
+        const float irs = __builtin_amdgcn_rsqf(r2);
+        const float rs = r2 * irs;
+        const float ux = vx * irs, uy = vy * irs, uz = vz * irs;
+        const float4 mi = reinterpret_cast<const float4*>(W.imom)[il];
+        const float4 mj = g.rec_mom32[j];
+        float B[3], F[3];
+        pair_terms32<FORCE>(pp, rs, irs, ux, uy, uz, mi.x, mi.y, mi.z, mj.x, mj.y, mj.z, B, F);
+        atomicAdd(&W.acc[3][il], (double)B[0]);
+        atomicAdd(&W.acc[4][il], (double)B[1]);
+        atomicAdd(&W.acc[5][il], (double)B[2]);
+        if (FORCE) {
+            if (i_lo) {
+                press[0] += (double)(rs * ux * F[1]);
+                press[1] += (double)(rs * ux * F[2]);
+                press[2] += (double)(rs * uy * F[2]);
+            }
+            atomicAdd(&W.acc[0][il], -(double)F[0]);
+            atomicAdd(&W.acc[1][il], -(double)F[1]);
+            atomicAdd(&W.acc[2][il], -(double)F[2]);
+        }
+        return;
+    }
     // canonical i < j orientation of CompInteractVecs (src/interaction.cpp:99-104)
     const double L = g.L;
     const double vx = min_image(i_lo ? pj.x - pi.x : pi.x - pj.x, L);
@@ -663,7 +772,10 @@ __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __rest
     if (vi) {
         pi = g.rec_pos[k];
         W.ipos[lane] = pi;
-        W.imom[lane] = rec_mom[k];
+        if (MIXED && !COATING)
+            reinterpret_cast<float4*>(W.imom)[lane] = g.rec_mom32[k];
+        else
+            W.imom[lane] = rec_mom[k];
         W.iorig[lane] = g.sorted_idx[k];
     }
     for (int d = 0; d < 6; ++d) W.acc[d][lane] = 0.0;
@@ -914,13 +1026,17 @@ __global__ void k_sincos_tab(int64_t n, const Rep* __restrict__ reps, const int3
 // particles of a chunk are walked in step by the whole wavefront (broadcast
 // loads), and the segment's nx values are accumulated in registers through
 // the recurrence e^{i(a+1)x} = e^{iax} e^{ix}. Each chunk writes its partial
-// sums; k_kreduce adds them in a fixed order.
-// Replica of work item b: the last replica whose first item is <= b.
-__device__ __forceinline__ int find_rep(const Rep* __restrict__ reps, int32_t nrep, int64_t b, bool by_k) {
+// sums; k_kpack adds them in a fixed order.
+// Replica of item b of a concatenated per-replica range (WHICH: 0 S(k)
+// build items, 1 k vectors, 2 packed scatter entries): the last replica whose
+// first item is <= b.
+template <int WHICH>
+__device__ __forceinline__ int find_rep(const Rep* __restrict__ reps, int32_t nrep, int64_t b) {
     int lo = 0, hi = nrep - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if ((by_k ? reps[mid].k_off : reps[mid].kb_off) <= b) lo = mid; else hi = mid - 1;
+        const int32_t first = WHICH == 0 ? reps[mid].kb_off : (WHICH == 1 ? reps[mid].k_off : reps[mid].pk_off);
+        if (first <= b) lo = mid; else hi = mid - 1;
     }
     return lo;
 }
@@ -930,7 +1046,7 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
                                                const KSeg* __restrict__ segs_all, const double* __restrict__ tab_all,
                                                const double* __restrict__ mom, double* __restrict__ partial_all) {
     // work item = (replica, wavefront of row segments, particle chunk)
-    const Rep& R = reps[find_rep(reps, nrep, blockIdx.x, false)];
+    const Rep& R = reps[find_rep<0>(reps, nrep, blockIdx.x)];
     const int32_t li = (int32_t)blockIdx.x - R.kb_off;
     const int32_t segw = li % R.nsegw, ch = li / R.nsegw;
     const int32_t nc = R.nc, nseg = R.nseg, chunk = R.kchunk, nk = R.nk;
@@ -994,36 +1110,54 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
     }
 }
 
-__global__ void k_kreduce(int32_t nk_all, const Rep* __restrict__ reps, int32_t nrep,
-                          const double* __restrict__ partial_all, double* __restrict__ sk) {
-    const int kg = blockIdx.x * blockDim.x + threadIdx.x;
-    if (kg >= nk_all) return;
-    const Rep& R = reps[find_rep(reps, nrep, kg, true)];
-    const int k = kg - R.k_off, nk = R.nk;
+// Sum the S(k) partials of both k vectors of a packed entry over the
+// particle chunks (fixed order) and store (U+ + U-, U+ - U-, V+ + V-,
+// V+ - V-), U = e Re S, V = e Im S.
+__global__ void k_kpack(int32_t npk_all, const Rep* __restrict__ reps, int32_t nrep, const KPack* __restrict__ desc,
+                        const double* __restrict__ partial_all, double4* __restrict__ pack) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= npk_all) return;
+    const Rep& R = reps[find_rep<2>(reps, nrep, j)];
+    const KPack d = desc[j];
+    const int nk = R.nk;
     const double* partial = partial_all + R.part_off;
-    double c = 0, s = 0;
+    double cp = 0, sp = 0, cm = 0, sm = 0;
     for (int ch = 0; ch < R.nchunks; ++ch) {
-        c += partial[((int64_t)ch * nk + k) * 2];
-        s += partial[((int64_t)ch * nk + k) * 2 + 1];
+        const double* q = partial + (int64_t)ch * nk * 2;
+        if (d.kp >= 0) {
+            cp += q[2 * d.kp];
+            sp += q[2 * d.kp + 1];
+        }
+        if (d.km >= 0) {
+            cm += q[2 * d.km];
+            sm += q[2 * d.km + 1];
+        }
     }
-    sk[2 * kg] = c;
-    sk[2 * kg + 1] = s;
+    const double up = d.e * cp, vp = d.e * sp, um = d.e * cm, vm = d.e * sm;
+    pack[j] = make_double4(up + um, up - um, vp + vm, vp - vm);
 }
 
-// One particle of the k-space scatter (k_kscatter) in replica R.
+// One particle of the k-space scatter (k_kscatter) in replica R. Per row
+// (ny, nz) and |nx| = a the pair of k vectors (+-a, ny, nz) is folded into
+// four products with the particle's phase: with G1 = SU wc + SV ws,
+// G2 = DV wc - DU ws, G3 = DU wc + DV ws, G4 = SV wc - SU ws (S/D: sum and
+// difference of the two vectors' e S(k), w = e^{i(ny y + nz z)}) and
+// x = e^{i a x}:
+//   sum over the pair of e Re(S conj(phase))        = xc G1 + xs G2
+//   its nx-weighted difference (the nx component)   = xc G3 + xs G4
+//   sum of e Im(S conj(phase)) (force), difference  = xs G3 - xc G4, xs G1 - xc G2
+// The ny and nz components share the row's sums; constants are applied once.
 template <bool FORCE>
 __device__ __forceinline__ void kscatter_one(int64_t n, int64_t i, int32_t rid, const Rep& R,
-                                             const KRow* __restrict__ rows_all, const double* __restrict__ ek_all,
-                                             const double* __restrict__ sk_all, const double* __restrict__ tab_all,
-                                             const double* __restrict__ mom, const double* __restrict__ msum_all,
+                                             const KRow* __restrict__ rows_all, const double4* __restrict__ pack_all,
+                                             const double* __restrict__ tab_all, const double* __restrict__ mom,
+                                             const double* __restrict__ msum_all,
                                              const double* __restrict__ add_force, double* __restrict__ force,
                                              double* __restrict__ field) {
     const int32_t nc = R.nc, nrows = R.nrows;
     const KRow* rows = rows_all + R.row_off;
-    const double* ek = ek_all + R.k_off;
-    const double* sk = sk_all + 2 * (int64_t)R.k_off;
+    const double4* pack = pack_all + R.pk_off;
     const double* msum = msum_all + 3 * rid;
-    const double c_force = R.c_force, c_flux = R.c_flux, c_self = R.c_self, c_surf = R.c_surf;
     const int64_t stride = 3 * (int64_t)(nc + 1) * 2;
     const double* t = tab_all + R.tab_off + (i - R.off) * stride;
     const double* ty = t + (nc + 1) * 2;
@@ -1049,45 +1183,45 @@ __device__ __forceinline__ void kscatter_one(int64_t n, int64_t i, int32_t rid, 
         }
         const double wc = yc * zc - ys * zs, ws = ys * zc + yc * zs;
         const double A = row.ny * my + row.nz * mz;
-        const bool half = (row.ny == 0 && row.nz == 0);
-        double xc = 1.0, xs = 0.0;
+        const double4* P = pack + row.poff;
+        double xc = 1.0, xs = 0.0, ad = 0.0, am = 0.0;
+        double rb = 0.0, rs = 0.0;
+        // the next entry is loaded one iteration ahead (the table carries
+        // one entry of padding past its end)
+        double4 pn = P[0];
         for (int a = 0; a <= row.nxmax; ++a) {
-            const double p1 = xc * wc, p2 = xs * ws, p3 = xc * ws, p4 = xs * wc;
-            for (int sgn = 1; sgn >= -1; sgn -= 2) {
-                if (a == 0 && sgn < 0) break;
-                if (half && (a == 0 || sgn < 0)) continue;
-                const int nx = sgn * a;
-                const double pc = sgn > 0 ? p1 - p2 : p1 + p2;
-                const double ps = sgn > 0 ? p3 + p4 : p3 - p4;
-                const int k = half ? row.koff + nx - 1 : row.koff + nx + row.nxmax;
-                const double sc = sk[2 * k], ss = sk[2 * k + 1], e = ek[k];
-                const double b = c_flux * e * (sc * pc + ss * ps);
-                B0 -= b * nx;
-                B1 -= b * row.ny;
-                B2 -= b * row.nz;
-                if (FORCE) {
-                    const double q = nx * mx + A;
-                    const double s = c_force * e * q * (sc * ps - ss * pc);
-                    F0 += s * nx;
-                    F1 += s * row.ny;
-                    F2 += s * row.nz;
-                }
+            const double4 p = pn;
+            pn = P[a + 1];
+            const double G1 = p.x * wc + p.z * ws, G2 = p.w * wc - p.y * ws;
+            const double G3 = p.y * wc + p.w * ws, G4 = p.z * wc - p.x * ws;
+            rb += xc * G1 + xs * G2;
+            B0 += ad * (xc * G3 + xs * G4);
+            if (FORCE) {
+                const double ts = xs * G3 - xc * G4, td = xs * G1 - xc * G2;
+                rs += A * ts + am * td;
+                F0 += ad * (A * td + am * ts);
+                am += mx;
             }
+            ad += 1.0;
             const double nxc = xc * ex - xs * sx, nxs = xs * ex + xc * sx;
             xc = nxc;
             xs = nxs;
         }
+        B1 += row.ny * rb;
+        B2 += row.nz * rb;
+        if (FORCE) {
+            F1 += row.ny * rs;
+            F2 += row.nz * rs;
+        }
     }
-    B0 += c_self * mx - c_surf * msum[0];
-    B1 += c_self * my - c_surf * msum[1];
-    B2 += c_self * mz - c_surf * msum[2];
-    field[i] += B0;
-    field[n + i] += B1;
-    field[2 * n + i] += B2;
+    const double cb = R.c_flux, cf = R.c_force, c_self = R.c_self, c_surf = R.c_surf;
+    field[i] += -cb * B0 + c_self * mx - c_surf * msum[0];
+    field[n + i] += -cb * B1 + c_self * my - c_surf * msum[1];
+    field[2 * n + i] += -cb * B2 + c_self * mz - c_surf * msum[2];
     if (FORCE) {
-        force[i] += F0 + (add_force ? add_force[i] : 0.0);
-        force[n + i] += F1 + (add_force ? add_force[n + i] : 0.0);
-        force[2 * n + i] += F2 + (add_force ? add_force[2 * n + i] : 0.0);
+        force[i] += cf * F0 + (add_force ? add_force[i] : 0.0);
+        force[n + i] += cf * F1 + (add_force ? add_force[n + i] : 0.0);
+        force[2 * n + i] += cf * F2 + (add_force ? add_force[2 * n + i] : 0.0);
     }
 }
 
@@ -1100,7 +1234,7 @@ __device__ __forceinline__ void kscatter_one(int64_t n, int64_t i, int32_t rid, 
 template <bool FORCE>
 __global__ __launch_bounds__(256) void k_kscatter(int64_t n, const Rep* __restrict__ reps,
                                                   const int32_t* __restrict__ rep_of, const KRow* __restrict__ rows_all,
-                                                  const double* __restrict__ ek_all, const double* __restrict__ sk_all,
+                                                  const double4* __restrict__ pack_all,
                                                   const double* __restrict__ tab_all, const double* __restrict__ mom,
                                                   const double* __restrict__ msum_all,
                                                   const double* __restrict__ add_force, double* __restrict__ force,
@@ -1112,11 +1246,10 @@ __global__ __launch_bounds__(256) void k_kscatter(int64_t n, const Rep* __restri
     const bool uniform = __ballot(rid != rid0) == 0;
     if (i >= n) return;
     if (uniform)
-        kscatter_one<FORCE>(n, i, rid0, reps[rid0], rows_all, ek_all, sk_all, tab_all, mom, msum_all, add_force, force,
+        kscatter_one<FORCE>(n, i, rid0, reps[rid0], rows_all, pack_all, tab_all, mom, msum_all, add_force, force,
                             field);
     else
-        kscatter_one<FORCE>(n, i, rid, reps[rid], rows_all, ek_all, sk_all, tab_all, mom, msum_all, add_force, force,
-                            field);
+        kscatter_one<FORCE>(n, i, rid, reps[rid], rows_all, pack_all, tab_all, mom, msum_all, add_force, force, field);
 }
 
 // ------------------------------------------------------------------ integrator
@@ -1194,7 +1327,10 @@ __global__ __launch_bounds__(256) void k_stage(StageParams sp, int32_t stage, in
                 mo[d] = ym * sp.mag_mom[i];
                 mom[d * n + i] = mo[d];
             }
-            rec_mom[sp.inv_perm[i]] = make_double4(mo[0], mo[1], mo[2], 0.0);
+            const int32_t k = sp.inv_perm[i];
+            rec_mom[k] = make_double4(mo[0], mo[1], mo[2], 0.0);
+            const double s = sp.reps[sp.rep_of[i]].inv_mref;
+            sp.rec_mom32[k] = make_float4((float)(mo[0] * s), (float)(mo[1] * s), (float)(mo[2] * s), 0.0f);
         } else {
             double nm[3], ne[3], ev[3];
             for (int d = 0; d < 3; ++d) {
@@ -1380,12 +1516,24 @@ hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* p
     LAUNCH_CHECK();
 }
 
+int32_t scan_tiles(int64_t m) { return (int32_t)((m + SCAN_TILE - 1) / SCAN_TILE); }
+
+// tmp: scan_tiles(m) ints
+hipError_t launch_scan(hipStream_t s, const int32_t* count, int32_t m, int32_t* start, int32_t* end, int32_t* cursor,
+                       int32_t* total, int32_t* tmp) {
+    const int32_t nt = std::max(1, scan_tiles(m));
+    hipLaunchKernelGGL(k_scan_sums, dim3(nt), dim3(256), 0, s, count, m, tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, tmp, nt, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(256), 0, s, count, m, (const int32_t*)tmp, start, end, cursor);
+    LAUNCH_CHECK();
+}
+
 hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
-                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx) {
+                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp) {
     hipError_t e = launch_fill_i32(s, count, ncells, 0);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_count, dim3(grid_for(n, 256)), dim3(256), 0, s, n, key, count);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, count, ncells, start, end, cursor, (int32_t*)nullptr);
+    if ((e = launch_scan(s, count, ncells, start, end, cursor, nullptr, scan_tmp)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_place, dim3(grid_for(n, 256)), dim3(256), 0, s, n, key, cursor, sorted_idx);
     LAUNCH_CHECK();
 }
@@ -1405,15 +1553,16 @@ hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start
 
 hipError_t launch_contacts(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
                            const double* r_mag, const int32_t* fstart, const int32_t* fend, const int32_t* fsorted,
-                           int32_t* cand_count, int32_t* cand_off, int32_t* cand_j, int32_t cand_cap, int32_t* err) {
+                           int32_t* cand_count, int32_t* cand_off, int32_t* cand_j, int32_t cand_cap, int32_t* err,
+                           int32_t* scan_tmp) {
     // cand_off has n+1 slots: offsets then the total at [n]; cand_j holds
     // 2*cand_cap ints: partner j then owner i.
     int32_t* cand_i = cand_j + cand_cap;
     hipLaunchKernelGGL(k_contacts<0>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart,
                        fend, fsorted, cand_count, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
                        cand_cap, err);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, cand_count, (int32_t)n, cand_off, (int32_t*)nullptr,
-                       (int32_t*)nullptr, cand_off + n);
+    hipError_t e = launch_scan(s, cand_count, (int32_t)n, cand_off, nullptr, nullptr, cand_off + n, scan_tmp);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_contacts<1>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart,
                        fend, fsorted, cand_count, cand_off, cand_i, cand_j, cand_cap, err);
     LAUNCH_CHECK();
@@ -1441,19 +1590,25 @@ hipError_t launch_clear_events(hipStream_t s, int64_t n, int32_t* ev_head, int32
 }
 
 hipError_t launch_moments(hipStream_t s, int64_t n, const double* mm, const double* mag_mom, const int32_t* inv_perm,
-                          double* mom, double4* rec_mom) {
-    hipLaunchKernelGGL(k_moments, dim3(grid_for(n, 256)), dim3(256), 0, s, n, mm, mag_mom, inv_perm, mom, rec_mom);
+                          const Rep* reps, const int32_t* rep_of, double* mom, double4* rec_mom, float4* rec_mom32) {
+    hipLaunchKernelGGL(k_moments, dim3(grid_for(n, 256)), dim3(256), 0, s, n, mm, mag_mom, inv_perm, reps, rep_of, mom,
+                       rec_mom, rec_mom32);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_rep_sum3(hipStream_t s, const Rep* reps, int32_t nrep, int64_t n, const double* v, double* out) {
-    hipLaunchKernelGGL(k_rep_sum3, dim3(nrep), dim3(256), 0, s, reps, n, v, out);
+hipError_t launch_rep_sum3(hipStream_t s, const Rep* reps, int32_t nrep, const int2* chunks, int32_t nch, int64_t n,
+                           const double* v, double* part, double* out) {
+    hipLaunchKernelGGL(k_red_part, dim3(nch), dim3(256), 0, s, chunks, reps, n, v, 0, 3, part);
+    hipLaunchKernelGGL(k_rep_sum3, dim3((nrep + 63) / 64), dim3(64), 0, s, reps, nrep, (const double*)part, out);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, int64_t n, const double* v, double* out,
-                             const int32_t* counter, double* series, int32_t series_cap) {
-    hipLaunchKernelGGL(k_rep_mean_z, dim3(nrep), dim3(256), 0, s, reps, n, v, out, counter, series, series_cap, nrep);
+hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, const int2* chunks, int32_t nch, int64_t n,
+                             const double* v, double* part, double* out, const int32_t* counter, double* series,
+                             int32_t series_cap) {
+    hipLaunchKernelGGL(k_red_part, dim3(nch), dim3(256), 0, s, chunks, reps, n, v, 2, 1, part);
+    hipLaunchKernelGGL(k_rep_mean_z, dim3((nrep + 63) / 64), dim3(64), 0, s, reps, nrep, (const double*)part, out,
+                       counter, series, series_cap);
     LAUNCH_CHECK();
 }
 
@@ -1507,21 +1662,22 @@ hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep
     LAUNCH_CHECK();
 }
 
-hipError_t launch_kreduce(hipStream_t s, int32_t nk_all, const Rep* reps, int32_t nrep, const double* partial,
-                          double* sk) {
-    if (nk_all <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_kreduce, dim3((nk_all + 255) / 256), dim3(256), 0, s, nk_all, reps, nrep, partial, sk);
+hipError_t launch_kpack(hipStream_t s, int32_t npk_all, const Rep* reps, int32_t nrep, const KPack* desc,
+                        const double* partial, double4* pack) {
+    if (npk_all <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kpack, dim3((npk_all + 255) / 256), dim3(256), 0, s, npk_all, reps, nrep, desc, partial,
+                       pack);
     LAUNCH_CHECK();
 }
 
 hipError_t launch_kscatter(hipStream_t s, bool force, int64_t n, const Rep* reps, const int32_t* rep_of,
-                           const KRow* rows, const double* ek, const double* sk, const double* tab, const double* mom,
+                           const KRow* rows, const double4* pack, const double* tab, const double* mom,
                            const double* msum, const double* add_force, double* force_out, double* field) {
     if (force)
-        hipLaunchKernelGGL(k_kscatter<true>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, rows, ek, sk, tab,
+        hipLaunchKernelGGL(k_kscatter<true>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, rows, pack, tab,
                            mom, msum, add_force, force_out, field);
     else
-        hipLaunchKernelGGL(k_kscatter<false>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, rows, ek, sk,
+        hipLaunchKernelGGL(k_kscatter<false>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, rows, pack,
                            tab, mom, msum, add_force, force_out, field);
     LAUNCH_CHECK();
 }
