@@ -201,18 +201,19 @@ def ensemble(mds, workload, precision, replicas, steps, warmup, master=20261203)
     return out
 
 
-def serving_leg(precision, n_req, steps_scale, budget_gib, load=0.8, seed=20261203):
+def serving_leg(precision, n_req, steps_scale, budget_gib, load=0.8, seed=20261203, cu_fractions=None):
     """Request stream through the continuous-batching loop: N log-uniform in
     [1e3, 2e5], steps log-uniform in [1e3, 1e5] x steps_scale, field uniform
     in [0, 100] mT, volume fraction uniform in [1 %, 10 %], own seeds. First
     all requests at t = 0 (saturated throughput), then the same requests with
     Poisson arrivals at `load` x the saturated request rate; latency of each
-    request against its isolated-solve time (alone in its own context)."""
+    request against its isolated-solve time (the same request served alone on
+    an idle server)."""
     import copy
     from md_sim_wolfschwenger_amd import serving
     rng = np.random.default_rng(seed)
     base = serving.poisson_stream(n_req, 0.0, rng, steps_range=(1e3 * steps_scale, 1e5 * steps_scale))
-    kw = dict(precision=precision, hbm_budget=budget_gib * 2 ** 30)
+    kw = dict(precision=precision, hbm_budget=budget_gib * 2 ** 30, cu_fractions=cu_fractions)
     srv = serving.LaneServer(**kw)
     sat = srv.serve(copy.deepcopy(base))
     srv.close()
@@ -226,19 +227,22 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, load=0.8, seed=202612
     srv = serving.LaneServer(**kw)
     ld = srv.serve(reqs)
     srv.close()
-    iso = np.array([serving.isolated_seconds({"precision": precision}, q) for q in reqs])
+    iso = np.array(serving.isolated_latencies(reqs, precision))
     lat = np.array([q.latency for q in reqs])
     ratio = lat / iso
     return {"requests": n_req, "steps_range": [1e3 * steps_scale, 1e5 * steps_scale], "n_range": [1e3, 2e5],
             "hbm_budget_gib": budget_gib, "precision": precision,
             "saturated": {"throughput": sat["throughput"], "wall_s": sat["wall_s"], "p50_latency_s":
                           sat["p50_latency_s"], "p99_latency_s": sat["p99_latency_s"]},
-            "lanes": {"bounds_n": list(srv.bounds), "requests": [l["requests"] for l in ld["lanes"]]},
+            "lanes": {"bounds_n": list(srv.bounds), "requests": [l["requests"] for l in ld["lanes"]],
+                      "cu_ranges": srv.cu_ranges},
             "load": load, "arrival_rate_per_s": rate, "throughput": ld["throughput"], "wall_s": ld["wall_s"],
             "p50_latency_s": ld["p50_latency_s"], "p99_latency_s": ld["p99_latency_s"],
             "isolated_p50_s": float(np.percentile(iso, 50)), "isolated_p99_s": float(np.percentile(iso, 99)),
             "latency_over_isolated_p50": float(np.percentile(ratio, 50)),
-            "latency_over_isolated_p99": float(np.percentile(ratio, 99))}
+            "latency_over_isolated_p99": float(np.percentile(ratio, 99)),
+            "worst": [{"n": q.n, "steps": q.steps, "lane": srv.lane_of(q), "latency_s": float(q.latency),
+                       "isolated_s": float(i)} for q, i in sorted(zip(reqs, iso), key=lambda x: -x[0].latency / x[1])[:3]]}
 
 
 def cpu_baseline(sysinfo, threads):

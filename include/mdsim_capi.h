@@ -291,9 +291,20 @@ mds_status mds_replica_get_state(mds_ctx* ctx, int32_t index, double* pos, doubl
 /* <m_z> of every replica after the last step (src/simulation.cpp:513). With
  * several replicas, mds_run's mz_series is steps x replicas, row-major. */
 mds_status mds_replica_mz(mds_ctx* ctx, double* mz);
+/* Capacity for n_particles in total, so that adding and removing replicas
+ * below it allocates and frees no device memory (freeing waits for the whole
+ * device, which stalls other contexts on it). */
+mds_status mds_reserve(mds_ctx* ctx, int64_t n_particles);
 /* With on != 0, mds_run (fixedDt) captures one step into a HIP graph and
  * replays it (re-captured after any change of replicas, parameters or dt). */
 mds_status mds_set_graph(mds_ctx* ctx, int32_t on);
+/* Restrict the context's work to the compute units set in mask (words x 32
+ * bits, bit i = CU i; words = 0 restores all CUs). Contexts that serve
+ * different request classes on one device can so be kept from queueing
+ * behind each other's kernels. */
+mds_status mds_set_cu_mask(mds_ctx* ctx, const uint32_t* mask, int32_t words);
+/* Compute units of HIP device `device` (0 when unknown). */
+int32_t mds_device_cus(int device);
 /* Device memory held by the context, in bytes. */
 int64_t mds_device_bytes(const mds_ctx* ctx);
 

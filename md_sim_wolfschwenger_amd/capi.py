@@ -116,6 +116,9 @@ SIGNATURES = [
                                    C.POINTER(C.c_uint64)]),
     ("mds_replica_get_state", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp]),
     ("mds_replica_mz", C.c_int, [C.c_void_p, _dp]),
+    ("mds_reserve", C.c_int, [C.c_void_p, C.c_int64]),
+    ("mds_set_cu_mask", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_int32]),
+    ("mds_device_cus", C.c_int32, [C.c_int]),
     ("mds_set_graph", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_device_bytes", C.c_int64, [C.c_void_p]),
 ]
@@ -406,6 +409,20 @@ class Context:
         mz = np.zeros(self.replica_count())
         self._chk(self._L.mds_replica_mz(self.h, _p(mz)), "replica_mz")
         return mz
+
+    def reserve(self, n_particles):
+        self._chk(self._L.mds_reserve(self.h, int(n_particles)), "reserve")
+
+    def set_cu_mask(self, cus=None):
+        """Run on the given compute-unit ids only (None: all)."""
+        if not cus:
+            self._chk(self._L.mds_set_cu_mask(self.h, None, 0), "set_cu_mask")
+            return
+        words = (max(cus) // 32) + 1
+        m = np.zeros(words, np.uint32)
+        for c in cus:
+            m[c // 32] |= np.uint32(1 << (c % 32))
+        self._chk(self._L.mds_set_cu_mask(self.h, m.ctypes.data_as(C.POINTER(C.c_uint32)), words), "set_cu_mask")
 
     def set_graph(self, on=True):
         self._chk(self._L.mds_set_graph(self.h, 1 if on else 0), "set_graph")

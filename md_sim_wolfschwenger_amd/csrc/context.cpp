@@ -57,34 +57,34 @@ struct Seg {
 };
 
 // Rebuild a column-major array of `cols` columns (old column stride
-// old_n, new stride new_n) from segments of the old one; rows not covered
-// by a segment are left for the caller to fill. Stream-ordered.
+// old_n, new stride new_n) from segments of the old one into `spare`, then
+// swap the two; rows not covered by a segment are left for the caller to
+// fill. The old buffer is kept as the next spare, so a layout change that
+// fits the capacity allocates and frees nothing (hipFree waits for the whole
+// device, which would stall other contexts sharing it). Stream-ordered.
 template <typename T>
-hipError_t repack(DevBuf<T>& b, int cols, int64_t old_n, int64_t new_n, const std::vector<Seg>& segs,
-                  hipStream_t s) {
-    T* np = nullptr;
-    const size_t need = std::max<size_t>(1, (size_t)cols * (size_t)new_n);
-    hipError_t e = hipMalloc((void**)&np, need * sizeof(T));
+hipError_t repack(DevBuf<T>& b, DevBuf<T>& spare, int cols, int64_t old_n, int64_t new_n, int64_t min_n,
+                  const std::vector<Seg>& segs, hipStream_t s) {
+    const size_t need = std::max<size_t>(1, (size_t)cols * (size_t)std::max(new_n, min_n));
+    hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
+    if (spare.n < need) {
+        spare.release();
+        if ((e = hipMalloc((void**)&spare.p, need * sizeof(T))) != hipSuccess) return e;
+        spare.n = need;
+    }
     if (b.p && old_n > 0)
         for (int d = 0; d < cols; ++d)
             for (const Seg& g : segs)
                 if (g.len > 0) {
-                    e = hipMemcpyAsync(np + (size_t)d * new_n + g.dst, b.p + (size_t)d * old_n + g.src,
+                    e = hipMemcpyAsync(spare.p + (size_t)d * new_n + g.dst, b.p + (size_t)d * old_n + g.src,
                                        (size_t)g.len * sizeof(T), hipMemcpyDeviceToDevice, s);
-                    if (e != hipSuccess) {
-                        (void)hipFree(np);
-                        return e;
-                    }
+                    if (e != hipSuccess) return e;
                 }
     e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
-        (void)hipFree(np);
-        return e;
-    }
-    b.release();
-    b.p = np;
-    b.n = need;
+    if (e != hipSuccess) return e;
+    std::swap(b.p, spare.p);
+    std::swap(b.n, spare.n);
     return hipSuccess;
 }
 
@@ -210,6 +210,8 @@ struct mds_ctx {
     // per-particle constants and state (kept across replica changes)
     DevBuf<double> r_mag, mag_mom, zeta_rot, zeta_trans, vol_mag, c1, c2;
     DevBuf<double> pos, mm, ea, n_t, n_b, n_f;
+    DevBuf<double> spare[10];  // re-layout targets of the ten arrays above that are kept
+    int64_t reserve_n = 0;     // capacity floor in particles (mds_reserve)
     // per-step scratch
     DevBuf<double> pos0, pos_cur, force, field, mom;
     DevBuf<double4> rec_pos, rec_mom;
@@ -429,6 +431,7 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
     // e(k) = 2 exp(-w n^2) / n^2 (src/interaction.cpp:450-452)
     auto ek = [&](int64_t len) { return len > 0 ? 2.0 * std::exp(-w * (double)len) / (double)len : 0.0; };
     int32_t koff = 0;
+    std::vector<KSeg> cls[3];  // segments of 8, 4 and 2 values
     for (int nz = 0; nz <= nc; ++nz)
         for (int ny = (nz == 0 ? 0 : -nc); ny <= nc; ++ny) {
             const int64_t r2 = (int64_t)ny * ny + (int64_t)nz * nz;
@@ -438,9 +441,16 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
             const bool half = (ny == 0 && nz == 0);
             if (half && nxmax == 0) continue;
             c->h_rows.push_back({ny, nz, nxmax, (int32_t)c->h_pack.size() - R.pk_off});
-            const int a0 = half ? 1 : 0;
-            for (int a = a0; a <= nxmax; a += KSEG)
-                c->h_segs.push_back({ny, nz, a, std::min(nxmax, a + KSEG - 1), koff, nxmax});
+            // full segments of 8 values, then a tail of 5-7 (class 0), 3-4
+            // (class 1) or 1-2 (class 2) values
+            int a = half ? 1 : 0;
+            while (a <= nxmax) {
+                const int rem = nxmax - a + 1;
+                const int k = rem >= 5 ? 0 : (rem >= 3 ? 1 : 2);
+                const int len = std::min(rem, k == 0 ? 8 : (k == 1 ? 4 : 2));
+                cls[k].push_back({ny, nz, a, a + len - 1, koff, nxmax});
+                a += len;
+            }
             // scatter table: |nx| = a pairs (+a, ny, nz) with (-a, ny, nz)
             for (int a = 0; a <= nxmax; ++a) {
                 KPack d;
@@ -454,9 +464,19 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
                 d.e = d.kp >= 0 ? ek((int64_t)a * a + r2) : 0.0;
                 c->h_pack.push_back(d);
             }
+            // pad to an even entry count (the scatter takes two at a time)
+            if ((nxmax + 1) % 2) c->h_pack.push_back(KPack{-1, -1, 0.0});
             koff += half ? nxmax : 2 * nxmax + 1;
         }
     R.nrows = (int32_t)c->h_rows.size() - R.row_off;
+    int32_t nsegw = 0;
+    for (int k = 0; k < 3; ++k) {
+        R.cseg_off[k] = (int32_t)c->h_segs.size() - R.seg_off;
+        R.cnseg[k] = (int32_t)cls[k].size();
+        R.cnsegw[k] = (R.cnseg[k] + 63) / 64;
+        nsegw += R.cnsegw[k];
+        c->h_segs.insert(c->h_segs.end(), cls[k].begin(), cls[k].end());
+    }
     R.nseg = (int32_t)c->h_segs.size() - R.seg_off;
     R.npk = (int32_t)c->h_pack.size() - R.pk_off;
     R.nk = koff;
@@ -468,7 +488,7 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
     R.c_surf = f * 4.0 * pi / (3 * std::pow(L, 3.0));
     R.c_self = f * 4.0 * std::pow(m.alpha, 3.0) / (3.0 * std::sqrt(pi));
     R.two_pi_over_L = 2 * pi / L;
-    R.nsegw = (R.nseg + 63) / 64;
+    R.nsegw = nsegw;
     int64_t chunk = (m.n * std::max<int64_t>(1, R.nsegw) + 4095) / 4096;
     chunk = std::max<int64_t>(32, std::min<int64_t>(1024, chunk));
     R.kchunk = (int32_t)chunk;
@@ -574,7 +594,7 @@ mds_status plan_all(mds_ctx* c) {
     HIPCHK(c->rows.ensure(std::max<size_t>(1, c->h_rows.size())));
     HIPCHK(c->segs.ensure(std::max<size_t>(1, c->h_segs.size())));
     HIPCHK(c->pack_desc.ensure(std::max<size_t>(1, c->h_pack.size())));
-    HIPCHK(c->pack.ensure(c->h_pack.size() + 1));  // + 1: the scatter loop reads one entry ahead
+    HIPCHK(c->pack.ensure(std::max<size_t>(1, c->h_pack.size())));
     HIPCHK(c->kpartial.ensure(std::max<int64_t>(1, part)));
     HIPCHK(c->tab.ensure(std::max<int64_t>(1, tab)));
     HIPCHK(c->reps.ensure(c->nrep));
@@ -610,7 +630,7 @@ mds_status plan_all(mds_ctx* c) {
 
 // Per-particle scratch of n particles (contents not kept).
 mds_status size_scratch(mds_ctx* c) {
-    const int64_t n = c->n;
+    const int64_t n = std::max(c->n, c->reserve_n);
     const size_t n3 = 3 * (size_t)n;
     for (auto* b : {&c->pos0, &c->pos_cur, &c->force, &c->field, &c->mom, &c->new_mm, &c->new_ea}) HIPCHK(b->ensure(n3));
     HIPCHK(c->pt[0].ensure(2 * n3));
@@ -1029,20 +1049,23 @@ mds_status particle_props(mds_ctx* c, int64_t m, const double* r_mag, const doub
 // Rebuild the kept per-particle arrays for a new particle layout.
 mds_status relayout(mds_ctx* c, int64_t old_n, int64_t new_n, const std::vector<Seg>& segs) {
     hipStream_t s = c->stream;
-    for (auto* b : {&c->r_mag, &c->mag_mom, &c->zeta_rot, &c->zeta_trans, &c->vol_mag, &c->c1, &c->c2})
-        HIPCHK(repack(*b, 1, old_n, new_n, segs, s));
-    for (auto* b : {&c->pos, &c->mm, &c->ea}) HIPCHK(repack(*b, 3, old_n, new_n, segs, s));
+    const int64_t mn = c->reserve_n;
+    DevBuf<double>* one[7] = {&c->r_mag, &c->mag_mom, &c->zeta_rot, &c->zeta_trans, &c->vol_mag, &c->c1, &c->c2};
+    for (int k = 0; k < 7; ++k) HIPCHK(repack(*one[k], c->spare[k], 1, old_n, new_n, mn, segs, s));
+    DevBuf<double>* three[3] = {&c->pos, &c->mm, &c->ea};
+    for (int k = 0; k < 3; ++k) HIPCHK(repack(*three[k], c->spare[7 + k], 3, old_n, new_n, mn, segs, s));
     // injected noise does not survive a change of layout
     for (auto* b : {&c->n_t, &c->n_b, &c->n_f}) {
-        HIPCHK(b->ensure(3 * (size_t)new_n));
+        HIPCHK(b->ensure(3 * (size_t)std::max(new_n, mn)));
         HIPCHK(launch_fill_f64(s, b->p, 3 * new_n, 0.0));
     }
     return MDS_OK;
 }
 
-// Per-replica device values (time, step counter) for a new replica list.
+// Per-replica device values (time, step counter) for a new replica list
+// (c->nrep is still the old count).
 mds_status relayout_rep_values(mds_ctx* c, const std::vector<int>& keep, int added, uint64_t counter) {
-    const int old_nrep = (int)(c->rep_t.p ? c->rep_t.n : 0);
+    const int old_nrep = c->rep_t.p ? c->nrep : 0;
     std::vector<double> t_old(old_nrep);
     std::vector<uint64_t> s_old(old_nrep);
     if (old_nrep > 0) {
@@ -1062,16 +1085,15 @@ mds_status relayout_rep_values(mds_ctx* c, const std::vector<int>& keep, int add
         s_new.push_back(counter);
     }
     const size_t m = t_new.size();
-    c->rep_t.release();
-    c->rep_step.release();
-    // exact size: the allocation size records the replica count
-    HIPCHK(hipMalloc((void**)&c->rep_t.p, std::max<size_t>(1, m) * sizeof(double)));
-    c->rep_t.n = m;
-    HIPCHK(hipMalloc((void**)&c->rep_step.p, std::max<size_t>(1, m) * sizeof(uint64_t)));
-    c->rep_step.n = m;
+    if (c->rep_t.n < m) {
+        // grow with headroom; the contents are rewritten below
+        HIPCHK(c->rep_t.ensure(2 * m));
+        HIPCHK(c->rep_step.ensure(2 * m));
+    }
     if (m > 0) {
-        HIPCHK(hipMemcpy(c->rep_t.p, t_new.data(), m * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->rep_step.p, s_new.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(c->rep_t.p, t_new.data(), m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->rep_step.p, s_new.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     }
     return MDS_OK;
 }
@@ -1434,7 +1456,8 @@ mds_status mds_use_philox(mds_ctx* c, uint64_t seed, uint64_t counter) {
         c->planned = false;
     }
     std::vector<uint64_t> v(c->nrep, counter);
-    HIPCHK(hipMemcpy(c->rep_step.p, v.data(), v.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(c->rep_step.p, v.data(), v.size() * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     c->drop_graph();
     return MDS_OK;
 }
@@ -1691,8 +1714,10 @@ mds_status mds_replica_info(mds_ctx* c, int32_t index, int64_t* n, int64_t* offs
     if (t || steps) {
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipStreamSynchronize(c->stream));
-        if (t) HIPCHK(hipMemcpy(t, c->rep_t.p + index, sizeof(double), hipMemcpyDeviceToHost));
-        if (steps) HIPCHK(hipMemcpy(steps, c->rep_step.p + index, sizeof(uint64_t), hipMemcpyDeviceToHost));
+        if (t) HIPCHK(hipMemcpyAsync(t, c->rep_t.p + index, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (steps)
+            HIPCHK(hipMemcpyAsync(steps, c->rep_step.p + index, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     }
     return MDS_OK;
 }
@@ -1715,6 +1740,47 @@ mds_status mds_replica_mz(mds_ctx* c, double* mz) {
     HIPCHK(hipMemcpyAsync(mz, c->mz_out.p, c->nrep * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return MDS_OK;
+}
+
+mds_status mds_reserve(mds_ctx* c, int64_t n_particles) {
+    if (!c || n_particles < 0 || n_particles > ((int64_t)1 << 25)) return fail(c, MDS_EINVAL, "bad reservation");
+    HIPCHK(hipSetDevice(c->device));
+    c->reserve_n = n_particles;
+    if (c->n > 0) {
+        // grow the kept arrays now (identity layout)
+        std::vector<Seg> segs{{0, 0, c->n}};
+        mds_status st = relayout(c, c->n, c->n, segs);
+        if (st != MDS_OK) return st;
+    }
+    mds_status st = size_scratch(c);
+    if (st != MDS_OK) return st;
+    for (int k = 0; k < 10; ++k) {
+        const size_t need = (size_t)(k < 7 ? 1 : 3) * (size_t)std::max<int64_t>(1, n_particles);
+        if (c->spare[k].n < need) HIPCHK(c->spare[k].ensure(need));
+    }
+    return MDS_OK;
+}
+
+mds_status mds_set_cu_mask(mds_ctx* c, const uint32_t* mask, int32_t words) {
+    if (!c || words < 0 || (words > 0 && !mask)) return fail(c, MDS_EINVAL, "bad CU mask");
+    HIPCHK(hipSetDevice(c->device));
+    c->collect();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->drop_graph();
+    hipStream_t s = nullptr;
+    if (words > 0)
+        HIPCHK(hipExtStreamCreateWithCUMask(&s, (uint32_t)words, mask));
+    else
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    (void)hipStreamDestroy(c->stream);
+    c->stream = s;
+    return MDS_OK;
+}
+
+int32_t mds_device_cus(int device) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    return v;
 }
 
 mds_status mds_set_graph(mds_ctx* c, int32_t on) {
@@ -1746,6 +1812,7 @@ int64_t mds_device_bytes(const mds_ctx* c) {
     add(c->segs.n, sizeof(KSeg));
     add(c->pack_desc.n, sizeof(KPack));
     add(c->scan_tmp.n, sizeof(int32_t));
+    for (const auto& b : c->spare) add(b.n, sizeof(double));
     add(c->red_chunks.n, sizeof(int2));
     add(c->red_part.n, sizeof(double));
     add(c->pack.n, sizeof(double4));

@@ -24,6 +24,13 @@ __device__ __forceinline__ double min_image(double v, double L) {
     return v;
 }
 
+// The same shift for |v| < L as v - L rint(v / L) (v - L and v + L are the
+// same roundings as above; only |v| = L/2 exactly may differ, where the
+// reference's choice is arbitrary too): three instructions, no selects.
+__device__ __forceinline__ double min_image_r(double v, double L, double inv_L) {
+    return fma(-rint(v * inv_L), L, v);
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -533,8 +540,51 @@ __device__ __forceinline__ void version_pair(const Geometry& g, int32_t lo, int3
     }
 }
 
+// 1/d by the hardware reciprocal and two Newton steps (d > 0, normal)
+__device__ __forceinline__ double rcp64(double d) {
+    double y = __builtin_amdgcn_rcp(d);
+    y = fma(y, fma(-d, y, 1.0), y);
+    y = fma(y, fma(-d, y, 1.0), y);
+    return y;
+}
+
+// 1/sqrt(q) by the hardware estimate and two Newton steps (q > 0, normal)
+__device__ __forceinline__ double rsqrt64(double q) {
+    double y = __builtin_amdgcn_rsq(q);
+    const double h = 0.5 * q;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
+// exp(-y) for 0 <= y < 700: Cody-Waite reduction by ln 2 (ln 2 split into a
+// 32-bit head and its remainder, so k * head is exact) to |f| <= ln2 / 2 and
+// the degree-12 Taylor polynomial of e^f (truncation < 2e-16 relative).
+__device__ __forceinline__ double exp_neg(double y) {
+    constexpr double kLog2e = 1.4426950408889634;
+    constexpr double kLn2Hi = 0x1.62e42ff000000p-1;
+    constexpr double kLn2Lo = -0x1.718432a1b0e26p-35;
+    const double k = rint(y * kLog2e);
+    double f = fma(k, kLn2Hi, -y);  // k ln2 - y
+    f = fma(k, kLn2Lo, f);
+    double p = 2.08767569878681e-09;
+    p = fma(p, f, 2.505210838544172e-08);
+    p = fma(p, f, 2.755731922398589e-07);
+    p = fma(p, f, 2.7557319223985893e-06);
+    p = fma(p, f, 2.48015873015873e-05);
+    p = fma(p, f, 0.0001984126984126984);
+    p = fma(p, f, 0.001388888888888889);
+    p = fma(p, f, 0.008333333333333333);
+    p = fma(p, f, 0.041666666666666664);
+    p = fma(p, f, 0.16666666666666666);
+    p = fma(p, f, 0.5);
+    p = fma(p, f, 1.0);
+    p = fma(p, f, 1.0);
+    return ldexp(p, -(int)k);
+}
+
 __device__ __forceinline__ double erfcx64(double x) {
-    const double t = 1.0 - 2.0 * kErfcxK64 / (x + kErfcxK64);
+    const double t = fma(-2.0 * kErfcxK64, rcp64(x + kErfcxK64), 1.0);
     double p = kErfcx64[kErfcxDeg64];
 #pragma unroll
     for (int k = kErfcxDeg64 - 1; k >= 0; --k) p = fma(p, t, kErfcx64[k]);
@@ -589,7 +639,7 @@ __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, doubl
     if (!MIXED) {
         const double ir2 = ir * ir;
         const double x = pp.alpha * r;
-        const double gau = exp(-x * x);
+        const double gau = exp_neg(x * x);
         const double t = 2.0 * pp.alpha * gau * pp.ir_pi * ir2;
         const double a1 = gau * erfcx64(x) * (ir2 * ir) + t;
         const double a2 = 3.0 * a1 * ir2 + 2.0 * pp.alpha2 * t;
@@ -672,10 +722,10 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
     if (MIXED && !COATING) {
         // separation in FP64 (positions are absolute), everything after it in
         // FP32 in units of ell = r_c, moments in units of mref (FP32 copies)
-        const double L = g.L;
-        const float vx = (float)(min_image(pj.x - pi.x, L) * pp.inv_ell);
-        const float vy = (float)(min_image(pj.y - pi.y, L) * pp.inv_ell);
-        const float vz = (float)(min_image(pj.z - pi.z, L) * pp.inv_ell);
+        const double L = g.L, iL = g.inv_L;
+        const float vx = (float)(min_image_r(pj.x - pi.x, L, iL) * pp.inv_ell);
+        const float vy = (float)(min_image_r(pj.y - pi.y, L, iL) * pp.inv_ell);
+        const float vz = (float)(min_image_r(pj.z - pi.z, L, iL) * pp.inv_ell);
         const float r2 = vx * vx + vy * vy + vz * vz;
         if (!(r2 < 1.0f)) return;
         ++npairs;
@@ -702,14 +752,15 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         return;
     }
     // canonical i < j orientation of CompInteractVecs (src/interaction.cpp:99-104)
-    const double L = g.L;
-    const double vx = min_image(i_lo ? pj.x - pi.x : pi.x - pj.x, L);
-    const double vy = min_image(i_lo ? pj.y - pi.y : pi.y - pj.y, L);
-    const double vz = min_image(i_lo ? pj.z - pi.z : pi.z - pj.z, L);
-    const double r = sqrt(vx * vx + vy * vy + vz * vz);
+    const double L = g.L, iL = g.inv_L;
+    const double vx = min_image_r(i_lo ? pj.x - pi.x : pi.x - pj.x, L, iL);
+    const double vy = min_image_r(i_lo ? pj.y - pi.y : pi.y - pj.y, L, iL);
+    const double vz = min_image_r(i_lo ? pj.z - pi.z : pi.z - pj.z, L, iL);
+    const double r2 = vx * vx + vy * vy + vz * vz;
+    const double ir = rsqrt64(r2);
+    const double r = r2 * ir;
     if (!(r < pp.rc)) return;
     ++npairs;
-    const double ir = 1.0 / r;
     const double sir = i_lo ? ir : -ir;  // u from i to j
     const double ux = vx * sir, uy = vy * sir, uz = vz * sir;
     double B[3], F[3];
@@ -743,9 +794,9 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
 }
 
 template <bool FORCE, bool COATING, bool MIXED>
-__global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
-                                               PairOut out, const int4* __restrict__ stencil_all,
-                                               unsigned long long* __restrict__ cnt_pairs) {
+__device__ __forceinline__ void pairs_block(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
+                                            PairOut out, const int4* __restrict__ stencil_all,
+                                            unsigned long long* __restrict__ cnt_pairs) {
     // One block per chunk of up to 64 particles of one cell; its 4 wavefronts
     // split the stencil between them and merge their sums in LDS at the end.
     __shared__ IWave waves[4];
@@ -761,6 +812,7 @@ __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __rest
     // the replica of this cell: its box, grid, stencil and Ewald constants
     const Rep& R = g.reps[g.cell_rep[cell]];
     g.L = R.L;
+    g.inv_L = 1.0 / R.L;
     apply_rep(pp, R);
     const int4* stencil = stencil_all + R.sten_off;
     const int32_t nsten = R.nsten;
@@ -887,6 +939,23 @@ __global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __rest
             out.press_partial[3 * (int64_t)w + lane] =
                 ((wpress[0][lane] + wpress[1][lane]) + (wpress[2][lane] + wpress[3][lane])) * pp.pscale;
     }
+}
+
+// Five wavefronts per SIMD (the LDS allows five blocks per CU): the FP64
+// instantiations are held to the matching register budget.
+template <bool FORCE, bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_pairs(
+    Geometry g, const double4* __restrict__ rec_mom, PairParams pp, PairOut out, const int4* __restrict__ stencil_all,
+    unsigned long long* __restrict__ cnt_pairs) {
+    pairs_block<FORCE, false, MIXED>(g, rec_mom, pp, out, stencil_all, cnt_pairs);
+}
+
+// With the coating potential (short-range forces, force pass only).
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_pairs_coat(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
+                                                    PairOut out, const int4* __restrict__ stencil_all,
+                                                    unsigned long long* __restrict__ cnt_pairs) {
+    pairs_block<true, true, MIXED>(g, rec_mom, pp, out, stencil_all, cnt_pairs);
 }
 
 // Pairs with at least one endpoint moved by the overlap correction, in FP64
@@ -1041,16 +1110,12 @@ __device__ __forceinline__ int find_rep(const Rep* __restrict__ reps, int32_t nr
     return lo;
 }
 
+// One wavefront of row segments of one class over one particle chunk.
 template <int SEG>
-__global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restrict__ reps, int32_t nrep,
-                                               const KSeg* __restrict__ segs_all, const double* __restrict__ tab_all,
-                                               const double* __restrict__ mom, double* __restrict__ partial_all) {
-    // work item = (replica, wavefront of row segments, particle chunk)
-    const Rep& R = reps[find_rep<0>(reps, nrep, blockIdx.x)];
-    const int32_t li = (int32_t)blockIdx.x - R.kb_off;
-    const int32_t segw = li % R.nsegw, ch = li / R.nsegw;
-    const int32_t nc = R.nc, nseg = R.nseg, chunk = R.kchunk, nk = R.nk;
-    const KSeg* segs = segs_all + R.seg_off;
+__device__ __forceinline__ void kbuild_wave(const Rep& R, int64_t n_all, const KSeg* __restrict__ segs, int32_t nseg,
+                                            int32_t segw, int32_t ch, const double* __restrict__ tab_all,
+                                            const double* __restrict__ mom, double* __restrict__ partial_all) {
+    const int32_t nc = R.nc, chunk = R.kchunk, nk = R.nk;
     const double* tab = tab_all + R.tab_off;
     const int64_t n = n_all;
     const int r = segw * WAVE + threadIdx.x;
@@ -1108,6 +1173,32 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
             out[2 * km + 1] = ms_[q];
         }
     }
+}
+
+__global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restrict__ reps, int32_t nrep,
+                                               int32_t kb_items, const KSeg* __restrict__ segs_all,
+                                               const double* __restrict__ tab_all, const double* __restrict__ mom,
+                                               double* __restrict__ partial_all) {
+    // work item = (replica, segment class, particle chunk, wavefront of
+    // segments); the class is uniform over the block
+    const int32_t item = blockIdx.x;
+    if (item >= kb_items) return;
+    const Rep& R = reps[find_rep<0>(reps, nrep, item)];
+    int32_t li = item - R.kb_off;
+    int c = 0;
+    while (c < 2 && li >= R.cnsegw[c] * R.nchunks) {
+        li -= R.cnsegw[c] * R.nchunks;
+        ++c;
+    }
+    const int32_t nw = R.cnsegw[c];
+    const int32_t segw = li % nw, ch = li / nw;
+    const KSeg* segs = segs_all + R.seg_off + R.cseg_off[c];
+    if (c == 0)
+        kbuild_wave<8>(R, n_all, segs, R.cnseg[0], segw, ch, tab_all, mom, partial_all);
+    else if (c == 1)
+        kbuild_wave<4>(R, n_all, segs, R.cnseg[1], segw, ch, tab_all, mom, partial_all);
+    else
+        kbuild_wave<2>(R, n_all, segs, R.cnseg[2], segw, ch, tab_all, mom, partial_all);
 }
 
 // Sum the S(k) partials of both k vectors of a packed entry over the
@@ -1186,26 +1277,28 @@ __device__ __forceinline__ void kscatter_one(int64_t n, int64_t i, int32_t rid, 
         const double4* P = pack + row.poff;
         double xc = 1.0, xs = 0.0, ad = 0.0, am = 0.0;
         double rb = 0.0, rs = 0.0;
-        // the next entry is loaded one iteration ahead (the table carries
-        // one entry of padding past its end)
-        double4 pn = P[0];
-        for (int a = 0; a <= row.nxmax; ++a) {
-            const double4 p = pn;
-            pn = P[a + 1];
-            const double G1 = p.x * wc + p.z * ws, G2 = p.w * wc - p.y * ws;
-            const double G3 = p.y * wc + p.w * ws, G4 = p.z * wc - p.x * ws;
-            rb += xc * G1 + xs * G2;
-            B0 += ad * (xc * G3 + xs * G4);
-            if (FORCE) {
-                const double ts = xs * G3 - xc * G4, td = xs * G1 - xc * G2;
-                rs += A * ts + am * td;
-                F0 += ad * (A * td + am * ts);
-                am += mx;
+        // two entries per iteration (rows are padded with zero entries to an
+        // even length), loaded together so one wait covers both
+        for (int a = 0; a <= row.nxmax; a += 2) {
+            const double4 p2[2] = {P[a], P[a + 1]};
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const double4 p = p2[u];
+                const double G1 = p.x * wc + p.z * ws, G2 = p.w * wc - p.y * ws;
+                const double G3 = p.y * wc + p.w * ws, G4 = p.z * wc - p.x * ws;
+                rb += xc * G1 + xs * G2;
+                B0 += ad * (xc * G3 + xs * G4);
+                if (FORCE) {
+                    const double ts = xs * G3 - xc * G4, td = xs * G1 - xc * G2;
+                    rs += A * ts + am * td;
+                    F0 += ad * (A * td + am * ts);
+                    am += mx;
+                }
+                ad += 1.0;
+                const double nxc = xc * ex - xs * sx, nxs = xs * ex + xc * sx;
+                xc = nxc;
+                xs = nxs;
             }
-            ad += 1.0;
-            const double nxc = xc * ex - xs * sx, nxs = xs * ex + xc * sx;
-            xc = nxc;
-            xs = nxs;
         }
         B1 += row.ny * rb;
         B2 += row.nz * rb;
@@ -1619,16 +1712,18 @@ hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geom
     if (blocks <= 0) return hipSuccess;
     unsigned long long* cp = (unsigned long long*)cnt_pairs;
     const bool mixed = precision == 1;
-#define KP(F, C, M) \
-    hipLaunchKernelGGL((k_pairs<F, C, M>), dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out, stencil, cp)
+#define KP(F, M) \
+    hipLaunchKernelGGL((k_pairs<F, M>), dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out, stencil, cp)
+#define KC(M) hipLaunchKernelGGL((k_pairs_coat<M>), dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out, stencil, cp)
     if (force && pp.coating) {
-        if (mixed) KP(true, true, true); else KP(true, true, false);
+        if (mixed) KC(true); else KC(false);
     } else if (force) {
-        if (mixed) KP(true, false, true); else KP(true, false, false);
+        if (mixed) KP(true, true); else KP(true, false);
     } else {
-        if (mixed) KP(false, false, true); else KP(false, false, false);
+        if (mixed) KP(false, true); else KP(false, false);
     }
 #undef KP
+#undef KC
     LAUNCH_CHECK();
 }
 
@@ -1658,7 +1753,7 @@ hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const
 hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t kb_items, const KSeg* segs,
                          const double* tab, const double* mom, double* partial) {
     if (kb_items <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_kbuild<KSEG>, dim3(kb_items), dim3(WAVE), 0, s, n, reps, nrep, segs, tab, mom, partial);
+    hipLaunchKernelGGL(k_kbuild, dim3(kb_items), dim3(WAVE), 0, s, n, reps, nrep, kb_items, segs, tab, mom, partial);
     LAUNCH_CHECK();
 }
 

@@ -95,7 +95,7 @@ class Server:
 
     def __init__(self, precision: str = "fp64", hbm_budget: float = 64 * 2 ** 30, chunk_seconds: float = 0.25,
                  rc_scale: float = 0.225, dt: float = 1e-12, graph: bool = True, base: str = "c1", device: int = 0,
-                 budget: Budget | None = None):
+                 budget: Budget | None = None, reserve_particles: int = 0):
         self.cfg = capi.config_from_string(workload.config_text(base, precision=precision))
         self.precision = precision
         self.hbm_budget = hbm_budget
@@ -107,6 +107,8 @@ class Server:
         self.ctx.set_tiling(0)
         self.graph = graph
         self.ctx.set_graph(graph)
+        if reserve_particles > 0:
+            self.ctx.reserve(reserve_particles)
         self.active: list[Request] = []
         self.done_steps: dict[int, int] = {}
         self.step_time = None  # s per step of the active set, per particle
@@ -252,12 +254,27 @@ class LaneServer:
     boundaries)."""
 
     def __init__(self, precision: str = "fp64", hbm_budget: float = 64 * 2 ** 30,
-                 bounds: tuple = (6e3, 4e4), chunk_seconds: tuple = (0.01, 0.04, 0.2), **kw):
+                 bounds: tuple = (6e3, 4e4), chunk_seconds: tuple = (0.01, 0.04, 0.2),
+                 cu_fractions: tuple | None = None, device: int = 0, **kw):
+        """cu_fractions: share of the device's compute units given to each
+        lane (disjoint ranges of CU ids); None lets every lane use all."""
         assert len(chunk_seconds) == len(bounds) + 1
         self.bounds = bounds
         self.budget = Budget(hbm_budget)
-        self.lanes = [Server(precision=precision, hbm_budget=hbm_budget, chunk_seconds=cs, budget=self.budget, **kw)
-                      for cs in chunk_seconds]
+        # each lane reserves room for a few of its largest requests, so
+        # admissions and retirements reuse device memory
+        tops = list(bounds) + [2e5]
+        self.lanes = [Server(precision=precision, hbm_budget=hbm_budget, chunk_seconds=cs, budget=self.budget,
+                             reserve_particles=int(4 * top), device=device, **kw)
+                      for cs, top in zip(chunk_seconds, tops)]
+        self.cu_ranges = None
+        if cu_fractions:
+            assert len(cu_fractions) == len(self.lanes)
+            total = capi.lib().mds_device_cus(device)
+            edges = np.round(np.cumsum([0.0] + list(cu_fractions)) / sum(cu_fractions) * total).astype(int)
+            self.cu_ranges = [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:])]
+            for lane, (a, b) in zip(self.lanes, self.cu_ranges):
+                lane.ctx.set_cu_mask(list(range(a, max(b, a + 1))))
 
     def lane_of(self, req: Request) -> int:
         k = 0
@@ -297,22 +314,24 @@ class LaneServer:
         return res
 
 
-def isolated_seconds(server_kwargs: dict, req: Request, probe_steps: int = 5) -> float:
-    """Wall time of `req` solved alone (its own context, same settings):
-    probe_steps timed after one warm-up step, scaled to req.steps."""
-    s = Server(**server_kwargs)
+def isolated_latencies(requests: list[Request], precision: str = "fp64", **kw) -> list[float]:
+    """Latency of each request served alone: arriving at an idle server
+    (context already created) and leaving when its state is copied out, so
+    admission (initial suspension, planning, graph capture) and retirement
+    count as they do in the loop. The requests are solved one after another
+    on copies; the originals are not modified."""
+    import copy
+    s = Server(precision=precision, **kw)
+    out = []
     try:
-        q = Request(req.rid, req.n, probe_steps + 1, req.field_z, req.phi, req.seed)
-        s.admit([q], 0.0)
-        s.ctx.run(1, 0.0, s.dt)
-        s.ctx.synchronize()
-        t0 = time.perf_counter()
-        s.ctx.run(probe_steps, 0.0, s.dt)
-        s.ctx.synchronize()
-        per_step = (time.perf_counter() - t0) / probe_steps
+        for q in requests:
+            a = copy.deepcopy(q)
+            a.arrival, a.state, a.ewald = 0.0, None, {}
+            s.serve([a])
+            out.append(a.latency)
     finally:
         s.close()
-    return per_step * req.steps
+    return out
 
 
 def poisson_stream(n_requests: int, rate: float, rng: np.random.Generator, n_range=(1e3, 2e5),
