@@ -92,8 +92,8 @@ def tune(ctx, info, args):
 # kernel class -> kernel symbol (short form) per path, for the committed
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summary
 SYMBOL = {
-    ("pairs_force", "fp64"): "k_pairs<true,false,false>", ("pairs_force", "mixed"): "k_pairs<true,false,true>",
-    ("pairs_field", "fp64"): "k_pairs<false,false,false>", ("pairs_field", "mixed"): "k_pairs<false,false,true>",
+    ("pairs_force", "fp64"): "k_pairs<true,false>", ("pairs_force", "mixed"): "k_pairs<true,true>",
+    ("pairs_field", "fp64"): "k_pairs<false,false>", ("pairs_field", "mixed"): "k_pairs<false,true>",
     ("kbuild", "fp64"): "k_kbuild<8>", ("kbuild", "mixed"): "k_kbuild<8>",
     ("kscatter_force", "fp64"): "k_kscatter<true>", ("kscatter_force", "mixed"): "k_kscatter<true>",
     ("kscatter_field", "fp64"): "k_kscatter<false>", ("kscatter_field", "mixed"): "k_kscatter<false>",
@@ -235,14 +235,14 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, load=0.8, seed=202612
             "saturated": {"throughput": sat["throughput"], "wall_s": sat["wall_s"], "p50_latency_s":
                           sat["p50_latency_s"], "p99_latency_s": sat["p99_latency_s"]},
             "lanes": {"bounds_n": list(srv.bounds), "requests": [l["requests"] for l in ld["lanes"]],
-                      "cu_ranges": srv.cu_ranges},
+                      "cu_ranges": srv.cu_ranges, "seconds": [l["seconds"] for l in ld["lanes"]]},
             "load": load, "arrival_rate_per_s": rate, "throughput": ld["throughput"], "wall_s": ld["wall_s"],
             "p50_latency_s": ld["p50_latency_s"], "p99_latency_s": ld["p99_latency_s"],
             "isolated_p50_s": float(np.percentile(iso, 50)), "isolated_p99_s": float(np.percentile(iso, 99)),
             "latency_over_isolated_p50": float(np.percentile(ratio, 50)),
             "latency_over_isolated_p99": float(np.percentile(ratio, 99)),
             "worst": [{"n": q.n, "steps": q.steps, "lane": srv.lane_of(q), "latency_s": float(q.latency),
-                       "isolated_s": float(i)} for q, i in sorted(zip(reqs, iso), key=lambda x: -x[0].latency / x[1])[:3]]}
+                       "wait_s": float(q.admitted - q.arrival), "isolated_s": float(i)} for q, i in sorted(zip(reqs, iso), key=lambda x: -x[0].latency / x[1])[:3]]}
 
 
 def cpu_baseline(sysinfo, threads):

@@ -431,7 +431,6 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
     // e(k) = 2 exp(-w n^2) / n^2 (src/interaction.cpp:450-452)
     auto ek = [&](int64_t len) { return len > 0 ? 2.0 * std::exp(-w * (double)len) / (double)len : 0.0; };
     int32_t koff = 0;
-    std::vector<KSeg> cls[3];  // segments of 8, 4 and 2 values
     for (int nz = 0; nz <= nc; ++nz)
         for (int ny = (nz == 0 ? 0 : -nc); ny <= nc; ++ny) {
             const int64_t r2 = (int64_t)ny * ny + (int64_t)nz * nz;
@@ -441,16 +440,8 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
             const bool half = (ny == 0 && nz == 0);
             if (half && nxmax == 0) continue;
             c->h_rows.push_back({ny, nz, nxmax, (int32_t)c->h_pack.size() - R.pk_off});
-            // full segments of 8 values, then a tail of 5-7 (class 0), 3-4
-            // (class 1) or 1-2 (class 2) values
-            int a = half ? 1 : 0;
-            while (a <= nxmax) {
-                const int rem = nxmax - a + 1;
-                const int k = rem >= 5 ? 0 : (rem >= 3 ? 1 : 2);
-                const int len = std::min(rem, k == 0 ? 8 : (k == 1 ? 4 : 2));
-                cls[k].push_back({ny, nz, a, a + len - 1, koff, nxmax});
-                a += len;
-            }
+            for (int a = half ? 1 : 0; a <= nxmax; a += KSEG)
+                c->h_segs.push_back({ny, nz, a, std::min(nxmax, a + KSEG - 1), koff, nxmax});
             // scatter table: |nx| = a pairs (+a, ny, nz) with (-a, ny, nz)
             for (int a = 0; a <= nxmax; ++a) {
                 KPack d;
@@ -469,16 +460,19 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
             koff += half ? nxmax : 2 * nxmax + 1;
         }
     R.nrows = (int32_t)c->h_rows.size() - R.row_off;
-    int32_t nsegw = 0;
-    for (int k = 0; k < 3; ++k) {
-        R.cseg_off[k] = (int32_t)c->h_segs.size() - R.seg_off;
-        R.cnseg[k] = (int32_t)cls[k].size();
-        R.cnsegw[k] = (R.cnseg[k] + 63) / 64;
-        nsegw += R.cnsegw[k];
-        c->h_segs.insert(c->h_segs.end(), cls[k].begin(), cls[k].end());
-    }
     R.nseg = (int32_t)c->h_segs.size() - R.seg_off;
     R.npk = (int32_t)c->h_pack.size() - R.pk_off;
+    // quarters of the table by entries for the four wavefronts of a scatter block
+    {
+        int32_t q = 1, acc = 0;
+        R.ksplit[0] = 0;
+        for (int r = 0; r < R.nrows && q < 4; ++r) {
+            const KRow& row = c->h_rows[R.row_off + r];
+            acc += ((row.nxmax + 2) / 2) * 2;
+            while (q < 4 && (int64_t)acc * 4 >= (int64_t)q * R.npk) R.ksplit[q++] = r + 1;
+        }
+        while (q < 4) R.ksplit[q++] = R.nrows;
+    }
     R.nk = koff;
     R.nc = nc;
     const double pi = c->cfg.my_pi, f = c->cfg.fac_ewald, L = m.L;
@@ -488,8 +482,9 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
     R.c_surf = f * 4.0 * pi / (3 * std::pow(L, 3.0));
     R.c_self = f * 4.0 * std::pow(m.alpha, 3.0) / (3.0 * std::sqrt(pi));
     R.two_pi_over_L = 2 * pi / L;
-    R.nsegw = nsegw;
-    int64_t chunk = (m.n * std::max<int64_t>(1, R.nsegw) + 4095) / 4096;
+    R.nsegw = (R.nseg + 63) / 64;
+    // particle chunks so that the build has ~8 k wavefronts (8 per SIMD)
+    int64_t chunk = (m.n * std::max<int64_t>(1, R.nsegw) + 8191) / 8192;
     chunk = std::max<int64_t>(32, std::min<int64_t>(1024, chunk));
     R.kchunk = (int32_t)chunk;
     R.nchunks = (int32_t)((m.n + chunk - 1) / chunk);

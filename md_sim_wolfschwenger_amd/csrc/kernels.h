@@ -26,9 +26,6 @@ struct KPack {
 
 // A segment of a k-space row for the structure-factor build: |nx| in
 // [alo, ahi] (both signs unless the row is ny = nz = 0), at most KSEG values.
-// Segments come in classes of 8, 4 and 2 values (a row is cut into full
-// segments of 8 and a tail of the class that fits it best), so that lanes of
-// one wavefront do about the same work.
 constexpr int KSEG = 8;
 struct KSeg {
     int32_t ny, nz, alo, ahi, koff, nxmax;
@@ -48,9 +45,7 @@ struct Rep {
     int32_t nc, row_off, nrows, seg_off, nseg, k_off, nk, kchunk, nchunks, kb_off, nsegw, pad2_;
     int32_t pk_off, npk;         // slice of the packed scatter table
     int32_t red_off, nred;       // slice of the reduction chunk table
-    // S(k) build segments of 8, 4 and 2 |nx| values (classes 0, 1, 2): first
-    // segment (relative to seg_off), count and wavefronts of each class
-    int32_t cseg_off[3], cnseg[3], cnsegw[3], pad3_;
+    int32_t ksplit[4];           // first row of each quarter of the scatter table (by entries)
     int64_t tab_off, part_off;   // offsets (doubles) into the sin/cos tables and S(k) partials
     double c_force, c_flux, c_self, c_surf, two_pi_over_L;
     double field_z;              // magFluxDens of this replica

@@ -509,12 +509,19 @@ __global__ void k_rep_mean_z(const Rep* __restrict__ reps, int32_t nrep, const d
 // the owning lane's LDS accumulator. Every pair is evaluated from its lower
 // original index (the reference's i < j orientation), so both of its
 // particles see bit-identical geometry.
+template <bool MIXED, bool COATING>
 struct IWave {
     int32_t q[128];
     double acc[6][WAVE];   // force, field of the wave's particles
+    // positions; without the coating potential .w holds the original index
+    // (bits of an int64) instead of r_mag, which only the short-range terms use
     double4 ipos[WAVE];
-    double4 imom[WAVE];
-    int32_t iorig[WAVE];
+    typename std::conditional<MIXED && !COATING, float4, double4>::type imom[WAVE];
+    int32_t iorig[COATING ? WAVE : 1];
+    __device__ int32_t orig(int l) const {
+        if constexpr (COATING) return iorig[l];
+        else return (int32_t)__double_as_longlong(ipos[l].w);
+    }
 };
 
 // Positions of lo and hi as the reference's sequential scan sees them when it
@@ -708,18 +715,19 @@ __device__ __forceinline__ void apply_rep(PairParams& p, const Rep& R) {
 // endpoint moved by the overlap correction are left to k_pairs_moved.
 template <bool FORCE, bool COATING, bool MIXED>
 __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __restrict__ rec_mom, const PairParams& pp,
-                                           IWave& W, int32_t entry, bool active, bool moved_any, double press[3],
+                                           IWave<MIXED, COATING>& W, int32_t entry, bool active, bool moved_any,
+                                           double press[3],
                                            int64_t& npairs) {
     if (!active) return;
     const int il = entry & 63;
     const int32_t j = entry >> 6;
-    const int32_t oi = W.iorig[il];
+    const int32_t oi = W.orig(il);
     const int32_t oj = g.sorted_idx[j];
     if (moved_any && (g.ev_head[oi] >= 0 || g.ev_head[oj] >= 0)) return;
     const double4 pi = W.ipos[il];
     const double4 pj = g.rec_pos[j];
     const bool i_lo = oi < oj;
-    if (MIXED && !COATING) {
+    if constexpr (MIXED && !COATING) {
         // separation in FP64 (positions are absolute), everything after it in
         // FP32 in units of ell = r_c, moments in units of mref (FP32 copies)
         const double L = g.L, iL = g.inv_L;
@@ -732,7 +740,7 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         const float irs = __builtin_amdgcn_rsqf(r2);
         const float rs = r2 * irs;
         const float ux = vx * irs, uy = vy * irs, uz = vz * irs;
-        const float4 mi = reinterpret_cast<const float4*>(W.imom)[il];
+        const float4 mi = W.imom[il];
         const float4 mj = g.rec_mom32[j];
         float B[3], F[3];
         pair_terms32<FORCE>(pp, rs, irs, ux, uy, uz, mi.x, mi.y, mi.z, mj.x, mj.y, mj.z, B, F);
@@ -749,47 +757,47 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
             atomicAdd(&W.acc[1][il], -(double)F[1]);
             atomicAdd(&W.acc[2][il], -(double)F[2]);
         }
-        return;
-    }
-    // canonical i < j orientation of CompInteractVecs (src/interaction.cpp:99-104)
-    const double L = g.L, iL = g.inv_L;
-    const double vx = min_image_r(i_lo ? pj.x - pi.x : pi.x - pj.x, L, iL);
-    const double vy = min_image_r(i_lo ? pj.y - pi.y : pi.y - pj.y, L, iL);
-    const double vz = min_image_r(i_lo ? pj.z - pi.z : pi.z - pj.z, L, iL);
-    const double r2 = vx * vx + vy * vy + vz * vz;
-    const double ir = rsqrt64(r2);
-    const double r = r2 * ir;
-    if (!(r < pp.rc)) return;
-    ++npairs;
-    const double sir = i_lo ? ir : -ir;  // u from i to j
-    const double ux = vx * sir, uy = vy * sir, uz = vz * sir;
-    double B[3], F[3];
-    pair_terms<FORCE, MIXED>(pp, r, ir, ux, uy, uz, W.imom[il], rec_mom[j], B, F);
-    atomicAdd(&W.acc[3][il], B[0]);
-    atomicAdd(&W.acc[4][il], B[1]);
-    atomicAdd(&W.acc[5][il], B[2]);
-    if (FORCE) {
-        if (i_lo) {
-            const double rr = MIXED ? r * pp.inv_ell : r;
-            press[0] += rr * ux * F[1];
-            press[1] += rr * ux * F[2];
-            press[2] += rr * uy * F[2];
-        }
-        if (COATING && r < pp.rc_sr) {
-            const double sf = short_range(pp, r, i_lo ? pi.w : pj.w, i_lo ? pj.w : pi.w) / pp.fscale;
-            F[0] += sf * ux;
-            F[1] += sf * uy;
-            F[2] += sf * uz;
+    } else {
+        // canonical i < j orientation of CompInteractVecs (src/interaction.cpp:99-104)
+        const double L = g.L, iL = g.inv_L;
+        const double vx = min_image_r(i_lo ? pj.x - pi.x : pi.x - pj.x, L, iL);
+        const double vy = min_image_r(i_lo ? pj.y - pi.y : pi.y - pj.y, L, iL);
+        const double vz = min_image_r(i_lo ? pj.z - pi.z : pi.z - pj.z, L, iL);
+        const double r2 = vx * vx + vy * vy + vz * vz;
+        const double ir = rsqrt64(r2);
+        const double r = r2 * ir;
+        if (!(r < pp.rc)) return;
+        ++npairs;
+        const double sir = i_lo ? ir : -ir;  // u from i to j
+        const double ux = vx * sir, uy = vy * sir, uz = vz * sir;
+        double B[3], F[3];
+        pair_terms<FORCE, MIXED>(pp, r, ir, ux, uy, uz, W.imom[il], rec_mom[j], B, F);
+        atomicAdd(&W.acc[3][il], B[0]);
+        atomicAdd(&W.acc[4][il], B[1]);
+        atomicAdd(&W.acc[5][il], B[2]);
+        if (FORCE) {
             if (i_lo) {
-                const double rr = r * pp.inv_ell;
-                press[0] += rr * ux * (sf * uy);
-                press[1] += rr * ux * (sf * uz);
-                press[2] += rr * uy * (sf * uz);
+                const double rr = MIXED ? r * pp.inv_ell : r;
+                press[0] += rr * ux * F[1];
+                press[1] += rr * ux * F[2];
+                press[2] += rr * uy * F[2];
             }
+            if (COATING && r < pp.rc_sr) {
+                const double sf = short_range(pp, r, i_lo ? pi.w : pj.w, i_lo ? pj.w : pi.w) / pp.fscale;
+                F[0] += sf * ux;
+                F[1] += sf * uy;
+                F[2] += sf * uz;
+                if (i_lo) {
+                    const double rr = r * pp.inv_ell;
+                    press[0] += rr * ux * (sf * uy);
+                    press[1] += rr * ux * (sf * uz);
+                    press[2] += rr * uy * (sf * uz);
+                }
+            }
+            atomicAdd(&W.acc[0][il], -F[0]);
+            atomicAdd(&W.acc[1][il], -F[1]);
+            atomicAdd(&W.acc[2][il], -F[2]);
         }
-        atomicAdd(&W.acc[0][il], -F[0]);
-        atomicAdd(&W.acc[1][il], -F[1]);
-        atomicAdd(&W.acc[2][il], -F[2]);
     }
 }
 
@@ -799,11 +807,11 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
                                             unsigned long long* __restrict__ cnt_pairs) {
     // One block per chunk of up to 64 particles of one cell; its 4 wavefronts
     // split the stencil between them and merge their sums in LDS at the end.
-    __shared__ IWave waves[4];
+    __shared__ IWave<MIXED, COATING> waves[4];
     __shared__ double wpress[4][3];
     const int wv = threadIdx.x / WAVE;
     const int lane = lane_id();
-    IWave& W = waves[wv];
+    IWave<MIXED, COATING>& W = waves[wv];
     const int32_t w = blockIdx.x;
     const int32_t nwork = *g.nwork;
     if (w >= nwork) return;  // uniform over the block
@@ -823,12 +831,18 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     double4 pi = make_double4(0, 0, 0, 0);
     if (vi) {
         pi = g.rec_pos[k];
-        W.ipos[lane] = pi;
-        if (MIXED && !COATING)
-            reinterpret_cast<float4*>(W.imom)[lane] = g.rec_mom32[k];
-        else
+        const int32_t o = g.sorted_idx[k];
+        if constexpr (COATING) {
+            W.ipos[lane] = pi;
             W.imom[lane] = rec_mom[k];
-        W.iorig[lane] = g.sorted_idx[k];
+            W.iorig[lane] = o;
+        } else {
+            W.ipos[lane] = make_double4(pi.x, pi.y, pi.z, __longlong_as_double((long long)o));
+            if constexpr (MIXED)
+                W.imom[lane] = g.rec_mom32[k];
+            else
+                W.imom[lane] = rec_mom[k];
+        }
     }
     for (int d = 0; d < 6; ++d) W.acc[d][lane] = 0.0;
     const bool moved_any = *g.n_events > 0;
@@ -921,7 +935,7 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     __syncthreads();
     if (wv == 0) {
         if (vi) {
-            const int32_t o = W.iorig[lane];
+            const int32_t o = W.orig(lane);
             const int64_t n = g.n;
             double a[6];
             for (int d = 0; d < 6; ++d)
@@ -1110,11 +1124,22 @@ __device__ __forceinline__ int find_rep(const Rep* __restrict__ reps, int32_t nr
     return lo;
 }
 
-// One wavefront of row segments of one class over one particle chunk.
 template <int SEG>
-__device__ __forceinline__ void kbuild_wave(const Rep& R, int64_t n_all, const KSeg* __restrict__ segs, int32_t nseg,
-                                            int32_t segw, int32_t ch, const double* __restrict__ tab_all,
-                                            const double* __restrict__ mom, double* __restrict__ partial_all) {
+__global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restrict__ reps, int32_t nrep,
+                                               int32_t kb_items, const KSeg* __restrict__ segs_all,
+                                               const double* __restrict__ tab_all, const double* __restrict__ mom,
+                                               double* __restrict__ partial_all) {
+    // work item = (replica, particle chunk, wavefront of row segments); the
+    // segment waves of one chunk are adjacent blocks. (An XCD-contiguous
+    // block order and segment classes of 8/4/2 values were both measured
+    // slower.)
+    const int32_t item = blockIdx.x;
+    if (item >= kb_items) return;
+    const Rep& R = reps[find_rep<0>(reps, nrep, item)];
+    const int32_t li = item - R.kb_off;
+    const int32_t segw = li % R.nsegw, ch = li / R.nsegw;
+    const KSeg* segs = segs_all + R.seg_off;
+    const int32_t nseg = R.nseg;
     const int32_t nc = R.nc, chunk = R.kchunk, nk = R.nk;
     const double* tab = tab_all + R.tab_off;
     const int64_t n = n_all;
@@ -1175,32 +1200,6 @@ __device__ __forceinline__ void kbuild_wave(const Rep& R, int64_t n_all, const K
     }
 }
 
-__global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restrict__ reps, int32_t nrep,
-                                               int32_t kb_items, const KSeg* __restrict__ segs_all,
-                                               const double* __restrict__ tab_all, const double* __restrict__ mom,
-                                               double* __restrict__ partial_all) {
-    // work item = (replica, segment class, particle chunk, wavefront of
-    // segments); the class is uniform over the block
-    const int32_t item = blockIdx.x;
-    if (item >= kb_items) return;
-    const Rep& R = reps[find_rep<0>(reps, nrep, item)];
-    int32_t li = item - R.kb_off;
-    int c = 0;
-    while (c < 2 && li >= R.cnsegw[c] * R.nchunks) {
-        li -= R.cnsegw[c] * R.nchunks;
-        ++c;
-    }
-    const int32_t nw = R.cnsegw[c];
-    const int32_t segw = li % nw, ch = li / nw;
-    const KSeg* segs = segs_all + R.seg_off + R.cseg_off[c];
-    if (c == 0)
-        kbuild_wave<8>(R, n_all, segs, R.cnseg[0], segw, ch, tab_all, mom, partial_all);
-    else if (c == 1)
-        kbuild_wave<4>(R, n_all, segs, R.cnseg[1], segw, ch, tab_all, mom, partial_all);
-    else
-        kbuild_wave<2>(R, n_all, segs, R.cnseg[2], segw, ch, tab_all, mom, partial_all);
-}
-
 // Sum the S(k) partials of both k vectors of a packed entry over the
 // particle chunks (fixed order) and store (U+ + U-, U+ - U-, V+ + V-,
 // V+ - V-), U = e Re S, V = e Im S.
@@ -1228,7 +1227,8 @@ __global__ void k_kpack(int32_t npk_all, const Rep* __restrict__ reps, int32_t n
     pack[j] = make_double4(up + um, up - um, vp + vm, vp - vm);
 }
 
-// One particle of the k-space scatter (k_kscatter) in replica R. Per row
+// Rows [r0, r1) of the k-space scatter (k_kscatter) for particle i of
+// replica R, into out[0..5] = (B0, B1, B2, F0, F1, F2) before constants. Per row
 // (ny, nz) and |nx| = a the pair of k vectors (+-a, ny, nz) is folded into
 // four products with the particle's phase: with G1 = SU wc + SV ws,
 // G2 = DV wc - DU ws, G3 = DU wc + DV ws, G4 = SV wc - SU ws (S/D: sum and
@@ -1239,16 +1239,13 @@ __global__ void k_kpack(int32_t npk_all, const Rep* __restrict__ reps, int32_t n
 //   sum of e Im(S conj(phase)) (force), difference  = xs G3 - xc G4, xs G1 - xc G2
 // The ny and nz components share the row's sums; constants are applied once.
 template <bool FORCE>
-__device__ __forceinline__ void kscatter_one(int64_t n, int64_t i, int32_t rid, const Rep& R,
-                                             const KRow* __restrict__ rows_all, const double4* __restrict__ pack_all,
-                                             const double* __restrict__ tab_all, const double* __restrict__ mom,
-                                             const double* __restrict__ msum_all,
-                                             const double* __restrict__ add_force, double* __restrict__ force,
-                                             double* __restrict__ field) {
-    const int32_t nc = R.nc, nrows = R.nrows;
+__device__ __forceinline__ void kscatter_rows(int64_t n, int64_t i, const Rep& R, int32_t r0, int32_t r1,
+                                              const KRow* __restrict__ rows_all, const double4* __restrict__ pack_all,
+                                              const double* __restrict__ tab_all, const double* __restrict__ mom,
+                                              double out[6]) {
+    const int32_t nc = R.nc;
     const KRow* rows = rows_all + R.row_off;
     const double4* pack = pack_all + R.pk_off;
-    const double* msum = msum_all + 3 * rid;
     const int64_t stride = 3 * (int64_t)(nc + 1) * 2;
     const double* t = tab_all + R.tab_off + (i - R.off) * stride;
     const double* ty = t + (nc + 1) * 2;
@@ -1258,7 +1255,7 @@ __device__ __forceinline__ void kscatter_one(int64_t n, int64_t i, int32_t rid, 
     double F0 = 0, F1 = 0, F2 = 0, B0 = 0, B1 = 0, B2 = 0;
     double yc = 1, ys = 0, zc = 1, zs = 0;
     int cur_nz = -1;
-    for (int r = 0; r < nrows; ++r) {
+    for (int r = r0; r < r1; ++r) {
         const KRow row = rows[r];
         if (row.nz != cur_nz) {
             cur_nz = row.nz;
@@ -1307,42 +1304,67 @@ __device__ __forceinline__ void kscatter_one(int64_t n, int64_t i, int32_t rid, 
             F2 += row.nz * rs;
         }
     }
-    const double cb = R.c_flux, cf = R.c_force, c_self = R.c_self, c_surf = R.c_surf;
-    field[i] += -cb * B0 + c_self * mx - c_surf * msum[0];
-    field[n + i] += -cb * B1 + c_self * my - c_surf * msum[1];
-    field[2 * n + i] += -cb * B2 + c_self * mz - c_surf * msum[2];
-    if (FORCE) {
-        force[i] += cf * F0 + (add_force ? add_force[i] : 0.0);
-        force[n + i] += cf * F1 + (add_force ? add_force[n + i] : 0.0);
-        force[2 * n + i] += cf * F2 + (add_force ? add_force[2 * n + i] : 0.0);
-    }
+    out[0] = B0;
+    out[1] = B1;
+    out[2] = B2;
+    out[3] = F0;
+    out[4] = F1;
+    out[5] = F2;
 }
 
 // Second pass of CompDipInteractEwaldF / CompDipFieldEwaldF
 // (src/interaction.cpp:485-504, 644-662) plus the self and vacuum-surface
-// terms (510-513): one lane per particle, all k in row order; the particle's
-// phases advance by the recurrences in y (row to row) and x (within a row).
-// A wavefront whose particles share one replica (all but those straddling a
-// replica boundary) reads the k tables through scalar loads.
+// terms (510-513). A block holds 64 particles (one per lane) in four
+// wavefronts; wavefront w sums the rows [ksplit[w], ksplit[w+1]) (quarters of
+// the table by entries), so that 1e5 particles make ~6 k waves rather than
+// 1.6 k; the four partial sums are added in a fixed order in LDS. Phases
+// advance by the recurrences in y (row to row) and x (within a row). A
+// wavefront whose particles share one replica (all but those straddling a
+// replica boundary) reads the table through scalar loads.
 template <bool FORCE>
-__global__ __launch_bounds__(256) void k_kscatter(int64_t n, const Rep* __restrict__ reps,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_kscatter(int64_t n, const Rep* __restrict__ reps,
                                                   const int32_t* __restrict__ rep_of, const KRow* __restrict__ rows_all,
                                                   const double4* __restrict__ pack_all,
                                                   const double* __restrict__ tab_all, const double* __restrict__ mom,
                                                   const double* __restrict__ msum_all,
                                                   const double* __restrict__ add_force, double* __restrict__ force,
                                                   double* __restrict__ field) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    __shared__ double part[4][6][WAVE];
+    const int wv = threadIdx.x / WAVE, lane = lane_id();
+    const int64_t i = blockIdx.x * (int64_t)WAVE + lane;
     const int64_t ic = i < n ? i : n - 1;
     const int32_t rid = rep_of[ic];
     const int32_t rid0 = __builtin_amdgcn_readfirstlane(rid);
     const bool uniform = __ballot(rid != rid0) == 0;
-    if (i >= n) return;
-    if (uniform)
-        kscatter_one<FORCE>(n, i, rid0, reps[rid0], rows_all, pack_all, tab_all, mom, msum_all, add_force, force,
-                            field);
-    else
-        kscatter_one<FORCE>(n, i, rid, reps[rid], rows_all, pack_all, tab_all, mom, msum_all, add_force, force, field);
+    double o[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        if (uniform) {
+            const Rep& R = reps[rid0];
+            const int32_t r0 = R.ksplit[wv], r1 = wv < 3 ? R.ksplit[wv + 1] : R.nrows;
+            kscatter_rows<FORCE>(n, i, R, r0, r1, rows_all, pack_all, tab_all, mom, o);
+        } else {
+            const Rep& R = reps[rid];
+            const int32_t r0 = R.ksplit[wv], r1 = wv < 3 ? R.ksplit[wv + 1] : R.nrows;
+            kscatter_rows<FORCE>(n, i, R, r0, r1, rows_all, pack_all, tab_all, mom, o);
+        }
+    }
+    for (int d = 0; d < 6; ++d) part[wv][d][lane] = o[d];
+    __syncthreads();
+    if (wv != 0 || i >= n) return;
+    double a[6];
+    for (int d = 0; d < 6; ++d) a[d] = ((part[0][d][lane] + part[1][d][lane]) + part[2][d][lane]) + part[3][d][lane];
+    const Rep& R = reps[rid];
+    const double* msum = msum_all + 3 * rid;
+    const double mx = mom[i], my = mom[n + i], mz = mom[2 * n + i];
+    const double cb = R.c_flux, cf = R.c_force, c_self = R.c_self, c_surf = R.c_surf;
+    field[i] += -cb * a[0] + c_self * mx - c_surf * msum[0];
+    field[n + i] += -cb * a[1] + c_self * my - c_surf * msum[1];
+    field[2 * n + i] += -cb * a[2] + c_self * mz - c_surf * msum[2];
+    if (FORCE) {
+        force[i] += cf * a[3] + (add_force ? add_force[i] : 0.0);
+        force[n + i] += cf * a[4] + (add_force ? add_force[n + i] : 0.0);
+        force[2 * n + i] += cf * a[5] + (add_force ? add_force[2 * n + i] : 0.0);
+    }
 }
 
 // ------------------------------------------------------------------ integrator
@@ -1753,7 +1775,8 @@ hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const
 hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t kb_items, const KSeg* segs,
                          const double* tab, const double* mom, double* partial) {
     if (kb_items <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_kbuild, dim3(kb_items), dim3(WAVE), 0, s, n, reps, nrep, kb_items, segs, tab, mom, partial);
+    hipLaunchKernelGGL(k_kbuild<KSEG>, dim3(kb_items), dim3(WAVE), 0, s, n, reps, nrep, kb_items, segs, tab, mom,
+                       partial);
     LAUNCH_CHECK();
 }
 
@@ -1768,12 +1791,14 @@ hipError_t launch_kpack(hipStream_t s, int32_t npk_all, const Rep* reps, int32_t
 hipError_t launch_kscatter(hipStream_t s, bool force, int64_t n, const Rep* reps, const int32_t* rep_of,
                            const KRow* rows, const double4* pack, const double* tab, const double* mom,
                            const double* msum, const double* add_force, double* force_out, double* field) {
+    // one block per 64 particles (four wavefronts over quarters of the rows)
+    const int blocks = (int)((n + WAVE - 1) / WAVE);
     if (force)
-        hipLaunchKernelGGL(k_kscatter<true>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, rows, pack, tab,
-                           mom, msum, add_force, force_out, field);
+        hipLaunchKernelGGL(k_kscatter<true>, dim3(blocks), dim3(256), 0, s, n, reps, rep_of, rows, pack, tab, mom, msum,
+                           add_force, force_out, field);
     else
-        hipLaunchKernelGGL(k_kscatter<false>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, rows, pack,
-                           tab, mom, msum, add_force, force_out, field);
+        hipLaunchKernelGGL(k_kscatter<false>, dim3(blocks), dim3(256), 0, s, n, reps, rep_of, rows, pack, tab, mom,
+                           msum, add_force, force_out, field);
     LAUNCH_CHECK();
 }
 

@@ -112,6 +112,8 @@ class Server:
         self.active: list[Request] = []
         self.done_steps: dict[int, int] = {}
         self.step_time = None  # s per step of the active set, per particle
+        self.min_steps = 4     # steps per chunk at least (a chunk has a fixed cost of a few syncs)
+        self.times = {"admit": 0.0, "run": 0.0, "retire": 0.0, "idle": 0.0}
         self.philox_on = False
 
     def close(self):
@@ -206,36 +208,44 @@ class Server:
                 used += f
                 batch.append(pending.pop(0))
             if batch:
+                a0 = clock()
                 self.admit(batch, now)
+                self.times["admit"] += clock() - a0
             if not self.active:
                 # idle until the next arrival (or until another lane frees memory)
                 wait = pending[0].arrival - (clock() - t0)
                 time.sleep(wait if wait > 0 else 0.002)
+                self.times["idle"] += max(wait, 0.002)
                 continue
             # a chunk: no request beyond its step count, about chunk_seconds long
             remaining = min(q.steps - self.done_steps[q.rid] for q in self.active)
             n_act = sum(q.n for q in self.active)
             if self.step_time:
-                s = max(1, int(self.chunk_seconds / (self.step_time * n_act)))
+                s = max(self.min_steps, int(self.chunk_seconds / (self.step_time * n_act)))
             else:
-                s = 2
+                s = self.min_steps
             s = min(s, remaining)
             c0 = clock()
             # the clock argument only matters for a single replica (several keep theirs on the device)
             self.ctx.run(s, self.done_steps[self.active[0].rid] * self.dt, self.dt)
             self.ctx.synchronize()
             el = clock() - c0
+            self.times["run"] += el
             self.step_time = el / (s * n_act)
             total_ps += s * n_act
             chunks += 1
             for q in self.active:
                 self.done_steps[q.rid] += s
+            r0 = clock()
             self.retire(clock() - t0)
+            self.times["retire"] += clock() - r0
             if log:
                 log(f"t={clock() - t0:8.3f}s chunk {chunks}: {s} steps x {len(self.active)} left, "
                     f"{n_act} particles, {el / s * 1e3:.2f} ms/step, queue {len(pending)}")
         wall = clock() - t0
-        return summary(requests, wall, total_ps, chunks)
+        out = summary(requests, wall, total_ps, chunks)
+        out["seconds"] = dict(self.times)
+        return out
 
 
 def summary(requests: list[Request], wall: float, particle_steps: int, chunks: int) -> dict:
@@ -310,7 +320,8 @@ class LaneServer:
             raise errs[0]
         wall = clock() - t0
         res = summary(requests, wall, sum(o["particle_steps"] for o in out), sum(o["chunks"] for o in out))
-        res["lanes"] = [{"requests": len(p), "chunks": o["chunks"]} for p, o in zip(parts, out)]
+        res["lanes"] = [{"requests": len(p), "chunks": o["chunks"], "seconds": o.get("seconds")}
+                        for p, o in zip(parts, out)]
         return res
 
 
