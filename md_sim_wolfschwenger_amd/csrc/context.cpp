@@ -708,7 +708,7 @@ mds_status prepare_geometry_once(mds_ctx* c) {
                               c->key_fine.p));
         HIPCHK(launch_cell_sort(s, n, c->key_main.p, c->cells_main, c->cnt_main.p, c->start_main.p, c->end_main.p,
                                 c->cur_main.p, c->sorted_main.p, c->scan_tmp.p));
-        HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->r_mag.p, c->rec_pos.p, c->rec_pos32.p,
+        HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->rec_pos.p, c->rec_pos32.p,
                                     c->inv_perm.p));
         HIPCHK(launch_build_work(s, c->cells_main, c->start_main.p, c->end_main.p, c->work.p, c->nwork.p,
                                  c->work_cap, c->err_flag.p));
@@ -728,6 +728,7 @@ mds_status prepare_geometry_once(mds_ctx* c) {
                               c->pos0.p, c->ev_head.p, c->ev_tail.p, c->ev_next.p, c->ev_key.p, c->ev_pos.p,
                               2 * c->cand_cap, c->n_events.p, c->max_disp.p, dl, c->moved_list.p, c->n_moved.p,
                               c->err_flag.p));
+        HIPCHK(launch_mark_moved(s, c->n_moved.p, c->moved_list.p, c->inv_perm.p, c->rec_pos.p));
     }
     {
         mds_ctx::Span sp(c, KC_SINCOS);

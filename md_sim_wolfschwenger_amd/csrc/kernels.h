@@ -66,7 +66,7 @@ struct Geometry {
     const int32_t* cell_start;   // [ncell^3]
     const int32_t* cell_end;     // [ncell^3]
     const int32_t* sorted_idx;   // [n] original index of sorted slot
-    const double4* rec_pos;      // [n] (x, y, z, rMag) in sorted order, step-start positions
+    const double4* rec_pos;      // [n] (x, y, z, tag) in sorted order, step-start positions
     const int32_t* work;         // [nwork] packed (cell << 8) | chunk
     const int32_t* nwork;        // device scalar
     // overlap-correction versions
@@ -114,8 +114,12 @@ hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* p
 int32_t scan_tiles(int64_t m);
 hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
                             int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp);
+// rec_pos[k].w carries a tag: the original index in the low 32 bits
 hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_idx, const double* pos0,
-                                const double* r_mag, double4* rec_pos, float4* rec_pos32, int32_t* inv_perm);
+                                double4* rec_pos, float4* rec_pos32, int32_t* inv_perm);
+// after the overlap correction: flag the moved particles' tags
+hipError_t launch_mark_moved(hipStream_t s, const int32_t* n_moved, const int32_t* moved_list,
+                             const int32_t* inv_perm, double4* rec_pos);
 hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
                              int32_t* work, int32_t* nwork, int32_t cap, int32_t* err);
 // contact search + sequential overlap resolution

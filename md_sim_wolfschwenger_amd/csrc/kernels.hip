@@ -205,16 +205,30 @@ __global__ void k_place(int64_t n, const int32_t* __restrict__ key, int32_t* __r
     if (i < n) sorted_idx[atomicAdd(&cursor[key[i]], 1)] = (int32_t)i;
 }
 
+// Cell-sorted records. The fourth component of the FP64 record is a tag,
+// not a number: the particle's original index in the low 32 bits of its bit
+// pattern (the overlap correction later sets bit 32 of the moved ones), so
+// the pair kernels need no further gathers for either.
 __global__ void k_records(int64_t n, const int32_t* __restrict__ sorted_idx, const double* __restrict__ pos0,
-                          const double* __restrict__ r_mag, double4* __restrict__ rec, float4* __restrict__ rec32,
-                          int32_t* __restrict__ inv_perm) {
+                          double4* __restrict__ rec, float4* __restrict__ rec32, int32_t* __restrict__ inv_perm) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int32_t i = sorted_idx[k];
-    const double4 r = make_double4(pos0[i], pos0[n + i], pos0[2 * n + i], r_mag[i]);
-    rec[k] = r;
-    rec32[k] = make_float4((float)r.x, (float)r.y, (float)r.z, (float)r.w);
+    const double x = pos0[i], y = pos0[n + i], z = pos0[2 * n + i];
+    rec[k] = make_double4(x, y, z, __longlong_as_double((long long)(uint32_t)i));
+    rec32[k] = make_float4((float)x, (float)y, (float)z, 0.0f);
     inv_perm[i] = (int32_t)k;
+}
+
+// Bit 32 of the tag of every particle moved by the overlap correction.
+__global__ void k_mark_moved(const int32_t* __restrict__ n_moved, const int32_t* __restrict__ moved_list,
+                             const int32_t* __restrict__ inv_perm, double4* __restrict__ rec) {
+    const int32_t m = *n_moved;
+    for (int32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < m; t += gridDim.x * blockDim.x) {
+        const int32_t k = inv_perm[moved_list[t]];
+        const long long b = __double_as_longlong(rec[k].w) | (1ll << 32);
+        rec[k].w = __longlong_as_double(b);
+    }
 }
 
 // Work items of the pair kernel: one per 64 particles of a cell.
@@ -513,15 +527,10 @@ template <bool MIXED, bool COATING>
 struct IWave {
     int32_t q[128];
     double acc[6][WAVE];   // force, field of the wave's particles
-    // positions; without the coating potential .w holds the original index
-    // (bits of an int64) instead of r_mag, which only the short-range terms use
+    // positions; .w is the record's tag (original index, moved flag)
     double4 ipos[WAVE];
     typename std::conditional<MIXED && !COATING, float4, double4>::type imom[WAVE];
-    int32_t iorig[COATING ? WAVE : 1];
-    __device__ int32_t orig(int l) const {
-        if constexpr (COATING) return iorig[l];
-        else return (int32_t)__double_as_longlong(ipos[l].w);
-    }
+    __device__ int32_t orig(int l) const { return (int32_t)__double_as_longlong(ipos[l].w); }
 };
 
 // Positions of lo and hi as the reference's sequential scan sees them when it
@@ -598,7 +607,8 @@ __device__ __forceinline__ double erfcx64(double x) {
     return p;
 }
 __device__ __forceinline__ float erfcx32(float x) {
-    const float t = 1.0f - 2.0f * kErfcxK32 * __frcp_rn(x + kErfcxK32);
+    // hardware reciprocal (1 ulp): far below the 8.7e-7 error of the fit
+    const float t = 1.0f - 2.0f * kErfcxK32 * __builtin_amdgcn_rcpf(x + kErfcxK32);
     float p = kErfcx32[kErfcxDeg32];
 #pragma unroll
     for (int k = kErfcxDeg32 - 1; k >= 0; --k) p = fmaf(p, t, kErfcx32[k]);
@@ -721,11 +731,11 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
     if (!active) return;
     const int il = entry & 63;
     const int32_t j = entry >> 6;
-    const int32_t oi = W.orig(il);
-    const int32_t oj = g.sorted_idx[j];
-    if (moved_any && (g.ev_head[oi] >= 0 || g.ev_head[oj] >= 0)) return;
     const double4 pi = W.ipos[il];
     const double4 pj = g.rec_pos[j];
+    const long long ti = __double_as_longlong(pi.w), tj = __double_as_longlong(pj.w);
+    if (moved_any && ((ti | tj) >> 32)) return;  // an endpoint moved
+    const int32_t oi = (int32_t)ti, oj = (int32_t)tj;
     const bool i_lo = oi < oj;
     if constexpr (MIXED && !COATING) {
         // separation in FP64 (positions are absolute), everything after it in
@@ -783,7 +793,8 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
                 press[2] += rr * uy * F[2];
             }
             if (COATING && r < pp.rc_sr) {
-                const double sf = short_range(pp, r, i_lo ? pi.w : pj.w, i_lo ? pj.w : pi.w) / pp.fscale;
+                const double rmi = pp.r_mag[oi], rmj = pp.r_mag[oj];
+                const double sf = short_range(pp, r, i_lo ? rmi : rmj, i_lo ? rmj : rmi) / pp.fscale;
                 F[0] += sf * ux;
                 F[1] += sf * uy;
                 F[2] += sf * uz;
@@ -831,18 +842,11 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     double4 pi = make_double4(0, 0, 0, 0);
     if (vi) {
         pi = g.rec_pos[k];
-        const int32_t o = g.sorted_idx[k];
-        if constexpr (COATING) {
-            W.ipos[lane] = pi;
+        W.ipos[lane] = pi;
+        if constexpr (MIXED && !COATING)
+            W.imom[lane] = g.rec_mom32[k];
+        else
             W.imom[lane] = rec_mom[k];
-            W.iorig[lane] = o;
-        } else {
-            W.ipos[lane] = make_double4(pi.x, pi.y, pi.z, __longlong_as_double((long long)o));
-            if constexpr (MIXED)
-                W.imom[lane] = g.rec_mom32[k];
-            else
-                W.imom[lane] = rec_mom[k];
-        }
     }
     for (int d = 0; d < 6; ++d) W.acc[d][lane] = 0.0;
     const bool moved_any = *g.n_events > 0;
@@ -1020,7 +1024,8 @@ __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* _
                 const double r0 = sqrt(vx * vx + vy * vy + vz * vz);
                 if (!(r0 < pp.rc)) continue;
                 ++npairs;
-                const double r = event ? pp0.w + pj.w + 1e-10 : r0;
+                const double rmp = pp.r_mag[p], rmj = pp.r_mag[oj];
+                const double r = event ? rmp + rmj + 1e-10 : r0;
                 const double ir0 = 1.0 / r0;
                 const double ir = event ? 1.0 / r : ir0;
                 // u from lo to hi, taken from the pre-correction separation
@@ -1038,7 +1043,7 @@ __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* _
                 }
                 if (FORCE) {
                     double sf = 0;
-                    if (COATING && r0 < pp.rc_sr) sf = short_range(pp, r, p_lo ? pp0.w : pj.w, p_lo ? pj.w : pp0.w);
+                    if (COATING && r0 < pp.rc_sr) sf = short_range(pp, r, p_lo ? rmp : rmj, p_lo ? rmj : rmp);
                     const double u3[3] = {ux, uy, uz};
                     for (int d = 0; d < 3; ++d) {
                         const double Fd = f * Fh[d] + sf * u3[d];
@@ -1201,28 +1206,43 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
 }
 
 // Sum the S(k) partials of both k vectors of a packed entry over the
-// particle chunks (fixed order) and store (U+ + U-, U+ - U-, V+ + V-,
-// V+ - V-), U = e Re S, V = e Im S.
-__global__ void k_kpack(int32_t npk_all, const Rep* __restrict__ reps, int32_t nrep, const KPack* __restrict__ desc,
-                        const double* __restrict__ partial_all, double4* __restrict__ pack) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= npk_all) return;
-    const Rep& R = reps[find_rep<2>(reps, nrep, j)];
-    const KPack d = desc[j];
-    const int nk = R.nk;
-    const double* partial = partial_all + R.part_off;
+// particle chunks and store (U+ + U-, U+ - U-, V+ + V-, V+ - V-),
+// U = e Re S, V = e Im S. A block takes 64 entries; its four wavefronts sum
+// a quarter of the chunks each and the quarters are added in a fixed order.
+__global__ __launch_bounds__(256) void k_kpack(int32_t npk_all, const Rep* __restrict__ reps, int32_t nrep,
+                                               const KPack* __restrict__ desc, const double* __restrict__ partial_all,
+                                               double4* __restrict__ pack) {
+    __shared__ double4 part[4][WAVE];
+    const int wv = threadIdx.x / WAVE, lane = lane_id();
+    const int j = blockIdx.x * WAVE + lane;
     double cp = 0, sp = 0, cm = 0, sm = 0;
-    for (int ch = 0; ch < R.nchunks; ++ch) {
-        const double* q = partial + (int64_t)ch * nk * 2;
-        if (d.kp >= 0) {
-            cp += q[2 * d.kp];
-            sp += q[2 * d.kp + 1];
-        }
-        if (d.km >= 0) {
-            cm += q[2 * d.km];
-            sm += q[2 * d.km + 1];
+    KPack d{-1, -1, 0.0};
+    if (j < npk_all) {
+        const Rep& R = reps[find_rep<2>(reps, nrep, j)];
+        d = desc[j];
+        const int nk = R.nk;
+        const double* partial = partial_all + R.part_off;
+        const int c0 = (R.nchunks * wv) / 4, c1 = (R.nchunks * (wv + 1)) / 4;
+        for (int ch = c0; ch < c1; ++ch) {
+            const double* q = partial + (int64_t)ch * nk * 2;
+            if (d.kp >= 0) {
+                cp += q[2 * d.kp];
+                sp += q[2 * d.kp + 1];
+            }
+            if (d.km >= 0) {
+                cm += q[2 * d.km];
+                sm += q[2 * d.km + 1];
+            }
         }
     }
+    part[wv][lane] = make_double4(cp, sp, cm, sm);
+    __syncthreads();
+    if (wv != 0 || j >= npk_all) return;
+    const double4 a = part[0][lane], b = part[1][lane], c = part[2][lane], e = part[3][lane];
+    cp = ((a.x + b.x) + c.x) + e.x;
+    sp = ((a.y + b.y) + c.y) + e.y;
+    cm = ((a.z + b.z) + c.z) + e.z;
+    sm = ((a.w + b.w) + c.w) + e.w;
     const double up = d.e * cp, vp = d.e * sp, um = d.e * cm, vm = d.e * sm;
     pack[j] = make_double4(up + um, up - um, vp + vm, vp - vm);
 }
@@ -1654,9 +1674,15 @@ hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_
 }
 
 hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_idx, const double* pos0,
-                                const double* r_mag, double4* rec_pos, float4* rec_pos32, int32_t* inv_perm) {
-    hipLaunchKernelGGL(k_records, dim3(grid_for(n, 256)), dim3(256), 0, s, n, sorted_idx, pos0, r_mag, rec_pos,
-                       rec_pos32, inv_perm);
+                                double4* rec_pos, float4* rec_pos32, int32_t* inv_perm) {
+    hipLaunchKernelGGL(k_records, dim3(grid_for(n, 256)), dim3(256), 0, s, n, sorted_idx, pos0, rec_pos, rec_pos32,
+                       inv_perm);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_mark_moved(hipStream_t s, const int32_t* n_moved, const int32_t* moved_list,
+                             const int32_t* inv_perm, double4* rec_pos) {
+    hipLaunchKernelGGL(k_mark_moved, dim3(64), dim3(256), 0, s, n_moved, moved_list, inv_perm, rec_pos);
     LAUNCH_CHECK();
 }
 
@@ -1783,8 +1809,8 @@ hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep
 hipError_t launch_kpack(hipStream_t s, int32_t npk_all, const Rep* reps, int32_t nrep, const KPack* desc,
                         const double* partial, double4* pack) {
     if (npk_all <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_kpack, dim3((npk_all + 255) / 256), dim3(256), 0, s, npk_all, reps, nrep, desc, partial,
-                       pack);
+    hipLaunchKernelGGL(k_kpack, dim3((npk_all + WAVE - 1) / WAVE), dim3(256), 0, s, npk_all, reps, nrep, desc,
+                       partial, pack);
     LAUNCH_CHECK();
 }
 
