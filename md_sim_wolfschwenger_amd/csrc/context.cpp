@@ -215,6 +215,8 @@ struct mds_ctx {
     // per-step scratch
     DevBuf<double> pos0, pos_cur, force, field, mom;
     DevBuf<double4> rec_pos, rec_mom;
+    DevBuf<uint64_t> pair_acc;  // [12n] cell-sorted fixed-point pair sums
+    DevBuf<int32_t> order_tmp;  // [n] cell-order scratch
     DevBuf<float4> rec_pos32, rec_mom32;
     DevBuf<int32_t> rep_of;
     // replica table and per-replica device values
@@ -404,16 +406,25 @@ void plan_grid(mds_ctx* c, const RepMeta& m, Rep& R) {
     R.fcell = m.L / fn;
     // stencil of neighbour-cell offsets whose nearest points lie within the
     // candidate radius (corner cells of the cube of +-reach are dropped)
+    // The cell itself comes first, then the offsets positive in (z, y, x)
+    // order (the pair kernel's half stencil: each pair of cells once), then
+    // their negatives in the same order.
     const int H = nc == 1 ? 0 : reach;
     const double lim = m.rc * (1 + 1e-9) + R.margin;
     R.sten_off = (int32_t)c->h_sten.size();
+    std::vector<int4> pos_half;
     for (int dz = -H; dz <= H; ++dz)
         for (int dy = -H; dy <= H; ++dy)
             for (int dx = -H; dx <= H; ++dx) {
                 auto gap = [&](int o) { return std::max(std::abs(o) - 1, 0) * R.cell; };
                 const double d2 = gap(dx) * gap(dx) + gap(dy) * gap(dy) + gap(dz) * gap(dz);
-                if (d2 < lim * lim) c->h_sten.push_back(make_int4(dx, dy, dz, 0));
+                const bool positive = dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0)));
+                if (d2 < lim * lim && positive) pos_half.push_back(make_int4(dx, dy, dz, 0));
             }
+    c->h_sten.push_back(make_int4(0, 0, 0, 0));
+    for (const int4& o : pos_half) c->h_sten.push_back(o);
+    for (const int4& o : pos_half) c->h_sten.push_back(make_int4(-o.x, -o.y, -o.z, 0));
+    R.nhalf = 1 + (int32_t)pos_half.size();
     R.nsten = (int32_t)c->h_sten.size() - R.sten_off;
 }
 
@@ -512,6 +523,17 @@ void plan_ewald(mds_ctx* c, const RepMeta& m, Rep& R) {
     R.al_s = (float)(m.alpha * R.ell);
     R.al2_s = R.al_s * R.al_s;
     R.al4_s = R.al2_s * R.al2_s;
+    // Fixed-point units of the pair sums: s1 = 2^(e - 17) below the field
+    // (M / D^3) and force (M^2 / D^4) scales of the largest moment M at
+    // distance D = r_max, in the sums' units; resolution s1 2^-40, range
+    // 2^50 s1 (about 1e10 times the scale).
+    const double M = m.mref * R.inv_mref, D = m.rmax * R.inv_ell;
+    const double scale[2] = {M / (D * D * D), M * M / (D * D * D * D)};
+    for (int q = 0; q < 2; ++q) {
+        const int e = (std::isfinite(scale[q]) && scale[q] > 0) ? std::ilogb(scale[q]) - 17 : 0;
+        R.fx_s[q] = std::ldexp(1.0, e);
+        R.fx_is[q] = std::ldexp(1.0, -e);
+    }
 }
 
 // Plan grids, stencils and k space of every replica, size the scratch
@@ -560,9 +582,11 @@ mds_status plan_all(mds_ctx* c) {
         for (int64_t b = m.off; b < m.off + m.n; b += RED_CHUNK_PARTICLES) c->h_chunks.push_back(make_int2(r, (int)b));
         R.nred = (int32_t)c->h_chunks.size() - R.red_off;
     }
-    // work items pack (cell << 8) | chunk into an int32
+    // work items pack (cell << 8) | chunk into an int32; pair-queue entries
+    // (j << 6) | lane, with bit 31 free
     if (cells >= (1 << 23) || fcells >= (1 << 30) || work_cap >= (1 << 30) || kb >= (1 << 30))
         return fail(c, MDS_ECAPACITY, "too many cells or k-space work items in one context");
+    if (c->n >= (int64_t)1 << 25) return fail(c, MDS_ECAPACITY, "more than 2^25 particles in one context");
     c->cells_main = (int32_t)cells;
     c->cells_fine = (int32_t)fcells;
     c->work_cap = (int32_t)work_cap;
@@ -584,7 +608,7 @@ mds_status plan_all(mds_ctx* c) {
     HIPCHK(c->ev_key.ensure(2 * (size_t)c->cand_cap));
     HIPCHK(c->ev_pos.ensure(6 * (size_t)c->cand_cap));
     HIPCHK(c->moved_list.ensure(2 * (size_t)c->cand_cap));
-    HIPCHK(c->press_partial.ensure(3 * ((size_t)work_cap + c->nrep)));
+    HIPCHK(c->press_partial.ensure(3 * ((size_t)work_cap * PAIR_SLICES + c->nrep)));
     HIPCHK(c->stencil.ensure(std::max<size_t>(1, c->h_sten.size())));
     HIPCHK(c->rows.ensure(std::max<size_t>(1, c->h_rows.size())));
     HIPCHK(c->segs.ensure(std::max<size_t>(1, c->h_segs.size())));
@@ -637,6 +661,8 @@ mds_status size_scratch(mds_ctx* c) {
     HIPCHK(c->rec_pos32.ensure(n));
     HIPCHK(c->rec_mom.ensure(n));
     HIPCHK(c->rec_mom32.ensure(n));
+    HIPCHK(c->pair_acc.ensure(12 * (size_t)n));
+    HIPCHK(c->order_tmp.ensure(n));
     for (auto* b : {&c->key_main, &c->key_fine, &c->sorted_main, &c->sorted_fine, &c->inv_perm, &c->cand_count,
                     &c->ev_head, &c->ev_tail})
         HIPCHK(b->ensure(n));
@@ -708,6 +734,8 @@ mds_status prepare_geometry_once(mds_ctx* c) {
                               c->key_fine.p));
         HIPCHK(launch_cell_sort(s, n, c->key_main.p, c->cells_main, c->cnt_main.p, c->start_main.p, c->end_main.p,
                                 c->cur_main.p, c->sorted_main.p, c->scan_tmp.p));
+        HIPCHK(launch_cell_order(s, c->cells_main, c->start_main.p, c->end_main.p, c->sorted_main.p,
+                                 c->order_tmp.p));
         HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->rec_pos.p, c->rec_pos32.p,
                                     c->inv_perm.p));
         HIPCHK(launch_build_work(s, c->cells_main, c->start_main.p, c->end_main.p, c->work.p, c->nwork.p,
@@ -767,13 +795,15 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
     hipStream_t s = c->stream;
     Geometry g = geometry(c);
     PairParams pp = pair_params(c);
-    PairOut po{force_pass ? c->force.p : nullptr, c->field.p, c->press_partial.p};
+    PairOut po{force_pass ? c->force.p : nullptr, c->field.p, c->press_partial.p,
+               (unsigned long long*)c->pair_acc.p, c->err_flag.p};
+    const size_t moved_press = 3 * (size_t)c->work_cap * PAIR_SLICES;
     {
         mds_ctx::Span sp(c, force_pass ? KC_PAIRS_FORCE : KC_PAIRS_FIELD);
         int64_t* cnt = c->count_work ? (int64_t*)c->pair_counter.p : nullptr;
-        if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + 3 * (size_t)c->work_cap, 3 * c->nrep, 0.0));
-        HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p, cnt));
-        HIPCHK(launch_pairs_moved(s, force_pass, g, c->rec_mom.p, pp, po, c->stencil.p, c->work_cap, cnt));
+        if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + moved_press, 3 * c->nrep, 0.0));
+        HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p,
+                            c->work_cap * PAIR_SLICES, cnt));
     }
     if (c->nk_all > 0) {
         {
@@ -841,6 +871,7 @@ mds_status read_flags(mds_ctx* c) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->last_events = nev;
     if (f & 32) return fail(c, MDS_EOVERLAP, "overlap correction moved a particle beyond the contact search margin");
+    if (f & 64) return fail(c, MDS_ENONFINITE, "a pair sum is not finite or beyond the fixed-point range");
     if (f) return fail(c, MDS_ECAPACITY, "device capacity exceeded (flag " + std::to_string(f) + ")");
     return MDS_OK;
 }
@@ -1471,15 +1502,17 @@ mds_status mds_comp_interactions(mds_ctx* c, double* force, double* field, doubl
     if (pos_after && (st = download(c, pos_after, c->pos_cur.p, n3)) != MDS_OK) return st;
     if (press) {
         // pressure sums over all replicas (one replica: the reference's)
-        std::vector<double> pp(3 * (size_t)c->work_cap);
+        const size_t slots = (size_t)c->work_cap * PAIR_SLICES;
+        std::vector<double> pp(3 * slots);
         int32_t nw = 0;
         HIPCHK(hipMemcpy(&nw, c->nwork.p, sizeof(nw), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(pp.data(), c->press_partial.p, 3 * (size_t)nw * sizeof(double), hipMemcpyDeviceToHost));
+        const size_t used = (size_t)nw * PAIR_SLICES;
+        HIPCHK(hipMemcpy(pp.data(), c->press_partial.p, 3 * used * sizeof(double), hipMemcpyDeviceToHost));
         press[0] = press[1] = press[2] = 0;
-        for (int32_t w = 0; w < nw; ++w)
-            for (int d = 0; d < 3; ++d) press[d] += pp[3 * (size_t)w + d];
+        for (size_t w = 0; w < used; ++w)
+            for (int d = 0; d < 3; ++d) press[d] += pp[3 * w + d];
         std::vector<double> pm(3 * (size_t)c->nrep);
-        HIPCHK(hipMemcpy(pm.data(), c->press_partial.p + 3 * (size_t)c->work_cap, pm.size() * sizeof(double),
+        HIPCHK(hipMemcpy(pm.data(), c->press_partial.p + 3 * slots, pm.size() * sizeof(double),
                          hipMemcpyDeviceToHost));
         for (int32_t r = 0; r < c->nrep; ++r)
             for (int d = 0; d < 3; ++d) press[d] += pm[3 * (size_t)r + d];
@@ -1795,6 +1828,7 @@ int64_t mds_device_bytes(const mds_ctx* c) {
                     &c->kea, &c->pt[0], &c->pt[1], &c->new_mm, &c->new_ea, &c->ev_pos, &c->kpartial,
                     &c->tab, &c->press_partial, &c->series, &c->msum, &c->mz_out, &c->rep_t, &c->err_partial})
         add(d->n, sizeof(double));
+    add(c->pair_acc.n + c->order_tmp.n / 2, sizeof(uint64_t));
     for (auto* d : {&c->rep_of, &c->cell_rep, &c->key_main, &c->key_fine, &c->cnt_main, &c->start_main, &c->end_main,
                     &c->cur_main, &c->sorted_main, &c->inv_perm, &c->work, &c->cnt_fine, &c->start_fine, &c->end_fine,
                     &c->cur_fine, &c->sorted_fine, &c->cand_count, &c->cand_off, &c->cand_j, &c->ev_head, &c->ev_tail,

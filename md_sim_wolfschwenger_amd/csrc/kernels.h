@@ -38,9 +38,12 @@ struct Rep {
     int64_t off, n;              // particle range [off, off + n)
     double L, cell, fcell, margin;
     int32_t ncell, fncell, cell_off, fcell_off;  // grids; global cell id = cell_off + local
-    int32_t sten_off, nsten;     // slice of the stencil list
+    int32_t sten_off, nsten;     // slice of the stencil list: the cell itself, the
+    int32_t nhalf, pad3_;        // offsets positive in (z, y, x) order (nhalf with
+                                 // the cell itself), then their negatives
     double alpha, alpha2, alpha4, rc, rc_sr;
     double ell, inv_ell, inv_mref, bscale, fscale, pscale;  // units of the FP32 pair path
+    double fx_s[2], fx_is[2];    // fixed-point unit s1 of the pair sums (field, force) and 1/s1
     float al_s, al2_s, al4_s, pad_;
     int32_t nc, row_off, nrows, seg_off, nseg, k_off, nk, kchunk, nchunks, kb_off, nsegw, pad2_;
     int32_t pk_off, npk;         // slice of the packed scatter table
@@ -86,8 +89,16 @@ struct Geometry {
 struct PairOut {
     double* force;   // [3n] SoA, written (not accumulated); may be null (field only)
     double* field;   // [3n] SoA, written
-    double* press_partial;  // [nwaves*3]
+    double* press_partial;  // [3 * work items * PAIR_SLICES], then 3 per replica (moved pairs)
+    // [12n] cell-sorted pair sums as fixed-point word pairs: component c
+    // (force 0..2, field 3..5), word w at (2c + w) n + k
+    unsigned long long* acc;
+    int32_t* err;    // flag 64: a pair sum beyond the fixed-point range or not finite
 };
+
+// The pair kernel splits each work item's half stencil over this many
+// wavefronts (one block each).
+constexpr int PAIR_SLICES = 4;
 
 struct PairParams {
     double alpha, alpha2, alpha4, ir_pi, fac, rc, rc_sr;
@@ -150,14 +161,17 @@ hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, const
                              int32_t series_cap);
 
 // real space: one block per work item (max_work blocks, surplus blocks exit)
+// Real-space pair sums (force pass: force, field, pressure; else field):
+// the cell-tiled pair kernel, the pairs with an endpoint moved by the overlap
+// correction (their pressure to press_partial slot moved_press_off +
+// replica), then the fixed-point sums to out.force / out.field.
 hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
                         const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
-                        int64_t* cnt_pairs);
-// Pairs with an endpoint moved by the overlap correction (after launch_pairs);
-// pressure goes to press_partial slot press_off + replica.
-hipError_t launch_pairs_moved(hipStream_t s, bool force, const Geometry& g, const double4* rec_mom,
-                              const PairParams& pp, const PairOut& out, const int4* stencil, int32_t press_off,
-                              int64_t* cnt_pairs);
+                        int32_t moved_press_off, int64_t* cnt_pairs);
+// Within each cell of a sorted list, order the entries by value (original
+// index), so the pair sums' order does not depend on scheduling; tmp: [n].
+hipError_t launch_cell_order(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
+                             int32_t* sorted_idx, int32_t* tmp);
 
 // k space
 hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,

@@ -515,22 +515,37 @@ __global__ void k_rep_mean_z(const Rep* __restrict__ reps, int32_t nrep, const d
 // src/postprocessing.cpp:195-200) and, with the coating potential, the
 // short-range forces (src/interaction.cpp:766-832).
 //
-// One wavefront owns up to 64 particles of one cell (one per lane) and visits
-// the stencil cells. Each candidate j is tested by all lanes (a cheap distance
-// test); lanes that hit push (lane, j) into a wavefront-private LDS queue, and
-// every 64 queued pairs are evaluated densely, one pair per lane, so the
-// expensive erfc/exp path never runs with idle lanes. Results are added into
-// the owning lane's LDS accumulator. Every pair is evaluated from its lower
-// original index (the reference's i < j orientation), so both of its
-// particles see bit-identical geometry.
+// One block of PAIR_SLICES wavefronts takes up to 64 particles of one cell
+// (the i particles, one per lane; their positions and moments are shared in
+// LDS) and each wavefront one interleaved slice of the half stencil: the cell
+// itself (pairs j > i in cell-sorted order) and the neighbour offsets that
+// are positive in (z, y, x) order, so every pair within r_c is met exactly
+// once. Each candidate j is tested by all lanes (a cheap FP32 distance test);
+// lanes that hit push (lane, j) into the wavefront's LDS queue, and queued
+// pairs are evaluated densely, one pair per lane, so the expensive erfc/exp
+// path never runs with idle lanes. The queue is read with a stride of the
+// number of batches it fills, which spreads the lanes that share a j (one
+// candidate hits many lanes) over the batches and keeps LDS atomic conflicts
+// low. Each evaluation updates both particles: i in the wavefront's
+// accumulator, j in an accumulator over the current 64-particle block of the
+// neighbour cell (the window), which goes to the cell-sorted fixed-point sums
+// when the window is done; the wavefronts' i sums are added in a fixed order
+// at the end. Every pair is evaluated in the orientation of its lower
+// original index (the reference's i < j), so both particles see the
+// reference's geometry.
+constexpr int PAIR_QFLUSH = 192;                // evaluate the queue at this fill (slots, whole quads)
+constexpr int PAIR_QCAP = PAIR_QFLUSH + WAVE;   // one candidate adds at most 64
 template <bool MIXED, bool COATING>
-struct IWave {
-    int32_t q[128];
-    double acc[6][WAVE];   // force, field of the wave's particles
-    // positions; .w is the record's tag (original index, moved flag)
-    double4 ipos[WAVE];
+struct IChunk {
+    double4 ipos[WAVE];      // positions; .w is the record's tag
     typename std::conditional<MIXED && !COATING, float4, double4>::type imom[WAVE];
-    __device__ int32_t orig(int l) const { return (int32_t)__double_as_longlong(ipos[l].w); }
+};
+template <bool FORCE>
+struct IWave {
+    static constexpr int NA = FORCE ? 6 : 3;    // (force,) field components
+    int32_t q[PAIR_QCAP];
+    double acc[NA][WAVE];    // this wavefront's sums of the i particles
+    double jacc[NA][WAVE];   // sums of the current window's j particles
 };
 
 // Positions of lo and hi as the reference's sequential scan sees them when it
@@ -620,7 +635,7 @@ __device__ __forceinline__ float erfcx32(float x) {
 template <bool FORCE>
 __device__ __forceinline__ void pair_terms32(const PairParams& pp, float rs, float irs, float fx, float fy, float fz,
                                              float mix, float miy, float miz, float mjx, float mjy, float mjz,
-                                             float B[3], float F[3]) {
+                                             float B[3], float Bo[3], float F[3]) {
     const float irs2 = irs * irs;
     const float x = pp.al_s * rs;
     const float gau = __expf(-x * x);
@@ -628,12 +643,16 @@ __device__ __forceinline__ void pair_terms32(const PairParams& pp, float rs, flo
     const float a1 = gau * erfcx32(x) * (irs2 * irs) + t;
     const float a2 = 3.0f * a1 * irs2 + 2.0f * pp.al2_s * t;
     const float dj = mjx * fx + mjy * fy + mjz * fz;
+    const float di = mix * fx + miy * fy + miz * fz;
     const float g2 = a2 * dj * rs * rs;
     B[0] = g2 * fx - a1 * mjx;
     B[1] = g2 * fy - a1 * mjy;
     B[2] = g2 * fz - a1 * mjz;
+    const float h2 = a2 * di * rs * rs;
+    Bo[0] = h2 * fx - a1 * mix;
+    Bo[1] = h2 * fy - a1 * miy;
+    Bo[2] = h2 * fz - a1 * miz;
     if (FORCE) {
-        const float di = mix * fx + miy * fy + miz * fz;
         const float a3 = 5.0f * a2 * irs2 + 4.0f * pp.al4_s * t;
         const float mimj = mix * mjx + miy * mjy + miz * mjz;
         const float c0 = (a2 * mimj - a3 * di * dj * rs * rs) * rs;
@@ -652,7 +671,8 @@ __device__ __forceinline__ void pair_terms32(const PairParams& pp, float rs, flo
 // exp(-(alpha r)^2) erfcx(alpha r), sharing the Gaussian of the t term.
 template <bool FORCE, bool MIXED>
 __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, double ir, double ux, double uy, double uz,
-                                           const double4& mi, const double4& mj, double B[3], double F[3]) {
+                                           const double4& mi, const double4& mj, double B[3], double Bo[3],
+                                           double F[3]) {
     if (!MIXED) {
         const double ir2 = ir * ir;
         const double x = pp.alpha * r;
@@ -661,12 +681,16 @@ __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, doubl
         const double a1 = gau * erfcx64(x) * (ir2 * ir) + t;
         const double a2 = 3.0 * a1 * ir2 + 2.0 * pp.alpha2 * t;
         const double dj = mj.x * ux + mj.y * uy + mj.z * uz;
+        const double di = mi.x * ux + mi.y * uy + mi.z * uz;
         const double g2 = a2 * dj * r * r;
         B[0] = g2 * ux - a1 * mj.x;
         B[1] = g2 * uy - a1 * mj.y;
         B[2] = g2 * uz - a1 * mj.z;
+        const double h2 = a2 * di * r * r;
+        Bo[0] = h2 * ux - a1 * mi.x;
+        Bo[1] = h2 * uy - a1 * mi.y;
+        Bo[2] = h2 * uz - a1 * mi.z;
         if (FORCE) {
-            const double di = mi.x * ux + mi.y * uy + mi.z * uz;
             const double a3 = 5.0 * a2 * ir2 + 4.0 * pp.alpha4 * t;
             const double mimj = mi.x * mj.x + mi.y * mj.y + mi.z * mj.z;
             const double c0 = (a2 * mimj - a3 * di * dj * r * r) * r;
@@ -677,12 +701,13 @@ __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, doubl
         }
     } else {
         const double s = pp.inv_mref;
-        float b[3], f[3];
+        float b[3], bo[3], f[3];
         pair_terms32<FORCE>(pp, (float)(r * pp.inv_ell), (float)(ir * pp.ell), (float)ux, (float)uy, (float)uz,
                             (float)(mi.x * s), (float)(mi.y * s), (float)(mi.z * s), (float)(mj.x * s),
-                            (float)(mj.y * s), (float)(mj.z * s), b, f);
+                            (float)(mj.y * s), (float)(mj.z * s), b, bo, f);
         for (int d = 0; d < 3; ++d) {
             B[d] = b[d];
+            Bo[d] = bo[d];
             if (FORCE) F[d] = f[d];
         }
     }
@@ -721,54 +746,125 @@ __device__ __forceinline__ void apply_rep(PairParams& p, const Rep& R) {
     p.al4_s = R.al4_s;
 }
 
-// One queued pair (lane il of the wave, sorted index j). Pairs with an
-// endpoint moved by the overlap correction are left to k_pairs_moved.
+// Pair sums cross wavefronts as two 64-bit fixed-point words,
+// x = H s1 + L s2 with s1 = 2^e (per replica and quantity) and s2 = s1 2^-40:
+// integer atomics, so the totals do not depend on the order in which the
+// wavefronts add them. Both words are read off the binary64 pattern of
+// v + 1.5 2^52, which for |v| < 2^51 holds round(v) in its low bits; the
+// residual x - H s1 is exact (Sterbenz). Returns false when |x| / s1 is not
+// below 2^50 (or x is not finite).
+constexpr double FX_MAGIC = 6755399441055744.0;  // 1.5 * 2^52
+constexpr double FX_LO = 1099511627776.0;        // 2^40 = s1 / s2
+__device__ __forceinline__ bool fx_add(unsigned long long* __restrict__ hi, unsigned long long* __restrict__ lo,
+                                       double x, double s1, double is1) {
+    const double t = fma(x, is1, FX_MAGIC);
+    const double h = t - FX_MAGIC;
+    const double r = fma(-h, s1, x);
+    const double t2 = fma(r, is1 * FX_LO, FX_MAGIC);
+    const long long H = __double_as_longlong(t) - __double_as_longlong(FX_MAGIC);
+    const long long L = __double_as_longlong(t2) - __double_as_longlong(FX_MAGIC);
+    if (H) atomicAdd(hi, (unsigned long long)H);
+    if (L) atomicAdd(lo, (unsigned long long)L);
+    return fabs(h) < 0x1p50;
+}
+__device__ __forceinline__ double fx_value(unsigned long long hi, unsigned long long lo, double s1) {
+    return fma((double)(long long)hi, s1, (double)(long long)lo * (s1 * (1.0 / FX_LO)));
+}
+
+// Add component c (force 0..2, field 3..5) of a pair sum for cell-sorted
+// particle k.
+__device__ __forceinline__ bool fx_add_comp(const PairOut& out, const Rep& R, int64_t n, int c, int64_t k, double x) {
+    const int q = c < 3 ? 1 : 0;  // force, field
+    return fx_add(&out.acc[(int64_t)(2 * c) * n + k], &out.acc[(int64_t)(2 * c + 1) * n + k], x, R.fx_s[q],
+                  R.fx_is[q]);
+}
+
+// Sum over the four lanes of a quad (DPP quad permutations, fixed order:
+// every lane gets (v0 + v1) + (v2 + v3) of its quad).
+__device__ __forceinline__ float quad_sum(float v) {
+    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, true));
+    return v;
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ double quad_sum(double v) {
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    return v;
+}
+
+// Queue entries: (j << 6) | lane of i; PAIR_PAD marks a padding slot.
+constexpr int32_t PAIR_PAD = (int32_t)0x80000000u;
+
+// One queued pair (lane il of the chunk, sorted index j in the window that
+// starts at jb), evaluated once for both particles. The four lanes of a quad
+// share j (a candidate's hits are queued in whole quads); j's terms are summed
+// over the quad and added once by its first lane (`leader`). Every load is
+// issued first (one memory round trip); padding and inactive lanes read
+// entry 0's records, which exist, and are masked. Pairs with an endpoint
+// moved by the overlap correction are left to k_pairs_moved.
 template <bool FORCE, bool COATING, bool MIXED>
 __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __restrict__ rec_mom, const PairParams& pp,
-                                           IWave<MIXED, COATING>& W, int32_t entry, bool active, bool moved_any,
-                                           double press[3],
+                                           const IChunk<MIXED, COATING>& C, IWave<FORCE>& W, int32_t entry,
+                                           bool active, bool leader, bool moved_any, int32_t jb, double press[3],
                                            int64_t& npairs) {
-    if (!active) return;
+    constexpr int FB = FORCE ? 3 : 0;  // first field component of the accumulators
+    active = active && entry >= 0;
     const int il = entry & 63;
-    const int32_t j = entry >> 6;
-    const double4 pi = W.ipos[il];
+    const int32_t j = (entry >> 6) & 0x1FFFFFF;  // padding: 0
+    const int js = j - jb;
+    const double4 pi = C.ipos[il];
     const double4 pj = g.rec_pos[j];
+    const auto mi = C.imom[il];
+    const auto mj = [&] {
+        if constexpr (MIXED && !COATING) return g.rec_mom32[j];
+        else return rec_mom[j];
+    }();
     const long long ti = __double_as_longlong(pi.w), tj = __double_as_longlong(pj.w);
-    if (moved_any && ((ti | tj) >> 32)) return;  // an endpoint moved
-    const int32_t oi = (int32_t)ti, oj = (int32_t)tj;
-    const bool i_lo = oi < oj;
+    bool valid = active && !(moved_any && ((ti | tj) >> 32));  // not if an endpoint moved
+    const bool i_lo = (int32_t)ti < (int32_t)tj;
     if constexpr (MIXED && !COATING) {
         // separation in FP64 (positions are absolute), everything after it in
-        // FP32 in units of ell = r_c, moments in units of mref (FP32 copies)
+        // FP32 in units of ell = r_c, moments in units of mref (FP32 copies);
+        // u from i to j
         const double L = g.L, iL = g.inv_L;
         const float vx = (float)(min_image_r(pj.x - pi.x, L, iL) * pp.inv_ell);
         const float vy = (float)(min_image_r(pj.y - pi.y, L, iL) * pp.inv_ell);
         const float vz = (float)(min_image_r(pj.z - pi.z, L, iL) * pp.inv_ell);
         const float r2 = vx * vx + vy * vy + vz * vz;
-        if (!(r2 < 1.0f)) return;
-        ++npairs;
+        valid = valid && r2 < 1.0f;
         const float irs = __builtin_amdgcn_rsqf(r2);
         const float rs = r2 * irs;
         const float ux = vx * irs, uy = vy * irs, uz = vz * irs;
-        const float4 mi = W.imom[il];
-        const float4 mj = g.rec_mom32[j];
-        float B[3], F[3];
-        pair_terms32<FORCE>(pp, rs, irs, ux, uy, uz, mi.x, mi.y, mi.z, mj.x, mj.y, mj.z, B, F);
-        atomicAdd(&W.acc[3][il], (double)B[0]);
-        atomicAdd(&W.acc[4][il], (double)B[1]);
-        atomicAdd(&W.acc[5][il], (double)B[2]);
-        if (FORCE) {
-            if (i_lo) {
+        float Bi[3], Bj[3], F[3];
+        pair_terms32<FORCE>(pp, rs, irs, ux, uy, uz, mi.x, mi.y, mi.z, mj.x, mj.y, mj.z, Bi, Bj, F);
+        if (valid) {
+            ++npairs;
+            for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[FB + d][il], (double)Bi[d]);
+            if (FORCE) {
                 press[0] += (double)(rs * ux * F[1]);
                 press[1] += (double)(rs * ux * F[2]);
                 press[2] += (double)(rs * uy * F[2]);
+                for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[d][il], -(double)F[d]);
             }
-            atomicAdd(&W.acc[0][il], -(double)F[0]);
-            atomicAdd(&W.acc[1][il], -(double)F[1]);
-            atomicAdd(&W.acc[2][il], -(double)F[2]);
         }
+        for (int d = 0; d < 3; ++d) {
+            const float b = quad_sum(valid ? Bj[d] : 0.0f);
+            if (leader) atomicAdd(&W.jacc[FB + d][js], (double)b);
+        }
+        if (FORCE)
+            for (int d = 0; d < 3; ++d) {
+                const float f = quad_sum(valid ? F[d] : 0.0f);
+                if (leader) atomicAdd(&W.jacc[d][js], (double)f);
+            }
     } else {
-        // canonical i < j orientation of CompInteractVecs (src/interaction.cpp:99-104)
+        // canonical lo -> hi orientation of CompInteractVecs (src/interaction.cpp:99-104)
         const double L = g.L, iL = g.inv_L;
         const double vx = min_image_r(i_lo ? pj.x - pi.x : pi.x - pj.x, L, iL);
         const double vy = min_image_r(i_lo ? pj.y - pi.y : pi.y - pj.y, L, iL);
@@ -776,53 +872,75 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         const double r2 = vx * vx + vy * vy + vz * vz;
         const double ir = rsqrt64(r2);
         const double r = r2 * ir;
-        if (!(r < pp.rc)) return;
-        ++npairs;
-        const double sir = i_lo ? ir : -ir;  // u from i to j
-        const double ux = vx * sir, uy = vy * sir, uz = vz * sir;
-        double B[3], F[3];
-        pair_terms<FORCE, MIXED>(pp, r, ir, ux, uy, uz, W.imom[il], rec_mom[j], B, F);
-        atomicAdd(&W.acc[3][il], B[0]);
-        atomicAdd(&W.acc[4][il], B[1]);
-        atomicAdd(&W.acc[5][il], B[2]);
-        if (FORCE) {
-            if (i_lo) {
+        valid = valid && r < pp.rc;
+        const double ux = vx * ir, uy = vy * ir, uz = vz * ir;  // u from lo to hi
+        double Bl[3], Bh[3], F[3];  // field on lo, on hi; force on hi
+        pair_terms<FORCE, MIXED>(pp, r, ir, ux, uy, uz, i_lo ? mi : mj, i_lo ? mj : mi, Bl, Bh, F);
+        if (valid) {
+            ++npairs;
+            for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[FB + d][il], i_lo ? Bl[d] : Bh[d]);
+            if (FORCE) {
                 const double rr = MIXED ? r * pp.inv_ell : r;
                 press[0] += rr * ux * F[1];
                 press[1] += rr * ux * F[2];
                 press[2] += rr * uy * F[2];
-            }
-            if (COATING && r < pp.rc_sr) {
-                const double rmi = pp.r_mag[oi], rmj = pp.r_mag[oj];
-                const double sf = short_range(pp, r, i_lo ? rmi : rmj, i_lo ? rmj : rmi) / pp.fscale;
-                F[0] += sf * ux;
-                F[1] += sf * uy;
-                F[2] += sf * uz;
-                if (i_lo) {
-                    const double rr = r * pp.inv_ell;
+                if (COATING && r < pp.rc_sr) {
+                    const double rml = pp.r_mag[i_lo ? (int32_t)ti : (int32_t)tj];
+                    const double rmh = pp.r_mag[i_lo ? (int32_t)tj : (int32_t)ti];
+                    const double sf = short_range(pp, r, rml, rmh) / pp.fscale;
+                    F[0] += sf * ux;
+                    F[1] += sf * uy;
+                    F[2] += sf * uz;
                     press[0] += rr * ux * (sf * uy);
                     press[1] += rr * ux * (sf * uz);
                     press[2] += rr * uy * (sf * uz);
                 }
+                for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[d][il], i_lo ? -F[d] : F[d]);
             }
-            atomicAdd(&W.acc[0][il], -F[0]);
-            atomicAdd(&W.acc[1][il], -F[1]);
-            atomicAdd(&W.acc[2][il], -F[2]);
         }
+        for (int d = 0; d < 3; ++d) {
+            const double b = quad_sum(valid ? (i_lo ? Bh[d] : Bl[d]) : 0.0);
+            if (leader) atomicAdd(&W.jacc[FB + d][js], b);
+        }
+        if (FORCE)
+            for (int d = 0; d < 3; ++d) {
+                const double f = quad_sum(valid ? (i_lo ? F[d] : -F[d]) : 0.0);
+                if (leader) atomicAdd(&W.jacc[d][js], f);
+            }
     }
+}
+
+// Evaluate the queued pairs (qn slots, whole quads) in ceil(qn / 64)
+// batches; batch b takes quads b, b + nb, b + 2 nb, ... Afterwards the used
+// slots are padding again.
+template <bool FORCE, bool COATING, bool MIXED>
+__device__ __forceinline__ void pair_flush(const Geometry& g, const double4* __restrict__ rec_mom,
+                                           const PairParams& pp, const IChunk<MIXED, COATING>& C, IWave<FORCE>& W,
+                                           int qn, bool moved_any, int32_t jb, double press[3], int64_t& npairs) {
+    wave_sync();
+    const int lane = lane_id();
+    const int nq = qn >> 2;
+    const int nb = (nq + WAVE / 4 - 1) / (WAVE / 4);
+    for (int b = 0; b < nb; ++b) {
+        const int qi = (lane >> 2) * nb + b;
+        const bool act = qi < nq;
+        pair_dense<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, act ? W.q[4 * qi + (lane & 3)] : PAIR_PAD, act,
+                                          act && (lane & 3) == 0, moved_any, jb, press, npairs);
+    }
+    wave_sync();
+    for (int p = lane; p < qn; p += WAVE) W.q[p] = PAIR_PAD;
+    wave_sync();
 }
 
 template <bool FORCE, bool COATING, bool MIXED>
 __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
                                             PairOut out, const int4* __restrict__ stencil_all,
                                             unsigned long long* __restrict__ cnt_pairs) {
-    // One block per chunk of up to 64 particles of one cell; its 4 wavefronts
-    // split the stencil between them and merge their sums in LDS at the end.
-    __shared__ IWave<MIXED, COATING> waves[4];
-    __shared__ double wpress[4][3];
-    const int wv = threadIdx.x / WAVE;
-    const int lane = lane_id();
-    IWave<MIXED, COATING>& W = waves[wv];
+    using IW = IWave<FORCE>;
+    __shared__ IChunk<MIXED, COATING> C;
+    __shared__ IW waves[PAIR_SLICES];
+    const int wv = threadIdx.x / WAVE, lane = lane_id();
+    IW& W = waves[wv];
     const int32_t w = blockIdx.x;
     const int32_t nwork = *g.nwork;
     if (w >= nwork) return;  // uniform over the block
@@ -833,8 +951,9 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     g.L = R.L;
     g.inv_L = 1.0 / R.L;
     apply_rep(pp, R);
-    const int4* stencil = stencil_all + R.sten_off;
-    const int32_t nsten = R.nsten;
+    const int4* stencil = stencil_all + R.sten_off;  // [0] is the cell itself, then the positive half
+    const int32_t nhalf = R.nhalf;
+    const int64_t n = g.n;
     const int32_t s0 = g.cell_start[cell] + chunk * WAVE;
     const int32_t e0 = min(s0 + WAVE, g.cell_end[cell]);
     const int32_t k = s0 + lane;
@@ -842,13 +961,20 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     double4 pi = make_double4(0, 0, 0, 0);
     if (vi) {
         pi = g.rec_pos[k];
-        W.ipos[lane] = pi;
-        if constexpr (MIXED && !COATING)
-            W.imom[lane] = g.rec_mom32[k];
-        else
-            W.imom[lane] = rec_mom[k];
+        if (wv == 0) {
+            C.ipos[lane] = pi;
+            if constexpr (MIXED && !COATING)
+                C.imom[lane] = g.rec_mom32[k];
+            else
+                C.imom[lane] = rec_mom[k];
+        }
     }
-    for (int d = 0; d < 6; ++d) W.acc[d][lane] = 0.0;
+    for (int d = 0; d < IW::NA; ++d) {
+        W.acc[d][lane] = 0.0;
+        W.jacc[d][lane] = 0.0;
+    }
+    for (int p = lane; p < PAIR_QCAP; p += WAVE) W.q[p] = PAIR_PAD;
+    __syncthreads();
     const bool moved_any = *g.n_events > 0;
     const double L = g.L;
     const int nc = R.ncell;
@@ -864,8 +990,9 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     double press[3] = {0, 0, 0};
     int64_t npairs = 0;
     int qn = 0;
-    wave_sync();
-    for (int si = wv; si < nsten; si += 4) {
+    bool ok = true;
+    for (int si = wv; si < nhalf; si += PAIR_SLICES) {
+        const bool self = si == 0;
         const int4 o = stencil[si];
         int jx = cx + o.x, jy = cy + o.y, jz = cz + o.z;
         // periodic image of the neighbour cell nearest this cell
@@ -874,13 +1001,15 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
         if (jy < 0) { jy += nc; sy = -L; } else if (jy >= nc) { jy -= nc; sy = L; }
         if (jz < 0) { jz += nc; sz = -L; } else if (jz >= nc) { jz -= nc; sz = L; }
         const int c = R.cell_off + (jz * nc + jy) * nc + jx;
-        const int32_t cs = g.cell_start[c], ce = g.cell_end[c];
+        const int32_t ce = g.cell_end[c];
+        // in the cell itself only j > i: the blocks from this chunk on
+        const int32_t cs = self ? s0 : g.cell_start[c];
         for (int32_t jb = cs; jb < ce; jb += WAVE) {
             const int32_t jl = jb + lane;
             // j's image nearest this cell (exact: every j within r_c of an i
             // of this cell lies in exactly one stencil cell), in FP32 for the
-            // distance test; the exact r < r_c decision is made in FP64 on the
-            // canonical pair
+            // distance test; the exact r < r_c decision is made on the
+            // evaluated pair
             float qx = 0, qy = 0, qz = 0;
             if (jl < ce) {
                 const float4 pj = g.rec_pos32[jl];
@@ -889,6 +1018,7 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
                 qz = pj.z + (float)sz;
             }
             const int cnt = min(WAVE, ce - jb);
+            bool any = false;
             for (int t = 0; t < cnt; ++t) {
                 float dx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qx), t)) - pxf;
                 float dy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qy), t)) - pyf;
@@ -899,72 +1029,57 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
                     dz = dz > Lf / 2 ? dz - Lf : (dz < -Lf / 2 ? dz + Lf : dz);
                 }
                 const bool inside = dx * dx + dy * dy + dz * dz < rr2f;
-                const bool hit = vi && (jb + t != k) && inside;
+                const bool hit = vi && (!self || jb + t > k) && inside;
                 const uint64_t m = __ballot(hit);
                 if (m) {
+                    // the hits of one candidate fill whole quads
+                    any = true;
                     if (hit) W.q[qn + mbcnt(m)] = ((jb + t) << 6) | lane;
-                    qn += __popcll(m);
-                    if (qn >= WAVE) {
-                        wave_sync();
-                        pair_dense<FORCE, COATING, MIXED>(g, rec_mom, pp, W, W.q[lane], true, moved_any, press,
-                                                          npairs);
-                        wave_sync();
-                        const int rest = qn - WAVE;
-                        int32_t moved = lane < rest ? W.q[WAVE + lane] : 0;
-                        wave_sync();
-                        if (lane < rest) W.q[lane] = moved;
-                        qn = rest;
-                        wave_sync();
+                    qn += (__popcll(m) + 3) & ~3;
+                    if (qn >= PAIR_QFLUSH) {
+                        pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, press, npairs);
+                        qn = 0;
                     }
                 }
             }
+            if (any) {
+                // window done: its j sums go to the global sums
+                if (qn > 0) pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, press, npairs);
+                qn = 0;
+                if (lane < cnt)
+                    for (int d = 0; d < IW::NA; ++d)
+                        ok &= fx_add_comp(out, R, n, FORCE ? d : d + 3, jl, W.jacc[d][lane]);
+                for (int d = 0; d < IW::NA; ++d) W.jacc[d][lane] = 0.0;
+                wave_sync();
+            }
         }
     }
-    wave_sync();
-    if (qn > 0)
-        pair_dense<FORCE, COATING, MIXED>(g, rec_mom, pp, W, lane < qn ? W.q[lane] : 0, lane < qn, moved_any, press,
-                                          npairs);
     if (FORCE) {
         const double p0 = wave_sum(press[0]), p1 = wave_sum(press[1]), p2 = wave_sum(press[2]);
-        if (lane == 0) {
-            wpress[wv][0] = p0;
-            wpress[wv][1] = p1;
-            wpress[wv][2] = p2;
-        }
+        if (lane < 3)
+            out.press_partial[3 * ((int64_t)w * PAIR_SLICES + wv) + lane] =
+                (lane == 0 ? p0 : lane == 1 ? p1 : p2) * pp.pscale;
     }
     if (cnt_pairs) {
         const int64_t np = wave_sum(npairs);
-        if (lane == 0) atomicAdd(cnt_pairs, (unsigned long long)np);
+        if (lane == 0) atomicAdd(cnt_pairs, (unsigned long long)(2 * np));  // ordered pairs
     }
+    // the wavefronts' i sums, in a fixed order
     __syncthreads();
-    if (wv == 0) {
-        if (vi) {
-            const int32_t o = W.orig(lane);
-            const int64_t n = g.n;
-            double a[6];
-            for (int d = 0; d < 6; ++d)
-                a[d] = (waves[0].acc[d][lane] + waves[1].acc[d][lane]) + (waves[2].acc[d][lane] + waves[3].acc[d][lane]);
-            out.field[o] = a[3] * pp.bscale;
-            out.field[n + o] = a[4] * pp.bscale;
-            out.field[2 * n + o] = a[5] * pp.bscale;
-            if (FORCE) {
-                out.force[o] = a[0] * pp.fscale;
-                out.force[n + o] = a[1] * pp.fscale;
-                out.force[2 * n + o] = a[2] * pp.fscale;
-            }
+    if (wv == 0 && vi)
+        for (int d = 0; d < IW::NA; ++d) {
+            const double x = (waves[0].acc[d][lane] + waves[1].acc[d][lane]) +
+                              (waves[2].acc[d][lane] + waves[3].acc[d][lane]);
+            ok &= fx_add_comp(out, R, n, FORCE ? d : d + 3, k, x);
         }
-        if (FORCE && lane < 3)
-            out.press_partial[3 * (int64_t)w + lane] =
-                ((wpress[0][lane] + wpress[1][lane]) + (wpress[2][lane] + wpress[3][lane])) * pp.pscale;
-    }
+    if (!ok) atomicOr(out.err, 64);
 }
 
-// Five wavefronts per SIMD (the LDS allows five blocks per CU): the FP64
-// instantiations are held to the matching register budget.
+// Five blocks per CU (the LDS allows five).
 template <bool FORCE, bool MIXED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_pairs(
-    Geometry g, const double4* __restrict__ rec_mom, PairParams pp, PairOut out, const int4* __restrict__ stencil_all,
-    unsigned long long* __restrict__ cnt_pairs) {
+__global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
+                                               PairOut out, const int4* __restrict__ stencil_all,
+                                               unsigned long long* __restrict__ cnt_pairs) {
     pairs_block<FORCE, false, MIXED>(g, rec_mom, pp, out, stencil_all, cnt_pairs);
 }
 
@@ -976,12 +1091,57 @@ __global__ __launch_bounds__(256) void k_pairs_coat(Geometry g, const double4* _
     pairs_block<true, true, MIXED>(g, rec_mom, pp, out, stencil_all, cnt_pairs);
 }
 
+// Cell-sorted fixed-point pair sums -> original order, in SI units.
+__global__ void k_pairs_out(int64_t n, const int32_t* __restrict__ sorted_idx, const int32_t* __restrict__ rep_of,
+                            const Rep* __restrict__ reps, const unsigned long long* __restrict__ acc,
+                            double* __restrict__ force, double* __restrict__ field) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int32_t o = sorted_idx[k];
+    const Rep& R = reps[rep_of[o]];
+    for (int d = 0; d < 3; ++d)
+        field[d * n + o] = fx_value(acc[(6 + 2 * d) * n + k], acc[(7 + 2 * d) * n + k], R.fx_s[0]) * R.bscale;
+    if (force)
+        for (int d = 0; d < 3; ++d)
+            force[d * n + o] = fx_value(acc[(2 * d) * n + k], acc[(2 * d + 1) * n + k], R.fx_s[1]) * R.fscale;
+}
+
+// Within each cell, order the particle list by original index (k_place
+// fills it in atomic order): one wavefront per cell, ranks by counting.
+__global__ __launch_bounds__(64) void k_cell_order(int32_t ncells, const int32_t* __restrict__ start,
+                                                   const int32_t* __restrict__ end, int32_t* __restrict__ sorted_idx,
+                                                   int32_t* __restrict__ tmp) {
+    const int lane = lane_id();
+    for (int32_t c = blockIdx.x; c < ncells; c += gridDim.x) {
+        const int32_t s = start[c], m = end[c] - s;
+        if (m <= 1) continue;
+        if (m <= WAVE) {
+            const int32_t v = lane < m ? sorted_idx[s + lane] : INT32_MAX;
+            int r = 0;
+            for (int t = 0; t < m; ++t) r += __builtin_amdgcn_readlane(v, t) < v;
+            if (lane < m) sorted_idx[s + r] = v;
+        } else {
+            for (int32_t e = lane; e < m; e += WAVE) {
+                const int32_t v = sorted_idx[s + e];
+                int32_t r = 0;
+                for (int32_t t = 0; t < m; ++t) r += sorted_idx[s + t] < v;
+                tmp[s + r] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            for (int32_t e = lane; e < m; e += WAVE) sorted_idx[s + e] = tmp[s + e];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
 // Pairs with at least one endpoint moved by the overlap correction, in FP64
 // with the positions the reference's sequential scan gives each pair
 // (CompInteractVecs, src/interaction.cpp:99-129): one wavefront per moved
 // particle p, lanes over the stencil's particles; a pair of two moved
-// particles is done by the lower index's wavefront. Results are added with
-// atomics after k_pairs has written its sums.
+// particles is done by the lower index's wavefront. Results go into the same
+// fixed-point sums as k_pairs' (before k_pairs_out).
 template <bool FORCE, bool COATING>
 __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
                                                     PairOut out, const int4* __restrict__ stencil_all,
@@ -989,6 +1149,7 @@ __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* _
     const int32_t nmv = *g.n_moved;
     const int lane = lane_id();
     const int64_t n = g.n;
+    bool ok = true;
     for (int32_t mi = blockIdx.x; mi < nmv; mi += gridDim.x) {
         const int32_t p = g.moved_list[mi];
         const Rep& R = g.reps[g.rep_of[p]];
@@ -1031,24 +1192,26 @@ __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* _
                 // u from lo to hi, taken from the pre-correction separation
                 const double ux = vx * ir0, uy = vy * ir0, uz = vz * ir0;
                 const double4 mlo = p_lo ? mp : rec_mom[j], mhi = p_lo ? rec_mom[j] : mp;
-                double Bl[3], Fh[3], Bh[3], dummy[3];
-                // field on lo from hi and force on hi (u from lo to hi) ...
-                pair_terms<FORCE, false>(pp, r, ir, ux, uy, uz, mlo, mhi, Bl, Fh);
-                // ... and field on hi from lo (u from hi to lo)
-                pair_terms<false, false>(pp, r, ir, -ux, -uy, -uz, mhi, mlo, Bh, dummy);
+                double Bl[3], Bh[3], Fh[3];
+                // fields on lo and on hi, force on hi (u from lo to hi)
+                pair_terms<FORCE, false>(pp, r, ir, ux, uy, uz, mlo, mhi, Bl, Bh, Fh);
+                // into the fixed-point sums (units of the pair kernel's sums)
                 const double f = pp.fac;
+                const int32_t klo = p_lo ? kp : j, khi = p_lo ? j : kp;
+                const double cb = f / pp.bscale;
                 for (int d = 0; d < 3; ++d) {
-                    atomicAdd(&out.field[d * n + lo], f * Bl[d]);
-                    atomicAdd(&out.field[d * n + hi], f * Bh[d]);
+                    ok &= fx_add_comp(out, R, n, 3 + d, klo, cb * Bl[d]);
+                    ok &= fx_add_comp(out, R, n, 3 + d, khi, cb * Bh[d]);
                 }
                 if (FORCE) {
                     double sf = 0;
                     if (COATING && r0 < pp.rc_sr) sf = short_range(pp, r, p_lo ? rmp : rmj, p_lo ? rmj : rmp);
                     const double u3[3] = {ux, uy, uz};
+                    const double cf = 1.0 / pp.fscale;
                     for (int d = 0; d < 3; ++d) {
                         const double Fd = f * Fh[d] + sf * u3[d];
-                        atomicAdd(&out.force[d * n + lo], -Fd);
-                        atomicAdd(&out.force[d * n + hi], Fd);
+                        ok &= fx_add_comp(out, R, n, d, klo, -Fd * cf);
+                        ok &= fx_add_comp(out, R, n, d, khi, Fd * cf);
                     }
                     press[0] += r * ux * (f * Fh[1] + sf * uy);
                     press[1] += r * ux * (f * Fh[2] + sf * uz);
@@ -1070,6 +1233,7 @@ __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* _
             if (lane == 0) atomicAdd(cnt_pairs, (unsigned long long)(2 * np));
         }
     }
+    if (!ok) atomicOr(out.err, 64);
 }
 
 // ------------------------------------------------------------------ k space
@@ -1755,14 +1919,20 @@ hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, const
 
 hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
                         const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
-                        int64_t* cnt_pairs) {
+                        int32_t moved_press_off, int64_t* cnt_pairs) {
+    const int64_t n = g.n;
+    if (max_work <= 0 || n <= 0) return hipSuccess;
+    // the pass's words of the fixed-point sums start at zero (field words are
+    // 6n..12n, force words 0..6n)
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(6 * n, 256)), dim3(256), 0, s, (double*)out.acc + (force ? 0 : 6 * n),
+                       (force ? 12 : 6) * n, 0.0);
     const int blocks = max_work;
-    if (blocks <= 0) return hipSuccess;
     unsigned long long* cp = (unsigned long long*)cnt_pairs;
     const bool mixed = precision == 1;
 #define KP(F, M) \
-    hipLaunchKernelGGL((k_pairs<F, M>), dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out, stencil, cp)
-#define KC(M) hipLaunchKernelGGL((k_pairs_coat<M>), dim3(blocks), dim3(256), 0, s, g, rec_mom, pp, out, stencil, cp)
+    hipLaunchKernelGGL((k_pairs<F, M>), dim3(blocks), dim3(WAVE * PAIR_SLICES), 0, s, g, rec_mom, pp, out, stencil, cp)
+#define KC(M) \
+    hipLaunchKernelGGL((k_pairs_coat<M>), dim3(blocks), dim3(WAVE * PAIR_SLICES), 0, s, g, rec_mom, pp, out, stencil, cp)
     if (force && pp.coating) {
         if (mixed) KC(true); else KC(false);
     } else if (force) {
@@ -1772,23 +1942,27 @@ hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geom
     }
 #undef KP
 #undef KC
+    // pairs with an endpoint moved by the overlap correction (FP64)
+    const int mblocks = 512;
+    if (force && pp.coating)
+        hipLaunchKernelGGL((k_pairs_moved<true, true>), dim3(mblocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
+                           moved_press_off, cp);
+    else if (force)
+        hipLaunchKernelGGL((k_pairs_moved<true, false>), dim3(mblocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
+                           moved_press_off, cp);
+    else
+        hipLaunchKernelGGL((k_pairs_moved<false, false>), dim3(mblocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
+                           moved_press_off, cp);
+    hipLaunchKernelGGL(k_pairs_out, dim3(grid_for(n, 256)), dim3(256), 0, s, n, g.sorted_idx, g.rep_of, g.reps,
+                       out.acc, out.force, out.field);
     LAUNCH_CHECK();
 }
 
-hipError_t launch_pairs_moved(hipStream_t s, bool force, const Geometry& g, const double4* rec_mom,
-                              const PairParams& pp, const PairOut& out, const int4* stencil, int32_t press_off,
-                              int64_t* cnt_pairs) {
-    unsigned long long* cp = (unsigned long long*)cnt_pairs;
-    const int blocks = 512;
-    if (force && pp.coating)
-        hipLaunchKernelGGL((k_pairs_moved<true, true>), dim3(blocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
-                           press_off, cp);
-    else if (force)
-        hipLaunchKernelGGL((k_pairs_moved<true, false>), dim3(blocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
-                           press_off, cp);
-    else
-        hipLaunchKernelGGL((k_pairs_moved<false, false>), dim3(blocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
-                           press_off, cp);
+hipError_t launch_cell_order(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
+                             int32_t* sorted_idx, int32_t* tmp) {
+    if (ncells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cell_order, dim3(ncells < 8192 ? ncells : 8192), dim3(WAVE), 0, s, ncells, start, end, sorted_idx,
+                       tmp);
     LAUNCH_CHECK();
 }
 
