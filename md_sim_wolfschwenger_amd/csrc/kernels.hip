@@ -38,6 +38,9 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+// Wavefront index within the block, as a value the compiler knows to be
+// uniform over the wavefront (so what it indexes stays in scalar registers).
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); }
 
 __device__ __forceinline__ int mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
@@ -939,7 +942,7 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     using IW = IWave<FORCE>;
     __shared__ IChunk<MIXED, COATING> C;
     __shared__ IW waves[PAIR_SLICES];
-    const int wv = threadIdx.x / WAVE, lane = lane_id();
+    const int wv = wave_id(), lane = lane_id();
     IW& W = waves[wv];
     const int32_t w = blockIdx.x;
     const int32_t nwork = *g.nwork;
@@ -1377,7 +1380,7 @@ __global__ __launch_bounds__(256) void k_kpack(int32_t npk_all, const Rep* __res
                                                const KPack* __restrict__ desc, const double* __restrict__ partial_all,
                                                double4* __restrict__ pack) {
     __shared__ double4 part[4][WAVE];
-    const int wv = threadIdx.x / WAVE, lane = lane_id();
+    const int wv = wave_id(), lane = lane_id();
     const int j = blockIdx.x * WAVE + lane;
     double cp = 0, sp = 0, cm = 0, sm = 0;
     KPack d{-1, -1, 0.0};
@@ -1514,7 +1517,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                                   const double* __restrict__ add_force, double* __restrict__ force,
                                                   double* __restrict__ field) {
     __shared__ double part[4][6][WAVE];
-    const int wv = threadIdx.x / WAVE, lane = lane_id();
+    const int wv = wave_id(), lane = lane_id();
     const int64_t i = blockIdx.x * (int64_t)WAVE + lane;
     const int64_t ic = i < n ? i : n - 1;
     const int32_t rid = rep_of[ic];
