@@ -1462,7 +1462,9 @@ __device__ __forceinline__ void kscatter_rows(int64_t n, int64_t i, const Rep& R
         double xc = 1.0, xs = 0.0, ad = 0.0, am = 0.0;
         double rb = 0.0, rs = 0.0;
         // two entries per iteration (rows are padded with zero entries to an
-        // even length), loaded together so one wait covers both
+        // even length), loaded together so one wait covers both; the field
+        // pass (fewer registers) takes two such pairs per wait
+#pragma unroll(FORCE ? 1 : 2)
         for (int a = 0; a <= row.nxmax; a += 2) {
             const double4 p2[2] = {P[a], P[a + 1]};
 #pragma unroll
