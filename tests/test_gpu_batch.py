@@ -1,8 +1,10 @@
 """Replicas in one context (batched ensembles): every result of a replica in
 a batch equals the same system run alone in its own context, including
 Philox noise, graph-replayed steps, and replicas admitted and retired
-between runs. Differences can only come from the order of floating-point
-atomics in the overlap-pair kernel, so the tolerance is 1e-12 relative."""
+between runs. The real-space sums are fixed-point integers added by atomics
+and every other sum has a fixed order, so states agree bit for bit
+(`test_batch_state_bitwise_equal_alone`); the other checks keep a 1e-12
+relative tolerance."""
 import numpy as np
 import pytest
 
@@ -154,3 +156,23 @@ def test_batch_autotune_keeps_accuracy_model():
         assert info["alpha"] == pytest.approx(alpha, rel=1e-12)
         assert info["nc"] == nc
         assert info["r_cut"] == pytest.approx(rc, rel=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_state_bitwise_equal_alone(precision):
+    """Positions, moments and easy axes after graph-replayed Philox steps of
+    three replicas (two with planted overlaps) are bitwise equal to each
+    replica run alone: no sum depends on co-scheduling."""
+    print(f"batch bitwise {precision}", flush=True)
+    sys_ = [make_system(n, phi, rcf, ov, sd, precision) for n, phi, rcf, ov, _, sd in SYSTEMS]
+    fields = [f for *_, f, _ in SYSTEMS]
+    seeds = [9 + 77 * k for k in range(3)]
+    ctx = batch(sys_, fields, seeds)
+    ctx.use_philox(0, 0)
+    ctx.set_graph(True)
+    ctx.run(5, 0.0, 1e-12)
+    for k, (s, f, sd) in enumerate(zip(sys_, fields, seeds)):
+        a = alone(s, f, sd)
+        a.run(5, 0.0, 1e-12)
+        for x, y in zip(ctx.replica_state(k), a.get_state()):
+            assert np.array_equal(x, y), (k, float(np.max(np.abs(x - y))))

@@ -1,8 +1,8 @@
 """Continuous-batching request loop: each request's result is the same when
 it is served in a mix (admitted and retired at chunk boundaries, beside
 other requests, under an HBM budget that forces queueing) as when it is
-solved alone. Tolerance 1e-12 relative (only the order of atomics in the
-overlap-pair kernel and cell grids can differ)."""
+solved alone: bitwise for the state (no sum depends on co-scheduling), and
+the <m_z> series within 1e-11."""
 import copy
 
 import numpy as np
@@ -49,7 +49,7 @@ def test_each_result_same_alone_and_in_mix(precision):
         solo.close()
         assert alone.ewald == q.ewald
         for a, b in zip(q.state, alone.state):
-            assert rel_err(a, b) < TOL, q.rid
+            assert np.array_equal(a, b), (q.rid, rel_err(a, b))
         assert q.mz == pytest.approx(alone.mz, rel=1e-11, abs=1e-14)
         assert q.finished >= q.arrival
 
@@ -71,7 +71,7 @@ def test_lanes_each_result_same_alone():
         solo.serve([alone])
         solo.close()
         for a, b in zip(q.state, alone.state):
-            assert rel_err(a, b) < TOL, q.rid
+            assert np.array_equal(a, b), (q.rid, rel_err(a, b))
 
 
 def test_footprint_estimate_covers_device_bytes():
