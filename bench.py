@@ -201,7 +201,8 @@ def ensemble(mds, workload, precision, replicas, steps, warmup, master=20261203)
     return out
 
 
-def serving_leg(precision, n_req, steps_scale, budget_gib, load=0.8, seed=20261203, cu_fractions=None):
+def serving_leg(precision, n_req, steps_scale, budget_gib, load=0.8, seed=20261203, cu_fractions=None,
+                cu_spans=None):
     """Request stream through the continuous-batching loop: N log-uniform in
     [1e3, 2e5], steps log-uniform in [1e3, 1e5] x steps_scale, field uniform
     in [0, 100] mT, volume fraction uniform in [1 %, 10 %], own seeds. First
@@ -213,7 +214,7 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, load=0.8, seed=202612
     from md_sim_wolfschwenger_amd import serving
     rng = np.random.default_rng(seed)
     base = serving.poisson_stream(n_req, 0.0, rng, steps_range=(1e3 * steps_scale, 1e5 * steps_scale))
-    kw = dict(precision=precision, hbm_budget=budget_gib * 2 ** 30, cu_fractions=cu_fractions)
+    kw = dict(precision=precision, hbm_budget=budget_gib * 2 ** 30, cu_fractions=cu_fractions, cu_spans=cu_spans)
     srv = serving.LaneServer(**kw)
     sat = srv.serve(copy.deepcopy(base))
     srv.close()

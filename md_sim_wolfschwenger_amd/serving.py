@@ -265,9 +265,12 @@ class LaneServer:
 
     def __init__(self, precision: str = "fp64", hbm_budget: float = 64 * 2 ** 30,
                  bounds: tuple = (6e3, 4e4), chunk_seconds: tuple = (0.01, 0.04, 0.2),
-                 cu_fractions: tuple | None = None, device: int = 0, **kw):
+                 cu_fractions: tuple | None = None, cu_spans: tuple | None = None, device: int = 0, **kw):
         """cu_fractions: share of the device's compute units given to each
-        lane (disjoint ranges of CU ids); None lets every lane use all."""
+        lane (disjoint ranges of CU ids); cu_spans: per lane a (first, last)
+        fraction of the CU id range, which may overlap (a small-request lane
+        can keep a slice to itself while a larger lane shares the rest);
+        None lets every lane use all."""
         assert len(chunk_seconds) == len(bounds) + 1
         self.bounds = bounds
         self.budget = Budget(hbm_budget)
@@ -283,6 +286,11 @@ class LaneServer:
             total = capi.lib().mds_device_cus(device)
             edges = np.round(np.cumsum([0.0] + list(cu_fractions)) / sum(cu_fractions) * total).astype(int)
             self.cu_ranges = [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:])]
+        elif cu_spans:
+            assert len(cu_spans) == len(self.lanes)
+            total = capi.lib().mds_device_cus(device)
+            self.cu_ranges = [(int(round(a * total)), int(round(b * total))) for a, b in cu_spans]
+        if self.cu_ranges:
             for lane, (a, b) in zip(self.lanes, self.cu_ranges):
                 lane.ctx.set_cu_mask(list(range(a, max(b, a + 1))))
 
