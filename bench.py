@@ -271,6 +271,45 @@ def cpu_baseline(sysinfo, threads):
     return n / (t1 - t0), t1 - t0
 
 
+def mz_run(args):
+    """The C3 system stepped `--mz-run` times (fixed dt 1e-12 s, Philox noise
+    of the bench's seed, so both precisions draw the same noise), in chunks
+    of 1000 graph-replayed steps with a progress line each; writes the
+    per-step <m_z>, the final state and the step counter, from which
+    `--mz-resume` continues (same noise stream)."""
+    import md_sim_wolfschwenger_amd as mds
+    from md_sim_wolfschwenger_amd import workload
+    ctx, info = make_context(mds, workload, args.mz_precision, 0.225, args.seed)
+    ctx.set_tiling(3)
+    t, dt, done, series = 0.0, 1e-12, 0, []
+    if args.mz_resume:
+        z = np.load(args.mz_resume)
+        ctx.set_state(z["pos"], z["mm"], z["ea"])
+        t, done, series = float(z["t"]), int(z["steps"]), [z["mz"]]
+        ctx.use_philox(args.seed, done)
+    ctx.set_graph(True)
+    os.makedirs(os.path.dirname(args.mz_out) or ".", exist_ok=True)
+
+    def save():
+        pos, mm, ea = ctx.get_state()
+        np.savez(args.mz_out, mz=np.concatenate(series), pos=pos, mm=mm, ea=ea, t=t, steps=done)
+
+    t0 = time.perf_counter()
+    left = args.mz_run
+    while left > 0:
+        k = min(1000, left)
+        t, dt, mz = ctx.run(k, t, dt, want_mz=True)
+        series.append(np.asarray(mz))
+        done += k
+        left -= k
+        log(f"{args.mz_precision} step {done}: <m_z> {series[-1][-1]:.6f} ({time.perf_counter() - t0:.0f} s)")
+        if done % 10000 == 0:
+            save()  # a resumable point
+    save()
+    ctx.close()
+    print(json.dumps({"mz_run": args.mz_precision, "steps": done, "mz_last": float(np.concatenate(series)[-1])}))
+
+
 def copy_calibration(mib):
     """Device-to-device copies of `mib` MiB (the HIP runtime's blit kernel):
     the reference reading for scaling the FETCH_SIZE / WRITE_SIZE counters of
@@ -309,6 +348,12 @@ def main():
     ap.add_argument("--steps-scale", type=float, default=0.01,
                     help="serving steps are log-uniform in [1e3, 1e5] times this")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0)
+    ap.add_argument("--mz-run", type=int, default=0,
+                    help="only run the C3 system for this many steps at r_c = 0.225 L, reach 3, and write its "
+                         "<m_z> series (the long-run mixed-vs-FP64 magnetisation check)")
+    ap.add_argument("--mz-precision", default="mixed", choices=("fp64", "mixed"))
+    ap.add_argument("--mz-resume", default="", help="state file of an earlier --mz-run to continue from")
+    ap.add_argument("--mz-out", default="gpurun_out/mz_run.npz")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run device-to-device copies of this size (profiler counter calibration)")
     args = ap.parse_args()
@@ -318,6 +363,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
+        return
+    if args.mz_run > 0:
+        mz_run(args)
         return
     import md_sim_wolfschwenger_amd as mds
     from md_sim_wolfschwenger_amd import workload
