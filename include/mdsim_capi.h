@@ -229,6 +229,21 @@ mds_status mds_step(mds_ctx* ctx, double* t, double* dt, double* dt_used, double
  * context's stream (captured once into a HIP graph and replayed). mz_series
  * (may be NULL) receives <m_z> after each step. */
 mds_status mds_run(mds_ctx* ctx, int64_t steps, double* t, double* dt, double* mz_series);
+
+/* Transport observables of EvalTransCoeffs (src/postprocessing.cpp:167-187,
+ * initialised at src/simulation.cpp:108-109). on != 0: the current positions
+ * become the reference and the first unwrapped positions, and every later
+ * step records per replica (t before the step, MSDx, MSDy, MSDz, Pxy, Pxz,
+ * Pyz): the mean squared displacement of the unwrapped positions from the
+ * reference per axis after the step's translation, and the pressure tensor
+ * components of the step's force evaluation over the box volume (the values
+ * the reference pushes to MSD* and P*Vec each step). Adding or removing
+ * replicas or setting particles switches it off. */
+mds_status mds_observe(mds_ctx* ctx, int32_t on);
+/* mds_run that also returns obs[step][replica][7] in the order above. */
+mds_status mds_run_observed(mds_ctx* ctx, int64_t steps, double* t, double* dt, double* mz_series, double* obs);
+/* The last step's observables, obs[replica][7] (after mds_step). */
+mds_status mds_last_observables(mds_ctx* ctx, double* obs);
 /* Wait for queued work. */
 mds_status mds_synchronize(mds_ctx* ctx);
 

@@ -99,6 +99,9 @@ SIGNATURES = [
     ("mds_rk_coeffs", C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp]),
     ("mds_step", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
     ("mds_run", C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, _dp]),
+    ("mds_observe", C.c_int, [C.c_void_p, C.c_int32]),
+    ("mds_run_observed", C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp]),
+    ("mds_last_observables", C.c_int, [C.c_void_p, _dp]),
     ("mds_synchronize", C.c_int, [C.c_void_p]),
     ("mds_enable_timing", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_kernel_class_count", C.c_int32, []),
@@ -320,6 +323,34 @@ class Context:
         if mz is not None and R > 1:
             mz = mz.reshape(steps, R)
         return tt.value, dd.value, mz
+
+    # fields of the per-step transport observables (mds_observe)
+    OBSERVABLES = ("t", "msd_x", "msd_y", "msd_z", "p_xy", "p_xz", "p_yz")
+
+    def observe(self, on=True):
+        """EvalTransCoeffs observables from now on, with the current
+        positions as the reference (see OBSERVABLES)."""
+        self._chk(self._L.mds_observe(self.h, 1 if on else 0), "observe")
+
+    def run_observed(self, steps, t, dt):
+        """run() that also returns <m_z> and the observables per step:
+        mz (steps,) and obs (steps, 7) for one replica, (steps, R) and
+        (steps, R, 7) for several."""
+        tt, dd = C.c_double(t), C.c_double(dt)
+        R = max(1, self.replica_count())
+        mz = np.zeros(steps * R)
+        obs = np.zeros(steps * R * 7)
+        self._chk(self._L.mds_run_observed(self.h, int(steps), C.byref(tt), C.byref(dd), _p(mz), _p(obs)),
+                  "run_observed")
+        if R > 1:
+            return tt.value, dd.value, mz.reshape(steps, R), obs.reshape(steps, R, 7)
+        return tt.value, dd.value, mz, obs.reshape(steps, 7)
+
+    def last_observables(self):
+        R = max(1, self.replica_count())
+        obs = np.zeros(R * 7)
+        self._chk(self._L.mds_last_observables(self.h, _p(obs)), "last_observables")
+        return obs.reshape(R, 7) if R > 1 else obs
 
     def synchronize(self):
         self._chk(self._L.mds_synchronize(self.h), "synchronize")

@@ -217,6 +217,10 @@ struct mds_ctx {
     DevBuf<double4> rec_pos, rec_mom;
     DevBuf<uint64_t> pair_acc;  // [12n] cell-sorted fixed-point pair sums
     DevBuf<int32_t> order_tmp;  // [n] cell-order scratch
+    // EvalTransCoeffs observables (mds_observe): unwrapped and reference
+    // positions, squared displacements, per-step series and last values
+    DevBuf<double> true_pos, pos_ref, obs_d2, obs_series, obs_last;
+    bool observe = false;
     DevBuf<float4> rec_pos32, rec_mom32;
     DevBuf<int32_t> rep_of;
     // replica table and per-replica device values
@@ -876,6 +880,18 @@ mds_status read_flags(mds_ctx* c) {
     return MDS_OK;
 }
 
+// The step's EvalTransCoeffs observables (src/postprocessing.cpp:167-187),
+// after translation and <m_z>, before the step counter advances.
+mds_status observables(mds_ctx* c) {
+    if (!c->observe) return MDS_OK;
+    HIPCHK(launch_observables(c->stream, c->reps.p, c->nrep, c->rep_of.p, c->red_chunks.p, (int32_t)c->h_chunks.size(),
+                              c->n, c->pos.p, c->true_pos.p, c->pos_ref.p, c->obs_d2.p, c->red_part.p, c->rep_t.p,
+                              c->work.p, c->nwork.p, c->cell_rep.p, c->press_partial.p,
+                              3 * (int64_t)c->work_cap * PAIR_SLICES, c->obs_last.p, c->series_counter.p,
+                              c->obs_series.p, kSeriesCap));
+    return MDS_OK;
+}
+
 // The device work of one step of size h after the force evaluation's inputs
 // are in place: noise, geometry, force pass, RK stages (one tableau pass, no
 // controller), translation, <m_z>, counters. Used for fixed-dt steps and for
@@ -919,6 +935,7 @@ mds_status fixed_step_body(mds_ctx* c, double h, bool check) {
                                 c->new_mm.p, c->new_ea.p, c->mm.p, c->ea.p));
         HIPCHK(launch_rep_mean_z(s, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), n, c->mm.p,
                                  c->red_part.p, c->mz_out.p, c->series_counter.p, c->series.p, kSeriesCap));
+        if ((st = observables(c)) != MDS_OK) return st;
         HIPCHK(launch_step_count(s, c->nrep, c->rep_step.p, c->rep_t.p, h, c->series_counter.p));
     }
     return MDS_OK;
@@ -1000,6 +1017,7 @@ mds_status adaptive_step(mds_ctx* c, double* t, double* dt, double* dt_used) {
                                 c->new_mm.p, c->new_ea.p, c->mm.p, c->ea.p));
         HIPCHK(launch_rep_mean_z(s, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), n, c->mm.p,
                                  c->red_part.p, c->mz_out.p, c->series_counter.p, c->series.p, kSeriesCap));
+        if ((st = observables(c)) != MDS_OK) return st;
         HIPCHK(launch_step_count(s, c->nrep, c->rep_step.p, c->rep_t.p, h, c->series_counter.p));
     }
     *t += h;
@@ -1076,6 +1094,7 @@ mds_status particle_props(mds_ctx* c, int64_t m, const double* r_mag, const doub
 // Rebuild the kept per-particle arrays for a new particle layout.
 mds_status relayout(mds_ctx* c, int64_t old_n, int64_t new_n, const std::vector<Seg>& segs) {
     hipStream_t s = c->stream;
+    c->observe = false;  // reference positions no longer match the layout
     const int64_t mn = c->reserve_n;
     DevBuf<double>* one[7] = {&c->r_mag, &c->mag_mom, &c->zeta_rot, &c->zeta_trans, &c->vol_mag, &c->c1, &c->c2};
     for (int k = 0; k < 7; ++k) HIPCHK(repack(*one[k], c->spare[k], 1, old_n, new_n, mn, segs, s));
@@ -1283,6 +1302,7 @@ mds_status mds_set_particles(mds_ctx* c, int64_t n, const double* r_mag, const d
     if (n > (int64_t)1 << 25) return fail(c, MDS_EINVAL, "at most 2^25 particles per context");
     HIPCHK(hipSetDevice(c->device));
     // a single replica: drop any previous ones
+    c->observe = false;
     const uint64_t seed = c->meta.empty() ? 0 : c->meta[0].seed;
     c->meta.clear();
     c->h_r_mag.clear();
@@ -1587,19 +1607,26 @@ mds_status mds_step(mds_ctx* c, double* t, double* dt, double* dt_used, double* 
     return MDS_OK;
 }
 
-mds_status mds_run(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz_series) {
+namespace {
+mds_status run_impl(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz_series, double* obs) {
     mds_status st = check_ready(c);
     if (st != MDS_OK) return st;
     if (!t || !dt || steps < 0) return MDS_EINVAL;
     if (steps == 0) return MDS_OK;
+    if (obs && !c->observe) return fail(c, MDS_EINVAL, "call mds_observe first");
     if (c->nrep == 1 && (st = set_rep_time(c, 0, *t)) != MDS_OK) return st;
     const int32_t R = c->nrep;
     if (!c->cfg.fixed_dt) {
         if (R != 1) return fail(c, MDS_EINVAL, "adaptive time stepping takes a single replica");
         for (int64_t k = 0; k < steps; ++k) {
             if ((st = adaptive_step(c, t, dt, nullptr)) != MDS_OK) return st;
-            if (mz_series) HIPCHK(hipMemcpy(mz_series + k, c->mz_out.p, sizeof(double), hipMemcpyDeviceToHost));
+            if (mz_series)
+                HIPCHK(hipMemcpyAsync(mz_series + k, c->mz_out.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            if (obs)
+                HIPCHK(hipMemcpyAsync(obs + 7 * k, c->obs_last.p, 7 * sizeof(double), hipMemcpyDeviceToHost,
+                                      c->stream));
         }
+        HIPCHK(hipStreamSynchronize(c->stream));
         return read_flags(c);
     }
     const double h = *dt;
@@ -1620,15 +1647,56 @@ mds_status mds_run(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz_
                 HIPCHK(hipGraphLaunch(c->gexec, c->stream));
             }
         }
-        if (mz_series) {
+        if (mz_series)
             HIPCHK(hipMemcpyAsync(mz_series + done * R, c->series.p, (size_t)m * R * sizeof(double),
                                   hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-        }
+        if (obs)
+            HIPCHK(hipMemcpyAsync(obs + done * R * 7, c->obs_series.p, (size_t)m * R * 7 * sizeof(double),
+                                  hipMemcpyDeviceToHost, c->stream));
+        if (mz_series || obs) HIPCHK(hipStreamSynchronize(c->stream));
         done += m;
     }
     for (int64_t k = 0; k < steps; ++k) *t += h;  // as tInt accumulates (src/simulation.cpp:518)
     return read_flags(c);
+}
+}  // namespace
+
+mds_status mds_run(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz_series) {
+    return run_impl(c, steps, t, dt, mz_series, nullptr);
+}
+
+mds_status mds_run_observed(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz_series, double* obs) {
+    if (!obs) return MDS_EINVAL;
+    return run_impl(c, steps, t, dt, mz_series, obs);
+}
+
+mds_status mds_observe(mds_ctx* c, int32_t on) {
+    mds_status st = check_ready(c);
+    if (st != MDS_OK) return st;
+    c->drop_graph();
+    c->observe = false;
+    if (!on) return MDS_OK;
+    if ((st = ensure_planned(c)) != MDS_OK) return st;
+    const size_t n3 = 3 * (size_t)c->n;
+    for (auto* b : {&c->true_pos, &c->pos_ref, &c->obs_d2}) HIPCHK(b->ensure(n3));
+    HIPCHK(c->obs_series.ensure((size_t)kSeriesCap * c->nrep * 7));
+    HIPCHK(c->obs_last.ensure((size_t)c->nrep * 7));
+    // positions at this call: the reference and the first true positions
+    // (src/simulation.cpp:108-109)
+    HIPCHK(hipMemcpyAsync(c->pos_ref.p, c->pos.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->true_pos.p, c->pos.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(launch_fill_f64(c->stream, c->obs_last.p, (int64_t)c->nrep * 7, 0.0));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->observe = true;
+    return MDS_OK;
+}
+
+mds_status mds_last_observables(mds_ctx* c, double* obs) {
+    if (!c || !obs) return MDS_EINVAL;
+    if (!c->observe) return fail(c, MDS_EINVAL, "call mds_observe first");
+    HIPCHK(hipMemcpyAsync(obs, c->obs_last.p, (size_t)c->nrep * 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MDS_OK;
 }
 
 mds_status mds_synchronize(mds_ctx* c) {
@@ -1826,7 +1894,8 @@ int64_t mds_device_bytes(const mds_ctx* c) {
     for (auto* d : {&c->r_mag, &c->mag_mom, &c->zeta_rot, &c->zeta_trans, &c->vol_mag, &c->c1, &c->c2, &c->pos, &c->mm,
                     &c->ea, &c->n_t, &c->n_b, &c->n_f, &c->pos0, &c->pos_cur, &c->force, &c->field, &c->mom, &c->kmm,
                     &c->kea, &c->pt[0], &c->pt[1], &c->new_mm, &c->new_ea, &c->ev_pos, &c->kpartial,
-                    &c->tab, &c->press_partial, &c->series, &c->msum, &c->mz_out, &c->rep_t, &c->err_partial})
+                    &c->tab, &c->press_partial, &c->series, &c->msum, &c->mz_out, &c->rep_t, &c->err_partial,
+                    &c->true_pos, &c->pos_ref, &c->obs_d2, &c->obs_series, &c->obs_last})
         add(d->n, sizeof(double));
     add(c->pair_acc.n + c->order_tmp.n / 2, sizeof(uint64_t));
     for (auto* d : {&c->rep_of, &c->cell_rep, &c->key_main, &c->key_fine, &c->cnt_main, &c->start_main, &c->end_main,

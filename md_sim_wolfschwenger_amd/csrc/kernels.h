@@ -168,6 +168,16 @@ hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, const
 hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
                         const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
                         int32_t moved_press_off, int64_t* cnt_pairs);
+// EvalTransCoeffs per step (src/postprocessing.cpp:167-187): unwrap the
+// true positions, per replica t, MSD x/y/z from pos_ref and the force pass's
+// Pxy, Pxz, Pyz over the box volume into obs[7 r ..] and the series at the
+// running step index. d2: [3n] scratch; part: the chunk sums buffer.
+hipError_t launch_observables(hipStream_t s, const Rep* reps, int32_t nrep, const int32_t* rep_of,
+                              const int2* chunks, int32_t nch, int64_t n, const double* pos, double* true_pos,
+                              const double* pos_ref, double* d2, double* part, const double* rep_t,
+                              const int32_t* work, const int32_t* nwork, const int32_t* cell_rep,
+                              const double* press_partial, int64_t moved_press_off, double* obs,
+                              const int32_t* counter, double* series, int32_t series_cap);
 // Within each cell of a sorted list, order the entries by value (original
 // index), so the pair sums' order does not depend on scheduling; tmp: [n].
 hipError_t launch_cell_order(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,

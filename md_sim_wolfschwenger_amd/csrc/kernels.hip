@@ -1770,6 +1770,79 @@ __global__ __launch_bounds__(256) void k_noise(int64_t n, const Rep* __restrict_
 }
 
 // Advance every replica's step counter and time by one accepted step h.
+// Transport observables of EvalTransCoeffs (src/postprocessing.cpp:167-187).
+// Positions unwrapped against the previous true positions (Rapaport's
+// nearest-image rule, rounding half away from zero as Eigen's round()) and
+// the squared displacement from the reference positions per axis, for the
+// chunked per-replica sums.
+__global__ void k_unwrap_d2(int64_t n, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
+                            const double* __restrict__ pos, double* __restrict__ true_pos,
+                            const double* __restrict__ pos_ref, double* __restrict__ d2) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double L = reps[rep_of[i]].L;
+    for (int d = 0; d < 3; ++d) {
+        const int64_t k = (int64_t)d * n + i;
+        const double x = pos[k];
+        const double tp = x + round((true_pos[k] - x) / L) * L;
+        true_pos[k] = tp;
+        const double e = tp - pos_ref[k];
+        d2[k] = e * e;
+    }
+}
+
+// Per replica (one block each): t before the step, MSD per axis (its chunk
+// sums over the particle count), and Pxy, Pxz, Pyz of the step's force pass
+// (its work items' slots in order, then the moved-pair slot) over the box
+// volume, into obs[r] and, at the running step index, the series.
+__global__ __launch_bounds__(256) void k_rep_obs(const Rep* __restrict__ reps, int32_t nrep,
+                                                 const double* __restrict__ part, const double* __restrict__ rep_t,
+                                                 const int32_t* __restrict__ work, const int32_t* __restrict__ nwork,
+                                                 const int32_t* __restrict__ cell_rep,
+                                                 const double* __restrict__ press_partial, int64_t moved_press_off,
+                                                 double* __restrict__ obs, const int32_t* __restrict__ counter,
+                                                 double* __restrict__ series, int32_t series_cap) {
+    const int r = blockIdx.x;
+    const Rep& R = reps[r];
+    __shared__ int32_t range[2];
+    if (threadIdx.x < 2) {
+        // first work item of replica r (threadIdx 0) or of r + 1 (threadIdx 1):
+        // work items are in cell order and cells in replica order
+        const int target = r + (int)threadIdx.x;
+        int lo = 0, hi = *nwork;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (cell_rep[work[mid] >> 8] < target) lo = mid + 1; else hi = mid;
+        }
+        range[threadIdx.x] = lo;
+    }
+    __syncthreads();
+    const int64_t s0 = (int64_t)range[0] * PAIR_SLICES, s1 = (int64_t)range[1] * PAIR_SLICES;
+    double a = 0, b = 0, c = 0;
+    for (int64_t k = s0 + threadIdx.x; k < s1; k += blockDim.x) {
+        a += press_partial[3 * k];
+        b += press_partial[3 * k + 1];
+        c += press_partial[3 * k + 2];
+    }
+    __shared__ double p3[3];
+    block_sum3<256>(a, b, c, p3);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double* pm = press_partial + moved_press_off + 3 * (int64_t)r;
+        const double vol = R.L * R.L * R.L;
+        double m[3] = {0, 0, 0};
+        for (int k = R.red_off; k < R.red_off + R.nred; ++k)
+            for (int d = 0; d < 3; ++d) m[d] += part[3 * (int64_t)k + d];
+        const double v[7] = {rep_t[r], m[0] / (double)R.n, m[1] / (double)R.n, m[2] / (double)R.n,
+                             (p3[0] + pm[0]) / vol, (p3[1] + pm[1]) / vol, (p3[2] + pm[2]) / vol};
+        const int32_t k = counter ? *counter : series_cap;
+        for (int f = 0; f < 7; ++f) {
+            obs[7 * r + f] = v[f];
+            if (series && k < series_cap) series[((int64_t)k * nrep + r) * 7 + f] = v[f];
+        }
+    }
+}
+
 __global__ void k_step_count(int32_t nrep, uint64_t* __restrict__ rep_step, double* __restrict__ rep_t, double h,
                              int32_t* __restrict__ series_counter) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1919,6 +1992,20 @@ hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, const
     hipLaunchKernelGGL(k_red_part, dim3(nch), dim3(256), 0, s, chunks, reps, n, v, 2, 1, part);
     hipLaunchKernelGGL(k_rep_mean_z, dim3((nrep + 63) / 64), dim3(64), 0, s, reps, nrep, (const double*)part, out,
                        counter, series, series_cap);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_observables(hipStream_t s, const Rep* reps, int32_t nrep, const int32_t* rep_of,
+                              const int2* chunks, int32_t nch, int64_t n, const double* pos, double* true_pos,
+                              const double* pos_ref, double* d2, double* part, const double* rep_t,
+                              const int32_t* work, const int32_t* nwork, const int32_t* cell_rep,
+                              const double* press_partial, int64_t moved_press_off, double* obs,
+                              const int32_t* counter, double* series, int32_t series_cap) {
+    hipLaunchKernelGGL(k_unwrap_d2, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, pos, true_pos, pos_ref,
+                       d2);
+    hipLaunchKernelGGL(k_red_part, dim3(nch), dim3(256), 0, s, chunks, reps, n, (const double*)d2, 0, 3, part);
+    hipLaunchKernelGGL(k_rep_obs, dim3(nrep), dim3(256), 0, s, reps, nrep, (const double*)part, rep_t, work, nwork,
+                       cell_rep, press_partial, moved_press_off, obs, counter, series, series_cap);
     LAUNCH_CHECK();
 }
 

@@ -179,3 +179,16 @@ class System:
         return {"pos": rowmajor(x, n), "mm": rowmajor(m, n), "ea": rowmajor(e, n), "t": tt[0], "dt": dd[0],
                 "dt_used": du[0], "mz": mz[0], "force": rowmajor(F, n), "field": rowmajor(B, n), "press": P,
                 "field_out": rowmajor(Bo, n), "n_t": rowmajor(nt, n), "n_b": rowmajor(nb, n), "status": st}
+
+
+def trans_coeffs(pos, true_pos, pos_ref, l_box, press):
+    """EvalTransCoeffs (src/postprocessing.cpp:167-187): unwrap the periodic
+    positions against the previous unwrapped ones (Rapaport's minimum-image
+    continuation, line 171; Eigen's ``round`` rounds halves away from zero),
+    the per-axis mean square displacement from the reference positions
+    (173-176) and the step's pressure-tensor sums divided by the box volume
+    (179-181). Returns (true_pos, msd[3], p[3])."""
+    q = (np.asarray(true_pos) - pos) / l_box
+    true_pos = pos + np.where(q >= 0, np.floor(q + 0.5), np.ceil(q - 0.5)) * l_box
+    d = true_pos - pos_ref
+    return true_pos, (d * d).mean(axis=0), np.asarray(press, dtype=np.float64) / l_box ** 3.0

@@ -22,8 +22,11 @@
 //                          (3 column-major N x 3 double arrays back to back)
 //
 // Output: <out_prefix>.idx (one "name rows cols" line per array) and
-//         <out_prefix>.bin (column-major little-endian doubles, concatenated).
+//         <out_prefix>.bin (column-major little-endian doubles, concatenated);
+//         the reference's own text outputs (data.txt, coords.txt, mz.txt,
+//         t.txt, dtMean.txt, P*.txt, MSD*.txt) in the working directory.
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -119,6 +122,12 @@ int main(int argc, char** argv) {
         }
     }
 
+    // Transport observables start from the state the loop starts from
+    // (src/simulation.cpp:107-112); data.txt goes to the working directory.
+    ov.posRef = wv.coordsTrans.pos;
+    ov.truePos = wv.coordsTrans.pos;
+    if (N >= 22) WriteData2TXT(&wv, &prm, "data.txt");
+
     Dump d(prefix);
     d.scal("N", N);
     d.scal("lBox", prm.lBox);
@@ -156,6 +165,7 @@ int main(int argc, char** argv) {
     double tInt = 0.0;
     double deltaT = config->getDeltaTInit();
     double deltaTused = 0.0;
+    int countCoords = 0;
     for (int s = 0; s < steps; ++s) {
         const std::string p = "s" + std::to_string(s) + "_";
         d.arr(p + "pos_in", wv.coordsTrans.pos);
@@ -195,8 +205,31 @@ int main(int argc, char** argv) {
         ApplyBoundaryCondArr(&wv.coordsTrans.pos, &buf.buffer3d, &prm);
         d.arr(p + "pos_out", wv.coordsTrans.pos);
         d.scal(p + "mz", wv.coordsRot.posMm.col(2).mean());
+
+        // RunSimulation's post-processing (src/simulation.cpp:490-516)
+        if (std::floor(tInt / config->getTEnd() * config->getNPCoords()) == countCoords) {
+            countCoords++;
+            WriteCoords2TXT(&wv, "coords.txt", tInt);
+        }
+        EvalTransCoeffs(&wv, &buf, &ov, &prm);
+        ov.t.push_back(tInt);
+        ov.mz.push_back(wv.coordsRot.posMm.col(2).mean());
+        {
+            double M[3] = {ov.MSDx.back(), ov.MSDy.back(), ov.MSDz.back()};
+            d.put(p + "msd", M, 3, 1);
+            double P[3] = {ov.Pxy, ov.Pxz, ov.Pyz};
+            d.put(p + "press_vol", P, 3, 1);
+        }
+        d.arr(p + "true_pos", ov.truePos);
         tInt += deltaTused;
     }
     d.scal("t_end", tInt);
+    // end-of-run exports (src/simulation.cpp:528-537)
+    if (steps > 0) {
+        WriteCoords2TXT(&wv, "coords.txt", tInt);
+        ExportDiffVis(&ov, tInt, steps);
+        saveData("mz.txt", Map<VectorXd, Unaligned>(ov.mz.data(), ov.mz.size()), 0);
+        saveData("t.txt", Map<VectorXd, Unaligned>(ov.t.data(), ov.t.size()), 0);
+    }
     return 0;
 }

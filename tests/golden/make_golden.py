@@ -10,8 +10,10 @@ Every array is the reference's own output: particle properties and initial
 state from ``InitParticleProps``/``InitCoordsFromSketch``
 (src/simulation.cpp:118-351), thermal draws from ``DefThermFluct`` (563-603),
 forces/fields/pressure from ``CompInteractions`` (628-680), rotation from
-``IntegrationRot`` (707-868) and positions from ``IntegrationTrans`` +
-``ApplyBoundaryCondArr`` (959-962, 610-620).
+``IntegrationRot`` (707-868), positions from ``IntegrationTrans`` +
+``ApplyBoundaryCondArr`` (959-962, 610-620), the transport observables of
+``EvalTransCoeffs`` (src/postprocessing.cpp:167-187) and the text files the
+reference writes (``saveData``, src/postprocessing.cpp:15-32), kept as bytes.
 
 Usage: python tests/golden/make_golden.py [case ...]
 """
@@ -138,6 +140,11 @@ CASES = {
 }
 
 
+# files the reference writes through saveData (src/postprocessing.cpp:15-32)
+TEXT_OUTPUTS = ("data.txt", "coords.txt", "mz.txt", "t.txt", "dtMean.txt", "Pxy.txt", "Pxz.txt", "Pyz.txt",
+                "MSDx.txt", "MSDy.txt", "MSDz.txt")
+
+
 def read_dump(prefix: str) -> dict:
     out = {}
     with open(prefix + ".idx") as f, open(prefix + ".bin", "rb") as b:
@@ -159,9 +166,11 @@ def run_case(name: str) -> str:
         args = [HARNESS, os.path.join(run, "out"), repr(alpha_l), repr(rc), str(nc), str(steps)]
         if state is not None:
             # first pass with no steps to learn the box and radii
-            subprocess.run([HARNESS, os.path.join(run, "probe"), repr(alpha_l), repr(rc), str(nc), "0"],
-                           cwd=run, check=True, stdout=subprocess.DEVNULL, timeout=120)
-            probe = read_dump(os.path.join(run, "probe"))
+            pdir = os.path.join(tmp, "probe")
+            os.makedirs(pdir)
+            subprocess.run([HARNESS, os.path.join(pdir, "probe"), repr(alpha_l), repr(rc), str(nc), "0"],
+                           cwd=pdir, check=True, stdout=subprocess.DEVNULL, timeout=120)
+            probe = read_dump(os.path.join(pdir, "probe"))
             n_dim = round(probe["N"][0] ** (1 / 3))
             r_mag = float(probe["rMag"].min())
             st = overlap_state(7 if state == "overlap" else 11, n_dim, float(probe["lBox"][0]), r_mag)
@@ -178,6 +187,12 @@ def run_case(name: str) -> str:
         # (src/simulation.cpp:917-927, 725); a case that does is a bad case
         subprocess.run(args, cwd=run, check=True, stdout=subprocess.DEVNULL, timeout=300)
         data = read_dump(os.path.join(run, "out"))
+        # the reference's text outputs, byte for byte
+        for fn in TEXT_OUTPUTS:
+            fp = os.path.join(run, fn)
+            if os.path.exists(fp):
+                with open(fp, "rb") as f:
+                    data["txt_" + fn[:-4]] = np.frombuffer(f.read(), dtype=np.uint8)
     data["ini"] = np.frombuffer(ini_text(over).encode(), dtype=np.uint8)
     data["steps"] = np.array([steps], dtype=np.float64)
     path = os.path.join(HERE, name + ".npz")

@@ -1,0 +1,119 @@
+"""Output files and transport observables against what the reference itself
+wrote (tests/golden/make_golden.py runs the reference's RunSimulation
+post-processing and keeps its text files as bytes).
+
+* The text writers reproduce the reference's files byte for byte from the
+  reference's own values: data.txt from the parameters, coords.txt (a block
+  after step 0, the progress point at t = 0, and one appended at the end),
+  t.txt, mz.txt, MSD*.txt, P*.txt and dtMean.txt.
+* The oracle's restatement of EvalTransCoeffs (unwrap, per-axis MSD, P / V)
+  reproduces the reference's unwrapped positions bit for bit and its MSD and
+  pressure series, replayed over the reference's own trajectory.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import md_sim_wolfschwenger_amd as mds
+from md_sim_wolfschwenger_amd import output
+from refcfg import CASES, load, rel_err
+
+
+def _steps(d):
+    return int(d["steps"][0])
+
+
+def _txt(d, name):
+    return bytes(d["txt_" + name]).decode()
+
+
+def _read(path):
+    with open(path, newline="") as f:
+        return f.read()
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_text_files_match_reference_bytes(case, tmp_path):
+    d = load(case)
+    n, k = int(d["N"][0]), _steps(d)
+    cfg = mds.config_from_string(d["ini_text"])
+    L = float(d["lBox"][0])
+    p = str(tmp_path)
+
+    output.write_data(os.path.join(p, "data.txt"), d["rHydr"], d["rMag"], L, cfg)
+    assert _read(os.path.join(p, "data.txt")) == _txt(d, "data"), "data.txt"
+
+    # progress point of step 0 (t = 0 replaces the file), final block appended
+    coords = os.path.join(p, "coords.txt")
+    with open(coords, "w") as f:
+        f.write("stale content\n")
+    output.write_coords(coords, d["s0_pos_out"], d["s0_mm_out"], d["s0_ea_out"], float(d["s0_t_in"][0]))
+    last = f"s{k - 1}_"
+    output.write_coords(coords, d[last + "pos_out"], d[last + "mm_out"], d[last + "ea_out"], float(d["t_end"][0]))
+    assert _read(coords) == _txt(d, "coords"), "coords.txt"
+
+    t = np.array([d[f"s{i}_t_in"][0] for i in range(k)])
+    mz = np.array([d[f"s{i}_mz"][0] for i in range(k)])
+    obs = np.array([[d[f"s{i}_t_in"][0], *d[f"s{i}_msd"], *d[f"s{i}_press_vol"]] for i in range(k)])
+    output.export_series(p, t, mz, obs, float(d["t_end"][0]), k)
+    for name in ("mz", "t", "dtMean", "Pxy", "Pxz", "Pyz", "MSDx", "MSDy", "MSDz"):
+        assert _read(os.path.join(p, name + ".txt")) == _txt(d, name), name
+
+
+def test_format_matrix_alignment_and_special_values():
+    # widths follow the widest entry; %g switches to exponent form at 1e-5 / 1e6
+    assert output.format_matrix([[1.0, -2.5], [1e-5, 123456789.0]]) == ("          1        -2.5\n"
+                                                                                  "      1e-05 1.23457e+08")
+    assert output.format_matrix(np.zeros(0)) == ""
+    assert output.format_matrix(np.array([[np.inf, -np.inf]])) == " inf -inf"
+    assert output.format_matrix([0.0, -0.0]) == " 0\n-0"
+
+
+def test_save_data_replaces_at_t0_and_appends_otherwise(tmp_path):
+    f = str(tmp_path / "x.txt")
+    output.save_data(f, [1.0], 0)
+    output.save_data(f, [2.0], 3e-9)
+    assert _read(f) == "1\n\n2\n\n"
+    output.save_data(f, [3.0], 0)
+    assert _read(f) == "3\n\n"
+    assert not output.save_data(str(tmp_path / "missing" / "y.txt"), [1.0], 0)
+
+
+def test_write_data_needs_22_rows(tmp_path):
+    cfg = mds.config_from_string(load("coating_n27")["ini_text"])
+    with pytest.raises(ValueError):
+        output.write_data(str(tmp_path / "data.txt"), np.ones(21), np.ones(21), 1.0, cfg)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_oracle_trans_coeffs_match_reference(case):
+    import oracle
+    d = load(case)
+    L = float(d["lBox"][0])
+    ref = d["s0_pos_in"]
+    true_pos = ref.copy()
+    for i in range(_steps(d)):
+        p = f"s{i}_"
+        true_pos, msd, pv = oracle.trans_coeffs(d[p + "pos_out"], true_pos, ref, L, d[p + "press"])
+        assert np.array_equal(true_pos, d[p + "true_pos"]), i
+        assert rel_err(msd, d[p + "msd"]) < 1e-13, i
+        assert np.array_equal(pv, d[p + "press_vol"]), i
+
+
+def test_oracle_trans_coeffs_unwraps_boundary_crossings():
+    """A particle leaving through a face re-enters on the other side; the
+    unwrapped track continues and the MSD sees the true displacement."""
+    import oracle
+    L = 1.0
+    ref = np.array([[0.49, 0.0, -0.49], [0.0, 0.2, 0.0]])
+    true_pos = ref.copy()
+    # step 1: +0.02 in x through +L/2 and -0.02 in z through -L/2
+    pos = np.array([[0.51 - L, 0.0, -0.51 + L], [0.0, 0.2, 0.0]])
+    true_pos, msd, _ = oracle.trans_coeffs(pos, true_pos, ref, L, np.zeros(3))
+    assert np.allclose(true_pos, [[0.51, 0.0, -0.51], [0.0, 0.2, 0.0]], atol=1e-15)
+    assert np.allclose(msd, [0.0002, 0.0, 0.0002], rtol=1e-9)
+    # step 2: back across the face
+    true_pos, msd, _ = oracle.trans_coeffs(ref.copy(), true_pos, ref, L, np.zeros(3))
+    assert np.allclose(true_pos, ref, atol=1e-15)
+    assert np.allclose(msd, 0.0, atol=1e-30)
