@@ -149,6 +149,17 @@ mds_status mds_init_positions(const mds_config* cfg, int64_t n, const double* r_
  * 2: on the easy axes, 3: easy axes and moments along z). */
 mds_status mds_init_orientations(const mds_config* cfg, int64_t n, uint64_t seed, double* pos_mm,
                                  double* pos_ea);
+/* READ_FROM_FILE: the state from a coordinates file (replaces
+ * InitCoordsFromSim, src/simulation.cpp:357-382, reading with openData's
+ * rules, src/postprocessing.cpp:37-75). The file is WriteCoords2TXT's N x 10
+ * text (pos | posMm | posEa | tInt), one block or several separated by blank
+ * lines; the last block is read into column-major N x 3 arrays, posMm from
+ * columns 3-5 and posEa from 6-8 (the reference reads 6-8 and 9-11, past the
+ * columns WriteCoords2TXT writes). t_int (may be NULL) gets the block's time.
+ * MDS_EINVAL for an unreadable file or a last block that is not N x 10. The
+ * text holds 6 significant digits. */
+mds_status mds_init_coords_from_sim(const char* path, int64_t n, double* pos, double* pos_mm, double* pos_ea,
+                                    double* t_int);
 
 /* ----------------------------------------------------------------------- */
 /* Device context                                                          */
@@ -244,6 +255,19 @@ mds_status mds_observe(mds_ctx* ctx, int32_t on);
 mds_status mds_run_observed(mds_ctx* ctx, int64_t steps, double* t, double* dt, double* mz_series, double* obs);
 /* The last step's observables, obs[replica][7] (after mds_step). */
 mds_status mds_last_observables(mds_ctx* ctx, double* obs);
+/* Checkpoint/resume (no counterpart in the reference, whose only restart is
+ * READ_FROM_FILE above). mds_save_checkpoint writes, to a binary file, every
+ * replica's particles, box, field, seed, Ewald parameters, clock and step
+ * counter (the Philox counter), the state, the cell tiling, the noise source
+ * and, with mds_observe on, the reference and unwrapped positions, with the
+ * caller's t and dt; all doubles exactly. mds_load_checkpoint replaces the
+ * context's replicas with the file's (the context must be created with the
+ * same precision, solver and fixedDt) and returns t and dt: a run continued
+ * from it with Philox noise is bit for bit the run that never stopped.
+ * Injected noise (mds_set_noise) is not part of a checkpoint. MDS_EINVAL for
+ * an unreadable, truncated or corrupted file (FNV-1a 64 checksum). */
+mds_status mds_save_checkpoint(mds_ctx* ctx, const char* path, double t, double dt);
+mds_status mds_load_checkpoint(mds_ctx* ctx, const char* path, double* t, double* dt);
 /* Wait for queued work. */
 mds_status mds_synchronize(mds_ctx* ctx);
 

@@ -79,6 +79,7 @@ SIGNATURES = [
     ("mds_init_radii", C.c_int, [C.POINTER(Config), C.c_int64, C.c_uint64, _dp]),
     ("mds_init_positions", C.c_int, [C.POINTER(Config), C.c_int64, _dp, C.c_double, C.c_uint64, C.c_int32, _dp]),
     ("mds_init_orientations", C.c_int, [C.POINTER(Config), C.c_int64, C.c_uint64, _dp, _dp]),
+    ("mds_init_coords_from_sim", C.c_int, [C.c_char_p, C.c_int64, _dp, _dp, _dp, _dp]),
     ("mds_create", C.c_int, [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]),
     ("mds_destroy", None, [C.c_void_p]),
     ("mds_last_error", C.c_char_p, [C.c_void_p]),
@@ -102,6 +103,8 @@ SIGNATURES = [
     ("mds_observe", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_run_observed", C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp]),
     ("mds_last_observables", C.c_int, [C.c_void_p, _dp]),
+    ("mds_save_checkpoint", C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double]),
+    ("mds_load_checkpoint", C.c_int, [C.c_void_p, C.c_char_p, _dp, _dp]),
     ("mds_synchronize", C.c_int, [C.c_void_p]),
     ("mds_enable_timing", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_kernel_class_count", C.c_int32, []),
@@ -203,6 +206,16 @@ def init_orientations(cfg: Config, n: int, seed: int):
     if st:
         raise MdsError(st, "init_orientations")
     return rm(mm, n), rm(ea, n)
+
+
+def init_coords_from_sim(path: str, n: int):
+    """READ_FROM_FILE: (pos, mm, ea, t) from the last N x 10 block of a
+    WriteCoords2TXT file (mds_init_coords_from_sim)."""
+    pos, mm, ea, t = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n), C.c_double()
+    st = lib().mds_init_coords_from_sim(os.fsencode(path), int(n), _p(pos), _p(mm), _p(ea), C.byref(t))
+    if st != 0:
+        raise MdsError(st, f"cannot read {n} x 10 coordinates from {path}")
+    return rm(pos, n), rm(mm, n), rm(ea, n), t.value
 
 
 def device_count() -> int:
@@ -351,6 +364,17 @@ class Context:
         obs = np.zeros(R * 7)
         self._chk(self._L.mds_last_observables(self.h, _p(obs)), "last_observables")
         return obs.reshape(R, 7) if R > 1 else obs
+
+    def save_checkpoint(self, path, t, dt):
+        """Binary checkpoint of the replicas, state, clocks, noise counters
+        and observables reference, with the caller's t and dt."""
+        self._chk(self._L.mds_save_checkpoint(self.h, os.fsencode(path), float(t), float(dt)), "save_checkpoint")
+
+    def load_checkpoint(self, path):
+        """Replace this context's replicas by a checkpoint's; returns (t, dt)."""
+        t, dt = C.c_double(), C.c_double()
+        self._chk(self._L.mds_load_checkpoint(self.h, os.fsencode(path), C.byref(t), C.byref(dt)), "load_checkpoint")
+        return t.value, dt.value
 
     def synchronize(self):
         self._chk(self._L.mds_synchronize(self.h), "synchronize")

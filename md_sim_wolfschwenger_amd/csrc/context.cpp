@@ -1293,6 +1293,19 @@ mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out) {
     return MDS_OK;
 }
 
+// Drop every replica (the buffers stay allocated for reuse).
+static void clear_replicas(mds_ctx* c) {
+    c->observe = false;
+    c->meta.clear();
+    c->h_r_mag.clear();
+    c->h_mag_mom.clear();
+    c->h_r_hydr.clear();
+    c->rep_t.release();
+    c->rep_step.release();
+    c->n = 0;
+    c->nrep = 0;
+}
+
 void mds_destroy(mds_ctx* c) { delete c; }
 
 const char* mds_last_error(const mds_ctx* c) { return c ? c->err.c_str() : "null context"; }
@@ -1302,16 +1315,8 @@ mds_status mds_set_particles(mds_ctx* c, int64_t n, const double* r_mag, const d
     if (n > (int64_t)1 << 25) return fail(c, MDS_EINVAL, "at most 2^25 particles per context");
     HIPCHK(hipSetDevice(c->device));
     // a single replica: drop any previous ones
-    c->observe = false;
     const uint64_t seed = c->meta.empty() ? 0 : c->meta[0].seed;
-    c->meta.clear();
-    c->h_r_mag.clear();
-    c->h_mag_mom.clear();
-    c->h_r_hydr.clear();
-    c->rep_t.release();
-    c->rep_step.release();
-    c->n = 0;
-    c->nrep = 0;
+    clear_replicas(c);
     const double lb = l_box;
     mds_status st = add_replicas(c, 1, &n, r_mag, r_hydr, &lb, nullptr, &seed, nullptr, nullptr, nullptr, nullptr);
     if (st != MDS_OK) return st;
@@ -1884,6 +1889,238 @@ mds_status mds_set_graph(mds_ctx* c, int32_t on) {
     if (!c) return MDS_EINVAL;
     c->use_graph = on != 0;
     c->drop_graph();
+    return MDS_OK;
+}
+
+// ---------------------------------------------------------------- checkpoint
+// A binary checkpoint holds what a later run needs to continue as if it had
+// never stopped: per replica its particles, box, field, seed, Ewald
+// parameters, contact margin, clock and step counter (the Philox counter),
+// the state, and with observables on the reference and unwrapped positions;
+// the cell tiling, the noise source, and the caller's t and dt. Doubles are
+// stored as the context holds them, so a run resumed with Philox noise
+// matches the uninterrupted run bit for bit.
+//
+// Layout (host byte order, little-endian on every ROCm host):
+//   "MDSCKPT1"  u32 version  i32 precision solver fixed_dt nrep reach philox
+//   observe  f64 t dt  i64 n_total
+//   nrep x { i64 n  f64 L field_z  u64 seed  i32 ewald_set nc  f64 alpha r_cut
+//            margin_scale t  u64 step }
+//   f64 r_mag[n_total] r_hydr[n_total] pos[3 n_total] pos_mm[..] pos_ea[..]
+//   (observe) f64 pos_ref[3 n_total] true_pos[3 n_total]      (column-major)
+//   u64 FNV-1a 64 of all preceding bytes
+}  // extern "C"
+
+namespace {
+
+constexpr char kCkptMagic[8] = {'M', 'D', 'S', 'C', 'K', 'P', 'T', '1'};
+constexpr uint32_t kCkptVersion = 1;
+
+uint64_t fnv1a64(const unsigned char* p, size_t n) {
+    uint64_t h = 14695981039346656037ull;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+struct CkptOut {
+    std::vector<unsigned char> b;
+    void raw(const void* p, size_t n) {
+        const unsigned char* q = static_cast<const unsigned char*>(p);
+        b.insert(b.end(), q, q + n);
+    }
+    template <typename T>
+    void put(T v) { raw(&v, sizeof(T)); }
+};
+
+struct CkptIn {
+    const unsigned char* p;
+    size_t n, at = 0;
+    bool ok = true;
+    void raw(void* dst, size_t k) {
+        if (!ok || at + k > n) {
+            ok = false;
+            return;
+        }
+        std::memcpy(dst, p + at, k);
+        at += k;
+    }
+    template <typename T>
+    T get() {
+        T v{};
+        raw(&v, sizeof(T));
+        return v;
+    }
+};
+
+struct CkptRep {
+    int64_t n;
+    double L, field_z;
+    uint64_t seed;
+    int32_t ewald_set, nc;
+    double alpha, rc, margin, t;
+    uint64_t step;
+};
+
+}  // namespace
+
+extern "C" {
+
+mds_status mds_save_checkpoint(mds_ctx* c, const char* path, double t, double dt) {
+    if (!c || !path) return MDS_EINVAL;
+    if (c->n <= 0 || !c->state_set) return fail(c, MDS_EINVAL, "nothing to save: particles and state not set");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t n = c->n;
+    const size_t n3 = 3 * (size_t)n;
+    std::vector<double> rt(c->nrep);
+    std::vector<uint64_t> rs(c->nrep);
+    HIPCHK(hipMemcpy(rt.data(), c->rep_t.p, c->nrep * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rs.data(), c->rep_step.p, c->nrep * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    CkptOut o;
+    o.raw(kCkptMagic, 8);
+    o.put<uint32_t>(kCkptVersion);
+    o.put<int32_t>(c->cfg.precision);
+    o.put<int32_t>(c->cfg.solver);
+    o.put<int32_t>(c->cfg.fixed_dt);
+    o.put<int32_t>(c->nrep);
+    o.put<int32_t>(c->reach);
+    o.put<int32_t>(c->philox ? 1 : 0);
+    o.put<int32_t>(c->observe ? 1 : 0);
+    o.put<double>(t);
+    o.put<double>(dt);
+    o.put<int64_t>(n);
+    for (int r = 0; r < c->nrep; ++r) {
+        const RepMeta& m = c->meta[r];
+        o.put<int64_t>(m.n);
+        o.put<double>(m.L);
+        o.put<double>(m.field_z);
+        o.put<uint64_t>(m.seed);
+        o.put<int32_t>(m.ewald_set ? 1 : 0);
+        o.put<int32_t>(m.nc);
+        o.put<double>(m.alpha);
+        o.put<double>(m.rc);
+        o.put<double>(m.margin_scale);
+        o.put<double>(rt[r]);
+        o.put<uint64_t>(rs[r]);
+    }
+    o.raw(c->h_r_mag.data(), n * sizeof(double));
+    o.raw(c->h_r_hydr.data(), n * sizeof(double));
+    std::vector<double> tmp(n3);
+    std::vector<const DevBuf<double>*> arrays = {&c->pos, &c->mm, &c->ea};
+    if (c->observe) {
+        arrays.push_back(&c->pos_ref);
+        arrays.push_back(&c->true_pos);
+    }
+    for (const DevBuf<double>* a : arrays) {
+        HIPCHK(hipMemcpy(tmp.data(), a->p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+        o.raw(tmp.data(), n3 * sizeof(double));
+    }
+    o.put<uint64_t>(fnv1a64(o.b.data(), o.b.size()));
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(c, MDS_EINVAL, std::string("cannot open ") + path + " for writing");
+    const size_t w = std::fwrite(o.b.data(), 1, o.b.size(), f);
+    const int cl = std::fclose(f);
+    if (w != o.b.size() || cl != 0) return fail(c, MDS_EINVAL, std::string("short write to ") + path);
+    return MDS_OK;
+}
+
+mds_status mds_load_checkpoint(mds_ctx* c, const char* path, double* t, double* dt) {
+    if (!c || !path) return MDS_EINVAL;
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(c, MDS_EINVAL, std::string("cannot open ") + path);
+    std::vector<unsigned char> b;
+    unsigned char chunk[1 << 16];
+    size_t k;
+    while ((k = std::fread(chunk, 1, sizeof(chunk), f)) > 0) b.insert(b.end(), chunk, chunk + k);
+    std::fclose(f);
+    if (b.size() < 8 + sizeof(uint64_t) || std::memcmp(b.data(), kCkptMagic, 8) != 0)
+        return fail(c, MDS_EINVAL, "not a checkpoint file");
+    uint64_t sum;
+    std::memcpy(&sum, b.data() + b.size() - sizeof(uint64_t), sizeof(uint64_t));
+    if (fnv1a64(b.data(), b.size() - sizeof(uint64_t)) != sum) return fail(c, MDS_EINVAL, "checkpoint checksum mismatch");
+    CkptIn in{b.data(), b.size() - sizeof(uint64_t)};
+    in.at = 8;
+    if (in.get<uint32_t>() != kCkptVersion) return fail(c, MDS_EINVAL, "unsupported checkpoint version");
+    const int32_t precision = in.get<int32_t>(), solver = in.get<int32_t>(), fixed = in.get<int32_t>();
+    if (precision != c->cfg.precision || solver != c->cfg.solver || fixed != c->cfg.fixed_dt)
+        return fail(c, MDS_EINVAL, "checkpoint written with another precision, solver or fixedDt setting");
+    const int32_t nrep = in.get<int32_t>(), reach = in.get<int32_t>();
+    const int32_t philox = in.get<int32_t>(), observe = in.get<int32_t>();
+    const double t0 = in.get<double>(), dt0 = in.get<double>();
+    const int64_t n = in.get<int64_t>();
+    if (!in.ok || nrep <= 0 || nrep > (1 << 20) || n <= 0 || n > ((int64_t)1 << 25) || reach < 0 || reach > 8)
+        return fail(c, MDS_EINVAL, "corrupt checkpoint header");
+    std::vector<CkptRep> reps(nrep);
+    int64_t total = 0;
+    for (CkptRep& r : reps) {
+        r.n = in.get<int64_t>();
+        r.L = in.get<double>();
+        r.field_z = in.get<double>();
+        r.seed = in.get<uint64_t>();
+        r.ewald_set = in.get<int32_t>();
+        r.nc = in.get<int32_t>();
+        r.alpha = in.get<double>();
+        r.rc = in.get<double>();
+        r.margin = in.get<double>();
+        r.t = in.get<double>();
+        r.step = in.get<uint64_t>();
+        if (r.n <= 0 || !(r.L > 0)) in.ok = false;
+        total += r.n;
+    }
+    const size_t n3 = 3 * (size_t)n;
+    const size_t need = (2 * (size_t)n + (observe ? 5 : 3) * n3) * sizeof(double);
+    if (!in.ok || total != n || in.n - in.at != need) return fail(c, MDS_EINVAL, "corrupt checkpoint body");
+    std::vector<double> r_mag(n), r_hydr(n), st(3 * n3), obs_pos;
+    in.raw(r_mag.data(), n * sizeof(double));
+    in.raw(r_hydr.data(), n * sizeof(double));
+    in.raw(st.data(), 3 * n3 * sizeof(double));
+    if (observe) {
+        obs_pos.resize(2 * n3);
+        in.raw(obs_pos.data(), 2 * n3 * sizeof(double));
+    }
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    clear_replicas(c);
+    std::vector<int64_t> ns(nrep);
+    std::vector<double> Ls(nrep), fz(nrep);
+    std::vector<uint64_t> seeds(nrep);
+    for (int r = 0; r < nrep; ++r) {
+        ns[r] = reps[r].n;
+        Ls[r] = reps[r].L;
+        fz[r] = reps[r].field_z;
+        seeds[r] = reps[r].seed;
+    }
+    mds_status s = add_replicas(c, nrep, ns.data(), r_mag.data(), r_hydr.data(), Ls.data(), fz.data(), seeds.data(),
+                                st.data(), st.data() + n3, st.data() + 2 * n3, nullptr);
+    if (s != MDS_OK) return s;
+    for (int r = 0; r < nrep; ++r) {
+        c->meta[r].margin_scale = reps[r].margin;
+        if (reps[r].ewald_set && (s = set_rep_ewald(c, r, reps[r].alpha, reps[r].rc, reps[r].nc)) != MDS_OK) return s;
+    }
+    std::vector<double> rt(nrep);
+    std::vector<uint64_t> rs(nrep);
+    for (int r = 0; r < nrep; ++r) {
+        rt[r] = reps[r].t;
+        rs[r] = reps[r].step;
+    }
+    HIPCHK(hipMemcpy(c->rep_t.p, rt.data(), nrep * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->rep_step.p, rs.data(), nrep * sizeof(uint64_t), hipMemcpyHostToDevice));
+    c->reach = reach;
+    c->planned = false;
+    c->geometry_ready = false;
+    c->philox = philox != 0;
+    c->state_set = true;
+    c->drop_graph();
+    if (observe) {
+        if ((s = mds_observe(c, 1)) != MDS_OK) return s;
+        HIPCHK(hipMemcpy(c->pos_ref.p, obs_pos.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->true_pos.p, obs_pos.data() + n3, n3 * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (t) *t = t0;
+    if (dt) *dt = dt0;
     return MDS_OK;
 }
 

@@ -13,6 +13,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "mdsim_capi.h"
@@ -190,6 +193,62 @@ mds_status mds_init_orientations(const mds_config* cfg, int64_t n, uint64_t seed
             pos_mm[k * n + i] = rot == 1 ? (k == 2 ? 1.0 : 0.0) : e[k];
         }
     }
+    return MDS_OK;
+}
+
+// InitCoordsFromSim (src/simulation.cpp:357-382) over openData's format
+// (src/postprocessing.cpp:37-75: whitespace-separated numbers, one matrix row
+// per line). The file is what WriteCoords2TXT writes (src/postprocessing.cpp:
+// 80-100: N x 10, pos | posMm | posEa | tInt in (0, 9)), one block or, as in
+// coords.txt, a block per progress point separated by blank lines; the last
+// block is read. The reference reads posMm from columns 6-8 and posEa from
+// 9-11, which a 10-column file does not have; here they come from 3-5 and
+// 6-8, where WriteCoords2TXT puts them.
+mds_status mds_init_coords_from_sim(const char* path, int64_t n, double* pos, double* pos_mm, double* pos_ea,
+                                    double* t_int) {
+    if (!path || n <= 0 || !pos || !pos_mm || !pos_ea) return MDS_EINVAL;
+    std::FILE* f = std::fopen(path, "r");
+    if (!f) return MDS_EINVAL;
+    std::vector<std::vector<double>> block, last;
+    std::string line;
+    bool bad = false;
+    auto end_block = [&]() {
+        if (!block.empty()) last.swap(block);
+        block.clear();
+    };
+    for (int ch = 0; !bad;) {
+        line.clear();
+        while ((ch = std::fgetc(f)) != EOF && ch != '\n') line.push_back((char)ch);
+        if (line.empty() && ch == EOF) break;
+        std::vector<double> row;
+        const char* q = line.c_str();
+        for (;;) {
+            while (*q == ' ' || *q == '\t' || *q == '\r') ++q;
+            if (!*q) break;
+            char* e = nullptr;
+            const double v = std::strtod(q, &e);
+            if (e == q) {
+                bad = true;
+                break;
+            }
+            row.push_back(v);
+            q = e;
+        }
+        if (row.empty()) end_block(); else block.push_back(std::move(row));
+        if (ch == EOF) break;
+    }
+    std::fclose(f);
+    end_block();
+    if (bad || (int64_t)last.size() != n) return MDS_EINVAL;
+    for (const auto& row : last)
+        if (row.size() != 10) return MDS_EINVAL;
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            pos[d * n + i] = last[i][d];
+            pos_mm[d * n + i] = last[i][3 + d];
+            pos_ea[d * n + i] = last[i][6 + d];
+        }
+    if (t_int) *t_int = last[0][9];
     return MDS_OK;
 }
 

@@ -117,3 +117,48 @@ def test_oracle_trans_coeffs_unwraps_boundary_crossings():
     true_pos, msd, _ = oracle.trans_coeffs(ref.copy(), true_pos, ref, L, np.zeros(3))
     assert np.allclose(true_pos, ref, atol=1e-15)
     assert np.allclose(msd, 0.0, atol=1e-30)
+
+
+def _as_printed(a):
+    """What strtod returns for each value as saveData prints it."""
+    return np.vectorize(lambda v: float("%.6g" % v))(a)
+
+
+def test_read_from_file_takes_last_block_of_reference_coords(tmp_path):
+    """mds_init_coords_from_sim on the coords.txt the reference wrote (two
+    blocks: step 0 and the end): the last block, posMm from columns 3-5 and
+    posEa from 6-8, every value as the text holds it."""
+    d = load("heun_n125")
+    n, k = int(d["N"][0]), _steps(d)
+    f = tmp_path / "coords.txt"
+    f.write_bytes(bytes(d["txt_coords"]))
+    pos, mm, ea, t = mds.init_coords_from_sim(str(f), n)
+    last = f"s{k - 1}_"
+    assert np.array_equal(pos, _as_printed(d[last + "pos_out"]))
+    assert np.array_equal(mm, _as_printed(d[last + "mm_out"]))
+    assert np.array_equal(ea, _as_printed(d[last + "ea_out"]))
+    assert t == float("%.6g" % d["t_end"][0])
+    assert rel_err(mm, d[last + "mm_out"]) < 5e-6
+
+
+def test_read_from_file_round_trip_and_errors(tmp_path):
+    rng = np.random.default_rng(4)
+    n = 30
+    pos, mm, ea = rng.standard_normal((3, n, 3)) * np.array([1e-7, 1.0, 1.0])[:, None, None]
+    f = str(tmp_path / "coords_final.txt")
+    output.write_coords(f, pos, mm, ea, 0)          # the reference's coords_final.txt: one block
+    p2, m2, e2, t = mds.init_coords_from_sim(f, n)
+    assert np.array_equal(p2, _as_printed(pos)) and np.array_equal(m2, _as_printed(mm))
+    assert np.array_equal(e2, _as_printed(ea)) and t == 0.0
+    with pytest.raises(mds.MdsError):
+        mds.init_coords_from_sim(f, n + 1)          # block of another size
+    with pytest.raises(mds.MdsError):
+        mds.init_coords_from_sim(str(tmp_path / "missing.txt"), n)
+    g = tmp_path / "nine.txt"
+    g.write_text(output.format_matrix(np.zeros((n, 9))) + "\n\n")
+    with pytest.raises(mds.MdsError):
+        mds.init_coords_from_sim(str(g), n)         # not WriteCoords2TXT's 10 columns
+    h = tmp_path / "junk.txt"
+    h.write_text(output.format_matrix(np.zeros((n, 10))).replace("0", "x", 1) + "\n")
+    with pytest.raises(mds.MdsError):
+        mds.init_coords_from_sim(str(h), n)
