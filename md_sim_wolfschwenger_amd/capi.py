@@ -374,6 +374,8 @@ class Context:
         """Replace this context's replicas by a checkpoint's; returns (t, dt)."""
         t, dt = C.c_double(), C.c_double()
         self._chk(self._L.mds_load_checkpoint(self.h, os.fsencode(path), C.byref(t), C.byref(dt)), "load_checkpoint")
+        # the particle count of the arrays this wrapper allocates for the state
+        self.n = sum(self.replica_info(r)["n"] for r in range(self.replica_count()))
         return t.value, dt.value
 
     def synchronize(self):

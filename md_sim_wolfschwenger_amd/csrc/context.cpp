@@ -538,6 +538,13 @@ void plan_ewald(mds_ctx* c, const RepMeta& m, Rep& R) {
         R.fx_s[q] = std::ldexp(1.0, e);
         R.fx_is[q] = std::ldexp(1.0, -e);
     }
+    // pressure of the pairs with a moved endpoint (r u F summed by atomics in
+    // any order): fixed point too, in units of the force unit times the box
+    // edge (SI), so that the sum does not depend on the order of the adds
+    const double ps = R.fx_s[1] * R.fscale * m.L;
+    const int ep = (std::isfinite(ps) && ps > 0) ? std::ilogb(ps) + 1 : 0;
+    R.px_s = std::ldexp(1.0, ep);
+    R.px_is = std::ldexp(1.0, -ep);
 }
 
 // Plan grids, stencils and k space of every replica, size the scratch
@@ -612,7 +619,9 @@ mds_status plan_all(mds_ctx* c) {
     HIPCHK(c->ev_key.ensure(2 * (size_t)c->cand_cap));
     HIPCHK(c->ev_pos.ensure(6 * (size_t)c->cand_cap));
     HIPCHK(c->moved_list.ensure(2 * (size_t)c->cand_cap));
-    HIPCHK(c->press_partial.ensure(3 * ((size_t)work_cap * PAIR_SLICES + c->nrep)));
+    // per-wavefront slots of the pair kernels, then 6 fixed-point words per
+    // replica for the moved pairs
+    HIPCHK(c->press_partial.ensure(3 * (size_t)work_cap * PAIR_SLICES + 6 * (size_t)c->nrep));
     HIPCHK(c->stencil.ensure(std::max<size_t>(1, c->h_sten.size())));
     HIPCHK(c->rows.ensure(std::max<size_t>(1, c->h_rows.size())));
     HIPCHK(c->segs.ensure(std::max<size_t>(1, c->h_segs.size())));
@@ -805,7 +814,7 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
     {
         mds_ctx::Span sp(c, force_pass ? KC_PAIRS_FORCE : KC_PAIRS_FIELD);
         int64_t* cnt = c->count_work ? (int64_t*)c->pair_counter.p : nullptr;
-        if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + moved_press, 3 * c->nrep, 0.0));
+        if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + moved_press, 6 * c->nrep, 0.0));
         HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p,
                             c->work_cap * PAIR_SLICES, cnt));
     }
@@ -1536,11 +1545,16 @@ mds_status mds_comp_interactions(mds_ctx* c, double* force, double* field, doubl
         press[0] = press[1] = press[2] = 0;
         for (size_t w = 0; w < used; ++w)
             for (int d = 0; d < 3; ++d) press[d] += pp[3 * w + d];
-        std::vector<double> pm(3 * (size_t)c->nrep);
-        HIPCHK(hipMemcpy(pm.data(), c->press_partial.p + 3 * slots, pm.size() * sizeof(double),
+        // moved pairs: two fixed-point words per component and replica
+        std::vector<long long> pm(6 * (size_t)c->nrep);
+        HIPCHK(hipMemcpy(pm.data(), c->press_partial.p + 3 * slots, pm.size() * sizeof(long long),
                          hipMemcpyDeviceToHost));
-        for (int32_t r = 0; r < c->nrep; ++r)
-            for (int d = 0; d < 3; ++d) press[d] += pm[3 * (size_t)r + d];
+        for (int32_t r = 0; r < c->nrep; ++r) {
+            const double u = c->h_rep[r].px_s;
+            for (int d = 0; d < 3; ++d)
+                press[d] += std::fma((double)pm[6 * (size_t)r + 2 * d], u,
+                                     (double)pm[6 * (size_t)r + 2 * d + 1] * (u * (1.0 / 1099511627776.0)));
+        }
     }
     return MDS_OK;
 }

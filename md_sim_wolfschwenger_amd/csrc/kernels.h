@@ -44,6 +44,7 @@ struct Rep {
     double alpha, alpha2, alpha4, rc, rc_sr;
     double ell, inv_ell, inv_mref, bscale, fscale, pscale;  // units of the FP32 pair path
     double fx_s[2], fx_is[2];    // fixed-point unit s1 of the pair sums (field, force) and 1/s1
+    double px_s, px_is;          // fixed-point unit of the moved pairs' pressure sums (SI) and 1/unit
     float al_s, al2_s, al4_s, pad_;
     int32_t nc, row_off, nrows, seg_off, nseg, k_off, nk, kchunk, nchunks, kb_off, nsegw, pad2_;
     int32_t pk_off, npk;         // slice of the packed scatter table

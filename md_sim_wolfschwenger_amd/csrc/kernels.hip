@@ -1225,10 +1225,12 @@ __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* _
         if (FORCE) {
             const double p0 = wave_sum(press[0]), p1 = wave_sum(press[1]), p2 = wave_sum(press[2]);
             if (lane == 0) {
-                const int64_t slot = 3 * ((int64_t)press_off + g.rep_of[p]);
-                atomicAdd(&out.press_partial[slot], p0);
-                atomicAdd(&out.press_partial[slot + 1], p1);
-                atomicAdd(&out.press_partial[slot + 2], p2);
+                // fixed-point words (hi, lo) per component: order-independent
+                unsigned long long* w =
+                    reinterpret_cast<unsigned long long*>(out.press_partial) + 3 * (int64_t)press_off + 6 * g.rep_of[p];
+                ok &= fx_add(w, w + 1, p0, R.px_s, R.px_is);
+                ok &= fx_add(w + 2, w + 3, p1, R.px_s, R.px_is);
+                ok &= fx_add(w + 4, w + 5, p2, R.px_s, R.px_is);
             }
         }
         if (cnt_pairs) {
@@ -1828,7 +1830,10 @@ __global__ __launch_bounds__(256) void k_rep_obs(const Rep* __restrict__ reps, i
     block_sum3<256>(a, b, c, p3);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const double* pm = press_partial + moved_press_off + 3 * (int64_t)r;
+        const unsigned long long* pw =
+            reinterpret_cast<const unsigned long long*>(press_partial + moved_press_off) + 6 * (int64_t)r;
+        const double pm[3] = {fx_value(pw[0], pw[1], R.px_s), fx_value(pw[2], pw[3], R.px_s),
+                              fx_value(pw[4], pw[5], R.px_s)};
         const double vol = R.L * R.L * R.L;
         double m[3] = {0, 0, 0};
         for (int k = R.red_off; k < R.red_off + R.nred; ++k)

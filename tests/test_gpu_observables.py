@@ -118,6 +118,7 @@ def test_observables_batch_equal_alone():
     seeds = [41, 42]
     steps, dt = 12, float(sys_[0]["cfg"].delta_t_init)
     ctx = batch(sys_, fields, seeds)
+    ctx.use_philox(0, 0)                    # each replica's own seed, counters at 0
     ctx.set_graph(True)
     ctx.observe(True)
     _, _, mz, obs = ctx.run_observed(steps, 0.0, dt)
@@ -147,3 +148,53 @@ def test_observe_requires_call_and_resets_reference():
     _, _, _, obs2 = ctx.run_observed(1, t, dt)
     assert np.all(obs2[0, 1:4] < obs[-1, 1:4])
     ctx.close()
+
+
+def test_run_driver_writes_reference_files(tmp_path):
+    """output.Run: RunSimulation's loop split at the progress points gives
+    the same series as one uninterrupted run, coords.txt holds one block per
+    progress point (labelled with the step's start time) plus the final one,
+    and every series file holds one value per step."""
+    from md_sim_wolfschwenger_amd import output
+    s = _face_system(seed=13, n=300)
+    cfg, r, rh, L = s[0], s[1], s[2], s[3]
+    dt = float(cfg.delta_t_init)
+    cfg.np_coords = 7.0
+    t_end = 30 * dt
+    ctx = _face_ctx(s, 17)
+    ctx.set_graph(True)
+    run = output.Run(ctx, cfg, str(tmp_path), r, rh, L, t_end=t_end, chunk=8)
+    run.run()
+    pos_end = ctx.get_state()
+    ctx.close()
+
+    # the reference's loop over the same clock
+    t, count, labels, steps = 0.0, 0, [], 0
+    while t <= t_end:
+        if np.floor(t / t_end * 7.0) == count:
+            count += 1
+            labels.append(t)
+        t += dt
+        steps += 1
+    assert run.steps == steps
+
+    ref = _face_ctx(s, 17)
+    ref.set_graph(True)
+    ref.observe(True)
+    _, _, mz, obs = ref.run_observed(steps, 0.0, dt)
+    assert np.array_equal(run.series("obs"), obs)
+    assert np.array_equal(run.series("mz"), mz)
+    ref.close()
+
+    blocks = (tmp_path / "coords.txt").read_text().split("\n\n")[:-1]
+    assert len(blocks) == len(labels) + 1
+    for b, lab in zip(blocks, labels + [t]):
+        rows = [list(map(float, ln.split())) for ln in b.strip("\n").split("\n")]
+        assert len(rows) == len(r) and all(len(x) == 10 for x in rows)
+        assert rows[0][9] == float("%.6g" % lab)
+    last = np.array([list(map(float, ln.split())) for ln in blocks[-1].strip("\n").split("\n")])
+    assert np.allclose(last[:, 0:3], pos_end[0], rtol=1e-5, atol=1e-5 * L)
+    for name in ("t", "mz", "MSDx", "MSDy", "MSDz", "Pxy", "Pxz", "Pyz"):
+        vals = (tmp_path / f"{name}.txt").read_text().split()
+        assert len(vals) == steps, name
+    assert (tmp_path / "data.txt").read_text().count("\n") == len(r) + 1
