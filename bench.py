@@ -57,15 +57,23 @@ def make_context(mds, workload, precision, rc_frac, seed, info=None):
 
 
 def time_path(ctx, steps, warmup):
+    """Wall time of `steps` graph-replayed steps (the fixed-dt run path, the
+    k-space build beside the pair kernels), then the per-kernel breakdown
+    from a second run of `steps` eager steps with HIP-event timing on the
+    context's stream (kernels in order, one stream)."""
     t, dt = 0.0, 1e-12
+    ctx.set_graph(True)
     t, dt, _ = ctx.run(warmup, t, dt)
-    ctx.synchronize()
-    ctx.enable_timing(True)
     ctx.synchronize()
     t0 = time.perf_counter()
     t, dt, _ = ctx.run(steps, t, dt)
     ctx.synchronize()
     t1 = time.perf_counter()
+    ctx.set_graph(False)
+    ctx.enable_timing(True)
+    ctx.synchronize()
+    t, dt, _ = ctx.run(steps, t, dt)
+    ctx.synchronize()
     stats = ctx.kernel_stats()
     counts = ctx.work_counts()
     ctx.enable_timing(False)

@@ -337,6 +337,11 @@ mds_status mds_reserve(mds_ctx* ctx, int64_t n_particles);
 /* With on != 0, mds_run (fixedDt) captures one step into a HIP graph and
  * replays it (re-captured after any change of replicas, parameters or dt). */
 mds_status mds_set_graph(mds_ctx* ctx, int32_t on);
+/* Build the k-space sums S(k) on a second stream of the context while the
+ * real-space pair kernels run (default on; off: one stream, in order).
+ * Results are identical either way; kernel timing (mds_enable_timing) runs
+ * them in order. No counterpart in the reference. */
+mds_status mds_set_kspace_overlap(mds_ctx* ctx, int32_t on);
 /* Restrict the context's work to the compute units set in mask (words x 32
  * bits, bit i = CU i; words = 0 restores all CUs). Contexts that serve
  * different request classes on one device can so be kept from queueing

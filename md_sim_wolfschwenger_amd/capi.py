@@ -126,6 +126,7 @@ SIGNATURES = [
     ("mds_set_cu_mask", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_int32]),
     ("mds_device_cus", C.c_int32, [C.c_int]),
     ("mds_set_graph", C.c_int, [C.c_void_p, C.c_int32]),
+    ("mds_set_kspace_overlap", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_device_bytes", C.c_int64, [C.c_void_p]),
 ]
 
@@ -480,6 +481,10 @@ class Context:
         for c in cus:
             m[c // 32] |= np.uint32(1 << (c % 32))
         self._chk(self._L.mds_set_cu_mask(self.h, m.ctypes.data_as(C.POINTER(C.c_uint32)), words), "set_cu_mask")
+
+    def set_kspace_overlap(self, on=True):
+        """S(k) on a second stream beside the pair kernels (default on)."""
+        self._chk(self._L.mds_set_kspace_overlap(self.h, 1 if on else 0), "set_kspace_overlap")
 
     def set_graph(self, on=True):
         self._chk(self._L.mds_set_graph(self.h, 1 if on else 0), "set_graph")

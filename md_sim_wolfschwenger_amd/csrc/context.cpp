@@ -192,6 +192,11 @@ struct mds_ctx {
     mds_config cfg{};
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream for the k-space build, which runs beside the pair
+    // kernels (same CU mask as `stream`); fork/join events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool overlap_k = true;
     std::string err;
 
     // replicas
@@ -325,6 +330,12 @@ struct mds_ctx {
             (void)hipEventDestroy(p.second.second);
         }
         for (auto e : pool) (void)hipEventDestroy(e);
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -811,29 +822,46 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
     PairOut po{force_pass ? c->force.p : nullptr, c->field.p, c->press_partial.p,
                (unsigned long long*)c->pair_acc.p, c->err_flag.p};
     const size_t moved_press = 3 * (size_t)c->work_cap * PAIR_SLICES;
-    {
-        mds_ctx::Span sp(c, force_pass ? KC_PAIRS_FORCE : KC_PAIRS_FIELD);
-        int64_t* cnt = c->count_work ? (int64_t*)c->pair_counter.p : nullptr;
-        if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + moved_press, 6 * c->nrep, 0.0));
-        HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p,
-                            c->work_cap * PAIR_SLICES, cnt));
-    }
-    if (c->nk_all > 0) {
-        {
-            mds_ctx::Span sp(c, KC_KBUILD);
-            HIPCHK(launch_kbuild(s, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p, c->kpartial.p));
+    // k space: S(k), its chunk reduction, then the scatter that writes the
+    // field (and force) with the self, surface and thermal-force terms; the
+    // pair sums are added to it last. The k-space chain needs only the tables
+    // and moments, so it runs on the side stream beside the pair kernels
+    // (in order on one stream when kernels are timed).
+    auto kspace = [&](hipStream_t ks) -> mds_status {
+        if (c->nk_all > 0) {
+            {
+                mds_ctx::Span sp(c, KC_KBUILD);
+                HIPCHK(launch_kbuild(ks, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p,
+                                     c->kpartial.p));
+            }
+            {
+                mds_ctx::Span sp(c, KC_KREDUCE);
+                HIPCHK(launch_kpack(ks, (int32_t)c->h_pack.size(), c->reps.p, c->nrep, c->pack_desc.p, c->kpartial.p,
+                                    c->pack.p));
+            }
         }
-        {
-            mds_ctx::Span sp(c, KC_KREDUCE);
-            HIPCHK(launch_kpack(s, (int32_t)c->h_pack.size(), c->reps.p, c->nrep, c->pack_desc.p, c->kpartial.p,
-                                c->pack.p));
-        }
-    }
-    {
         mds_ctx::Span sp(c, force_pass ? KC_KSCATTER_FORCE : KC_KSCATTER_FIELD);
-        HIPCHK(launch_kscatter(s, force_pass, n, c->reps.p, c->rep_of.p, c->rows.p, c->pack.p, c->tab.p,
-                               c->mom.p, c->msum.p, force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
+        HIPCHK(launch_kscatter(ks, force_pass, n, c->reps.p, c->rep_of.p, c->rows.p, c->pack.p, c->tab.p, c->mom.p,
+                               c->msum.p, force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
+        return MDS_OK;
+    };
+    const bool split = c->overlap_k && !c->timing;
+    mds_status st;
+    if (split) {
+        HIPCHK(hipEventRecord(c->ev_fork, s));
+        HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        if ((st = kspace(c->side)) != MDS_OK) return st;
+        HIPCHK(hipEventRecord(c->ev_join, c->side));
+    } else if ((st = kspace(s)) != MDS_OK) {
+        return st;
     }
+    mds_ctx::Span sp(c, force_pass ? KC_PAIRS_FORCE : KC_PAIRS_FIELD);
+    int64_t* cnt = c->count_work ? (int64_t*)c->pair_counter.p : nullptr;
+    if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + moved_press, 6 * c->nrep, 0.0));
+    HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p,
+                        c->work_cap * PAIR_SLICES, cnt));
+    if (split) HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
+    HIPCHK(launch_pairs_out(s, g, po));
     return MDS_OK;
 }
 
@@ -1282,7 +1310,10 @@ mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out) {
     mds_ctx* c = new mds_ctx();
     c->cfg = *cfg;
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return MDS_EDEVICE;
     }
@@ -1883,13 +1914,18 @@ mds_status mds_set_cu_mask(mds_ctx* c, const uint32_t* mask, int32_t words) {
     c->collect();
     HIPCHK(hipStreamSynchronize(c->stream));
     c->drop_graph();
-    hipStream_t s = nullptr;
-    if (words > 0)
-        HIPCHK(hipExtStreamCreateWithCUMask(&s, (uint32_t)words, mask));
-    else
-        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPCHK(hipStreamSynchronize(c->side));
+    hipStream_t s[2] = {nullptr, nullptr};
+    for (int k = 0; k < 2; ++k) {
+        if (words > 0)
+            HIPCHK(hipExtStreamCreateWithCUMask(&s[k], (uint32_t)words, mask));
+        else
+            HIPCHK(hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking));
+    }
     (void)hipStreamDestroy(c->stream);
-    c->stream = s;
+    (void)hipStreamDestroy(c->side);
+    c->stream = s[0];
+    c->side = s[1];
     return MDS_OK;
 }
 
@@ -1897,6 +1933,13 @@ int32_t mds_device_cus(int device) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
     return v;
+}
+
+mds_status mds_set_kspace_overlap(mds_ctx* c, int32_t on) {
+    if (!c) return MDS_EINVAL;
+    c->overlap_k = on != 0;
+    c->drop_graph();
+    return MDS_OK;
 }
 
 mds_status mds_set_graph(mds_ctx* c, int32_t on) {

@@ -163,12 +163,15 @@ hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, const
 
 // real space: one block per work item (max_work blocks, surplus blocks exit)
 // Real-space pair sums (force pass: force, field, pressure; else field):
-// the cell-tiled pair kernel, the pairs with an endpoint moved by the overlap
-// correction (their pressure to press_partial slot moved_press_off +
-// replica), then the fixed-point sums to out.force / out.field.
+// the cell-tiled pair kernel and the pairs with an endpoint moved by the
+// overlap correction (their pressure as fixed-point words at press_partial
+// slot moved_press_off, 6 words per replica), into the fixed-point sums.
+// launch_pairs_out then adds those sums to out.force / out.field, which the
+// k-space scatter has written (it writes, the pair sums are added).
 hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
                         const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
                         int32_t moved_press_off, int64_t* cnt_pairs);
+hipError_t launch_pairs_out(hipStream_t s, const Geometry& g, const PairOut& out);
 // EvalTransCoeffs per step (src/postprocessing.cpp:167-187): unwrap the
 // true positions, per replica t, MSD x/y/z from pos_ref and the force pass's
 // Pxy, Pxz, Pyz over the box volume into obs[7 r ..] and the series at the
@@ -192,6 +195,9 @@ hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep
 // S(k) partials -> packed scatter table (fixed summation order)
 hipError_t launch_kpack(hipStream_t s, int32_t npk_all, const Rep* reps, int32_t nrep, const KPack* desc,
                         const double* partial, double4* pack);
+// field (and force) of the k-space sum, self and surface terms (and the
+// thermal force add_force), written to field/force_out in original order;
+// launch_pairs_out adds the real-space sums afterwards
 hipError_t launch_kscatter(hipStream_t s, bool force, int64_t n, const Rep* reps, const int32_t* rep_of,
                            const KRow* rows, const double4* pack, const double* tab, const double* mom,
                            const double* msum, const double* add_force, double* force_out, double* field);

@@ -1098,15 +1098,22 @@ __global__ __launch_bounds__(256) void k_pairs_coat(Geometry g, const double4* _
 __global__ void k_pairs_out(int64_t n, const int32_t* __restrict__ sorted_idx, const int32_t* __restrict__ rep_of,
                             const Rep* __restrict__ reps, const unsigned long long* __restrict__ acc,
                             double* __restrict__ force, double* __restrict__ field) {
+    // the rounded pair sum is added to the k-space value (one rounding of the
+    // sum of the two, whichever was written first), so no contraction
+#pragma clang fp contract(off)
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int32_t o = sorted_idx[k];
     const Rep& R = reps[rep_of[o]];
-    for (int d = 0; d < 3; ++d)
-        field[d * n + o] = fx_value(acc[(6 + 2 * d) * n + k], acc[(7 + 2 * d) * n + k], R.fx_s[0]) * R.bscale;
+    for (int d = 0; d < 3; ++d) {
+        const double v = fx_value(acc[(6 + 2 * d) * n + k], acc[(7 + 2 * d) * n + k], R.fx_s[0]) * R.bscale;
+        field[d * n + o] += v;
+    }
     if (force)
-        for (int d = 0; d < 3; ++d)
-            force[d * n + o] = fx_value(acc[(2 * d) * n + k], acc[(2 * d + 1) * n + k], R.fx_s[1]) * R.fscale;
+        for (int d = 0; d < 3; ++d) {
+            const double v = fx_value(acc[(2 * d) * n + k], acc[(2 * d + 1) * n + k], R.fx_s[1]) * R.fscale;
+            force[d * n + o] += v;
+        }
 }
 
 // Within each cell, order the particle list by original index (k_place
@@ -1548,13 +1555,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const double* msum = msum_all + 3 * rid;
     const double mx = mom[i], my = mom[n + i], mz = mom[2 * n + i];
     const double cb = R.c_flux, cf = R.c_force, c_self = R.c_self, c_surf = R.c_surf;
-    field[i] += -cb * a[0] + c_self * mx - c_surf * msum[0];
-    field[n + i] += -cb * a[1] + c_self * my - c_surf * msum[1];
-    field[2 * n + i] += -cb * a[2] + c_self * mz - c_surf * msum[2];
+    // written here; the real-space pair sums are added after (k_pairs_out)
+    field[i] = -cb * a[0] + c_self * mx - c_surf * msum[0];
+    field[n + i] = -cb * a[1] + c_self * my - c_surf * msum[1];
+    field[2 * n + i] = -cb * a[2] + c_self * mz - c_surf * msum[2];
     if (FORCE) {
-        force[i] += cf * a[3] + (add_force ? add_force[i] : 0.0);
-        force[n + i] += cf * a[4] + (add_force ? add_force[n + i] : 0.0);
-        force[2 * n + i] += cf * a[5] + (add_force ? add_force[2 * n + i] : 0.0);
+        force[i] = cf * a[3] + (add_force ? add_force[i] : 0.0);
+        force[n + i] = cf * a[4] + (add_force ? add_force[n + i] : 0.0);
+        force[2 * n + i] = cf * a[5] + (add_force ? add_force[2 * n + i] : 0.0);
     }
 }
 
@@ -2050,7 +2058,12 @@ hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geom
     else
         hipLaunchKernelGGL((k_pairs_moved<false, false>), dim3(mblocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
                            moved_press_off, cp);
-    hipLaunchKernelGGL(k_pairs_out, dim3(grid_for(n, 256)), dim3(256), 0, s, n, g.sorted_idx, g.rep_of, g.reps,
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_pairs_out(hipStream_t s, const Geometry& g, const PairOut& out) {
+    if (g.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pairs_out, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.sorted_idx, g.rep_of, g.reps,
                        out.acc, out.force, out.field);
     LAUNCH_CHECK();
 }

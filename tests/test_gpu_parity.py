@@ -220,3 +220,22 @@ def test_mixed_precision_cell_list_vs_oracle():
     assert rel_err(B, Bo, b_al) < MIXED_TOL
     assert rel_err(F, Fo, f_al) < MIXED_TOL
     assert rel_err(X, Xo) < 1e-14
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_kspace_overlap_changes_nothing(graph):
+    """S(k) built on the second stream beside the pair kernels (the default)
+    gives the same bits as the one-stream order, eager and graph-replayed."""
+    d = load("heun_n1000")
+    states = []
+    for overlap in (True, False):
+        ctx = gpu_ctx(d, fixed=True)
+        ctx.set_state(d["s0_pos_in"], d["s0_mm_in"], d["s0_ea_in"])
+        ctx.use_philox(9, 0)
+        ctx.set_kspace_overlap(overlap)
+        ctx.set_graph(graph)
+        t, dt, mz = ctx.run(6, 0.0, 1e-12, want_mz=True)
+        states.append((*ctx.get_state(), mz))
+        ctx.close()
+    for a, b in zip(*states):
+        assert np.array_equal(a, b)
