@@ -349,6 +349,9 @@ def main():
     ap.add_argument("--reach", type=int, default=0, help="cell reach of the pair search; 0 = autotune")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="k-space chain in order on one stream also in the timed run (for profiles whose kernel "
+                         "durations match the per-kernel breakdown)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--replicas", type=int, default=256, help="replicas of the batched-ensemble leg (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
@@ -388,6 +391,7 @@ def main():
         if prec == "mixed":
             ctx.close()
             ctx, _ = make_context(mds, workload, "mixed", args.rc_frac, args.seed, info)
+        ctx.set_kspace_overlap(not args.no_overlap)
         tuned = tune(ctx, info, args)
         tiling = dict(ctx.tiling(), **ctx.params())
         nk = ctx.work_counts()["kvecs"]
