@@ -95,7 +95,7 @@ class Server:
 
     def __init__(self, precision: str = "fp64", hbm_budget: float = 64 * 2 ** 30, chunk_seconds: float = 0.25,
                  rc_scale: float = 0.225, dt: float = 1e-12, graph: bool = True, base: str = "c1", device: int = 0,
-                 budget: Budget | None = None, reserve_particles: int = 0):
+                 budget: Budget | None = None, reserve_particles: int = 0, kspace_overlap: bool = True):
         self.cfg = capi.config_from_string(workload.config_text(base, precision=precision))
         self.precision = precision
         self.hbm_budget = hbm_budget
@@ -105,6 +105,7 @@ class Server:
         self.dt = dt
         self.ctx = capi.Context(self.cfg, device)
         self.ctx.set_tiling(0)
+        self.ctx.set_kspace_overlap(kspace_overlap)
         self.graph = graph
         self.ctx.set_graph(graph)
         if reserve_particles > 0:
@@ -277,6 +278,11 @@ class LaneServer:
         # each lane reserves room for a few of its largest requests, so
         # admissions and retirements reuse device memory
         tops = list(bounds) + [2e5]
+        # one stream per lane: the lanes already run side by side, and a
+        # second (k-space) stream per lane would exceed the four hardware
+        # queues a process gets, so that streams of different lanes would
+        # share a queue and wait on each other
+        kw.setdefault("kspace_overlap", False)
         self.lanes = [Server(precision=precision, hbm_budget=hbm_budget, chunk_seconds=cs, budget=self.budget,
                              reserve_particles=int(4 * top), device=device, **kw)
                       for cs, top in zip(chunk_seconds, tops)]
