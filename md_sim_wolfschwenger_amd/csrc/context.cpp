@@ -1917,10 +1917,12 @@ mds_status mds_set_cu_mask(mds_ctx* c, const uint32_t* mask, int32_t words) {
     HIPCHK(hipStreamSynchronize(c->side));
     hipStream_t s[2] = {nullptr, nullptr};
     for (int k = 0; k < 2; ++k) {
-        if (words > 0)
-            HIPCHK(hipExtStreamCreateWithCUMask(&s[k], (uint32_t)words, mask));
-        else
-            HIPCHK(hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking));
+        const hipError_t e = words > 0 ? hipExtStreamCreateWithCUMask(&s[k], (uint32_t)words, mask)
+                                       : hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            if (k == 1) (void)hipStreamDestroy(s[0]);
+            return fail(c, MDS_EDEVICE, std::string("stream creation: ") + hipGetErrorString(e));
+        }
     }
     (void)hipStreamDestroy(c->stream);
     (void)hipStreamDestroy(c->side);
