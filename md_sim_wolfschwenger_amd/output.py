@@ -123,13 +123,17 @@ class Run:
 
     Steps between progress points run as one Context.run_observed call (a
     replayed graph for fixed dt); a call never crosses a progress point or
-    t_end: its length is bounded by the largest step the controller can take
-    (delta_t_max, or dt itself at fixed dt). One replica per context."""
+    t_end: at fixed dt its length follows from dt; with the adaptive
+    controller, whose step may grow five-fold per step and is not held below
+    deltaTmax (src/simulation.cpp:935-950), from that growth bound. One
+    replica per context."""
 
     def __init__(self, ctx, cfg, directory: str, r_mag, r_hydr, l_box: float, t_end: float | None = None,
                  chunk: int = 1000):
         self.ctx, self.cfg, self.dir = ctx, cfg, directory
         self.t_end = float(cfg.t_end if t_end is None else t_end)
+        if not self.t_end > 0:
+            raise ValueError("t_end must be positive")
         self.chunk = int(chunk)
         self.t, self.dt = 0.0, float(cfg.delta_t_init)
         self.steps, self.count = 0, 0
@@ -146,15 +150,21 @@ class Run:
     def _free_steps(self) -> int:
         """Steps that surely start before the next progress point and at or
         before t_end (one step of margin against rounding)."""
-        h = float(self.cfg.delta_t_init) if self.cfg.fixed_dt else float(self.cfg.delta_t_max)
-        h = max(h, self.dt)
         limit = self.t_end
         level = math.floor(self.t / self.t_end * self.cfg.np_coords)
         if level == self.count:
             return 0
         if level < self.count:
             limit = min(limit, self.count * self.t_end / self.cfg.np_coords)
-        return max(0, min(self.chunk, int((limit - self.t) / h) - 1))
+        gap = limit - self.t
+        if gap <= 0 or self.dt <= 0:
+            return 0
+        if self.cfg.fixed_dt:
+            k = int(gap / self.dt) - 1
+        else:
+            # k steps advance at most dt (1 + 5 + ... + 5^(k-1)) = dt (5^k - 1) / 4
+            k = int(math.log(4.0 * gap / self.dt + 1.0) / math.log(5.0)) - 1
+        return max(0, min(self.chunk, k))
 
     def _advance(self, k: int) -> None:
         t, dt, mz, obs = self.ctx.run_observed(k, self.t, self.dt)

@@ -162,3 +162,73 @@ def test_read_from_file_round_trip_and_errors(tmp_path):
     h.write_text(output.format_matrix(np.zeros((n, 10))).replace("0", "x", 1) + "\n")
     with pytest.raises(mds.MdsError):
         mds.init_coords_from_sim(str(h), n)
+
+
+class _FakeContext:
+    """Stands in for a device context in the driver's control logic: steps
+    advance the clock by dt and (adaptive) change dt by a seeded factor in
+    [0.2, 5], the controller's bounds (src/simulation.cpp:935-950); the state
+    records the number of steps taken."""
+
+    def __init__(self, n, adaptive, seed):
+        self.n, self.adaptive = n, adaptive
+        self.rng = np.random.default_rng(seed)
+        self.steps = 0
+
+    def replica_count(self):
+        return 1
+
+    def observe(self, on=True):
+        pass
+
+    def run_observed(self, k, t, dt):
+        obs, mz = np.zeros((k, 7)), np.zeros(k)
+        for i in range(k):
+            obs[i, 0] = t
+            obs[i, 1:] = self.steps
+            mz[i] = self.steps
+            t += dt
+            if self.adaptive:
+                dt *= float(self.rng.uniform(0.2, 5.0))
+            self.steps += 1
+        return t, dt, mz, obs
+
+    def get_state(self):
+        a = np.full((self.n, 3), float(self.steps))
+        return a, a, a
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_run_driver_writes_coords_where_the_reference_loop_does(adaptive, tmp_path):
+    """output.Run over a stand-in context: coords.txt blocks, their time
+    labels and the step after which each is taken match RunSimulation's loop
+    (src/simulation.cpp:464-519) replayed over the same sequence of dt."""
+    d = load("heun_n125")
+    cfg = mds.config_from_string(d["ini_text"])
+    cfg.fixed_dt = 0 if adaptive else 1
+    cfg.np_coords = 13.0
+    cfg.delta_t_init = 1e-12
+    t_end = 400e-12 if not adaptive else 3e-9
+    n = 30
+    run = output.Run(_FakeContext(n, adaptive, 5), cfg, str(tmp_path), np.ones(n), np.ones(n), 1.0, t_end=t_end,
+                     chunk=64)
+    run.run()
+    # the reference loop over the same dt sequence
+    ref = _FakeContext(n, adaptive, 5)
+    t, dt, count, expect = 0.0, 1e-12, 0, []
+    while t <= t_end:
+        due = np.floor(t / t_end * 13.0) == count
+        t0 = t
+        t, dt, _, _ = ref.run_observed(1, t, dt)
+        if due:
+            count += 1
+            expect.append((t0, ref.steps))
+    expect.append((t, ref.steps))
+    assert run.steps == ref.steps
+    blocks = (tmp_path / "coords.txt").read_text().split("\n\n")[:-1]
+    assert len(blocks) == len(expect) and len(expect) > 3
+    for b, (lab, steps) in zip(blocks, expect):
+        rows = [list(map(float, ln.split())) for ln in b.strip("\n").split("\n")]
+        assert rows[0][9] == float("%.6g" % lab)
+        assert rows[0][0] == float("%.6g" % steps)
+    assert len((tmp_path / "t.txt").read_text().split()) == ref.steps
