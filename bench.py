@@ -365,6 +365,8 @@ def main():
     ap.add_argument("--mz-precision", default="mixed", choices=("fp64", "mixed"))
     ap.add_argument("--mz-resume", default="", help="state file of an earlier --mz-run to continue from")
     ap.add_argument("--mz-out", default="gpurun_out/mz_run.npz")
+    ap.add_argument("--peak", action="store_true",
+                    help="measure the device's FP64 / FP32 vector FMA rate and copy bandwidth, print one JSON line")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run device-to-device copies of this size (profiler counter calibration)")
     args = ap.parse_args()
@@ -377,6 +379,14 @@ def main():
         return
     if args.mz_run > 0:
         mz_run(args)
+        return
+    if args.peak:
+        from md_sim_wolfschwenger_amd import capi
+        res = capi.peak_probe(0)
+        res.update(cus=capi.lib().mds_device_cus(0), datasheet={"fp64_vector_tflops": PEAK_FP64_TFLOPS,
+                                                               "fp32_packed_tflops": PEAK_FP32_TFLOPS,
+                                                               "hbm_gbs": 8000.0})
+        print(json.dumps(res))
         return
     import md_sim_wolfschwenger_amd as mds
     from md_sim_wolfschwenger_amd import workload

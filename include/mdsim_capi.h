@@ -349,6 +349,12 @@ mds_status mds_set_kspace_overlap(mds_ctx* ctx, int32_t on);
 mds_status mds_set_cu_mask(mds_ctx* ctx, const uint32_t* mask, int32_t words);
 /* Compute units of HIP device `device` (0 when unknown). */
 int32_t mds_device_cus(int device);
+/* Measured peaks of a device (no counterpart in the reference; for the
+ * roofline): vector FMA throughput in FP64 and FP32 (8 independent chains
+ * per lane, 8 waves per SIMD; packed FP32 is disabled in this build, so the
+ * FP32 figure is the unpacked rate) and the device-to-device copy rate of a
+ * 1 GiB buffer, bytes read plus written. Any output may be NULL. */
+mds_status mds_peak_probe(int device, double* fp64_tflops, double* fp32_tflops, double* copy_gbs);
 /* Device memory held by the context, in bytes. */
 int64_t mds_device_bytes(const mds_ctx* ctx);
 

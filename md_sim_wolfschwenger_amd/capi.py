@@ -125,6 +125,7 @@ SIGNATURES = [
     ("mds_reserve", C.c_int, [C.c_void_p, C.c_int64]),
     ("mds_set_cu_mask", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_int32]),
     ("mds_device_cus", C.c_int32, [C.c_int]),
+    ("mds_peak_probe", C.c_int, [C.c_int, _dp, _dp, _dp]),
     ("mds_set_graph", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_set_kspace_overlap", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_device_bytes", C.c_int64, [C.c_void_p]),
@@ -217,6 +218,15 @@ def init_coords_from_sim(path: str, n: int):
     if st != 0:
         raise MdsError(st, f"cannot read {n} x 10 coordinates from {path}")
     return rm(pos, n), rm(mm, n), rm(ea, n), t.value
+
+
+def peak_probe(device: int = 0) -> dict:
+    """Measured FP64 / FP32 vector FMA TFLOP/s and device copy GB/s."""
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    st = lib().mds_peak_probe(int(device), C.byref(a), C.byref(b), C.byref(c))
+    if st != 0:
+        raise MdsError(st, "peak probe")
+    return {"fp64_vector_tflops": a.value, "fp32_vector_tflops": b.value, "copy_gbs": c.value}
 
 
 def device_count() -> int:

@@ -1931,6 +1931,56 @@ mds_status mds_set_cu_mask(mds_ctx* c, const uint32_t* mask, int32_t words) {
     return MDS_OK;
 }
 
+mds_status mds_peak_probe(int device, double* fp64_tflops, double* fp32_tflops, double* copy_gbs) {
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || device < 0 || device >= cnt) return MDS_EDEVICE;
+    if (hipSetDevice(device) != hipSuccess) return MDS_EDEVICE;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        return MDS_EDEVICE;
+    const int32_t blocks = 8 * cus, iters = 4096;  // 8 waves per SIMD
+    const size_t copy_bytes = (size_t)1 << 30;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    void *out = nullptr, *src = nullptr, *dst = nullptr;
+    mds_status st = MDS_OK;
+    auto timed = [&](auto&& launch, int reps, float* ms) -> bool {
+        if (launch() != hipSuccess) return false;  // warm-up
+        if (hipEventRecord(e0, s) != hipSuccess) return false;
+        for (int r = 0; r < reps; ++r)
+            if (launch() != hipSuccess) return false;
+        return hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+               hipEventElapsedTime(ms, e0, e1) == hipSuccess;
+    };
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipMalloc(&out, (size_t)blocks * 256 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&src, copy_bytes) != hipSuccess || hipMalloc(&dst, copy_bytes) != hipSuccess) {
+        st = MDS_ENOMEM;
+    } else {
+        const double flop = 2.0 * 8.0 * iters * 256.0 * blocks;
+        float ms = 0;
+        for (int fp64 = 1; fp64 >= 0 && st == MDS_OK; --fp64) {
+            if (!timed([&] { return launch_peak_fma(s, fp64 != 0, blocks, iters, out); }, 5, &ms)) st = MDS_EDEVICE;
+            double* dst_v = fp64 ? fp64_tflops : fp32_tflops;
+            if (st == MDS_OK && dst_v) *dst_v = 5.0 * flop / (ms * 1e-3) / 1e12;
+        }
+        if (st == MDS_OK && hipMemsetAsync(src, 0, copy_bytes, s) != hipSuccess) st = MDS_EDEVICE;
+        if (st == MDS_OK &&
+            !timed([&] { return hipMemcpyAsync(dst, src, copy_bytes, hipMemcpyDeviceToDevice, s); }, 5, &ms))
+            st = MDS_EDEVICE;
+        // bytes read plus bytes written
+        if (st == MDS_OK && copy_gbs) *copy_gbs = 5.0 * 2.0 * (double)copy_bytes / (ms * 1e-3) / 1e9;
+    }
+    if (s) (void)hipStreamSynchronize(s);
+    if (out) (void)hipFree(out);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    return st;
+}
+
 int32_t mds_device_cus(int device) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;

@@ -116,6 +116,9 @@ struct PairParams {
 
 // Launchers: each launches its kernel(s) on stream s and returns the launch status.
 hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v);
+// peak probe: blocks x 256 threads, each `iters` x 8 FMAs; out holds one
+// value per thread (double or float)
+hipError_t launch_peak_fma(hipStream_t s, bool fp64, int32_t blocks, int32_t iters, void* out);
 hipError_t launch_fill_f64(hipStream_t s, double* p, int64_t count, double v);
 hipError_t launch_scale(hipStream_t s, double* p, int64_t count, double f);
 

@@ -1890,6 +1890,33 @@ hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v) 
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, p, count, v);
     LAUNCH_CHECK();
 }
+// ------------------------------------------------------------ peak probe
+// Vector FMA issue rate: eight independent chains per lane (issue, not
+// latency, limits the rate), the sum stored so that nothing is dead code.
+// Packed FP32 is disabled for every compile, so the FP32 probe measures the
+// unpacked rate.
+template <typename T>
+__global__ __launch_bounds__(256) void k_peak_fma(int32_t iters, T a, T b, T* __restrict__ out) {
+    T x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = (T)(threadIdx.x + k);
+    for (int32_t it = 0; it < iters; ++it)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = fma(x[k], a, b);
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sum += x[k];
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
+hipError_t launch_peak_fma(hipStream_t s, bool fp64, int32_t blocks, int32_t iters, void* out) {
+    if (fp64)
+        hipLaunchKernelGGL(k_peak_fma<double>, dim3(blocks), dim3(256), 0, s, iters, 0.999999, 1e-6, (double*)out);
+    else
+        hipLaunchKernelGGL(k_peak_fma<float>, dim3(blocks), dim3(256), 0, s, iters, 0.999999f, 1e-6f, (float*)out);
+    LAUNCH_CHECK();
+}
+
 hipError_t launch_fill_f64(hipStream_t s, double* p, int64_t count, double v) {
     hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, p, count, v);
     LAUNCH_CHECK();
