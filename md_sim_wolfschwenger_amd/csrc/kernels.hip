@@ -857,15 +857,20 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
                 for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[d][il], -(double)F[d]);
             }
         }
-        for (int d = 0; d < 3; ++d) {
-            const float b = quad_sum(valid ? Bj[d] : 0.0f);
-            if (leader) atomicAdd(&W.jacc[FB + d][js], (double)b);
-        }
+        // j's quad sums: every component's reduction first (their DPP steps
+        // interleave), then the leader's adds under one exec mask
+        float b[3], f[3];
+        for (int d = 0; d < 3; ++d) b[d] = valid ? Bj[d] : 0.0f;
         if (FORCE)
-            for (int d = 0; d < 3; ++d) {
-                const float f = quad_sum(valid ? F[d] : 0.0f);
-                if (leader) atomicAdd(&W.jacc[d][js], (double)f);
-            }
+            for (int d = 0; d < 3; ++d) f[d] = valid ? F[d] : 0.0f;
+        for (int d = 0; d < 3; ++d) b[d] = quad_sum(b[d]);
+        if (FORCE)
+            for (int d = 0; d < 3; ++d) f[d] = quad_sum(f[d]);
+        if (leader) {
+            for (int d = 0; d < 3; ++d) atomicAdd(&W.jacc[FB + d][js], (double)b[d]);
+            if (FORCE)
+                for (int d = 0; d < 3; ++d) atomicAdd(&W.jacc[d][js], (double)f[d]);
+        }
     } else {
         // canonical lo -> hi orientation of CompInteractVecs (src/interaction.cpp:99-104)
         const double L = g.L, iL = g.inv_L;
