@@ -364,7 +364,7 @@ def main():
                          "<m_z> series (the long-run mixed-vs-FP64 magnetisation check)")
     ap.add_argument("--mz-precision", default="mixed", choices=("fp64", "mixed"))
     ap.add_argument("--mz-resume", default="", help="state file of an earlier --mz-run to continue from")
-    ap.add_argument("--mz-out", default="gpurun_out/mz_run.npz")
+    ap.add_argument("--mz-out", default="mz_run.npz")
     ap.add_argument("--peak", action="store_true",
                     help="measure the device's FP64 / FP32 vector FMA rate and copy bandwidth, print one JSON line")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
