@@ -156,8 +156,9 @@ def _p(a):
 
 
 def cm(a) -> np.ndarray:
-    """(N,3) -> contiguous column-major flat buffer."""
-    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).T).reshape(-1)
+    """(N,3) -> a new contiguous column-major flat buffer (a copy also for
+    N = 1, where the transpose would already be contiguous)."""
+    return np.array(np.asarray(a, dtype=np.float64).T, order="C", copy=True).reshape(-1)
 
 
 def rm(flat, n) -> np.ndarray:

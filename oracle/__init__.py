@@ -72,8 +72,12 @@ def _p(a):
 
 
 def colmajor(a: np.ndarray) -> np.ndarray:
-    """(N,3) array -> contiguous column-major buffer as a flat (3N,) array."""
-    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).T).reshape(-1)
+    """(N,3) array -> a new contiguous column-major buffer as a flat (3N,)
+    array. Always a copy: the C side writes into the state buffers, and for
+    N = 1 the transpose of a row-major array is already contiguous, so a view
+    would hand the caller's own array (and, for the same array passed twice,
+    one buffer for two arguments) to the C side."""
+    return np.array(np.asarray(a, dtype=np.float64).T, order="C", copy=True).reshape(-1)
 
 
 def rowmajor(flat: np.ndarray, n: int) -> np.ndarray:

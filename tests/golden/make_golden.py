@@ -137,6 +137,9 @@ CASES = {
                                                "ENABLE_THERM_FORCE": "false"}}, 9.0, 0.4, 10, 1, None),
     # C1 configuration at its full size: N=1000, phi=1 %
     "heun_n1000": ({"particle_properties": {"npDim": 10}}, 5.85, 0.5, 5, 2, None),
+    # a single particle with its periodic images, field on, mobile easy axis
+    "single_n1": ({"particle_properties": {"npDim": 1}, "external_magnetic_field_t": {"magFluxDens": 0.02},
+                   "time_settings": {"tMag": 1e-6}}, 6.0, 0.45, 6, 4, None),
 }
 
 

@@ -108,4 +108,4 @@ def contact_scales(d: dict) -> tuple:
 
 
 CASES = ["heun_n125", "heun_n125_nonoise", "overlap_n64", "bs_lognormal_n125", "dp_immobile_n64",
-         "heun_reject_n64", "coating_n27", "grid_sc_n216", "heun_n1000"]
+         "heun_reject_n64", "coating_n27", "grid_sc_n216", "heun_n1000", "single_n1"]

@@ -239,3 +239,57 @@ def test_kspace_overlap_changes_nothing(graph):
         ctx.close()
     for a, b in zip(*states):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5])
+def test_tiny_systems_vs_oracle(n):
+    """One, two and five particles (no pair within r_c or two): the k-space
+    self-interaction through the periodic images, the empty pair lists and
+    one adaptive step, against the oracle. Ewald parameters are set directly
+    (the error model has no root for a single particle)."""
+    import oracle
+    from refcfg import derived, parse_ini
+    print(f"tiny n={n}", flush=True)
+    text = workload.config_text("c2", seed=3, noise=True, fixed=False, n=n).replace("volFrac = 0.05",
+                                                                                   "volFrac = 0.02")
+    cfg = mds.config_from_string(text)
+    rng = np.random.default_rng(n)
+    r, rh = workload.radii(cfg, n, rng)
+    L = workload.box_length(cfg, rh)
+    pos = workload.place(rh, L, rng)
+    ea = workload.orientations(n, rng, cfg.my_pi)
+    alpha, rc, nc = 6.0 / L, 0.45 * L, 6
+    ctx = mds.Context(cfg)
+    ctx.set_particles(r, rh, L)
+    ctx.set_ewald(alpha, rc, nc)
+    ctx.set_state(pos, ea, ea)
+    z = np.zeros((n, 3))
+    nT, nB, nF = (1e-3 * rng.standard_normal((n, 3)) for _ in range(3))
+    ctx.set_noise(z, z, z)
+    F, B, P, X = ctx.comp_interactions()
+    c = derived(parse_ini(text))
+    c["vis"] = cfg.vis
+    mu = cfg.sat_mag * 4 / 3 * cfg.my_pi * r ** 3
+    props = {"r_mag": r, "r_hydr": rh, "vol_mag": 4 / 3 * cfg.my_pi * r ** 3, "mag_mom": mu,
+             "zeta_rot": 8 * cfg.my_pi * cfg.vis * rh ** 3, "zeta_trans": 6 * cfg.my_pi * cfg.vis * rh}
+    s = oracle.System(n, L, alpha, rc, 2 * r.max(), nc, props, c, threads=1)
+    Fo, Bo, Po, Xo = s.comp_interactions(pos, ea, z)
+    m = mu.min()
+    b_al = cfg.mu0 * m / (2 * cfg.my_pi * (2 * rh.min()) ** 3)
+    f_al = 3 * cfg.mu0 * m * m / (2 * cfg.my_pi * (2 * rh.min()) ** 4)
+    assert rel_err(X, Xo) < 1e-14
+    assert rel_err(B, Bo, b_al) < TOL
+    assert rel_err(F, Fo, f_al) < TOL
+    assert rel_err(P, Po, f_al * 2 * rh.min()) < 1e-9
+    # draws of the size DefThermFluct gives at this temperature and dt
+    nT, nB, nF = nT * 2e-14, nB * 50.0, nF * 1.4e-6
+    ctx.set_state(pos, ea, ea)
+    ctx.set_noise(nT, nB, nF)
+    t, dt, du, mz = ctx.step(0.0, float(cfg.delta_t_init))
+    out = s.step(pos, ea, ea, 0.0, float(cfg.delta_t_init), nT, nB, nF)
+    pos1, mm1, ea1 = ctx.get_state()
+    assert du == pytest.approx(out["dt_used"], rel=1e-12)
+    assert rel_err(mm1, out["mm"]) < TOL and rel_err(ea1, out["ea"]) < TOL
+    assert rel_err(pos1, out["pos"]) < 1e-13
+    assert mz == pytest.approx(out["mz"], rel=1e-11, abs=1e-13)
+    ctx.close()
