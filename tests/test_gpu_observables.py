@@ -26,7 +26,8 @@ def gpu_ctx(d):
     return ctx
 
 
-@pytest.mark.parametrize("case", ["heun_n125", "overlap_n64", "bs_lognormal_n125", "heun_n1000"])
+@pytest.mark.parametrize("case", ["heun_n125", "overlap_n64", "bs_lognormal_n125", "heun_n1000", "dp_immobile_n64",
+                                  "coating_n27", "heun_reject_n64", "single_n1"])
 def test_observables_vs_reference(case):
     d = load(case)
     print(f"case {case}", flush=True)
