@@ -126,10 +126,15 @@ class Run:
     t_end: at fixed dt its length follows from dt; with the adaptive
     controller, whose step may grow five-fold per step and is not held below
     deltaTmax (src/simulation.cpp:935-950), from that growth bound. One
-    replica per context."""
+    replica per context.
+
+    A run can stop (``run(max_steps=...)``), be saved (``checkpoint``: the
+    context's binary checkpoint plus the loop's own state, ``<path>.run.npz``)
+    and continue in another process (``Run.resume``); the files it then
+    writes are those of the run that never stopped."""
 
     def __init__(self, ctx, cfg, directory: str, r_mag, r_hydr, l_box: float, t_end: float | None = None,
-                 chunk: int = 1000):
+                 chunk: int = 1000, _resume: bool = False):
         self.ctx, self.cfg, self.dir = ctx, cfg, directory
         self.t_end = float(cfg.t_end if t_end is None else t_end)
         if not self.t_end > 0:
@@ -141,8 +146,29 @@ class Run:
         os.makedirs(directory, exist_ok=True)
         if ctx.replica_count() > 1:
             raise ValueError("one replica per context")
-        write_data(os.path.join(directory, "data.txt"), r_hydr, r_mag, l_box, cfg)
-        ctx.observe(True)
+        if not _resume:
+            write_data(os.path.join(directory, "data.txt"), r_hydr, r_mag, l_box, cfg)
+            ctx.observe(True)
+
+    def checkpoint(self, path: str) -> None:
+        """The context's checkpoint at `path` and the loop's state beside it."""
+        self.ctx.save_checkpoint(path, self.t, self.dt)
+        np.savez(path + ".run.npz", t=self.t, dt=self.dt, steps=self.steps, count=self.count, t_end=self.t_end,
+                 chunk=self.chunk, series_t=self.series("t"), series_mz=self.series("mz"),
+                 series_obs=self.series("obs"))
+
+    @classmethod
+    def resume(cls, ctx, cfg, directory: str, path: str) -> "Run":
+        """Continue a checkpointed run in `directory` (whose data.txt and
+        coords.txt the first part wrote) on a context created from `cfg`."""
+        z = np.load(path + ".run.npz")
+        ctx.load_checkpoint(path)
+        run = cls(ctx, cfg, directory, None, None, 0.0, t_end=float(z["t_end"]), chunk=int(z["chunk"]),
+                  _resume=True)
+        run.t, run.dt = float(z["t"]), float(z["dt"])
+        run.steps, run.count = int(z["steps"]), int(z["count"])
+        run._t, run._mz, run._obs = [z["series_t"]], [z["series_mz"]], [z["series_obs"].reshape(-1, 7)]
+        return run
 
     def _progress_due(self) -> bool:
         return math.floor(self.t / self.t_end * self.cfg.np_coords) == self.count
@@ -174,10 +200,17 @@ class Run:
         self.t, self.dt = t, dt
         self.steps += k
 
-    def run(self) -> None:
+    def run(self, max_steps: int | None = None) -> bool:
+        """Step to the end and write the end-of-run files; with max_steps,
+        stop once that many steps are done in total (no end files). Returns
+        whether the run is finished."""
         coords = os.path.join(self.dir, "coords.txt")
         while self.t <= self.t_end:
+            if max_steps is not None and self.steps >= max_steps:
+                return False
             k = self._free_steps()
+            if max_steps is not None:
+                k = min(k, max_steps - self.steps)
             if k > 0:
                 self._advance(k)
                 continue
@@ -190,6 +223,7 @@ class Run:
         pos, mm, ea = self.ctx.get_state()
         write_coords(coords, pos, mm, ea, self.t)
         export_series(self.dir, self.series("t"), self.series("mz"), self.series("obs"), self.t, self.steps)
+        return True
 
     def series(self, name: str) -> np.ndarray:
         parts = {"t": self._t, "mz": self._mz, "obs": self._obs}[name]

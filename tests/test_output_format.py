@@ -177,8 +177,20 @@ class _FakeContext:
 
     def __init__(self, n, adaptive, seed):
         self.n, self.adaptive = n, adaptive
-        self.rng = np.random.default_rng(seed)
+        self.seed = seed
         self.steps = 0
+
+    def _factor(self):
+        # a fixed pseudo-random factor in [0.2, 5] per step index
+        return 0.2 + 4.8 * ((self.steps * 0.6180339887498949 + self.seed * 0.1) % 1.0)
+
+    def save_checkpoint(self, path, t, dt):
+        np.savez(path, steps=self.steps, t=t, dt=dt)
+
+    def load_checkpoint(self, path):
+        z = np.load(path if path.endswith(".npz") else path + ".npz")
+        self.steps = int(z["steps"])
+        return float(z["t"]), float(z["dt"])
 
     def replica_count(self):
         return 1
@@ -194,7 +206,7 @@ class _FakeContext:
             mz[i] = self.steps
             t += dt
             if self.adaptive:
-                dt *= float(self.rng.uniform(0.2, 5.0))
+                dt *= self._factor()
             self.steps += 1
         return t, dt, mz, obs
 
@@ -237,3 +249,33 @@ def test_run_driver_writes_coords_where_the_reference_loop_does(adaptive, tmp_pa
         assert rows[0][9] == float("%.6g" % lab)
         assert rows[0][0] == float("%.6g" % steps)
     assert len((tmp_path / "t.txt").read_text().split()) == ref.steps
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_run_driver_resumed_writes_the_same_files(adaptive, tmp_path):
+    """A run stopped after some steps, checkpointed and resumed on a fresh
+    context writes the same bytes as the run that never stopped."""
+    d = load("heun_n125")
+    cfg = mds.config_from_string(d["ini_text"])
+    cfg.fixed_dt = 0 if adaptive else 1
+    cfg.np_coords = 11.0
+    cfg.delta_t_init = 1e-12
+    t_end = 300e-12 if not adaptive else 3e-9
+    n = 25
+    a, b = tmp_path / "a", tmp_path / "b"
+    whole = output.Run(_FakeContext(n, adaptive, 3), cfg, str(a), np.ones(n), np.ones(n), 1.0, t_end=t_end,
+                       chunk=16)
+    whole.run()
+    half = whole.steps // 2
+    assert half > 5
+    first = output.Run(_FakeContext(n, adaptive, 3), cfg, str(b), np.ones(n), np.ones(n), 1.0, t_end=t_end,
+                       chunk=16)
+    assert not first.run(max_steps=half)
+    ck = str(tmp_path / "run.ckpt")
+    first.checkpoint(ck)
+    second = output.Run.resume(_FakeContext(n, adaptive, 3), cfg, str(b), ck)
+    assert second.steps == half
+    assert second.run()
+    assert second.steps == whole.steps
+    for name in output.TEXT_OUTPUTS:
+        assert (a / name).read_bytes() == (b / name).read_bytes(), name
