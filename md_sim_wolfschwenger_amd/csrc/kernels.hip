@@ -871,8 +871,47 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
             if (FORCE)
                 for (int d = 0; d < 3; ++d) atomicAdd(&W.jacc[d][js], (double)f[d]);
         }
+    } else if constexpr (!COATING) {
+        // FP64 without coating: u from i to j. The reference orients every
+        // pair from the lower to the higher index (src/interaction.cpp:99-104);
+        // the separation and each term here are odd or even in u, so the other
+        // orientation changes them only by a sign (and by rounding where a sum
+        // of two products is contracted the other way round), and no select
+        // on the orientation is needed.
+        const double L = g.L, iL = g.inv_L;
+        const double vx = min_image_r(pj.x - pi.x, L, iL);
+        const double vy = min_image_r(pj.y - pi.y, L, iL);
+        const double vz = min_image_r(pj.z - pi.z, L, iL);
+        const double r2 = vx * vx + vy * vy + vz * vz;
+        const double ir = rsqrt64(r2);
+        const double r = r2 * ir;
+        valid = valid && r < pp.rc;
+        const double ux = vx * ir, uy = vy * ir, uz = vz * ir;
+        double Bi[3], Bj[3], F[3];  // field on i, on j; force on j
+        pair_terms<FORCE, MIXED>(pp, r, ir, ux, uy, uz, mi, mj, Bi, Bj, F);
+        if (valid) {
+            ++npairs;
+            for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[FB + d][il], Bi[d]);
+            if (FORCE) {
+                const double rr = MIXED ? r * pp.inv_ell : r;
+                press[0] += rr * ux * F[1];
+                press[1] += rr * ux * F[2];
+                press[2] += rr * uy * F[2];
+                for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[d][il], -F[d]);
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            const double b = quad_sum(valid ? Bj[d] : 0.0);
+            if (leader) atomicAdd(&W.jacc[FB + d][js], b);
+        }
+        if (FORCE)
+            for (int d = 0; d < 3; ++d) {
+                const double f = quad_sum(valid ? F[d] : 0.0);
+                if (leader) atomicAdd(&W.jacc[d][js], f);
+            }
     } else {
-        // canonical lo -> hi orientation of CompInteractVecs (src/interaction.cpp:99-104)
+        // canonical lo -> hi orientation of CompInteractVecs (src/interaction.cpp:99-104):
+        // the short-range terms take the lo particle's radius first
         const double L = g.L, iL = g.inv_L;
         const double vx = min_image_r(i_lo ? pj.x - pi.x : pi.x - pj.x, L, iL);
         const double vy = min_image_r(i_lo ? pj.y - pi.y : pi.y - pj.y, L, iL);
