@@ -677,26 +677,30 @@ __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, doubl
                                            const double4& mi, const double4& mj, double B[3], double Bo[3],
                                            double F[3]) {
     if (!MIXED) {
+        // B = a1, C = a2, D = a3 of the reference with the products regrouped
+        // so that r^2 and the uniform constants are formed once
         const double ir2 = ir * ir;
+        const double r2 = r * r;
         const double x = pp.alpha * r;
         const double gau = exp_neg(x * x);
-        const double t = 2.0 * pp.alpha * gau * pp.ir_pi * ir2;
+        const double t = (2.0 * pp.alpha * pp.ir_pi) * (gau * ir2);
         const double a1 = gau * erfcx64(x) * (ir2 * ir) + t;
         const double a2 = 3.0 * a1 * ir2 + 2.0 * pp.alpha2 * t;
         const double dj = mj.x * ux + mj.y * uy + mj.z * uz;
         const double di = mi.x * ux + mi.y * uy + mi.z * uz;
-        const double g2 = a2 * dj * r * r;
+        const double a2r2 = a2 * r2;
+        const double g2 = a2r2 * dj;
         B[0] = g2 * ux - a1 * mj.x;
         B[1] = g2 * uy - a1 * mj.y;
         B[2] = g2 * uz - a1 * mj.z;
-        const double h2 = a2 * di * r * r;
+        const double h2 = a2r2 * di;
         Bo[0] = h2 * ux - a1 * mi.x;
         Bo[1] = h2 * uy - a1 * mi.y;
         Bo[2] = h2 * uz - a1 * mi.z;
         if (FORCE) {
             const double a3 = 5.0 * a2 * ir2 + 4.0 * pp.alpha4 * t;
             const double mimj = mi.x * mj.x + mi.y * mj.y + mi.z * mj.z;
-            const double c0 = (a2 * mimj - a3 * di * dj * r * r) * r;
+            const double c0 = (a2 * mimj - (a3 * r2) * (di * dj)) * r;
             const double c1 = a2 * r;
             F[0] = c0 * ux + c1 * (dj * mi.x + di * mj.x);
             F[1] = c0 * uy + c1 * (dj * mi.y + di * mj.y);
