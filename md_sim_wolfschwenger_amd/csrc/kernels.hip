@@ -1126,9 +1126,11 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     if (!ok) atomicOr(out.err, 64);
 }
 
-// Five blocks per CU (the LDS allows five).
+// Five blocks per CU (the LDS allows five). The FP64 field pass is held to
+// 5 waves per SIMD (89 VGPRs without spills, from 111): 5 % faster alone and
+// more room for the k-space chain beside it; the FP64 force pass would spill.
 template <bool FORCE, bool MIXED>
-__global__ __launch_bounds__(256) void k_pairs(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FORCE || MIXED ? 1 : 5))) void k_pairs(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
                                                PairOut out, const int4* __restrict__ stencil_all,
                                                unsigned long long* __restrict__ cnt_pairs) {
     pairs_block<FORCE, false, MIXED>(g, rec_mom, pp, out, stencil_all, cnt_pairs);
