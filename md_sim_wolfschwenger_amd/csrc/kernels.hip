@@ -281,8 +281,12 @@ __global__ void k_contacts(int64_t n, const Rep* __restrict__ reps, const int32_
                            const int32_t* __restrict__ fsorted, int32_t* __restrict__ cand_count,
                            const int32_t* __restrict__ cand_off, int32_t* __restrict__ cand_i,
                            int32_t* __restrict__ cand_j, int32_t cap, int32_t* __restrict__ err) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    // threads walk the particles in fine-cell order, so a wavefront's
+    // particles share their neighbour cells (the candidate lists stay indexed
+    // by the original index)
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = fsorted[k];
     const Rep& R = reps[rep_of[i]];
     const double L = R.L, fcell = R.fcell, extra = R.margin;
     const int32_t fncell = R.fncell, foff = R.fcell_off;
