@@ -293,3 +293,25 @@ def test_tiny_systems_vs_oracle(n):
     assert rel_err(pos1, out["pos"]) < 1e-13
     assert mz == pytest.approx(out["mz"], rel=1e-11, abs=1e-13)
     ctx.close()
+
+
+def test_ewald_sum_independent_of_alpha_on_device():
+    """The reference's own self-check (CheckEwaldSum, src/interaction.cpp:
+    678-755) as a test of the device path: two converged splittings of the
+    same sum agree to 1e-7 (N = 1000, alpha L = 8 and 10); an unconverged
+    k sum differs visibly, so the check can fail."""
+    d = load("heun_n1000")
+    L = float(d["lBox"][0])
+    out = []
+    for aL, nc in ((8.0, 20), (10.0, 24), (9.0, 6)):
+        ctx = gpu_ctx(d)
+        ctx.set_ewald(aL / L, 0.5 * L, nc)
+        ctx.set_state(d["s0_pos_in"], d["s0_mm_in"], d["s0_ea_in"])
+        ctx.set_noise(None, None, None)
+        F, B, _, _ = ctx.comp_interactions()
+        out.append((F, B))
+        ctx.close()
+    f_al, b_al, _ = contact_scales(d)
+    assert rel_err(out[0][1], out[1][1], b_al) < 1e-7
+    assert rel_err(out[0][0], out[1][0], f_al) < 1e-7
+    assert rel_err(out[2][1], out[1][1], b_al) > 1e-6
