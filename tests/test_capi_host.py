@@ -3,7 +3,6 @@ exports every entry point of include/mdsim_capi.h; the ini reader and the
 derived constants, particle properties, box edge, short-range cut-off and Ewald
 error models match the values the reference itself produced."""
 import ctypes as C
-import math
 import os
 import re
 
@@ -12,7 +11,7 @@ import pytest
 
 import md_sim_wolfschwenger_amd as mds
 from md_sim_wolfschwenger_amd import capi
-from refcfg import CASES, load, parse_ini
+from refcfg import CASES, load
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 

@@ -8,7 +8,6 @@ right-hand side in closed form. No GPU."""
 import math
 
 import numpy as np
-import pytest
 
 from refcfg import load, oracle_system, props, rel_err
 
