@@ -1042,6 +1042,20 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     const float rr2f = rcf * rcf;
     const float Lf = (float)L;
     const float pxf = (float)pi.x, pyf = (float)pi.y, pzf = (float)pi.z;
+    // bounding box of the chunk's particles: a j farther than the candidate
+    // radius from the box is farther from every i, so the per-i tests skip it.
+    // Componentwise |fl(q - p)| >= the box gap for every p in the box
+    // (rounding is monotone); the slack on the squared radius covers the
+    // rounding of the sums of squares.
+    // The bounds are uniform over the wave and kept in scalar registers.
+    const float big = 3.0e38f;
+    auto wmax = [](float v) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, wave_max(v))));
+    };
+    const float bx0 = -wmax(vi ? -pxf : -big), bx1 = wmax(vi ? pxf : -big);
+    const float by0 = -wmax(vi ? -pyf : -big), by1 = wmax(vi ? pyf : -big);
+    const float bz0 = -wmax(vi ? -pzf : -big), bz1 = wmax(vi ? pzf : -big);
+    const float rb2f = rr2f * 1.0001f;
     double press[3] = {0, 0, 0};
     int64_t npairs = 0;
     int qn = 0;
@@ -1066,15 +1080,22 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
             // distance test; the exact r < r_c decision is made on the
             // evaluated pair
             float qx = 0, qy = 0, qz = 0;
+            bool near = false;
             if (jl < ce) {
                 const float4 pj = g.rec_pos32[jl];
                 qx = pj.x + (float)sx;
                 qy = pj.y + (float)sy;
                 qz = pj.z + (float)sz;
+                const float ex = fmaxf(fmaxf(bx0 - qx, qx - bx1), 0.0f);
+                const float ey = fmaxf(fmaxf(by0 - qy, qy - by1), 0.0f);
+                const float ez = fmaxf(fmaxf(bz0 - qz, qz - bz1), 0.0f);
+                near = allpairs || ex * ex + ey * ey + ez * ez < rb2f;
             }
             const int cnt = min(WAVE, ce - jb);
             bool any = false;
-            for (int t = 0; t < cnt; ++t) {
+            // the candidates near the box, in increasing order (uniform mask)
+            for (uint64_t cm = __ballot(near); cm; cm &= cm - 1) {
+                const int t = __builtin_ctzll(cm);
                 float dx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qx), t)) - pxf;
                 float dy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qy), t)) - pyf;
                 float dz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qz), t)) - pzf;
