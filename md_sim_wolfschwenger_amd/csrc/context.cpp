@@ -248,7 +248,7 @@ struct mds_ctx {
     std::vector<KRow> h_rows;
     std::vector<KSeg> h_segs;
     std::vector<KPack> h_pack;
-    int32_t nk_all = 0, kb_items = 0;
+    int32_t nk_all = 0, kb_items = 0, kbm_items = 0;
     DevBuf<KRow> rows;
     DevBuf<KSeg> segs;
     DevBuf<KPack> pack_desc;
@@ -465,7 +465,7 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
             while ((int64_t)(nxmax + 1) * (nxmax + 1) + r2 <= nc2) ++nxmax;
             const bool half = (ny == 0 && nz == 0);
             if (half && nxmax == 0) continue;
-            c->h_rows.push_back({ny, nz, nxmax, (int32_t)c->h_pack.size() - R.pk_off});
+            c->h_rows.push_back({ny, nz, nxmax, (int32_t)c->h_pack.size() - R.pk_off, koff});
             for (int a = half ? 1 : 0; a <= nxmax; a += KSEG)
                 c->h_segs.push_back({ny, nz, a, std::min(nxmax, a + KSEG - 1), koff, nxmax});
             // scatter table: |nx| = a pairs (+a, ny, nz) with (-a, ny, nz)
@@ -514,6 +514,9 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
     chunk = std::max<int64_t>(32, std::min<int64_t>(1024, chunk));
     R.kchunk = (int32_t)chunk;
     R.nchunks = (int32_t)((m.n + chunk - 1) / chunk);
+    // matrix-core build: blocks of 4 x 16 rows, passes of KBM_NX values of |nx|
+    R.nrowg = (R.nrows + KBM_ROWS - 1) / KBM_ROWS;
+    R.kbm_pass = (nc + KBM_NX) / KBM_NX;
 }
 
 // Ewald constants and FP32 units of replica r.
@@ -572,7 +575,7 @@ mds_status plan_all(mds_ctx* c) {
     c->h_pack.clear();
     std::vector<int32_t> cell_rep, rep_of((size_t)c->n);
     c->h_chunks.clear();
-    int64_t cells = 0, fcells = 0, work_cap = 0, tab = 0, part = 0, kb = 0, kall = 0;
+    int64_t cells = 0, fcells = 0, work_cap = 0, tab = 0, part = 0, kb = 0, kbm = 0, kall = 0;
     for (int r = 0; r < c->nrep; ++r) {
         const RepMeta& m = c->meta[r];
         Rep& R = c->h_rep[r];
@@ -599,6 +602,8 @@ mds_status plan_all(mds_ctx* c) {
         part += 2 * (int64_t)R.nk * R.nchunks;
         R.kb_off = (int32_t)kb;
         kb += (R.nk > 0) ? (int64_t)R.nsegw * R.nchunks : 0;
+        R.kbm_off = (int32_t)kbm;
+        kbm += (R.nk > 0) ? (int64_t)((R.nrowg + 3) / 4) * R.kbm_pass * R.nchunks : 0;
         std::fill(rep_of.begin() + m.off, rep_of.begin() + m.off + m.n, r);
         R.red_off = (int32_t)c->h_chunks.size();
         for (int64_t b = m.off; b < m.off + m.n; b += RED_CHUNK_PARTICLES) c->h_chunks.push_back(make_int2(r, (int)b));
@@ -606,7 +611,7 @@ mds_status plan_all(mds_ctx* c) {
     }
     // work items pack (cell << 8) | chunk into an int32; pair-queue entries
     // (j << 6) | lane, with bit 31 free
-    if (cells >= (1 << 23) || fcells >= (1 << 30) || work_cap >= (1 << 30) || kb >= (1 << 30))
+    if (cells >= (1 << 23) || fcells >= (1 << 30) || work_cap >= (1 << 30) || kb >= (1 << 30) || kbm >= (1 << 30))
         return fail(c, MDS_ECAPACITY, "too many cells or k-space work items in one context");
     if (c->n >= (int64_t)1 << 25) return fail(c, MDS_ECAPACITY, "more than 2^25 particles in one context");
     c->cells_main = (int32_t)cells;
@@ -615,6 +620,7 @@ mds_status plan_all(mds_ctx* c) {
     c->cand_cap = (int32_t)std::min<int64_t>(2 * c->n + 4096, (int64_t)1 << 29);
     c->nk_all = (int32_t)kall;
     c->kb_items = (int32_t)kb;
+    c->kbm_items = (int32_t)kbm;
     HIPCHK(c->cnt_main.ensure(cells));
     HIPCHK(c->start_main.ensure(cells));
     HIPCHK(c->end_main.ensure(cells));
@@ -831,8 +837,12 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
         if (c->nk_all > 0) {
             {
                 mds_ctx::Span sp(c, KC_KBUILD);
-                HIPCHK(launch_kbuild(ks, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p,
-                                     c->kpartial.p));
+                if (c->cfg.precision == MDS_PREC_MIXED)
+                    HIPCHK(launch_kbuild_mfma(ks, n, c->reps.p, c->nrep, c->kbm_items, c->rows.p, c->tab.p, c->mom.p,
+                                              c->kpartial.p));
+                else
+                    HIPCHK(launch_kbuild(ks, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p,
+                                         c->kpartial.p));
             }
             {
                 mds_ctx::Span sp(c, KC_KREDUCE);

@@ -12,6 +12,7 @@ namespace mds {
 // |nx| = 0..nxmax, padded with a zero entry to an even count).
 struct KRow {
     int32_t ny, nz, nxmax, poff;
+    int32_t koff;  // index of the row's first k vector in the S(k) arrays
 };
 
 // Packed scatter table entry for the k pair n = (+a, ny, nz), (-a, ny, nz):
@@ -27,6 +28,9 @@ struct KPack {
 // A segment of a k-space row for the structure-factor build: |nx| in
 // [alo, ahi] (both signs unless the row is ny = nz = 0), at most KSEG values.
 constexpr int KSEG = 8;
+// The matrix-core S(k) build (FP32 path) takes 16 rows and KBM_NX values of
+// |nx| per wavefront and pass.
+constexpr int KBM_ROWS = 16, KBM_NX = 24;
 struct KSeg {
     int32_t ny, nz, alo, ahi, koff, nxmax;
 };
@@ -46,7 +50,8 @@ struct Rep {
     double fx_s[2], fx_is[2];    // fixed-point unit s1 of the pair sums (field, force) and 1/s1
     double px_s, px_is;          // fixed-point unit of the moved pairs' pressure sums (SI) and 1/unit
     float al_s, al2_s, al4_s, pad_;
-    int32_t nc, row_off, nrows, seg_off, nseg, k_off, nk, kchunk, nchunks, kb_off, nsegw, pad2_;
+    int32_t nc, row_off, nrows, seg_off, nseg, k_off, nk, kchunk, nchunks, kb_off, nsegw, nrowg;
+    int32_t kbm_off, kbm_pass;   // matrix-core S(k) build: first work item, passes of KBM_NX |nx| values
     int32_t pk_off, npk;         // slice of the packed scatter table
     int32_t red_off, nred;       // slice of the reduction chunk table
     int32_t ksplit[4];           // first row of each quarter of the scatter table (by entries)
@@ -195,6 +200,10 @@ hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const
                                 double* tab);
 hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t kb_items, const KSeg* segs,
                          const double* tab, const double* mom, double* partial);
+// The same S(k) partials (same chunks and layout) from FP32 matrix-core
+// products, for the mixed-precision path.
+hipError_t launch_kbuild_mfma(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
+                              const KRow* rows, const double* tab, const double* mom, double* partial);
 // S(k) partials -> packed scatter table (fixed summation order)
 hipError_t launch_kpack(hipStream_t s, int32_t npk_all, const Rep* reps, int32_t nrep, const KPack* desc,
                         const double* partial, double4* pack);

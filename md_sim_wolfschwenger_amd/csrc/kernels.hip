@@ -1374,7 +1374,10 @@ __device__ __forceinline__ int find_rep(const Rep* __restrict__ reps, int32_t nr
     int lo = 0, hi = nrep - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const int32_t first = WHICH == 0 ? reps[mid].kb_off : (WHICH == 1 ? reps[mid].k_off : reps[mid].pk_off);
+        const int32_t first = WHICH == 0   ? reps[mid].kb_off
+                              : WHICH == 1 ? reps[mid].k_off
+                              : WHICH == 2 ? reps[mid].pk_off
+                                           : reps[mid].kbm_off;
         if (first <= b) lo = mid; else hi = mid - 1;
     }
     return lo;
@@ -1454,6 +1457,132 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
             out[2 * km + 1] = ms_[q];
         }
     }
+}
+
+// The same S(k) partials from FP32 matrix-core products (mixed-precision
+// path). Per row (ny, nz) and |nx| = a, with X = (c, s) = e^{i a x_j},
+// W = e^{i (ny y_j + nz z_j)}, A = ny my_j + nz mz_j, P = W A and Q = W mx_j:
+//   S(+a) = (aP - bP) + a (aQ - bQ) + i [(cP + dP) + a (cQ + dQ)],
+//   S(-a) = (aP + bP) - a (aQ + bQ) + i [(cP - dP) - a (cQ - dQ)],
+// aP = sum c P.re, bP = sum s P.im, cP = sum c P.im, dP = sum s P.re (Q
+// alike), which is the first pass of CompDipInteractEwaldF
+// (src/interaction.cpp:459-483) with e^{i k.r} = X W. The four real sums are
+// products (a, c|s) x particles times particles x rows, taken with
+// v_mfma_f32_16x16x4f32: M = 8 values of a times (c, s), N = 16 rows, K = 4
+// particles; four products (P.re, P.im, Q.re, Q.im) per M tile. In that
+// instruction lane l gives A[l % 16][l / 16] and B[l / 16][l % 16] and holds
+// D[4 (l / 16) + r][l % 16], r = 0..3, so a lane ends with all eight sums of
+// two values of a and one row. Accumulation is FP32 within a particle chunk;
+// the chunks are added in FP64 by k_kpack.
+// Block: four wavefronts on four groups of 16 rows, one particle chunk and
+// one pass of KBM_NX values of a; the chunk's table rows are staged in LDS
+// as FP32, KBM_SUB particles at a time.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int KBM_SUB = 32, KBM_WAVES = 4, KBM_TMAX = 2 * 65;  // n_c <= 64
+
+__global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* __restrict__ reps, int32_t nrep,
+                                                     int32_t items, const KRow* __restrict__ rows_all,
+                                                     const double* __restrict__ tab_all,
+                                                     const double* __restrict__ mom,
+                                                     double* __restrict__ partial_all) {
+    __shared__ float sx[KBM_SUB][2 * KBM_NX];
+    __shared__ float syz[KBM_SUB][2 * KBM_TMAX];
+    __shared__ float sm[KBM_SUB][4];
+    const int32_t item = blockIdx.x;
+    if (item >= items) return;  // uniform over the block
+    const Rep& R = reps[find_rep<3>(reps, nrep, item)];
+    const int32_t li = item - R.kbm_off;
+    const int32_t nblk = (R.nrowg + KBM_WAVES - 1) / KBM_WAVES;
+    const int32_t blk = li % nblk, rest = li / nblk;
+    const int32_t pass = rest % R.kbm_pass, ch = rest / R.kbm_pass;
+    const int wv = wave_id(), lane = lane_id(), tid = threadIdx.x;
+    const int kk = lane >> 4, col = lane & 15;
+    const int nc = R.nc, tw = 2 * (nc + 1);
+    const int64_t stride = 3 * (int64_t)tw;
+    const double* tab = tab_all + R.tab_off;
+    const int64_t n = n_all;
+    const int a0 = pass * KBM_NX;
+    // this lane's row (B side)
+    const int r = (blk * KBM_WAVES + wv) * KBM_ROWS + col;
+    const bool vrow = r < R.nrows;
+    const KRow row = vrow ? rows_all[R.row_off + r] : KRow{0, 0, -1, 0, 0};
+    const int yo = 2 * (row.ny < 0 ? -row.ny : row.ny), zo = tw + 2 * row.nz;
+    const float ysg = row.ny < 0 ? -1.0f : 1.0f;
+    const float fny = (float)row.ny, fnz = (float)row.nz;
+    f32x4 acc[3][4];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[t][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int64_t jb = (int64_t)ch * R.kchunk;
+    const int64_t je = jb + R.kchunk < R.n ? jb + R.kchunk : R.n;
+    for (int64_t s0 = jb; s0 < je; s0 += KBM_SUB) {
+        // stage KBM_SUB particles: X for a in [a0, a0 + KBM_NX), the y and z
+        // tables, the moment; zeros past the chunk's end and past n_c
+        __syncthreads();
+        for (int e = tid; e < KBM_SUB * 2 * KBM_NX; e += 256) {
+            const int p = e / (2 * KBM_NX), o = e % (2 * KBM_NX);
+            const int64_t jr = s0 + p;
+            const int a = a0 + (o >> 1);
+            sx[p][o] = (jr < je && a <= nc) ? (float)tab[jr * stride + 2 * a + (o & 1)] : 0.0f;
+        }
+        for (int e = tid; e < KBM_SUB * 2 * tw; e += 256) {
+            const int p = e / (2 * tw), o = e % (2 * tw);
+            const int64_t jr = s0 + p;
+            syz[p][o] = jr < je ? (float)tab[jr * stride + tw + o] : 0.0f;
+        }
+        if (tid < KBM_SUB * 3) {
+            const int p = tid / 3, d = tid % 3;
+            const int64_t jr = s0 + p;
+            sm[p][d] = jr < je ? (float)mom[d * n + R.off + jr] : 0.0f;
+        }
+        __syncthreads();
+        const int steps = (int)((je - s0 < KBM_SUB ? je - s0 : KBM_SUB) + 3) / 4;
+        for (int st = 0; st < steps; ++st) {
+            const int p = 4 * st + kk;
+            float bv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (vrow) {
+                const float yc = syz[p][yo], ys = ysg * syz[p][yo + 1];
+                const float zc = syz[p][zo], zs = syz[p][zo + 1];
+                const float wc = yc * zc - ys * zs, ws = ys * zc + yc * zs;
+                const float mx = sm[p][0];
+                const float A = fny * sm[p][1] + fnz * sm[p][2];
+                bv[0] = wc * A;
+                bv[1] = ws * A;
+                bv[2] = wc * mx;
+                bv[3] = ws * mx;
+            }
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const float xa = sx[p][16 * t + col];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, bv[b], acc[t][b], 0, 0, 0);
+            }
+        }
+    }
+    if (!vrow) return;
+    double* out = partial_all + R.part_off + (int64_t)ch * R.nk * 2;
+    const bool half = row.ny == 0 && row.nz == 0;
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int qq = 0; qq < 2; ++qq) {
+            const int a = a0 + 8 * t + 2 * kk + qq;
+            if (a > row.nxmax || (half && a == 0)) continue;
+            const double aP = acc[t][0][2 * qq], dP = acc[t][0][2 * qq + 1];
+            const double cP = acc[t][1][2 * qq], bP = acc[t][1][2 * qq + 1];
+            const double aQ = acc[t][2][2 * qq], dQ = acc[t][2][2 * qq + 1];
+            const double cQ = acc[t][3][2 * qq], bQ = acc[t][3][2 * qq + 1];
+            const double fa = (double)a;
+            const int kp = half ? row.koff + a - 1 : row.koff + a + row.nxmax;
+            out[2 * kp] = (aP - bP) + fa * (aQ - bQ);
+            out[2 * kp + 1] = (cP + dP) + fa * (cQ + dQ);
+            if (a >= 1 && !half) {
+                const int km = row.koff - a + row.nxmax;
+                out[2 * km] = (aP + bP) - fa * (aQ + bQ);
+                out[2 * km + 1] = (cP - dP) - fa * (cQ - dQ);
+            }
+        }
 }
 
 // Sum the S(k) partials of both k vectors of a packed entry over the
@@ -2189,6 +2318,13 @@ hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep
     if (kb_items <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_kbuild<KSEG>, dim3(kb_items), dim3(WAVE), 0, s, n, reps, nrep, kb_items, segs, tab, mom,
                        partial);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_kbuild_mfma(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
+                              const KRow* rows, const double* tab, const double* mom, double* partial) {
+    if (items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kbuild_mfma, dim3(items), dim3(256), 0, s, n, reps, nrep, items, rows, tab, mom, partial);
     LAUNCH_CHECK();
 }
 
