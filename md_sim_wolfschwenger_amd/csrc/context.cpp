@@ -248,7 +248,7 @@ struct mds_ctx {
     std::vector<KRow> h_rows;
     std::vector<KSeg> h_segs;
     std::vector<KPack> h_pack;
-    int32_t nk_all = 0, kb_items = 0, kbm_items = 0;
+    int32_t nk_all = 0, kb_items = 0, kbm_items = 0, ksm_items = 0, nc_max = 0;
     DevBuf<KRow> rows;
     DevBuf<KSeg> segs;
     DevBuf<KPack> pack_desc;
@@ -575,7 +575,8 @@ mds_status plan_all(mds_ctx* c) {
     c->h_pack.clear();
     std::vector<int32_t> cell_rep, rep_of((size_t)c->n);
     c->h_chunks.clear();
-    int64_t cells = 0, fcells = 0, work_cap = 0, tab = 0, part = 0, kb = 0, kbm = 0, kall = 0;
+    int64_t cells = 0, fcells = 0, work_cap = 0, tab = 0, part = 0, kb = 0, kbm = 0, ksm = 0, kall = 0;
+    int32_t nc_max = 0;
     for (int r = 0; r < c->nrep; ++r) {
         const RepMeta& m = c->meta[r];
         Rep& R = c->h_rep[r];
@@ -602,6 +603,9 @@ mds_status plan_all(mds_ctx* c) {
         part += 2 * (int64_t)R.nk * R.nchunks;
         R.kb_off = (int32_t)kb;
         kb += (R.nk > 0) ? (int64_t)R.nsegw * R.nchunks : 0;
+        R.ksm_off = (int32_t)ksm;
+        ksm += (m.n + 63) / 64;
+        nc_max = std::max(nc_max, R.nc);
         R.kbm_off = (int32_t)kbm;
         kbm += (R.nk > 0) ? (int64_t)((R.nrowg + 3) / 4) * R.kbm_pass * R.nchunks : 0;
         std::fill(rep_of.begin() + m.off, rep_of.begin() + m.off + m.n, r);
@@ -621,6 +625,8 @@ mds_status plan_all(mds_ctx* c) {
     c->nk_all = (int32_t)kall;
     c->kb_items = (int32_t)kb;
     c->kbm_items = (int32_t)kbm;
+    c->ksm_items = (int32_t)ksm;
+    c->nc_max = nc_max;
     HIPCHK(c->cnt_main.ensure(cells));
     HIPCHK(c->start_main.ensure(cells));
     HIPCHK(c->end_main.ensure(cells));
@@ -851,8 +857,13 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
             }
         }
         mds_ctx::Span sp(c, force_pass ? KC_KSCATTER_FORCE : KC_KSCATTER_FIELD);
-        HIPCHK(launch_kscatter(ks, force_pass, n, c->reps.p, c->rep_of.p, c->rows.p, c->pack.p, c->tab.p, c->mom.p,
-                               c->msum.p, force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
+        if (c->cfg.precision == MDS_PREC_MIXED && c->nc_max <= KSM_NCMAX)
+            HIPCHK(launch_kscatter_mfma(ks, force_pass, n, c->reps.p, c->nrep, c->ksm_items, c->nc_max, c->rows.p,
+                                        c->pack.p, c->tab.p, c->mom.p, c->msum.p, force_pass ? c->n_f.p : nullptr,
+                                        c->force.p, c->field.p));
+        else
+            HIPCHK(launch_kscatter(ks, force_pass, n, c->reps.p, c->rep_of.p, c->rows.p, c->pack.p, c->tab.p,
+                                   c->mom.p, c->msum.p, force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
         return MDS_OK;
     };
     const bool split = c->overlap_k && !c->timing;

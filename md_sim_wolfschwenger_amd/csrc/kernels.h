@@ -52,6 +52,7 @@ struct Rep {
     float al_s, al2_s, al4_s, pad_;
     int32_t nc, row_off, nrows, seg_off, nseg, k_off, nk, kchunk, nchunks, kb_off, nsegw, nrowg;
     int32_t kbm_off, kbm_pass;   // matrix-core S(k) build: first work item, passes of KBM_NX |nx| values
+    int32_t ksm_off, pad4_;      // matrix-core scatter: first block (64 particles each)
     int32_t pk_off, npk;         // slice of the packed scatter table
     int32_t red_off, nred;       // slice of the reduction chunk table
     int32_t ksplit[4];           // first row of each quarter of the scatter table (by entries)
@@ -205,6 +206,13 @@ hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep
 hipError_t launch_kbuild_mfma(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
                               const KRow* rows, const double* tab, const double* mom, double* partial);
 // S(k) partials -> packed scatter table (fixed summation order)
+// The k-space scatter from FP32 matrix-core products (mixed-precision path,
+// n_c <= KSM_NCMAX in every replica); blocks: ceil(n_r / 64) per replica.
+constexpr int KSM_NCMAX = 32;
+hipError_t launch_kscatter_mfma(hipStream_t s, bool force, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
+                                int32_t nc_max, const KRow* rows, const double4* pack, const double* tab,
+                                const double* mom, const double* msum, const double* add_force, double* force_out,
+                                double* field);
 hipError_t launch_kpack(hipStream_t s, int32_t npk_all, const Rep* reps, int32_t nrep, const KPack* desc,
                         const double* partial, double4* pack);
 // field (and force) of the k-space sum, self and surface terms (and the

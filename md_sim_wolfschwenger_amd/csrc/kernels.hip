@@ -1377,7 +1377,8 @@ __device__ __forceinline__ int find_rep(const Rep* __restrict__ reps, int32_t nr
         const int32_t first = WHICH == 0   ? reps[mid].kb_off
                               : WHICH == 1 ? reps[mid].k_off
                               : WHICH == 2 ? reps[mid].pk_off
-                                           : reps[mid].kbm_off;
+                              : WHICH == 3 ? reps[mid].kbm_off
+                                           : reps[mid].ksm_off;
         if (first <= b) lo = mid; else hi = mid - 1;
     }
     return lo;
@@ -1764,6 +1765,174 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     field[n + i] = -cb * a[1] + c_self * my - c_surf * msum[1];
     field[2 * n + i] = -cb * a[2] + c_self * mz - c_surf * msum[2];
     if (FORCE) {
+        force[i] = cf * a[3] + (add_force ? add_force[i] : 0.0);
+        force[n + i] = cf * a[4] + (add_force ? add_force[n + i] : 0.0);
+        force[2 * n + i] = cf * a[5] + (add_force ? add_force[2 * n + i] : 0.0);
+    }
+}
+
+// The scatter from FP32 matrix-core products (mixed-precision path). With
+// the packed entries (SU, DU, SV, DV) of a row, x = (c, s) = e^{i a x_i} and
+// w = e^{i (ny y_i + nz z_i)}, the sums over |nx| = a of k_kscatter's
+// per-entry terms are, per particle and row,
+//   E1 = sum (c SU + s DV),     E2 = sum (c SV - s DU),
+//   O1 = sum a (c DU + s SV),   O2 = sum a (c DV - s SU),
+//   E1'', E2'': E1, E2 with weight a^2,
+//   field: B0 += wc O1 + ws O2, (B1, B2) += (ny, nz) (wc E1 + ws E2),
+//   force: with A = ny my + nz mz, rs = A (ws E1 - wc E2) + mx (ws O1 - wc O2),
+//          F0 += A (ws O1 - wc O2) + mx (ws E1'' - wc E2''), (F1, F2) += (ny, nz) rs.
+// The E and O sums are products particles x (a, c|s) times (a, c|s) x rows:
+// v_mfma_f32_16x16x4f32 with M = 16 particles, N = 16 rows, K = two values
+// of a times (c, s); a lane then holds them for four particles and one row
+// and adds the row's terms in FP64. A block takes 64 particles of one
+// replica (16 per wavefront) and walks the rows 16 at a time; the particles'
+// phase tables and each row tile's entries are staged in LDS as FP32, in
+// units of the replica's largest moment.
+template <bool FORCE>
+__global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __restrict__ reps, int32_t nrep,
+                                                       int32_t items, const KRow* __restrict__ rows_all,
+                                                       const double4* __restrict__ pack_all,
+                                                       const double* __restrict__ tab_all,
+                                                       const double* __restrict__ mom,
+                                                       const double* __restrict__ msum_all,
+                                                       const double* __restrict__ add_force,
+                                                       double* __restrict__ force, double* __restrict__ field) {
+    extern __shared__ float4 smem4[];
+    const int32_t item = blockIdx.x;
+    if (item >= items) return;  // uniform over the block
+    const int rid = find_rep<4>(reps, nrep, item);
+    const Rep& R = reps[rid];
+    const int64_t pb = (int64_t)(item - R.ksm_off) * WAVE;  // replica-local first particle
+    const int wv = wave_id(), lane = lane_id(), tid = threadIdx.x;
+    const int kk = lane >> 4, col = lane & 15;
+    const int nc = R.nc, tw = 2 * (nc + 1), xw = tw + 2, nent = nc + 2;
+    const int64_t stride = 3 * (int64_t)tw;
+    const double* tab = tab_all + R.tab_off;
+    const double im = R.inv_mref;
+    float4* sp = smem4;                          // [nent][16] row tile entries
+    float* sx = (float*)(smem4 + nent * 16);     // [64][xw] x phases (c, s) per a, zero padded
+    float* syz = sx + WAVE * xw;                 // [64][2 tw] y, then z tables
+    for (int e = tid; e < WAVE * xw; e += 256) {
+        const int p = e / xw, o = e % xw;
+        const int64_t jr = pb + p;
+        sx[e] = (jr < R.n && o < tw) ? (float)tab[jr * stride + o] : 0.0f;
+    }
+    for (int e = tid; e < WAVE * 2 * tw; e += 256) {
+        const int p = e / (2 * tw), o = e % (2 * tw);
+        const int64_t jr = pb + p;
+        syz[e] = jr < R.n ? (float)tab[jr * stride + tw + o] : 0.0f;
+    }
+    // this lane's particles (rows of the products): 16 wv + 4 kk + r
+    float mxs[4], mys[4], mzs[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int64_t jr = pb + 16 * wv + 4 * kk + r;
+        const int64_t j = R.off + jr;
+        mxs[r] = jr < R.n ? (float)(mom[j] * im) : 0.0f;
+        mys[r] = jr < R.n ? (float)(mom[n + j] * im) : 0.0f;
+        mzs[r] = jr < R.n ? (float)(mom[2 * n + j] * im) : 0.0f;
+    }
+    constexpr int NA = FORCE ? 6 : 3;
+    double acc[4][NA];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int d = 0; d < NA; ++d) acc[r][d] = 0.0;
+    const KRow* rows = rows_all + R.row_off;
+    const double4* pack = pack_all + R.pk_off;
+    const float* xrow = sx + (16 * wv + col) * xw + kk;
+    for (int r0 = 0; r0 < R.nrows; r0 += 16) {
+        const int nt = R.nrows - r0 < 16 ? R.nrows - r0 : 16;
+        int amax = 0;
+        for (int q = 0; q < nt; ++q) amax = max(amax, rows[r0 + q].nxmax);
+        const int kn = (amax + 2) / 2;  // a in [0, 2 kn), within [0, n_c + 1]
+        __syncthreads();                // the previous tile's entries are read
+        for (int e = tid; e < 2 * kn * 16; e += 256) {
+            const int a = e >> 4, q = e & 15;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (q < nt) {
+                const KRow rw = rows[r0 + q];
+                if (a <= rw.nxmax) {
+                    const double4 d = pack[rw.poff + a];
+                    v = make_float4((float)(d.x * im), (float)(d.y * im), (float)(d.z * im), (float)(d.w * im));
+                }
+            }
+            sp[e] = v;
+        }
+        __syncthreads();
+        f32x4 c[NA == 6 ? 6 : 4];
+#pragma unroll
+        for (int u = 0; u < (NA == 6 ? 6 : 4); ++u) c[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        const bool xs_ = kk & 1;
+        for (int ks = 0; ks < kn; ++ks) {
+            const int a = 2 * ks + (kk >> 1);
+            const float xa = xrow[4 * ks];
+            const float4 p = sp[a * 16 + col];
+            const float fa = (float)a;
+            const float e1 = xs_ ? p.w : p.x;          // SU | DV
+            const float e2 = xs_ ? -p.y : p.z;         // SV | -DU
+            const float o1 = fa * (xs_ ? p.z : p.y);   // a (DU | SV)
+            const float o2 = fa * (xs_ ? -p.x : p.w);  // a (DV | -SU)
+            c[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, e1, c[0], 0, 0, 0);
+            c[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, e2, c[1], 0, 0, 0);
+            c[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, o1, c[2], 0, 0, 0);
+            c[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, o2, c[3], 0, 0, 0);
+            if constexpr (FORCE) {
+                const float a2 = fa * fa;
+                c[4] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, a2 * e1, c[4], 0, 0, 0);
+                c[5] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, a2 * e2, c[5], 0, 0, 0);
+            }
+        }
+        if (col < nt) {
+            const KRow rw = rows[r0 + col];
+            const int yo = 2 * (rw.ny < 0 ? -rw.ny : rw.ny), zo = tw + 2 * rw.nz;
+            const float ysg = rw.ny < 0 ? -1.0f : 1.0f, fny = (float)rw.ny, fnz = (float)rw.nz;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float* yz = syz + (16 * wv + 4 * kk + r) * 2 * tw;
+                const float yc = yz[yo], ys = ysg * yz[yo + 1], zc = yz[zo], zs = yz[zo + 1];
+                const float wc = yc * zc - ys * zs, ws = ys * zc + yc * zs;
+                const float E1 = c[0][r], E2 = c[1][r], O1 = c[2][r], O2 = c[3][r];
+                const float rb = wc * E1 + ws * E2;
+                acc[r][0] += (double)(wc * O1 + ws * O2);
+                acc[r][1] += (double)(fny * rb);
+                acc[r][2] += (double)(fnz * rb);
+                if constexpr (FORCE) {
+                    const float A = fny * mys[r] + fnz * mzs[r];
+                    const float td1 = ws * O1 - wc * O2;
+                    const float rs = A * (ws * E1 - wc * E2) + mxs[r] * td1;
+                    acc[r][3] += (double)(A * td1 + mxs[r] * (ws * c[4][r] - wc * c[5][r]));
+                    acc[r][4] += (double)(fny * rs);
+                    acc[r][5] += (double)(fnz * rs);
+                }
+            }
+        }
+    }
+    // totals over the 16 row lanes of each group (the shuffles stay in it)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int d = 0; d < NA; ++d) {
+            double v = acc[r][d];
+            for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            acc[r][d] = v;
+        }
+    if (col >= 4) return;
+    const int64_t jr = pb + 16 * wv + 4 * kk + col;
+    if (jr >= R.n) return;
+    double a[NA];
+#pragma unroll
+    for (int d = 0; d < NA; ++d) a[d] = col == 0 ? acc[0][d] : col == 1 ? acc[1][d] : col == 2 ? acc[2][d] : acc[3][d];
+    const int64_t i = R.off + jr;
+    const double* msum = msum_all + 3 * rid;
+    const double mref = 1.0 / im;
+    const double mx = mom[i], my = mom[n + i], mz = mom[2 * n + i];
+    const double cb = R.c_flux * mref, cf = R.c_force * mref * mref, c_self = R.c_self, c_surf = R.c_surf;
+    // written here; the real-space pair sums are added after (k_pairs_out)
+    field[i] = -cb * a[0] + c_self * mx - c_surf * msum[0];
+    field[n + i] = -cb * a[1] + c_self * my - c_surf * msum[1];
+    field[2 * n + i] = -cb * a[2] + c_self * mz - c_surf * msum[2];
+    if constexpr (FORCE) {
         force[i] = cf * a[3] + (add_force ? add_force[i] : 0.0);
         force[n + i] = cf * a[4] + (add_force ? add_force[n + i] : 0.0);
         force[2 * n + i] = cf * a[5] + (add_force ? add_force[2 * n + i] : 0.0);
@@ -2325,6 +2494,24 @@ hipError_t launch_kbuild_mfma(hipStream_t s, int64_t n, const Rep* reps, int32_t
                               const KRow* rows, const double* tab, const double* mom, double* partial) {
     if (items <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_kbuild_mfma, dim3(items), dim3(256), 0, s, n, reps, nrep, items, rows, tab, mom, partial);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_kscatter_mfma(hipStream_t s, bool force, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
+                                int32_t nc_max, const KRow* rows, const double4* pack, const double* tab,
+                                const double* mom, const double* msum, const double* add_force, double* force_out,
+                                double* field) {
+    if (items <= 0) return hipSuccess;
+    if (nc_max > KSM_NCMAX) return hipErrorInvalidValue;
+    const int tw = 2 * (nc_max + 1);
+    const size_t lds = (size_t)(nc_max + 2) * 16 * sizeof(float4) + (size_t)WAVE * (tw + 2) * sizeof(float) +
+                       (size_t)WAVE * 2 * tw * sizeof(float);
+    if (force)
+        hipLaunchKernelGGL(k_kscatter_mfma<true>, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, pack, tab,
+                           mom, msum, add_force, force_out, field);
+    else
+        hipLaunchKernelGGL(k_kscatter_mfma<false>, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, pack,
+                           tab, mom, msum, add_force, force_out, field);
     LAUNCH_CHECK();
 }
 
