@@ -146,7 +146,9 @@ def roofline(stats, counts, n, nk, steps, prec):
     peak = PEAK_FP64_TFLOPS if prec == "fp64" else PEAK_FP32_TFLOPS
     unit = ("FP64 vector ALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)" if prec == "fp64" else
             "FP32 vector ALU (packed FP32 disabled; 157.3 TFLOP/s is the packed peak)")
-    return {"kernel": dom, "bound": "mfma", "compute_unit": unit + "; no dense contraction, MFMA unused",
+    note = ("; the pair kernels are vector code (no dense contraction)" if prec == "fp64" else
+            "; the pair kernels are vector code (no dense contraction), the k-space kernels FP32 matrix-core code")
+    return {"kernel": dom, "bound": "mfma", "compute_unit": unit + note,
             "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": profiled_traffic(dom, prec), "traffic_source": os.path.relpath(PROFILE, ROOT),
             "avg_launch_ms": avg_s * 1e3, "flop_per_launch": per_launch[dom],
@@ -459,7 +461,7 @@ def main():
         "metric": "particle-steps/s at N=1e5 mixed vs FP64; p50/p99 ensemble-request latency",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "mixed (f32 real-space pair terms, f64 geometry/accumulation/k-space)", "data": "synthetic",
+        "dtype": "mixed (f32 real-space pair terms and k-space matrix-core products, f64 geometry and accumulation)", "data": "synthetic",
         "config": {"workload": "C3: N=100000 lognormal core (median 10 nm radius, sigma_ln 0.25) + 5 nm shell, "
                                "phi=0.10, B=50 mT z, alpha_G=0.05, T=300 K, eta=1e-3 Pa s, Ewald 1e-6, "
                                "fixed dt=1e-12 s Heun, Philox noise",

@@ -222,6 +222,36 @@ def test_mixed_precision_cell_list_vs_oracle():
     assert rel_err(X, Xo) < 1e-14
 
 
+@pytest.mark.parametrize("nc", [3, 22, 40])
+def test_mixed_kspace_on_matrix_cores_vs_fp64(nc):
+    """The mixed path builds S(k) and scatters it with FP32 matrix-core
+    products (FP32 within a particle chunk or row tile, FP64 across them);
+    n_c = 40 takes two |nx| passes in the build and the FP64 scatter (the
+    matrix-core scatter holds n_c <= 32). Field and force against the FP64
+    path on the same system: within 2e-6 (contact-value floors)."""
+    n = 3000
+    cfg, text, r, rh, L, pos, ea, alpha, rc, _ = _random_system(n, 0.08, 0.3, 91, 0)
+    out = {}
+    for prec in (mds.PREC_FP64, mds.PREC_MIXED):
+        c = mds.config_from_string(text)
+        c.precision = prec
+        ctx = mds.Context(c)
+        ctx.set_particles(r, rh, L)
+        ctx.set_ewald(alpha, rc, nc)
+        ctx.set_state(pos, ea, ea)
+        ctx.set_noise(None, None, None)
+        out[prec] = ctx.comp_interactions()
+        ctx.close()
+    mu = cfg.sat_mag * 4 / 3 * cfg.my_pi * r ** 3
+    m = mu.min()
+    b_al = cfg.mu0 * m / (2 * cfg.my_pi * (2 * rh.min()) ** 3)
+    f_al = 3 * cfg.mu0 * m * m / (2 * cfg.my_pi * (2 * rh.min()) ** 4)
+    F64, B64 = out[mds.PREC_FP64][:2]
+    F32, B32 = out[mds.PREC_MIXED][:2]
+    assert rel_err(B32, B64, b_al) < 2e-6
+    assert rel_err(F32, F64, f_al) < 2e-6
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_kspace_overlap_changes_nothing(graph):
     """S(k) built on the second stream beside the pair kernels (the default)
