@@ -88,7 +88,7 @@ def tune(ctx, info, args):
         ctx.set_tiling(args.reach)
         res = None
     else:
-        rcs = (args.rc_frac,) if args.rc_frac > 0 else (0.2, 0.225, 0.25, 0.275, 0.3, 0.35)
+        rcs = (args.rc_frac,) if args.rc_frac > 0 else (0.15, 0.175, 0.2, 0.225, 0.25, 0.275, 0.3, 0.35)
         reaches = (args.reach,) if args.reach > 0 else (2, 3, 4)
         res = ctx.autotune(rcs, reaches, reps=2)
         log(f"  autotune: {res} -> {ctx.params()} {ctx.tiling()}")

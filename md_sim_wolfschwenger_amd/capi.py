@@ -279,7 +279,7 @@ class Context:
         self._chk(self._L.mds_get_tiling(self.h, *[C.byref(x) for x in v]), "get_tiling")
         return {"cell_reach": v[0].value, "ncell": v[1].value, "nstencil": v[2].value}
 
-    def autotune(self, rc_fracs=(0.2, 0.225, 0.25, 0.275, 0.3), reaches=(2, 3, 4), reps=2):
+    def autotune(self, rc_fracs=(0.175, 0.2, 0.225, 0.25, 0.275, 0.3), reaches=(2, 3, 4), reps=2):
         """Time each (r_c, cell reach) candidate on the device; install the fastest."""
         rcs = np.ascontiguousarray(rc_fracs, dtype=np.float64)
         rea = np.ascontiguousarray(reaches, dtype=np.int32)
