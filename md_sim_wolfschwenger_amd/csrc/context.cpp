@@ -843,9 +843,9 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
         if (c->nk_all > 0) {
             {
                 mds_ctx::Span sp(c, KC_KBUILD);
-                if (c->cfg.precision == MDS_PREC_MIXED)
-                    HIPCHK(launch_kbuild_mfma(ks, n, c->reps.p, c->nrep, c->kbm_items, c->rows.p, c->tab.p, c->mom.p,
-                                              c->kpartial.p));
+                if (c->cfg.precision == MDS_PREC_MIXED && c->nc_max <= KBM_NCMAX)
+                    HIPCHK(launch_kbuild_mfma(ks, n, c->reps.p, c->nrep, c->kbm_items, c->nc_max, c->rows.p, c->tab.p,
+                                              c->mom.p, c->kpartial.p));
                 else
                     HIPCHK(launch_kbuild(ks, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p,
                                          c->kpartial.p));

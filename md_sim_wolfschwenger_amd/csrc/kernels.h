@@ -203,8 +203,11 @@ hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep
                          const double* tab, const double* mom, double* partial);
 // The same S(k) partials (same chunks and layout) from FP32 matrix-core
 // products, for the mixed-precision path.
+// Every replica needs n_c <= KBM_NCMAX (the block's LDS).
+constexpr int KBM_NCMAX = 50;
 hipError_t launch_kbuild_mfma(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
-                              const KRow* rows, const double* tab, const double* mom, double* partial);
+                              int32_t nc_max, const KRow* rows, const double* tab, const double* mom,
+                              double* partial);
 // S(k) partials -> packed scatter table (fixed summation order)
 // The k-space scatter from FP32 matrix-core products (mixed-precision path,
 // n_c <= KSM_NCMAX in every replica); blocks: ceil(n_r / 64) per replica.

@@ -1477,18 +1477,21 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
 // the chunks are added in FP64 by k_kpack.
 // Block: four wavefronts on four groups of 16 rows, one particle chunk and
 // one pass of KBM_NX values of a; the chunk's table rows are staged in LDS
-// as FP32, KBM_SUB particles at a time.
+// as FP32, KBM_SUB particles at a time (n_c <= KBM_NCMAX: the LDS).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-constexpr int KBM_SUB = 32, KBM_WAVES = 4, KBM_TMAX = 2 * 65;  // n_c <= 64
+constexpr int KBM_SUB = 64, KBM_WAVES = 4;
 
 __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* __restrict__ reps, int32_t nrep,
                                                      int32_t items, const KRow* __restrict__ rows_all,
                                                      const double* __restrict__ tab_all,
                                                      const double* __restrict__ mom,
                                                      double* __restrict__ partial_all) {
-    __shared__ float sx[KBM_SUB][2 * KBM_NX];
-    __shared__ float syz[KBM_SUB][2 * KBM_TMAX];
-    __shared__ float sm[KBM_SUB][4];
+    // dynamic LDS: [KBM_SUB][2 KBM_NX] x phases, [KBM_SUB][4] moments,
+    // [KBM_SUB][2 tw] y and z tables (tw of this replica's n_c)
+    extern __shared__ float kbm_smem[];
+    float(*sx)[2 * KBM_NX] = (float(*)[2 * KBM_NX])kbm_smem;
+    float(*sm)[4] = (float(*)[4])(kbm_smem + KBM_SUB * 2 * KBM_NX);
+    float* syz = kbm_smem + KBM_SUB * (2 * KBM_NX + 4);
     const int32_t item = blockIdx.x;
     if (item >= items) return;  // uniform over the block
     const Rep& R = reps[find_rep<3>(reps, nrep, item)];
@@ -1530,7 +1533,7 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
         for (int e = tid; e < KBM_SUB * 2 * tw; e += 256) {
             const int p = e / (2 * tw), o = e % (2 * tw);
             const int64_t jr = s0 + p;
-            syz[p][o] = jr < je ? (float)tab[jr * stride + tw + o] : 0.0f;
+            syz[e] = jr < je ? (float)tab[jr * stride + tw + o] : 0.0f;
         }
         if (tid < KBM_SUB * 3) {
             const int p = tid / 3, d = tid % 3;
@@ -1543,8 +1546,9 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
             const int p = 4 * st + kk;
             float bv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             if (vrow) {
-                const float yc = syz[p][yo], ys = ysg * syz[p][yo + 1];
-                const float zc = syz[p][zo], zs = syz[p][zo + 1];
+                const float* yz = syz + p * 2 * tw;
+                const float yc = yz[yo], ys = ysg * yz[yo + 1];
+                const float zc = yz[zo], zs = yz[zo + 1];
                 const float wc = yc * zc - ys * zs, ws = ys * zc + yc * zs;
                 const float mx = sm[p][0];
                 const float A = fny * sm[p][1] + fnz * sm[p][2];
@@ -2491,9 +2495,12 @@ hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep
 }
 
 hipError_t launch_kbuild_mfma(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
-                              const KRow* rows, const double* tab, const double* mom, double* partial) {
+                              int32_t nc_max, const KRow* rows, const double* tab, const double* mom,
+                              double* partial) {
     if (items <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_kbuild_mfma, dim3(items), dim3(256), 0, s, n, reps, nrep, items, rows, tab, mom, partial);
+    if (nc_max > KBM_NCMAX) return hipErrorInvalidValue;
+    const size_t lds = (size_t)KBM_SUB * (2 * KBM_NX + 4 + 4 * (nc_max + 1)) * sizeof(float);
+    hipLaunchKernelGGL(k_kbuild_mfma, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, tab, mom, partial);
     LAUNCH_CHECK();
 }
 
