@@ -607,7 +607,7 @@ mds_status plan_all(mds_ctx* c) {
         ksm += (m.n + 63) / 64;
         nc_max = std::max(nc_max, R.nc);
         R.kbm_off = (int32_t)kbm;
-        kbm += (R.nk > 0) ? (int64_t)((R.nrowg + 3) / 4) * R.kbm_pass * R.nchunks : 0;
+        kbm += (R.nk > 0) ? (int64_t)((R.nrowg + 3) / 4) * R.kbm_pass * 8 * ((R.nchunks + 7) / 8) : 0;
         std::fill(rep_of.begin() + m.off, rep_of.begin() + m.off + m.n, r);
         R.red_off = (int32_t)c->h_chunks.size();
         for (int64_t b = m.off; b < m.off + m.n; b += RED_CHUNK_PARTICLES) c->h_chunks.push_back(make_int2(r, (int)b));

@@ -1495,10 +1495,16 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
     const int32_t item = blockIdx.x;
     if (item >= items) return;  // uniform over the block
     const Rep& R = reps[find_rep<3>(reps, nrep, item)];
+    // XCD-aware order: blocks are dealt to the eight XCDs by index mod 8, so
+    // all row blocks of chunk ch get indices = ch (mod 8) and the chunk's
+    // tables are fetched into one XCD's L2 (chunks in groups of eight; the
+    // last group's missing chunks return at once)
     const int32_t li = item - R.kbm_off;
-    const int32_t nblk = (R.nrowg + KBM_WAVES - 1) / KBM_WAVES;
-    const int32_t blk = li % nblk, rest = li / nblk;
-    const int32_t pass = rest % R.kbm_pass, ch = rest / R.kbm_pass;
+    const int32_t nblk = (R.nrowg + KBM_WAVES - 1) / KBM_WAVES, nb = nblk * R.kbm_pass;
+    const int32_t grp = li / (8 * nb), rr = li % (8 * nb);
+    const int32_t ch = 8 * grp + (rr & 7), b = rr >> 3;
+    if (ch >= R.nchunks) return;  // uniform over the block
+    const int32_t blk = b % nblk, pass = b / nblk;
     const int wv = wave_id(), lane = lane_id(), tid = threadIdx.x;
     const int kk = lane >> 4, col = lane & 15;
     const int nc = R.nc, tw = 2 * (nc + 1);
