@@ -166,7 +166,10 @@ mds_status mds_init_coords_from_sim(const char* path, int64_t n, double* pos, do
 /* ----------------------------------------------------------------------- */
 typedef struct mds_ctx mds_ctx;
 
-/* Create a context on HIP device `device` (one HIP stream per context). */
+/* Create a context on HIP device `device`. A context owns two HIP streams:
+ * its main stream, on which every call's work is ordered, and a side stream
+ * that takes the k-space chain beside the pair kernels within an evaluation
+ * (joined by events; see mds_set_kspace_overlap). */
 mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out);
 void mds_destroy(mds_ctx* ctx);
 const char* mds_last_error(const mds_ctx* ctx);
@@ -217,6 +220,21 @@ mds_status mds_get_state(mds_ctx* ctx, double* pos, double* pos_mm, double* pos_
 mds_status mds_set_noise(mds_ctx* ctx, const double* therm_torque, const double* therm_field,
                          const double* therm_force);
 mds_status mds_use_philox(mds_ctx* ctx, uint64_t seed, uint64_t counter);
+
+/* The thermal draws of the last step (or the injected ones), scaled as
+ * DefThermFluct leaves them in thermTorque, thermField, thermForce
+ * (src/simulation.cpp:563-603; src/typedefs.h WorkingVar_S). Any output may
+ * be NULL. */
+mds_status mds_get_noise(mds_ctx* ctx, double* therm_torque, double* therm_field, double* therm_force);
+/* Host copy of the in-kernel generator (no counterpart in the reference,
+ * whose draws come from EigenRand's Mersenne twister, src/simulation.cpp:570,
+ * 584, 596): one Philox4x32-10 block, out = Philox(ctr, key), and the nine
+ * standard normals of particle `index` (within its replica) at the replica's
+ * accepted step `step` (torque xyz, field xyz, force xyz before scaling by
+ * sigma_T, sigma_B, sigma_F), computed by the same code as the noise kernel
+ * (the device's sincospi may differ from the host's sin/cos by an ulp). */
+void mds_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void mds_philox_normals(uint64_t seed, int64_t index, uint64_t step, double* z9);
 
 /* CompInteractions (src/simulation.cpp:628-680) at time t: moves overlapping
  * pairs, returns force (incl. thermal force), demag field, pressure sums

@@ -95,6 +95,9 @@ SIGNATURES = [
     ("mds_get_state", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     ("mds_set_noise", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     ("mds_use_philox", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
+    ("mds_get_noise", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("mds_philox4x32_10", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("mds_philox_normals", None, [C.c_uint64, C.c_int64, C.c_uint64, _dp]),
     ("mds_comp_interactions", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
     ("mds_comp_demag_field", C.c_int, [C.c_void_p, _dp, _dp]),
     ("mds_rk_coeffs", C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp]),
@@ -221,6 +224,23 @@ def init_coords_from_sim(path: str, n: int):
     return rm(pos, n), rm(mm, n), rm(ea, n), t.value
 
 
+def philox4x32_10(ctr, key) -> tuple:
+    """One Philox4x32-10 block of the host copy of the in-kernel generator."""
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    o = (C.c_uint32 * 4)()
+    lib().mds_philox4x32_10(c, k, o)
+    return tuple(int(x) for x in o)
+
+
+def philox_normals(seed: int, index: int, step: int) -> np.ndarray:
+    """The nine unscaled normals (torque, field, force; xyz each) the noise
+    kernel draws for particle `index` of a replica at its step `step`."""
+    z = np.zeros(9)
+    lib().mds_philox_normals(int(seed), int(index), int(step), _p(z))
+    return z
+
+
 def peak_probe(device: int = 0) -> dict:
     """Measured FP64 / FP32 vector FMA TFLOP/s and device copy GB/s."""
     a, b, c = C.c_double(), C.c_double(), C.c_double()
@@ -311,6 +331,13 @@ class Context:
 
     def use_philox(self, seed, counter=0):
         self._chk(self._L.mds_use_philox(self.h, int(seed), int(counter)), "use_philox")
+
+    def get_noise(self):
+        """(torque, field, force) thermal draws of the last step, N x 3."""
+        n = self.n
+        a, b, c = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+        self._chk(self._L.mds_get_noise(self.h, _p(a), _p(b), _p(c)), "get_noise")
+        return rm(a, n), rm(b, n), rm(c, n)
 
     def comp_interactions(self):
         n = self.n

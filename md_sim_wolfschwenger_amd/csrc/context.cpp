@@ -21,6 +21,7 @@
 
 #include "kernels.h"
 #include "mdsim_capi.h"
+#include "philox.h"
 
 using namespace mds;
 
@@ -1573,6 +1574,30 @@ mds_status mds_use_philox(mds_ctx* c, uint64_t seed, uint64_t counter) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->drop_graph();
     return MDS_OK;
+}
+
+mds_status mds_get_noise(mds_ctx* c, double* tt, double* tb, double* tf) {
+    if (!c || c->n <= 0) return MDS_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t n3 = 3 * (size_t)c->n;
+    mds_status st;
+    if (tt && (st = download(c, tt, c->n_t.p, n3)) != MDS_OK) return st;
+    if (tb && (st = download(c, tb, c->n_b.p, n3)) != MDS_OK) return st;
+    if (tf && (st = download(c, tf, c->n_f.p, n3)) != MDS_OK) return st;
+    return MDS_OK;
+}
+
+void mds_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox4x32_10(c, key[0], key[1]);
+    for (int k = 0; k < 4; ++k) out[k] = c[k];
+}
+
+void mds_philox_normals(uint64_t seed, int64_t index, uint64_t step, double* z9) {
+    double z[10];
+    philox_normals(seed, index, step, z);
+    for (int k = 0; k < 9; ++k) z9[k] = z[k];
 }
 
 mds_status mds_comp_interactions(mds_ctx* c, double* force, double* field, double* press, double* pos_after) {

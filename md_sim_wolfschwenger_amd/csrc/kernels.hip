@@ -10,6 +10,7 @@
 
 #include "erfcx_coef.h"
 #include "kernels.h"
+#include "philox.h"
 
 namespace mds {
 
@@ -2103,24 +2104,9 @@ __global__ __launch_bounds__(256) void k_translate(int64_t n, const Rep* __restr
 }
 
 // ------------------------------------------------------------------ noise
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
-// easy as 1, 2, 3", SC'11) keyed by the seed, counter (particle, step, call);
-// Box-Muller turns each pair of 53-bit uniforms into two standard normals.
-__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
-    for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
-        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
-        c[0] = n0;
-        c[1] = (uint32_t)p1;
-        c[2] = n2;
-        c[3] = (uint32_t)p0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-}
-
+// Philox4x32-10 keyed by the replica's seed, counter (particle within the
+// replica, call, replica's step), Box-Muller normals (philox.h, shared with
+// the host copy that the published known-answer vectors check).
 __global__ __launch_bounds__(256) void k_noise(int64_t n, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
                                                const uint64_t* __restrict__ rep_step, double dt,
                                                const double* __restrict__ zeta_rot,
@@ -2138,19 +2124,7 @@ __global__ __launch_bounds__(256) void k_noise(int64_t n, const Rep* __restrict_
     const uint64_t seed = R.seed;
     const int64_t li = i - R.off;
     double z[10];
-    for (int call = 0; call < 5; ++call) {
-        uint32_t c[4] = {(uint32_t)li, (uint32_t)(li >> 32) ^ ((uint32_t)call << 24), (uint32_t)step,
-                         (uint32_t)(step >> 32)};
-        philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-        const uint64_t a = ((uint64_t)c[0] << 32) | c[1], b = ((uint64_t)c[2] << 32) | c[3];
-        const double u1 = ((a >> 11) + 1) * 0x1.0p-53;  // (0, 1]
-        const double u2 = (b >> 11) * 0x1.0p-53;        // [0, 1)
-        const double rr = sqrt(-2.0 * log(u1));
-        double s, co;
-        sincospi(2.0 * u2, &s, &co);
-        z[2 * call] = rr * co;
-        z[2 * call + 1] = rr * s;
-    }
+    philox_normals(seed, li, step, z);
     // DefThermFluct, src/simulation.cpp:565-602: torque, field, force
     const double sT = en_t ? sqrt((2.0 * kT / dt) * zeta_rot[i]) : 0.0;
     const double sB = en_b ? sqrt(2.0 * kT * alpha_g / (gyro * sat_mag * dt) * (1.0 / vol_mag[i])) : 0.0;

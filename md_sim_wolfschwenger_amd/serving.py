@@ -61,7 +61,7 @@ def footprint_bytes(n: int, nc: int) -> int:
     nk = int(2.2 * nc ** 3) + 1
     nseg = max(1, nk // 8)
     nsegw = (nseg + 63) // 64
-    chunk = min(1024, max(32, (n * nsegw + 4095) // 4096))
+    chunk = min(1024, max(32, (n * nsegw + 8191) // 8192))  # the planner's chunking
     nchunks = (n + chunk - 1) // chunk
     kspace = nk * 8 * (3 + 2 * nchunks)
     cells = 64 * n

@@ -3,6 +3,12 @@
 Each step is started from the reference's own state and thermal draws for that
 step (tests/golden/make_golden.py), so every comparison is per evaluation /
 per step. Tolerance: 1e-10 relative (max-norm over the array).
+
+Every case runs with threads = 1 (the stored pair list, as the reference) and
+threads = 4: without an overlap move and without the coating potential, the
+oracle then streams the pairs over OpenMP threads instead of storing them
+(the form that checks the device at N = 1e4 .. 1e5), so that form is held to
+the same vectors.
 """
 import numpy as np
 import pytest
@@ -12,10 +18,11 @@ from refcfg import CASES, contact_scales, load, oracle_system, rel_err
 TOL = 1e-10
 
 
+@pytest.mark.parametrize("threads", [1, 4])
 @pytest.mark.parametrize("case", CASES)
-def test_comp_interactions_matches_reference(case):
+def test_comp_interactions_matches_reference(case, threads):
     d = load(case)
-    s = oracle_system(d)
+    s = oracle_system(d, threads=threads)
     f_al, b_al, _ = contact_scales(d)
     F, B, P, pos = s.comp_interactions(d["s0_pos_in"], d["s0_mm_in"], d["s0_nF"])
     assert rel_err(F, d["s0_force"], f_al) < TOL
@@ -26,10 +33,11 @@ def test_comp_interactions_matches_reference(case):
     assert len(pl["i"]) == int(d["s0_pairs"][0])
 
 
+@pytest.mark.parametrize("threads", [1, 4])
 @pytest.mark.parametrize("case", CASES)
-def test_steps_match_reference(case):
+def test_steps_match_reference(case, threads):
     d = load(case)
-    s = oracle_system(d)
+    s = oracle_system(d, threads=threads)
     _, b_al, _ = contact_scales(d)
     for k in range(int(d["steps"][0])):
         p = f"s{k}_"
