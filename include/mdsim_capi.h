@@ -286,6 +286,19 @@ mds_status mds_last_observables(mds_ctx* ctx, double* obs);
  * an unreadable, truncated or corrupted file (FNV-1a 64 checksum). */
 mds_status mds_save_checkpoint(mds_ctx* ctx, const char* path, double t, double dt);
 mds_status mds_load_checkpoint(mds_ctx* ctx, const char* path, double* t, double* dt);
+/* Allocation-free <m_z> (no counterpart in the reference; for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). pos_mm: device address of one
+ * system's moments, column-major N x 3; mz: device address of one double,
+ * which gets the mean z component in the fixed chunk order of
+ * mds_replica_mz (src/simulation.cpp:514-515), bit for bit. Enqueued on
+ * `stream` (a hipStream_t; NULL: the null stream) of the current device,
+ * asynchronously. The chunk partials live in a static 512 KiB device array
+ * that each device holds until the process ends, so the call allocates
+ * nothing; n is limited to 2^27 (MDS_ECAPACITY beyond, MDS_EINVAL for
+ * n <= 0 or a NULL pointer). Calls that overlap on one device race on that
+ * array. */
+mds_status mds_mean_z_noalloc(const double* pos_mm, int64_t n, double* mz, void* stream);
 /* Wait for queued work. */
 mds_status mds_synchronize(mds_ctx* ctx);
 

@@ -96,6 +96,7 @@ SIGNATURES = [
     ("mds_set_noise", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     ("mds_use_philox", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     ("mds_get_noise", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("mds_mean_z_noalloc", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     ("mds_philox4x32_10", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("mds_philox_normals", None, [C.c_uint64, C.c_int64, C.c_uint64, _dp]),
     ("mds_comp_interactions", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),

@@ -1576,6 +1576,12 @@ mds_status mds_use_philox(mds_ctx* c, uint64_t seed, uint64_t counter) {
     return MDS_OK;
 }
 
+mds_status mds_mean_z_noalloc(const double* pos_mm, int64_t n, double* mz, void* stream) {
+    if (!pos_mm || !mz || n <= 0) return MDS_EINVAL;
+    if (n > NOALLOC_MAX_PARTICLES) return MDS_ECAPACITY;
+    return launch_mean_z_noalloc((hipStream_t)stream, n, pos_mm, mz) == hipSuccess ? MDS_OK : MDS_EDEVICE;
+}
+
 mds_status mds_get_noise(mds_ctx* c, double* tt, double* tb, double* tf) {
     if (!c || c->n <= 0) return MDS_EINVAL;
     HIPCHK(hipSetDevice(c->device));

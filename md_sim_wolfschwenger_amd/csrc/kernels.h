@@ -169,6 +169,11 @@ hipError_t launch_rep_sum3(hipStream_t s, const Rep* reps, int32_t nrep, const i
 hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, const int2* chunks, int32_t nch, int64_t n,
                              const double* v, double* part, double* out, const int32_t* counter, double* series,
                              int32_t series_cap);
+// <m_z> of one system of n particles (moments column-major N x 3 on the
+// device) into out[0], in the chunk order of launch_rep_mean_z, with the
+// chunk partials in the library's static device array (no allocation).
+constexpr int64_t NOALLOC_MAX_PARTICLES = (int64_t)1 << 27;
+hipError_t launch_mean_z_noalloc(hipStream_t s, int64_t n, const double* mm, double* out);
 
 // real space: one block per work item (max_work blocks, surplus blocks exit)
 // Real-space pair sums (force pass: force, field, pressure; else field):

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "erfcx_coef.h"
 #include "kernels.h"
 #include "philox.h"
@@ -515,6 +517,34 @@ __global__ void k_rep_mean_z(const Rep* __restrict__ reps, int32_t nrep, const d
         const int32_t k = *counter;
         if (k < series_cap) series[(int64_t)k * nrep + r] = m;
     }
+}
+
+// ------------------------------------------------------------------ allocation-free <m_z>
+// Scratch of the allocation-free <m_z> entry point: one partial sum per chunk
+// of RED_CHUNK particles (512 KiB). Every device holds its own copy for the
+// life of the process; host code reaches it only through the address that
+// hipGetSymbolAddress returns with that device current.
+constexpr int64_t NOALLOC_MAX_CHUNKS = NOALLOC_MAX_PARTICLES / RED_CHUNK_PARTICLES;
+__device__ __align__(256) double scratch[NOALLOC_MAX_CHUNKS];
+
+// The z column of chunk blockIdx.x, summed as k_red_part sums it.
+__global__ __launch_bounds__(256) void k_red_part_z(int64_t n, const double* __restrict__ mm,
+                                                    double* __restrict__ part) {
+    const int64_t b = (int64_t)blockIdx.x * RED_CHUNK, e = min(b + RED_CHUNK, n);
+    double a = 0;
+    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) a += mm[2 * n + i];
+    __shared__ double r3[3];
+    block_sum3<256>(a, 0.0, 0.0, r3);
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = r3[0];
+}
+
+// The chunks' sums in order, over n (as k_rep_mean_z).
+__global__ void k_mean_z_final(int64_t nch, int64_t n, const double* __restrict__ part, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0;
+    for (int64_t k = 0; k < nch; ++k) a += part[k];
+    out[0] = a / (double)n;
 }
 
 // ------------------------------------------------------------------ real space
@@ -2389,6 +2419,33 @@ hipError_t launch_rep_mean_z(hipStream_t s, const Rep* reps, int32_t nrep, const
     hipLaunchKernelGGL(k_red_part, dim3(nch), dim3(256), 0, s, chunks, reps, n, v, 2, 1, part);
     hipLaunchKernelGGL(k_rep_mean_z, dim3((nrep + 63) / 64), dim3(64), 0, s, reps, nrep, (const double*)part, out,
                        counter, series, series_cap);
+    LAUNCH_CHECK();
+}
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[64];  // device address of `scratch`, per device
+}  // namespace
+
+hipError_t launch_mean_z_noalloc(hipStream_t s, int64_t n, const double* mm, double* out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void* p = nullptr;
+            e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess) return e;
+            g_scratch_ptr[dev] = (double*)p;
+        }
+        part = g_scratch_ptr[dev];
+    }
+    const int64_t nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+    hipLaunchKernelGGL(k_red_part_z, dim3((uint32_t)nch), dim3(256), 0, s, n, mm, part);
+    hipLaunchKernelGGL(k_mean_z_final, dim3(1), dim3(64), 0, s, nch, n, (const double*)part, out);
     LAUNCH_CHECK();
 }
 
