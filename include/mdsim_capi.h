@@ -259,6 +259,17 @@ mds_status mds_step(mds_ctx* ctx, double* t, double* dt, double* dt_used, double
  * (may be NULL) receives <m_z> after each step. */
 mds_status mds_run(mds_ctx* ctx, int64_t steps, double* t, double* dt, double* mz_series);
 
+/* Graph-replayed runs (no counterpart in the reference, whose loop checks
+ * every step in line): mds_run replays unchecked steps in chunks of at most
+ * `steps` (default 128) and reads the flags after each chunk. A chunk that
+ * raised the overlap-margin or a capacity flag is rolled back to its start
+ * (positions, orientations, clocks, Philox counters) and redone eagerly with
+ * every step checked, widening the contact-search margin where a move needs
+ * it; a step that still fails ends the run with MDS_EOVERLAP/MDS_ECAPACITY,
+ * the state it started from, and t advanced to it. mds_chunks_redone counts
+ * the chunks redone since the context was created. */
+mds_status mds_set_check_interval(mds_ctx* ctx, int64_t steps);
+int64_t mds_chunks_redone(const mds_ctx* ctx);
 /* Transport observables of EvalTransCoeffs (src/postprocessing.cpp:167-187,
  * initialised at src/simulation.cpp:108-109). on != 0: the current positions
  * become the reference and the first unwrapped positions, and every later

@@ -96,6 +96,8 @@ SIGNATURES = [
     ("mds_set_noise", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     ("mds_use_philox", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     ("mds_get_noise", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("mds_set_check_interval", C.c_int, [C.c_void_p, C.c_int64]),
+    ("mds_chunks_redone", C.c_int64, [C.c_void_p]),
     ("mds_mean_z_noalloc", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     ("mds_philox4x32_10", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("mds_philox_normals", None, [C.c_uint64, C.c_int64, C.c_uint64, _dp]),
@@ -524,6 +526,13 @@ class Context:
     def set_kspace_overlap(self, on=True):
         """S(k) on a second stream beside the pair kernels (default on)."""
         self._chk(self._L.mds_set_kspace_overlap(self.h, 1 if on else 0), "set_kspace_overlap")
+
+    def set_check_interval(self, steps):
+        """Graph-replayed steps per checked chunk (see mds_set_check_interval)."""
+        self._chk(self._L.mds_set_check_interval(self.h, int(steps)), "set_check_interval")
+
+    def chunks_redone(self):
+        return int(self._L.mds_chunks_redone(self.h))
 
     def set_graph(self, on=True):
         self._chk(self._L.mds_set_graph(self.h, 1 if on else 0), "set_graph")

@@ -250,6 +250,10 @@ struct mds_ctx {
     std::vector<KSeg> h_segs;
     std::vector<KPack> h_pack;
     int32_t nk_all = 0, kb_items = 0, kbm_items = 0, ksm_items = 0, nc_max = 0;
+    // k space per replica (Rep::kmode): replicas on the matrix cores and their
+    // largest n_c (sizes the kernels' LDS), and whether any is on the vector kernels
+    int32_t nc_mfma_build = 0, nc_mfma_scatter = 0;
+    bool any_mfma_build = false, any_mfma_scatter = false, any_vec_build = false, any_vec_scatter = false;
     DevBuf<KRow> rows;
     DevBuf<KSeg> segs;
     DevBuf<KPack> pack_desc;
@@ -277,6 +281,14 @@ struct mds_ctx {
     double total_ms[KC_COUNT] = {};
     bool count_work = false;
     int64_t last_events = 0;
+    // graph-replayed runs: steps per checked chunk, the chunk-start state
+    // (pos, mm, ea, unwrapped positions; replica clocks and step counters) and
+    // the number of chunks redone after a raised flag
+    int64_t check_every = 128;
+    DevBuf<double> snap;
+    DevBuf<double> snap_t;
+    DevBuf<uint64_t> snap_step;
+    int64_t chunks_redone = 0;
 
     hipEvent_t get_event() {
         if (!pool.empty()) {
@@ -578,6 +590,8 @@ mds_status plan_all(mds_ctx* c) {
     c->h_chunks.clear();
     int64_t cells = 0, fcells = 0, work_cap = 0, tab = 0, part = 0, kb = 0, kbm = 0, ksm = 0, kall = 0;
     int32_t nc_max = 0;
+    int32_t nc_mfma_build = 0, nc_mfma_scatter = 0;
+    bool any_mfma_build = false, any_mfma_scatter = false, any_vec_build = false, any_vec_scatter = false;
     for (int r = 0; r < c->nrep; ++r) {
         const RepMeta& m = c->meta[r];
         Rep& R = c->h_rep[r];
@@ -607,6 +621,17 @@ mds_status plan_all(mds_ctx* c) {
         R.ksm_off = (int32_t)ksm;
         ksm += (m.n + 63) / 64;
         nc_max = std::max(nc_max, R.nc);
+        // the mixed path takes each replica's k space to the matrix cores when
+        // its n_c fits their LDS tiles; the choice is the replica's own, so
+        // its results do not depend on the other replicas of the context
+        const bool mixed = c->cfg.precision == MDS_PREC_MIXED;
+        R.kmode = (mixed && R.nc <= KBM_NCMAX ? 1 : 0) | (mixed && R.nc <= KSM_NCMAX ? 2 : 0);
+        if (R.nk > 0) {
+            if (R.kmode & 1) nc_mfma_build = std::max(nc_mfma_build, R.nc), any_mfma_build = true;
+            else any_vec_build = true;
+        }
+        if (R.kmode & 2) nc_mfma_scatter = std::max(nc_mfma_scatter, R.nc), any_mfma_scatter = true;
+        else any_vec_scatter = true;
         R.kbm_off = (int32_t)kbm;
         kbm += (R.nk > 0) ? (int64_t)((R.nrowg + 3) / 4) * R.kbm_pass * 8 * ((R.nchunks + 7) / 8) : 0;
         std::fill(rep_of.begin() + m.off, rep_of.begin() + m.off + m.n, r);
@@ -628,6 +653,12 @@ mds_status plan_all(mds_ctx* c) {
     c->kbm_items = (int32_t)kbm;
     c->ksm_items = (int32_t)ksm;
     c->nc_max = nc_max;
+    c->nc_mfma_build = nc_mfma_build;
+    c->nc_mfma_scatter = nc_mfma_scatter;
+    c->any_mfma_build = any_mfma_build;
+    c->any_mfma_scatter = any_mfma_scatter;
+    c->any_vec_build = any_vec_build;
+    c->any_vec_scatter = any_vec_scatter;
     HIPCHK(c->cnt_main.ensure(cells));
     HIPCHK(c->start_main.ensure(cells));
     HIPCHK(c->end_main.ensure(cells));
@@ -844,10 +875,10 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
         if (c->nk_all > 0) {
             {
                 mds_ctx::Span sp(c, KC_KBUILD);
-                if (c->cfg.precision == MDS_PREC_MIXED && c->nc_max <= KBM_NCMAX)
-                    HIPCHK(launch_kbuild_mfma(ks, n, c->reps.p, c->nrep, c->kbm_items, c->nc_max, c->rows.p, c->tab.p,
-                                              c->mom.p, c->kpartial.p));
-                else
+                if (c->any_mfma_build)
+                    HIPCHK(launch_kbuild_mfma(ks, n, c->reps.p, c->nrep, c->kbm_items, c->nc_mfma_build, c->rows.p,
+                                              c->tab.p, c->mom.p, c->kpartial.p));
+                if (c->any_vec_build)
                     HIPCHK(launch_kbuild(ks, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p,
                                          c->kpartial.p));
             }
@@ -858,11 +889,11 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
             }
         }
         mds_ctx::Span sp(c, force_pass ? KC_KSCATTER_FORCE : KC_KSCATTER_FIELD);
-        if (c->cfg.precision == MDS_PREC_MIXED && c->nc_max <= KSM_NCMAX)
-            HIPCHK(launch_kscatter_mfma(ks, force_pass, n, c->reps.p, c->nrep, c->ksm_items, c->nc_max, c->rows.p,
-                                        c->pack.p, c->tab.p, c->mom.p, c->msum.p, force_pass ? c->n_f.p : nullptr,
-                                        c->force.p, c->field.p));
-        else
+        if (c->any_mfma_scatter)
+            HIPCHK(launch_kscatter_mfma(ks, force_pass, n, c->reps.p, c->nrep, c->ksm_items, c->nc_mfma_scatter,
+                                        c->rows.p, c->pack.p, c->tab.p, c->mom.p, c->msum.p,
+                                        force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
+        if (c->any_vec_scatter)
             HIPCHK(launch_kscatter(ks, force_pass, n, c->reps.p, c->rep_of.p, c->rows.p, c->pack.p, c->tab.p,
                                    c->mom.p, c->msum.p, force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
         return MDS_OK;
@@ -1710,6 +1741,31 @@ mds_status mds_step(mds_ctx* c, double* t, double* dt, double* dt_used, double* 
 }
 
 namespace {
+// Save (restore = false) or restore the state a graph-replayed chunk starts
+// from: positions, moments, easy axes, the unwrapped positions of the
+// observables, and each replica's clock and step counter.
+mds_status snapshot(mds_ctx* c, bool restore) {
+    const size_t n3 = 3 * (size_t)c->n, nr = (size_t)c->nrep;
+    const size_t need = 3 * n3 + (c->observe ? n3 : 0);
+    if (!restore) {
+        HIPCHK(c->snap.ensure(need));
+        HIPCHK(c->snap_t.ensure(nr));
+        HIPCHK(c->snap_step.ensure(nr));
+    }
+    double* live[4] = {c->pos.p, c->mm.p, c->ea.p, c->observe ? c->true_pos.p : nullptr};
+    for (int k = 0; k < 4; ++k) {
+        if (!live[k]) continue;
+        double* sp = c->snap.p + k * n3;
+        HIPCHK(hipMemcpyAsync(restore ? live[k] : sp, restore ? sp : live[k], n3 * sizeof(double),
+                              hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(restore ? c->rep_t.p : c->snap_t.p, restore ? c->snap_t.p : c->rep_t.p, nr * sizeof(double),
+                          hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(restore ? c->rep_step.p : c->snap_step.p, restore ? c->snap_step.p : c->rep_step.p,
+                          nr * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
+    return MDS_OK;
+}
+
 mds_status run_impl(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz_series, double* obs) {
     mds_status st = check_ready(c);
     if (st != MDS_OK) return st;
@@ -1732,21 +1788,56 @@ mds_status run_impl(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz
         return read_flags(c);
     }
     const double h = *dt;
-    // steps in chunks of the device <m_z> buffer; the first step checks the
-    // overlap margin (and clears the flags), later ones replay a graph when enabled
+    // Steps in chunks of at most check_every (and of the device <m_z>
+    // buffer). A chunk's first step checks the overlap margin (and clears the
+    // flags); later ones replay a graph when enabled, unchecked, so the flags
+    // are read after the chunk. A chunk that raised the overlap-margin or a
+    // capacity flag is rolled back to its start and redone eagerly, every
+    // step checked (a wider margin where a move needs it); its Philox draws
+    // are functions of the restored step counters, so the redone chunk is the
+    // run that was checked throughout. A step that still raises a flag ends
+    // the run with the state it started from.
     int64_t done = 0;
     while (done < steps) {
-        const int64_t m = std::min<int64_t>(kSeriesCap, steps - done);
+        const int64_t m = std::min<int64_t>(std::min<int64_t>(kSeriesCap, c->check_every), steps - done);
+        const bool guard = c->use_graph && m > 1;
+        if (guard && (st = snapshot(c, false)) != MDS_OK) return st;
         HIPCHK(launch_fill_i32(c->stream, c->series_counter.p, 1, 0));
         for (int64_t k = 0; k < m; ++k) {
-            const bool first = done + k == 0;
-            if (first || !c->use_graph) {
-                if ((st = fixed_step_body(c, h, first)) != MDS_OK) return st;
+            if (k == 0 || !c->use_graph) {
+                if ((st = fixed_step_body(c, h, k == 0)) != MDS_OK) return st;
             } else {
                 if (!c->gexec || c->graph_h != h) {
                     if ((st = capture_step(c, h)) != MDS_OK) return st;
                 }
                 HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+            }
+        }
+        if (guard) {
+            int32_t f = 0;
+            HIPCHK(hipMemcpyAsync(&f, c->err_flag.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (f & 64) return read_flags(c);
+            if (f) {
+                ++c->chunks_redone;
+                if ((st = snapshot(c, true)) != MDS_OK) return st;
+                c->drop_graph();
+                HIPCHK(launch_fill_i32(c->stream, c->series_counter.p, 1, 0));
+                for (int64_t k = 0; k < m; ++k) {
+                    if ((st = snapshot(c, false)) != MDS_OK) return st;
+                    if ((st = fixed_step_body(c, h, true)) != MDS_OK) return st;
+                    HIPCHK(hipMemcpyAsync(&f, c->err_flag.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+                    HIPCHK(hipStreamSynchronize(c->stream));
+                    if (f) {
+                        st = read_flags(c);
+                        mds_status rs = snapshot(c, true);
+                        HIPCHK(hipStreamSynchronize(c->stream));
+                        for (int64_t q = 0; q < done + k; ++q) *t += h;
+                        if (rs != MDS_OK) return rs;
+                        return fail(c, st, std::string(mds_last_error(c)) + " at step " + std::to_string(done + k) +
+                                               " of the run (the state is the one that step started from)");
+                    }
+                }
             }
         }
         if (mz_series)
@@ -1762,6 +1853,14 @@ mds_status run_impl(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz
     return read_flags(c);
 }
 }  // namespace
+
+mds_status mds_set_check_interval(mds_ctx* c, int64_t steps) {
+    if (!c || steps < 1) return MDS_EINVAL;
+    c->check_every = steps;
+    return MDS_OK;
+}
+
+int64_t mds_chunks_redone(const mds_ctx* c) { return c ? c->chunks_redone : 0; }
 
 mds_status mds_run(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz_series) {
     return run_impl(c, steps, t, dt, mz_series, nullptr);
@@ -2293,7 +2392,7 @@ int64_t mds_device_bytes(const mds_ctx* c) {
                     &c->ea, &c->n_t, &c->n_b, &c->n_f, &c->pos0, &c->pos_cur, &c->force, &c->field, &c->mom, &c->kmm,
                     &c->kea, &c->pt[0], &c->pt[1], &c->new_mm, &c->new_ea, &c->ev_pos, &c->kpartial,
                     &c->tab, &c->press_partial, &c->series, &c->msum, &c->mz_out, &c->rep_t, &c->err_partial,
-                    &c->true_pos, &c->pos_ref, &c->obs_d2, &c->obs_series, &c->obs_last})
+                    &c->true_pos, &c->pos_ref, &c->obs_d2, &c->obs_series, &c->obs_last, &c->snap, &c->snap_t})
         add(d->n, sizeof(double));
     add(c->pair_acc.n + c->order_tmp.n / 2, sizeof(uint64_t));
     for (auto* d : {&c->rep_of, &c->cell_rep, &c->key_main, &c->key_fine, &c->cnt_main, &c->start_main, &c->end_main,
@@ -2303,7 +2402,7 @@ int64_t mds_device_bytes(const mds_ctx* c) {
         add(d->n, sizeof(int32_t));
     add(c->rec_pos.n + c->rec_mom.n, sizeof(double4));
     add(c->rec_pos32.n + c->rec_mom32.n, sizeof(float4));
-    add(c->ev_key.n + c->rep_step.n, sizeof(uint64_t));
+    add(c->ev_key.n + c->rep_step.n + c->snap_step.n, sizeof(uint64_t));
     add(c->stencil.n, sizeof(int4));
     add(c->rows.n, sizeof(KRow));
     add(c->segs.n, sizeof(KSeg));

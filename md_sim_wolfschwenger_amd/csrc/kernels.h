@@ -52,7 +52,8 @@ struct Rep {
     float al_s, al2_s, al4_s, pad_;
     int32_t nc, row_off, nrows, seg_off, nseg, k_off, nk, kchunk, nchunks, kb_off, nsegw, nrowg;
     int32_t kbm_off, kbm_pass;   // matrix-core S(k) build: first work item, passes of KBM_NX |nx| values
-    int32_t ksm_off, pad4_;      // matrix-core scatter: first block (64 particles each)
+    int32_t ksm_off;             // matrix-core scatter: first block (64 particles each)
+    int32_t kmode;               // k space of this replica on the matrix cores: bit 0 S(k) build, bit 1 scatter
     int32_t pk_off, npk;         // slice of the packed scatter table
     int32_t red_off, nred;       // slice of the reduction chunk table
     int32_t ksplit[4];           // first row of each quarter of the scatter table (by entries)

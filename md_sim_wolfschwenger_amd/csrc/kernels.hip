@@ -554,8 +554,8 @@ __global__ void k_mean_z_final(int64_t nch, int64_t n, const double* __restrict_
 // short-range forces (src/interaction.cpp:766-832).
 //
 // One block of PAIR_SLICES wavefronts takes up to 64 particles of one cell
-// (the i particles, one per lane; their positions and moments are shared in
-// LDS) and each wavefront one interleaved slice of the half stencil: the cell
+// (the i particles, one per lane; the FP64 kernels share their positions and
+// moments in LDS, the mixed kernel reads them through L1) and each wavefront one interleaved slice of the half stencil: the cell
 // itself (pairs j > i in cell-sorted order) and the neighbour offsets that
 // are positive in (z, y, x) order, so every pair within r_c is met exactly
 // once. Each candidate j is tested by all lanes (a cheap FP32 distance test);
@@ -855,15 +855,22 @@ template <bool FORCE, bool COATING, bool MIXED>
 __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __restrict__ rec_mom, const PairParams& pp,
                                            const IChunk<MIXED, COATING>& C, IWave<FORCE>& W, int32_t entry,
                                            bool active, bool leader, bool moved_any, int32_t jb, double press[3],
-                                           int64_t& npairs) {
+                                           int64_t& npairs, int32_t s0) {
     constexpr int FB = FORCE ? 3 : 0;  // first field component of the accumulators
     active = active && entry >= 0;
     const int il = entry & 63;
     const int32_t j = (entry >> 6) & 0x1FFFFFF;  // padding: 0
     const int js = j - jb;
-    const double4 pi = C.ipos[il];
+    // the mixed kernel reads its i records through L1 (the chunk's 64 records
+    // stay cached), which frees the LDS for the accumulators; the FP64
+    // kernels, ALU-bound, keep them in LDS
+    constexpr bool kIGlobal = MIXED && !COATING;
+    const double4 pi = kIGlobal ? g.rec_pos[s0 + il] : C.ipos[il];
     const double4 pj = g.rec_pos[j];
-    const auto mi = C.imom[il];
+    const auto mi = [&] {
+        if constexpr (kIGlobal) return g.rec_mom32[s0 + il];
+        else return C.imom[il];
+    }();
     const auto mj = [&] {
         if constexpr (MIXED && !COATING) return g.rec_mom32[j];
         else return rec_mom[j];
@@ -1002,7 +1009,7 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
 template <bool FORCE, bool COATING, bool MIXED>
 __device__ __forceinline__ void pair_flush(const Geometry& g, const double4* __restrict__ rec_mom,
                                            const PairParams& pp, const IChunk<MIXED, COATING>& C, IWave<FORCE>& W,
-                                           int qn, bool moved_any, int32_t jb, double press[3], int64_t& npairs) {
+                                           int qn, bool moved_any, int32_t jb, double press[3], int64_t& npairs, int32_t s0) {
     wave_sync();
     const int lane = lane_id();
     const int nq = qn >> 2;
@@ -1011,7 +1018,7 @@ __device__ __forceinline__ void pair_flush(const Geometry& g, const double4* __r
         const int qi = (lane >> 2) * nb + b;
         const bool act = qi < nq;
         pair_dense<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, act ? W.q[4 * qi + (lane & 3)] : PAIR_PAD, act,
-                                          act && (lane & 3) == 0, moved_any, jb, press, npairs);
+                                          act && (lane & 3) == 0, moved_any, jb, press, npairs, s0);
     }
     wave_sync();
     for (int p = lane; p < qn; p += WAVE) W.q[p] = PAIR_PAD;
@@ -1047,12 +1054,11 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     double4 pi = make_double4(0, 0, 0, 0);
     if (vi) {
         pi = g.rec_pos[k];
-        if (wv == 0) {
-            C.ipos[lane] = pi;
-            if constexpr (MIXED && !COATING)
-                C.imom[lane] = g.rec_mom32[k];
-            else
+        if constexpr (!(MIXED && !COATING)) {  // the mixed kernel reads them through L1
+            if (wv == 0) {
+                C.ipos[lane] = pi;
                 C.imom[lane] = rec_mom[k];
+            }
         }
     }
     for (int d = 0; d < IW::NA; ++d) {
@@ -1144,14 +1150,14 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
                     if (hit) W.q[qn + mbcnt(m)] = ((jb + t) << 6) | lane;
                     qn += (__popcll(m) + 3) & ~3;
                     if (qn >= PAIR_QFLUSH) {
-                        pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, press, npairs);
+                        pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, press, npairs, s0);
                         qn = 0;
                     }
                 }
             }
             if (any) {
                 // window done: its j sums go to the global sums
-                if (qn > 0) pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, press, npairs);
+                if (qn > 0) pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, press, npairs, s0);
                 qn = 0;
                 if (lane < cnt)
                     for (int d = 0; d < IW::NA; ++d)
@@ -1427,6 +1433,7 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
     const int32_t item = blockIdx.x;
     if (item >= kb_items) return;
     const Rep& R = reps[find_rep<0>(reps, nrep, item)];
+    if (R.kmode & 1) return;  // this replica's S(k) is built on the matrix cores
     const int32_t li = item - R.kb_off;
     const int32_t segw = li % R.nsegw, ch = li / R.nsegw;
     const KSeg* segs = segs_all + R.seg_off;
@@ -1526,6 +1533,7 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
     const int32_t item = blockIdx.x;
     if (item >= items) return;  // uniform over the block
     const Rep& R = reps[find_rep<3>(reps, nrep, item)];
+    if (!(R.kmode & 1)) return;  // uniform over the block: this replica is built by k_kbuild
     // XCD-aware order: blocks are dealt to the eight XCDs by index mod 8, so
     // all row blocks of chunk ch get indices = ch (mod 8) and the chunk's
     // tables are fetched into one XCD's L2 (chunks in groups of eight; the
@@ -1780,8 +1788,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const int32_t rid = rep_of[ic];
     const int32_t rid0 = __builtin_amdgcn_readfirstlane(rid);
     const bool uniform = __ballot(rid != rid0) == 0;
+    // particles of replicas scattered on the matrix cores are left to k_kscatter_mfma
+    const bool mine = !(reps[rid].kmode & 2);
     double o[6] = {0, 0, 0, 0, 0, 0};
-    if (i < n) {
+    if (i < n && mine) {
         if (uniform) {
             const Rep& R = reps[rid0];
             const int32_t r0 = R.ksplit[wv], r1 = wv < 3 ? R.ksplit[wv + 1] : R.nrows;
@@ -1794,7 +1804,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     }
     for (int d = 0; d < 6; ++d) part[wv][d][lane] = o[d];
     __syncthreads();
-    if (wv != 0 || i >= n) return;
+    if (wv != 0 || i >= n || !mine) return;
     double a[6];
     for (int d = 0; d < 6; ++d) a[d] = ((part[0][d][lane] + part[1][d][lane]) + part[2][d][lane]) + part[3][d][lane];
     const Rep& R = reps[rid];
@@ -1843,6 +1853,7 @@ __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __r
     if (item >= items) return;  // uniform over the block
     const int rid = find_rep<4>(reps, nrep, item);
     const Rep& R = reps[rid];
+    if (!(R.kmode & 2)) return;  // uniform over the block: this replica is scattered by k_kscatter
     const int64_t pb = (int64_t)(item - R.ksm_off) * WAVE;  // replica-local first particle
     const int wv = wave_id(), lane = lane_id(), tid = threadIdx.x;
     const int kk = lane >> 4, col = lane & 15;

@@ -57,7 +57,7 @@ def footprint_bytes(n: int, nc: int) -> int:
     """Device bytes a replica of n particles with k-space cut-off nc adds to
     a context (per-particle arrays, sin/cos tables, S(k) partials, cells),
     with 25 % headroom."""
-    per_particle = 1000 + 48 * (nc + 1)
+    per_particle = 1100 + 48 * (nc + 1)
     nk = int(2.2 * nc ** 3) + 1
     nseg = max(1, nk // 8)
     nsegw = (nseg + 63) // 64

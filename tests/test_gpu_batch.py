@@ -176,3 +176,26 @@ def test_batch_state_bitwise_equal_alone(precision):
         a.run(5, 0.0, 1e-12)
         for x, y in zip(ctx.replica_state(k), a.get_state()):
             assert np.array_equal(x, y), (k, float(np.max(np.abs(x - y))))
+
+
+def test_mixed_kspace_path_is_chosen_per_replica():
+    """The mixed path puts a replica's S(k) build on the matrix cores when its
+    n_c <= 50 and its scatter when n_c <= 32, each replica by its own n_c:
+    replicas with n_c = 58 (both on the vector kernels), 39 (matrix-core
+    build, vector scatter) and <= 22 (both on the matrix cores) in one
+    context give the results each gives alone, bit for bit."""
+    print("mixed k-space path per replica", flush=True)
+    spec = [(600, 0.02, 0.1, 0, 0.03, 41), (800, 0.05, 0.15, 0, 0.0, 42), (700, 0.03, 0.25, 0, 0.06, 43)]
+    sys_ = [make_system(n, phi, rcf, ov, sd, "mixed") for n, phi, rcf, ov, _, sd in spec]
+    assert [s["nc"] > 50 for s in sys_] == [True, False, False] and 32 < sys_[1]["nc"] <= 50 and sys_[2]["nc"] <= 32
+    fields = [f for *_, f, _ in spec]
+    seeds = [5, 6, 7]
+    ctx = batch(sys_, fields, seeds)
+    ctx.use_philox(0, 0)
+    ctx.set_graph(True)
+    ctx.run(3, 0.0, 1e-12)
+    for k, (s, f, sd) in enumerate(zip(sys_, fields, seeds)):
+        a = alone(s, f, sd)
+        a.run(3, 0.0, 1e-12)
+        for x, y in zip(ctx.replica_state(k), a.get_state()):
+            assert np.array_equal(x, y), k
