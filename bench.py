@@ -259,15 +259,16 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, load=0.8, seed=202612
                        "wait_s": float(q.admitted - q.arrival), "isolated_s": float(i)} for q, i in sorted(zip(reqs, iso), key=lambda x: -x[0].latency / x[1])[:3]]}
 
 
-def cpu_baseline(sysinfo, threads):
-    """One Heun step of the same N = 1e5 system with the CPU oracle (the
-    reference's algorithm restated: O(N^2) pair scan, k-space two-pass sums)."""
+def cpu_baseline(sysinfo, threads, name="c3"):
+    """One Heun step of a benchmark system (the bench's N = 1e5 one, or C2)
+    with the CPU oracle (the reference's algorithm restated: O(N^2) pair
+    scan, k-space two-pass sums) on `threads` OpenMP threads."""
     import oracle
     from tests.refcfg import derived, parse_ini
     from md_sim_wolfschwenger_amd import workload
     cfg = sysinfo["cfg"]
     n = sysinfo["n"]
-    c = derived(parse_ini(workload.config_text("c3")))
+    c = derived(parse_ini(workload.config_text(name)))
     c["fixed_dt"] = 1
     r, rh = sysinfo["r"], sysinfo["rh"]
     props = {"r_mag": r, "r_hydr": rh, "vol_mag": 4 / 3 * cfg.my_pi * r ** 3, "mag_mom": sysinfo["mu"],
@@ -282,6 +283,41 @@ def cpu_baseline(sysinfo, threads):
     t1 = time.perf_counter()
     assert out["status"] == 0
     return n / (t1 - t0), t1 - t0
+
+
+def c2_system(mds, workload, seed):
+    cfg, r, rh, L, pos, mm, ea = workload.system("c2", seed=seed)
+    mu = cfg.sat_mag * 4.0 / 3.0 * cfg.my_pi * r ** 3
+    alpha, rc, nc = workload.ewald_for(cfg, mu, rh, L, 0.3)
+    return {"n": len(r), "L": L, "alpha": alpha, "rc": rc, "nc": nc, "cfg": cfg, "r": r, "rh": rh, "pos": pos,
+            "mm": mm, "ea": ea, "mu": mu}
+
+
+def adaptive_leg(mds, workload, precision, steps, seed):
+    """The reference's default loop on C2 (N = 1e4): adaptive Dormand-Prince
+    (SOLVER = 2, src/configuration.ini:22; src/simulation.cpp:796-862 with
+    AdjustTimeStep 912-951), Philox noise, r_c = 0.3 L; one host read of the
+    error norm per attempt. Particle-steps/s over accepted steps."""
+    s = c2_system(mds, workload, seed)
+    text = workload.config_text("c2", seed=seed, fixed=False, precision=precision).replace("SOLVER = 0", "SOLVER = 2")
+    cfg = mds.config_from_string(text)
+    ctx = mds.Context(cfg)
+    ctx.set_particles(s["r"], s["rh"], s["L"])
+    ctx.set_ewald(s["alpha"], s["rc"], s["nc"])
+    ctx.set_state(s["pos"], s["mm"], s["ea"])
+    ctx.use_philox(seed, 0)
+    t, dt = 0.0, cfg.delta_t_init
+    for _ in range(3):
+        t, dt, _, _ = ctx.step(t, dt)
+    ctx.synchronize()
+    t_start, t0 = t, time.perf_counter()
+    for _ in range(steps):
+        t, dt, _, _ = ctx.step(t, dt)
+    el = time.perf_counter() - t0
+    ctx.close()
+    return {"value": s["n"] * steps / el, "unit": "particle-steps/s", "precision": precision, "n": s["n"],
+            "solver": "Dormand-Prince (SOLVER=2), adaptive", "steps": steps, "ms_per_step": el / steps * 1e3,
+            "mean_dt_s": (t - t_start) / steps, "absTol": cfg.abs_tol}
 
 
 def mz_run(args):
@@ -357,7 +393,11 @@ def main():
     ap.add_argument("--no-overlap", action="store_true",
                     help="k-space chain in order on one stream also in the timed run (for profiles whose kernel "
                          "durations match the per-kernel breakdown)")
+    # the GPU host's CPU share is 16 cores per GPU; more threads would only
+    # oversubscribe it
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--adaptive-steps", type=int, default=50,
+                    help="accepted steps of the adaptive Dormand-Prince leg at C2 (0 = skip)")
     ap.add_argument("--replicas", type=int, default=256, help="replicas of the batched-ensemble leg (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--requests", type=int, default=24, help="requests of the serving leg (0 = skip)")
@@ -438,6 +478,20 @@ def main():
                  "torque": rel(np.cross(mom, Bm), np.cross(mom, Bf), 0.0),
                  "norm": "max|mixed - fp64| / max(max|fp64|, contact value), over all particles and components"}
     log(f"mixed vs fp64 error: {mixed_err}")
+    from md_sim_wolfschwenger_amd import capi
+    pred = {}
+    for prec in ("fp64", "mixed"):
+        tl = results[prec]["tiling"]
+        pf, pt = capi.ewald_predicted_errors(cfg, info["mu"], info["rh"], info["L"], tl["alpha"], tl["r_cut"],
+                                             tl["nc"])
+        pred[prec] = {"force_over_contact": pf, "torque_over_contact": pt}
+    log(f"predicted Ewald rms errors: {pred}")
+    adapt = {}
+    if args.adaptive_steps > 0:
+        for prec in ("fp64", "mixed"):
+            log(f"adaptive Dormand-Prince at C2, {prec}")
+            adapt[prec] = adaptive_leg(mds, workload, prec, args.adaptive_steps, args.seed)
+            log(f"  {adapt[prec]}")
     value = results["mixed"]["value"]
     el = results["mixed"]["el"]
     roof, breakdown = results["mixed"]["roof"], results["mixed"]["breakdown"]
@@ -457,9 +511,23 @@ def main():
     if not args.no_cpu:
         log(f"CPU baseline: one oracle Heun step at N={n} on {args.cpu_threads} threads")
         cv, cs = cpu_baseline(info, args.cpu_threads)
+        nproc = os.cpu_count() or 0
         cpu = {"value": cv, "unit": "particle-steps/s", "cores": args.cpu_threads, "kind": "port",
+               "host_cpus": nproc, "affinity_cpus": len(os.sched_getaffinity(0)),
                "sample": f"one fixed-dt Heun step of the same N={n} system ({cs:.1f} s), CPU oracle "
-                         f"(the reference's O(N^2) pair scan and two-pass k sums restated in C++, OpenMP)"}
+                         f"(the reference's O(N^2) pair scan and two-pass k sums restated in C++, OpenMP on "
+                         f"{args.cpu_threads} threads: the GPU host's CPU share is 16 cores per GPU)"}
+        # the reference is single-threaded: 1 thread against all cores on C2
+        # (N = 1e4, one Heun step), where one thread finishes in seconds
+        c2 = c2_system(mds, workload, args.seed)
+        legs = {}
+        for th in (1, args.cpu_threads):
+            log(f"CPU baseline: one oracle Heun step at C2 (N=1e4) on {th} threads")
+            v, secs = cpu_baseline(c2, th, "c2")
+            legs[str(th)] = {"value": v, "seconds": secs}
+        cpu["c2_threads"] = {"unit": "particle-steps/s", "sample": "one fixed-dt Heun step of C2 (N=1e4, "
+                             "lognormal sigma_ln 0.2, phi=0.05, 10 mT, r_c=0.3 L)", "by_threads": legs,
+                             "speedup_all_cores": legs[str(args.cpu_threads)]["value"] / legs["1"]["value"]}
     line = {
         "metric": "particle-steps/s at N=1e5 mixed vs FP64; p50/p99 ensemble-request latency",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -473,6 +541,8 @@ def main():
         "fp64_value": results["fp64"]["value"], "mixed_value": results["mixed"]["value"],
         "mixed_over_fp64": results["mixed"]["value"] / results["fp64"]["value"],
         "mixed_err_vs_fp64": mixed_err,
+        "ewald_predicted_rms_error": pred,
+        "adaptive_dormand_prince_c2": adapt or None,
         "roofline": roof, "roofline_fp64": results["fp64"]["roof"], "kernel_ms_per_step": breakdown,
         "kernel_ms_per_step_fp64": results["fp64"]["breakdown"], "cpu_baseline": cpu,
         "ensemble": ens or None,

@@ -128,6 +128,14 @@ mds_status mds_ewald_params_for_rc(const mds_config* cfg, int64_t n, const doubl
                                    const double* r_hydr, double l_box, double r_cut, double* alpha,
                                    double* kc);
 
+/* The rms force and torque errors the reference's error model predicts for
+ * (alpha, r_c, n_c), relative to the contact force and torque of the
+ * smallest particles: the estimates CompEwaldParams prints
+ * (src/interaction.cpp:311-323). Outputs may be NULL. */
+mds_status mds_ewald_predicted_errors(const mds_config* cfg, int64_t n, const double* mag_mom, const double* r_hydr,
+                                      double l_box, double alpha, double r_cut, int32_t nc, double* force_rel,
+                                      double* torque_rel);
+
 /* Initial suspension: InitParticleProps' radii (src/simulation.cpp:132-214)
  * and InitCoordsFromSketch (src/simulation.cpp:220-351), drawn from this
  * library's seeded xoshiro256** stream instead of EigenRand's Mersenne

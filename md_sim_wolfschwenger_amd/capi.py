@@ -76,6 +76,8 @@ SIGNATURES = [
                                          C.c_double, C.c_double]),
     ("mds_ewald_params_for_rc", C.c_int, [C.POINTER(Config), C.c_int64, _dp, _dp, C.c_double, C.c_double, _dp,
                                           _dp]),
+    ("mds_ewald_predicted_errors", C.c_int, [C.POINTER(Config), C.c_int64, _dp, _dp, C.c_double, C.c_double,
+                                             C.c_double, C.c_int32, _dp, _dp]),
     ("mds_init_radii", C.c_int, [C.POINTER(Config), C.c_int64, C.c_uint64, _dp]),
     ("mds_init_positions", C.c_int, [C.POINTER(Config), C.c_int64, _dp, C.c_double, C.c_uint64, C.c_int32, _dp]),
     ("mds_init_orientations", C.c_int, [C.POINTER(Config), C.c_int64, C.c_uint64, _dp, _dp]),
@@ -225,6 +227,19 @@ def init_coords_from_sim(path: str, n: int):
     if st != 0:
         raise MdsError(st, f"cannot read {n} x 10 coordinates from {path}")
     return rm(pos, n), rm(mm, n), rm(ea, n), t.value
+
+
+def ewald_predicted_errors(cfg: Config, mag_mom, r_hydr, l_box: float, alpha: float, r_cut: float, nc: int):
+    """(force, torque) rms errors predicted by the reference's error model,
+    relative to the contact force and torque of the smallest particles."""
+    mu = np.ascontiguousarray(mag_mom, dtype=np.float64)
+    rh = np.ascontiguousarray(r_hydr, dtype=np.float64)
+    f, t = C.c_double(), C.c_double()
+    st = lib().mds_ewald_predicted_errors(C.byref(cfg), len(mu), _p(mu), _p(rh), float(l_box), float(alpha),
+                                          float(r_cut), int(nc), C.byref(f), C.byref(t))
+    if st:
+        raise MdsError(st, "ewald_predicted_errors")
+    return f.value, t.value
 
 
 def philox4x32_10(ctr, key) -> tuple:

@@ -330,6 +330,39 @@ double mds_ewald_rez_error(const mds_config* c, int64_t n, double M, double f_al
            c->err_tol_ewald / std::sqrt(2.0);
 }
 
+mds_status mds_ewald_predicted_errors(const mds_config* c, int64_t n, const double* mag_mom, const double* r_hydr,
+                                      double l_box, double alpha, double r_cut, int32_t nc, double* force_rel,
+                                      double* torque_rel) {
+    // The error estimates CompEwaldParams prints for the parameters it chose
+    // (src/interaction.cpp:311-323), relative to the contact force and torque
+    // of the smallest particles (178-180).
+    if (!c || n <= 0 || !mag_mom || !r_hydr || l_box <= 0 || alpha <= 0 || r_cut <= 0) return MDS_EINVAL;
+    double M = 0, mMin = mag_mom[0], hMin = r_hydr[0];
+    for (int64_t i = 0; i < n; ++i) {
+        M += std::pow(mag_mom[i], 2.0);
+        mMin = std::min(mMin, mag_mom[i]);
+        hMin = std::min(hMin, r_hydr[i]);
+    }
+    const double pi = c->my_pi, f = c->fac_ewald, N = (double)n, k = (double)nc;
+    const double fAl = 3 * c->mu0 * std::pow(mMin, 2.0) / (2.0 * pi * std::pow(2.0 * hMin, 4.0));
+    const double tAl = std::pow(mMin, 2.0) * c->mu0 / (16.0 * pi * std::pow(hMin, 3.0));
+    const double ar = alpha * r_cut;
+    const double cc = 4.0 * std::pow(ar, 4.0) + 6.0 * std::pow(ar, 2.0) + 3.0;
+    const double dc = 8.0 * std::pow(ar, 6.0) + 20.0 * std::pow(ar, 4.0) + 30.0 * std::pow(ar, 2.0) + 15.0;
+    const double bc = 2.0 * std::pow(ar, 2.0) + 1.0;
+    const double gk = std::exp(-std::pow(pi * k / (alpha * l_box), 2.0));
+    const double eRF = f * M / std::sqrt(std::pow(l_box, 3.0) * std::pow(alpha, 4.0) * std::pow(r_cut, 9.0) * N) *
+                       std::sqrt(13.0 / 6.0 * cc * cc + 2.0 / 15.0 * dc * dc - 13.0 / 15.0 * cc * dc) *
+                       std::exp(-ar * ar);
+    const double eKF = f * 8.0 * pi * M / std::pow(l_box, 3.0) * alpha * std::sqrt(2.0 * pi * k * k * k / (15.0 * N)) * gk;
+    const double eRT = f * M / std::sqrt(std::pow(l_box, 3.0) * std::pow(alpha, 4.0) * std::pow(r_cut, 7.0) * N) *
+                       std::sqrt(0.5 * bc * bc + 0.2 * cc * cc) * std::exp(-ar * ar);
+    const double eKT = f * 4.0 * M / std::pow(l_box, 2.0) * alpha * std::sqrt(pi * k / (5.0 * N)) * gk;
+    if (force_rel) *force_rel = std::sqrt(eKF * eKF + eRF * eRF) / fAl;
+    if (torque_rel) *torque_rel = std::sqrt(eKT * eKT + eRT * eRT) / tAl;
+    return MDS_OK;
+}
+
 mds_status mds_ewald_params_for_rc(const mds_config* c, int64_t n, const double* mag_mom, const double* r_hydr,
                                    double l_box, double r_cut, double* alpha, double* kc) {
     // The deterministic part of CompEwaldParams (src/interaction.cpp:179-257):

@@ -131,3 +131,36 @@ def test_no_device_in_build_container_is_an_error_not_a_fallback():
     cfg = mds.config_from_string(load("heun_n125")["ini_text"])
     with pytest.raises(mds.MdsError):
         mds.Context(cfg)
+
+
+def test_predicted_ewald_errors_match_measured_error_of_oracle():
+    """mds_ewald_predicted_errors (the estimates CompEwaldParams prints,
+    src/interaction.cpp:311-323) against the rms error the CPU oracle
+    measures at the same parameters, as CheckEwaldSum does
+    (src/interaction.cpp:678-755): C2 at r_c = 0.3 L against a converged
+    splitting (alpha L = 18.6, r_c = 0.35 L, n_c = 40). Both models are rms
+    estimates; they agree within 30 %, and the force error is the
+    configured 1e-6 of the contact force within the rounding of n_c."""
+    import oracle
+    from md_sim_wolfschwenger_amd import workload
+    from refcfg import derived, parse_ini
+    cfg, r, rh, L, pos, mm, ea = workload.system("c2", seed=3)
+    mu = cfg.sat_mag * 4 / 3 * cfg.my_pi * r ** 3
+    c = derived(parse_ini(workload.config_text("c2", seed=3)))
+    props = {"r_mag": r, "r_hydr": rh, "vol_mag": 4 / 3 * cfg.my_pi * r ** 3, "mag_mom": mu,
+             "zeta_rot": 8 * cfg.my_pi * cfg.vis * rh ** 3, "zeta_trans": 6 * cfg.my_pi * cfg.vis * rh}
+    a, rc, nc = workload.ewald_for(cfg, mu, rh, L, 0.3)
+    th = max(1, min(8, os.cpu_count() or 1))
+    F, B, _, _ = oracle.System(len(r), L, a, rc, 2 * r.max(), nc, props, c, threads=th).comp_interactions(pos, mm)
+    Fx, Bx, _, _ = oracle.System(len(r), L, 18.6 / L, 0.35 * L, 2 * r.max(), 40, props, c,
+                                 threads=th).comp_interactions(pos, mm)
+    mu0 = 4e-7 * cfg.my_pi
+    f_al = 3 * mu0 * mu.min() ** 2 / (2 * cfg.my_pi * (2 * rh.min()) ** 4)
+    t_al = mu.min() ** 2 * mu0 / (16 * cfg.my_pi * rh.min() ** 3)
+    rms = lambda d: float(np.sqrt((d * d).sum(axis=1).mean()))
+    e_f = rms(F - Fx) / f_al
+    e_t = rms(np.cross(mm * mu[:, None], B - Bx)) / t_al
+    pf, pt = capi.ewald_predicted_errors(cfg, mu, rh, L, a, rc, nc)
+    assert e_f == pytest.approx(pf, rel=0.3)
+    assert e_t == pytest.approx(pt, rel=0.3)
+    assert pf == pytest.approx(cfg.err_tol_ewald, rel=0.3)
