@@ -157,6 +157,20 @@ mds_status mds_init_positions(const mds_config* cfg, int64_t n, const double* r_
  * 2: on the easy axes, 3: easy axes and moments along z). */
 mds_status mds_init_orientations(const mds_config* cfg, int64_t n, uint64_t seed, double* pos_mm,
                                  double* pos_ea);
+/* The initial suspension drawn on HIP device `device`: radii as
+ * mds_init_radii, the box from volFrac as the reference (src/simulation.cpp:
+ * 56) unless *l_box > 0 is given (it returns the edge used), positions with
+ * the acceptance rule of mds_init_positions (`periodic` as there;
+ * INIT_CONFIG_TRANS 2: the npDim^3 grid) by parallel random sequential
+ * addition (rounds in which every unplaced particle proposes a position and
+ * is accepted when it overlaps no placed particle and no proposal of a lower
+ * index), orientations as mds_init_orientations; all from the Philox4x32-10
+ * stream keyed by `seed` (the host functions' distributions, other numbers).
+ * Host outputs: r_mag[n] and pos, pos_mm, pos_ea (N x 3 column-major); any
+ * but pos may be NULL. MDS_ECAPACITY when 20000 rounds leave particles
+ * unplaced (a volume fraction too high for random addition). */
+mds_status mds_init_suspension_device(const mds_config* cfg, int device, int64_t n, uint64_t seed, int32_t periodic,
+                                      double* l_box, double* r_mag, double* pos, double* pos_mm, double* pos_ea);
 /* READ_FROM_FILE: the state from a coordinates file (replaces
  * InitCoordsFromSim, src/simulation.cpp:357-382, reading with openData's
  * rules, src/postprocessing.cpp:37-75). The file is WriteCoords2TXT's N x 10

@@ -82,6 +82,8 @@ SIGNATURES = [
     ("mds_init_positions", C.c_int, [C.POINTER(Config), C.c_int64, _dp, C.c_double, C.c_uint64, C.c_int32, _dp]),
     ("mds_init_orientations", C.c_int, [C.POINTER(Config), C.c_int64, C.c_uint64, _dp, _dp]),
     ("mds_init_coords_from_sim", C.c_int, [C.c_char_p, C.c_int64, _dp, _dp, _dp, _dp]),
+    ("mds_init_suspension_device", C.c_int, [C.POINTER(Config), C.c_int, C.c_int64, C.c_uint64, C.c_int32, _dp, _dp,
+                                             _dp, _dp, _dp]),
     ("mds_create", C.c_int, [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]),
     ("mds_destroy", None, [C.c_void_p]),
     ("mds_last_error", C.c_char_p, [C.c_void_p]),
@@ -217,6 +219,19 @@ def init_orientations(cfg: Config, n: int, seed: int):
     if st:
         raise MdsError(st, "init_orientations")
     return rm(mm, n), rm(ea, n)
+
+
+def init_suspension_device(cfg: Config, n: int, seed: int, periodic: bool = True, l_box: float = 0.0,
+                           device: int = 0):
+    """(r_mag, L, pos, mm, ea) of an initial suspension drawn on the device
+    (mds_init_suspension_device)."""
+    L = C.c_double(l_box)
+    r, pos, mm, ea = np.zeros(n), np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+    st = lib().mds_init_suspension_device(C.byref(cfg), int(device), int(n), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                          1 if periodic else 0, C.byref(L), _p(r), _p(pos), _p(mm), _p(ea))
+    if st:
+        raise MdsError(st, "init_suspension_device")
+    return r, L.value, rm(pos, n), rm(mm, n), rm(ea, n)
 
 
 def init_coords_from_sim(path: str, n: int):

@@ -117,9 +117,10 @@ __global__ void k_prepare(int64_t n, const double* __restrict__ pos, double* __r
                   cell_coord(x, L, R.fcell, fn);
 }
 
+// Counting sort by key; a negative key leaves the item out of the sort.
 __global__ void k_count(int64_t n, const int32_t* __restrict__ key, int32_t* __restrict__ count) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&count[key[i]], 1);
+    if (i < n && key[i] >= 0) atomicAdd(&count[key[i]], 1);
 }
 
 // Exclusive scan of `count` in three passes over tiles of SCAN_TILE entries:
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int32_t* __restrict__ 
 __global__ void k_place(int64_t n, const int32_t* __restrict__ key, int32_t* __restrict__ cursor,
                         int32_t* __restrict__ sorted_idx) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < n) sorted_idx[atomicAdd(&cursor[key[i]], 1)] = (int32_t)i;
+    if (i < n && key[i] >= 0) sorted_idx[atomicAdd(&cursor[key[i]], 1)] = (int32_t)i;
 }
 
 // Cell-sorted records. The fourth component of the FP64 record is a tag,

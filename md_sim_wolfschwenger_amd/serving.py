@@ -103,6 +103,7 @@ class Server:
         self.chunk_seconds = chunk_seconds
         self.rc_scale = rc_scale
         self.dt = dt
+        self.device = device
         self.ctx = capi.Context(self.cfg, device)
         self.ctx.set_tiling(0)
         self.ctx.set_kspace_overlap(kspace_overlap)
@@ -136,15 +137,14 @@ class Server:
         return req.ewald
 
     def system_of(self, req: Request) -> dict:
-        """Initial suspension of a request: positions by random sequential
-        addition, orientations uniform on the sphere, seeded by the request."""
+        """Initial suspension of a request, drawn on the device: positions by
+        parallel random sequential addition, orientations uniform on the
+        sphere, seeded by the request."""
         cfg = self.cfg
         p = self.params_of(req)
-        r = np.full(req.n, cfg.r_mag_mean)
-        rh = r + cfg.d_shell
-        pos = capi.init_positions(cfg, rh, p["l_box"], req.seed, periodic=True)
-        mm, ea = capi.init_orientations(cfg, req.n, req.seed ^ 0x5BD1E995)
-        return dict(r_mag=r, r_hydr=rh, l_box=p["l_box"], field_z=req.field_z, seed=req.seed, pos=pos, mm=mm,
+        r, L, pos, mm, ea = capi.init_suspension_device(cfg, req.n, req.seed, periodic=True, l_box=p["l_box"],
+                                                        device=self.device)
+        return dict(r_mag=r, r_hydr=r + cfg.d_shell, l_box=L, field_z=req.field_z, seed=req.seed, pos=pos, mm=mm,
                     ea=ea)
 
     def footprint(self, req: Request) -> int:
