@@ -214,49 +214,61 @@ def ensemble(mds, workload, precision, replicas, steps, warmup, master=20261203)
     return out
 
 
-def serving_leg(precision, n_req, steps_scale, budget_gib, load=0.8, seed=20261203, cu_fractions=None,
-                cu_spans=None):
+def serving_leg(precision, n_req, steps_scale, budget_gib, loads=(0.8, 0.5), seed=20261203, cu_fractions=None,
+                cu_spans=None, **lane_kw):
     """Request stream through the continuous-batching loop: N log-uniform in
     [1e3, 2e5], steps log-uniform in [1e3, 1e5] x steps_scale, field uniform
     in [0, 100] mT, volume fraction uniform in [1 %, 10 %], own seeds. First
     all requests at t = 0 (saturated throughput), then the same requests with
-    Poisson arrivals at `load` x the saturated request rate; latency of each
-    request against its isolated-solve time (the same request served alone on
-    an idle server)."""
+    Poisson arrivals at each `loads` x the saturated request rate; latency of
+    each request against its isolated-solve time (the same request served
+    alone on an idle server)."""
     import copy
     from md_sim_wolfschwenger_amd import serving
     rng = np.random.default_rng(seed)
     base = serving.poisson_stream(n_req, 0.0, rng, steps_range=(1e3 * steps_scale, 1e5 * steps_scale))
-    kw = dict(precision=precision, hbm_budget=budget_gib * 2 ** 30, cu_fractions=cu_fractions, cu_spans=cu_spans)
+    kw = dict(precision=precision, hbm_budget=budget_gib * 2 ** 30, cu_fractions=cu_fractions, cu_spans=cu_spans,
+              **lane_kw)
     srv = serving.LaneServer(**kw)
     sat = srv.serve(copy.deepcopy(base))
     srv.close()
     work = sum(q.n * q.steps for q in base)
-    rate = load * sat["throughput"] / (work / n_req)
-    reqs = copy.deepcopy(base)
-    t = 0.0
-    for q in reqs:
-        t += rng.exponential(1.0 / rate)
-        q.arrival = t
-    srv = serving.LaneServer(**kw)
-    ld = srv.serve(reqs)
-    srv.close()
-    iso = np.array(serving.isolated_latencies(reqs, precision))
-    lat = np.array([q.latency for q in reqs])
-    ratio = lat / iso
-    return {"requests": n_req, "steps_range": [1e3 * steps_scale, 1e5 * steps_scale], "n_range": [1e3, 2e5],
-            "hbm_budget_gib": budget_gib, "precision": precision,
-            "saturated": {"throughput": sat["throughput"], "wall_s": sat["wall_s"], "p50_latency_s":
-                          sat["p50_latency_s"], "p99_latency_s": sat["p99_latency_s"]},
-            "lanes": {"bounds_n": list(srv.bounds), "requests": [l["requests"] for l in ld["lanes"]],
-                      "cu_ranges": srv.cu_ranges, "seconds": [l["seconds"] for l in ld["lanes"]]},
-            "load": load, "arrival_rate_per_s": rate, "throughput": ld["throughput"], "wall_s": ld["wall_s"],
+    iso = np.array(serving.isolated_latencies(copy.deepcopy(base), precision))
+    out = {"requests": n_req, "steps_range": [1e3 * steps_scale, 1e5 * steps_scale], "n_range": [1e3, 2e5],
+           "hbm_budget_gib": budget_gib, "precision": precision,
+           "saturated": {"throughput": sat["throughput"], "wall_s": sat["wall_s"], "p50_latency_s":
+                         sat["p50_latency_s"], "p99_latency_s": sat["p99_latency_s"]},
+           "isolated_p50_s": float(np.percentile(iso, 50)), "isolated_p99_s": float(np.percentile(iso, 99)),
+           "by_load": []}
+    for load in loads:
+        rate = load * sat["throughput"] / (work / n_req)
+        reqs = copy.deepcopy(base)
+        arr = np.random.default_rng(seed + 1)
+        t = 0.0
+        for q in reqs:
+            t += arr.exponential(1.0 / rate)
+            q.arrival = t
+        srv = serving.LaneServer(**kw)
+        ld = srv.serve(reqs)
+        srv.close()
+        lat = np.array([q.latency for q in reqs])
+        ratio = lat / iso
+        out["by_load"].append({
+            "load": load, "arrival_rate_per_s": rate, "offered_particle_steps_per_s": load * sat["throughput"],
+            "throughput": ld["throughput"], "wall_s": ld["wall_s"],
             "p50_latency_s": ld["p50_latency_s"], "p99_latency_s": ld["p99_latency_s"],
-            "isolated_p50_s": float(np.percentile(iso, 50)), "isolated_p99_s": float(np.percentile(iso, 99)),
             "latency_over_isolated_p50": float(np.percentile(ratio, 50)),
             "latency_over_isolated_p99": float(np.percentile(ratio, 99)),
+            "lanes": {"bounds_n": list(srv.bounds), "requests": [l["requests"] for l in ld["lanes"]],
+                      "seconds": [l["seconds"] for l in ld["lanes"]]},
             "worst": [{"n": q.n, "steps": q.steps, "lane": srv.lane_of(q), "latency_s": float(q.latency),
-                       "wait_s": float(q.admitted - q.arrival), "isolated_s": float(i)} for q, i in sorted(zip(reqs, iso), key=lambda x: -x[0].latency / x[1])[:3]]}
+                       "wait_s": float(q.admitted - q.arrival), "isolated_s": float(i)}
+                      for q, i in sorted(zip(reqs, iso), key=lambda x: -x[0].latency / x[1])[:3]]})
+    # the headline figures at the first load (80 %)
+    first = out["by_load"][0]
+    out.update({k: first[k] for k in ("load", "throughput", "p50_latency_s", "p99_latency_s",
+                                      "latency_over_isolated_p50", "latency_over_isolated_p99")})
+    return out
 
 
 def cpu_baseline(sysinfo, threads, name="c3"):
@@ -400,9 +412,11 @@ def main():
                     help="accepted steps of the adaptive Dormand-Prince leg at C2 (0 = skip)")
     ap.add_argument("--replicas", type=int, default=256, help="replicas of the batched-ensemble leg (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
-    ap.add_argument("--requests", type=int, default=24, help="requests of the serving leg (0 = skip)")
-    ap.add_argument("--steps-scale", type=float, default=0.01,
-                    help="serving steps are log-uniform in [1e3, 1e5] times this")
+    ap.add_argument("--requests", type=int, default=100, help="requests of the serving leg (0 = skip)")
+    ap.add_argument("--steps-scale", type=float, default=0.005,
+                    help="serving steps are log-uniform in [1e3, 1e5] times this (the configured stream's mean "
+                         "request is 8e8 particle-steps, about a minute of the whole device: 100 of them take hours)")
+    ap.add_argument("--serving-precisions", default="mixed", help="comma-separated: mixed, fp64")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0)
     ap.add_argument("--mz-run", type=int, default=0,
                     help="only run the C3 system for this many steps at r_c = 0.225 L, reach 3, and write its "
@@ -503,7 +517,7 @@ def main():
             log(f"  {ens[prec]}")
     serv = {}
     if args.requests > 0:
-        for prec in ("fp64", "mixed"):
+        for prec in args.serving_precisions.split(","):
             log(f"serving: {args.requests} requests, {prec}")
             serv[prec] = serving_leg(prec, args.requests, args.steps_scale, args.hbm_budget_gib)
             log(f"  {serv[prec]}")

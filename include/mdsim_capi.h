@@ -391,6 +391,13 @@ mds_status mds_replica_info(mds_ctx* ctx, int32_t index, int64_t* n, int64_t* of
                             double* alpha, double* r_cut, int32_t* nc, double* t, uint64_t* steps);
 /* State of one replica (N_r x 3, column-major). */
 mds_status mds_replica_get_state(mds_ctx* ctx, int32_t index, double* pos, double* pos_mm, double* pos_ea);
+/* Replica `index` stops stepping when its step counter (accepted steps
+ * since it joined) reaches `step`: later steps of the context leave its
+ * state, clock and Philox counter as they are, so a request that needs k
+ * steps can share runs of any length with others and still end with its
+ * k-step state. UINT64_MAX (the default) never stops. No counterpart in the
+ * reference; not part of a checkpoint. */
+mds_status mds_replica_stop_at(mds_ctx* ctx, int32_t index, uint64_t step);
 /* <m_z> of every replica after the last step (src/simulation.cpp:513). With
  * several replicas, mds_run's mz_series is steps x replicas, row-major. */
 mds_status mds_replica_mz(mds_ctx* ctx, double* mz);

@@ -132,6 +132,7 @@ SIGNATURES = [
                                    C.POINTER(C.c_uint64)]),
     ("mds_replica_get_state", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp]),
     ("mds_replica_mz", C.c_int, [C.c_void_p, _dp]),
+    ("mds_replica_stop_at", C.c_int, [C.c_void_p, C.c_int32, C.c_uint64]),
     ("mds_reserve", C.c_int, [C.c_void_p, C.c_int64]),
     ("mds_set_cu_mask", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_int32]),
     ("mds_device_cus", C.c_int32, [C.c_int]),
@@ -533,6 +534,10 @@ class Context:
         a, b, c = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
         self._chk(self._L.mds_replica_get_state(self.h, int(index), _p(a), _p(b), _p(c)), "replica_get_state")
         return rm(a, n), rm(b, n), rm(c, n)
+
+    def replica_stop_at(self, index, step):
+        """Replica `index` stops stepping at its step count `step`."""
+        self._chk(self._L.mds_replica_stop_at(self.h, int(index), int(step)), "replica_stop_at")
 
     def replica_mz(self):
         mz = np.zeros(self.replica_count())

@@ -183,6 +183,7 @@ struct RepMeta {
     bool ewald_set = false;
     uint64_t seed = 0;
     double margin_scale = 0.1;
+    uint64_t stop_at = ~(uint64_t)0;  // step count at which the replica stops (mds_replica_stop_at)
 };
 
 constexpr int32_t kSeriesCap = 1024;  // <m_z> rows buffered on the device per run chunk
@@ -600,6 +601,7 @@ mds_status plan_all(mds_ctx* c) {
         R.L = m.L;
         R.field_z = m.field_z;
         R.seed = m.seed;
+        R.stop_at = m.stop_at;
         plan_grid(c, m, R);
         plan_ewald(c, m, R);
         plan_kspace(c, m, R);
@@ -1021,12 +1023,12 @@ mds_status fixed_step_body(mds_ctx* c, double h, bool check) {
     }
     {
         mds_ctx::Span sp2(c, KC_TRANSLATE);
-        HIPCHK(launch_translate(s, n, c->reps.p, c->rep_of.p, h, c->pos_cur.p, c->force.p, c->zeta_trans.p, c->pos.p,
+        HIPCHK(launch_translate(s, n, c->reps.p, c->rep_of.p, c->rep_step.p, h, c->pos_cur.p, c->force.p, c->zeta_trans.p, c->pos.p,
                                 c->new_mm.p, c->new_ea.p, c->mm.p, c->ea.p));
         HIPCHK(launch_rep_mean_z(s, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), n, c->mm.p,
                                  c->red_part.p, c->mz_out.p, c->series_counter.p, c->series.p, kSeriesCap));
         if ((st = observables(c)) != MDS_OK) return st;
-        HIPCHK(launch_step_count(s, c->nrep, c->rep_step.p, c->rep_t.p, h, c->series_counter.p));
+        HIPCHK(launch_step_count(s, c->nrep, c->reps.p, c->rep_step.p, c->rep_t.p, h, c->series_counter.p));
     }
     return MDS_OK;
 }
@@ -1103,12 +1105,12 @@ mds_status adaptive_step(mds_ctx* c, double* t, double* dt, double* dt_used) {
     {
         mds_ctx::Span sp(c, KC_TRANSLATE);
         // accepted, renormalised state -> mm/ea; IntegrationTrans + wrap; <m_z>
-        HIPCHK(launch_translate(s, n, c->reps.p, c->rep_of.p, h, c->pos_cur.p, c->force.p, c->zeta_trans.p, c->pos.p,
+        HIPCHK(launch_translate(s, n, c->reps.p, c->rep_of.p, c->rep_step.p, h, c->pos_cur.p, c->force.p, c->zeta_trans.p, c->pos.p,
                                 c->new_mm.p, c->new_ea.p, c->mm.p, c->ea.p));
         HIPCHK(launch_rep_mean_z(s, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), n, c->mm.p,
                                  c->red_part.p, c->mz_out.p, c->series_counter.p, c->series.p, kSeriesCap));
         if ((st = observables(c)) != MDS_OK) return st;
-        HIPCHK(launch_step_count(s, c->nrep, c->rep_step.p, c->rep_t.p, h, c->series_counter.p));
+        HIPCHK(launch_step_count(s, c->nrep, c->reps.p, c->rep_step.p, c->rep_t.p, h, c->series_counter.p));
     }
     *t += h;
     if (dt_used) *dt_used = h;
@@ -1853,6 +1855,18 @@ mds_status run_impl(mds_ctx* c, int64_t steps, double* t, double* dt, double* mz
     return read_flags(c);
 }
 }  // namespace
+
+mds_status mds_replica_stop_at(mds_ctx* c, int32_t index, uint64_t step) {
+    if (!c || index < 0 || index >= c->nrep) return MDS_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    c->meta[index].stop_at = step;
+    if (c->planned && (size_t)index < c->h_rep.size()) {
+        c->h_rep[index].stop_at = step;
+        HIPCHK(hipMemcpyAsync(c->reps.p + index, &c->h_rep[index], sizeof(Rep), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return MDS_OK;
+}
 
 mds_status mds_set_check_interval(mds_ctx* c, int64_t steps) {
     if (!c || steps < 1) return MDS_EINVAL;

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "kernels.h"
@@ -174,13 +175,30 @@ __global__ void k_rsa_accept(int64_t n, RsaGrid g, const double* __restrict__ r,
     atomicAdd(n_placed, 1);
 }
 
-template <typename T>
-struct Dev {
-    T* p = nullptr;
-    ~Dev() {
-        if (p) (void)hipFree(p);
+// Device memory and stream of the initialisation, per device, kept until the
+// process ends: a call may run while other host threads capture HIP graphs
+// on their own streams, and freeing device memory (or the legacy default
+// stream) would synchronise with those streams and break their captures. So
+// the workspace only grows (a larger block is taken and the old one kept),
+// and all work and copies go to a non-blocking stream of its own.
+struct Workspace {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    char* base = nullptr;
+    size_t cap = 0;
+    std::vector<char*> old;  // outgrown blocks, never freed (see above)
+};
+Workspace g_ws[64];
+
+// Carves aligned pieces out of the workspace block.
+struct Carver {
+    char* p;
+    template <typename T>
+    T* take(size_t count) {
+        T* q = (T*)p;
+        p += (std::max<size_t>(1, count) * sizeof(T) + 255) & ~(size_t)255;
+        return q;
     }
-    hipError_t alloc(size_t count) { return hipMalloc((void**)&p, std::max<size_t>(1, count) * sizeof(T)); }
 };
 
 #define DCHK(x)                                      \
@@ -196,9 +214,12 @@ mds_status mds_init_suspension_device(const mds_config* cfg, int device, int64_t
                                       double* l_box, double* r_mag, double* pos, double* pos_mm, double* pos_ea) {
     if (!cfg || n <= 0 || n >= ((int64_t)1 << 30) || !l_box || !pos) return MDS_EINVAL;
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return MDS_EDEVICE;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev || device >= 64) return MDS_EDEVICE;
     DCHK(hipSetDevice(device));
-    hipStream_t s = nullptr;
+    Workspace& ws = g_ws[device];
+    std::lock_guard<std::mutex> lk(ws.mu);
+    if (!ws.stream) DCHK(hipStreamCreateWithFlags(&ws.stream, hipStreamNonBlocking));
+    hipStream_t s = ws.stream;
     const int blocks = (int)((n + 255) / 256);
     // radii (mds_init_radii's modes)
     int32_t mode = 0;
@@ -216,12 +237,46 @@ mds_status mds_init_suspension_device(const mds_config* cfg, int device, int64_t
     } else if (!(cfg->r_mag_mean > 0)) {
         return MDS_ECONFIG;
     }
-    Dev<double> d_r, d_pos, d_cand, d_mm, d_ea;
-    DCHK(d_r.alloc(n));
-    hipLaunchKernelGGL(k_init_radii, dim3(blocks), dim3(256), 0, s, n, stream_seed(seed, 1), mode, a, b, d_r.p);
+    // the cell grid is fixed by the radii; the workspace is sized for the
+    // largest grid that n particles can ask for
+    const int64_t nc_cap = std::min<int64_t>(512, (int64_t)std::cbrt(8.0 * (double)n) + 1);
+    const int64_t cells_cap = nc_cap * nc_cap * nc_cap;
+    // 14 doubles and 4 int32 per particle, 4 int32 per cell, the scan tiles,
+    // and 256 bytes of alignment for each of the 16 pieces
+    const size_t need = (size_t)n * (14 * 8 + 4 * 4) + (size_t)cells_cap * 16 +
+                        (size_t)(mds::scan_tiles(cells_cap) + 2) * 4 + 16 * 256;
+    if (ws.cap < need) {
+        char* p = nullptr;
+        const size_t cap = need + need / 2;
+        DCHK(hipMalloc((void**)&p, cap));
+        if (ws.base) ws.old.push_back(ws.base);
+        ws.base = p;
+        ws.cap = cap;
+    }
+    Carver cv{ws.base};
+    double* d_r = cv.take<double>(n);
+    double* d_rh = cv.take<double>(n);
+    double* d_mm = cv.take<double>(3 * n);
+    double* d_ea = cv.take<double>(3 * n);
+    double* d_pos = cv.take<double>(3 * n);
+    double* d_cand = cv.take<double>(3 * n);
+    int32_t* placed = cv.take<int32_t>(n);
+    int32_t* key_prop = cv.take<int32_t>(n);
+    int32_t* key_placed = cv.take<int32_t>(n);
+    int32_t* sorted = cv.take<int32_t>(n);
+    int32_t* cnt = cv.take<int32_t>(cells_cap);
+    int32_t* start = cv.take<int32_t>(cells_cap);
+    int32_t* end = cv.take<int32_t>(cells_cap);
+    int32_t* cur = cv.take<int32_t>(cells_cap);
+    int32_t* tmp = cv.take<int32_t>(mds::scan_tiles(cells_cap) + 1);
+    int32_t* counter = cv.take<int32_t>(1);
+    if ((size_t)(cv.p - ws.base) > ws.cap) return MDS_EINVAL;  // the pieces must fit the block sized above
+
+    hipLaunchKernelGGL(k_init_radii, dim3(blocks), dim3(256), 0, s, n, stream_seed(seed, 1), mode, a, b, d_r);
     DCHK(hipGetLastError());
     std::vector<double> hr((size_t)n), hh((size_t)n);
-    DCHK(hipMemcpy(hr.data(), d_r.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    DCHK(hipMemcpyAsync(hr.data(), d_r, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    DCHK(hipStreamSynchronize(s));
     if (r_mag) std::copy(hr.begin(), hr.end(), r_mag);
     // hydrodynamic radii and box (InitParticleProps, src/simulation.cpp:56)
     mds_status st = mds_particle_props(cfg, n, hr.data(), hh.data(), nullptr, nullptr, nullptr, nullptr);
@@ -234,18 +289,17 @@ mds_status mds_init_suspension_device(const mds_config* cfg, int device, int64_t
         rmax = std::max(rmax, v);
     }
     // orientations
-    DCHK(d_mm.alloc(3 * n));
-    DCHK(d_ea.alloc(3 * n));
     hipLaunchKernelGGL(k_init_orient, dim3(blocks), dim3(256), 0, s, n, stream_seed(seed, 3), cfg->init_config_rot,
-                       2.0 * cfg->my_pi, d_mm.p, d_ea.p);
+                       2.0 * cfg->my_pi, d_mm, d_ea);
     DCHK(hipGetLastError());
-    if (pos_mm) DCHK(hipMemcpy(pos_mm, d_mm.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
-    if (pos_ea) DCHK(hipMemcpy(pos_ea, d_ea.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    if (pos_mm) DCHK(hipMemcpyAsync(pos_mm, d_mm, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (pos_ea) DCHK(hipMemcpyAsync(pos_ea, d_ea, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    DCHK(hipStreamSynchronize(s));
     if (cfg->init_config_trans == 2) return mds_init_positions(cfg, n, hh.data(), L, seed, periodic, pos);
     // positions: rounds of parallel random sequential addition
     RsaGrid g{};
     int64_t nc = (int64_t)std::floor(L / (2 * rmax));
-    nc = std::max<int64_t>(1, std::min<int64_t>(nc, std::min<int64_t>(512, (int64_t)std::cbrt(8.0 * (double)n) + 1)));
+    nc = std::max<int64_t>(1, std::min<int64_t>(nc, nc_cap));
     if (periodic && nc < 3) nc = 1;
     g.nc = (int32_t)nc;
     g.dump = -1;
@@ -253,44 +307,31 @@ mds_status mds_init_suspension_device(const mds_config* cfg, int device, int64_t
     g.L = L;
     g.cell = L / (double)nc;
     const int32_t ncells = (int32_t)(nc * nc * nc);
-    Dev<int32_t> placed, key_prop, key_placed, sorted, cnt, start, end, cur, tmp, counter;
-    Dev<double> d_rh;
-    DCHK(d_rh.alloc(n));
-    DCHK(hipMemcpy(d_rh.p, hh.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    DCHK(d_pos.alloc(3 * n));
-    DCHK(d_cand.alloc(3 * n));
-    DCHK(placed.alloc(n));
-    DCHK(key_prop.alloc(n));
-    DCHK(key_placed.alloc(n));
-    DCHK(sorted.alloc(n));
-    DCHK(cnt.alloc(ncells));
-    DCHK(start.alloc(ncells));
-    DCHK(end.alloc(ncells));
-    DCHK(cur.alloc(ncells));
-    DCHK(tmp.alloc(mds::scan_tiles(ncells) + 1));
-    DCHK(counter.alloc(1));
-    DCHK(mds::launch_fill_i32(s, placed.p, n, 0));
-    DCHK(mds::launch_fill_i32(s, key_placed.p, n, g.dump));
-    DCHK(mds::launch_fill_i32(s, counter.p, 1, 0));
-    DCHK(mds::launch_cell_sort(s, n, key_placed.p, ncells, cnt.p, start.p, end.p, cur.p, sorted.p, tmp.p));
+    DCHK(hipMemcpyAsync(d_rh, hh.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    DCHK(mds::launch_fill_i32(s, placed, n, 0));
+    DCHK(mds::launch_fill_i32(s, key_placed, n, g.dump));
+    DCHK(mds::launch_fill_i32(s, counter, 1, 0));
+    DCHK(mds::launch_cell_sort(s, n, key_placed, ncells, cnt, start, end, cur, sorted, tmp));
     const uint64_t ps = stream_seed(seed, 2);
     const int64_t max_rounds = 20000;
     int32_t done = 0;
     for (int64_t round = 0; done < n; ++round) {
         if (round >= max_rounds) return MDS_ECAPACITY;  // volume fraction too high for random addition
-        hipLaunchKernelGGL(k_rsa_propose, dim3(blocks), dim3(256), 0, s, n, ps, (uint64_t)round, g, d_rh.p, placed.p,
-                           d_cand.p);
-        hipLaunchKernelGGL(k_rsa_check, dim3(blocks), dim3(256), 0, s, n, g, d_rh.p, placed.p, d_cand.p, d_pos.p,
-                           start.p, end.p, sorted.p, key_prop.p);
+        hipLaunchKernelGGL(k_rsa_propose, dim3(blocks), dim3(256), 0, s, n, ps, (uint64_t)round, g, d_rh, placed,
+                           d_cand);
+        hipLaunchKernelGGL(k_rsa_check, dim3(blocks), dim3(256), 0, s, n, g, d_rh, placed, d_cand, d_pos, start, end,
+                           sorted, key_prop);
         DCHK(hipGetLastError());
-        DCHK(mds::launch_cell_sort(s, n, key_prop.p, ncells, cnt.p, start.p, end.p, cur.p, sorted.p, tmp.p));
-        hipLaunchKernelGGL(k_rsa_accept, dim3(blocks), dim3(256), 0, s, n, g, d_rh.p, key_prop.p, d_cand.p, start.p,
-                           end.p, sorted.p, d_pos.p, placed.p, key_placed.p, counter.p);
+        DCHK(mds::launch_cell_sort(s, n, key_prop, ncells, cnt, start, end, cur, sorted, tmp));
+        hipLaunchKernelGGL(k_rsa_accept, dim3(blocks), dim3(256), 0, s, n, g, d_rh, key_prop, d_cand, start, end,
+                           sorted, d_pos, placed, key_placed, counter);
         DCHK(hipGetLastError());
-        DCHK(mds::launch_cell_sort(s, n, key_placed.p, ncells, cnt.p, start.p, end.p, cur.p, sorted.p, tmp.p));
-        DCHK(hipMemcpy(&done, counter.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+        DCHK(mds::launch_cell_sort(s, n, key_placed, ncells, cnt, start, end, cur, sorted, tmp));
+        DCHK(hipMemcpyAsync(&done, counter, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        DCHK(hipStreamSynchronize(s));
     }
-    DCHK(hipMemcpy(pos, d_pos.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    DCHK(hipMemcpyAsync(pos, d_pos, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    DCHK(hipStreamSynchronize(s));
     return MDS_OK;
 }
 

@@ -61,6 +61,7 @@ struct Rep {
     double c_force, c_flux, c_self, c_surf, two_pi_over_L;
     double field_z;              // magFluxDens of this replica
     uint64_t seed;               // Philox key of this replica
+    uint64_t stop_at;            // the replica stops stepping when its step counter reaches this
 };
 
 // Per-step geometry of the real-space sum (cell list + overlap versions).
@@ -256,14 +257,15 @@ hipError_t launch_stage(hipStream_t s, const StageParams& sp, int32_t stage, int
                         const double* b_acc, const double* e_err, double* ymm_out, double* yea_out, double* mom,
                         double4* rec_mom, double* err_partial, double* new_mm, double* new_ea, int32_t nblk);
 hipError_t launch_reduce_max(hipStream_t s, const double* partial, int32_t count, double scale, double* out);
-hipError_t launch_translate(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, double h,
+hipError_t launch_translate(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const uint64_t* rep_step,
+                            double h,
                             const double* pos_cur, const double* force, const double* zeta_trans, double* pos,
                             const double* new_mm, const double* new_ea, double* mm, double* ea);
 hipError_t launch_noise(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const uint64_t* rep_step,
                         double dt, const double* zeta_rot, const double* vol_mag, const double* zeta_trans,
                         double kT, double alpha_g, double gyro, double sat_mag, int32_t en_t, int32_t en_b,
                         int32_t en_f, double* n_t, double* n_b, double* n_f);
-hipError_t launch_step_count(hipStream_t s, int32_t nrep, uint64_t* rep_step, double* rep_t, double h,
+hipError_t launch_step_count(hipStream_t s, int32_t nrep, const Rep* reps, uint64_t* rep_step, double* rep_t, double h,
                              int32_t* series_counter);
 
 }  // namespace mds

@@ -2126,14 +2126,17 @@ __global__ __launch_bounds__(256) void k_reduce_max(const double* __restrict__ p
 // IntegrationTrans (src/simulation.cpp:959-962) + ApplyBoundaryCondArr
 // (610-620), and the accepted orientations into the state.
 __global__ __launch_bounds__(256) void k_translate(int64_t n, const Rep* __restrict__ reps,
-                                                   const int32_t* __restrict__ rep_of, double h,
+                                                   const int32_t* __restrict__ rep_of,
+                                                   const uint64_t* __restrict__ rep_step, double h,
                                                    const double* __restrict__ pos_cur, const double* __restrict__ force,
                                                    const double* __restrict__ zeta_trans, double* __restrict__ pos,
                                                    const double* __restrict__ new_mm, const double* __restrict__ new_ea,
                                                    double* __restrict__ mm, double* __restrict__ ea) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double L = reps[rep_of[i]].L;
+    const int32_t ri = rep_of[i];
+    if (rep_step[ri] >= reps[ri].stop_at) return;  // a replica that has run its steps keeps its state
+    const double L = reps[ri].L;
     const double zt = zeta_trans[i];
     for (int d = 0; d < 3; ++d) {
         double x = pos_cur[d * n + i] + h * (force[d * n + i] / zt);
@@ -2255,10 +2258,10 @@ __global__ __launch_bounds__(256) void k_rep_obs(const Rep* __restrict__ reps, i
     }
 }
 
-__global__ void k_step_count(int32_t nrep, uint64_t* __restrict__ rep_step, double* __restrict__ rep_t, double h,
-                             int32_t* __restrict__ series_counter) {
+__global__ void k_step_count(int32_t nrep, const Rep* __restrict__ reps, uint64_t* __restrict__ rep_step,
+                             double* __restrict__ rep_t, double h, int32_t* __restrict__ series_counter) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nrep) {
+    if (r < nrep && rep_step[r] < reps[r].stop_at) {
         rep_step[r] += 1;
         rep_t[r] += h;
     }
@@ -2614,11 +2617,11 @@ hipError_t launch_reduce_max(hipStream_t s, const double* partial, int32_t count
     LAUNCH_CHECK();
 }
 
-hipError_t launch_translate(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, double h,
-                            const double* pos_cur, const double* force, const double* zeta_trans, double* pos,
-                            const double* new_mm, const double* new_ea, double* mm, double* ea) {
-    hipLaunchKernelGGL(k_translate, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, h, pos_cur, force,
-                       zeta_trans, pos, new_mm, new_ea, mm, ea);
+hipError_t launch_translate(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const uint64_t* rep_step,
+                            double h, const double* pos_cur, const double* force, const double* zeta_trans,
+                            double* pos, const double* new_mm, const double* new_ea, double* mm, double* ea) {
+    hipLaunchKernelGGL(k_translate, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, rep_step, h, pos_cur,
+                       force, zeta_trans, pos, new_mm, new_ea, mm, ea);
     LAUNCH_CHECK();
 }
 
@@ -2631,9 +2634,9 @@ hipError_t launch_noise(hipStream_t s, int64_t n, const Rep* reps, const int32_t
     LAUNCH_CHECK();
 }
 
-hipError_t launch_step_count(hipStream_t s, int32_t nrep, uint64_t* rep_step, double* rep_t, double h,
-                             int32_t* series_counter) {
-    hipLaunchKernelGGL(k_step_count, dim3((nrep + 255) / 256), dim3(256), 0, s, nrep, rep_step, rep_t, h,
+hipError_t launch_step_count(hipStream_t s, int32_t nrep, const Rep* reps, uint64_t* rep_step, double* rep_t,
+                             double h, int32_t* series_counter) {
+    hipLaunchKernelGGL(k_step_count, dim3((nrep + 255) / 256), dim3(256), 0, s, nrep, reps, rep_step, rep_t, h,
                        series_counter);
     LAUNCH_CHECK();
 }

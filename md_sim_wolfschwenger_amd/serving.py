@@ -68,6 +68,33 @@ def footprint_bytes(n: int, nc: int) -> int:
     return int(1.25 * (n * per_particle + kspace + cells))
 
 
+class Arbiter:
+    """Chunks of smaller-request lanes before those of larger ones, without
+    stream priorities (which have hung on this hardware): before each chunk
+    a lane waits while a lane of smaller requests has a chunk on the device,
+    for at most `max_hold` seconds, so a large request's chunks run between
+    the small ones' instead of beside them."""
+
+    def __init__(self, lanes: int, max_hold: float = 0.05):
+        import threading
+        self.busy = [0] * lanes
+        self.cv = threading.Condition()
+        self.max_hold = max_hold
+        self.held = [0.0] * lanes
+
+    def begin(self, k: int):
+        t0 = time.perf_counter()
+        with self.cv:
+            self.cv.wait_for(lambda: not any(self.busy[:k]), timeout=self.max_hold)
+            self.busy[k] += 1
+        self.held[k] += time.perf_counter() - t0
+
+    def end(self, k: int):
+        with self.cv:
+            self.busy[k] -= 1
+            self.cv.notify_all()
+
+
 class Budget:
     """Device bytes shared by the lanes of a server (thread-safe)."""
 
@@ -158,6 +185,9 @@ class Server:
         for k, q in enumerate(reqs):
             p = self.params_of(q)
             self.ctx.replica_set_ewald(first + k, p["alpha"], p["r_cut"], p["nc"])
+            # the replica stops at its own step count, so chunks need not end
+            # when the first of the active requests is done
+            self.ctx.replica_stop_at(first + k, q.steps)
             q.admitted = now
             self.done_steps[q.rid] = 0
             self.active.append(q)
@@ -183,10 +213,12 @@ class Server:
         self.active = list(reversed(keep))
 
     # ------------------------------------------------------------ loop
-    def serve(self, requests: list[Request], clock=time.perf_counter, log=None, t0: float | None = None) -> dict:
+    def serve(self, requests: list[Request], clock=time.perf_counter, log=None, t0: float | None = None,
+              arbiter: Arbiter | None = None, lane: int = 0) -> dict:
         """Run until every request is finished. Arrivals are wall-clock
         offsets from t0 (default: now). Returns throughput and latency
-        statistics."""
+        statistics. With an arbiter, each chunk is bracketed by its
+        begin/end for lane `lane`."""
         pending = sorted(requests, key=lambda q: q.arrival)
         t0 = clock() if t0 is None else t0
         total_ps = 0
@@ -218,25 +250,33 @@ class Server:
                 time.sleep(wait if wait > 0 else 0.002)
                 self.times["idle"] += max(wait, 0.002)
                 continue
-            # a chunk: no request beyond its step count, about chunk_seconds long
-            remaining = min(q.steps - self.done_steps[q.rid] for q in self.active)
+            # a chunk of about chunk_seconds (requests that finish inside it stop
+            # at their step count on the device), at most until the last is done
+            remaining = max(q.steps - self.done_steps[q.rid] for q in self.active)
             n_act = sum(q.n for q in self.active)
             if self.step_time:
                 s = max(self.min_steps, int(self.chunk_seconds / (self.step_time * n_act)))
             else:
                 s = self.min_steps
             s = min(s, remaining)
+            if arbiter is not None:
+                arbiter.begin(lane)
             c0 = clock()
-            # the clock argument only matters for a single replica (several keep theirs on the device)
-            self.ctx.run(s, self.done_steps[self.active[0].rid] * self.dt, self.dt)
-            self.ctx.synchronize()
+            try:
+                # the clock argument only matters for a single replica (several keep theirs on the device)
+                self.ctx.run(s, self.done_steps[self.active[0].rid] * self.dt, self.dt)
+                self.ctx.synchronize()
+            finally:
+                if arbiter is not None:
+                    arbiter.end(lane)
             el = clock() - c0
             self.times["run"] += el
             self.step_time = el / (s * n_act)
-            total_ps += s * n_act
+            # particle-steps actually taken (finished requests stop on the device)
+            total_ps += sum(q.n * min(s, q.steps - self.done_steps[q.rid]) for q in self.active)
             chunks += 1
             for q in self.active:
-                self.done_steps[q.rid] += s
+                self.done_steps[q.rid] = min(q.steps, self.done_steps[q.rid] + s)
             r0 = clock()
             self.retire(clock() - t0)
             self.times["retire"] += clock() - r0
@@ -266,7 +306,8 @@ class LaneServer:
 
     def __init__(self, precision: str = "fp64", hbm_budget: float = 64 * 2 ** 30,
                  bounds: tuple = (6e3, 4e4), chunk_seconds: tuple = (0.01, 0.04, 0.2),
-                 cu_fractions: tuple | None = None, cu_spans: tuple | None = None, device: int = 0, **kw):
+                 cu_fractions: tuple | None = None, cu_spans: tuple | None = None, device: int = 0,
+                 arbiter_hold: float = 0.0, **kw):
         """cu_fractions: share of the device's compute units given to each
         lane (disjoint ranges of CU ids); cu_spans: per lane a (first, last)
         fraction of the CU id range, which may overlap (a small-request lane
@@ -274,6 +315,9 @@ class LaneServer:
         None lets every lane use all."""
         assert len(chunk_seconds) == len(bounds) + 1
         self.bounds = bounds
+        # arbiter_hold > 0: a lane holds its chunks while a lane of smaller
+        # requests runs one (at most that many seconds per chunk)
+        self.arbiter = Arbiter(len(bounds) + 1, arbiter_hold) if arbiter_hold > 0 else None
         self.budget = Budget(hbm_budget)
         # each lane reserves room for a few of its largest requests, so
         # admissions and retirements reuse device memory
@@ -321,7 +365,8 @@ class LaneServer:
 
         def work(k):
             try:
-                out[k] = self.lanes[k].serve(parts[k], clock, log, t0) if parts[k] else summary([], 0.0, 0, 0)
+                out[k] = (self.lanes[k].serve(parts[k], clock, log, t0, self.arbiter, k) if parts[k]
+                          else summary([], 0.0, 0, 0))
             except Exception as e:  # surfaced below
                 errs.append(e)
 
@@ -334,8 +379,9 @@ class LaneServer:
             raise errs[0]
         wall = clock() - t0
         res = summary(requests, wall, sum(o["particle_steps"] for o in out), sum(o["chunks"] for o in out))
-        res["lanes"] = [{"requests": len(p), "chunks": o["chunks"], "seconds": o.get("seconds")}
-                        for p, o in zip(parts, out)]
+        res["lanes"] = [{"requests": len(p), "chunks": o["chunks"], "seconds": o.get("seconds"),
+                         "held_s": self.arbiter.held[k] if self.arbiter else 0.0}
+                        for k, (p, o) in enumerate(zip(parts, out))]
         return res
 
 

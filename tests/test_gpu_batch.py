@@ -199,3 +199,26 @@ def test_mixed_kspace_path_is_chosen_per_replica():
         a.run(3, 0.0, 1e-12)
         for x, y in zip(ctx.replica_state(k), a.get_state()):
             assert np.array_equal(x, y), k
+
+
+def test_replica_stops_at_its_step_count():
+    """A replica told to stop at step 3 keeps its 3-step state (and clock and
+    Philox counter) while the context runs on: after 7 graph-replayed steps
+    it equals itself run alone for 3 steps, bit for bit, and the other
+    replica equals itself run alone for 7."""
+    print("batch stop_at", flush=True)
+    sys_ = [make_system(n, phi, rcf, ov, sd) for n, phi, rcf, ov, _, sd in SYSTEMS[:2]]
+    fields = [f for *_, f, _ in SYSTEMS[:2]]
+    seeds = [31, 32]
+    ctx = batch(sys_, fields, seeds)
+    ctx.use_philox(0, 0)
+    ctx.replica_stop_at(0, 3)
+    ctx.set_graph(True)
+    ctx.run(7, 0.0, 1e-12)
+    for k, steps in ((0, 3), (1, 7)):
+        a = alone(sys_[k], fields[k], seeds[k])
+        a.run(steps, 0.0, 1e-12)
+        for x, y in zip(ctx.replica_state(k), a.get_state()):
+            assert np.array_equal(x, y), k
+        info = ctx.replica_info(k, clock=True)
+        assert info["steps"] == steps and info["t"] == pytest.approx(steps * 1e-12)
