@@ -227,8 +227,11 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, loads=(0.8, 0.5), see
     from md_sim_wolfschwenger_amd import serving
     rng = np.random.default_rng(seed)
     base = serving.poisson_stream(n_req, 0.0, rng, steps_range=(1e3 * steps_scale, 1e5 * steps_scale))
-    kw = dict(precision=precision, hbm_budget=budget_gib * 2 ** 30, cu_fractions=cu_fractions, cu_spans=cu_spans,
-              **lane_kw)
+    kw = dict(precision=precision, hbm_budget=budget_gib * 2 ** 30, **lane_kw)
+    if cu_fractions is not None:
+        kw["cu_fractions"] = cu_fractions
+    if cu_spans is not None:
+        kw["cu_spans"] = cu_spans
     srv = serving.LaneServer(**kw)
     sat = srv.serve(copy.deepcopy(base))
     srv.close()
@@ -260,7 +263,7 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, loads=(0.8, 0.5), see
             "latency_over_isolated_p50": float(np.percentile(ratio, 50)),
             "latency_over_isolated_p99": float(np.percentile(ratio, 99)),
             "lanes": {"bounds_n": list(srv.bounds), "requests": [l["requests"] for l in ld["lanes"]],
-                      "seconds": [l["seconds"] for l in ld["lanes"]]},
+                      "cu_ranges": srv.cu_ranges, "seconds": [l["seconds"] for l in ld["lanes"]]},
             "worst": [{"n": q.n, "steps": q.steps, "lane": srv.lane_of(q), "latency_s": float(q.latency),
                        "wait_s": float(q.admitted - q.arrival), "isolated_s": float(i)}
                       for q, i in sorted(zip(reqs, iso), key=lambda x: -x[0].latency / x[1])[:3]]})

@@ -306,13 +306,18 @@ class LaneServer:
 
     def __init__(self, precision: str = "fp64", hbm_budget: float = 64 * 2 ** 30,
                  bounds: tuple = (6e3, 4e4), chunk_seconds: tuple = (0.01, 0.04, 0.2),
-                 cu_fractions: tuple | None = None, cu_spans: tuple | None = None, device: int = 0,
+                 cu_fractions: tuple | None = None,
+                 cu_spans: tuple | None = ((0.0, 0.125), (0.125, 0.375), (0.375, 1.0)), device: int = 0,
                  arbiter_hold: float = 0.0, **kw):
         """cu_fractions: share of the device's compute units given to each
         lane (disjoint ranges of CU ids); cu_spans: per lane a (first, last)
         fraction of the CU id range, which may overlap (a small-request lane
         can keep a slice to itself while a larger lane shares the rest);
-        None lets every lane use all."""
+        None lets every lane use all. The default spans give the three lanes
+        1/8, 1/4 and 5/8 of the CUs to themselves: a small system's ~45
+        dependent launches per step otherwise each wait for CU slots behind
+        the long-running blocks of the large systems' pair kernels (measured
+        7-50x slower steps), at the cost of the saturated throughput."""
         assert len(chunk_seconds) == len(bounds) + 1
         self.bounds = bounds
         # arbiter_hold > 0: a lane holds its chunks while a lane of smaller
