@@ -845,6 +845,24 @@ __device__ __forceinline__ double quad_sum(double v) {
 // Queue entries: (j << 6) | lane of i; PAIR_PAD marks a padding slot.
 constexpr int32_t PAIR_PAD = (int32_t)0x80000000u;
 
+// A quad leader's running j sums: with the strided queue order, a lane group
+// meets the consecutive quads of one candidate in consecutive batches, so
+// the leader adds their quad sums in registers and writes the window
+// accumulator once per candidate (and at the end of the flush) instead of
+// once per batch.
+struct JRun {
+    int32_t js = -1;  // window slot of the candidate being summed, -1: none
+    double s[6] = {0, 0, 0, 0, 0, 0};
+};
+template <bool FORCE>
+__device__ __forceinline__ void jrun_flush(IWave<FORCE>& W, JRun& run) {
+    constexpr int NA = IWave<FORCE>::NA;
+    if (run.js >= 0)
+        for (int d = 0; d < NA; ++d) atomicAdd(&W.jacc[d][run.js], run.s[d]);
+    run.js = -1;
+    for (int d = 0; d < NA; ++d) run.s[d] = 0.0;
+}
+
 // One queued pair (lane il of the chunk, sorted index j in the window that
 // starts at jb), evaluated once for both particles. The four lanes of a quad
 // share j (a candidate's hits are queued in whole quads); j's terms are summed
@@ -856,7 +874,7 @@ template <bool FORCE, bool COATING, bool MIXED>
 __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __restrict__ rec_mom, const PairParams& pp,
                                            const IChunk<MIXED, COATING>& C, IWave<FORCE>& W, int32_t entry,
                                            bool active, bool leader, bool moved_any, int32_t jb, double press[3],
-                                           int64_t& npairs, int32_t s0) {
+                                           int64_t& npairs, int32_t s0, JRun& run) {
     constexpr int FB = FORCE ? 3 : 0;  // first field component of the accumulators
     active = active && entry >= 0;
     const int il = entry & 63;
@@ -914,9 +932,13 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         if (FORCE)
             for (int d = 0; d < 3; ++d) f[d] = quad_sum(f[d]);
         if (leader) {
-            for (int d = 0; d < 3; ++d) atomicAdd(&W.jacc[FB + d][js], (double)b[d]);
+            if (js != run.js) {
+                jrun_flush<FORCE>(W, run);
+                run.js = js;
+            }
+            for (int d = 0; d < 3; ++d) run.s[FB + d] += (double)b[d];
             if (FORCE)
-                for (int d = 0; d < 3; ++d) atomicAdd(&W.jacc[d][js], (double)f[d]);
+                for (int d = 0; d < 3; ++d) run.s[d] += (double)f[d];
         }
     } else if constexpr (!COATING) {
         // FP64 without coating: u from i to j. The reference orients every
@@ -947,15 +969,19 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
                 for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[d][il], -F[d]);
             }
         }
-        for (int d = 0; d < 3; ++d) {
-            const double b = quad_sum(valid ? Bj[d] : 0.0);
-            if (leader) atomicAdd(&W.jacc[FB + d][js], b);
-        }
+        double b[3], f[3];
+        for (int d = 0; d < 3; ++d) b[d] = quad_sum(valid ? Bj[d] : 0.0);
         if (FORCE)
-            for (int d = 0; d < 3; ++d) {
-                const double f = quad_sum(valid ? F[d] : 0.0);
-                if (leader) atomicAdd(&W.jacc[d][js], f);
+            for (int d = 0; d < 3; ++d) f[d] = quad_sum(valid ? F[d] : 0.0);
+        if (leader) {
+            if (js != run.js) {
+                jrun_flush<FORCE>(W, run);
+                run.js = js;
             }
+            for (int d = 0; d < 3; ++d) run.s[FB + d] += b[d];
+            if (FORCE)
+                for (int d = 0; d < 3; ++d) run.s[d] += f[d];
+        }
     } else {
         // canonical lo -> hi orientation of CompInteractVecs (src/interaction.cpp:99-104):
         // the short-range terms take the lo particle's radius first
@@ -1015,12 +1041,14 @@ __device__ __forceinline__ void pair_flush(const Geometry& g, const double4* __r
     const int lane = lane_id();
     const int nq = qn >> 2;
     const int nb = (nq + WAVE / 4 - 1) / (WAVE / 4);
+    JRun run;
     for (int b = 0; b < nb; ++b) {
         const int qi = (lane >> 2) * nb + b;
         const bool act = qi < nq;
         pair_dense<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, act ? W.q[4 * qi + (lane & 3)] : PAIR_PAD, act,
-                                          act && (lane & 3) == 0, moved_any, jb, press, npairs, s0);
+                                          act && (lane & 3) == 0, moved_any, jb, press, npairs, s0, run);
     }
+    jrun_flush<FORCE>(W, run);
     wave_sync();
     for (int p = lane; p < qn; p += WAVE) W.q[p] = PAIR_PAD;
     wave_sync();
