@@ -586,6 +586,15 @@ struct IWave {
     double acc[NA][WAVE];    // this wavefront's sums of the i particles
     double jacc[NA][WAVE];   // sums of the current window's j particles
 };
+// The periodic image of the current window's cell: j + (x, y, z) is the image
+// nearest the i cell, which for a pair within r_c < L/2 is the minimum image,
+// so the separation needs one add per component. With one cell per box side
+// (`wrap`) every image is in the one cell and the minimum image is taken per
+// pair.
+struct WinImage {
+    double x, y, z;
+    bool wrap;
+};
 
 // Positions of lo and hi as the reference's sequential scan sees them when it
 // reaches pair (lo, hi): the last overlap move of each with a smaller
@@ -670,35 +679,38 @@ __device__ __forceinline__ float erfcx32(float x) {
 }
 
 // FP32 terms of one pair in units of ell (lengths) and mref (moments): field
-// on i from j and (FORCE) the force on j, u the unit vector from i to j.
+// on i from j and (FORCE) the force on j, from the separation v = r u from i
+// to j (|v| = rs, irs = 1 / rs). With the reference's B, C, D (a1, a2, a3
+// here) and u = v / r: C (mj.u) r^2 u = C (mj.v) v, and the force's
+// (C mi.mj - D r^2 (mi.u)(mj.u)) r u + C r ((mj.u) mi + (mi.u) mj)
+// = (C mi.mj - D (mi.v)(mj.v)) v + C ((mj.v) mi + (mi.v) mj), so u is never
+// formed.
 template <bool FORCE>
-__device__ __forceinline__ void pair_terms32(const PairParams& pp, float rs, float irs, float fx, float fy, float fz,
+__device__ __forceinline__ void pair_terms32(const PairParams& pp, float rs, float irs, float vx, float vy, float vz,
                                              float mix, float miy, float miz, float mjx, float mjy, float mjz,
                                              float B[3], float Bo[3], float F[3]) {
     const float irs2 = irs * irs;
     const float x = pp.al_s * rs;
     const float gau = __expf(-x * x);
-    const float t = 2.0f * pp.al_s * gau * 0.5641895835477563f * irs2;
+    const float t = (1.1283791670955126f * pp.al_s) * (gau * irs2);  // 2 alpha / sqrt(pi) e^-x^2 / r^2
     const float a1 = gau * erfcx32(x) * (irs2 * irs) + t;
-    const float a2 = 3.0f * a1 * irs2 + 2.0f * pp.al2_s * t;
-    const float dj = mjx * fx + mjy * fy + mjz * fz;
-    const float di = mix * fx + miy * fy + miz * fz;
-    const float g2 = a2 * dj * rs * rs;
-    B[0] = g2 * fx - a1 * mjx;
-    B[1] = g2 * fy - a1 * mjy;
-    B[2] = g2 * fz - a1 * mjz;
-    const float h2 = a2 * di * rs * rs;
-    Bo[0] = h2 * fx - a1 * mix;
-    Bo[1] = h2 * fy - a1 * miy;
-    Bo[2] = h2 * fz - a1 * miz;
+    const float a2 = 3.0f * a1 * irs2 + (2.0f * pp.al2_s) * t;
+    const float dj = mjx * vx + mjy * vy + mjz * vz;
+    const float di = mix * vx + miy * vy + miz * vz;
+    const float g = a2 * dj, h = a2 * di;
+    B[0] = g * vx - a1 * mjx;
+    B[1] = g * vy - a1 * mjy;
+    B[2] = g * vz - a1 * mjz;
+    Bo[0] = h * vx - a1 * mix;
+    Bo[1] = h * vy - a1 * miy;
+    Bo[2] = h * vz - a1 * miz;
     if (FORCE) {
-        const float a3 = 5.0f * a2 * irs2 + 4.0f * pp.al4_s * t;
+        const float a3 = 5.0f * a2 * irs2 + (4.0f * pp.al4_s) * t;
         const float mimj = mix * mjx + miy * mjy + miz * mjz;
-        const float c0 = (a2 * mimj - a3 * di * dj * rs * rs) * rs;
-        const float c1 = a2 * rs;
-        F[0] = c0 * fx + c1 * (dj * mix + di * mjx);
-        F[1] = c0 * fy + c1 * (dj * miy + di * mjy);
-        F[2] = c0 * fz + c1 * (dj * miz + di * mjz);
+        const float c0 = a2 * mimj - a3 * di * dj;
+        F[0] = c0 * vx + (g * mix + h * mjx);
+        F[1] = c0 * vy + (g * miy + h * mjy);
+        F[2] = c0 * vz + (g * miz + h * mjz);
     }
 }
 
@@ -744,8 +756,9 @@ __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, doubl
         }
     } else {
         const double s = pp.inv_mref;
+        const double rl = r * pp.inv_ell;
         float b[3], bo[3], f[3];
-        pair_terms32<FORCE>(pp, (float)(r * pp.inv_ell), (float)(ir * pp.ell), (float)ux, (float)uy, (float)uz,
+        pair_terms32<FORCE>(pp, (float)rl, (float)(ir * pp.ell), (float)(ux * rl), (float)(uy * rl), (float)(uz * rl),
                             (float)(mi.x * s), (float)(mi.y * s), (float)(mi.z * s), (float)(mj.x * s),
                             (float)(mj.y * s), (float)(mj.z * s), b, bo, f);
         for (int d = 0; d < 3; ++d) {
@@ -850,46 +863,63 @@ constexpr int32_t PAIR_PAD = (int32_t)0x80000000u;
 // the leader adds their quad sums in registers and writes the window
 // accumulator once per candidate (and at the end of the flush) instead of
 // once per batch.
+// The mixed kernel keeps the run in FP32: it spans the batches of one flush
+// (at most PAIR_QCAP / 64 quad sums of FP32 terms).
+template <typename T>
 struct JRun {
     int32_t js = -1;  // window slot of the candidate being summed, -1: none
-    double s[6] = {0, 0, 0, 0, 0, 0};
+    T s[6] = {0, 0, 0, 0, 0, 0};
 };
-template <bool FORCE>
-__device__ __forceinline__ void jrun_flush(IWave<FORCE>& W, JRun& run) {
+template <bool FORCE, typename T>
+__device__ __forceinline__ void jrun_flush(IWave<FORCE>& W, JRun<T>& run) {
     constexpr int NA = IWave<FORCE>::NA;
     if (run.js >= 0)
-        for (int d = 0; d < NA; ++d) atomicAdd(&W.jacc[d][run.js], run.s[d]);
+        for (int d = 0; d < NA; ++d) atomicAdd(&W.jacc[d][run.js], (double)run.s[d]);
     run.js = -1;
-    for (int d = 0; d < NA; ++d) run.s[d] = 0.0;
+    for (int d = 0; d < NA; ++d) run.s[d] = 0;
 }
 
 // One queued pair (lane il of the chunk, sorted index j in the window that
-// starts at jb), evaluated once for both particles. The four lanes of a quad
+// starts at jb, whose cell lies at image im), evaluated once for both particles. The four lanes of a quad
 // share j (a candidate's hits are queued in whole quads); j's terms are summed
 // over the quad and added once by its first lane (`leader`). Every load is
 // issued first (one memory round trip); padding and inactive lanes read
 // entry 0's records, which exist, and are masked. Pairs with an endpoint
 // moved by the overlap correction are left to k_pairs_moved.
-template <bool FORCE, bool COATING, bool MIXED>
+template <bool FORCE, bool COATING, bool MIXED, typename RT>
 __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __restrict__ rec_mom, const PairParams& pp,
                                            const IChunk<MIXED, COATING>& C, IWave<FORCE>& W, int32_t entry,
-                                           bool active, bool leader, bool moved_any, int32_t jb, double press[3],
-                                           int64_t& npairs, int32_t s0, JRun& run) {
+                                           bool active, bool leader, bool moved_any, int32_t jb, const WinImage& im,
+                                           double press[3], float press32[3], int64_t& npairs, uint32_t s0,
+                                           JRun<RT>& run) {
     constexpr int FB = FORCE ? 3 : 0;  // first field component of the accumulators
     active = active && entry >= 0;
-    const int il = entry & 63;
-    const int32_t j = (entry >> 6) & 0x1FFFFFF;  // padding: 0
-    const int js = j - jb;
+    const uint32_t il = entry & 63;
+    const uint32_t j = (entry >> 6) & 0x1FFFFFF;  // padding: 0
+    const int js = (int)j - jb;
     // the mixed kernel reads its i records through L1 (the chunk's 64 records
     // stay cached), which frees the LDS for the accumulators; the FP64
     // kernels, ALU-bound, keep them in LDS
     constexpr bool kIGlobal = MIXED && !COATING;
-    const double4 pi = kIGlobal ? g.rec_pos[s0 + il] : C.ipos[il];
+    const double4 pi = kIGlobal ? (g.rec_pos + s0)[il] : C.ipos[il];  // chunk base + lane: 32-bit offsets
     const double4 pj = g.rec_pos[j];
     const auto mi = [&] {
-        if constexpr (kIGlobal) return g.rec_mom32[s0 + il];
+        if constexpr (kIGlobal) return (g.rec_mom32 + s0)[il];
         else return C.imom[il];
     }();
+    // separation from i to the minimum image of j
+    auto sep = [&](double& vx, double& vy, double& vz) {
+        if (im.wrap) {
+            const double L = g.L, iL = g.inv_L;
+            vx = min_image_r(pj.x - pi.x, L, iL);
+            vy = min_image_r(pj.y - pi.y, L, iL);
+            vz = min_image_r(pj.z - pi.z, L, iL);
+        } else {
+            vx = (pj.x + im.x) - pi.x;
+            vy = (pj.y + im.y) - pi.y;
+            vz = (pj.z + im.z) - pi.z;
+        }
+    };
     const auto mj = [&] {
         if constexpr (MIXED && !COATING) return g.rec_mom32[j];
         else return rec_mom[j];
@@ -900,25 +930,27 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
     if constexpr (MIXED && !COATING) {
         // separation in FP64 (positions are absolute), everything after it in
         // FP32 in units of ell = r_c, moments in units of mref (FP32 copies);
-        // u from i to j
-        const double L = g.L, iL = g.inv_L;
-        const float vx = (float)(min_image_r(pj.x - pi.x, L, iL) * pp.inv_ell);
-        const float vy = (float)(min_image_r(pj.y - pi.y, L, iL) * pp.inv_ell);
-        const float vz = (float)(min_image_r(pj.z - pi.z, L, iL) * pp.inv_ell);
+        // v from i to j
+        double dx, dy, dz;
+        sep(dx, dy, dz);
+        const float vx = (float)(dx * pp.inv_ell);
+        const float vy = (float)(dy * pp.inv_ell);
+        const float vz = (float)(dz * pp.inv_ell);
         const float r2 = vx * vx + vy * vy + vz * vz;
         valid = valid && r2 < 1.0f;
         const float irs = __builtin_amdgcn_rsqf(r2);
         const float rs = r2 * irs;
-        const float ux = vx * irs, uy = vy * irs, uz = vz * irs;
         float Bi[3], Bj[3], F[3];
-        pair_terms32<FORCE>(pp, rs, irs, ux, uy, uz, mi.x, mi.y, mi.z, mj.x, mj.y, mj.z, Bi, Bj, F);
+        pair_terms32<FORCE>(pp, rs, irs, vx, vy, vz, mi.x, mi.y, mi.z, mj.x, mj.y, mj.z, Bi, Bj, F);
         if (valid) {
             ++npairs;
             for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[FB + d][il], (double)Bi[d]);
             if (FORCE) {
-                press[0] += (double)(rs * ux * F[1]);
-                press[1] += (double)(rs * ux * F[2]);
-                press[2] += (double)(rs * uy * F[2]);
+                // the flush's FP32 partials (a few pairs per lane) join the
+                // FP64 sums at its end
+                press32[0] += vx * F[1];
+                press32[1] += vx * F[2];
+                press32[2] += vy * F[2];
                 for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[d][il], -(double)F[d]);
             }
         }
@@ -936,9 +968,9 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
                 jrun_flush<FORCE>(W, run);
                 run.js = js;
             }
-            for (int d = 0; d < 3; ++d) run.s[FB + d] += (double)b[d];
+            for (int d = 0; d < 3; ++d) run.s[FB + d] += b[d];
             if (FORCE)
-                for (int d = 0; d < 3; ++d) run.s[d] += (double)f[d];
+                for (int d = 0; d < 3; ++d) run.s[d] += f[d];
         }
     } else if constexpr (!COATING) {
         // FP64 without coating: u from i to j. The reference orients every
@@ -947,10 +979,8 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         // orientation changes them only by a sign (and by rounding where a sum
         // of two products is contracted the other way round), and no select
         // on the orientation is needed.
-        const double L = g.L, iL = g.inv_L;
-        const double vx = min_image_r(pj.x - pi.x, L, iL);
-        const double vy = min_image_r(pj.y - pi.y, L, iL);
-        const double vz = min_image_r(pj.z - pi.z, L, iL);
+        double vx, vy, vz;
+        sep(vx, vy, vz);
         const double r2 = vx * vx + vy * vy + vz * vz;
         const double ir = rsqrt64(r2);
         const double r = r2 * ir;
@@ -1036,18 +1066,23 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
 template <bool FORCE, bool COATING, bool MIXED>
 __device__ __forceinline__ void pair_flush(const Geometry& g, const double4* __restrict__ rec_mom,
                                            const PairParams& pp, const IChunk<MIXED, COATING>& C, IWave<FORCE>& W,
-                                           int qn, bool moved_any, int32_t jb, double press[3], int64_t& npairs, int32_t s0) {
+                                           int qn, bool moved_any, int32_t jb, const WinImage& im, double press[3],
+                                           int64_t& npairs, int32_t s0) {
     wave_sync();
     const int lane = lane_id();
     const int nq = qn >> 2;
     const int nb = (nq + WAVE / 4 - 1) / (WAVE / 4);
-    JRun run;
+    JRun<typename std::conditional<MIXED && !COATING, float, double>::type> run;
+    float press32[3] = {0.0f, 0.0f, 0.0f};
     for (int b = 0; b < nb; ++b) {
         const int qi = (lane >> 2) * nb + b;
         const bool act = qi < nq;
         pair_dense<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, act ? W.q[4 * qi + (lane & 3)] : PAIR_PAD, act,
-                                          act && (lane & 3) == 0, moved_any, jb, press, npairs, s0, run);
+                                          act && (lane & 3) == 0, moved_any, jb, im, press, press32, npairs,
+                                          (uint32_t)s0, run);
     }
+    if constexpr (FORCE && MIXED && !COATING)
+        for (int d = 0; d < 3; ++d) press[d] += (double)press32[d];
     jrun_flush<FORCE>(W, run);
     wave_sync();
     for (int p = lane; p < qn; p += WAVE) W.q[p] = PAIR_PAD;
@@ -1136,6 +1171,7 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
         if (jy < 0) { jy += nc; sy = -L; } else if (jy >= nc) { jy -= nc; sy = L; }
         if (jz < 0) { jz += nc; sz = -L; } else if (jz >= nc) { jz -= nc; sz = L; }
         const int c = R.cell_off + (jz * nc + jy) * nc + jx;
+        const WinImage im{sx, sy, sz, allpairs};
         const int32_t ce = g.cell_end[c];
         // in the cell itself only j > i: the blocks from this chunk on
         const int32_t cs = self ? s0 : g.cell_start[c];
@@ -1179,14 +1215,16 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
                     if (hit) W.q[qn + mbcnt(m)] = ((jb + t) << 6) | lane;
                     qn += (__popcll(m) + 3) & ~3;
                     if (qn >= PAIR_QFLUSH) {
-                        pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, press, npairs, s0);
+                        pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, im, press, npairs,
+                                                          s0);
                         qn = 0;
                     }
                 }
             }
             if (any) {
                 // window done: its j sums go to the global sums
-                if (qn > 0) pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, press, npairs, s0);
+                if (qn > 0)
+                    pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, im, press, npairs, s0);
                 qn = 0;
                 if (lane < cnt)
                     for (int d = 0; d < IW::NA; ++d)
