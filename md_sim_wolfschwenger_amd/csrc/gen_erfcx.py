@@ -59,7 +59,8 @@ def main():
             f.write("    %s,\n" % np.format_float_scientific(a, unique=True))
         f.write("};\n\n")
         f.write("constexpr float kErfcxK32 = %rf;\nconstexpr int kErfcxDeg32 = %d;\n" % (k32, d32))
-        f.write("__constant__ float kErfcx32[%d] = {\n" % (d32 + 1))
+        # constexpr: the FP32 Horner steps take them as instruction literals
+        f.write("constexpr float kErfcx32[%d] = {\n" % (d32 + 1))
         for a in m32.astype(np.float32):
             f.write("    %sf,\n" % np.format_float_scientific(a, unique=True))
         f.write("};\n\n}  // namespace mds\n")

@@ -45,6 +45,10 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 // uniform over the wavefront (so what it indexes stays in scalar registers).
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); }
 
+// Ballot of a condition, taken from the compare's lane mask (HIP's __ballot
+// takes an int and makes the condition a 0/1 value and back first).
+__device__ __forceinline__ uint64_t ballot(bool c) { return __builtin_amdgcn_ballot_w64(c); }
+
 __device__ __forceinline__ int mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
 }
@@ -572,17 +576,29 @@ __global__ void k_mean_z_final(int64_t nch, int64_t n, const double* __restrict_
 // at the end. Every pair is evaluated in the orientation of its lower
 // original index (the reference's i < j), so both particles see the
 // reference's geometry.
-constexpr int PAIR_QFLUSH = 192;                // evaluate the queue at this fill (slots, whole quads)
-constexpr int PAIR_QCAP = PAIR_QFLUSH + WAVE;   // one candidate adds at most 64
+// A wavefront evaluates its queue once it holds this many slots (whole
+// quads), and at the end of each window; one candidate adds at most 64 slots
+// more. Longer queues give more batches per flush (loads of one batch overlap
+// the arithmetic of others) at the cost of LDS; the sizes are the fastest of
+// 128-640 measured at C3 for each kernel.
+template <bool FORCE, bool MIXED>
+constexpr int pair_qflush() {
+    return MIXED ? 384 : (FORCE ? 512 : 640);
+}
 template <bool MIXED, bool COATING>
 struct IChunk {
     double4 ipos[WAVE];      // positions; .w is the record's tag
     typename std::conditional<MIXED && !COATING, float4, double4>::type imom[WAVE];
 };
-template <bool FORCE>
+// A wavefront's queue and FP64 accumulators (LDS atomics: `ds_add_f64`; FP32
+// accumulators with `ds_add_f32` ran the mixed kernel 4x slower).
+template <bool FORCE, int QF>
 struct IWave {
+    static_assert(QF % WAVE == 0, "the flush reads whole 64-slot batches of the queue");
     static constexpr int NA = FORCE ? 6 : 3;    // (force,) field components
-    int32_t q[PAIR_QCAP];
+    static constexpr int QFLUSH = QF;
+    static constexpr int QCAP = QF + WAVE;
+    int32_t q[QCAP];
     double acc[NA][WAVE];    // this wavefront's sums of the i particles
     double jacc[NA][WAVE];   // sums of the current window's j particles
 };
@@ -685,16 +701,24 @@ __device__ __forceinline__ float erfcx32(float x) {
 // (C mi.mj - D r^2 (mi.u)(mj.u)) r u + C r ((mj.u) mi + (mi.u) mj)
 // = (C mi.mj - D (mi.v)(mj.v)) v + C ((mj.v) mi + (mi.v) mj), so u is never
 // formed.
+// With G = e^-x^2 / r^2 and c = 2 alpha / sqrt(pi) (t = c G):
+// a1 = G (erfcx(x) / r + c), a2 = 3 a1 / r^2 + 2 alpha^2 c G,
+// a3 = 5 a2 / r^2 + 4 alpha^4 c G. e^-x^2 is 2^(r^2 (-alpha^2 log2 e)).
+// `on` false gives exact zeros (a1 = a2 = a3 = 0) whatever r is.
 template <bool FORCE>
-__device__ __forceinline__ void pair_terms32(const PairParams& pp, float rs, float irs, float vx, float vy, float vz,
-                                             float mix, float miy, float miz, float mjx, float mjy, float mjz,
-                                             float B[3], float Bo[3], float F[3]) {
+__device__ __forceinline__ void pair_terms32(const PairParams& pp, float r2, float rs, float irs, float vx, float vy,
+                                             float vz, float mix, float miy, float miz, float mjx, float mjy,
+                                             float mjz, float B[3], float Bo[3], float F[3], bool on = true) {
     const float irs2 = irs * irs;
     const float x = pp.al_s * rs;
-    const float gau = __expf(-x * x);
-    const float t = (1.1283791670955126f * pp.al_s) * (gau * irs2);  // 2 alpha / sqrt(pi) e^-x^2 / r^2
-    const float a1 = gau * erfcx32(x) * (irs2 * irs) + t;
-    const float a2 = 3.0f * a1 * irs2 + (2.0f * pp.al2_s) * t;
+    const float c = 1.1283791670955126f * pp.al_s;
+    const float G = __builtin_amdgcn_exp2f(r2 * (-1.4426950408889634f * pp.al2_s)) * irs2;
+    float a1 = G * fmaf(erfcx32(x), irs, c);
+    float a2 = fmaf(a1, 3.0f * irs2, (2.0f * pp.al2_s * c) * G);
+    if (!on) {
+        a1 = 0.0f;
+        a2 = 0.0f;
+    }
     const float dj = mjx * vx + mjy * vy + mjz * vz;
     const float di = mix * vx + miy * vy + miz * vz;
     const float g = a2 * dj, h = a2 * di;
@@ -705,7 +729,7 @@ __device__ __forceinline__ void pair_terms32(const PairParams& pp, float rs, flo
     Bo[1] = h * vy - a1 * miy;
     Bo[2] = h * vz - a1 * miz;
     if (FORCE) {
-        const float a3 = 5.0f * a2 * irs2 + (4.0f * pp.al4_s) * t;
+        const float a3 = on ? fmaf(a2, 5.0f * irs2, (4.0f * pp.al4_s * c) * G) : 0.0f;
         const float mimj = mix * mjx + miy * mjy + miz * mjz;
         const float c0 = a2 * mimj - a3 * di * dj;
         F[0] = c0 * vx + (g * mix + h * mjx);
@@ -758,7 +782,8 @@ __device__ __forceinline__ void pair_terms(const PairParams& pp, double r, doubl
         const double s = pp.inv_mref;
         const double rl = r * pp.inv_ell;
         float b[3], bo[3], f[3];
-        pair_terms32<FORCE>(pp, (float)rl, (float)(ir * pp.ell), (float)(ux * rl), (float)(uy * rl), (float)(uz * rl),
+        pair_terms32<FORCE>(pp, (float)(rl * rl), (float)rl, (float)(ir * pp.ell), (float)(ux * rl), (float)(uy * rl),
+                            (float)(uz * rl),
                             (float)(mi.x * s), (float)(mi.y * s), (float)(mi.z * s), (float)(mj.x * s),
                             (float)(mj.y * s), (float)(mj.z * s), b, bo, f);
         for (int d = 0; d < 3; ++d) {
@@ -864,15 +889,15 @@ constexpr int32_t PAIR_PAD = (int32_t)0x80000000u;
 // accumulator once per candidate (and at the end of the flush) instead of
 // once per batch.
 // The mixed kernel keeps the run in FP32: it spans the batches of one flush
-// (at most PAIR_QCAP / 64 quad sums of FP32 terms).
+// (at most 16 quad sums of FP32 terms: one candidate's hits).
 template <typename T>
 struct JRun {
     int32_t js = -1;  // window slot of the candidate being summed, -1: none
     T s[6] = {0, 0, 0, 0, 0, 0};
 };
-template <bool FORCE, typename T>
-__device__ __forceinline__ void jrun_flush(IWave<FORCE>& W, JRun<T>& run) {
-    constexpr int NA = IWave<FORCE>::NA;
+template <typename IW, typename T>
+__device__ __forceinline__ void jrun_flush(IW& W, JRun<T>& run) {
+    constexpr int NA = IW::NA;
     if (run.js >= 0)
         for (int d = 0; d < NA; ++d) atomicAdd(&W.jacc[d][run.js], (double)run.s[d]);
     run.js = -1;
@@ -888,12 +913,14 @@ __device__ __forceinline__ void jrun_flush(IWave<FORCE>& W, JRun<T>& run) {
 // moved by the overlap correction are left to k_pairs_moved.
 template <bool FORCE, bool COATING, bool MIXED, typename RT>
 __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __restrict__ rec_mom, const PairParams& pp,
-                                           const IChunk<MIXED, COATING>& C, IWave<FORCE>& W, int32_t entry,
-                                           bool active, bool leader, bool moved_any, int32_t jb, const WinImage& im,
-                                           double press[3], float press32[3], int64_t& npairs, uint32_t s0,
-                                           JRun<RT>& run) {
+                                           const IChunk<MIXED, COATING>& C,
+                                           IWave<FORCE, pair_qflush<FORCE, MIXED>()>& W, int32_t entry,
+                                           bool moved_any, int32_t jb, const WinImage& im, double press[3],
+                                           float press32[3], int64_t& npairs, uint32_t s0, JRun<RT>& run) {
     constexpr int FB = FORCE ? 3 : 0;  // first field component of the accumulators
-    active = active && entry >= 0;
+    const bool active = entry >= 0;
+    // a quad's first slot holds a hit whenever the quad holds one
+    const bool leader = active && (lane_id() & 3) == 0;
     const uint32_t il = entry & 63;
     const uint32_t j = (entry >> 6) & 0x1FFFFFF;  // padding: 0
     const int js = (int)j - jb;
@@ -925,7 +952,8 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         else return rec_mom[j];
     }();
     const long long ti = __double_as_longlong(pi.w), tj = __double_as_longlong(pj.w);
-    bool valid = active && !(moved_any && ((ti | tj) >> 32));  // not if an endpoint moved
+    // not if an endpoint moved (flag in the tag's high word)
+    bool valid = active && !(moved_any && ((uint32_t)(ti >> 32) | (uint32_t)(tj >> 32)));
     const bool i_lo = (int32_t)ti < (int32_t)tj;
     if constexpr (MIXED && !COATING) {
         // separation in FP64 (positions are absolute), everything after it in
@@ -938,39 +966,47 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         const float vz = (float)(dz * pp.inv_ell);
         const float r2 = vx * vx + vy * vy + vz * vz;
         valid = valid && r2 < 1.0f;
+        // the pair count from compare masks (a ballot of the combined
+        // condition would first turn it into a 0/1 value)
+        npairs += __popcll(ballot(active) & ballot(r2 < 1.0f) &
+                           (moved_any ? ballot(((uint32_t)(ti >> 32) | (uint32_t)(tj >> 32)) == 0) : ~0ull));
         const float irs = __builtin_amdgcn_rsqf(r2);
         const float rs = r2 * irs;
+        // every lane evaluates (no branch); the terms of lanes without a pair
+        // are exact zeros, so the quad sums need no selects
         float Bi[3], Bj[3], F[3];
-        pair_terms32<FORCE>(pp, rs, irs, vx, vy, vz, mi.x, mi.y, mi.z, mj.x, mj.y, mj.z, Bi, Bj, F);
-        if (valid) {
-            ++npairs;
-            for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[FB + d][il], (double)Bi[d]);
-            if (FORCE) {
-                // the flush's FP32 partials (a few pairs per lane) join the
-                // FP64 sums at its end
-                press32[0] += vx * F[1];
-                press32[1] += vx * F[2];
-                press32[2] += vy * F[2];
-                for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[d][il], -(double)F[d]);
-            }
+        pair_terms32<FORCE>(pp, r2, rs, irs, vx, vy, vz, mi.x, mi.y, mi.z, mj.x, mj.y, mj.z, Bi, Bj, F, valid);
+        if (FORCE) {
+            // the window's FP32 partials (a few pairs per lane) join the
+            // FP64 sums when the window is done
+            press32[0] += vx * F[1];
+            press32[1] += vx * F[2];
+            press32[2] += vy * F[2];
         }
-        // j's quad sums: every component's reduction first (their DPP steps
-        // interleave), then the leader's adds under one exec mask
-        float b[3], f[3];
-        for (int d = 0; d < 3; ++d) b[d] = valid ? Bj[d] : 0.0f;
-        if (FORCE)
-            for (int d = 0; d < 3; ++d) f[d] = valid ? F[d] : 0.0f;
-        for (int d = 0; d < 3; ++d) b[d] = quad_sum(b[d]);
-        if (FORCE)
-            for (int d = 0; d < 3; ++d) f[d] = quad_sum(f[d]);
-        if (leader) {
-            if (js != run.js) {
-                jrun_flush<FORCE>(W, run);
-                run.js = js;
-            }
-            for (int d = 0; d < 3; ++d) run.s[FB + d] += b[d];
+        if (valid) {
+            for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[FB + d][il], (double)Bi[d]);
             if (FORCE)
-                for (int d = 0; d < 3; ++d) run.s[d] += f[d];
+                for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[d][il], -(double)F[d]);
+        }
+        // j's quad sums, in every lane, added to every lane's run before the
+        // leader's branch (so each DPP step is one add); when a leader's
+        // candidate changes, the run up to this batch goes to the window
+        // accumulator and the new run starts with this batch's sum (only
+        // the leaders' runs are ever written)
+        constexpr int NA = FORCE ? 6 : 3;
+        float qs[6], prev[6];
+        for (int d = 0; d < 3; ++d) qs[FB + d] = quad_sum(Bj[d]);
+        if (FORCE)
+            for (int d = 0; d < 3; ++d) qs[d] = quad_sum(F[d]);
+        for (int d = 0; d < NA; ++d) {
+            prev[d] = run.s[d];
+            run.s[d] += qs[d];
+        }
+        if (leader && js != run.js) {
+            if (run.js >= 0)
+                for (int d = 0; d < NA; ++d) atomicAdd(&W.jacc[d][run.js], (double)prev[d]);
+            run.js = js;
+            for (int d = 0; d < NA; ++d) run.s[d] = qs[d];
         }
     } else if constexpr (!COATING) {
         // FP64 without coating: u from i to j. The reference orients every
@@ -985,11 +1021,11 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         const double ir = rsqrt64(r2);
         const double r = r2 * ir;
         valid = valid && r < pp.rc;
+        npairs += __popcll(ballot(valid));
         const double ux = vx * ir, uy = vy * ir, uz = vz * ir;
         double Bi[3], Bj[3], F[3];  // field on i, on j; force on j
         pair_terms<FORCE, MIXED>(pp, r, ir, ux, uy, uz, mi, mj, Bi, Bj, F);
         if (valid) {
-            ++npairs;
             for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[FB + d][il], Bi[d]);
             if (FORCE) {
                 const double rr = MIXED ? r * pp.inv_ell : r;
@@ -1005,7 +1041,7 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
             for (int d = 0; d < 3; ++d) f[d] = quad_sum(valid ? F[d] : 0.0);
         if (leader) {
             if (js != run.js) {
-                jrun_flush<FORCE>(W, run);
+                jrun_flush(W, run);
                 run.js = js;
             }
             for (int d = 0; d < 3; ++d) run.s[FB + d] += b[d];
@@ -1023,11 +1059,11 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         const double ir = rsqrt64(r2);
         const double r = r2 * ir;
         valid = valid && r < pp.rc;
+        npairs += __popcll(ballot(valid));
         const double ux = vx * ir, uy = vy * ir, uz = vz * ir;  // u from lo to hi
         double Bl[3], Bh[3], F[3];  // field on lo, on hi; force on hi
         pair_terms<FORCE, MIXED>(pp, r, ir, ux, uy, uz, i_lo ? mi : mj, i_lo ? mj : mi, Bl, Bh, F);
         if (valid) {
-            ++npairs;
             for (int d = 0; d < 3; ++d) atomicAdd(&W.acc[FB + d][il], i_lo ? Bl[d] : Bh[d]);
             if (FORCE) {
                 const double rr = MIXED ? r * pp.inv_ell : r;
@@ -1065,25 +1101,25 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
 // slots are padding again.
 template <bool FORCE, bool COATING, bool MIXED>
 __device__ __forceinline__ void pair_flush(const Geometry& g, const double4* __restrict__ rec_mom,
-                                           const PairParams& pp, const IChunk<MIXED, COATING>& C, IWave<FORCE>& W,
-                                           int qn, bool moved_any, int32_t jb, const WinImage& im, double press[3],
-                                           int64_t& npairs, int32_t s0) {
+                                           const PairParams& pp, const IChunk<MIXED, COATING>& C,
+                                           IWave<FORCE, pair_qflush<FORCE, MIXED>()>& W, int qn, bool moved_any,
+                                           int32_t jb,
+                                           const WinImage& im, double press[3], float press32[3], int64_t& npairs,
+                                           int32_t s0) {
     wave_sync();
     const int lane = lane_id();
     const int nq = qn >> 2;
     const int nb = (nq + WAVE / 4 - 1) / (WAVE / 4);
     JRun<typename std::conditional<MIXED && !COATING, float, double>::type> run;
-    float press32[3] = {0.0f, 0.0f, 0.0f};
+    // qn <= QCAP, a multiple of 64, so nb <= QCAP / 64 and every
+    // slot read, 4 qi + 3 <= 64 nb - 1, is inside the queue; the slots from
+    // qn on hold padding
     for (int b = 0; b < nb; ++b) {
         const int qi = (lane >> 2) * nb + b;
-        const bool act = qi < nq;
-        pair_dense<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, act ? W.q[4 * qi + (lane & 3)] : PAIR_PAD, act,
-                                          act && (lane & 3) == 0, moved_any, jb, im, press, press32, npairs,
-                                          (uint32_t)s0, run);
+        pair_dense<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, W.q[4 * qi + (lane & 3)], moved_any, jb, im, press,
+                                          press32, npairs, (uint32_t)s0, run);
     }
-    if constexpr (FORCE && MIXED && !COATING)
-        for (int d = 0; d < 3; ++d) press[d] += (double)press32[d];
-    jrun_flush<FORCE>(W, run);
+    jrun_flush(W, run);
     wave_sync();
     for (int p = lane; p < qn; p += WAVE) W.q[p] = PAIR_PAD;
     wave_sync();
@@ -1093,7 +1129,8 @@ template <bool FORCE, bool COATING, bool MIXED>
 __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
                                             PairOut out, const int4* __restrict__ stencil_all,
                                             unsigned long long* __restrict__ cnt_pairs) {
-    using IW = IWave<FORCE>;
+    constexpr bool F32 = MIXED && !COATING;
+    using IW = IWave<FORCE, pair_qflush<FORCE, MIXED>()>;
     __shared__ IChunk<MIXED, COATING> C;
     __shared__ IW waves[PAIR_SLICES];
     const int wv = wave_id(), lane = lane_id();
@@ -1129,9 +1166,10 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
         W.acc[d][lane] = 0.0;
         W.jacc[d][lane] = 0.0;
     }
-    for (int p = lane; p < PAIR_QCAP; p += WAVE) W.q[p] = PAIR_PAD;
+    for (int p = lane; p < IW::QCAP; p += WAVE) W.q[p] = PAIR_PAD;
     __syncthreads();
     const bool moved_any = *g.n_events > 0;
+    const uint64_t vmask = ballot(vi);
     const double L = g.L;
     const int nc = R.ncell;
     const bool allpairs = nc == 1;
@@ -1158,6 +1196,7 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     const float bz0 = -wmax(vi ? -pzf : -big), bz1 = wmax(vi ? pzf : -big);
     const float rb2f = rr2f * 1.0001f;
     double press[3] = {0, 0, 0};
+    float press32[3] = {0.0f, 0.0f, 0.0f};  // F32: the current window's pressure partials
     int64_t npairs = 0;
     int qn = 0;
     bool ok = true;
@@ -1196,7 +1235,7 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
             const int cnt = min(WAVE, ce - jb);
             bool any = false;
             // the candidates near the box, in increasing order (uniform mask)
-            for (uint64_t cm = __ballot(near); cm; cm &= cm - 1) {
+            for (uint64_t cm = ballot(near); cm; cm &= cm - 1) {
                 const int t = __builtin_ctzll(cm);
                 float dx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qx), t)) - pxf;
                 float dy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qy), t)) - pyf;
@@ -1206,17 +1245,22 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
                     dy = dy > Lf / 2 ? dy - Lf : (dy < -Lf / 2 ? dy + Lf : dy);
                     dz = dz > Lf / 2 ? dz - Lf : (dz < -Lf / 2 ? dz + Lf : dz);
                 }
+                // lanes that take the pair: valid i, inside, and in the cell
+                // itself only i < j (lane < jb + t - s0); the masks are
+                // combined as scalars
+                const int32_t lo = jb + t - s0;
+                const uint64_t self_mask = !self || lo >= WAVE ? ~0ull : (lo <= 0 ? 0ull : (1ull << lo) - 1);
                 const bool inside = dx * dx + dy * dy + dz * dz < rr2f;
-                const bool hit = vi && (!self || jb + t > k) && inside;
-                const uint64_t m = __ballot(hit);
+                const uint64_t m = ballot(inside) & vmask & self_mask;
+                const bool hit = inside && vi && (!self || lane < lo);  // the lanes of m
                 if (m) {
                     // the hits of one candidate fill whole quads
                     any = true;
                     if (hit) W.q[qn + mbcnt(m)] = ((jb + t) << 6) | lane;
                     qn += (__popcll(m) + 3) & ~3;
-                    if (qn >= PAIR_QFLUSH) {
-                        pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, im, press, npairs,
-                                                          s0);
+                    if (qn >= IW::QFLUSH) {
+                        pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, im, press, press32,
+                                                          npairs, s0);
                         qn = 0;
                     }
                 }
@@ -1224,12 +1268,18 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
             if (any) {
                 // window done: its j sums go to the global sums
                 if (qn > 0)
-                    pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, im, press, npairs, s0);
+                    pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, im, press, press32,
+                                                      npairs, s0);
                 qn = 0;
                 if (lane < cnt)
                     for (int d = 0; d < IW::NA; ++d)
                         ok &= fx_add_comp(out, R, n, FORCE ? d : d + 3, jl, W.jacc[d][lane]);
                 for (int d = 0; d < IW::NA; ++d) W.jacc[d][lane] = 0.0;
+                if (F32 && FORCE)
+                    for (int d = 0; d < 3; ++d) {
+                        press[d] += (double)press32[d];
+                        press32[d] = 0.0f;
+                    }
                 wave_sync();
             }
         }
@@ -1240,10 +1290,7 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
             out.press_partial[3 * ((int64_t)w * PAIR_SLICES + wv) + lane] =
                 (lane == 0 ? p0 : lane == 1 ? p1 : p2) * pp.pscale;
     }
-    if (cnt_pairs) {
-        const int64_t np = wave_sum(npairs);
-        if (lane == 0) atomicAdd(cnt_pairs, (unsigned long long)(2 * np));  // ordered pairs
-    }
+    if (cnt_pairs && lane == 0) atomicAdd(cnt_pairs, (unsigned long long)(2 * npairs));  // ordered pairs (npairs is the wave's)
     // the wavefronts' i sums, in a fixed order
     __syncthreads();
     if (wv == 0 && vi)
