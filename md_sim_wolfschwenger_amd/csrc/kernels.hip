@@ -580,10 +580,12 @@ __global__ void k_mean_z_final(int64_t nch, int64_t n, const double* __restrict_
 // quads), and at the end of each window; one candidate adds at most 64 slots
 // more. Longer queues give more batches per flush (loads of one batch overlap
 // the arithmetic of others) at the cost of LDS; the sizes are the fastest of
-// 128-640 measured at C3 for each kernel.
+// 128-640 measured at C3 for each kernel, the mixed force pass's with the
+// k-space chain beside it (a longer queue runs it alone as fast but leaves
+// less LDS to the chain's blocks).
 template <bool FORCE, bool MIXED>
 constexpr int pair_qflush() {
-    return MIXED ? 384 : (FORCE ? 512 : 640);
+    return MIXED ? (FORCE ? 320 : 384) : (FORCE ? 512 : 640);
 }
 template <bool MIXED, bool COATING>
 struct IChunk {
