@@ -260,6 +260,8 @@ struct mds_ctx {
     DevBuf<KPack> pack_desc;
     DevBuf<double4> pack;
     DevBuf<double> kpartial, tab;
+    DevBuf<float> tab32;     // FP32 sin/cos tables of the matrix-core replicas
+    DevBuf<float4> pack32;   // FP32 tile-major scatter entries of the matrix-core replicas
     // reductions
     DevBuf<double> press_partial, err_partial, scal;  // scal[0] = error norm
     DevBuf<int32_t> scan_tmp;
@@ -480,6 +482,8 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
             const bool half = (ny == 0 && nz == 0);
             if (half && nxmax == 0) continue;
             c->h_rows.push_back({ny, nz, nxmax, (int32_t)c->h_pack.size() - R.pk_off, koff});
+            const int32_t rl = (int32_t)c->h_rows.size() - 1 - R.row_off;
+            auto t32 = [&](int a) { return ((rl / 16) * (nc + 2) + a) * 16 + rl % 16; };
             for (int a = half ? 1 : 0; a <= nxmax; a += KSEG)
                 c->h_segs.push_back({ny, nz, a, std::min(nxmax, a + KSEG - 1), koff, nxmax});
             // scatter table: |nx| = a pairs (+a, ny, nz) with (-a, ny, nz)
@@ -493,10 +497,12 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
                     d.km = a >= 1 ? koff - a + nxmax : -1;
                 }
                 d.e = d.kp >= 0 ? ek((int64_t)a * a + r2) : 0.0;
+                d.t32 = t32(a);
+                d.pad_ = 0;
                 c->h_pack.push_back(d);
             }
             // pad to an even entry count (the scatter takes two at a time)
-            if ((nxmax + 1) % 2) c->h_pack.push_back(KPack{-1, -1, 0.0});
+            if ((nxmax + 1) % 2) c->h_pack.push_back(KPack{-1, -1, 0.0, t32(nxmax + 1), 0});
             koff += half ? nxmax : 2 * nxmax + 1;
         }
     R.nrows = (int32_t)c->h_rows.size() - R.row_off;
@@ -590,6 +596,7 @@ mds_status plan_all(mds_ctx* c) {
     std::vector<int32_t> cell_rep, rep_of((size_t)c->n);
     c->h_chunks.clear();
     int64_t cells = 0, fcells = 0, work_cap = 0, tab = 0, part = 0, kb = 0, kbm = 0, ksm = 0, kall = 0;
+    int64_t tab32 = 0, pk32 = 0;
     int32_t nc_max = 0;
     int32_t nc_mfma_build = 0, nc_mfma_scatter = 0;
     bool any_mfma_build = false, any_mfma_scatter = false, any_vec_build = false, any_vec_scatter = false;
@@ -634,6 +641,10 @@ mds_status plan_all(mds_ctx* c) {
         }
         if (R.kmode & 2) nc_mfma_scatter = std::max(nc_mfma_scatter, R.nc), any_mfma_scatter = true;
         else any_vec_scatter = true;
+        R.tab32_off = tab32;
+        if (R.kmode) tab32 += m.n * 3 * (int64_t)tw4_of(R.nc);
+        R.pk32_off = pk32;
+        if (R.kmode & 2) pk32 += (int64_t)((R.nrows + 15) / 16) * (R.nc + 2) * 16;
         R.kbm_off = (int32_t)kbm;
         kbm += (R.nk > 0) ? (int64_t)((R.nrowg + 3) / 4) * R.kbm_pass * 8 * ((R.nchunks + 7) / 8) : 0;
         std::fill(rep_of.begin() + m.off, rep_of.begin() + m.off + m.n, r);
@@ -686,6 +697,8 @@ mds_status plan_all(mds_ctx* c) {
     HIPCHK(c->pack.ensure(std::max<size_t>(1, c->h_pack.size())));
     HIPCHK(c->kpartial.ensure(std::max<int64_t>(1, part)));
     HIPCHK(c->tab.ensure(std::max<int64_t>(1, tab)));
+    HIPCHK(c->tab32.ensure(std::max<int64_t>(1, tab32)));
+    HIPCHK(c->pack32.ensure(std::max<int64_t>(1, pk32)));
     HIPCHK(c->reps.ensure(c->nrep));
     HIPCHK(c->rep_of.ensure(c->n));
     HIPCHK(c->msum.ensure(3 * (size_t)c->nrep));
@@ -711,6 +724,9 @@ mds_status plan_all(mds_ctx* c) {
     if (!c->h_pack.empty())
         HIPCHK(hipMemcpyAsync(c->pack_desc.p, c->h_pack.data(), c->h_pack.size() * sizeof(KPack),
                               hipMemcpyHostToDevice, s));
+    // the FP32 scatter tiles' slots that no entry writes (rows past the
+    // replica's last, |nx| past the row's end) stay zero
+    if (pk32 > 0) HIPCHK(launch_fill_f64(s, (double*)c->pack32.p, 2 * pk32, 0.0));
     HIPCHK(hipStreamSynchronize(s));
     c->planned = true;
     c->geometry_ready = false;
@@ -830,7 +846,7 @@ mds_status prepare_geometry_once(mds_ctx* c) {
     }
     {
         mds_ctx::Span sp(c, KC_SINCOS);
-        HIPCHK(launch_sincos_tables(s, n, c->reps.p, c->rep_of.p, c->pos0.p, c->tab.p));
+        HIPCHK(launch_sincos_tables(s, n, c->reps.p, c->rep_of.p, c->pos0.p, c->tab.p, c->tab32.p));
     }
     c->geometry_ready = true;
     return MDS_OK;
@@ -879,7 +895,7 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
                 mds_ctx::Span sp(c, KC_KBUILD);
                 if (c->any_mfma_build)
                     HIPCHK(launch_kbuild_mfma(ks, n, c->reps.p, c->nrep, c->kbm_items, c->nc_mfma_build, c->rows.p,
-                                              c->tab.p, c->mom.p, c->kpartial.p));
+                                              c->tab32.p, c->mom.p, c->kpartial.p));
                 if (c->any_vec_build)
                     HIPCHK(launch_kbuild(ks, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p,
                                          c->kpartial.p));
@@ -887,13 +903,13 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
             {
                 mds_ctx::Span sp(c, KC_KREDUCE);
                 HIPCHK(launch_kpack(ks, (int32_t)c->h_pack.size(), c->reps.p, c->nrep, c->pack_desc.p, c->kpartial.p,
-                                    c->pack.p));
+                                    c->pack.p, c->pack32.p));
             }
         }
         mds_ctx::Span sp(c, force_pass ? KC_KSCATTER_FORCE : KC_KSCATTER_FIELD);
         if (c->any_mfma_scatter)
             HIPCHK(launch_kscatter_mfma(ks, force_pass, n, c->reps.p, c->nrep, c->ksm_items, c->nc_mfma_scatter,
-                                        c->rows.p, c->pack.p, c->tab.p, c->mom.p, c->msum.p,
+                                        c->rows.p, c->pack32.p, c->tab32.p, c->mom.p, c->msum.p,
                                         force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
         if (c->any_vec_scatter)
             HIPCHK(launch_kscatter(ks, force_pass, n, c->reps.p, c->rep_of.p, c->rows.p, c->pack.p, c->tab.p,
@@ -2426,6 +2442,8 @@ int64_t mds_device_bytes(const mds_ctx* c) {
     add(c->red_chunks.n, sizeof(int2));
     add(c->red_part.n, sizeof(double));
     add(c->pack.n, sizeof(double4));
+    add(c->tab32.n, sizeof(float));
+    add(c->pack32.n, sizeof(float4));
     add(c->reps.n, sizeof(Rep));
     return b;
 }

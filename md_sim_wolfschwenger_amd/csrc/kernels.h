@@ -20,10 +20,17 @@ struct KRow {
 // or the ny = nz = 0 row's negative side) and e(k) = 2 exp(-w n^2) / n^2
 // (the same for both). k_kpack turns it into (U+ + U-, U+ - U-, V+ + V-,
 // V+ - V-) with U = e Re S, V = e Im S.
+// t32: the entry's slot in the replica's FP32 tile-major copy for the
+// matrix-core scatter, ((row / 16) (n_c + 2) + a) 16 + row % 16.
 struct KPack {
     int32_t kp, km;
     double e;
+    int32_t t32, pad_;
 };
+
+// FP32 sin/cos tables of the matrix-core kernels: per particle and axis the
+// (cos, sin) pairs of n = 0..n_c, zero padded to a multiple of four floats.
+constexpr int tw4_of(int nc) { return (2 * (nc + 1) + 3) & ~3; }
 
 // A segment of a k-space row for the structure-factor build: |nx| in
 // [alo, ahi] (both signs unless the row is ny = nz = 0), at most KSEG values.
@@ -58,6 +65,8 @@ struct Rep {
     int32_t red_off, nred;       // slice of the reduction chunk table
     int32_t ksplit[4];           // first row of each quarter of the scatter table (by entries)
     int64_t tab_off, part_off;   // offsets (doubles) into the sin/cos tables and S(k) partials
+    int64_t tab32_off, pk32_off; // matrix-core path: offsets into the FP32 tables (floats) and the
+                                 // FP32 tile-major scatter entries (float4s)
     double c_force, c_flux, c_self, c_surf, two_pi_over_L;
     double field_z;              // magFluxDens of this replica
     uint64_t seed;               // Philox key of this replica
@@ -204,8 +213,10 @@ hipError_t launch_cell_order(hipStream_t s, int32_t ncells, const int32_t* start
                              int32_t* sorted_idx, int32_t* tmp);
 
 // k space
+// FP64 tables for the replicas with a vector k-space kernel, FP32 ones for
+// the replicas with a matrix-core kernel
 hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
-                                double* tab);
+                                double* tab, float* tab32);
 hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t kb_items, const KSeg* segs,
                          const double* tab, const double* mom, double* partial);
 // The same S(k) partials (same chunks and layout) from FP32 matrix-core
@@ -213,18 +224,20 @@ hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep
 // Every replica needs n_c <= KBM_NCMAX (the block's LDS).
 constexpr int KBM_NCMAX = 50;
 hipError_t launch_kbuild_mfma(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
-                              int32_t nc_max, const KRow* rows, const double* tab, const double* mom,
+                              int32_t nc_max, const KRow* rows, const float* tab32, const double* mom,
                               double* partial);
 // S(k) partials -> packed scatter table (fixed summation order)
 // The k-space scatter from FP32 matrix-core products (mixed-precision path,
 // n_c <= KSM_NCMAX in every replica); blocks: ceil(n_r / 64) per replica.
 constexpr int KSM_NCMAX = 32;
 hipError_t launch_kscatter_mfma(hipStream_t s, bool force, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
-                                int32_t nc_max, const KRow* rows, const double4* pack, const double* tab,
+                                int32_t nc_max, const KRow* rows, const float4* pack32, const float* tab32,
                                 const double* mom, const double* msum, const double* add_force, double* force_out,
                                 double* field);
+// also the FP32 tile-major copy (units of the replica's largest moment) for
+// the replicas scattered on the matrix cores
 hipError_t launch_kpack(hipStream_t s, int32_t npk_all, const Rep* reps, int32_t nrep, const KPack* desc,
-                        const double* partial, double4* pack);
+                        const double* partial, double4* pack, float4* pack32);
 // field (and force) of the k-space sum, self and surface terms (and the
 // thermal force add_force), written to field/force_out in original order;
 // launch_pairs_out adds the real-space sums afterwards

@@ -1478,37 +1478,51 @@ __global__ __launch_bounds__(64) void k_pairs_moved(Geometry g, const double4* _
 
 // ------------------------------------------------------------------ k space
 // cos/sin(2 pi n x_d / L), n = 0..nc, by the Rapaport recurrence of EvalSinCos
-// (src/interaction.cpp:521-548). Layout per particle: [d][n][cos, sin].
+// (src/interaction.cpp:521-548). Layout per particle: [d][n][cos, sin], in
+// FP64 for a replica with a vector k-space kernel and rounded to FP32 (each
+// axis zero padded to tw4_of(nc) floats) for one with a matrix-core kernel.
 __global__ void k_sincos_tab(int64_t n, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
-                             const double* __restrict__ pos0, double* __restrict__ tab_all) {
+                             const double* __restrict__ pos0, double* __restrict__ tab_all,
+                             float* __restrict__ tab32_all) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Rep& R = reps[rep_of[i]];
     const int32_t nc = R.nc;
+    const bool f64 = R.kmode != 3, f32 = R.kmode != 0;
     const int64_t stride = 3 * (int64_t)(nc + 1) * 2;
+    const int tw4 = tw4_of(nc);
     double* t = tab_all + R.tab_off + (i - R.off) * stride;
+    float* t32 = tab32_all + R.tab32_off + (i - R.off) * 3 * (int64_t)tw4;
     for (int d = 0; d < 3; ++d) {
         const double tt = R.two_pi_over_L * pos0[d * n + i];
         double* td = t + d * (nc + 1) * 2;
+        float* td32 = t32 + d * tw4;
+        auto put = [&](int k, double c, double s) {
+            if (f64) {
+                td[2 * k] = c;
+                td[2 * k + 1] = s;
+            }
+            if (f32) {
+                td32[2 * k] = (float)c;
+                td32[2 * k + 1] = (float)s;
+            }
+        };
         double s1, c1;
         sincos(tt, &s1, &c1);
-        td[0] = 1;
-        td[1] = 0;
-        if (nc >= 1) {
-            td[2] = c1;
-            td[3] = s1;
-        }
+        put(0, 1.0, 0.0);
+        if (nc >= 1) put(1, c1, s1);
         const double u = 2.0 * c1;
         double cm2 = 1, sm2 = 0, cm1 = c1, sm1 = s1;
         for (int k = 2; k <= nc; ++k) {
             const double ck = u * cm1 - cm2, sk = u * sm1 - sm2;
-            td[2 * k] = ck;
-            td[2 * k + 1] = sk;
+            put(k, ck, sk);
             cm2 = cm1;
             sm2 = sm1;
             cm1 = ck;
             sm1 = sk;
         }
+        if (f32)
+            for (int k = 2 * (nc + 1); k < tw4; ++k) td32[k] = 0.0f;
     }
 }
 
@@ -1630,18 +1644,18 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
 // two values of a and one row. Accumulation is FP32 within a particle chunk;
 // the chunks are added in FP64 by k_kpack.
 // Block: four wavefronts on four groups of 16 rows, one particle chunk and
-// one pass of KBM_NX values of a; the chunk's table rows are staged in LDS
-// as FP32, KBM_SUB particles at a time (n_c <= KBM_NCMAX: the LDS).
+// one pass of KBM_NX values of a; the chunk's FP32 table rows are copied to
+// LDS, KBM_SUB particles at a time (n_c <= KBM_NCMAX: the LDS).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int KBM_SUB = 64, KBM_WAVES = 4;
 
 __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* __restrict__ reps, int32_t nrep,
                                                      int32_t items, const KRow* __restrict__ rows_all,
-                                                     const double* __restrict__ tab_all,
+                                                     const float* __restrict__ tab32_all,
                                                      const double* __restrict__ mom,
                                                      double* __restrict__ partial_all) {
     // dynamic LDS: [KBM_SUB][2 KBM_NX] x phases, [KBM_SUB][4] moments,
-    // [KBM_SUB][2 tw] y and z tables (tw of this replica's n_c)
+    // [KBM_SUB][2 tw4] y and z tables (tw4 of this replica's n_c)
     extern __shared__ float kbm_smem[];
     float(*sx)[2 * KBM_NX] = (float(*)[2 * KBM_NX])kbm_smem;
     float(*sm)[4] = (float(*)[4])(kbm_smem + KBM_SUB * 2 * KBM_NX);
@@ -1662,16 +1676,16 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
     const int32_t blk = b % nblk, pass = b / nblk;
     const int wv = wave_id(), lane = lane_id(), tid = threadIdx.x;
     const int kk = lane >> 4, col = lane & 15;
-    const int nc = R.nc, tw = 2 * (nc + 1);
-    const int64_t stride = 3 * (int64_t)tw;
-    const double* tab = tab_all + R.tab_off;
+    const int nc = R.nc, tw4 = tw4_of(nc);
+    const int64_t stride = 3 * (int64_t)tw4;
+    const float* tab = tab32_all + R.tab32_off;
     const int64_t n = n_all;
     const int a0 = pass * KBM_NX;
     // this lane's row (B side)
     const int r = (blk * KBM_WAVES + wv) * KBM_ROWS + col;
     const bool vrow = r < R.nrows;
     const KRow row = vrow ? rows_all[R.row_off + r] : KRow{0, 0, -1, 0, 0};
-    const int yo = 2 * (row.ny < 0 ? -row.ny : row.ny), zo = tw + 2 * row.nz;
+    const int yo = 2 * (row.ny < 0 ? -row.ny : row.ny), zo = tw4 + 2 * row.nz;
     const float ysg = row.ny < 0 ? -1.0f : 1.0f;
     const float fny = (float)row.ny, fnz = (float)row.nz;
     f32x4 acc[3][4];
@@ -1682,19 +1696,22 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
     const int64_t jb = (int64_t)ch * R.kchunk;
     const int64_t je = jb + R.kchunk < R.n ? jb + R.kchunk : R.n;
     for (int64_t s0 = jb; s0 < je; s0 += KBM_SUB) {
-        // stage KBM_SUB particles: X for a in [a0, a0 + KBM_NX), the y and z
-        // tables, the moment; zeros past the chunk's end and past n_c
+        // stage KBM_SUB particles, four floats at a time: X for a in
+        // [a0, a0 + KBM_NX), the y and z tables, the moment; zeros past the
+        // chunk's end and past n_c (the tables' own padding up to tw4)
         __syncthreads();
-        for (int e = tid; e < KBM_SUB * 2 * KBM_NX; e += 256) {
-            const int p = e / (2 * KBM_NX), o = e % (2 * KBM_NX);
+        for (int e = tid; e < KBM_SUB * (2 * KBM_NX / 4); e += 256) {
+            const int p = e / (2 * KBM_NX / 4), q = e % (2 * KBM_NX / 4);
             const int64_t jr = s0 + p;
-            const int a = a0 + (o >> 1);
-            sx[p][o] = (jr < je && a <= nc) ? (float)tab[jr * stride + 2 * a + (o & 1)] : 0.0f;
+            const int o = 2 * a0 + 4 * q;
+            ((float4*)sx[p])[q] = (jr < je && o < tw4) ? *(const float4*)(tab + jr * stride + o)
+                                                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
-        for (int e = tid; e < KBM_SUB * 2 * tw; e += 256) {
-            const int p = e / (2 * tw), o = e % (2 * tw);
+        for (int e = tid; e < KBM_SUB * (tw4 / 2); e += 256) {
+            const int p = e / (tw4 / 2), q = e % (tw4 / 2);
             const int64_t jr = s0 + p;
-            syz[e] = jr < je ? (float)tab[jr * stride + tw + o] : 0.0f;
+            ((float4*)syz)[e] = jr < je ? *(const float4*)(tab + jr * stride + tw4 + 4 * q)
+                                        : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         if (tid < KBM_SUB * 3) {
             const int p = tid / 3, d = tid % 3;
@@ -1707,7 +1724,7 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
             const int p = 4 * st + kk;
             float bv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             if (vrow) {
-                const float* yz = syz + p * 2 * tw;
+                const float* yz = syz + p * 2 * tw4;
                 const float yc = yz[yo], ys = ysg * yz[yo + 1];
                 const float zc = yz[zo], zs = yz[zo + 1];
                 const float wc = yc * zc - ys * zs, ws = ys * zc + yc * zs;
@@ -1757,15 +1774,21 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
 // a quarter of the chunks each and the quarters are added in a fixed order.
 __global__ __launch_bounds__(256) void k_kpack(int32_t npk_all, const Rep* __restrict__ reps, int32_t nrep,
                                                const KPack* __restrict__ desc, const double* __restrict__ partial_all,
-                                               double4* __restrict__ pack) {
+                                               double4* __restrict__ pack, float4* __restrict__ pack32) {
     __shared__ double4 part[4][WAVE];
     const int wv = wave_id(), lane = lane_id();
     const int j = blockIdx.x * WAVE + lane;
     double cp = 0, sp = 0, cm = 0, sm = 0;
-    KPack d{-1, -1, 0.0};
+    KPack d{-1, -1, 0.0, -1, 0};
+    int64_t d32 = -1;  // slot in the FP32 tile-major copy
+    double im = 0.0;
     if (j < npk_all) {
         const Rep& R = reps[find_rep<2>(reps, nrep, j)];
         d = desc[j];
+        if (R.kmode & 2) {
+            d32 = R.pk32_off + d.t32;
+            im = R.inv_mref;
+        }
         const int nk = R.nk;
         const double* partial = partial_all + R.part_off;
         const int c0 = (R.nchunks * wv) / 4, c1 = (R.nchunks * (wv + 1)) / 4;
@@ -1790,7 +1813,9 @@ __global__ __launch_bounds__(256) void k_kpack(int32_t npk_all, const Rep* __res
     cm = ((a.z + b.z) + c.z) + e.z;
     sm = ((a.w + b.w) + c.w) + e.w;
     const double up = d.e * cp, vp = d.e * sp, um = d.e * cm, vm = d.e * sm;
-    pack[j] = make_double4(up + um, up - um, vp + vm, vp - vm);
+    const double4 v = make_double4(up + um, up - um, vp + vm, vp - vm);
+    pack[j] = v;
+    if (d32 >= 0) pack32[d32] = make_float4((float)(v.x * im), (float)(v.y * im), (float)(v.z * im), (float)(v.w * im));
 }
 
 // Rows [r0, r1) of the k-space scatter (k_kscatter) for particle i of
@@ -1953,13 +1978,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 // of a times (c, s); a lane then holds them for four particles and one row
 // and adds the row's terms in FP64. A block takes 64 particles of one
 // replica (16 per wavefront) and walks the rows 16 at a time; the particles'
-// phase tables and each row tile's entries are staged in LDS as FP32, in
-// units of the replica's largest moment.
+// FP32 phase tables and each row tile's FP32 entries (k_kpack's tile-major
+// copy, in units of the replica's largest moment) are copied to LDS.
 template <bool FORCE>
 __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __restrict__ reps, int32_t nrep,
                                                        int32_t items, const KRow* __restrict__ rows_all,
-                                                       const double4* __restrict__ pack_all,
-                                                       const double* __restrict__ tab_all,
+                                                       const float4* __restrict__ pack32_all,
+                                                       const float* __restrict__ tab32_all,
                                                        const double* __restrict__ mom,
                                                        const double* __restrict__ msum_all,
                                                        const double* __restrict__ add_force,
@@ -1973,22 +1998,24 @@ __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __r
     const int64_t pb = (int64_t)(item - R.ksm_off) * WAVE;  // replica-local first particle
     const int wv = wave_id(), lane = lane_id(), tid = threadIdx.x;
     const int kk = lane >> 4, col = lane & 15;
-    const int nc = R.nc, tw = 2 * (nc + 1), xw = tw + 2, nent = nc + 2;
-    const int64_t stride = 3 * (int64_t)tw;
-    const double* tab = tab_all + R.tab_off;
+    const int nc = R.nc, tw4 = tw4_of(nc), xw = (2 * (nc + 1) + 2 + 3) & ~3, nent = nc + 2;
+    const int64_t stride = 3 * (int64_t)tw4;
+    const float* tab = tab32_all + R.tab32_off;
+    const float4* pk32 = pack32_all + R.pk32_off;
     const double im = R.inv_mref;
     float4* sp = smem4;                          // [nent][16] row tile entries
-    float* sx = (float*)(smem4 + nent * 16);     // [64][xw] x phases (c, s) per a, zero padded
-    float* syz = sx + WAVE * xw;                 // [64][2 tw] y, then z tables
-    for (int e = tid; e < WAVE * xw; e += 256) {
-        const int p = e / xw, o = e % xw;
+    float* sx = (float*)(smem4 + nent * 16);     // [64][xw] x phases (c, s) per a, zero padded to a + 1 = n_c + 2
+    float* syz = sx + WAVE * xw;                 // [64][2 tw4] y, then z tables
+    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int e = tid; e < WAVE * (xw / 4); e += 256) {
+        const int p = e / (xw / 4), q = e % (xw / 4);
         const int64_t jr = pb + p;
-        sx[e] = (jr < R.n && o < tw) ? (float)tab[jr * stride + o] : 0.0f;
+        ((float4*)sx)[e] = (jr < R.n && 4 * q < tw4) ? *(const float4*)(tab + jr * stride + 4 * q) : zero4;
     }
-    for (int e = tid; e < WAVE * 2 * tw; e += 256) {
-        const int p = e / (2 * tw), o = e % (2 * tw);
+    for (int e = tid; e < WAVE * (tw4 / 2); e += 256) {
+        const int p = e / (tw4 / 2), q = e % (tw4 / 2);
         const int64_t jr = pb + p;
-        syz[e] = jr < R.n ? (float)tab[jr * stride + tw + o] : 0.0f;
+        ((float4*)syz)[e] = jr < R.n ? *(const float4*)(tab + jr * stride + tw4 + 4 * q) : zero4;
     }
     // this lane's particles (rows of the products): 16 wv + 4 kk + r
     float mxs[4], mys[4], mzs[4];
@@ -2007,7 +2034,6 @@ __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __r
 #pragma unroll
         for (int d = 0; d < NA; ++d) acc[r][d] = 0.0;
     const KRow* rows = rows_all + R.row_off;
-    const double4* pack = pack_all + R.pk_off;
     const float* xrow = sx + (16 * wv + col) * xw + kk;
     for (int r0 = 0; r0 < R.nrows; r0 += 16) {
         const int nt = R.nrows - r0 < 16 ? R.nrows - r0 : 16;
@@ -2015,18 +2041,9 @@ __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __r
         for (int q = 0; q < nt; ++q) amax = max(amax, rows[r0 + q].nxmax);
         const int kn = (amax + 2) / 2;  // a in [0, 2 kn), within [0, n_c + 1]
         __syncthreads();                // the previous tile's entries are read
-        for (int e = tid; e < 2 * kn * 16; e += 256) {
-            const int a = e >> 4, q = e & 15;
-            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (q < nt) {
-                const KRow rw = rows[r0 + q];
-                if (a <= rw.nxmax) {
-                    const double4 d = pack[rw.poff + a];
-                    v = make_float4((float)(d.x * im), (float)(d.y * im), (float)(d.z * im), (float)(d.w * im));
-                }
-            }
-            sp[e] = v;
-        }
+        // the tile's entries a < 2 kn (zero where a row has none)
+        const float4* tile = pk32 + (int64_t)(r0 / 16) * nent * 16;
+        for (int e = tid; e < 2 * kn * 16; e += 256) sp[e] = tile[e];
         __syncthreads();
         f32x4 c[NA == 6 ? 6 : 4];
 #pragma unroll
@@ -2053,11 +2070,11 @@ __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __r
         }
         if (col < nt) {
             const KRow rw = rows[r0 + col];
-            const int yo = 2 * (rw.ny < 0 ? -rw.ny : rw.ny), zo = tw + 2 * rw.nz;
+            const int yo = 2 * (rw.ny < 0 ? -rw.ny : rw.ny), zo = tw4 + 2 * rw.nz;
             const float ysg = rw.ny < 0 ? -1.0f : 1.0f, fny = (float)rw.ny, fnz = (float)rw.nz;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float* yz = syz + (16 * wv + 4 * kk + r) * 2 * tw;
+                const float* yz = syz + (16 * wv + 4 * kk + r) * 2 * tw4;
                 const float yc = yz[yo], ys = ysg * yz[yo + 1], zc = yz[zo], zs = yz[zo + 1];
                 const float wc = yc * zc - ys * zs, ws = ys * zc + yc * zs;
                 const float E1 = c[0][r], E2 = c[1][r], O1 = c[2][r], O2 = c[3][r];
@@ -2648,8 +2665,8 @@ hipError_t launch_cell_order(hipStream_t s, int32_t ncells, const int32_t* start
 }
 
 hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
-                                double* tab) {
-    hipLaunchKernelGGL(k_sincos_tab, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, pos0, tab);
+                                double* tab, float* tab32) {
+    hipLaunchKernelGGL(k_sincos_tab, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, pos0, tab, tab32);
     LAUNCH_CHECK();
 }
 
@@ -2662,38 +2679,39 @@ hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep
 }
 
 hipError_t launch_kbuild_mfma(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
-                              int32_t nc_max, const KRow* rows, const double* tab, const double* mom,
+                              int32_t nc_max, const KRow* rows, const float* tab32, const double* mom,
                               double* partial) {
     if (items <= 0) return hipSuccess;
     if (nc_max > KBM_NCMAX) return hipErrorInvalidValue;
-    const size_t lds = (size_t)KBM_SUB * (2 * KBM_NX + 4 + 4 * (nc_max + 1)) * sizeof(float);
-    hipLaunchKernelGGL(k_kbuild_mfma, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, tab, mom, partial);
+    const size_t lds = (size_t)KBM_SUB * (2 * KBM_NX + 4 + 2 * tw4_of(nc_max)) * sizeof(float);
+    hipLaunchKernelGGL(k_kbuild_mfma, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, tab32, mom,
+                       partial);
     LAUNCH_CHECK();
 }
 
 hipError_t launch_kscatter_mfma(hipStream_t s, bool force, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
-                                int32_t nc_max, const KRow* rows, const double4* pack, const double* tab,
+                                int32_t nc_max, const KRow* rows, const float4* pack32, const float* tab32,
                                 const double* mom, const double* msum, const double* add_force, double* force_out,
                                 double* field) {
     if (items <= 0) return hipSuccess;
     if (nc_max > KSM_NCMAX) return hipErrorInvalidValue;
-    const int tw = 2 * (nc_max + 1);
-    const size_t lds = (size_t)(nc_max + 2) * 16 * sizeof(float4) + (size_t)WAVE * (tw + 2) * sizeof(float) +
-                       (size_t)WAVE * 2 * tw * sizeof(float);
+    const int xw = (2 * (nc_max + 1) + 2 + 3) & ~3;
+    const size_t lds = (size_t)(nc_max + 2) * 16 * sizeof(float4) + (size_t)WAVE * xw * sizeof(float) +
+                       (size_t)WAVE * 2 * tw4_of(nc_max) * sizeof(float);
     if (force)
-        hipLaunchKernelGGL(k_kscatter_mfma<true>, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, pack, tab,
-                           mom, msum, add_force, force_out, field);
+        hipLaunchKernelGGL(k_kscatter_mfma<true>, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, pack32,
+                           tab32, mom, msum, add_force, force_out, field);
     else
-        hipLaunchKernelGGL(k_kscatter_mfma<false>, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, pack,
-                           tab, mom, msum, add_force, force_out, field);
+        hipLaunchKernelGGL(k_kscatter_mfma<false>, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, pack32,
+                           tab32, mom, msum, add_force, force_out, field);
     LAUNCH_CHECK();
 }
 
 hipError_t launch_kpack(hipStream_t s, int32_t npk_all, const Rep* reps, int32_t nrep, const KPack* desc,
-                        const double* partial, double4* pack) {
+                        const double* partial, double4* pack, float4* pack32) {
     if (npk_all <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_kpack, dim3((npk_all + WAVE - 1) / WAVE), dim3(256), 0, s, npk_all, reps, nrep, desc,
-                       partial, pack);
+                       partial, pack, pack32);
     LAUNCH_CHECK();
 }
 
