@@ -222,6 +222,7 @@ struct mds_ctx {
     // per-step scratch
     DevBuf<double> pos0, pos_cur, force, field, mom;
     DevBuf<double4> rec_pos, rec_mom;
+    DevBuf<double4> frec;    // positions and radii in fine-cell order (contact search)
     DevBuf<uint64_t> pair_acc;  // [12n] cell-sorted fixed-point pair sums
     DevBuf<int32_t> order_tmp;  // [n] cell-order scratch
     // EvalTransCoeffs observables (mds_observe): unwrapped and reference
@@ -745,6 +746,7 @@ mds_status size_scratch(mds_ctx* c) {
     HIPCHK(c->kea.ensure((size_t)stages * n3));
     HIPCHK(c->rec_pos.ensure(n));
     HIPCHK(c->rec_pos32.ensure(n));
+    HIPCHK(c->frec.ensure(n));
     HIPCHK(c->rec_mom.ensure(n));
     HIPCHK(c->rec_mom32.ensure(n));
     HIPCHK(c->pair_acc.ensure(12 * (size_t)n));
@@ -833,8 +835,8 @@ mds_status prepare_geometry_once(mds_ctx* c) {
                                 c->cur_fine.p, c->sorted_fine.p, c->scan_tmp.p));
         HIPCHK(launch_clear_events(s, n, c->ev_head.p, c->ev_tail.p, c->n_events.p, c->n_moved.p, c->max_disp.p));
         HIPCHK(launch_contacts(s, n, c->reps.p, c->rep_of.p, c->pos0.p, c->r_mag.p, c->start_fine.p, c->end_fine.p,
-                               c->sorted_fine.p, c->cand_count.p, c->cand_off.p, c->cand_j.p, c->cand_cap,
-                               c->err_flag.p, c->scan_tmp.p));
+                               c->sorted_fine.p, c->frec.p, c->cand_count.p, c->cand_off.p, c->cand_j.p,
+                               c->cand_cap, c->err_flag.p, c->scan_tmp.p));
         // displacement bound: half the smallest contact-search margin
         double dl = 1e300;
         for (const Rep& R : c->h_rep) dl = std::min(dl, 0.5 * R.margin);
@@ -2430,7 +2432,7 @@ int64_t mds_device_bytes(const mds_ctx* c) {
                     &c->cur_fine, &c->sorted_fine, &c->cand_count, &c->cand_off, &c->cand_j, &c->ev_head, &c->ev_tail,
                     &c->ev_next, &c->moved_list})
         add(d->n, sizeof(int32_t));
-    add(c->rec_pos.n + c->rec_mom.n, sizeof(double4));
+    add(c->rec_pos.n + c->rec_mom.n + c->frec.n, sizeof(double4));
     add(c->rec_pos32.n + c->rec_mom32.n, sizeof(float4));
     add(c->ev_key.n + c->rep_step.n + c->snap_step.n, sizeof(uint64_t));
     add(c->stencil.n, sizeof(int4));

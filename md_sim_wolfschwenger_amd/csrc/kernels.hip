@@ -282,71 +282,102 @@ __global__ __launch_bounds__(1024) void k_work(int32_t ncells, const int32_t* __
 // distance r_i + r_j + 1e-10 plus a search margin. Pass 0 counts, pass 1
 // writes each particle's partners in ascending j (so the candidate list is in
 // the reference's lexicographic (i, j) order).
+// Half a wavefront per particle: lane t < 27 scans neighbour cell t (dz, dy,
+// dx in increasing order), reading the cell's positions and radii from the
+// fine-cell ordered records (contiguous per cell). Particles are taken in
+// fine-cell order, so neighbouring half-waves share their cells (the lists
+// stay indexed by the original index). In pass 1
+// the lanes' hits go to LDS in cell-scan order (the first MAXC kept, as a
+// serial scan would keep them) and each is stored at its rank by j.
+constexpr int CONTACT_MAXC = 32;
+__global__ void k_frec(int64_t n, const int32_t* __restrict__ fsorted, const double* __restrict__ pos0,
+                       const double* __restrict__ r_mag, double4* __restrict__ frec) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = fsorted[s];
+    if ((uint64_t)i >= (uint64_t)n) return;  // every slot holds a particle; guard the reads anyway
+    frec[s] = make_double4(pos0[i], pos0[n + i], pos0[2 * n + i], r_mag[i]);
+}
 template <int PASS>
-__global__ void k_contacts(int64_t n, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
-                           const double* __restrict__ pos0, const double* __restrict__ r_mag,
-                           const int32_t* __restrict__ fstart, const int32_t* __restrict__ fend,
-                           const int32_t* __restrict__ fsorted, int32_t* __restrict__ cand_count,
-                           const int32_t* __restrict__ cand_off, int32_t* __restrict__ cand_i,
-                           int32_t* __restrict__ cand_j, int32_t cap, int32_t* __restrict__ err) {
-    // threads walk the particles in fine-cell order, so a wavefront's
-    // particles share their neighbour cells (the candidate lists stay indexed
-    // by the original index)
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int64_t i = fsorted[k];
-    const Rep& R = reps[rep_of[i]];
-    const double L = R.L, fcell = R.fcell, extra = R.margin;
-    const int32_t fncell = R.fncell, foff = R.fcell_off;
-    const double xi = pos0[i], yi = pos0[n + i], zi = pos0[2 * n + i], ri = r_mag[i];
-    const int cx = cell_coord(xi, L, fcell, fncell), cy = cell_coord(yi, L, fcell, fncell),
-              cz = cell_coord(zi, L, fcell, fncell);
-    const int reach = fncell >= 3 ? 1 : 0;
-    constexpr int MAXC = 32;
-    int32_t list[PASS ? MAXC : 1];
-    int cnt = 0;
-    for (int dz = -reach; dz <= reach; ++dz)
-        for (int dy = -reach; dy <= reach; ++dy)
-            for (int dx = -reach; dx <= reach; ++dx) {
-                const int c = foff + (((cz + dz + fncell) % fncell) * fncell + (cy + dy + fncell) % fncell) * fncell +
-                              (cx + dx + fncell) % fncell;
-                for (int32_t s = fstart[c]; s < fend[c]; ++s) {
-                    const int32_t j = fsorted[s];
-                    if (j <= i) continue;
-                    const double vx = min_image(pos0[j] - xi, L), vy = min_image(pos0[n + j] - yi, L),
-                                 vz = min_image(pos0[2 * n + j] - zi, L);
-                    const double lim = ri + r_mag[j] + 1e-10 + extra;
-                    if (vx * vx + vy * vy + vz * vz < lim * lim) {
-                        if (PASS) {
-                            if (cnt < MAXC) list[cnt] = j;
-                        }
-                        ++cnt;
-                    }
-                }
-            }
+__global__ __launch_bounds__(256) void k_contacts(int64_t n, const Rep* __restrict__ reps,
+                                                  const int32_t* __restrict__ rep_of, const double* __restrict__ pos0,
+                                                  const double* __restrict__ r_mag, const int32_t* __restrict__ fstart,
+                                                  const int32_t* __restrict__ fend, const int32_t* __restrict__ fsorted,
+                                                  const double4* __restrict__ frec,
+                                                  int32_t* __restrict__ cand_count, const int32_t* __restrict__ cand_off,
+                                                  int32_t* __restrict__ cand_i, int32_t* __restrict__ cand_j,
+                                                  int32_t cap, int32_t* __restrict__ err) {
+    __shared__ int32_t lst[8][PASS ? CONTACT_MAXC : 1];
+    const int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+    const int64_t k = blockIdx.x * 8LL + g;
+    const bool on = k < n;  // uniform over the half-wave
+    int64_t i = 0;
+    int32_t c = 0;
+    double xi = 0, yi = 0, zi = 0, ri = 0, L = 1, extra = 0;
+    bool act = false;
+    if (on) {
+        i = fsorted[k];
+        const Rep& R = reps[rep_of[i]];
+        L = R.L;
+        extra = R.margin;
+        const double fcell = R.fcell;
+        const int32_t fncell = R.fncell, foff = R.fcell_off;
+        xi = pos0[i];
+        yi = pos0[n + i];
+        zi = pos0[2 * n + i];
+        ri = r_mag[i];
+        const int cx = cell_coord(xi, L, fcell, fncell), cy = cell_coord(yi, L, fcell, fncell),
+                  cz = cell_coord(zi, L, fcell, fncell);
+        const int reach = fncell >= 3 ? 1 : 0;
+        act = reach ? t < 27 : t == 0;
+        const int dz = reach ? t / 9 - 1 : 0, dy = reach ? (t / 3) % 3 - 1 : 0, dx = reach ? t % 3 - 1 : 0;
+        c = foff + (((cz + dz + fncell) % fncell) * fncell + (cy + dy + fncell) % fncell) * fncell +
+            (cx + dx + fncell) % fncell;
+    }
+    int32_t s0 = 0, s1 = 0;
+    if (act) {
+        s0 = fstart[c];
+        s1 = fend[c];
+    }
+    auto hit = [&](int32_t s) {
+        const double4 q = frec[s];
+        const double vx = min_image(q.x - xi, L), vy = min_image(q.y - yi, L), vz = min_image(q.z - zi, L);
+        const double lim = ri + q.w + 1e-10 + extra;
+        return vx * vx + vy * vy + vz * vz < lim * lim;
+    };
+    int cl = 0;
+    for (int32_t s = s0; s < s1; ++s) {
+        const int32_t j = fsorted[s];
+        if (j > i && hit(s)) ++cl;
+    }
+    int tot = cl;
+    for (int o = 16; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 32);
     if (!PASS) {
-        cand_count[i] = cnt;
+        if (on && t == 0) cand_count[i] = tot;
         return;
     }
-    if (cnt > MAXC) {
-        atomicOr(err, 4);
-        cnt = MAXC;
+    // exclusive prefix of the lanes' counts, in cell-scan order
+    int pre = cl;
+    for (int o = 1; o < 32; o <<= 1) {
+        const int v = __shfl_up(pre, o, 32);
+        if (t >= o) pre += v;
     }
-    // ascending j
-    for (int a = 1; a < cnt; ++a) {
-        const int32_t v = list[a];
-        int b = a - 1;
-        while (b >= 0 && list[b] > v) {
-            list[b + 1] = list[b];
-            --b;
-        }
-        list[b + 1] = v;
+    pre -= cl;
+    if (on && t == 0 && tot > CONTACT_MAXC) atomicOr(err, 4);
+    for (int32_t s = s0, m = pre; s < s1 && m < CONTACT_MAXC; ++s) {
+        const int32_t j = fsorted[s];
+        if (j > i && hit(s)) lst[g][m++] = j;
     }
-    const int32_t off = cand_off[i];
-    for (int a = 0; a < cnt; ++a) {
-        if (off + a < cap) {
-            cand_i[off + a] = (int32_t)i;
-            cand_j[off + a] = list[a];
+    wave_sync();
+    const int cn = tot < CONTACT_MAXC ? tot : CONTACT_MAXC;
+    if (on && t < cn) {
+        const int32_t e = lst[g][t];
+        int rank = 0;
+        for (int u = 0; u < cn; ++u) rank += lst[g][u] < e;
+        const int32_t off = cand_off[i] + rank;
+        if (off < cap) {
+            cand_i[off] = (int32_t)i;
+            cand_j[off] = e;
         } else {
             atomicOr(err, 8);
         }
@@ -2510,18 +2541,20 @@ hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start
 
 hipError_t launch_contacts(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
                            const double* r_mag, const int32_t* fstart, const int32_t* fend, const int32_t* fsorted,
-                           int32_t* cand_count, int32_t* cand_off, int32_t* cand_j, int32_t cand_cap, int32_t* err,
-                           int32_t* scan_tmp) {
+                           double4* frec, int32_t* cand_count, int32_t* cand_off, int32_t* cand_j, int32_t cand_cap,
+                           int32_t* err, int32_t* scan_tmp) {
+    hipLaunchKernelGGL(k_frec, dim3(grid_for(n, 256)), dim3(256), 0, s, n, fsorted, pos0, r_mag, frec);
     // cand_off has n+1 slots: offsets then the total at [n]; cand_j holds
     // 2*cand_cap ints: partner j then owner i.
     int32_t* cand_i = cand_j + cand_cap;
-    hipLaunchKernelGGL(k_contacts<0>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart,
-                       fend, fsorted, cand_count, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
+    const unsigned blocks = (unsigned)((n + 7) / 8);  // 8 particles per block
+    hipLaunchKernelGGL(k_contacts<0>, dim3(blocks), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart,
+                       fend, fsorted, (const double4*)frec, cand_count, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
                        cand_cap, err);
     hipError_t e = launch_scan(s, cand_count, (int32_t)n, cand_off, nullptr, nullptr, cand_off + n, scan_tmp);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_contacts<1>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart,
-                       fend, fsorted, cand_count, cand_off, cand_i, cand_j, cand_cap, err);
+    hipLaunchKernelGGL(k_contacts<1>, dim3(blocks), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart,
+                       fend, fsorted, (const double4*)frec, cand_count, cand_off, cand_i, cand_j, cand_cap, err);
     LAUNCH_CHECK();
 }
 
