@@ -249,6 +249,7 @@ struct mds_ctx {
     DevBuf<double> ev_pos, max_disp;
     // k space
     std::vector<KRow> h_rows;
+    std::vector<KRow> h_rows_m;  // the same rows in the matrix-core kernels' order
     std::vector<KSeg> h_segs;
     std::vector<KPack> h_pack;
     int32_t nk_all = 0, kb_items = 0, kbm_items = 0, ksm_items = 0, nc_max = 0;
@@ -256,7 +257,7 @@ struct mds_ctx {
     // largest n_c (sizes the kernels' LDS), and whether any is on the vector kernels
     int32_t nc_mfma_build = 0, nc_mfma_scatter = 0;
     bool any_mfma_build = false, any_mfma_scatter = false, any_vec_build = false, any_vec_scatter = false;
-    DevBuf<KRow> rows;
+    DevBuf<KRow> rows, rows_m;
     DevBuf<KSeg> segs;
     DevBuf<KPack> pack_desc;
     DevBuf<double4> pack;
@@ -483,8 +484,6 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
             const bool half = (ny == 0 && nz == 0);
             if (half && nxmax == 0) continue;
             c->h_rows.push_back({ny, nz, nxmax, (int32_t)c->h_pack.size() - R.pk_off, koff});
-            const int32_t rl = (int32_t)c->h_rows.size() - 1 - R.row_off;
-            auto t32 = [&](int a) { return ((rl / 16) * (nc + 2) + a) * 16 + rl % 16; };
             for (int a = half ? 1 : 0; a <= nxmax; a += KSEG)
                 c->h_segs.push_back({ny, nz, a, std::min(nxmax, a + KSEG - 1), koff, nxmax});
             // scatter table: |nx| = a pairs (+a, ny, nz) with (-a, ny, nz)
@@ -498,17 +497,36 @@ void plan_kspace(mds_ctx* c, const RepMeta& m, Rep& R) {
                     d.km = a >= 1 ? koff - a + nxmax : -1;
                 }
                 d.e = d.kp >= 0 ? ek((int64_t)a * a + r2) : 0.0;
-                d.t32 = t32(a);
+                d.t32 = -1;
                 d.pad_ = 0;
                 c->h_pack.push_back(d);
             }
             // pad to an even entry count (the scatter takes two at a time)
-            if ((nxmax + 1) % 2) c->h_pack.push_back(KPack{-1, -1, 0.0, t32(nxmax + 1), 0});
+            if ((nxmax + 1) % 2) c->h_pack.push_back(KPack{-1, -1, 0.0, -1, 0});
             koff += half ? nxmax : 2 * nxmax + 1;
         }
     R.nrows = (int32_t)c->h_rows.size() - R.row_off;
     R.nseg = (int32_t)c->h_segs.size() - R.seg_off;
     R.npk = (int32_t)c->h_pack.size() - R.pk_off;
+    // the matrix-core kernels take the rows by decreasing |nx| range (ties in
+    // table order), so the rows of one 16-row tile have similar lengths and
+    // the tiles' products stop near their longest row; each packed entry
+    // gets its slot in the FP32 tile-major copy, ((row / 16) (n_c + 2) + a)
+    // 16 + row % 16 with the row's place in that order
+    {
+        std::vector<int32_t> ord(R.nrows);
+        for (int32_t r = 0; r < R.nrows; ++r) ord[r] = r;
+        std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+            return c->h_rows[R.row_off + x].nxmax > c->h_rows[R.row_off + y].nxmax;
+        });
+        for (int32_t q = 0; q < R.nrows; ++q) {
+            const KRow& row = c->h_rows[R.row_off + ord[q]];
+            c->h_rows_m.push_back(row);
+            const int32_t ne = ((row.nxmax + 2) / 2) * 2;  // entries with the padding one
+            for (int32_t a = 0; a < ne; ++a)
+                c->h_pack[R.pk_off + row.poff + a].t32 = ((q / 16) * (nc + 2) + a) * 16 + q % 16;
+        }
+    }
     // quarters of the table by entries for the four wavefronts of a scatter block
     {
         int32_t q = 1, acc = 0;
@@ -592,6 +610,7 @@ mds_status plan_all(mds_ctx* c) {
     c->h_rep.assign(c->nrep, Rep{});
     c->h_sten.clear();
     c->h_rows.clear();
+    c->h_rows_m.clear();
     c->h_segs.clear();
     c->h_pack.clear();
     std::vector<int32_t> cell_rep, rep_of((size_t)c->n);
@@ -693,6 +712,7 @@ mds_status plan_all(mds_ctx* c) {
     HIPCHK(c->press_partial.ensure(3 * (size_t)work_cap * PAIR_SLICES + 6 * (size_t)c->nrep));
     HIPCHK(c->stencil.ensure(std::max<size_t>(1, c->h_sten.size())));
     HIPCHK(c->rows.ensure(std::max<size_t>(1, c->h_rows.size())));
+    HIPCHK(c->rows_m.ensure(std::max<size_t>(1, c->h_rows_m.size())));
     HIPCHK(c->segs.ensure(std::max<size_t>(1, c->h_segs.size())));
     HIPCHK(c->pack_desc.ensure(std::max<size_t>(1, c->h_pack.size())));
     HIPCHK(c->pack.ensure(std::max<size_t>(1, c->h_pack.size())));
@@ -718,8 +738,11 @@ mds_status plan_all(mds_ctx* c) {
     if (!c->h_sten.empty())
         HIPCHK(hipMemcpyAsync(c->stencil.p, c->h_sten.data(), c->h_sten.size() * sizeof(int4), hipMemcpyHostToDevice,
                               s));
-    if (!c->h_rows.empty())
+    if (!c->h_rows.empty()) {
         HIPCHK(hipMemcpyAsync(c->rows.p, c->h_rows.data(), c->h_rows.size() * sizeof(KRow), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->rows_m.p, c->h_rows_m.data(), c->h_rows_m.size() * sizeof(KRow),
+                              hipMemcpyHostToDevice, s));
+    }
     if (!c->h_segs.empty())
         HIPCHK(hipMemcpyAsync(c->segs.p, c->h_segs.data(), c->h_segs.size() * sizeof(KSeg), hipMemcpyHostToDevice, s));
     if (!c->h_pack.empty())
@@ -896,7 +919,7 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
             {
                 mds_ctx::Span sp(c, KC_KBUILD);
                 if (c->any_mfma_build)
-                    HIPCHK(launch_kbuild_mfma(ks, n, c->reps.p, c->nrep, c->kbm_items, c->nc_mfma_build, c->rows.p,
+                    HIPCHK(launch_kbuild_mfma(ks, n, c->reps.p, c->nrep, c->kbm_items, c->nc_mfma_build, c->rows_m.p,
                                               c->tab32.p, c->mom.p, c->kpartial.p));
                 if (c->any_vec_build)
                     HIPCHK(launch_kbuild(ks, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p,
@@ -911,7 +934,7 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
         mds_ctx::Span sp(c, force_pass ? KC_KSCATTER_FORCE : KC_KSCATTER_FIELD);
         if (c->any_mfma_scatter)
             HIPCHK(launch_kscatter_mfma(ks, force_pass, n, c->reps.p, c->nrep, c->ksm_items, c->nc_mfma_scatter,
-                                        c->rows.p, c->pack32.p, c->tab32.p, c->mom.p, c->msum.p,
+                                        c->rows_m.p, c->pack32.p, c->tab32.p, c->mom.p, c->msum.p,
                                         force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
         if (c->any_vec_scatter)
             HIPCHK(launch_kscatter(ks, force_pass, n, c->reps.p, c->rep_of.p, c->rows.p, c->pack.p, c->tab.p,
@@ -2436,7 +2459,7 @@ int64_t mds_device_bytes(const mds_ctx* c) {
     add(c->rec_pos32.n + c->rec_mom32.n, sizeof(float4));
     add(c->ev_key.n + c->rep_step.n + c->snap_step.n, sizeof(uint64_t));
     add(c->stencil.n, sizeof(int4));
-    add(c->rows.n, sizeof(KRow));
+    add(c->rows.n + c->rows_m.n, sizeof(KRow));
     add(c->segs.n, sizeof(KSeg));
     add(c->pack_desc.n, sizeof(KPack));
     add(c->scan_tmp.n, sizeof(int32_t));

@@ -1719,6 +1719,13 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
     const int yo = 2 * (row.ny < 0 ? -row.ny : row.ny), zo = tw4 + 2 * row.nz;
     const float ysg = row.ny < 0 ? -1.0f : 1.0f;
     const float fny = (float)row.ny, fnz = (float)row.nz;
+    // M tiles (8 values of a each) this wavefront needs: its rows come by
+    // decreasing |nx| range, so short rows share waves that skip the upper
+    // tiles
+    int amax = row.nxmax;
+    for (int o = 32; o > 0; o >>= 1) amax = max(amax, __shfl_xor(amax, o));
+    amax = __builtin_amdgcn_readfirstlane(amax);
+    const int ntile = amax < a0 ? 0 : min(3, (amax - a0) / 8 + 1);
     f32x4 acc[3][4];
 #pragma unroll
     for (int t = 0; t < 3; ++t)
@@ -1768,6 +1775,7 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
             }
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
+                if (t >= ntile) break;  // uniform
                 const float xa = sx[p][16 * t + col];
 #pragma unroll
                 for (int b = 0; b < 4; ++b) acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, bv[b], acc[t][b], 0, 0, 0);
