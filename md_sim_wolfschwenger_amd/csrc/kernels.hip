@@ -2042,20 +2042,25 @@ __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __r
     const float* tab = tab32_all + R.tab32_off;
     const float4* pk32 = pack32_all + R.pk32_off;
     const double im = R.inv_mref;
-    float4* sp = smem4;                          // [nent][16] row tile entries
-    float* sx = (float*)(smem4 + nent * 16);     // [64][xw] x phases (c, s) per a, zero padded to a + 1 = n_c + 2
+    float* sx = (float*)smem4;                   // [64][xw] x phases (c, s) per a, zero padded to a + 1 = n_c + 2
     float* syz = sx + WAVE * xw;                 // [64][2 tw4] y, then z tables
-    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // (values, not a select between a table address and a local zero,
+    // which would put the zero in scratch memory)
     for (int e = tid; e < WAVE * (xw / 4); e += 256) {
         const int p = e / (xw / 4), q = e % (xw / 4);
         const int64_t jr = pb + p;
-        ((float4*)sx)[e] = (jr < R.n && 4 * q < tw4) ? *(const float4*)(tab + jr * stride + 4 * q) : zero4;
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (jr < R.n && 4 * q < tw4) v = *(const float4*)(tab + jr * stride + 4 * q);
+        ((float4*)sx)[e] = v;
     }
     for (int e = tid; e < WAVE * (tw4 / 2); e += 256) {
         const int p = e / (tw4 / 2), q = e % (tw4 / 2);
         const int64_t jr = pb + p;
-        ((float4*)syz)[e] = jr < R.n ? *(const float4*)(tab + jr * stride + tw4 + 4 * q) : zero4;
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (jr < R.n) v = *(const float4*)(tab + jr * stride + tw4 + 4 * q);
+        ((float4*)syz)[e] = v;
     }
+    __syncthreads();  // the tables are staged
     // this lane's particles (rows of the products): 16 wv + 4 kk + r
     float mxs[4], mys[4], mzs[4];
 #pragma unroll
@@ -2076,14 +2081,12 @@ __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __r
     const float* xrow = sx + (16 * wv + col) * xw + kk;
     for (int r0 = 0; r0 < R.nrows; r0 += 16) {
         const int nt = R.nrows - r0 < 16 ? R.nrows - r0 : 16;
-        int amax = 0;
-        for (int q = 0; q < nt; ++q) amax = max(amax, rows[r0 + q].nxmax);
+        // the rows come by decreasing |nx| range: the tile's first is its longest
+        const int amax = rows[r0].nxmax;
         const int kn = (amax + 2) / 2;  // a in [0, 2 kn), within [0, n_c + 1]
-        __syncthreads();                // the previous tile's entries are read
-        // the tile's entries a < 2 kn (zero where a row has none)
+        // the tile's entries (zero where a row has none), read through the
+        // caches (the tiles of a replica take a few hundred KB)
         const float4* tile = pk32 + (int64_t)(r0 / 16) * nent * 16;
-        for (int e = tid; e < 2 * kn * 16; e += 256) sp[e] = tile[e];
-        __syncthreads();
         f32x4 c[NA == 6 ? 6 : 4];
 #pragma unroll
         for (int u = 0; u < (NA == 6 ? 6 : 4); ++u) c[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -2091,7 +2094,7 @@ __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __r
         for (int ks = 0; ks < kn; ++ks) {
             const int a = 2 * ks + (kk >> 1);
             const float xa = xrow[4 * ks];
-            const float4 p = sp[a * 16 + col];
+            const float4 p = tile[a * 16 + col];
             const float fa = (float)a;
             const float e1 = xs_ ? p.w : p.x;          // SU | DV
             const float e2 = xs_ ? -p.y : p.z;         // SV | -DU
@@ -2737,8 +2740,7 @@ hipError_t launch_kscatter_mfma(hipStream_t s, bool force, int64_t n, const Rep*
     if (items <= 0) return hipSuccess;
     if (nc_max > KSM_NCMAX) return hipErrorInvalidValue;
     const int xw = (2 * (nc_max + 1) + 2 + 3) & ~3;
-    const size_t lds = (size_t)(nc_max + 2) * 16 * sizeof(float4) + (size_t)WAVE * xw * sizeof(float) +
-                       (size_t)WAVE * 2 * tw4_of(nc_max) * sizeof(float);
+    const size_t lds = (size_t)WAVE * xw * sizeof(float) + (size_t)WAVE * 2 * tw4_of(nc_max) * sizeof(float);
     if (force)
         hipLaunchKernelGGL(k_kscatter_mfma<true>, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, pack32,
                            tab32, mom, msum, add_force, force_out, field);
