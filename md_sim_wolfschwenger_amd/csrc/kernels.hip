@@ -1267,37 +1267,49 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
             }
             const int cnt = min(WAVE, ce - jb);
             bool any = false;
-            // the candidates near the box, in increasing order (uniform mask)
-            for (uint64_t cm = ballot(near); cm; cm &= cm - 1) {
-                const int t = __builtin_ctzll(cm);
-                float dx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qx), t)) - pxf;
-                float dy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qy), t)) - pyf;
-                float dz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qz), t)) - pzf;
-                if (allpairs) {
-                    dx = dx > Lf / 2 ? dx - Lf : (dx < -Lf / 2 ? dx + Lf : dx);
-                    dy = dy > Lf / 2 ? dy - Lf : (dy < -Lf / 2 ? dy + Lf : dy);
-                    dz = dz > Lf / 2 ? dz - Lf : (dz < -Lf / 2 ? dz + Lf : dz);
-                }
-                // lanes that take the pair: valid i, inside, and in the cell
-                // itself only i < j (lane < jb + t - s0); the masks are
-                // combined as scalars
-                const int32_t lo = jb + t - s0;
-                const uint64_t self_mask = !self || lo >= WAVE ? ~0ull : (lo <= 0 ? 0ull : (1ull << lo) - 1);
-                const bool inside = dx * dx + dy * dy + dz * dz < rr2f;
-                const uint64_t m = ballot(inside) & vmask & self_mask;
-                const bool hit = inside && vi && (!self || lane < lo);  // the lanes of m
-                if (m) {
-                    // the hits of one candidate fill whole quads
-                    any = true;
-                    if (hit) W.q[qn + mbcnt(m)] = ((jb + t) << 6) | lane;
-                    qn += (__popcll(m) + 3) & ~3;
-                    if (qn >= IW::QFLUSH) {
-                        pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, im, press, press32,
-                                                          npairs, s0);
-                        qn = 0;
+            // the candidates near the box, in increasing order (uniform
+            // mask); the cell itself (one stencil cell of a wavefront's
+            // dozens) takes only i < j and has its own copy of the loop
+            auto scan = [&](auto self_c) {
+                constexpr bool SELF = decltype(self_c)::value;
+                for (uint64_t cm = ballot(near); cm; cm &= cm - 1) {
+                    const int t = __builtin_ctzll(cm);
+                    float dx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qx), t)) - pxf;
+                    float dy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qy), t)) - pyf;
+                    float dz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qz), t)) - pzf;
+                    if (allpairs) {
+                        dx = dx > Lf / 2 ? dx - Lf : (dx < -Lf / 2 ? dx + Lf : dx);
+                        dy = dy > Lf / 2 ? dy - Lf : (dy < -Lf / 2 ? dy + Lf : dy);
+                        dz = dz > Lf / 2 ? dz - Lf : (dz < -Lf / 2 ? dz + Lf : dz);
+                    }
+                    // lanes that take the pair: valid i, inside, and in the
+                    // cell itself only i < j (lane < jb + t - s0); the masks
+                    // are combined as scalars
+                    const bool inside = dx * dx + dy * dy + dz * dz < rr2f;
+                    uint64_t m = ballot(inside) & vmask;
+                    bool hit = inside && vi;  // the lanes of m
+                    if constexpr (SELF) {
+                        const int32_t lo = jb + t - s0;
+                        m &= lo >= WAVE ? ~0ull : (lo <= 0 ? 0ull : (1ull << lo) - 1);
+                        hit = hit && lane < lo;
+                    }
+                    if (m) {
+                        // the hits of one candidate fill whole quads
+                        any = true;
+                        if (hit) W.q[qn + mbcnt(m)] = ((jb + t) << 6) | lane;
+                        qn += (__popcll(m) + 3) & ~3;
+                        if (qn >= IW::QFLUSH) {
+                            pair_flush<FORCE, COATING, MIXED>(g, rec_mom, pp, C, W, qn, moved_any, jb, im, press,
+                                                              press32, npairs, s0);
+                            qn = 0;
+                        }
                     }
                 }
-            }
+            };
+            if (self)
+                scan(std::true_type{});
+            else
+                scan(std::false_type{});
             if (any) {
                 // window done: its j sums go to the global sums
                 if (qn > 0)
@@ -1335,11 +1347,11 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     if (!ok) atomicOr(out.err, 64);
 }
 
-// Five blocks per CU (the LDS allows five). The FP64 field pass is held to
-// 5 waves per SIMD (89 VGPRs without spills, from 111): 5 % faster alone and
-// more room for the k-space chain beside it; the FP64 force pass would spill.
+// Register budgets: the FP64 field pass is held to 5 waves per SIMD and the
+// FP64 force pass to 4 (128 VGPRs with a 16-byte spill, from 134 and 3
+// waves: 3.27 -> 2.94 ms at C3); the mixed kernels fit 5 waves unforced.
 template <bool FORCE, bool MIXED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FORCE || MIXED ? 1 : 5))) void k_pairs(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 : (FORCE ? 4 : 5)))) void k_pairs(Geometry g, const double4* __restrict__ rec_mom, PairParams pp,
                                                PairOut out, const int4* __restrict__ stencil_all,
                                                unsigned long long* __restrict__ cnt_pairs) {
     pairs_block<FORCE, false, MIXED>(g, rec_mom, pp, out, stencil_all, cnt_pairs);
