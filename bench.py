@@ -90,7 +90,7 @@ def tune(ctx, info, args):
     else:
         rcs = (args.rc_frac,) if args.rc_frac > 0 else (0.15, 0.175, 0.2, 0.225, 0.25, 0.275, 0.3, 0.35)
         reaches = (args.reach,) if args.reach > 0 else (2, 3, 4)
-        res = ctx.autotune(rcs, reaches, reps=2)
+        res = ctx.autotune(rcs, reaches, reps=3)
         log(f"  autotune: {res} -> {ctx.params()} {ctx.tiling()}")
     ctx.set_state(info["pos"], info["mm"], info["ea"])
     ctx.use_philox(args.seed, 0)

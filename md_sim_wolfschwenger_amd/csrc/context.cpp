@@ -1497,9 +1497,11 @@ mds_status mds_autotune(mds_ctx* c, const double* rc_fracs, int32_t n_rc, const 
                         int32_t reps, double* best_rc_frac, int32_t* best_reach, double* best_ms) {
     // CompEwaldParams (src/interaction.cpp:164-324) picks r_c in [0.2, 0.5] L
     // by timing the field sums on the CPU; here every candidate (r_c, cell
-    // reach) is timed on the device over one force and one field evaluation,
-    // with alpha and k_c of every replica from the same error model, so each
-    // candidate has the accuracy the configuration asks for.
+    // reach) is timed on the device over the step's geometry (cell lists,
+    // contact search, tables: their cost depends on the tiling) and one force
+    // and one field evaluation, with alpha and k_c of every replica from the
+    // same error model, so each candidate has the accuracy the configuration
+    // asks for.
     if (!c || c->n <= 0 || !c->state_set || !rc_fracs || n_rc <= 0 || !reaches || n_reach <= 0)
         return fail(c, MDS_EINVAL, "autotune needs particles, state and candidates");
     HIPCHK(hipSetDevice(c->device));
@@ -1541,6 +1543,7 @@ mds_status mds_autotune(mds_ctx* c, const double* rc_fracs, int32_t n_rc, const 
             float best_ms_here = 1e30f;
             for (int r = 0; r < reps + 1 && st == MDS_OK; ++r) {
                 HIPCHK(hipEventRecord(e0, c->stream));
+                if ((st = prepare_geometry(c, false)) != MDS_OK) break;
                 if ((st = moments_of(c, c->mm.p)) != MDS_OK) break;
                 if ((st = evaluate(c, true)) != MDS_OK) break;
                 if ((st = evaluate(c, false)) != MDS_OK) break;
