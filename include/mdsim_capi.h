@@ -214,6 +214,26 @@ mds_status mds_set_ewald(mds_ctx* ctx, double alpha, double r_cut, int32_t nc);
  * average, so sparse replicas get larger cells. */
 mds_status mds_set_tiling(mds_ctx* ctx, int32_t cell_reach);
 mds_status mds_get_tiling(const mds_ctx* ctx, int32_t* cell_reach, int32_t* ncell, int32_t* nstencil);
+/* Pair lists (no counterpart in the reference, whose CompInteractVecs
+ * rebuilds the full pair list every call, src/interaction.cpp:94-134): each
+ * replica keeps the pairs within r_l = r_c (1 + skin) of its build
+ * positions, and both evaluations of a step read that list instead of
+ * searching the cells. A replica rebuilds its list when one of its particles
+ * has moved more than (r_l - r_c) / 2 from its build position (checked on the
+ * device each step), so every pair within r_c is always in it; the pairs
+ * evaluated, and their results, are those of the search. skin = 0 turns the
+ * lists off (every evaluation searches the cells). Default 0.005. The result
+ * of a replica depends only on its own trajectory (batches, resumes and
+ * layout changes rebuild a list from the stored build positions), but it
+ * differs from the search's in the order of the pair additions. */
+mds_status mds_set_pair_list(mds_ctx* ctx, double skin);
+/* Builds of replica `index`'s list so far (the first included), words of
+ * its last build and words of its region. */
+mds_status mds_pair_list_stats(mds_ctx* ctx, int32_t index, int64_t* builds, int64_t* words, int64_t* capacity);
+/* Words (int32) of the list region the planner gives a replica of n
+ * particles in a box of edge l_box with cut-off r_cut and list skin `skin`
+ * (0 when the arguments describe no list): for admission by footprint. */
+int64_t mds_pair_list_words(int64_t n, double l_box, double r_cut, double skin);
 /* Autotuner, in place of CompEwaldParams's wall-clock choice among 30 r_c in
  * [0.2, 0.5] L (src/interaction.cpp:164-324): for each r_c = rc_fracs[i] * L,
  * alpha and k_c come from the reference's error model (so accuracy is the

@@ -92,6 +92,10 @@ SIGNATURES = [
     ("mds_set_ewald", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32]),
     ("mds_set_tiling", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_get_tiling", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("mds_set_pair_list", C.c_int, [C.c_void_p, C.c_double]),
+    ("mds_pair_list_words", C.c_int64, [C.c_int64, C.c_double, C.c_double, C.c_double]),
+    ("mds_pair_list_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64)]),
     ("mds_autotune", C.c_int, [C.c_void_p, _dp, C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32, _dp,
                                C.POINTER(C.c_int32), _dp]),
     ("mds_get_params", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)]),
@@ -332,6 +336,15 @@ class Context:
         v = [C.c_int32() for _ in range(3)]
         self._chk(self._L.mds_get_tiling(self.h, *[C.byref(x) for x in v]), "get_tiling")
         return {"cell_reach": v[0].value, "ncell": v[1].value, "nstencil": v[2].value}
+
+    def set_pair_list(self, skin):
+        """Pair lists with skin r_c * skin (0: search the cells every evaluation)."""
+        self._chk(self._L.mds_set_pair_list(self.h, float(skin)), "set_pair_list")
+
+    def pair_list_stats(self, index=0):
+        v = [C.c_int64() for _ in range(3)]
+        self._chk(self._L.mds_pair_list_stats(self.h, int(index), *[C.byref(x) for x in v]), "pair_list_stats")
+        return {"builds": v[0].value, "words": v[1].value, "capacity": v[2].value}
 
     def autotune(self, rc_fracs=(0.175, 0.2, 0.225, 0.25, 0.275, 0.3), reaches=(2, 3, 4), reps=2):
         """Time each (r_c, cell reach) candidate on the device; install the fastest."""

@@ -104,11 +104,12 @@ enum KClass {
     KC_STAGE,
     KC_TRANSLATE,
     KC_NOISE,
+    KC_LIST,
     KC_COUNT
 };
 const char* kKClassNames[KC_COUNT] = {"setup",   "contacts", "sincos",          "moments",          "pairs_force",
                                       "pairs_field", "kbuild", "kreduce",        "kscatter_force",   "kscatter_field",
-                                      "stage",   "translate", "noise"};
+                                      "stage",   "translate", "noise",           "list_build"};
 
 // Butcher tableaux of IntegrationRot (src/simulation.cpp:721-862): stage
 // coefficients a, accepted combination b (BS: its 4th stage point), error
@@ -183,6 +184,7 @@ struct RepMeta {
     bool ewald_set = false;
     uint64_t seed = 0;
     double margin_scale = 0.1;
+    double list_scale = 1.0;  // pair-list region over its estimate (grows when a list does not fit)
     uint64_t stop_at = ~(uint64_t)0;  // step count at which the replica stops (mds_replica_stop_at)
 };
 
@@ -217,7 +219,20 @@ struct mds_ctx {
     // per-particle constants and state (kept across replica changes)
     DevBuf<double> r_mag, mag_mom, zeta_rot, zeta_trans, vol_mag, c1, c2;
     DevBuf<double> pos, mm, ea, n_t, n_b, n_f;
-    DevBuf<double> spare[10];  // re-layout targets of the ten arrays above that are kept
+    DevBuf<double> spare[11];  // re-layout targets of the ten arrays above and pos_build
+    // pair list (list_skin > 0, the skin as a fraction of r_c): build
+    // positions (original order, kept across layouts), per-replica state,
+    // this step's mode, largest squared displacement since the build, stream
+    // lengths and offsets per (sorted slot, wavefront), the streams, words
+    // per replica and builds per replica
+    double list_skin = 0.005;
+    DevBuf<double> pos_build;
+    DevBuf<int32_t> list_state, list_mode, lcount, loff, list;
+    DevBuf<uint64_t> list_maxd2, list_builds;
+    DevBuf<int64_t> list_used;
+    DevBuf<float4> rec_build32;
+    int64_t list_words = 0;
+    bool list_on() const { return list_skin > 0; }
     int64_t reserve_n = 0;     // capacity floor in particles (mds_reserve)
     // per-step scratch
     DevBuf<double> pos0, pos_cur, force, field, mom;
@@ -293,6 +308,7 @@ struct mds_ctx {
     DevBuf<double> snap;
     DevBuf<double> snap_t;
     DevBuf<uint64_t> snap_step;
+    DevBuf<int32_t> snap_list;
     int64_t chunks_redone = 0;
 
     hipEvent_t get_event() {
@@ -401,9 +417,10 @@ mds_status download_cols(mds_ctx* c, double* dst, const double* src, int64_t n, 
 // Cell grids of replica r: the main grid's stencil of +-reach cells must
 // cover the candidate radius r_c + margin (the margin covers positions moved
 // by the overlap correction); the fine grid covers the contact search radius.
-// Main-grid cells per dimension for a stencil half-width `reach`.
-int32_t main_cells(const RepMeta& m, double margin, int reach) {
-    const double cmin = (m.rc * (1 + 1e-9) + margin) / reach;
+// Main-grid cells per dimension for a stencil half-width `reach`, around the
+// pair search radius rl (r_c, or the pair list's r_c + skin).
+int32_t main_cells(const RepMeta& m, double rl, double margin, int reach) {
+    const double cmin = (rl * (1 + 1e-9) + margin) / reach;
     int32_t nc = (int32_t)std::floor(m.L / cmin);
     if (nc < 2 * reach + 1) nc = 1;
     return std::min(nc, 96);
@@ -412,10 +429,15 @@ int32_t main_cells(const RepMeta& m, double margin, int reach) {
 // Cell reach 0 picks one per replica: the largest of 4..1 whose cells hold
 // at least 32 particles on average (one i-chunk of the pair kernel is up to 64
 // particles of one cell, so emptier cells leave lanes idle), else 1.
+// Pair search radius: r_c, with a pair list r_c + skin. A list built at
+// positions b holds every pair with |b_i - b_j| < r_l; while every particle
+// is within skin / 2 of its b, a pair within r_c is within r_l at b.
+double list_radius(const mds_ctx* c, const RepMeta& m) { return m.rc * (1.0 + c->list_skin); }
+
 int replica_reach(const mds_ctx* c, const RepMeta& m, double margin) {
     if (c->reach > 0) return c->reach;
     for (int r = 4; r >= 1; --r) {
-        const double nc = main_cells(m, margin, r);
+        const double nc = main_cells(m, list_radius(c, m), margin, r);
         if ((double)m.n / (nc * nc * nc) >= 32.0) return r;
     }
     return 1;
@@ -425,7 +447,8 @@ void plan_grid(mds_ctx* c, const RepMeta& m, Rep& R) {
     const double dmax = 2 * m.rmax + 1e-10;
     R.margin = m.margin_scale * dmax;
     const int reach = replica_reach(c, m, R.margin);
-    const int32_t nc = main_cells(m, R.margin, reach);
+    const double rl = list_radius(c, m);
+    const int32_t nc = main_cells(m, rl, R.margin, reach);
     R.ncell = nc;
     R.cell = m.L / nc;
     // contact-search grid: cells of at least the search diameter, about two
@@ -443,7 +466,7 @@ void plan_grid(mds_ctx* c, const RepMeta& m, Rep& R) {
     // order (the pair kernel's half stencil: each pair of cells once), then
     // their negatives in the same order.
     const int H = nc == 1 ? 0 : reach;
-    const double lim = m.rc * (1 + 1e-9) + R.margin;
+    const double lim = rl * (1 + 1e-9) + R.margin;
     R.sten_off = (int32_t)c->h_sten.size();
     std::vector<int4> pos_half;
     for (int dz = -H; dz <= H; ++dz)
@@ -600,6 +623,30 @@ void plan_ewald(mds_ctx* c, const RepMeta& m, Rep& R) {
     R.px_is = std::ldexp(1.0, -ep);
 }
 
+// Words of a replica's pair-list region: the pairs within r_l of a uniform
+// suspension with a quarter extra for the quad padding of each candidate's
+// hits and two header words per window, times the replica's growth factor.
+int64_t list_capacity(const RepMeta& m, double rl) {
+    const double frac = std::min(1.0, 4.0 / 3.0 * 3.14159265358979 * rl * rl * rl / (m.L * m.L * m.L));
+    const double pairs = 0.5 * (double)m.n * (double)(m.n - 1) * frac;
+    const double words = (1.25 * pairs + 16.0 * (double)m.n + 4096.0) * m.list_scale;
+    return (int64_t)std::min(words, 4.0e18);
+}
+
+// Replica list states VALID -> STALE (rebuilt from their build positions at
+// the next geometry pass); INVALID stays.
+mds_status stale_lists(mds_ctx* c) {
+    if (!c->list_on() || c->nrep <= 0 || !c->list_state.p) return MDS_OK;
+    std::vector<int32_t> st(c->nrep);
+    HIPCHK(hipMemcpyAsync(st.data(), c->list_state.p, c->nrep * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int32_t& v : st)
+        if (v != LIST_INVALID) v = LIST_STALE;
+    HIPCHK(hipMemcpyAsync(c->list_state.p, st.data(), c->nrep * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MDS_OK;
+}
+
 // Plan grids, stencils and k space of every replica, size the scratch
 // buffers and upload the tables.
 mds_status plan_all(mds_ctx* c) {
@@ -616,7 +663,7 @@ mds_status plan_all(mds_ctx* c) {
     std::vector<int32_t> cell_rep, rep_of((size_t)c->n);
     c->h_chunks.clear();
     int64_t cells = 0, fcells = 0, work_cap = 0, tab = 0, part = 0, kb = 0, kbm = 0, ksm = 0, kall = 0;
-    int64_t tab32 = 0, pk32 = 0;
+    int64_t tab32 = 0, pk32 = 0, list_words = 0;
     int32_t nc_max = 0;
     int32_t nc_mfma_build = 0, nc_mfma_scatter = 0;
     bool any_mfma_build = false, any_mfma_scatter = false, any_vec_build = false, any_vec_scatter = false;
@@ -632,6 +679,13 @@ mds_status plan_all(mds_ctx* c) {
         plan_grid(c, m, R);
         plan_ewald(c, m, R);
         plan_kspace(c, m, R);
+        R.rlist = list_radius(c, m);
+        R.skin_half2 = 0.25 * (R.rlist - m.rc) * (R.rlist - m.rc);
+        R.list_base = list_words;
+        R.list_cap = c->list_on() ? list_capacity(m, R.rlist) : 0;
+        if (R.list_cap >= ((int64_t)1 << 31))
+            return fail(c, MDS_ECAPACITY, "the pair list of a replica exceeds 2^31 words (use a smaller r_c)");
+        list_words += R.list_cap;
         const int64_t ncm = (int64_t)R.ncell * R.ncell * R.ncell, ncf = (int64_t)R.fncell * R.fncell * R.fncell;
         R.cell_off = (int32_t)cells;
         R.fcell_off = (int32_t)fcells;
@@ -728,6 +782,16 @@ mds_status plan_all(mds_ctx* c) {
     HIPCHK(c->scan_tmp.ensure(scan_tiles(std::max<int64_t>({cells, fcells, c->n})) + 1));
     HIPCHK(c->red_chunks.ensure(c->h_chunks.size()));
     HIPCHK(c->red_part.ensure(3 * c->h_chunks.size()));
+    c->list_words = list_words;
+    if (c->list_on()) {
+        // grown by half again when it grows, never shrunk: a layout change
+        // (serving admits and retires replicas all the time) then rarely
+        // frees device memory, which waits for the whole device
+        if ((size_t)list_words > c->list.n) HIPCHK(c->list.ensure(std::max<int64_t>(1, list_words + list_words / 2)));
+        HIPCHK(c->list_mode.ensure(c->nrep));
+        HIPCHK(c->list_maxd2.ensure(c->nrep));
+        HIPCHK(c->list_used.ensure(c->nrep));
+    }
     hipStream_t s = c->stream;
     HIPCHK(hipMemcpyAsync(c->reps.p, c->h_rep.data(), c->nrep * sizeof(Rep), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->rep_of.p, rep_of.data(), rep_of.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -751,6 +815,14 @@ mds_status plan_all(mds_ctx* c) {
     // the FP32 scatter tiles' slots that no entry writes (rows past the
     // replica's last, |nx| past the row's end) stay zero
     if (pk32 > 0) HIPCHK(launch_fill_f64(s, (double*)c->pack32.p, 2 * pk32, 0.0));
+    if (c->list_on()) {
+        HIPCHK(launch_fill_f64(s, (double*)c->list_maxd2.p, c->nrep, 0.0));
+        HIPCHK(launch_fill_f64(s, (double*)c->list_used.p, c->nrep, 0.0));
+        // a list is a function of its build positions: the replicas that have
+        // them rebuild it from them (new regions, grids or offsets)
+        mds_status st = stale_lists(c);
+        if (st != MDS_OK) return st;
+    }
     HIPCHK(hipStreamSynchronize(s));
     c->planned = true;
     c->geometry_ready = false;
@@ -779,6 +851,11 @@ mds_status size_scratch(mds_ctx* c) {
         HIPCHK(b->ensure(n));
     HIPCHK(c->cand_off.ensure(n + 1));
     HIPCHK(c->err_partial.ensure(blocks_for(n)));
+    if (c->list_on()) {
+        HIPCHK(c->rec_build32.ensure(n));
+        HIPCHK(c->lcount.ensure((size_t)PAIR_SLICES * n));
+        HIPCHK(c->loff.ensure((size_t)PAIR_SLICES * n));
+    }
     return MDS_OK;
 }
 
@@ -816,6 +893,13 @@ Geometry geometry(mds_ctx* c) {
     g.inv_perm = c->inv_perm.p;
     g.rec_pos32 = c->rec_pos32.p;
     g.rec_mom32 = c->rec_mom32.p;
+    if (c->list_on()) {
+        g.rec_build32 = c->rec_build32.p;
+        g.list_mode = c->list_mode.p;
+        g.lcount = c->lcount.p;
+        g.loff = c->loff.p;
+        g.list = c->list.p;
+    }
     return g;
 }
 
@@ -834,23 +918,57 @@ PairParams pair_params(mds_ctx* c) {
     return pp;
 }
 
-// EvalSinCos + the pair geometry of CompInteractVecs for the current
-// positions: cell lists, contact candidates, sequential overlap correction.
-mds_status prepare_geometry_once(mds_ctx* c) {
+// Step-start positions, cell keys and the replicas' largest displacements
+// since their list builds, then the sin/cos tables of EvalSinCos: everything
+// the k-space chain reads of the geometry.
+mds_status prepare_positions(mds_ctx* c) {
     const int64_t n = c->n;
     hipStream_t s = c->stream;
+    const bool list = c->list_on();
     {
         mds_ctx::Span sp(c, KC_SETUP);
+        // with a pair list the main grid sorts the build positions: a kept
+        // replica's cells, order and work items come out as built
         HIPCHK(launch_prepare(s, n, c->pos.p, c->pos0.p, c->pos_cur.p, c->reps.p, c->rep_of.p, c->key_main.p,
-                              c->key_fine.p));
+                              c->key_fine.p, list ? c->pos_build.p : nullptr, c->list_state.p,
+                              (unsigned long long*)c->list_maxd2.p));
+    }
+    {
+        mds_ctx::Span sp(c, KC_SINCOS);
+        HIPCHK(launch_sincos_tables(s, n, c->reps.p, c->rep_of.p, c->pos0.p, c->tab.p, c->tab32.p));
+    }
+    return MDS_OK;
+}
+
+// The pair geometry of CompInteractVecs after prepare_positions: cell lists
+// (or the pair lists), contact candidates, sequential overlap correction.
+mds_status prepare_cells(mds_ctx* c) {
+    const int64_t n = c->n;
+    hipStream_t s = c->stream;
+    const bool list = c->list_on();
+    {
+        mds_ctx::Span sp(c, KC_SETUP);
+        if (list) {
+            HIPCHK(launch_list_decide(s, c->nrep, c->reps.p, c->list_state.p, (unsigned long long*)c->list_maxd2.p,
+                                      c->list_mode.p, (unsigned long long*)c->list_builds.p));
+            HIPCHK(launch_list_keys(s, n, c->reps.p, c->rep_of.p, c->list_mode.p, c->pos0.p, c->pos_build.p,
+                                    c->key_main.p));
+        }
         HIPCHK(launch_cell_sort(s, n, c->key_main.p, c->cells_main, c->cnt_main.p, c->start_main.p, c->end_main.p,
                                 c->cur_main.p, c->sorted_main.p, c->scan_tmp.p));
         HIPCHK(launch_cell_order(s, c->cells_main, c->start_main.p, c->end_main.p, c->sorted_main.p,
                                  c->order_tmp.p));
-        HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->rec_pos.p, c->rec_pos32.p,
-                                    c->inv_perm.p));
+        HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->rec_pos.p, c->rec_pos32.p, c->inv_perm.p,
+                                    c->reps.p, c->rep_of.p, list ? c->pos_build.p : nullptr, c->list_mode.p,
+                                    c->rec_build32.p, c->lcount.p));
         HIPCHK(launch_build_work(s, c->cells_main, c->start_main.p, c->end_main.p, c->work.p, c->nwork.p,
                                  c->work_cap, c->err_flag.p));
+    }
+    if (list) {
+        mds_ctx::Span sp(c, KC_LIST);
+        HIPCHK(launch_list_build(s, geometry(c), c->nrep, c->work_cap, c->stencil.p, c->lcount.p, c->loff.p,
+                                 c->list.p, c->list_mode.p, c->list_state.p, (long long*)c->list_used.p,
+                                 c->err_flag.p));
     }
     {
         mds_ctx::Span sp(c, KC_CONTACTS);
@@ -869,12 +987,16 @@ mds_status prepare_geometry_once(mds_ctx* c) {
                               c->err_flag.p));
         HIPCHK(launch_mark_moved(s, c->n_moved.p, c->moved_list.p, c->inv_perm.p, c->rec_pos.p));
     }
-    {
-        mds_ctx::Span sp(c, KC_SINCOS);
-        HIPCHK(launch_sincos_tables(s, n, c->reps.p, c->rep_of.p, c->pos0.p, c->tab.p, c->tab32.p));
-    }
     c->geometry_ready = true;
     return MDS_OK;
+}
+
+// EvalSinCos + the pair geometry of CompInteractVecs for the current
+// positions.
+mds_status prepare_geometry_once(mds_ctx* c) {
+    mds_status st = prepare_positions(c);
+    if (st != MDS_OK) return st;
+    return prepare_cells(c);
 }
 
 // With `check`, the error flags are cleared first and the overlap
@@ -889,6 +1011,17 @@ mds_status prepare_geometry(mds_ctx* c, bool check) {
         int32_t f = 0;
         HIPCHK(hipMemcpyAsync(&f, c->err_flag.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        if ((f & 128) && attempt <= 4) {
+            // a pair list outgrew its region: regions sized from the words
+            // the build counted, with a quarter to spare
+            std::vector<int64_t> used(c->nrep);
+            HIPCHK(hipMemcpy(used.data(), c->list_used.p, c->nrep * sizeof(int64_t), hipMemcpyDeviceToHost));
+            for (int r = 0; r < c->nrep; ++r)
+                if (used[r] > c->h_rep[r].list_cap)
+                    c->meta[r].list_scale *= 1.25 * (double)used[r] / (double)std::max<int64_t>(1, c->h_rep[r].list_cap);
+            if ((st = plan_all(c)) != MDS_OK) return st;
+            continue;
+        }
         double ms = 0;
         for (const RepMeta& m : c->meta) ms = std::max(ms, m.margin_scale);
         if (!(f & 32) || ms >= 2.0 || attempt > 4) return MDS_OK;
@@ -897,23 +1030,16 @@ mds_status prepare_geometry(mds_ctx* c, bool check) {
     }
 }
 
-// Real-space + k-space Ewald (+ short range) for the moments in c->mom /
-// c->rec_mom (their per-replica sums in c->msum). force_pass:
-// CompDipInteractEwaldR/F with forces, pressure and the thermal force added;
-// else the field-only CompDipFieldEwaldR/F of CompDemagField.
-mds_status evaluate(mds_ctx* c, bool force_pass) {
+// k space of an evaluation (CompDipInteractEwaldF / CompDipFieldEwaldF) for
+// the moments in c->mom (their per-replica sums in c->msum): S(k), its chunk
+// reduction, then the scatter that writes the field (and force) with the
+// self, surface and thermal-force terms; the pair sums are added to it last.
+// The chain reads only the sin/cos tables and the moments, so it is forked
+// onto the side stream (`forked`) and runs beside the cell and pair kernels
+// (in order on one stream when kernels are timed).
+mds_status kspace_chain(mds_ctx* c, bool force_pass, bool& forked) {
     const int64_t n = c->n;
     hipStream_t s = c->stream;
-    Geometry g = geometry(c);
-    PairParams pp = pair_params(c);
-    PairOut po{force_pass ? c->force.p : nullptr, c->field.p, c->press_partial.p,
-               (unsigned long long*)c->pair_acc.p, c->err_flag.p};
-    const size_t moved_press = 3 * (size_t)c->work_cap * PAIR_SLICES;
-    // k space: S(k), its chunk reduction, then the scatter that writes the
-    // field (and force) with the self, surface and thermal-force terms; the
-    // pair sums are added to it last. The k-space chain needs only the tables
-    // and moments, so it runs on the side stream beside the pair kernels
-    // (in order on one stream when kernels are timed).
     auto kspace = [&](hipStream_t ks) -> mds_status {
         if (c->nk_all > 0) {
             {
@@ -941,23 +1067,69 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
                                    c->mom.p, c->msum.p, force_pass ? c->n_f.p : nullptr, c->force.p, c->field.p));
         return MDS_OK;
     };
-    const bool split = c->overlap_k && !c->timing;
-    mds_status st;
-    if (split) {
+    forked = c->overlap_k && !c->timing;
+    if (forked) {
         HIPCHK(hipEventRecord(c->ev_fork, s));
         HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        if ((st = kspace(c->side)) != MDS_OK) return st;
+        mds_status st = kspace(c->side);
+        if (st != MDS_OK) return st;
         HIPCHK(hipEventRecord(c->ev_join, c->side));
-    } else if ((st = kspace(s)) != MDS_OK) {
-        return st;
+        return MDS_OK;
     }
+    return kspace(s);
+}
+
+// Real space of an evaluation (CompDipInteractEwaldR / CompDipFieldEwaldR,
+// with pressure and the short-range terms in the force pass) for the moments
+// in c->rec_mom, added to what the k-space chain wrote (joined first when it
+// was forked).
+mds_status pair_pass(mds_ctx* c, bool force_pass, bool forked) {
+    hipStream_t s = c->stream;
+    Geometry g = geometry(c);
+    PairParams pp = pair_params(c);
+    PairOut po{force_pass ? c->force.p : nullptr, c->field.p, c->press_partial.p,
+               (unsigned long long*)c->pair_acc.p, c->err_flag.p};
+    const size_t moved_press = 3 * (size_t)c->work_cap * PAIR_SLICES;
     mds_ctx::Span sp(c, force_pass ? KC_PAIRS_FORCE : KC_PAIRS_FIELD);
     int64_t* cnt = c->count_work ? (int64_t*)c->pair_counter.p : nullptr;
     if (force_pass) HIPCHK(launch_fill_f64(s, c->press_partial.p + moved_press, 6 * c->nrep, 0.0));
-    HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p,
-                        c->work_cap * PAIR_SLICES, cnt));
-    if (split) HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
+    if (c->list_on())
+        HIPCHK(launch_pairs_list(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p,
+                                 c->work_cap * PAIR_SLICES, cnt));
+    else
+        HIPCHK(launch_pairs(s, force_pass, c->cfg.precision, g, c->rec_mom.p, pp, po, c->work_cap, c->stencil.p,
+                            c->work_cap * PAIR_SLICES, cnt));
+    if (forked) HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
     HIPCHK(launch_pairs_out(s, g, po));
+    return MDS_OK;
+}
+
+// Real-space + k-space Ewald (+ short range): force_pass
+// CompDipInteractEwaldR/F with forces, pressure and the thermal force added,
+// else the field-only CompDipFieldEwaldR/F of CompDemagField.
+mds_status evaluate(mds_ctx* c, bool force_pass) {
+    bool forked = false;
+    mds_status st = kspace_chain(c, force_pass, forked);
+    if (st != MDS_OK) return st;
+    return pair_pass(c, force_pass, forked);
+}
+
+// Moments in original order and their per-replica sums (what the k-space
+// chain reads), without the cell-sorted copies.
+mds_status moments_orig(mds_ctx* c, const double* mm_dev) {
+    mds_ctx::Span sp(c, KC_MOMENTS);
+    HIPCHK(launch_moments(c->stream, c->n, mm_dev, c->mag_mom.p, nullptr, c->reps.p, c->rep_of.p, c->mom.p, nullptr,
+                          nullptr));
+    HIPCHK(launch_rep_sum3(c->stream, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), c->n,
+                           c->mom.p, c->red_part.p, c->msum.p));
+    return MDS_OK;
+}
+
+// The cell-sorted copies (FP64, FP32 in replica units) the pair kernels read.
+mds_status moments_sorted(mds_ctx* c, const double* mm_dev) {
+    mds_ctx::Span sp(c, KC_MOMENTS);
+    HIPCHK(launch_moments(c->stream, c->n, mm_dev, c->mag_mom.p, c->inv_perm.p, c->reps.p, c->rep_of.p, nullptr,
+                          c->rec_mom.p, c->rec_mom32.p));
     return MDS_OK;
 }
 
@@ -1009,6 +1181,7 @@ mds_status read_flags(mds_ctx* c) {
     c->last_events = nev;
     if (f & 32) return fail(c, MDS_EOVERLAP, "overlap correction moved a particle beyond the contact search margin");
     if (f & 64) return fail(c, MDS_ENONFINITE, "a pair sum is not finite or beyond the fixed-point range");
+    if (f & 128) return fail(c, MDS_ECAPACITY, "a pair list exceeded its region");
     if (f) return fail(c, MDS_ECAPACITY, "device capacity exceeded (flag " + std::to_string(f) + ")");
     return MDS_OK;
 }
@@ -1040,10 +1213,24 @@ mds_status fixed_step_body(mds_ctx* c, double h, bool check) {
                             c->cfg.sat_mag, c->cfg.enable_therm_torque, c->cfg.enable_therm_field,
                             c->cfg.enable_therm_force, c->n_t.p, c->n_b.p, c->n_f.p));
     }
-    mds_status st = prepare_geometry(c, check);
-    if (st != MDS_OK) return st;
-    if ((st = moments_of(c, c->mm.p)) != MDS_OK) return st;
-    if ((st = evaluate(c, true)) != MDS_OK) return st;
+    mds_status st;
+    if (!check && c->overlap_k && !c->timing) {
+        // the force pass's k-space chain needs only the step-start tables
+        // and moments: forked before the cell sort, the pair list and the
+        // contact search, so it runs beside them too (the same inputs, so
+        // the same results as after them)
+        if ((st = prepare_positions(c)) != MDS_OK) return st;
+        if ((st = moments_orig(c, c->mm.p)) != MDS_OK) return st;
+        bool forked = false;
+        if ((st = kspace_chain(c, true, forked)) != MDS_OK) return st;
+        if ((st = prepare_cells(c)) != MDS_OK) return st;
+        if ((st = moments_sorted(c, c->mm.p)) != MDS_OK) return st;
+        if ((st = pair_pass(c, true, forked)) != MDS_OK) return st;
+    } else {
+        if ((st = prepare_geometry(c, check)) != MDS_OK) return st;
+        if ((st = moments_of(c, c->mm.p)) != MDS_OK) return st;
+        if ((st = evaluate(c, true)) != MDS_OK) return st;
+    }
     const Tableau tb = make_tableau(MDS_HEUN_EULER);
     const int nb = blocks_for(n);
     const StageParams sp = stage_params(c, h);
@@ -1231,8 +1418,8 @@ mds_status relayout(mds_ctx* c, int64_t old_n, int64_t new_n, const std::vector<
     const int64_t mn = c->reserve_n;
     DevBuf<double>* one[7] = {&c->r_mag, &c->mag_mom, &c->zeta_rot, &c->zeta_trans, &c->vol_mag, &c->c1, &c->c2};
     for (int k = 0; k < 7; ++k) HIPCHK(repack(*one[k], c->spare[k], 1, old_n, new_n, mn, segs, s));
-    DevBuf<double>* three[3] = {&c->pos, &c->mm, &c->ea};
-    for (int k = 0; k < 3; ++k) HIPCHK(repack(*three[k], c->spare[7 + k], 3, old_n, new_n, mn, segs, s));
+    DevBuf<double>* three[4] = {&c->pos, &c->mm, &c->ea, &c->pos_build};
+    for (int k = 0; k < 4; ++k) HIPCHK(repack(*three[k], c->spare[7 + k], 3, old_n, new_n, mn, segs, s));
     // injected noise does not survive a change of layout
     for (auto* b : {&c->n_t, &c->n_b, &c->n_f}) {
         HIPCHK(b->ensure(3 * (size_t)std::max(new_n, mn)));
@@ -1246,32 +1433,47 @@ mds_status relayout(mds_ctx* c, int64_t old_n, int64_t new_n, const std::vector<
 mds_status relayout_rep_values(mds_ctx* c, const std::vector<int>& keep, int added, uint64_t counter) {
     const int old_nrep = c->rep_t.p ? c->nrep : 0;
     std::vector<double> t_old(old_nrep);
-    std::vector<uint64_t> s_old(old_nrep);
+    std::vector<uint64_t> s_old(old_nrep), b_old(old_nrep);
+    std::vector<int32_t> l_old(old_nrep);
     if (old_nrep > 0) {
         HIPCHK(hipMemcpyAsync(t_old.data(), c->rep_t.p, old_nrep * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(s_old.data(), c->rep_step.p, old_nrep * sizeof(uint64_t), hipMemcpyDeviceToHost,
                               c->stream));
+        HIPCHK(hipMemcpyAsync(l_old.data(), c->list_state.p, old_nrep * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(b_old.data(), c->list_builds.p, old_nrep * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     std::vector<double> t_new;
-    std::vector<uint64_t> s_new;
+    std::vector<uint64_t> s_new, b_new;
+    std::vector<int32_t> l_new;
     for (int k : keep) {
         t_new.push_back(t_old[k]);
         s_new.push_back(s_old[k]);
+        l_new.push_back(l_old[k]);
+        b_new.push_back(b_old[k]);
     }
     for (int a = 0; a < added; ++a) {
         t_new.push_back(0.0);
         s_new.push_back(counter);
+        l_new.push_back(LIST_INVALID);  // no build positions yet
+        b_new.push_back(0);
     }
     const size_t m = t_new.size();
     if (c->rep_t.n < m) {
         // grow with headroom; the contents are rewritten below
         HIPCHK(c->rep_t.ensure(2 * m));
         HIPCHK(c->rep_step.ensure(2 * m));
+        HIPCHK(c->list_state.ensure(2 * m));
+        HIPCHK(c->list_builds.ensure(2 * m));
     }
     if (m > 0) {
         HIPCHK(hipMemcpyAsync(c->rep_t.p, t_new.data(), m * sizeof(double), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(c->rep_step.p, s_new.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->list_state.p, l_new.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->list_builds.p, b_new.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice,
+                              c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     return MDS_OK;
@@ -1479,6 +1681,48 @@ mds_status mds_set_tiling(mds_ctx* c, int32_t cell_reach) {
     return MDS_OK;
 }
 
+mds_status mds_set_pair_list(mds_ctx* c, double skin) {
+    if (!c || !(skin >= 0) || skin > 0.25) return fail(c, MDS_EINVAL, "pair-list skin must be in [0, 0.25] of r_c");
+    HIPCHK(hipSetDevice(c->device));
+    c->list_skin = skin;
+    c->planned = false;
+    c->geometry_ready = false;
+    c->drop_graph();
+    // every list is built afresh from the current positions
+    if (c->nrep > 0) {
+        HIPCHK(launch_fill_i32(c->stream, c->list_state.p, c->nrep, LIST_INVALID));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return MDS_OK;
+}
+
+int64_t mds_pair_list_words(int64_t n, double l_box, double r_cut, double skin) {
+    if (n <= 0 || !(l_box > 0) || !(r_cut > 0) || !(skin > 0)) return 0;
+    RepMeta m;
+    m.n = n;
+    m.L = l_box;
+    m.rc = r_cut;
+    return list_capacity(m, r_cut * (1.0 + skin));
+}
+
+mds_status mds_pair_list_stats(mds_ctx* c, int32_t index, int64_t* builds, int64_t* words, int64_t* capacity) {
+    if (!c || index < 0 || index >= c->nrep) return MDS_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    uint64_t b = 0;
+    int64_t w = 0;
+    if (c->list_on() && c->planned) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(&b, c->list_builds.p + index, sizeof(b), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&w, c->list_used.p + index, sizeof(w), hipMemcpyDeviceToHost));
+    } else if (c->list_on()) {
+        HIPCHK(hipMemcpy(&b, c->list_builds.p + index, sizeof(b), hipMemcpyDeviceToHost));
+    }
+    if (builds) *builds = (int64_t)b;
+    if (words) *words = w;
+    if (capacity) *capacity = (c->list_on() && c->planned) ? c->h_rep[index].list_cap : 0;
+    return MDS_OK;
+}
+
 mds_status mds_get_tiling(const mds_ctx* c, int32_t* cell_reach, int32_t* ncell, int32_t* nstencil) {
     if (!c) return MDS_EINVAL;
     if (cell_reach) *cell_reach = c->reach;
@@ -1601,6 +1845,8 @@ mds_status mds_set_state(mds_ctx* c, const double* pos, const double* pos_mm, co
     if ((st = upload(c, c->pos.p, pos, n3)) != MDS_OK) return st;
     if ((st = upload(c, c->mm.p, pos_mm, n3)) != MDS_OK) return st;
     if ((st = upload(c, c->ea.p, pos_ea, n3)) != MDS_OK) return st;
+    // new positions: every list is built afresh
+    if (c->nrep > 0) HIPCHK(launch_fill_i32(c->stream, c->list_state.p, c->nrep, LIST_INVALID));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->state_set = true;
     c->geometry_ready = false;
@@ -1792,14 +2038,18 @@ namespace {
 // observables, and each replica's clock and step counter.
 mds_status snapshot(mds_ctx* c, bool restore) {
     const size_t n3 = 3 * (size_t)c->n, nr = (size_t)c->nrep;
-    const size_t need = 3 * n3 + (c->observe ? n3 : 0);
+    const size_t need = 5 * n3;
     if (!restore) {
         HIPCHK(c->snap.ensure(need));
         HIPCHK(c->snap_t.ensure(nr));
         HIPCHK(c->snap_step.ensure(nr));
+        HIPCHK(c->snap_list.ensure(nr));
     }
-    double* live[4] = {c->pos.p, c->mm.p, c->ea.p, c->observe ? c->true_pos.p : nullptr};
-    for (int k = 0; k < 4; ++k) {
+    // the pair lists' build positions and states: a restored chunk's lists
+    // are rebuilt from the build positions it started with
+    double* live[5] = {c->pos.p, c->mm.p, c->ea.p, c->observe ? c->true_pos.p : nullptr,
+                       c->list_on() ? c->pos_build.p : nullptr};
+    for (int k = 0; k < 5; ++k) {
         if (!live[k]) continue;
         double* sp = c->snap.p + k * n3;
         HIPCHK(hipMemcpyAsync(restore ? live[k] : sp, restore ? sp : live[k], n3 * sizeof(double),
@@ -1809,6 +2059,11 @@ mds_status snapshot(mds_ctx* c, bool restore) {
                           hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(restore ? c->rep_step.p : c->snap_step.p, restore ? c->snap_step.p : c->rep_step.p,
                           nr * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
+    if (c->list_on()) {
+        HIPCHK(hipMemcpyAsync(restore ? c->list_state.p : c->snap_list.p, restore ? c->snap_list.p : c->list_state.p,
+                              nr * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+        if (restore) return stale_lists(c);
+    }
     return MDS_OK;
 }
 
@@ -2110,7 +2365,7 @@ mds_status mds_reserve(mds_ctx* c, int64_t n_particles) {
     }
     mds_status st = size_scratch(c);
     if (st != MDS_OK) return st;
-    for (int k = 0; k < 10; ++k) {
+    for (int k = 0; k < 11; ++k) {
         const size_t need = (size_t)(k < 7 ? 1 : 3) * (size_t)std::max<int64_t>(1, n_particles);
         if (c->spare[k].n < need) HIPCHK(c->spare[k].ensure(need));
     }
@@ -2232,7 +2487,7 @@ mds_status mds_set_graph(mds_ctx* c, int32_t on) {
 namespace {
 
 constexpr char kCkptMagic[8] = {'M', 'D', 'S', 'C', 'K', 'P', 'T', '1'};
-constexpr uint32_t kCkptVersion = 1;
+constexpr uint32_t kCkptVersion = 2;
 
 uint64_t fnv1a64(const unsigned char* p, size_t n) {
     uint64_t h = 14695981039346656037ull;
@@ -2336,6 +2591,18 @@ mds_status mds_save_checkpoint(mds_ctx* c, const char* path, double t, double dt
         HIPCHK(hipMemcpy(tmp.data(), a->p, n3 * sizeof(double), hipMemcpyDeviceToHost));
         o.raw(tmp.data(), n3 * sizeof(double));
     }
+    // pair lists: the skin, each replica's state and the build positions (a
+    // resumed run rebuilds the lists from them, so it evaluates the pairs in
+    // the order the uninterrupted run does)
+    o.put<int32_t>(c->list_on() ? 1 : 0);
+    o.put<double>(c->list_skin);
+    if (c->list_on()) {
+        std::vector<int32_t> ls(c->nrep);
+        HIPCHK(hipMemcpy(ls.data(), c->list_state.p, c->nrep * sizeof(int32_t), hipMemcpyDeviceToHost));
+        o.raw(ls.data(), c->nrep * sizeof(int32_t));
+        HIPCHK(hipMemcpy(tmp.data(), c->pos_build.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+        o.raw(tmp.data(), n3 * sizeof(double));
+    }
     o.put<uint64_t>(fnv1a64(o.b.data(), o.b.size()));
     std::FILE* f = std::fopen(path, "wb");
     if (!f) return fail(c, MDS_EINVAL, std::string("cannot open ") + path + " for writing");
@@ -2390,8 +2657,9 @@ mds_status mds_load_checkpoint(mds_ctx* c, const char* path, double* t, double* 
     }
     const size_t n3 = 3 * (size_t)n;
     const size_t need = (2 * (size_t)n + (observe ? 5 : 3) * n3) * sizeof(double);
-    if (!in.ok || total != n || in.n - in.at != need) return fail(c, MDS_EINVAL, "corrupt checkpoint body");
-    std::vector<double> r_mag(n), r_hydr(n), st(3 * n3), obs_pos;
+    if (!in.ok || total != n || in.n - in.at < need + sizeof(int32_t) + sizeof(double))
+        return fail(c, MDS_EINVAL, "corrupt checkpoint body");
+    std::vector<double> r_mag(n), r_hydr(n), st(3 * n3), obs_pos, pos_build;
     in.raw(r_mag.data(), n * sizeof(double));
     in.raw(r_hydr.data(), n * sizeof(double));
     in.raw(st.data(), 3 * n3 * sizeof(double));
@@ -2399,9 +2667,23 @@ mds_status mds_load_checkpoint(mds_ctx* c, const char* path, double* t, double* 
         obs_pos.resize(2 * n3);
         in.raw(obs_pos.data(), 2 * n3 * sizeof(double));
     }
+    const int32_t list_on = in.get<int32_t>();
+    const double list_skin = in.get<double>();
+    std::vector<int32_t> list_state;
+    if (list_on) {
+        if (in.n - in.at != nrep * sizeof(int32_t) + n3 * sizeof(double) || !(list_skin > 0))
+            return fail(c, MDS_EINVAL, "corrupt checkpoint pair-list section");
+        list_state.resize(nrep);
+        pos_build.resize(n3);
+        in.raw(list_state.data(), nrep * sizeof(int32_t));
+        in.raw(pos_build.data(), n3 * sizeof(double));
+    } else if (in.n != in.at) {
+        return fail(c, MDS_EINVAL, "corrupt checkpoint body");
+    }
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
     clear_replicas(c);
+    c->list_skin = list_on ? list_skin : 0.0;
     std::vector<int64_t> ns(nrep);
     std::vector<double> Ls(nrep), fz(nrep);
     std::vector<uint64_t> seeds(nrep);
@@ -2426,6 +2708,13 @@ mds_status mds_load_checkpoint(mds_ctx* c, const char* path, double* t, double* 
     }
     HIPCHK(hipMemcpy(c->rep_t.p, rt.data(), nrep * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->rep_step.p, rs.data(), nrep * sizeof(uint64_t), hipMemcpyHostToDevice));
+    if (list_on) {
+        // rebuilt from the stored build positions at the next geometry pass
+        for (int32_t& v : list_state)
+            if (v != LIST_INVALID) v = LIST_STALE;
+        HIPCHK(hipMemcpy(c->list_state.p, list_state.data(), nrep * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->pos_build.p, pos_build.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+    }
     c->reach = reach;
     c->planned = false;
     c->geometry_ready = false;
@@ -2473,6 +2762,10 @@ int64_t mds_device_bytes(const mds_ctx* c) {
     add(c->tab32.n, sizeof(float));
     add(c->pack32.n, sizeof(float4));
     add(c->reps.n, sizeof(Rep));
+    add(c->pos_build.n, sizeof(double));
+    add(c->list.n + c->lcount.n + c->loff.n + c->list_state.n + c->list_mode.n + c->snap_list.n, sizeof(int32_t));
+    add(c->list_maxd2.n + c->list_builds.n + c->list_used.n, sizeof(uint64_t));
+    add(c->rec_build32.n, sizeof(float4));
     return b;
 }
 

@@ -71,7 +71,27 @@ struct Rep {
     double field_z;              // magFluxDens of this replica
     uint64_t seed;               // Philox key of this replica
     uint64_t stop_at;            // the replica stops stepping when its step counter reaches this
+    // pair list (list_cap = 0: no list, the pair kernels search the cells
+    // every evaluation): the list holds every pair within rlist = r_c + skin
+    // of the build positions and stays valid while no particle has moved more
+    // than skin / 2 from its build position (skin_half2 = (skin / 2)^2)
+    double rlist, skin_half2;
+    int64_t list_base, list_cap;  // the replica's region of the list (int32 words)
 };
+
+// Pair-list modes of a replica for the current step (list_mode): keep the
+// list, build it from the current positions, or build it again from the
+// stored build positions (after a change of layout or a resume: the same
+// list as before, since a list is a function of its build positions); BAD:
+// the list did not fit its region (flag 128), the replica's pairs are not
+// evaluated this step. list_state: INVALID (no build positions yet), VALID,
+// RESTORE (build positions valid, list to be rebuilt from them).
+constexpr int32_t LIST_KEEP = 0, LIST_FRESH = 1, LIST_RESTORE = 2, LIST_BAD = 3;
+constexpr int32_t LIST_INVALID = 0, LIST_VALID = 1, LIST_STALE = 2;
+// A list window: two header words (first sorted slot j0 of the 64-particle
+// block of a neighbour cell; E << 13 | count << 6 | image code) and E queue
+// entries (j << 6) | i lane, each candidate's hits padded to whole quads.
+constexpr int LIST_HDR = 2;
 
 // Per-step geometry of the real-space sum (cell list + overlap versions).
 struct Geometry {
@@ -102,6 +122,12 @@ struct Geometry {
     const int32_t* inv_perm;     // sorted slot of an original index
     const float4* rec_pos32;     // FP32 copy of rec_pos (candidate search)
     const float4* rec_mom32;     // FP32 moments / mref of the replica, sorted order (mixed path)
+    // pair list (null when off)
+    const float4* rec_build32;   // [n] FP32 build positions, sorted order (list build)
+    const int32_t* list_mode;    // [replicas] LIST_KEEP / FRESH / RESTORE / BAD of this step
+    const int32_t* lcount;       // [4n] words of the stream of (chunk start slot, wavefront)
+    const int32_t* loff;         // [4n] its offset in the replica's region
+    const int32_t* list;         // the streams
 };
 
 struct PairOut {
@@ -139,16 +165,41 @@ hipError_t launch_peak_fma(hipStream_t s, bool fp64, int32_t blocks, int32_t ite
 hipError_t launch_fill_f64(hipStream_t s, double* p, int64_t count, double v);
 hipError_t launch_scale(hipStream_t s, double* p, int64_t count, double f);
 
-// step setup: step-start positions and cell keys (global cell ids)
+// step setup: step-start positions and cell keys (global cell ids). With a
+// pair list (pos_build not null) the main-grid keys are left to
+// launch_list_keys, and each particle's squared displacement from its build
+// position goes into the replica's maximum (list_maxd2, double bits).
 hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* pos0, double* pos_cur, const Rep* reps,
-                          const int32_t* rep_of, int32_t* key_main, int32_t* key_fine);
+                          const int32_t* rep_of, int32_t* key_main, int32_t* key_fine, const double* pos_build,
+                          const int32_t* list_state, unsigned long long* list_maxd2);
+// Pair list: this step's mode per replica (from its state and maximum
+// displacement; resets the maximum, marks the state valid), then the build
+// positions of the replicas built afresh (<- pos0) and every particle's
+// main-grid key from its build position.
+hipError_t launch_list_decide(hipStream_t s, int32_t nrep, const Rep* reps, int32_t* list_state,
+                              unsigned long long* list_maxd2, int32_t* list_mode, unsigned long long* list_builds);
+hipError_t launch_list_keys(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of,
+                            const int32_t* list_mode, const double* pos0, double* pos_build, int32_t* key_main);
+// Count pass, per-replica offsets, fill pass of the lists of the replicas
+// whose mode is FRESH or RESTORE (grid: max_work blocks). list_used[r]: the
+// replica's words; a list beyond its region sets flag 128, mode BAD and
+// state INVALID.
+hipError_t launch_list_build(hipStream_t s, const Geometry& g, int32_t nrep, int32_t max_work, const int4* stencil,
+                             int32_t* lcount, int32_t* loff, int32_t* list, int32_t* list_mode,
+                             int32_t* list_state, long long* list_used, int32_t* err);
 // scan_tmp of scan_tiles(cells) ints
 int32_t scan_tiles(int64_t m);
 hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
                             int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp);
-// rec_pos[k].w carries a tag: the original index in the low 32 bits
+// rec_pos[k].w carries a tag: the original index in the low 32 bits. With a
+// pair list (pos_build not null) each record is the step-start position in
+// the periodic image nearest the build position (so a list window's image
+// shift stays right), rec_build32 gets the FP32 build position, and the
+// stream lengths lcount of the replicas being built are cleared.
 hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_idx, const double* pos0,
-                                double4* rec_pos, float4* rec_pos32, int32_t* inv_perm);
+                                double4* rec_pos, float4* rec_pos32, int32_t* inv_perm, const Rep* reps,
+                                const int32_t* rep_of, const double* pos_build, const int32_t* list_mode,
+                                float4* rec_build32, int32_t* lcount);
 // after the overlap correction: flag the moved particles' tags
 hipError_t launch_mark_moved(hipStream_t s, const int32_t* n_moved, const int32_t* moved_list,
                              const int32_t* inv_perm, double4* rec_pos);
@@ -170,7 +221,8 @@ hipError_t launch_resolve(hipStream_t s, int32_t nrep, int64_t n, const Rep* rep
 hipError_t launch_clear_events(hipStream_t s, int64_t n, int32_t* ev_head, int32_t* ev_tail, int32_t* n_events,
                                int32_t* n_moved, double* max_disp);
 
-// moments m_i = mm_i * mu_i (original and sorted order); per-replica sums
+// moments m_i = mm_i * mu_i (original and sorted order; mom null: sorted
+// only, inv_perm null: original only); per-replica sums
 hipError_t launch_moments(hipStream_t s, int64_t n, const double* mm, const double* mag_mom, const int32_t* inv_perm,
                           const Rep* reps, const int32_t* rep_of, double* mom, double4* rec_mom, float4* rec_mom32);
 // chunks: (replica, first particle) of RED_CHUNK-particle pieces that never
@@ -197,6 +249,12 @@ hipError_t launch_mean_z_noalloc(hipStream_t s, int64_t n, const double* mm, dou
 hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
                         const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
                         int32_t moved_press_off, int64_t* cnt_pairs);
+// The same sums from the pair lists (no cell search): every pair of a list
+// is evaluated with the exact r < r_c test on the records; the coating
+// force pass keeps the searching kernel.
+hipError_t launch_pairs_list(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
+                             const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
+                             int32_t moved_press_off, int64_t* cnt_pairs);
 hipError_t launch_pairs_out(hipStream_t s, const Geometry& g, const PairOut& out);
 // EvalTransCoeffs per step (src/postprocessing.cpp:167-187): unwrap the
 // true positions, per replica t, MSD x/y/z from pos_ref and the force pass's

@@ -100,25 +100,100 @@ int grid_for(int64_t n, int block, int cap = 65535 * 4) {
 // pos0 <- pos (step-start positions: EvalSinCos and the pair scan read these),
 // pos_cur <- pos (positions the overlap correction moves), cell keys in the
 // replica's main and fine grids (global cell ids).
+__device__ __forceinline__ int32_t main_key(const Rep& R, double x, double y, double z) {
+    const double L = R.L;
+    const int nc = R.ncell;
+    return R.cell_off + (cell_coord(z, L, R.cell, nc) * nc + cell_coord(y, L, R.cell, nc)) * nc +
+           cell_coord(x, L, R.cell, nc);
+}
+
 __global__ void k_prepare(int64_t n, const double* __restrict__ pos, double* __restrict__ pos0,
                           double* __restrict__ pos_cur, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
-                          int32_t* __restrict__ key_main, int32_t* __restrict__ key_fine) {
+                          int32_t* __restrict__ key_main, int32_t* __restrict__ key_fine,
+                          const double* __restrict__ pos_build, const int32_t* __restrict__ list_state,
+                          unsigned long long* __restrict__ list_maxd2) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    const int32_t r = live ? rep_of[i] : 0;
+    double d2 = 0.0;
+    if (live) {
+        const Rep& R = reps[r];
+        const double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
+        pos0[i] = x;
+        pos0[n + i] = y;
+        pos0[2 * n + i] = z;
+        pos_cur[i] = x;
+        pos_cur[n + i] = y;
+        pos_cur[2 * n + i] = z;
+        const double L = R.L;
+        const int fn = R.fncell;
+        if (!pos_build) {
+            key_main[i] = main_key(R, x, y, z);
+        } else if (list_state[r] != LIST_INVALID) {
+            // displacement from the build position (minimum image): a
+            // particle that crossed a face since the build is still near it
+            const double iL = 1.0 / L;
+            const double dx = min_image_r(x - pos_build[i], L, iL);
+            const double dy = min_image_r(y - pos_build[n + i], L, iL);
+            const double dz = min_image_r(z - pos_build[2 * n + i], L, iL);
+            d2 = dx * dx + dy * dy + dz * dz;
+        }
+        key_fine[i] = R.fcell_off + (cell_coord(z, L, R.fcell, fn) * fn + cell_coord(y, L, R.fcell, fn)) * fn +
+                      cell_coord(x, L, R.fcell, fn);
+    }
+    if (!pos_build) return;  // uniform
+    // the replica's largest displacement (non-negative doubles order as
+    // their bit patterns): one atomic per wavefront when its particles share
+    // a replica, else one per particle
+    const int32_t r0 = __builtin_amdgcn_readfirstlane(r);
+    if (ballot(live && r != r0) == 0) {
+        const double m = wave_max(d2);
+        if (lane_id() == 0 && m > 0) atomicMax(&list_maxd2[r0], (unsigned long long)__double_as_longlong(m));
+    } else if (live && d2 > 0) {
+        atomicMax(&list_maxd2[r], (unsigned long long)__double_as_longlong(d2));
+    }
+}
+
+// Pair list mode of each replica for this step.
+__global__ void k_list_decide(int32_t nrep, const Rep* __restrict__ reps, int32_t* __restrict__ list_state,
+                              unsigned long long* __restrict__ list_maxd2, int32_t* __restrict__ list_mode,
+                              unsigned long long* __restrict__ list_builds) {
+    for (int32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrep; r += gridDim.x * blockDim.x) {
+        const int32_t st = list_state[r];
+        const double d2 = __longlong_as_double((long long)list_maxd2[r]);
+        int32_t m;
+        if (st == LIST_INVALID || !(d2 <= reps[r].skin_half2)) m = LIST_FRESH;
+        else m = st == LIST_STALE ? LIST_RESTORE : LIST_KEEP;
+        list_mode[r] = m;
+        if (m != LIST_KEEP) list_builds[r] += 1;
+        list_state[r] = LIST_VALID;
+        list_maxd2[r] = 0ull;
+    }
+}
+
+// Build positions of the replicas built afresh; main-grid keys of every
+// particle from its build position (so a kept replica's cells, order and
+// work items come out as they were built).
+__global__ void k_list_keys(int64_t n, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
+                            const int32_t* __restrict__ list_mode, const double* __restrict__ pos0,
+                            double* __restrict__ pos_build, int32_t* __restrict__ key_main) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const Rep& R = reps[rep_of[i]];
-    const double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
-    pos0[i] = x;
-    pos0[n + i] = y;
-    pos0[2 * n + i] = z;
-    pos_cur[i] = x;
-    pos_cur[n + i] = y;
-    pos_cur[2 * n + i] = z;
-    const double L = R.L;
-    const int nc = R.ncell, fn = R.fncell;
-    key_main[i] = R.cell_off + (cell_coord(z, L, R.cell, nc) * nc + cell_coord(y, L, R.cell, nc)) * nc +
-                  cell_coord(x, L, R.cell, nc);
-    key_fine[i] = R.fcell_off + (cell_coord(z, L, R.fcell, fn) * fn + cell_coord(y, L, R.fcell, fn)) * fn +
-                  cell_coord(x, L, R.fcell, fn);
+    const int32_t r = rep_of[i];
+    double x, y, z;
+    if (list_mode[r] == LIST_FRESH) {
+        x = pos0[i];
+        y = pos0[n + i];
+        z = pos0[2 * n + i];
+        pos_build[i] = x;
+        pos_build[n + i] = y;
+        pos_build[2 * n + i] = z;
+    } else {
+        x = pos_build[i];
+        y = pos_build[n + i];
+        z = pos_build[2 * n + i];
+    }
+    key_main[i] = main_key(reps[r], x, y, z);
 }
 
 // Counting sort by key; a negative key leaves the item out of the sort.
@@ -221,11 +296,27 @@ __global__ void k_place(int64_t n, const int32_t* __restrict__ key, int32_t* __r
 // pattern (the overlap correction later sets bit 32 of the moved ones), so
 // the pair kernels need no further gathers for either.
 __global__ void k_records(int64_t n, const int32_t* __restrict__ sorted_idx, const double* __restrict__ pos0,
-                          double4* __restrict__ rec, float4* __restrict__ rec32, int32_t* __restrict__ inv_perm) {
+                          double4* __restrict__ rec, float4* __restrict__ rec32, int32_t* __restrict__ inv_perm,
+                          const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
+                          const double* __restrict__ pos_build, const int32_t* __restrict__ list_mode,
+                          float4* __restrict__ rec_build32, int32_t* __restrict__ lcount) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int32_t i = sorted_idx[k];
-    const double x = pos0[i], y = pos0[n + i], z = pos0[2 * n + i];
+    double x = pos0[i], y = pos0[n + i], z = pos0[2 * n + i];
+    if (pos_build) {
+        const int32_t r = rep_of[i];
+        const double L = reps[r].L, iL = 1.0 / L;
+        const double bx = pos_build[i], by = pos_build[n + i], bz = pos_build[2 * n + i];
+        // the image of the step-start position nearest the build position
+        // (no shift, and the same bits, unless the particle crossed a face)
+        x = fma(rint((bx - x) * iL), L, x);
+        y = fma(rint((by - y) * iL), L, y);
+        z = fma(rint((bz - z) * iL), L, z);
+        rec_build32[k] = make_float4((float)bx, (float)by, (float)bz, 0.0f);
+        if (list_mode[r] != LIST_KEEP)
+            for (int w = 0; w < PAIR_SLICES; ++w) lcount[PAIR_SLICES * k + w] = 0;
+    }
     rec[k] = make_double4(x, y, z, __longlong_as_double((long long)(uint32_t)i));
     rec32[k] = make_float4((float)x, (float)y, (float)z, 0.0f);
     inv_perm[i] = (int32_t)k;
@@ -489,9 +580,12 @@ __global__ __launch_bounds__(256) void k_moments(int64_t n, const double* __rest
     if (i >= n) return;
     const double a = mag_mom[i];
     const double mx = mm[i] * a, my = mm[n + i] * a, mz = mm[2 * n + i] * a;
-    mom[i] = mx;
-    mom[n + i] = my;
-    mom[2 * n + i] = mz;
+    if (mom) {
+        mom[i] = mx;
+        mom[n + i] = my;
+        mom[2 * n + i] = mz;
+    }
+    if (!inv_perm) return;
     const int32_t k = inv_perm[i];
     rec_mom[k] = make_double4(mx, my, mz, 0.0);
     // FP32 copy in units of the replica's largest moment (mixed pair path)
@@ -944,10 +1038,9 @@ __device__ __forceinline__ void jrun_flush(IW& W, JRun<T>& run) {
 // issued first (one memory round trip); padding and inactive lanes read
 // entry 0's records, which exist, and are masked. Pairs with an endpoint
 // moved by the overlap correction are left to k_pairs_moved.
-template <bool FORCE, bool COATING, bool MIXED, typename RT>
+template <bool FORCE, bool COATING, bool MIXED, typename RT, typename WT>
 __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __restrict__ rec_mom, const PairParams& pp,
-                                           const IChunk<MIXED, COATING>& C,
-                                           IWave<FORCE, pair_qflush<FORCE, MIXED>()>& W, int32_t entry,
+                                           const IChunk<MIXED, COATING>& C, WT& W, int32_t entry,
                                            bool moved_any, int32_t jb, const WinImage& im, double press[3],
                                            float press32[3], int64_t& npairs, uint32_t s0, JRun<RT>& run) {
     constexpr int FB = FORCE ? 3 : 0;  // first field component of the accumulators
@@ -1363,6 +1456,367 @@ __global__ __launch_bounds__(256) void k_pairs_coat(Geometry g, const double4* _
                                                     PairOut out, const int4* __restrict__ stencil_all,
                                                     unsigned long long* __restrict__ cnt_pairs) {
     pairs_block<true, true, MIXED>(g, rec_mom, pp, out, stencil_all, cnt_pairs);
+}
+
+// ------------------------------------------------------------------ pair lists
+// A pair list replaces the cell search of every evaluation by one search per
+// build: the list of a replica holds, for each work item and wavefront, the
+// queue the pair kernel's search would produce over the build positions with
+// the list radius r_l = r_c + skin, as a stream of windows (two header words
+// and the window's queue entries). While no particle is farther than skin / 2
+// from its build position, every pair within r_c is in the list; the
+// evaluation keeps the exact r < r_c test on the current records, so the extra
+// pairs give exact zeros. The build runs the same search as the pair kernel
+// (work items, half stencil split over the wavefronts, window order), twice:
+// pass 0 counts each stream's words, pass 1 writes them at the offsets
+// k_list_scan computes per replica.
+constexpr int LIST_QFLUSH = 512;
+constexpr int LIST_QCAP = LIST_QFLUSH + WAVE;
+// batches per flush of the evaluation from the lists
+constexpr int LIST_NB = 6;
+
+__device__ __forceinline__ double image_shift(int code, double L) { return code == 1 ? L : (code == 2 ? -L : 0.0); }
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_list_pass(Geometry g, const int4* __restrict__ stencil_all,
+                                                   int32_t* __restrict__ lcount, const int32_t* __restrict__ loff,
+                                                   int32_t* __restrict__ list) {
+    __shared__ int32_t qs[PAIR_SLICES][LIST_QCAP];
+    const int wv = wave_id(), lane = lane_id();
+    const int32_t w = blockIdx.x;
+    if (w >= *g.nwork) return;  // uniform over the block
+    const int32_t item = g.work[w];
+    const int32_t cell = item >> 8, chunk = item & 255;
+    const int32_t rep = g.cell_rep[cell];
+    const int32_t mode = g.list_mode[rep];
+    if (mode != LIST_FRESH && mode != LIST_RESTORE) return;
+    const Rep& R = g.reps[rep];
+    const int4* stencil = stencil_all + R.sten_off;
+    const int32_t nhalf = R.nhalf;
+    const int32_t s0 = g.cell_start[cell] + chunk * WAVE;
+    const int32_t e0 = min(s0 + WAVE, g.cell_end[cell]);
+    const bool vi = s0 + lane < e0;
+    int32_t* q = qs[wv];
+    for (int p = lane; p < LIST_QCAP; p += WAVE) q[p] = PAIR_PAD;
+    wave_sync();
+    const float4 pb = vi ? g.rec_build32[s0 + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint64_t vmask = ballot(vi);
+    const double L = R.L;
+    const int nc = R.ncell;
+    const bool allpairs = nc == 1;
+    const int lc = cell - R.cell_off;
+    const int cx = lc % nc, cy = (lc / nc) % nc, cz = lc / (nc * nc);
+    // list radius with the pair kernel's FP32 slack
+    const float rcf = (float)(R.rlist * (1.0 + 1e-5));
+    const float rr2f = rcf * rcf;
+    const float Lf = (float)L;
+    const float pxf = pb.x, pyf = pb.y, pzf = pb.z;
+    const float big = 3.0e38f;
+    auto wmax = [](float v) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, wave_max(v))));
+    };
+    const float bx0 = -wmax(vi ? -pxf : -big), bx1 = wmax(vi ? pxf : -big);
+    const float by0 = -wmax(vi ? -pyf : -big), by1 = wmax(vi ? pyf : -big);
+    const float bz0 = -wmax(vi ? -pzf : -big), bz1 = wmax(vi ? pzf : -big);
+    const float rb2f = rr2f * 1.0001f;
+    const int64_t sidx = (int64_t)PAIR_SLICES * s0 + wv;
+    // absolute word index of the stream (FILL), or words so far (count)
+    int64_t cur = FILL ? R.list_base + loff[sidx] : 0;
+    const int64_t wend = FILL ? cur + lcount[sidx] : 0;
+    int64_t whdr = -1;  // the current window's header words, once it has entries
+    int qn = 0;
+    auto emit = [&]() {
+        wave_sync();
+        if (whdr < 0) {
+            whdr = cur;
+            cur += LIST_HDR;
+        }
+        if (FILL)
+            for (int p = lane; p < qn; p += WAVE)
+                if (cur + p < wend) list[cur + p] = q[p];
+        cur += qn;
+        wave_sync();
+        for (int p = lane; p < qn; p += WAVE) q[p] = PAIR_PAD;
+        wave_sync();
+        qn = 0;
+    };
+    for (int si = wv; si < nhalf; si += PAIR_SLICES) {
+        const bool self = si == 0;
+        const int4 o = stencil[si];
+        int jx = cx + o.x, jy = cy + o.y, jz = cz + o.z;
+        int ix = 0, iy = 0, iz = 0;  // image codes: 1 = +L, 2 = -L
+        if (jx < 0) { jx += nc; ix = 2; } else if (jx >= nc) { jx -= nc; ix = 1; }
+        if (jy < 0) { jy += nc; iy = 2; } else if (jy >= nc) { jy -= nc; iy = 1; }
+        if (jz < 0) { jz += nc; iz = 2; } else if (jz >= nc) { jz -= nc; iz = 1; }
+        const float sx = (float)image_shift(ix, L), sy = (float)image_shift(iy, L), sz = (float)image_shift(iz, L);
+        const int img = ix | (iy << 2) | (iz << 4);
+        const int c = R.cell_off + (jz * nc + jy) * nc + jx;
+        const int32_t ce = g.cell_end[c];
+        const int32_t cs = self ? s0 : g.cell_start[c];
+        for (int32_t jb = cs; jb < ce; jb += WAVE) {
+            const int32_t jl = jb + lane;
+            float qx = 0, qy = 0, qz = 0;
+            bool near = false;
+            if (jl < ce) {
+                const float4 pj = g.rec_build32[jl];
+                qx = pj.x + sx;
+                qy = pj.y + sy;
+                qz = pj.z + sz;
+                const float ex = fmaxf(fmaxf(bx0 - qx, qx - bx1), 0.0f);
+                const float ey = fmaxf(fmaxf(by0 - qy, qy - by1), 0.0f);
+                const float ez = fmaxf(fmaxf(bz0 - qz, qz - bz1), 0.0f);
+                near = allpairs || ex * ex + ey * ey + ez * ez < rb2f;
+            }
+            const int cnt = min(WAVE, ce - jb);
+            bool any = false;
+            for (uint64_t cm = ballot(near); cm; cm &= cm - 1) {
+                const int t = __builtin_ctzll(cm);
+                float dx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qx), t)) - pxf;
+                float dy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qy), t)) - pyf;
+                float dz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qz), t)) - pzf;
+                if (allpairs) {
+                    dx = dx > Lf / 2 ? dx - Lf : (dx < -Lf / 2 ? dx + Lf : dx);
+                    dy = dy > Lf / 2 ? dy - Lf : (dy < -Lf / 2 ? dy + Lf : dy);
+                    dz = dz > Lf / 2 ? dz - Lf : (dz < -Lf / 2 ? dz + Lf : dz);
+                }
+                const bool inside = dx * dx + dy * dy + dz * dz < rr2f;
+                uint64_t m = ballot(inside) & vmask;
+                bool hit = inside && vi;
+                if (self) {
+                    // in the cell itself only i < j
+                    const int32_t lo = jb + t - s0;
+                    m &= lo >= WAVE ? ~0ull : (lo <= 0 ? 0ull : (1ull << lo) - 1);
+                    hit = hit && lane < lo;
+                }
+                if (m) {
+                    any = true;
+                    if (hit) q[qn + mbcnt(m)] = ((jb + t) << 6) | lane;
+                    qn += (__popcll(m) + 3) & ~3;
+                    if (qn >= LIST_QFLUSH) emit();
+                }
+            }
+            if (any) {
+                if (qn > 0) emit();
+                if (FILL && lane == 0 && whdr + 1 < wend) {
+                    list[whdr] = jb;
+                    list[whdr + 1] = (int32_t)(((cur - whdr - LIST_HDR) << 13) | ((int64_t)cnt << 6) | img);
+                }
+                whdr = -1;
+            }
+        }
+    }
+    if (!FILL && lane == 0) lcount[sidx] = (int32_t)cur;
+}
+
+// Offsets of the streams of each replica being built (one block per
+// replica, over its sorted slots x wavefronts), its word count, and the
+// region check.
+__global__ __launch_bounds__(1024) void k_list_scan(const Rep* __restrict__ reps, int32_t* __restrict__ list_mode,
+                                                    int32_t* __restrict__ list_state,
+                                                    const int32_t* __restrict__ lcount, int32_t* __restrict__ loff,
+                                                    long long* __restrict__ list_used, int32_t* __restrict__ err) {
+    __shared__ long long part[1024];
+    const int32_t r = blockIdx.x;
+    const int32_t mode = list_mode[r];
+    if (mode != LIST_FRESH && mode != LIST_RESTORE) return;  // uniform over the block
+    const Rep& R = reps[r];
+    const int64_t b0 = PAIR_SLICES * R.off, m = PAIR_SLICES * R.n;
+    const int t = threadIdx.x;
+    const int64_t per = (m + 1023) / 1024;
+    const int64_t b = t * per < m ? t * per : m, e = b + per < m ? b + per : m;
+    long long sum = 0;
+    for (int64_t i = b; i < e; ++i) sum += lcount[b0 + i];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const long long v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long run = part[t] - sum;
+    for (int64_t i = b; i < e; ++i) {
+        loff[b0 + i] = (int32_t)(run < 0x7fffffff ? run : 0x7fffffff);
+        run += lcount[b0 + i];
+    }
+    if (t == 1023) {
+        const long long total = part[1023];
+        list_used[r] = total;
+        if (total > R.list_cap) {
+            list_mode[r] = LIST_BAD;
+            list_state[r] = LIST_INVALID;
+            atomicOr(err, 128);
+        }
+    }
+}
+
+// Real-space pass from the pair lists: the pair kernel's block (64 i
+// particles of a work item, four wavefronts, the i sums added in a fixed
+// order at the end) with each wavefront's candidate search replaced by its
+// stream. A window's entries are read in batches of up to the kernel's queue
+// length, in the pair kernel's strided order (batch b of a flush takes quads
+// b, b + nb, ...), and the window's j sums go to the fixed-point sums at its
+// end, as in the pair kernel.
+template <bool FORCE>
+struct LWave {
+    static constexpr int NA = FORCE ? 6 : 3;
+    double acc[NA][WAVE];
+    double jacc[NA][WAVE];
+};
+
+template <bool FORCE, bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 : (FORCE ? 4 : 5)))) void k_pairs_list(
+    Geometry g, const double4* __restrict__ rec_mom, PairParams pp, PairOut out,
+    unsigned long long* __restrict__ cnt_pairs) {
+    constexpr bool F32 = MIXED;
+    using LW = LWave<FORCE>;
+    __shared__ IChunk<MIXED, false> C;
+    __shared__ LW waves[PAIR_SLICES];
+    const int wv = wave_id(), lane = lane_id();
+    LW& W = waves[wv];
+    const int32_t w = blockIdx.x;
+    if (w >= *g.nwork) return;  // uniform over the block
+    const int32_t item = g.work[w];
+    const int32_t cell = item >> 8, chunk = item & 255;
+    const int32_t rep = g.cell_rep[cell];
+    if (g.list_mode[rep] == LIST_BAD) return;  // flagged: no list this step
+    const Rep& R = g.reps[rep];
+    g.L = R.L;
+    g.inv_L = 1.0 / R.L;
+    apply_rep(pp, R);
+    const int64_t n = g.n;
+    const int32_t s0 = g.cell_start[cell] + chunk * WAVE;
+    const int32_t e0 = min(s0 + WAVE, g.cell_end[cell]);
+    const int32_t k = s0 + lane;
+    const bool vi = k < e0;
+    if constexpr (!MIXED) {  // the mixed kernel reads the i records through L1
+        if (vi && wv == 0) {
+            C.ipos[lane] = g.rec_pos[k];
+            C.imom[lane] = rec_mom[k];
+        }
+    }
+    for (int d = 0; d < LW::NA; ++d) {
+        W.acc[d][lane] = 0.0;
+        W.jacc[d][lane] = 0.0;
+    }
+    __syncthreads();
+    const bool moved_any = *g.n_events > 0;
+    const bool allpairs = R.ncell == 1;
+    const double L = R.L;
+    double press[3] = {0, 0, 0};
+    float press32[3] = {0.0f, 0.0f, 0.0f};
+    int64_t npairs = 0;
+    bool ok = true;
+    using RT = typename std::conditional<MIXED, float, double>::type;
+    const int64_t sidx = (int64_t)PAIR_SLICES * s0 + wv;
+    const int32_t len = __builtin_amdgcn_readfirstlane(g.lcount[sidx]);
+    const int32_t* st = g.list + (R.list_base + __builtin_amdgcn_readfirstlane(g.loff[sidx]));
+    // entries of a flush (up to LIST_NB batches of 64, in the strided order)
+    // and its batch count; the entries of the next flush (the next window's
+    // first one included, its header read ahead) are loaded while this one
+    // is evaluated, so a batch waits only for its records
+    auto flush_shape = [&](int32_t Ew, int32_t fw, int& nq, int& nb) {
+        nq = min(LIST_NB * WAVE, Ew - fw) >> 2;
+        nb = (nq + WAVE / 4 - 1) / (WAVE / 4);
+    };
+    auto fetch = [&](int32_t pw, int32_t Ew, int32_t fw, int32_t (&dst)[LIST_NB]) {
+        int nq, nb;
+        flush_shape(Ew, fw, nq, nb);
+        const int g4 = (lane >> 2) * nb;
+        const int32_t base = pw + LIST_HDR + fw;
+#pragma unroll
+        for (int b = 0; b < LIST_NB; ++b) {
+            const int qb = g4 + b;
+            dst[b] = st[(b < nb && qb < nq) ? base + 4 * qb + (lane & 3) : base];
+        }
+    };
+    // a header that does not fit the stream ends it (never written so)
+    auto window_len = [&](int32_t pw, int32_t w) {
+        const int32_t Ew = (int32_t)((uint32_t)w >> 13);
+        return (Ew >= 4 && Ew <= len - pw - LIST_HDR) ? Ew : -1;
+    };
+    if (len >= LIST_HDR + 4) {
+        int32_t p = 0, f = 0;
+        int32_t jb = __builtin_amdgcn_readfirstlane(st[0]), w1 = __builtin_amdgcn_readfirstlane(st[1]);
+        int32_t E = window_len(0, w1);
+        int32_t pn = 0, jbn = 0, w1n = 0;
+        auto read_ahead = [&]() {
+            pn = p + LIST_HDR + E;
+            if (pn + LIST_HDR <= len) {
+                jbn = __builtin_amdgcn_readfirstlane(st[pn]);
+                w1n = __builtin_amdgcn_readfirstlane(st[pn + 1]);
+            }
+        };
+        int32_t cur[LIST_NB], nxt[LIST_NB];
+        if (E > 0) {
+            fetch(0, E, 0, cur);
+            read_ahead();
+        }
+        while (E > 0) {
+            // the next flush: the rest of this window, or the next window
+            const bool more = f + LIST_NB * WAVE < E;
+            const int32_t En = more ? E : (pn + LIST_HDR <= len ? window_len(pn, w1n) : -1);
+            // one unconditional fetch (the loads stay in flight through this
+            // flush); past the stream's end it reads this flush again
+            const bool nw = !more && En > 0;
+            fetch(nw ? pn : p, nw ? En : E, more ? f + LIST_NB * WAVE : (nw ? 0 : f), nxt);
+            const WinImage im{image_shift(w1 & 3, L), image_shift((w1 >> 2) & 3, L),
+                              image_shift((w1 >> 4) & 3, L), allpairs};
+            int nq, nb;
+            flush_shape(E, f, nq, nb);
+            const int g4 = (lane >> 2) * nb;
+            JRun<RT> run;
+#pragma unroll
+            for (int b = 0; b < LIST_NB; ++b)
+                if (b < nb)
+                    pair_dense<FORCE, false, MIXED>(g, rec_mom, pp, C, W, g4 + b < nq ? cur[b] : PAIR_PAD, moved_any,
+                                                    jb, im, press, press32, npairs, (uint32_t)s0, run);
+            jrun_flush(W, run);
+            if (!more) {
+                wave_sync();
+                // window done: its j sums go to the global sums
+                const int cnt = (w1 >> 6) & 127;
+                if (lane < cnt)
+                    for (int d = 0; d < LW::NA; ++d)
+                        ok &= fx_add_comp(out, R, n, FORCE ? d : d + 3, jb + lane, W.jacc[d][lane]);
+                for (int d = 0; d < LW::NA; ++d) W.jacc[d][lane] = 0.0;
+                if (F32 && FORCE)
+                    for (int d = 0; d < 3; ++d) {
+                        press[d] += (double)press32[d];
+                        press32[d] = 0.0f;
+                    }
+                wave_sync();
+            }
+#pragma unroll
+            for (int b = 0; b < LIST_NB; ++b) cur[b] = nxt[b];
+            if (more) {
+                f += LIST_NB * WAVE;
+            } else {
+                p = pn;
+                jb = jbn;
+                w1 = w1n;
+                E = En;
+                f = 0;
+                if (E > 0) read_ahead();
+            }
+        }
+    }
+    if (FORCE) {
+        const double p0 = wave_sum(press[0]), p1 = wave_sum(press[1]), p2 = wave_sum(press[2]);
+        if (lane < 3)
+            out.press_partial[3 * ((int64_t)w * PAIR_SLICES + wv) + lane] =
+                (lane == 0 ? p0 : lane == 1 ? p1 : p2) * pp.pscale;
+    }
+    if (cnt_pairs && lane == 0) atomicAdd(cnt_pairs, (unsigned long long)(2 * npairs));
+    // the wavefronts' i sums, in a fixed order
+    __syncthreads();
+    if (wv == 0 && vi)
+        for (int d = 0; d < LW::NA; ++d) {
+            const double x = (waves[0].acc[d][lane] + waves[1].acc[d][lane]) +
+                             (waves[2].acc[d][lane] + waves[3].acc[d][lane]);
+            ok &= fx_add_comp(out, R, n, FORCE ? d : d + 3, k, x);
+        }
+    if (!ok) atomicOr(out.err, 64);
 }
 
 // Cell-sorted fixed-point pair sums -> original order, in SI units.
@@ -2515,9 +2969,39 @@ hipError_t launch_scale(hipStream_t s, double* p, int64_t count, double f) {
 }
 
 hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* pos0, double* pos_cur, const Rep* reps,
-                          const int32_t* rep_of, int32_t* key_main, int32_t* key_fine) {
+                          const int32_t* rep_of, int32_t* key_main, int32_t* key_fine, const double* pos_build,
+                          const int32_t* list_state, unsigned long long* list_maxd2) {
     hipLaunchKernelGGL(k_prepare, dim3(grid_for(n, 256)), dim3(256), 0, s, n, pos, pos0, pos_cur, reps, rep_of,
-                       key_main, key_fine);
+                       key_main, key_fine, pos_build, list_state, list_maxd2);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_list_decide(hipStream_t s, int32_t nrep, const Rep* reps, int32_t* list_state,
+                              unsigned long long* list_maxd2, int32_t* list_mode, unsigned long long* list_builds) {
+    if (nrep <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_list_decide, dim3(grid_for(nrep, 256)), dim3(256), 0, s, nrep, reps, list_state, list_maxd2,
+                       list_mode, list_builds);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_list_keys(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of,
+                            const int32_t* list_mode, const double* pos0, double* pos_build, int32_t* key_main) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_list_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, list_mode, pos0,
+                       pos_build, key_main);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_list_build(hipStream_t s, const Geometry& g, int32_t nrep, int32_t max_work, const int4* stencil,
+                             int32_t* lcount, int32_t* loff, int32_t* list, int32_t* list_mode,
+                             int32_t* list_state, long long* list_used, int32_t* err) {
+    if (max_work <= 0 || nrep <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_list_pass<false>, dim3(max_work), dim3(WAVE * PAIR_SLICES), 0, s, g, stencil, lcount,
+                       (const int32_t*)loff, list);
+    hipLaunchKernelGGL(k_list_scan, dim3(nrep), dim3(1024), 0, s, g.reps, list_mode, list_state,
+                       (const int32_t*)lcount, loff, list_used, err);
+    hipLaunchKernelGGL(k_list_pass<true>, dim3(max_work), dim3(WAVE * PAIR_SLICES), 0, s, g, stencil, lcount,
+                       (const int32_t*)loff, list);
     LAUNCH_CHECK();
 }
 
@@ -2544,9 +3028,11 @@ hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_
 }
 
 hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_idx, const double* pos0,
-                                double4* rec_pos, float4* rec_pos32, int32_t* inv_perm) {
+                                double4* rec_pos, float4* rec_pos32, int32_t* inv_perm, const Rep* reps,
+                                const int32_t* rep_of, const double* pos_build, const int32_t* list_mode,
+                                float4* rec_build32, int32_t* lcount) {
     hipLaunchKernelGGL(k_records, dim3(grid_for(n, 256)), dim3(256), 0, s, n, sorted_idx, pos0, rec_pos, rec_pos32,
-                       inv_perm);
+                       inv_perm, reps, rep_of, pos_build, list_mode, rec_build32, lcount);
     LAUNCH_CHECK();
 }
 
@@ -2694,6 +3180,46 @@ hipError_t launch_pairs(hipStream_t s, bool force, int32_t precision, const Geom
     // pairs with an endpoint moved by the overlap correction (FP64)
     const int mblocks = 512;
     if (force && pp.coating)
+        hipLaunchKernelGGL((k_pairs_moved<true, true>), dim3(mblocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
+                           moved_press_off, cp);
+    else if (force)
+        hipLaunchKernelGGL((k_pairs_moved<true, false>), dim3(mblocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
+                           moved_press_off, cp);
+    else
+        hipLaunchKernelGGL((k_pairs_moved<false, false>), dim3(mblocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
+                           moved_press_off, cp);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_pairs_list(hipStream_t s, bool force, int32_t precision, const Geometry& g, const double4* rec_mom,
+                             const PairParams& pp, const PairOut& out, int32_t max_work, const int4* stencil,
+                             int32_t moved_press_off, int64_t* cnt_pairs) {
+    const int64_t n = g.n;
+    if (max_work <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(6 * n, 256)), dim3(256), 0, s, (double*)out.acc + (force ? 0 : 6 * n),
+                       (force ? 12 : 6) * n, 0.0);
+    unsigned long long* cp = (unsigned long long*)cnt_pairs;
+    const bool mixed = precision == 1;
+    const bool coat = force && pp.coating;
+#define KL(F, M) \
+    hipLaunchKernelGGL((k_pairs_list<F, M>), dim3(max_work), dim3(WAVE * PAIR_SLICES), 0, s, g, rec_mom, pp, out, cp)
+    if (coat) {
+        // the short-range terms keep the searching kernel (valid between
+        // builds: the grid covers the list radius)
+        if (mixed)
+            hipLaunchKernelGGL((k_pairs_coat<true>), dim3(max_work), dim3(WAVE * PAIR_SLICES), 0, s, g, rec_mom, pp,
+                               out, stencil, cp);
+        else
+            hipLaunchKernelGGL((k_pairs_coat<false>), dim3(max_work), dim3(WAVE * PAIR_SLICES), 0, s, g, rec_mom, pp,
+                               out, stencil, cp);
+    } else if (force) {
+        if (mixed) KL(true, true); else KL(true, false);
+    } else {
+        if (mixed) KL(false, true); else KL(false, false);
+    }
+#undef KL
+    const int mblocks = 512;
+    if (coat)
         hipLaunchKernelGGL((k_pairs_moved<true, true>), dim3(mblocks), dim3(64), 0, s, g, rec_mom, pp, out, stencil,
                            moved_press_off, cp);
     else if (force)

@@ -53,11 +53,13 @@ def rc_frac_for(n: int, scale: float) -> float:
     return min(0.5, max(0.1, scale * (1e5 / n) ** (1.0 / 6.0)))
 
 
-def footprint_bytes(n: int, nc: int) -> int:
+def footprint_bytes(n: int, nc: int, l_box: float = 0.0, r_cut: float = 0.0, skin: float = 0.005) -> int:
     """Device bytes a replica of n particles with k-space cut-off nc adds to
     a context (per-particle arrays, sin/cos tables, S(k) partials, cells),
-    with 25 % headroom."""
-    per_particle = 1100 + 48 * (nc + 1)
+    with 25 % headroom, plus its pair-list region as the planner sizes it
+    (given the box, the cut-off and the list skin)."""
+    from .capi import lib
+    per_particle = 1250 + 48 * (nc + 1)
     nk = int(2.2 * nc ** 3) + 1
     nseg = max(1, nk // 8)
     nsegw = (nseg + 63) // 64
@@ -65,7 +67,8 @@ def footprint_bytes(n: int, nc: int) -> int:
     nchunks = (n + chunk - 1) // chunk
     kspace = nk * 8 * (3 + 2 * nchunks)
     cells = 64 * n
-    return int(1.25 * (n * per_particle + kspace + cells))
+    words = lib().mds_pair_list_words(n, l_box, r_cut, skin) if (l_box > 0 and r_cut > 0 and skin > 0) else 0
+    return int(1.25 * (n * per_particle + kspace + cells)) + 4 * int(words)
 
 
 class Arbiter:
@@ -175,7 +178,8 @@ class Server:
                     ea=ea)
 
     def footprint(self, req: Request) -> int:
-        return footprint_bytes(req.n, self.params_of(req)["nc"])
+        p = self.params_of(req)
+        return footprint_bytes(req.n, p["nc"], p["l_box"], p["r_cut"])
 
     def admit(self, reqs: list[Request], now: float):
         if not reqs:
