@@ -225,8 +225,13 @@ mds_status mds_get_tiling(const mds_ctx* ctx, int32_t* cell_reach, int32_t* ncel
  * lists off (every evaluation searches the cells). Default 0.005. The result
  * of a replica depends only on its own trajectory (batches, resumes and
  * layout changes rebuild a list from the stored build positions), but it
- * differs from the search's in the order of the pair additions. */
-mds_status mds_set_pair_list(mds_ctx* ctx, double skin);
+ * differs from the search's in the order of the pair additions.
+ * region_scale (default 1) sizes each replica's list region as a multiple of
+ * the planner's estimate (mds_pair_list_words); a list that outgrows its
+ * region raises flag 128 and the checked step that sees it grows the region
+ * to the counted size and redoes the geometry (graph-replayed chunks are
+ * redone that way). */
+mds_status mds_set_pair_list(mds_ctx* ctx, double skin, double region_scale);
 /* Builds of replica `index`'s list so far (the first included), words of
  * its last build and words of its region. */
 mds_status mds_pair_list_stats(mds_ctx* ctx, int32_t index, int64_t* builds, int64_t* words, int64_t* capacity);

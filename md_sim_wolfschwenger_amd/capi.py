@@ -92,7 +92,7 @@ SIGNATURES = [
     ("mds_set_ewald", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32]),
     ("mds_set_tiling", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_get_tiling", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
-    ("mds_set_pair_list", C.c_int, [C.c_void_p, C.c_double]),
+    ("mds_set_pair_list", C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     ("mds_pair_list_words", C.c_int64, [C.c_int64, C.c_double, C.c_double, C.c_double]),
     ("mds_pair_list_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_int64)]),
@@ -337,9 +337,15 @@ class Context:
         self._chk(self._L.mds_get_tiling(self.h, *[C.byref(x) for x in v]), "get_tiling")
         return {"cell_reach": v[0].value, "ncell": v[1].value, "nstencil": v[2].value}
 
-    def set_pair_list(self, skin):
-        """Pair lists with skin r_c * skin (0: search the cells every evaluation)."""
-        self._chk(self._L.mds_set_pair_list(self.h, float(skin)), "set_pair_list")
+    def set_pair_list(self, skin, region_scale=1.0):
+        """Pair lists with skin r_c * skin (0: search the cells every evaluation);
+        region_scale sizes the list regions against the planner's estimate."""
+        self._chk(self._L.mds_set_pair_list(self.h, float(skin), float(region_scale)), "set_pair_list")
+
+    @staticmethod
+    def pair_list_words(n, l_box, r_cut, skin=0.005):
+        """Words of the list region the planner gives such a replica."""
+        return int(lib().mds_pair_list_words(int(n), float(l_box), float(r_cut), float(skin)))
 
     def pair_list_stats(self, index=0):
         v = [C.c_int64() for _ in range(3)]

@@ -1681,10 +1681,12 @@ mds_status mds_set_tiling(mds_ctx* c, int32_t cell_reach) {
     return MDS_OK;
 }
 
-mds_status mds_set_pair_list(mds_ctx* c, double skin) {
+mds_status mds_set_pair_list(mds_ctx* c, double skin, double region_scale) {
     if (!c || !(skin >= 0) || skin > 0.25) return fail(c, MDS_EINVAL, "pair-list skin must be in [0, 0.25] of r_c");
+    if (!(region_scale > 0) || region_scale > 64) return fail(c, MDS_EINVAL, "pair-list region scale must be in (0, 64]");
     HIPCHK(hipSetDevice(c->device));
     c->list_skin = skin;
+    for (RepMeta& m : c->meta) m.list_scale = region_scale;
     c->planned = false;
     c->geometry_ready = false;
     c->drop_graph();

@@ -164,3 +164,21 @@ def test_predicted_ewald_errors_match_measured_error_of_oracle():
     assert e_f == pytest.approx(pf, rel=0.3)
     assert e_t == pytest.approx(pt, rel=0.3)
     assert pf == pytest.approx(cfg.err_tol_ewald, rel=0.3)
+
+
+def test_pair_list_region_estimate():
+    """The planner's list region (also the serving admission estimate): the
+    pairs of a uniform suspension within r_l = r_c (1 + skin) (N(N-1)/2 times
+    the sphere's share of the box), a quarter extra for padding, 16 words per
+    particle and 4096 for headers; no region without a list."""
+    n, L, rc, skin = 100_000, 2.54e-6, 0.225 * 2.54e-6, 0.005
+    rl = rc * (1 + skin)
+    pairs = 0.5 * n * (n - 1) * 4.0 / 3.0 * np.pi * rl ** 3 / L ** 3
+    w = mds.Context.pair_list_words(n, L, rc, skin)
+    assert abs(w - (1.25 * pairs + 16 * n + 4096)) <= 1 + 1e-12 * w
+    # the measured C3 list (257.1 M words for 238.5 M unique pairs within
+    # r_c) fits the estimate with room to spare
+    assert 257_107_208 < w < 1.3 * 257_107_208
+    # a sphere larger than the box: every pair
+    assert mds.Context.pair_list_words(1000, 1.0, 0.7, skin) == int(1.25 * 0.5 * 1000 * 999 + 16 * 1000 + 4096)
+    assert mds.Context.pair_list_words(1000, 1.0, 0.3, 0.0) == 0
