@@ -100,10 +100,10 @@ def tune(ctx, info, args):
 # kernel class -> kernel symbol (short form) per path, for the committed
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summary
 SYMBOL = {
-    ("pairs_force", "fp64"): "k_pairs<true,false>", ("pairs_force", "mixed"): "k_pairs<true,true>",
-    ("pairs_field", "fp64"): "k_pairs<false,false>", ("pairs_field", "mixed"): "k_pairs<false,true>",
-    ("kbuild", "fp64"): "k_kbuild<8>", ("kbuild", "mixed"): "k_kbuild<8>",
-    ("kscatter_force", "fp64"): "k_kscatter<true>", ("kscatter_force", "mixed"): "k_kscatter<true>",
+    ("pairs_force", "fp64"): "k_pairs_list<true,false>", ("pairs_force", "mixed"): "k_pairs_list<true,true>",
+    ("pairs_field", "fp64"): "k_pairs_list<false,false>", ("pairs_field", "mixed"): "k_pairs_list<false,true>",
+    ("kbuild", "fp64"): "k_kbuild<8>", ("kbuild", "mixed"): "k_kbuild_mfma",
+    ("kscatter_force", "fp64"): "k_kscatter<true>", ("kscatter_force", "mixed"): "k_kscatter_mfma<true>",
     ("kscatter_field", "fp64"): "k_kscatter<false>", ("kscatter_field", "mixed"): "k_kscatter<false>",
 }
 PROFILE = os.path.join(ROOT, "profiles", "c3_kernels_mi355x.md")

@@ -93,6 +93,7 @@ SIGNATURES = [
     ("mds_set_tiling", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_get_tiling", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("mds_set_pair_list", C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    ("mds_set_kspace_waves", C.c_int, [C.c_void_p, C.c_int32]),
     ("mds_pair_list_words", C.c_int64, [C.c_int64, C.c_double, C.c_double, C.c_double]),
     ("mds_pair_list_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_int64)]),
@@ -336,6 +337,10 @@ class Context:
         v = [C.c_int32() for _ in range(3)]
         self._chk(self._L.mds_get_tiling(self.h, *[C.byref(x) for x in v]), "get_tiling")
         return {"cell_reach": v[0].value, "ncell": v[1].value, "nstencil": v[2].value}
+
+    def set_kspace_waves(self, waves):
+        """Waves per SIMD the S(k) build may hold beside the pair kernels (0: no limit)."""
+        self._chk(self._L.mds_set_kspace_waves(self.h, int(waves)), "set_kspace_waves")
 
     def set_pair_list(self, skin, region_scale=1.0):
         """Pair lists with skin r_c * skin (0: search the cells every evaluation);

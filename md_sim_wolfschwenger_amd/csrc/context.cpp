@@ -207,6 +207,8 @@ struct mds_ctx {
     std::vector<RepMeta> meta;
     std::vector<Rep> h_rep;
     int64_t n = 0;  // all particles
+    int32_t ncu = 256;          // compute units of the device
+    int32_t kspace_waves = 0;   // waves per SIMD the forked S(k) builds may hold (0: no limit)
     int32_t nrep = 0;
     bool state_set = false;
     bool planned = false;
@@ -1040,16 +1042,21 @@ mds_status prepare_geometry(mds_ctx* c, bool check) {
 mds_status kspace_chain(mds_ctx* c, bool force_pass, bool& forked) {
     const int64_t n = c->n;
     hipStream_t s = c->stream;
+    const bool forked_here = c->overlap_k && !c->timing;
     auto kspace = [&](hipStream_t ks) -> mds_status {
         if (c->nk_all > 0) {
             {
                 mds_ctx::Span sp(c, KC_KBUILD);
+                // beside the pair kernels the builds take at most
+                // kspace_waves waves per SIMD (a strided grid); in order, all
+                const int ncu = c->ncu;
+                const bool capped = forked_here && c->kspace_waves > 0;
                 if (c->any_mfma_build)
                     HIPCHK(launch_kbuild_mfma(ks, n, c->reps.p, c->nrep, c->kbm_items, c->nc_mfma_build, c->rows_m.p,
-                                              c->tab32.p, c->mom.p, c->kpartial.p));
+                                              c->tab32.p, c->mom.p, c->kpartial.p, capped ? c->kspace_waves * ncu : 0));
                 if (c->any_vec_build)
                     HIPCHK(launch_kbuild(ks, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p,
-                                         c->kpartial.p));
+                                         c->kpartial.p, capped ? 4 * c->kspace_waves * ncu : 0));
             }
             {
                 mds_ctx::Span sp(c, KC_KREDUCE);
@@ -1608,6 +1615,7 @@ mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out) {
     mds_ctx* c = new mds_ctx();
     c->cfg = *cfg;
     c->device = device;
+    c->ncu = std::max(1, mds_device_cus(device));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -2451,6 +2459,13 @@ int32_t mds_device_cus(int device) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
     return v;
+}
+
+mds_status mds_set_kspace_waves(mds_ctx* c, int32_t waves) {
+    if (!c || waves < 0 || waves > 16) return fail(c, MDS_EINVAL, "k-space waves per SIMD must be in [0, 16]");
+    c->kspace_waves = waves;
+    c->drop_graph();
+    return MDS_OK;
 }
 
 mds_status mds_set_kspace_overlap(mds_ctx* c, int32_t on) {

@@ -276,15 +276,17 @@ hipError_t launch_cell_order(hipStream_t s, int32_t ncells, const int32_t* start
 // the replicas with a matrix-core kernel
 hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
                                 double* tab, float* tab32);
+// max_blocks > 0: a grid of at most that many blocks strides over the items
+// (the build then holds fewer CU slots beside the pair kernels)
 hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t kb_items, const KSeg* segs,
-                         const double* tab, const double* mom, double* partial);
+                         const double* tab, const double* mom, double* partial, int32_t max_blocks);
 // The same S(k) partials (same chunks and layout) from FP32 matrix-core
 // products, for the mixed-precision path.
 // Every replica needs n_c <= KBM_NCMAX (the block's LDS).
 constexpr int KBM_NCMAX = 50;
 hipError_t launch_kbuild_mfma(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
                               int32_t nc_max, const KRow* rows, const float* tab32, const double* mom,
-                              double* partial);
+                              double* partial, int32_t max_blocks);
 // S(k) partials -> packed scatter table (fixed summation order)
 // The k-space scatter from FP32 matrix-core products (mixed-precision path,
 // n_c <= KSM_NCMAX in every replica); blocks: ceil(n_r / 64) per replica.

@@ -1120,20 +1120,18 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         // accumulator and the new run starts with this batch's sum (only
         // the leaders' runs are ever written)
         constexpr int NA = FORCE ? 6 : 3;
-        float qs[6], prev[6];
+        float qs[6];
         for (int d = 0; d < 3; ++d) qs[FB + d] = quad_sum(Bj[d]);
         if (FORCE)
             for (int d = 0; d < 3; ++d) qs[d] = quad_sum(F[d]);
-        for (int d = 0; d < NA; ++d) {
-            prev[d] = run.s[d];
-            run.s[d] += qs[d];
-        }
-        if (leader && js != run.js) {
-            if (run.js >= 0)
-                for (int d = 0; d < NA; ++d) atomicAdd(&W.jacc[d][run.js], (double)prev[d]);
-            run.js = js;
-            for (int d = 0; d < NA; ++d) run.s[d] = qs[d];
-        }
+        // a leader whose candidate changes writes its run and starts a new
+        // one with this batch's sum; the run is updated by selects, so the
+        // compiler keeps it in place (no copies of a branch's result)
+        const bool change = leader && js != run.js;
+        if (change && run.js >= 0)
+            for (int d = 0; d < NA; ++d) atomicAdd(&W.jacc[d][run.js], (double)run.s[d]);
+        for (int d = 0; d < NA; ++d) run.s[d] = change ? qs[d] : run.s[d] + qs[d];
+        run.js = change ? js : run.js;
     } else if constexpr (!COATING) {
         // FP64 without coating: u from i to j. The reference orients every
         // pair from the lower to the higher index (src/interaction.cpp:99-104);
@@ -1165,15 +1163,15 @@ __device__ __forceinline__ void pair_dense(const Geometry& g, const double4* __r
         for (int d = 0; d < 3; ++d) b[d] = quad_sum(valid ? Bj[d] : 0.0);
         if (FORCE)
             for (int d = 0; d < 3; ++d) f[d] = quad_sum(valid ? F[d] : 0.0);
-        if (leader) {
-            if (js != run.js) {
-                jrun_flush(W, run);
-                run.js = js;
-            }
-            for (int d = 0; d < 3; ++d) run.s[FB + d] += b[d];
-            if (FORCE)
-                for (int d = 0; d < 3; ++d) run.s[d] += f[d];
-        }
+        // the same run update by selects as the mixed path's
+        constexpr int NA = FORCE ? 6 : 3;
+        const bool change = leader && js != run.js;
+        if (change && run.js >= 0)
+            for (int d = 0; d < NA; ++d) atomicAdd(&W.jacc[d][run.js], run.s[d]);
+        for (int d = 0; d < 3; ++d) run.s[FB + d] = change ? b[d] : run.s[FB + d] + b[d];
+        if (FORCE)
+            for (int d = 0; d < 3; ++d) run.s[d] = change ? f[d] : run.s[d] + f[d];
+        run.js = change ? js : run.js;
     } else {
         // canonical lo -> hi orientation of CompInteractVecs (src/interaction.cpp:99-104):
         // the short-range terms take the lo particle's radius first
@@ -2056,11 +2054,10 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
     // work item = (replica, particle chunk, wavefront of row segments); the
     // segment waves of one chunk are adjacent blocks. (An XCD-contiguous
     // block order and segment classes of 8/4/2 values were both measured
-    // slower.)
-    const int32_t item = blockIdx.x;
-    if (item >= kb_items) return;
+    // slower.) A grid smaller than the items walks them in strides.
+  for (int32_t item = blockIdx.x; item < kb_items; item += gridDim.x) {
     const Rep& R = reps[find_rep<0>(reps, nrep, item)];
-    if (R.kmode & 1) return;  // this replica's S(k) is built on the matrix cores
+    if (R.kmode & 1) continue;  // this replica's S(k) is built on the matrix cores
     const int32_t li = item - R.kb_off;
     const int32_t segw = li % R.nsegw, ch = li / R.nsegw;
     const KSeg* segs = segs_all + R.seg_off;
@@ -2108,7 +2105,7 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
             xs = nxs;
         }
     }
-    if (!vr) return;
+    if (!vr) continue;
     double* out = partial_all + R.part_off + (int64_t)ch * nk * 2;
 #pragma unroll
     for (int q = 0; q < SEG; ++q) {
@@ -2123,6 +2120,7 @@ __global__ __launch_bounds__(64) void k_kbuild(int64_t n_all, const Rep* __restr
             out[2 * km + 1] = ms_[q];
         }
     }
+  }
 }
 
 // The same S(k) partials from FP32 matrix-core products (mixed-precision
@@ -2157,10 +2155,13 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
     float(*sx)[2 * KBM_NX] = (float(*)[2 * KBM_NX])kbm_smem;
     float(*sm)[4] = (float(*)[4])(kbm_smem + KBM_SUB * 2 * KBM_NX);
     float* syz = kbm_smem + KBM_SUB * (2 * KBM_NX + 4);
-    const int32_t item = blockIdx.x;
-    if (item >= items) return;  // uniform over the block
+    // a grid smaller than the items walks them in strides (the grid is a
+    // multiple of eight, so an item keeps its XCD); every skip below is
+    // uniform over the block, and the staging loop's first barrier keeps one
+    // item's LDS from the next
+  for (int32_t item = blockIdx.x; item < items; item += gridDim.x) {
     const Rep& R = reps[find_rep<3>(reps, nrep, item)];
-    if (!(R.kmode & 1)) return;  // uniform over the block: this replica is built by k_kbuild
+    if (!(R.kmode & 1)) continue;  // this replica is built by k_kbuild
     // XCD-aware order: blocks are dealt to the eight XCDs by index mod 8, so
     // all row blocks of chunk ch get indices = ch (mod 8) and the chunk's
     // tables are fetched into one XCD's L2 (chunks in groups of eight; the
@@ -2169,7 +2170,7 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
     const int32_t nblk = (R.nrowg + KBM_WAVES - 1) / KBM_WAVES, nb = nblk * R.kbm_pass;
     const int32_t grp = li / (8 * nb), rr = li % (8 * nb);
     const int32_t ch = 8 * grp + (rr & 7), b = rr >> 3;
-    if (ch >= R.nchunks) return;  // uniform over the block
+    if (ch >= R.nchunks) continue;
     const int32_t blk = b % nblk, pass = b / nblk;
     const int wv = wave_id(), lane = lane_id(), tid = threadIdx.x;
     const int kk = lane >> 4, col = lane & 15;
@@ -2248,7 +2249,7 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
             }
         }
     }
-    if (!vrow) return;
+    if (!vrow) continue;
     double* out = partial_all + R.part_off + (int64_t)ch * R.nk * 2;
     const bool half = row.ny == 0 && row.nz == 0;
 #pragma unroll
@@ -2271,6 +2272,7 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
                 out[2 * km + 1] = (cP - dP) - fa * (cQ - dQ);
             }
         }
+  }
 }
 
 // Sum the S(k) partials of both k vectors of a packed entry over the
@@ -3252,22 +3254,29 @@ hipError_t launch_sincos_tables(hipStream_t s, int64_t n, const Rep* reps, const
     LAUNCH_CHECK();
 }
 
+// Grid of a build kernel whose blocks stride over `items`: all of them, or
+// at most `cap` blocks (a multiple of eight, so item and block keep their XCD).
+static int build_grid(int32_t items, int32_t cap) {
+    if (cap <= 0 || items <= cap) return items;
+    return cap < 8 ? 8 : cap & ~7;
+}
+
 hipError_t launch_kbuild(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t kb_items, const KSeg* segs,
-                         const double* tab, const double* mom, double* partial) {
+                         const double* tab, const double* mom, double* partial, int32_t max_blocks) {
     if (kb_items <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_kbuild<KSEG>, dim3(kb_items), dim3(WAVE), 0, s, n, reps, nrep, kb_items, segs, tab, mom,
-                       partial);
+    hipLaunchKernelGGL(k_kbuild<KSEG>, dim3(build_grid(kb_items, max_blocks)), dim3(WAVE), 0, s, n, reps, nrep,
+                       kb_items, segs, tab, mom, partial);
     LAUNCH_CHECK();
 }
 
 hipError_t launch_kbuild_mfma(hipStream_t s, int64_t n, const Rep* reps, int32_t nrep, int32_t items,
                               int32_t nc_max, const KRow* rows, const float* tab32, const double* mom,
-                              double* partial) {
+                              double* partial, int32_t max_blocks) {
     if (items <= 0) return hipSuccess;
     if (nc_max > KBM_NCMAX) return hipErrorInvalidValue;
     const size_t lds = (size_t)KBM_SUB * (2 * KBM_NX + 4 + 2 * tw4_of(nc_max)) * sizeof(float);
-    hipLaunchKernelGGL(k_kbuild_mfma, dim3(items), dim3(256), lds, s, n, reps, nrep, items, rows, tab32, mom,
-                       partial);
+    hipLaunchKernelGGL(k_kbuild_mfma, dim3(build_grid(items, max_blocks)), dim3(256), lds, s, n, reps, nrep, items,
+                       rows, tab32, mom, partial);
     LAUNCH_CHECK();
 }
 
