@@ -40,9 +40,13 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
-    // capacity >= count; contents are not kept
+    // capacity >= count; contents are not kept. A buffer that grows takes
+    // half again of its old capacity at least: freeing device memory waits
+    // for the whole device (every other context's work included), so layout
+    // changes (admitted and retired requests) should rarely reallocate.
     hipError_t ensure(size_t count) {
         if (count <= n && p) return hipSuccess;
+        if (p) count = std::max(count, n + n / 2);
         release();
         if (count == 0) count = 1;
         hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
@@ -200,6 +204,10 @@ struct mds_ctx {
     // kernels (same CU mask as `stream`); fork/join events
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // third stream for the contact search and overlap correction, which run
+    // beside the cell sort and the pair-list kernels (same CU mask)
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
     bool overlap_k = true;
     std::string err;
 
@@ -283,7 +291,7 @@ struct mds_ctx {
     DevBuf<float4> pack32;   // FP32 tile-major scatter entries of the matrix-core replicas
     // reductions
     DevBuf<double> press_partial, err_partial, scal;  // scal[0] = error norm
-    DevBuf<int32_t> scan_tmp;
+    DevBuf<int32_t> scan_tmp, scan_tmp2;  // scan tiles of the main chain and of the contact chain
     std::vector<int2> h_chunks;  // reduction chunks (replica, first particle)
     DevBuf<int2> red_chunks;
     DevBuf<double> red_part;
@@ -372,6 +380,12 @@ struct mds_ctx {
         }
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (aux) {
+            (void)hipStreamSynchronize(aux);
+            (void)hipStreamDestroy(aux);
+        }
+        if (ev_fork2) (void)hipEventDestroy(ev_fork2);
+        if (ev_join2) (void)hipEventDestroy(ev_join2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -782,14 +796,12 @@ mds_status plan_all(mds_ctx* c) {
     HIPCHK(c->mz_out.ensure(c->nrep));
     HIPCHK(c->series.ensure((size_t)kSeriesCap * c->nrep));
     HIPCHK(c->scan_tmp.ensure(scan_tiles(std::max<int64_t>({cells, fcells, c->n})) + 1));
+    HIPCHK(c->scan_tmp2.ensure(scan_tiles(std::max<int64_t>(fcells, c->n)) + 1));
     HIPCHK(c->red_chunks.ensure(c->h_chunks.size()));
     HIPCHK(c->red_part.ensure(3 * c->h_chunks.size()));
     c->list_words = list_words;
     if (c->list_on()) {
-        // grown by half again when it grows, never shrunk: a layout change
-        // (serving admits and retires replicas all the time) then rarely
-        // frees device memory, which waits for the whole device
-        if ((size_t)list_words > c->list.n) HIPCHK(c->list.ensure(std::max<int64_t>(1, list_words + list_words / 2)));
+        HIPCHK(c->list.ensure(std::max<int64_t>(1, list_words)));
         HIPCHK(c->list_mode.ensure(c->nrep));
         HIPCHK(c->list_maxd2.ensure(c->nrep));
         HIPCHK(c->list_used.ensure(c->nrep));
@@ -948,6 +960,38 @@ mds_status prepare_cells(mds_ctx* c) {
     const int64_t n = c->n;
     hipStream_t s = c->stream;
     const bool list = c->list_on();
+    // the contact search and overlap correction read the step-start
+    // positions and fine-grid keys only, and write nothing the cell sort or
+    // the lists read: beside them on the third stream (in order when kernels
+    // are timed); k_mark_moved, which needs both, joins them
+    const bool split = c->overlap_k && !c->timing;
+    hipStream_t cs = split ? c->aux : s;
+    if (split) {
+        HIPCHK(hipEventRecord(c->ev_fork2, s));
+        HIPCHK(hipStreamWaitEvent(c->aux, c->ev_fork2, 0));
+    }
+    auto contacts = [&]() -> mds_status {
+        mds_ctx::Span sp(c, KC_CONTACTS);
+        HIPCHK(launch_cell_sort(cs, n, c->key_fine.p, c->cells_fine, c->cnt_fine.p, c->start_fine.p, c->end_fine.p,
+                                c->cur_fine.p, c->sorted_fine.p, c->scan_tmp2.p));
+        HIPCHK(launch_clear_events(cs, n, c->ev_head.p, c->ev_tail.p, c->n_events.p, c->n_moved.p, c->max_disp.p));
+        HIPCHK(launch_contacts(cs, n, c->reps.p, c->rep_of.p, c->pos0.p, c->r_mag.p, c->start_fine.p, c->end_fine.p,
+                               c->sorted_fine.p, c->frec.p, c->cand_count.p, c->cand_off.p, c->cand_j.p,
+                               c->cand_cap, c->err_flag.p, c->scan_tmp2.p));
+        // displacement bound: half the smallest contact-search margin
+        double dl = 1e300;
+        for (const Rep& R : c->h_rep) dl = std::min(dl, 0.5 * R.margin);
+        HIPCHK(launch_resolve(cs, c->nrep, n, c->reps.p, c->r_mag.p, c->cand_off.p, c->cand_j.p, c->pos_cur.p,
+                              c->pos0.p, c->ev_head.p, c->ev_tail.p, c->ev_next.p, c->ev_key.p, c->ev_pos.p,
+                              2 * c->cand_cap, c->n_events.p, c->max_disp.p, dl, c->moved_list.p, c->n_moved.p,
+                              c->err_flag.p));
+        return MDS_OK;
+    };
+    mds_status st;
+    if (split) {
+        if ((st = contacts()) != MDS_OK) return st;
+        HIPCHK(hipEventRecord(c->ev_join2, c->aux));
+    }
     {
         mds_ctx::Span sp(c, KC_SETUP);
         if (list) {
@@ -972,21 +1016,13 @@ mds_status prepare_cells(mds_ctx* c) {
                                  c->list.p, c->list_mode.p, c->list_state.p, (long long*)c->list_used.p,
                                  c->err_flag.p));
     }
+    if (split) {
+        HIPCHK(hipStreamWaitEvent(s, c->ev_join2, 0));
+    } else if ((st = contacts()) != MDS_OK) {
+        return st;
+    }
     {
         mds_ctx::Span sp(c, KC_CONTACTS);
-        HIPCHK(launch_cell_sort(s, n, c->key_fine.p, c->cells_fine, c->cnt_fine.p, c->start_fine.p, c->end_fine.p,
-                                c->cur_fine.p, c->sorted_fine.p, c->scan_tmp.p));
-        HIPCHK(launch_clear_events(s, n, c->ev_head.p, c->ev_tail.p, c->n_events.p, c->n_moved.p, c->max_disp.p));
-        HIPCHK(launch_contacts(s, n, c->reps.p, c->rep_of.p, c->pos0.p, c->r_mag.p, c->start_fine.p, c->end_fine.p,
-                               c->sorted_fine.p, c->frec.p, c->cand_count.p, c->cand_off.p, c->cand_j.p,
-                               c->cand_cap, c->err_flag.p, c->scan_tmp.p));
-        // displacement bound: half the smallest contact-search margin
-        double dl = 1e300;
-        for (const Rep& R : c->h_rep) dl = std::min(dl, 0.5 * R.margin);
-        HIPCHK(launch_resolve(s, c->nrep, n, c->reps.p, c->r_mag.p, c->cand_off.p, c->cand_j.p, c->pos_cur.p,
-                              c->pos0.p, c->ev_head.p, c->ev_tail.p, c->ev_next.p, c->ev_key.p, c->ev_pos.p,
-                              2 * c->cand_cap, c->n_events.p, c->max_disp.p, dl, c->moved_list.p, c->n_moved.p,
-                              c->err_flag.p));
         HIPCHK(launch_mark_moved(s, c->n_moved.p, c->moved_list.p, c->inv_perm.p, c->rec_pos.p));
     }
     c->geometry_ready = true;
@@ -1618,8 +1654,11 @@ mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out) {
     c->ncu = std::max(1, mds_device_cus(device));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return MDS_EDEVICE;
     }
@@ -2389,19 +2428,22 @@ mds_status mds_set_cu_mask(mds_ctx* c, const uint32_t* mask, int32_t words) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->drop_graph();
     HIPCHK(hipStreamSynchronize(c->side));
-    hipStream_t s[2] = {nullptr, nullptr};
-    for (int k = 0; k < 2; ++k) {
+    HIPCHK(hipStreamSynchronize(c->aux));
+    hipStream_t s[3] = {nullptr, nullptr, nullptr};
+    for (int k = 0; k < 3; ++k) {
         const hipError_t e = words > 0 ? hipExtStreamCreateWithCUMask(&s[k], (uint32_t)words, mask)
                                        : hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking);
         if (e != hipSuccess) {
-            if (k == 1) (void)hipStreamDestroy(s[0]);
+            for (int q = 0; q < k; ++q) (void)hipStreamDestroy(s[q]);
             return fail(c, MDS_EDEVICE, std::string("stream creation: ") + hipGetErrorString(e));
         }
     }
     (void)hipStreamDestroy(c->stream);
     (void)hipStreamDestroy(c->side);
+    (void)hipStreamDestroy(c->aux);
     c->stream = s[0];
     c->side = s[1];
+    c->aux = s[2];
     return MDS_OK;
 }
 
@@ -2771,7 +2813,7 @@ int64_t mds_device_bytes(const mds_ctx* c) {
     add(c->rows.n + c->rows_m.n, sizeof(KRow));
     add(c->segs.n, sizeof(KSeg));
     add(c->pack_desc.n, sizeof(KPack));
-    add(c->scan_tmp.n, sizeof(int32_t));
+    add(c->scan_tmp.n + c->scan_tmp2.n, sizeof(int32_t));
     for (const auto& b : c->spare) add(b.n, sizeof(double));
     add(c->red_chunks.n, sizeof(int2));
     add(c->red_part.n, sizeof(double));
