@@ -420,6 +420,8 @@ def main():
                     help="serving steps are log-uniform in [1e3, 1e5] times this (the configured stream's mean "
                          "request is 8e8 particle-steps, about a minute of the whole device: 100 of them take hours)")
     ap.add_argument("--serving-precisions", default="mixed", help="comma-separated: mixed, fp64")
+    ap.add_argument("--serving-loads", default="0.8,0.5",
+                    help="comma-separated offered loads of the serving leg, as fractions of its saturated rate")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0)
     ap.add_argument("--mz-run", type=int, default=0,
                     help="only run the C3 system for this many steps at r_c = 0.225 L, reach 3, and write its "
@@ -522,7 +524,8 @@ def main():
     if args.requests > 0:
         for prec in args.serving_precisions.split(","):
             log(f"serving: {args.requests} requests, {prec}")
-            serv[prec] = serving_leg(prec, args.requests, args.steps_scale, args.hbm_budget_gib)
+            serv[prec] = serving_leg(prec, args.requests, args.steps_scale, args.hbm_budget_gib,
+                                     loads=tuple(float(x) for x in args.serving_loads.split(",")))
             log(f"  {serv[prec]}")
     cpu = None
     if not args.no_cpu:
