@@ -232,11 +232,20 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, loads=(0.8, 0.5), see
         kw["cu_fractions"] = cu_fractions
     if cu_spans is not None:
         kw["cu_spans"] = cu_spans
+    last = [time.perf_counter()]
+
+    def progress(msg):
+        # at most one line per 20 s: a long leg still shows that it runs
+        now = time.perf_counter()
+        if now - last[0] >= 20.0:
+            last[0] = now
+            log(f"    {msg}")
+
     srv = serving.LaneServer(**kw)
-    sat = srv.serve(copy.deepcopy(base))
+    sat = srv.serve(copy.deepcopy(base), log=progress)
     srv.close()
     work = sum(q.n * q.steps for q in base)
-    iso = np.array(serving.isolated_latencies(copy.deepcopy(base), precision))
+    iso = np.array(serving.isolated_latencies(copy.deepcopy(base), precision, progress=progress))
     out = {"requests": n_req, "steps_range": [1e3 * steps_scale, 1e5 * steps_scale], "n_range": [1e3, 2e5],
            "hbm_budget_gib": budget_gib, "precision": precision,
            "saturated": {"throughput": sat["throughput"], "wall_s": sat["wall_s"], "p50_latency_s":
@@ -252,7 +261,7 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, loads=(0.8, 0.5), see
             t += arr.exponential(1.0 / rate)
             q.arrival = t
         srv = serving.LaneServer(**kw)
-        ld = srv.serve(reqs)
+        ld = srv.serve(reqs, log=progress)
         srv.close()
         lat = np.array([q.latency for q in reqs])
         ratio = lat / iso

@@ -205,8 +205,12 @@ struct mds_ctx {
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // third stream for the contact search and overlap correction, which run
-    // beside the cell sort and the pair-list kernels (same CU mask)
+    // beside the cell sort and the pair-list kernels (same CU mask). The two
+    // helper streams exist only while the overlap is on: a process gets a
+    // few hardware queues (four by default), and contexts side by side
+    // (serving lanes) keep theirs to one stream each.
     hipStream_t aux = nullptr;
+    std::vector<uint32_t> cu_mask;  // the context's CU mask (empty: all CUs)
     hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
     bool overlap_k = true;
     std::string err;
@@ -249,7 +253,7 @@ struct mds_ctx {
     DevBuf<double4> rec_pos, rec_mom;
     DevBuf<double4> frec;    // positions and radii in fine-cell order (contact search)
     DevBuf<uint64_t> pair_acc;  // [12n] cell-sorted fixed-point pair sums
-    DevBuf<int32_t> order_tmp;  // [n] cell-order scratch
+    DevBuf<int32_t> order_tmp;  // [n] main-grid cell lists in placement (atomic) order
     // EvalTransCoeffs observables (mds_observe): unwrapped and reference
     // positions, squared displacements, per-step series and last values
     DevBuf<double> true_pos, pos_ref, obs_d2, obs_series, obs_last;
@@ -1001,9 +1005,9 @@ mds_status prepare_cells(mds_ctx* c) {
                                     c->key_main.p));
         }
         HIPCHK(launch_cell_sort(s, n, c->key_main.p, c->cells_main, c->cnt_main.p, c->start_main.p, c->end_main.p,
-                                c->cur_main.p, c->sorted_main.p, c->scan_tmp.p));
-        HIPCHK(launch_cell_order(s, c->cells_main, c->start_main.p, c->end_main.p, c->sorted_main.p,
-                                 c->order_tmp.p));
+                                c->cur_main.p, c->order_tmp.p, c->scan_tmp.p));
+        HIPCHK(launch_cell_order(s, n, c->cells_main, c->key_main.p, c->start_main.p, c->end_main.p, c->order_tmp.p,
+                                 c->sorted_main.p));
         HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->rec_pos.p, c->rec_pos32.p, c->inv_perm.p,
                                     c->reps.p, c->rep_of.p, list ? c->pos_build.p : nullptr, c->list_mode.p,
                                     c->rec_build32.p, c->lcount.p));
@@ -2421,30 +2425,44 @@ mds_status mds_reserve(mds_ctx* c, int64_t n_particles) {
     return MDS_OK;
 }
 
-mds_status mds_set_cu_mask(mds_ctx* c, const uint32_t* mask, int32_t words) {
-    if (!c || words < 0 || (words > 0 && !mask)) return fail(c, MDS_EINVAL, "bad CU mask");
+}  // extern "C"
+
+namespace {
+// Streams of a context, on its CU mask: the main stream always, the two
+// helper streams (k-space chain, contact search) while `helpers` is set.
+// Existing streams are drained and replaced.
+mds_status make_streams(mds_ctx* c, bool helpers) {
     HIPCHK(hipSetDevice(c->device));
     c->collect();
-    HIPCHK(hipStreamSynchronize(c->stream));
     c->drop_graph();
-    HIPCHK(hipStreamSynchronize(c->side));
-    HIPCHK(hipStreamSynchronize(c->aux));
+    for (hipStream_t q : {c->stream, c->side, c->aux})
+        if (q) HIPCHK(hipStreamSynchronize(q));
+    const int nq = helpers ? 3 : 1;
     hipStream_t s[3] = {nullptr, nullptr, nullptr};
-    for (int k = 0; k < 3; ++k) {
-        const hipError_t e = words > 0 ? hipExtStreamCreateWithCUMask(&s[k], (uint32_t)words, mask)
-                                       : hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking);
+    for (int k = 0; k < nq; ++k) {
+        const hipError_t e = c->cu_mask.empty()
+                                 ? hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking)
+                                 : hipExtStreamCreateWithCUMask(&s[k], (uint32_t)c->cu_mask.size(), c->cu_mask.data());
         if (e != hipSuccess) {
             for (int q = 0; q < k; ++q) (void)hipStreamDestroy(s[q]);
             return fail(c, MDS_EDEVICE, std::string("stream creation: ") + hipGetErrorString(e));
         }
     }
-    (void)hipStreamDestroy(c->stream);
-    (void)hipStreamDestroy(c->side);
-    (void)hipStreamDestroy(c->aux);
+    for (hipStream_t q : {c->stream, c->side, c->aux})
+        if (q) (void)hipStreamDestroy(q);
     c->stream = s[0];
     c->side = s[1];
     c->aux = s[2];
     return MDS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+mds_status mds_set_cu_mask(mds_ctx* c, const uint32_t* mask, int32_t words) {
+    if (!c || words < 0 || (words > 0 && !mask)) return fail(c, MDS_EINVAL, "bad CU mask");
+    c->cu_mask.assign(mask, mask + words);
+    return make_streams(c, c->side != nullptr);
 }
 
 mds_status mds_peak_probe(int device, double* fp64_tflops, double* fp32_tflops, double* copy_gbs) {
@@ -2514,6 +2532,8 @@ mds_status mds_set_kspace_overlap(mds_ctx* c, int32_t on) {
     if (!c) return MDS_EINVAL;
     c->overlap_k = on != 0;
     c->drop_graph();
+    // the helper streams exist only while they are used
+    if (c->overlap_k != (c->side != nullptr)) return make_streams(c, c->overlap_k);
     return MDS_OK;
 }
 

@@ -266,10 +266,11 @@ hipError_t launch_observables(hipStream_t s, const Rep* reps, int32_t nrep, cons
                               const int32_t* work, const int32_t* nwork, const int32_t* cell_rep,
                               const double* press_partial, int64_t moved_press_off, double* obs,
                               const int32_t* counter, double* series, int32_t series_cap);
-// Within each cell of a sorted list, order the entries by value (original
-// index), so the pair sums' order does not depend on scheduling; tmp: [n].
-hipError_t launch_cell_order(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
-                             int32_t* sorted_idx, int32_t* tmp);
+// Within each cell, the entries k_place left in `placed` (atomic order)
+// ordered by value (original index) into sorted_idx, so the pair sums' order
+// does not depend on scheduling; key: each particle's cell (as sorted).
+hipError_t launch_cell_order(hipStream_t s, int64_t n, int32_t ncells, const int32_t* key, const int32_t* start,
+                             const int32_t* end, const int32_t* placed, int32_t* sorted_idx);
 
 // k space
 // FP64 tables for the replicas with a vector k-space kernel, FP32 ones for

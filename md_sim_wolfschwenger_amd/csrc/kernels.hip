@@ -1839,33 +1839,84 @@ __global__ void k_pairs_out(int64_t n, const int32_t* __restrict__ sorted_idx, c
         }
 }
 
-// Within each cell, order the particle list by original index (k_place
-// fills it in atomic order): one wavefront per cell, ranks by counting.
-__global__ __launch_bounds__(64) void k_cell_order(int32_t ncells, const int32_t* __restrict__ start,
-                                                   const int32_t* __restrict__ end, int32_t* __restrict__ sorted_idx,
-                                                   int32_t* __restrict__ tmp) {
-    const int lane = lane_id();
-    for (int32_t c = blockIdx.x; c < ncells; c += gridDim.x) {
-        const int32_t s = start[c], m = end[c] - s;
-        if (m <= 1) continue;
-        if (m <= WAVE) {
-            const int32_t v = lane < m ? sorted_idx[s + lane] : INT32_MAX;
+// Within each cell, the particle list ordered by original index (k_place
+// fills `placed` in atomic order; the ordered list goes to sorted_idx). Ranks
+// by counting. Blocks [0, cell_blocks) take the cells of at most one
+// wavefront of particles, one cell per wave, from registers. The other
+// blocks take 64-position chunks of `placed` that lie in larger cells (a
+// small box is one cell holding its whole replica): the block's ORDER_SPLIT
+// wavefronts each count against one share of the cell, and the shares' counts
+// are summed in LDS, so a cell of m particles costs m / (64 ORDER_SPLIT)
+// steps on the critical path instead of m^2 / 64.
+constexpr int ORDER_SPLIT = 4;
+
+__global__ __launch_bounds__(WAVE * ORDER_SPLIT) void k_cell_order(int64_t n, int32_t ncells, int32_t cell_blocks,
+                                                                   const int32_t* __restrict__ key,
+                                                                   const int32_t* __restrict__ start,
+                                                                   const int32_t* __restrict__ end,
+                                                                   const int32_t* __restrict__ placed,
+                                                                   int32_t* __restrict__ sorted_idx) {
+    const int lane = lane_id(), wv = threadIdx.x / WAVE;
+    if ((int32_t)blockIdx.x < cell_blocks) {
+        for (int32_t c = blockIdx.x * ORDER_SPLIT + wv; c < ncells; c += cell_blocks * ORDER_SPLIT) {
+            const int32_t s = start[c], m = end[c] - s;
+            if (m <= 0 || m > WAVE) continue;
+            const int32_t v = lane < m ? placed[s + lane] : INT32_MAX;
             int r = 0;
             for (int t = 0; t < m; ++t) r += __builtin_amdgcn_readlane(v, t) < v;
             if (lane < m) sorted_idx[s + r] = v;
-        } else {
-            for (int32_t e = lane; e < m; e += WAVE) {
-                const int32_t v = sorted_idx[s + e];
-                int32_t r = 0;
-                for (int32_t t = 0; t < m; ++t) r += sorted_idx[s + t] < v;
-                tmp[s + r] = v;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            for (int32_t e = lane; e < m; e += WAVE) sorted_idx[s + e] = tmp[s + e];
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __builtin_amdgcn_wave_barrier();
         }
+        return;
+    }
+    __shared__ int32_t part[ORDER_SPLIT][WAVE];
+    const int64_t chunks = (n + WAVE - 1) / WAVE;
+    for (int64_t ch = blockIdx.x - cell_blocks; ch < chunks; ch += gridDim.x - cell_blocks) {
+        // the cell of position p, from the particle placed there; positions
+        // past the last cell hold stale entries and match no cell's range
+        const int64_t p = ch * WAVE + lane;
+        int32_t v = INT32_MAX, s = 0, m = 0;
+        if (p < n) {
+            const int32_t x = placed[p];
+            if (x >= 0 && x < n) {
+                const int32_t k = key[x];
+                if (k >= 0 && k < ncells) {
+                    const int32_t s1 = start[k], m1 = end[k] - s1;
+                    if (p >= s1 && p < (int64_t)s1 + m1) {
+                        v = x;
+                        s = s1;
+                        m = m1;
+                    }
+                }
+            }
+        }
+        const bool big = m > WAVE;
+        uint64_t todo = ballot(big);
+        if (!todo) continue;  // the same for every wave of the block
+        int32_t r = 0;
+        while (todo) {
+            // the lanes of one large cell at a time
+            const int l = __builtin_ctzll(todo);
+            const int32_t sc = __builtin_amdgcn_readlane(s, l), mc = __builtin_amdgcn_readlane(m, l);
+            const bool mine = big && s == sc;
+            const int32_t share = (mc + ORDER_SPLIT * WAVE - 1) / (ORDER_SPLIT * WAVE) * WAVE;
+            const int32_t b0 = wv * share, b1 = min(mc, b0 + share);
+            int32_t rr = 0;
+            for (int32_t base = b0; base < b1; base += WAVE) {
+                const int32_t u = base + lane < b1 ? placed[sc + base + lane] : INT32_MAX;
+#pragma unroll 16
+                for (int t = 0; t < WAVE; ++t) rr += __builtin_amdgcn_readlane(u, t) < v;
+            }
+            if (mine) r = rr;
+            todo &= ~ballot(mine);
+        }
+        part[wv][lane] = r;
+        __syncthreads();
+        if (wv == 0 && big) {
+            int32_t tot = 0;
+            for (int q = 0; q < ORDER_SPLIT; ++q) tot += part[q][lane];
+            sorted_idx[s + tot] = v;
+        }
+        __syncthreads();
     }
 }
 
@@ -3240,11 +3291,13 @@ hipError_t launch_pairs_out(hipStream_t s, const Geometry& g, const PairOut& out
     LAUNCH_CHECK();
 }
 
-hipError_t launch_cell_order(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
-                             int32_t* sorted_idx, int32_t* tmp) {
-    if (ncells <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_order, dim3(ncells < 8192 ? ncells : 8192), dim3(WAVE), 0, s, ncells, start, end, sorted_idx,
-                       tmp);
+hipError_t launch_cell_order(hipStream_t s, int64_t n, int32_t ncells, const int32_t* key, const int32_t* start,
+                             const int32_t* end, const int32_t* placed, int32_t* sorted_idx) {
+    if (ncells <= 0 || n <= 0) return hipSuccess;
+    const int64_t cb = (ncells + ORDER_SPLIT - 1) / ORDER_SPLIT, pb = (n + WAVE - 1) / WAVE;
+    const int32_t cell_blocks = (int32_t)(cb < 4096 ? cb : 4096), chunk_blocks = (int32_t)(pb < 8192 ? pb : 8192);
+    hipLaunchKernelGGL(k_cell_order, dim3(cell_blocks + chunk_blocks), dim3(WAVE * ORDER_SPLIT), 0, s, n, ncells,
+                       cell_blocks, key, start, end, placed, sorted_idx);
     LAUNCH_CHECK();
 }
 

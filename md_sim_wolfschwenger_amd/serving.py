@@ -394,21 +394,24 @@ class LaneServer:
         return res
 
 
-def isolated_latencies(requests: list[Request], precision: str = "fp64", **kw) -> list[float]:
+def isolated_latencies(requests: list[Request], precision: str = "fp64", progress=None, **kw) -> list[float]:
     """Latency of each request served alone: arriving at an idle server
     (context already created) and leaving when its state is copied out, so
     admission (initial suspension, planning, graph capture) and retirement
     count as they do in the loop. The requests are solved one after another
-    on copies; the originals are not modified."""
+    on copies; the originals are not modified. progress(msg), if given, is
+    called after each request."""
     import copy
     s = Server(precision=precision, **kw)
     out = []
     try:
-        for q in requests:
+        for k, q in enumerate(requests):
             a = copy.deepcopy(q)
             a.arrival, a.state, a.ewald = 0.0, None, {}
             s.serve([a])
             out.append(a.latency)
+            if progress:
+                progress(f"isolated {k + 1}/{len(requests)}: N={q.n} steps={q.steps} {a.latency:.3f} s")
     finally:
         s.close()
     return out

@@ -1,0 +1,100 @@
+"""Cells that hold more particles than a wavefront. A small box at
+r_c = 0.45 L is one cell holding every particle (what the planner picks for
+a system of a few thousand particles, as the serving loop makes them), and a
+cell reach of 1 at r_c = 0.3 L gives 27 cells of a few hundred particles.
+Their lists are ordered by particle index by the order kernel's large-cell
+path (ranks counted by several wavefronts per chunk of the list). Checks:
+the one-cell box equals the CPU oracle and repeats bit for bit in a fresh
+context; the large cells give the sums of the default grid to rounding, in
+FP64 and mixed precision, over graph-replayed steps too."""
+import os
+
+import numpy as np
+import pytest
+
+import md_sim_wolfschwenger_amd as mds
+from md_sim_wolfschwenger_amd import workload
+from refcfg import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def floors(cfg, r, rh):
+    mu0 = 4e-7 * cfg.my_pi
+    m = float(cfg.sat_mag * 4.0 / 3.0 * cfg.my_pi * r.min() ** 3)
+    d = 2 * float(rh.min())
+    return 3 * mu0 * m * m / (2 * cfg.my_pi * d ** 4), mu0 * m / (2 * cfg.my_pi * d ** 3)
+
+
+def build(n, seed, rc_frac):
+    cfg, r, rh, L, pos, mm, ea = workload.system("c2", seed=seed, n=n)
+    mu = cfg.sat_mag * 4.0 / 3.0 * cfg.my_pi * r ** 3
+    alpha, rc, nc = workload.ewald_for(cfg, mu, rh, L, rc_frac)
+    return dict(n=n, seed=seed, cfg=cfg, r=r, rh=rh, L=L, pos=pos, mm=mm, ea=ea, mu=mu, alpha=alpha, rc=rc, nc=nc)
+
+
+def context(s, precision, reach=0):
+    cfg = mds.config_from_string(workload.config_text("c2", seed=s["seed"], precision=precision, n=s["n"]))
+    ctx = mds.Context(cfg)
+    ctx.set_particles(s["r"], s["rh"], s["L"])
+    ctx.set_ewald(s["alpha"], s["rc"], s["nc"])
+    ctx.set_tiling(reach)
+    ctx.set_state(s["pos"], s["mm"], s["ea"])
+    ctx.set_noise(None, None, None)
+    return ctx
+
+
+def test_one_cell_box_equals_the_oracle_and_repeats():
+    import oracle
+    from refcfg import derived, parse_ini
+    s = build(1500, 8, 0.45)
+    print(f"one-cell box: N = {s['n']}, r_c = 0.45 L, n_c = {s['nc']}", flush=True)
+    ctx = context(s, "fp64")
+    F, B, P, X = ctx.comp_interactions()
+    assert ctx.tiling()["ncell"] == 1
+    ctx.close()
+    again = context(s, "fp64")
+    F2, B2, P2, X2 = again.comp_interactions()
+    again.close()
+    assert np.array_equal(F, F2) and np.array_equal(B, B2) and np.array_equal(P, P2)
+    cfg, r, rh = s["cfg"], s["r"], s["rh"]
+    c = derived(parse_ini(workload.config_text("c2", seed=s["seed"], n=s["n"])))
+    props = {"r_mag": r, "r_hydr": rh, "vol_mag": 4 / 3 * cfg.my_pi * r ** 3, "mag_mom": s["mu"],
+             "zeta_rot": 8 * cfg.my_pi * cfg.vis * rh ** 3, "zeta_trans": 6 * cfg.my_pi * cfg.vis * rh}
+    o = oracle.System(len(r), s["L"], s["alpha"], s["rc"], 2 * r.max(), s["nc"], props, c,
+                      threads=max(1, min(16, os.cpu_count() or 1)))
+    Fo, Bo, Po, Xo = o.comp_interactions(s["pos"], s["mm"])
+    f_fl, b_fl = floors(cfg, r, rh)
+    assert rel_err(X, Xo) < 1e-15
+    assert rel_err(F, Fo, f_fl) < 1e-10
+    assert rel_err(B, Bo, b_fl) < 1e-10
+    assert rel_err(P, Po, f_fl * 2 * float(rh.min())) < 1e-9
+    mx = context(s, "mixed")
+    Fm, Bm, _, _ = mx.comp_interactions()
+    mx.close()
+    assert rel_err(Fm, F, f_fl) < 1e-5 and rel_err(Bm, B, b_fl) < 1e-5
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
+def test_large_cells_give_the_default_grid_sums(precision, tol):
+    s = build(10000, 3, 0.3)
+    print(f"{precision}: N = 1e4 at r_c = 0.3 L, reach 1 (27 cells) against the default grid", flush=True)
+    big = context(s, precision, reach=1)
+    ref = context(s, precision)
+    Fa, Ba, Pa, _ = big.comp_interactions()
+    assert big.tiling()["ncell"] == 3 and ref.tiling()["ncell"] > 3
+    Fb, Bb, Pb, _ = ref.comp_interactions()
+    f_fl, b_fl = floors(s["cfg"], s["r"], s["rh"])
+    assert rel_err(Fa, Fb, f_fl) < tol and rel_err(Ba, Bb, b_fl) < tol
+    # graph-replayed Philox steps on both grids
+    for ctx in (big, ref):
+        ctx.set_state(s["pos"], s["mm"], s["ea"])
+        ctx.use_philox(5, 0)
+        ctx.set_graph(True)
+    _, _, mza = big.run(12, 0.0, 1e-12, want_mz=True)
+    _, _, mzb = ref.run(12, 0.0, 1e-12, want_mz=True)
+    for x, y in zip(big.get_state(), ref.get_state()):
+        assert rel_err(x, y) < 1e3 * tol
+    assert np.allclose(mza, mzb, rtol=1e3 * tol, atol=1e-14)
+    big.close()
+    ref.close()
