@@ -688,7 +688,9 @@ __global__ void k_mean_z_final(int64_t nch, int64_t n, const double* __restrict_
 // moments in LDS, the mixed kernel reads them through L1) and each wavefront one interleaved slice of the half stencil: the cell
 // itself (pairs j > i in cell-sorted order) and the neighbour offsets that
 // are positive in (z, y, x) order, so every pair within r_c is met exactly
-// once. Each candidate j is tested by all lanes (a cheap FP32 distance test);
+// once (in a box that is one cell, the wavefronts share that cell's windows,
+// taken as a ring: SliceWindows). Each candidate j is tested by all lanes (a
+// cheap FP32 distance test);
 // lanes that hit push (lane, j) into the wavefront's LDS queue, and queued
 // pairs are evaluated densely, one pair per lane, so the expensive erfc/exp
 // path never runs with idle lanes. The queue is read with a stride of the
@@ -738,6 +740,46 @@ struct WinImage {
     double x, y, z;
     bool wrap;
 };
+
+// One cell per box side (every pair is within one cell): the cell's 64-
+// particle chunks form a ring, and chunk k takes the windows at ring
+// distance d = 0 .. D from itself, D = nch / 2 (for an even nch, distance
+// nch / 2 only from the lower half), so every chunk pair is met once and the
+// work items of the cell hold about the same number of windows (against
+// nch - k from chunk k on); the wavefronts of the block take them in turn.
+// Other grids: each wavefront takes every PAIR_SLICES-th stencil cell, with
+// all of its windows. Windows of slice si for wavefront wv: count, and the
+// start of window wi.
+struct SliceWindows {
+    int32_t first, step, count, base, nch, chunk;
+    __device__ int32_t start(int32_t wi) const {
+        if (nch == 0) return base + wi * WAVE;
+        int32_t kk = chunk + wi;
+        if (kk >= nch) kk -= nch;
+        return base + kk * WAVE;
+    }
+};
+__device__ __forceinline__ SliceWindows slice_windows(bool allpairs, int wv, int32_t cs, int32_t ce, int32_t cell_s,
+                                                      int32_t chunk) {
+    SliceWindows sw;
+    if (allpairs) {
+        const int32_t nch = (ce - cell_s + WAVE - 1) / WAVE;
+        sw.first = wv;
+        sw.step = PAIR_SLICES;
+        sw.count = nch / 2 + 1 - ((nch % 2 == 0 && chunk >= nch / 2) ? 1 : 0);
+        sw.base = cell_s;
+        sw.nch = nch;
+        sw.chunk = chunk;
+    } else {
+        sw.first = 0;
+        sw.step = 1;
+        sw.count = (ce - cs + WAVE - 1) / WAVE;
+        sw.base = cs;
+        sw.nch = 0;
+        sw.chunk = 0;
+    }
+    return sw;
+}
 
 // Positions of lo and hi as the reference's sequential scan sees them when it
 // reaches pair (lo, hi): the last overlap move of each with a smaller
@@ -1324,7 +1366,9 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
     int64_t npairs = 0;
     int qn = 0;
     bool ok = true;
-    for (int si = wv; si < nhalf; si += PAIR_SLICES) {
+    // with one cell per box side every wavefront takes the cell itself (the
+    // only slice) and a share of its windows
+    for (int si = allpairs ? 0 : wv; si < nhalf; si += PAIR_SLICES) {
         const bool self = si == 0;
         const int4 o = stencil[si];
         int jx = cx + o.x, jy = cy + o.y, jz = cz + o.z;
@@ -1338,7 +1382,12 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
         const int32_t ce = g.cell_end[c];
         // in the cell itself only j > i: the blocks from this chunk on
         const int32_t cs = self ? s0 : g.cell_start[c];
-        for (int32_t jb = cs; jb < ce; jb += WAVE) {
+        const SliceWindows sw = slice_windows(allpairs, wv, cs, ce, g.cell_start[c], chunk);
+        for (int32_t wi = sw.first; wi < sw.count; wi += sw.step) {
+            const int32_t jb = sw.start(wi);
+            // the chunk's own window takes i < j only (later windows of the
+            // cell take every lane, earlier ones are not met)
+            const bool selfw = self && jb == s0;
             const int32_t jl = jb + lane;
             // j's image nearest this cell (exact: every j within r_c of an i
             // of this cell lies in exactly one stencil cell), in FP32 for the
@@ -1397,7 +1446,7 @@ __device__ __forceinline__ void pairs_block(Geometry g, const double4* __restric
                     }
                 }
             };
-            if (self)
+            if (selfw)
                 scan(std::true_type{});
             else
                 scan(std::false_type{});
@@ -1538,7 +1587,7 @@ __global__ __launch_bounds__(256) void k_list_pass(Geometry g, const int4* __res
         wave_sync();
         qn = 0;
     };
-    for (int si = wv; si < nhalf; si += PAIR_SLICES) {
+    for (int si = allpairs ? 0 : wv; si < nhalf; si += PAIR_SLICES) {
         const bool self = si == 0;
         const int4 o = stencil[si];
         int jx = cx + o.x, jy = cy + o.y, jz = cz + o.z;
@@ -1551,7 +1600,10 @@ __global__ __launch_bounds__(256) void k_list_pass(Geometry g, const int4* __res
         const int c = R.cell_off + (jz * nc + jy) * nc + jx;
         const int32_t ce = g.cell_end[c];
         const int32_t cs = self ? s0 : g.cell_start[c];
-        for (int32_t jb = cs; jb < ce; jb += WAVE) {
+        const SliceWindows sw = slice_windows(allpairs, wv, cs, ce, g.cell_start[c], chunk);
+        for (int32_t wi = sw.first; wi < sw.count; wi += sw.step) {
+            const int32_t jb = sw.start(wi);
+            const bool selfw = self && jb == s0;
             const int32_t jl = jb + lane;
             float qx = 0, qy = 0, qz = 0;
             bool near = false;
@@ -1580,8 +1632,8 @@ __global__ __launch_bounds__(256) void k_list_pass(Geometry g, const int4* __res
                 const bool inside = dx * dx + dy * dy + dz * dz < rr2f;
                 uint64_t m = ballot(inside) & vmask;
                 bool hit = inside && vi;
-                if (self) {
-                    // in the cell itself only i < j
+                if (selfw) {
+                    // in the chunk's own window only i < j
                     const int32_t lo = jb + t - s0;
                     m &= lo >= WAVE ? ~0ull : (lo <= 0 ? 0ull : (1ull << lo) - 1);
                     hit = hit && lane < lo;
