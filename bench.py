@@ -272,7 +272,8 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, loads=(0.8, 0.5), see
             "latency_over_isolated_p50": float(np.percentile(ratio, 50)),
             "latency_over_isolated_p99": float(np.percentile(ratio, 99)),
             "lanes": {"bounds_n": list(srv.bounds), "requests": [l["requests"] for l in ld["lanes"]],
-                      "cu_ranges": srv.cu_ranges, "seconds": [l["seconds"] for l in ld["lanes"]]},
+                      "cu_ranges": srv.cu_ranges, "seconds": [l["seconds"] for l in ld["lanes"]],
+                      "stats": [l["stats"] for l in ld["lanes"]]},
             "worst": [{"n": q.n, "steps": q.steps, "lane": srv.lane_of(q), "latency_s": float(q.latency),
                        "wait_s": float(q.admitted - q.arrival), "isolated_s": float(i)}
                       for q, i in sorted(zip(reqs, iso), key=lambda x: -x[0].latency / x[1])[:3]]})

@@ -274,6 +274,7 @@ struct mds_ctx {
     DevBuf<int32_t> key_main, key_fine, cnt_main, start_main, end_main, cur_main, sorted_main, inv_perm, work, nwork;
     DevBuf<int32_t> cnt_fine, start_fine, end_fine, cur_fine, sorted_fine;
     DevBuf<int32_t> cand_count, cand_off, cand_j, ev_head, ev_tail, ev_next, n_events, err_flag, moved_list, n_moved;
+    DevBuf<int32_t> cand_base, cand_tmp;  // contact runs: [n + 1] bases and the append counter, [cand_cap] runs
     DevBuf<uint64_t> ev_key;
     DevBuf<double> ev_pos, max_disp;
     // k space
@@ -777,6 +778,7 @@ mds_status plan_all(mds_ctx* c) {
     HIPCHK(c->end_fine.ensure(fcells));
     HIPCHK(c->cur_fine.ensure(fcells));
     HIPCHK(c->cand_j.ensure(2 * (size_t)c->cand_cap));
+    HIPCHK(c->cand_tmp.ensure((size_t)c->cand_cap));
     HIPCHK(c->ev_next.ensure(2 * (size_t)c->cand_cap));
     HIPCHK(c->ev_key.ensure(2 * (size_t)c->cand_cap));
     HIPCHK(c->ev_pos.ensure(6 * (size_t)c->cand_cap));
@@ -868,6 +870,7 @@ mds_status size_scratch(mds_ctx* c) {
                     &c->ev_head, &c->ev_tail})
         HIPCHK(b->ensure(n));
     HIPCHK(c->cand_off.ensure(n + 1));
+    HIPCHK(c->cand_base.ensure(n + 1));
     HIPCHK(c->err_partial.ensure(blocks_for(n)));
     if (c->list_on()) {
         HIPCHK(c->rec_build32.ensure(n));
@@ -937,51 +940,57 @@ PairParams pair_params(mds_ctx* c) {
 }
 
 // Step-start positions, cell keys and the replicas' largest displacements
-// since their list builds, then the sin/cos tables of EvalSinCos: everything
-// the k-space chain reads of the geometry.
-mds_status prepare_positions(mds_ctx* c) {
-    const int64_t n = c->n;
-    hipStream_t s = c->stream;
+// since their list builds.
+mds_status prepare_keys(mds_ctx* c) {
     const bool list = c->list_on();
-    {
-        mds_ctx::Span sp(c, KC_SETUP);
-        // with a pair list the main grid sorts the build positions: a kept
-        // replica's cells, order and work items come out as built
-        HIPCHK(launch_prepare(s, n, c->pos.p, c->pos0.p, c->pos_cur.p, c->reps.p, c->rep_of.p, c->key_main.p,
-                              c->key_fine.p, list ? c->pos_build.p : nullptr, c->list_state.p,
-                              (unsigned long long*)c->list_maxd2.p));
-    }
-    {
-        mds_ctx::Span sp(c, KC_SINCOS);
-        HIPCHK(launch_sincos_tables(s, n, c->reps.p, c->rep_of.p, c->pos0.p, c->tab.p, c->tab32.p));
-    }
+    mds_ctx::Span sp(c, KC_SETUP);
+    // with a pair list the main grid sorts the build positions: a kept
+    // replica's cells, order and work items come out as built
+    HIPCHK(launch_prepare(c->stream, c->n, c->pos.p, c->pos0.p, c->pos_cur.p, c->reps.p, c->rep_of.p, c->key_main.p,
+                          c->key_fine.p, list ? c->pos_build.p : nullptr, c->list_state.p,
+                          (unsigned long long*)c->list_maxd2.p));
     return MDS_OK;
 }
 
-// The pair geometry of CompInteractVecs after prepare_positions: cell lists
-// (or the pair lists), contact candidates, sequential overlap correction.
-mds_status prepare_cells(mds_ctx* c) {
+// The sin/cos tables of EvalSinCos: with the keys, everything the k-space
+// chain reads of the geometry.
+mds_status prepare_sincos(mds_ctx* c) {
+    mds_ctx::Span sp(c, KC_SINCOS);
+    HIPCHK(launch_sincos_tables(c->stream, c->n, c->reps.p, c->rep_of.p, c->pos0.p, c->tab.p, c->tab32.p));
+    return MDS_OK;
+}
+
+mds_status prepare_positions(mds_ctx* c) {
+    mds_status st = prepare_keys(c);
+    if (st != MDS_OK) return st;
+    return prepare_sincos(c);
+}
+
+// Whether the contact search runs on the third stream (not when kernels are
+// timed, which keeps them in order on one stream).
+bool contacts_split(const mds_ctx* c) { return c->overlap_k && !c->timing; }
+
+// Contact candidates and the sequential overlap correction of
+// CompInteractVecs. They read the step-start positions and fine-grid keys
+// only, and write nothing that the sin/cos tables, the k-space chain, the
+// cell sort or the pair lists read: with `split` they run on the third
+// stream from the point of this call (after prepare_keys), and k_mark_moved,
+// which needs them, joins them.
+mds_status contact_chain(mds_ctx* c, bool split) {
     const int64_t n = c->n;
-    hipStream_t s = c->stream;
-    const bool list = c->list_on();
-    // the contact search and overlap correction read the step-start
-    // positions and fine-grid keys only, and write nothing the cell sort or
-    // the lists read: beside them on the third stream (in order when kernels
-    // are timed); k_mark_moved, which needs both, joins them
-    const bool split = c->overlap_k && !c->timing;
-    hipStream_t cs = split ? c->aux : s;
+    hipStream_t cs = split ? c->aux : c->stream;
     if (split) {
-        HIPCHK(hipEventRecord(c->ev_fork2, s));
+        HIPCHK(hipEventRecord(c->ev_fork2, c->stream));
         HIPCHK(hipStreamWaitEvent(c->aux, c->ev_fork2, 0));
     }
-    auto contacts = [&]() -> mds_status {
+    {
         mds_ctx::Span sp(c, KC_CONTACTS);
         HIPCHK(launch_cell_sort(cs, n, c->key_fine.p, c->cells_fine, c->cnt_fine.p, c->start_fine.p, c->end_fine.p,
                                 c->cur_fine.p, c->sorted_fine.p, c->scan_tmp2.p));
         HIPCHK(launch_clear_events(cs, n, c->ev_head.p, c->ev_tail.p, c->n_events.p, c->n_moved.p, c->max_disp.p));
         HIPCHK(launch_contacts(cs, n, c->reps.p, c->rep_of.p, c->pos0.p, c->r_mag.p, c->start_fine.p, c->end_fine.p,
                                c->sorted_fine.p, c->frec.p, c->cand_count.p, c->cand_off.p, c->cand_j.p,
-                               c->cand_cap, c->err_flag.p, c->scan_tmp2.p));
+                               c->cand_cap, c->cand_base.p, c->cand_tmp.p, c->err_flag.p, c->scan_tmp2.p));
         // displacement bound: half the smallest contact-search margin
         double dl = 1e300;
         for (const Rep& R : c->h_rep) dl = std::min(dl, 0.5 * R.margin);
@@ -989,13 +998,21 @@ mds_status prepare_cells(mds_ctx* c) {
                               c->pos0.p, c->ev_head.p, c->ev_tail.p, c->ev_next.p, c->ev_key.p, c->ev_pos.p,
                               2 * c->cand_cap, c->n_events.p, c->max_disp.p, dl, c->moved_list.p, c->n_moved.p,
                               c->err_flag.p));
-        return MDS_OK;
-    };
-    mds_status st;
-    if (split) {
-        if ((st = contacts()) != MDS_OK) return st;
-        HIPCHK(hipEventRecord(c->ev_join2, c->aux));
     }
+    if (split) HIPCHK(hipEventRecord(c->ev_join2, c->aux));
+    return MDS_OK;
+}
+
+// The pair geometry of CompInteractVecs after prepare_positions: cell lists
+// (or the pair lists), contact candidates, sequential overlap correction.
+// `forked`: contact_chain was already started on the third stream.
+mds_status prepare_cells(mds_ctx* c, bool forked = false) {
+    const int64_t n = c->n;
+    hipStream_t s = c->stream;
+    const bool list = c->list_on();
+    const bool split = contacts_split(c);
+    mds_status st;
+    if (split && !forked && (st = contact_chain(c, true)) != MDS_OK) return st;
     {
         mds_ctx::Span sp(c, KC_SETUP);
         if (list) {
@@ -1022,7 +1039,7 @@ mds_status prepare_cells(mds_ctx* c) {
     }
     if (split) {
         HIPCHK(hipStreamWaitEvent(s, c->ev_join2, 0));
-    } else if ((st = contacts()) != MDS_OK) {
+    } else if ((st = contact_chain(c, false)) != MDS_OK) {
         return st;
     }
     {
@@ -1266,11 +1283,15 @@ mds_status fixed_step_body(mds_ctx* c, double h, bool check) {
         // and moments: forked before the cell sort, the pair list and the
         // contact search, so it runs beside them too (the same inputs, so
         // the same results as after them)
-        if ((st = prepare_positions(c)) != MDS_OK) return st;
+        // the contact search needs only the keys: forked first, so it ends
+        // before the cell sort and the pair lists (it ran past them)
+        if ((st = prepare_keys(c)) != MDS_OK) return st;
+        if ((st = contact_chain(c, true)) != MDS_OK) return st;
+        if ((st = prepare_sincos(c)) != MDS_OK) return st;
         if ((st = moments_orig(c, c->mm.p)) != MDS_OK) return st;
         bool forked = false;
         if ((st = kspace_chain(c, true, forked)) != MDS_OK) return st;
-        if ((st = prepare_cells(c)) != MDS_OK) return st;
+        if ((st = prepare_cells(c, true)) != MDS_OK) return st;
         if ((st = moments_sorted(c, c->mm.p)) != MDS_OK) return st;
         if ((st = pair_pass(c, true, forked)) != MDS_OK) return st;
     } else {
@@ -2824,7 +2845,7 @@ int64_t mds_device_bytes(const mds_ctx* c) {
     for (auto* d : {&c->rep_of, &c->cell_rep, &c->key_main, &c->key_fine, &c->cnt_main, &c->start_main, &c->end_main,
                     &c->cur_main, &c->sorted_main, &c->inv_perm, &c->work, &c->cnt_fine, &c->start_fine, &c->end_fine,
                     &c->cur_fine, &c->sorted_fine, &c->cand_count, &c->cand_off, &c->cand_j, &c->ev_head, &c->ev_tail,
-                    &c->ev_next, &c->moved_list})
+                    &c->ev_next, &c->moved_list, &c->cand_base, &c->cand_tmp})
         add(d->n, sizeof(int32_t));
     add(c->rec_pos.n + c->rec_mom.n + c->frec.n, sizeof(double4));
     add(c->rec_pos32.n + c->rec_mom32.n, sizeof(float4));

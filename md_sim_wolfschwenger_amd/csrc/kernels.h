@@ -206,11 +206,13 @@ hipError_t launch_mark_moved(hipStream_t s, const int32_t* n_moved, const int32_
 hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
                              int32_t* work, int32_t* nwork, int32_t cap, int32_t* err);
 // contact search + sequential overlap resolution
-// frec: [n] scratch for the positions and radii in fine-cell order
+// frec: [n] scratch for the positions and radii in fine-cell order;
+// cand_base: [n + 1] (each particle's run in cand_tmp, then the append
+// counter); cand_tmp: [cand_cap] the runs in append order
 hipError_t launch_contacts(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
                            const double* r_mag, const int32_t* fstart, const int32_t* fend, const int32_t* fsorted,
                            double4* frec, int32_t* cand_count, int32_t* cand_off, int32_t* cand_j, int32_t cand_cap,
-                           int32_t* err, int32_t* scan_tmp);
+                           int32_t* cand_base, int32_t* cand_tmp, int32_t* err, int32_t* scan_tmp);
 // one thread per replica walks its candidates in order
 hipError_t launch_resolve(hipStream_t s, int32_t nrep, int64_t n, const Rep* reps, const double* r_mag,
                           const int32_t* cand_off, const int32_t* cand_j, double* pos_cur, const double* pos0,

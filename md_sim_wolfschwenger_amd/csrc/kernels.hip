@@ -370,35 +370,38 @@ __global__ __launch_bounds__(1024) void k_work(int32_t ncells, const int32_t* __
 
 // ------------------------------------------------------------------ contacts
 // Candidate pairs (i < j) whose step-start distance is below the contact
-// distance r_i + r_j + 1e-10 plus a search margin. Pass 0 counts, pass 1
-// writes each particle's partners in ascending j (so the candidate list is in
-// the reference's lexicographic (i, j) order).
-// Half a wavefront per particle: lane t < 27 scans neighbour cell t (dz, dy,
-// dx in increasing order), reading the cell's positions and radii from the
-// fine-cell ordered records (contiguous per cell). Particles are taken in
-// fine-cell order, so neighbouring half-waves share their cells (the lists
-// stay indexed by the original index). In pass 1
-// the lanes' hits go to LDS in cell-scan order (the first MAXC kept, as a
-// serial scan would keep them) and each is stored at its rank by j.
+// distance r_i + r_j + 1e-10 plus a search margin, listed in the reference's
+// lexicographic (i, j) order.
+// One search pass, half a wavefront per particle: lane t < 27 scans
+// neighbour cell t (dz, dy, dx in increasing order), reading the cell's
+// positions and radii from the fine-cell ordered records (contiguous per
+// cell). Particles are taken in fine-cell order, so neighbouring half-waves
+// share their cells (the lists stay indexed by the original index). The
+// lanes' hits go to LDS in cell-scan order (the first MAXC kept, as a serial
+// scan would keep them), are ranked by j and appended as the particle's run
+// to cand_tmp, at a base taken with one atomic (cand_base[n] counts); the
+// count goes to cand_count. After the scan of the counts, k_cand_compact
+// moves each run to its particle's offset, so the list does not depend on
+// the order in which the runs were appended.
 constexpr int CONTACT_MAXC = 32;
 __global__ void k_frec(int64_t n, const int32_t* __restrict__ fsorted, const double* __restrict__ pos0,
-                       const double* __restrict__ r_mag, double4* __restrict__ frec) {
+                       const double* __restrict__ r_mag, double4* __restrict__ frec, int32_t* __restrict__ cand_base) {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s == 0) cand_base[n] = 0;  // the append counter of k_contacts
     if (s >= n) return;
     const int64_t i = fsorted[s];
     if ((uint64_t)i >= (uint64_t)n) return;  // every slot holds a particle; guard the reads anyway
     frec[s] = make_double4(pos0[i], pos0[n + i], pos0[2 * n + i], r_mag[i]);
 }
-template <int PASS>
 __global__ __launch_bounds__(256) void k_contacts(int64_t n, const Rep* __restrict__ reps,
                                                   const int32_t* __restrict__ rep_of, const double* __restrict__ pos0,
                                                   const double* __restrict__ r_mag, const int32_t* __restrict__ fstart,
                                                   const int32_t* __restrict__ fend, const int32_t* __restrict__ fsorted,
                                                   const double4* __restrict__ frec,
-                                                  int32_t* __restrict__ cand_count, const int32_t* __restrict__ cand_off,
-                                                  int32_t* __restrict__ cand_i, int32_t* __restrict__ cand_j,
-                                                  int32_t cap, int32_t* __restrict__ err) {
-    __shared__ int32_t lst[8][PASS ? CONTACT_MAXC : 1];
+                                                  int32_t* __restrict__ cand_count, int32_t* __restrict__ cand_base,
+                                                  int32_t* __restrict__ cand_tmp, int32_t cap,
+                                                  int32_t* __restrict__ err) {
+    __shared__ int32_t lst[8][CONTACT_MAXC];
     const int g = threadIdx.x >> 5, t = threadIdx.x & 31;
     const int64_t k = blockIdx.x * 8LL + g;
     const bool on = k < n;  // uniform over the half-wave
@@ -443,10 +446,6 @@ __global__ __launch_bounds__(256) void k_contacts(int64_t n, const Rep* __restri
     }
     int tot = cl;
     for (int o = 16; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 32);
-    if (!PASS) {
-        if (on && t == 0) cand_count[i] = tot;
-        return;
-    }
     // exclusive prefix of the lanes' counts, in cell-scan order
     int pre = cl;
     for (int o = 1; o < 32; o <<= 1) {
@@ -455,23 +454,50 @@ __global__ __launch_bounds__(256) void k_contacts(int64_t n, const Rep* __restri
     }
     pre -= cl;
     if (on && t == 0 && tot > CONTACT_MAXC) atomicOr(err, 4);
-    for (int32_t s = s0, m = pre; s < s1 && m < CONTACT_MAXC; ++s) {
-        const int32_t j = fsorted[s];
-        if (j > i && hit(s)) lst[g][m++] = j;
-    }
+    if (tot > 0)
+        for (int32_t s = s0, m = pre; s < s1 && m < CONTACT_MAXC; ++s) {
+            const int32_t j = fsorted[s];
+            if (j > i && hit(s)) lst[g][m++] = j;
+        }
     wave_sync();
     const int cn = tot < CONTACT_MAXC ? tot : CONTACT_MAXC;
+    int32_t base = 0;
+    if (on && t == 0) {
+        if (cn > 0) base = atomicAdd(&cand_base[n], cn);
+        cand_count[i] = tot;
+        cand_base[i] = base;
+    }
+    base = __shfl(base, 0, 32);
     if (on && t < cn) {
         const int32_t e = lst[g][t];
         int rank = 0;
         for (int u = 0; u < cn; ++u) rank += lst[g][u] < e;
-        const int32_t off = cand_off[i] + rank;
-        if (off < cap) {
-            cand_i[off] = (int32_t)i;
-            cand_j[off] = e;
-        } else {
+        if (base + rank < cap)
+            cand_tmp[base + rank] = e;
+        else
             atomicOr(err, 8);
+    }
+}
+
+// Each particle's run of candidates from the append area to its offset (the
+// exclusive scan of the counts): partner j, and owner i in the second half.
+__global__ void k_cand_compact(int64_t n, const int32_t* __restrict__ cand_count, const int32_t* __restrict__ cand_off,
+                               const int32_t* __restrict__ cand_base, const int32_t* __restrict__ cand_tmp,
+                               int32_t* __restrict__ cand_i, int32_t* __restrict__ cand_j, int32_t cap,
+                               int32_t* __restrict__ err) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t cn = min(cand_count[i], CONTACT_MAXC);
+    if (cn <= 0) return;
+    const int32_t b = cand_base[i], o = cand_off[i];
+    for (int32_t m = 0; m < cn; ++m) {
+        if (b + m >= cap) break;  // not written (flagged by k_contacts)
+        if (o + m >= cap) {
+            atomicOr(err, 8);
+            break;
         }
+        cand_i[o + m] = (int32_t)i;
+        cand_j[o + m] = cand_tmp[b + m];
     }
 }
 
@@ -3156,19 +3182,19 @@ hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start
 hipError_t launch_contacts(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of, const double* pos0,
                            const double* r_mag, const int32_t* fstart, const int32_t* fend, const int32_t* fsorted,
                            double4* frec, int32_t* cand_count, int32_t* cand_off, int32_t* cand_j, int32_t cand_cap,
-                           int32_t* err, int32_t* scan_tmp) {
-    hipLaunchKernelGGL(k_frec, dim3(grid_for(n, 256)), dim3(256), 0, s, n, fsorted, pos0, r_mag, frec);
+                           int32_t* cand_base, int32_t* cand_tmp, int32_t* err, int32_t* scan_tmp) {
+    hipLaunchKernelGGL(k_frec, dim3(grid_for(n, 256)), dim3(256), 0, s, n, fsorted, pos0, r_mag, frec, cand_base);
     // cand_off has n+1 slots: offsets then the total at [n]; cand_j holds
     // 2*cand_cap ints: partner j then owner i.
     int32_t* cand_i = cand_j + cand_cap;
     const unsigned blocks = (unsigned)((n + 7) / 8);  // 8 particles per block
-    hipLaunchKernelGGL(k_contacts<0>, dim3(blocks), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart,
-                       fend, fsorted, (const double4*)frec, cand_count, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
-                       cand_cap, err);
+    hipLaunchKernelGGL(k_contacts, dim3(blocks), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart, fend, fsorted,
+                       (const double4*)frec, cand_count, cand_base, cand_tmp, cand_cap, err);
     hipError_t e = launch_scan(s, cand_count, (int32_t)n, cand_off, nullptr, nullptr, cand_off + n, scan_tmp);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_contacts<1>, dim3(blocks), dim3(256), 0, s, n, reps, rep_of, pos0, r_mag, fstart,
-                       fend, fsorted, (const double4*)frec, cand_count, cand_off, cand_i, cand_j, cand_cap, err);
+    hipLaunchKernelGGL(k_cand_compact, dim3(grid_for(n, 256)), dim3(256), 0, s, n, (const int32_t*)cand_count,
+                       (const int32_t*)cand_off, (const int32_t*)cand_base, (const int32_t*)cand_tmp, cand_i, cand_j,
+                       cand_cap, err);
     LAUNCH_CHECK();
 }
 

@@ -146,6 +146,9 @@ class Server:
         self.step_time = None  # s per step of the active set, per particle
         self.min_steps = 4     # steps per chunk at least (a chunk has a fixed cost of a few syncs)
         self.times = {"admit": 0.0, "run": 0.0, "retire": 0.0, "idle": 0.0}
+        # chunks the context rolled back and redid (flags), and the slowest chunk
+        self.stats = {"chunks_redone": 0, "slowest_chunk_s": 0.0, "slowest_chunk_steps": 0,
+                      "slowest_chunk_particles": 0}
         self.philox_on = False
 
     def close(self):
@@ -266,6 +269,7 @@ class Server:
             if arbiter is not None:
                 arbiter.begin(lane)
             c0 = clock()
+            redone = self.ctx.chunks_redone()
             try:
                 # the clock argument only matters for a single replica (several keep theirs on the device)
                 self.ctx.run(s, self.done_steps[self.active[0].rid] * self.dt, self.dt)
@@ -275,6 +279,9 @@ class Server:
                     arbiter.end(lane)
             el = clock() - c0
             self.times["run"] += el
+            self.stats["chunks_redone"] += self.ctx.chunks_redone() - redone
+            if el > self.stats["slowest_chunk_s"]:
+                self.stats.update(slowest_chunk_s=el, slowest_chunk_steps=s, slowest_chunk_particles=n_act)
             self.step_time = el / (s * n_act)
             # particle-steps actually taken (finished requests stop on the device)
             total_ps += sum(q.n * min(s, q.steps - self.done_steps[q.rid]) for q in self.active)
@@ -290,6 +297,7 @@ class Server:
         wall = clock() - t0
         out = summary(requests, wall, total_ps, chunks)
         out["seconds"] = dict(self.times)
+        out["stats"] = dict(self.stats)
         return out
 
 
@@ -388,7 +396,7 @@ class LaneServer:
             raise errs[0]
         wall = clock() - t0
         res = summary(requests, wall, sum(o["particle_steps"] for o in out), sum(o["chunks"] for o in out))
-        res["lanes"] = [{"requests": len(p), "chunks": o["chunks"], "seconds": o.get("seconds"),
+        res["lanes"] = [{"requests": len(p), "chunks": o["chunks"], "seconds": o.get("seconds"), "stats": o.get("stats"),
                          "held_s": self.arbiter.held[k] if self.arbiter else 0.0}
                         for k, (p, o) in enumerate(zip(parts, out))]
         return res
