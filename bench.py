@@ -88,7 +88,8 @@ def tune(ctx, info, args):
         ctx.set_tiling(args.reach)
         res = None
     else:
-        rcs = (args.rc_frac,) if args.rc_frac > 0 else (0.15, 0.175, 0.2, 0.225, 0.25, 0.275, 0.3, 0.35)
+        rcs = (args.rc_frac,) if args.rc_frac > 0 else (0.15, 0.175, 0.2, 0.2125, 0.225, 0.2375, 0.25, 0.275, 0.3,
+                                                         0.35)
         reaches = (args.reach,) if args.reach > 0 else (2, 3, 4)
         res = ctx.autotune(rcs, reaches, reps=3)
         log(f"  autotune: {res} -> {ctx.params()} {ctx.tiling()}")
@@ -248,6 +249,7 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, loads=(0.8, 0.5), see
     iso = np.array(serving.isolated_latencies(copy.deepcopy(base), precision, progress=progress))
     out = {"requests": n_req, "steps_range": [1e3 * steps_scale, 1e5 * steps_scale], "n_range": [1e3, 2e5],
            "hbm_budget_gib": budget_gib, "precision": precision,
+           "lane_policy": {k: list(v) if isinstance(v, tuple) else v for k, v in lane_kw.items()},
            "saturated": {"throughput": sat["throughput"], "wall_s": sat["wall_s"], "p50_latency_s":
                          sat["p50_latency_s"], "p99_latency_s": sat["p99_latency_s"]},
            "isolated_p50_s": float(np.percentile(iso, 50)), "isolated_p99_s": float(np.percentile(iso, 99)),
@@ -275,7 +277,8 @@ def serving_leg(precision, n_req, steps_scale, budget_gib, loads=(0.8, 0.5), see
                       "cu_ranges": srv.cu_ranges, "seconds": [l["seconds"] for l in ld["lanes"]],
                       "stats": [l["stats"] for l in ld["lanes"]]},
             "worst": [{"n": q.n, "steps": q.steps, "lane": srv.lane_of(q), "latency_s": float(q.latency),
-                       "wait_s": float(q.admitted - q.arrival), "isolated_s": float(i)}
+                       "wait_s": float(q.admitted - q.arrival), "isolated_s": float(i), "chunks": q.chunks,
+                       "chunks_s": float(q.run_s)}
                       for q, i in sorted(zip(reqs, iso), key=lambda x: -x[0].latency / x[1])[:3]]})
     # the headline figures at the first load (80 %)
     first = out["by_load"][0]
@@ -432,6 +435,10 @@ def main():
     ap.add_argument("--serving-precisions", default="mixed", help="comma-separated: mixed, fp64")
     ap.add_argument("--serving-loads", default="0.8,0.5",
                     help="comma-separated offered loads of the serving leg, as fractions of its saturated rate")
+    ap.add_argument("--serving-max-active", default="0,0,0",
+                    help="per serving lane, at most this many requests at once (0: no limit)")
+    ap.add_argument("--serving-order", default="fifo,fifo,fifo",
+                    help="per serving lane, admission order: fifo (arrival) or sjf (least N x steps first)")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0)
     ap.add_argument("--mz-run", type=int, default=0,
                     help="only run the C3 system for this many steps at r_c = 0.225 L, reach 3, and write its "
@@ -535,7 +542,9 @@ def main():
         for prec in args.serving_precisions.split(","):
             log(f"serving: {args.requests} requests, {prec}")
             serv[prec] = serving_leg(prec, args.requests, args.steps_scale, args.hbm_budget_gib,
-                                     loads=tuple(float(x) for x in args.serving_loads.split(",")))
+                                     loads=tuple(float(x) for x in args.serving_loads.split(",")),
+                                     max_active=tuple(int(x) for x in args.serving_max_active.split(",")),
+                                     sjf=tuple(x == "sjf" for x in args.serving_order.split(",")))
             log(f"  {serv[prec]}")
     cpu = None
     if not args.no_cpu:

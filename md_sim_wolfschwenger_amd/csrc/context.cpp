@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -26,6 +27,63 @@
 using namespace mds;
 
 namespace {
+
+// Device memory and graph executables that a live context gives up are
+// released later, when a context of the device is destroyed: freeing device
+// memory waits for every stream of the device, so a context that re-plans or
+// re-captures while other contexts on the device run (the serving loop's
+// lanes) would otherwise stall until their work drains. Buffers grow by half
+// again at least, so the queued memory stays below twice the live size.
+struct Retired {
+    int dev;
+    void* mem;
+    hipGraphExec_t exec;
+};
+std::mutex g_retired_mu;
+std::vector<Retired> g_retired;
+
+int current_device() {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return dev;
+}
+
+void retire_device_memory(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_retired_mu);
+    g_retired.push_back({current_device(), p, nullptr});
+}
+
+void free_retired(int dev);
+
+// One executable per re-capture: after 1024 of them (a long serving session)
+// everything queued on the device is released, one wait in 1024 captures.
+void retire_graph_exec(hipGraphExec_t e) {
+    if (!e) return;
+    const int dev = current_device();
+    size_t queued = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_retired_mu);
+        g_retired.push_back({dev, nullptr, e});
+        queued = g_retired.size();
+    }
+    if (queued > 1024) free_retired(dev);
+}
+
+void free_retired(int dev) {
+    std::vector<Retired> out;
+    {
+        std::lock_guard<std::mutex> lk(g_retired_mu);
+        auto keep = std::stable_partition(g_retired.begin(), g_retired.end(),
+                                          [dev](const Retired& r) { return r.dev != dev; });
+        out.assign(keep, g_retired.end());
+        g_retired.erase(keep, g_retired.end());
+    }
+    for (const Retired& r : out) {
+        if (r.exec) (void)hipGraphExecDestroy(r.exec);
+        if (r.mem) (void)hipFree(r.mem);
+    }
+}
 
 template <typename T>
 struct DevBuf {
@@ -40,14 +98,19 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // give the memory up without waiting for the device (released later)
+    void retire() {
+        retire_device_memory(p);
+        p = nullptr;
+        n = 0;
+    }
     // capacity >= count; contents are not kept. A buffer that grows takes
-    // half again of its old capacity at least: freeing device memory waits
-    // for the whole device (every other context's work included), so layout
-    // changes (admitted and retired requests) should rarely reallocate.
+    // half again of its old capacity at least, so layout changes (admitted
+    // and retired requests) rarely reallocate.
     hipError_t ensure(size_t count) {
         if (count <= n && p) return hipSuccess;
         if (p) count = std::max(count, n + n / 2);
-        release();
+        retire();
         if (count == 0) count = 1;
         hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
         if (e == hipSuccess) n = count;
@@ -65,8 +128,7 @@ struct Seg {
 // old_n, new stride new_n) from segments of the old one into `spare`, then
 // swap the two; rows not covered by a segment are left for the caller to
 // fill. The old buffer is kept as the next spare, so a layout change that
-// fits the capacity allocates and frees nothing (hipFree waits for the whole
-// device, which would stall other contexts sharing it). Stream-ordered.
+// fits the capacity allocates nothing. Stream-ordered.
 template <typename T>
 hipError_t repack(DevBuf<T>& b, DevBuf<T>& spare, int cols, int64_t old_n, int64_t new_n, int64_t min_n,
                   const std::vector<Seg>& segs, hipStream_t s) {
@@ -74,7 +136,7 @@ hipError_t repack(DevBuf<T>& b, DevBuf<T>& spare, int cols, int64_t old_n, int64
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
     if (spare.n < need) {
-        spare.release();
+        spare.retire();
         if ((e = hipMalloc((void**)&spare.p, need * sizeof(T))) != hipSuccess) return e;
         spare.n = need;
     }
@@ -367,8 +429,10 @@ struct mds_ctx {
         }
         pending.clear();
     }
+    // the executable is released with the context's other retired objects
+    // (its destruction can wait for the whole device)
     void drop_graph() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
+        retire_graph_exec(gexec);
         gexec = nullptr;
     }
     ~mds_ctx() {
@@ -1710,13 +1774,19 @@ static void clear_replicas(mds_ctx* c) {
     c->h_r_mag.clear();
     c->h_mag_mom.clear();
     c->h_r_hydr.clear();
-    c->rep_t.release();
-    c->rep_step.release();
+    c->rep_t.retire();
+    c->rep_step.retire();
     c->n = 0;
     c->nrep = 0;
 }
 
-void mds_destroy(mds_ctx* c) { delete c; }
+void mds_destroy(mds_ctx* c) {
+    if (!c) return;
+    const int dev = c->device;
+    (void)hipSetDevice(dev);
+    delete c;
+    free_retired(dev);
+}
 
 const char* mds_last_error(const mds_ctx* c) { return c ? c->err.c_str() : "null context"; }
 

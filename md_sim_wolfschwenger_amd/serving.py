@@ -40,6 +40,8 @@ class Request:
     mz: float = math.nan
     state: tuple | None = None
     ewald: dict = field(default_factory=dict)
+    chunks: int = 0         # chunks of its lane it was part of
+    run_s: float = 0.0      # wall time of those chunks
 
     @property
     def latency(self):
@@ -125,12 +127,15 @@ class Server:
 
     def __init__(self, precision: str = "fp64", hbm_budget: float = 64 * 2 ** 30, chunk_seconds: float = 0.25,
                  rc_scale: float = 0.225, dt: float = 1e-12, graph: bool = True, base: str = "c1", device: int = 0,
-                 budget: Budget | None = None, reserve_particles: int = 0, kspace_overlap: bool = True):
+                 budget: Budget | None = None, reserve_particles: int = 0, kspace_overlap: bool = True,
+                 max_active: int = 0, sjf: bool = False):
         self.cfg = capi.config_from_string(workload.config_text(base, precision=precision))
         self.precision = precision
         self.hbm_budget = hbm_budget
         self.budget = budget
         self.chunk_seconds = chunk_seconds
+        self.max_active = max_active  # 0: no limit
+        self.sjf = sjf
         self.rc_scale = rc_scale
         self.dt = dt
         self.device = device
@@ -146,6 +151,7 @@ class Server:
         self.step_time = None  # s per step of the active set, per particle
         self.min_steps = 4     # steps per chunk at least (a chunk has a fixed cost of a few syncs)
         self.times = {"admit": 0.0, "run": 0.0, "retire": 0.0, "idle": 0.0}
+        self.trace = None  # a list: admissions, chunks and retirements with their times are appended
         # chunks the context rolled back and redid (flags), and the slowest chunk
         self.stats = {"chunks_redone": 0, "slowest_chunk_s": 0.0, "slowest_chunk_steps": 0,
                       "slowest_chunk_particles": 0}
@@ -233,11 +239,17 @@ class Server:
         used = 0
         while pending or self.active:
             now = clock() - t0
-            # admit arrived requests while they fit
+            # admit arrived requests while they fit (and, with max_active, while
+            # fewer than that many run), in arrival order or the least work first
             batch = []
             used = sum(self.footprint(q) for q in self.active)
-            while pending and pending[0].arrival <= now:
-                f = self.footprint(pending[0])
+            arrived = [q for q in pending if q.arrival <= now]
+            if self.sjf:
+                arrived.sort(key=lambda q: (q.n * q.steps, q.arrival))
+            for q in arrived:
+                if self.max_active and len(self.active) + len(batch) >= self.max_active:
+                    break
+                f = self.footprint(q)
                 alone = not (self.active or batch)
                 if self.budget is not None:
                     # a lane with nothing active may always take one request
@@ -246,11 +258,15 @@ class Server:
                 elif used + f > self.hbm_budget and not alone:
                     break
                 used += f
-                batch.append(pending.pop(0))
+                batch.append(q)
+            for q in batch:
+                pending.remove(q)
             if batch:
                 a0 = clock()
                 self.admit(batch, now)
                 self.times["admit"] += clock() - a0
+                if self.trace is not None:
+                    self.trace.append(("admit", a0 - t0, clock() - t0, [q.n for q in batch]))
             if not self.active:
                 # idle until the next arrival (or until another lane frees memory)
                 wait = pending[0].arrival - (clock() - t0)
@@ -280,6 +296,8 @@ class Server:
             el = clock() - c0
             self.times["run"] += el
             self.stats["chunks_redone"] += self.ctx.chunks_redone() - redone
+            if self.trace is not None:
+                self.trace.append(("chunk", c0 - t0, c0 - t0 + el, s, n_act, len(self.active)))
             if el > self.stats["slowest_chunk_s"]:
                 self.stats.update(slowest_chunk_s=el, slowest_chunk_steps=s, slowest_chunk_particles=n_act)
             self.step_time = el / (s * n_act)
@@ -288,9 +306,13 @@ class Server:
             chunks += 1
             for q in self.active:
                 self.done_steps[q.rid] = min(q.steps, self.done_steps[q.rid] + s)
+                q.chunks += 1
+                q.run_s += el
             r0 = clock()
             self.retire(clock() - t0)
             self.times["retire"] += clock() - r0
+            if self.trace is not None and clock() - r0 > 0.001:
+                self.trace.append(("retire", r0 - t0, clock() - t0))
             if log:
                 log(f"t={clock() - t0:8.3f}s chunk {chunks}: {s} steps x {len(self.active)} left, "
                     f"{n_act} particles, {el / s * 1e3:.2f} ms/step, queue {len(pending)}")
@@ -320,7 +342,8 @@ class LaneServer:
                  bounds: tuple = (6e3, 4e4), chunk_seconds: tuple = (0.01, 0.04, 0.2),
                  cu_fractions: tuple | None = None,
                  cu_spans: tuple | None = ((0.0, 0.125), (0.125, 0.375), (0.375, 1.0)), device: int = 0,
-                 arbiter_hold: float = 0.0, **kw):
+                 arbiter_hold: float = 0.0, max_active: tuple = (0, 0, 0), sjf: tuple = (False, False, False),
+                 **kw):
         """cu_fractions: share of the device's compute units given to each
         lane (disjoint ranges of CU ids); cu_spans: per lane a (first, last)
         fraction of the CU id range, which may overlap (a small-request lane
@@ -329,7 +352,10 @@ class LaneServer:
         1/8, 1/4 and 5/8 of the CUs to themselves: a small system's ~45
         dependent launches per step otherwise each wait for CU slots behind
         the long-running blocks of the large systems' pair kernels (measured
-        7-50x slower steps), at the cost of the saturated throughput."""
+        7-50x slower steps), at the cost of the saturated throughput.
+        max_active, sjf: per lane, at most that many requests at once (0: no
+        limit) and admission of the least work (N x steps) first instead of
+        in arrival order."""
         assert len(chunk_seconds) == len(bounds) + 1
         self.bounds = bounds
         # arbiter_hold > 0: a lane holds its chunks while a lane of smaller
@@ -344,9 +370,10 @@ class LaneServer:
         # queues a process gets, so that streams of different lanes would
         # share a queue and wait on each other
         kw.setdefault("kspace_overlap", False)
+        assert len(max_active) == len(self.bounds) + 1 and len(sjf) == len(self.bounds) + 1
         self.lanes = [Server(precision=precision, hbm_budget=hbm_budget, chunk_seconds=cs, budget=self.budget,
-                             reserve_particles=int(4 * top), device=device, **kw)
-                      for cs, top in zip(chunk_seconds, tops)]
+                             reserve_particles=int(4 * top), device=device, max_active=int(ma), sjf=bool(sj), **kw)
+                      for cs, top, ma, sj in zip(chunk_seconds, tops, max_active, sjf)]
         self.cu_ranges = None
         if cu_fractions:
             assert len(cu_fractions) == len(self.lanes)
