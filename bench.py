@@ -435,9 +435,9 @@ def main():
     ap.add_argument("--serving-precisions", default="mixed", help="comma-separated: mixed, fp64")
     ap.add_argument("--serving-loads", default="0.8,0.5",
                     help="comma-separated offered loads of the serving leg, as fractions of its saturated rate")
-    ap.add_argument("--serving-max-active", default="0,0,0",
+    ap.add_argument("--serving-max-active", default="0,0,3",
                     help="per serving lane, at most this many requests at once (0: no limit)")
-    ap.add_argument("--serving-order", default="fifo,fifo,fifo",
+    ap.add_argument("--serving-order", default="fifo,fifo,sjf",
                     help="per serving lane, admission order: fifo (arrival) or sjf (least N x steps first)")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0)
     ap.add_argument("--mz-run", type=int, default=0,

@@ -188,11 +188,16 @@ mds_status mds_init_coords_from_sim(const char* path, int64_t n, double* pos, do
 /* ----------------------------------------------------------------------- */
 typedef struct mds_ctx mds_ctx;
 
-/* Create a context on HIP device `device`. A context owns two HIP streams:
- * its main stream, on which every call's work is ordered, and a side stream
- * that takes the k-space chain beside the pair kernels within an evaluation
- * (joined by events; see mds_set_kspace_overlap). */
+/* Create a context on HIP device `device`. A context owns its main HIP
+ * stream, on which every call's work is ordered, and, while the k-space
+ * overlap is on (the default; mds_set_kspace_overlap), two helper streams:
+ * one takes the k-space chain beside the pair kernels, one the contact
+ * search beside the cell sort (both joined by events). */
 mds_status mds_create(const mds_config* cfg, int device, mds_ctx** out);
+/* Destroy a context. Device buffers that contexts outgrow and the graph
+ * executables they re-capture are not released while they run (releasing
+ * waits for every stream of the device, which would stall the other
+ * contexts on it); destroying any context of the device releases them. */
 void mds_destroy(mds_ctx* ctx);
 const char* mds_last_error(const mds_ctx* ctx);
 /* Number of HIP devices visible (0 when none); no context needed. */
@@ -427,16 +432,16 @@ mds_status mds_replica_stop_at(mds_ctx* ctx, int32_t index, uint64_t step);
  * several replicas, mds_run's mz_series is steps x replicas, row-major. */
 mds_status mds_replica_mz(mds_ctx* ctx, double* mz);
 /* Capacity for n_particles in total, so that adding and removing replicas
- * below it allocates and frees no device memory (freeing waits for the whole
- * device, which stalls other contexts on it). */
+ * below it reallocates no per-particle buffer. */
 mds_status mds_reserve(mds_ctx* ctx, int64_t n_particles);
 /* With on != 0, mds_run (fixedDt) captures one step into a HIP graph and
  * replays it (re-captured after any change of replicas, parameters or dt). */
 mds_status mds_set_graph(mds_ctx* ctx, int32_t on);
 /* Build the k-space sums S(k) on a second stream of the context while the
- * real-space pair kernels run (default on; off: one stream, in order).
- * Results are identical either way; kernel timing (mds_enable_timing) runs
- * them in order. No counterpart in the reference. */
+ * real-space pair kernels run, and the contact search on a third beside the
+ * cell sort (default on; off: one stream, in order, and the helper streams
+ * are released). Results are identical either way; kernel timing
+ * (mds_enable_timing) runs them in order. No counterpart in the reference. */
 mds_status mds_set_kspace_overlap(mds_ctx* ctx, int32_t on);
 /* Beside the pair kernels, the S(k) build takes at most `waves` wavefronts
  * per SIMD (a grid of blocks striding over its work items; 0: one block per

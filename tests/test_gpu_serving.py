@@ -74,6 +74,34 @@ def test_lanes_each_result_same_alone():
             assert np.array_equal(a, b), (q.rid, rel_err(a, b))
 
 
+def test_capped_lane_admits_least_work_first():
+    """A lane with max_active = 2 and least-work-first admission: never more
+    than two requests in a chunk, requests admitted in the order of their
+    work (N x steps) when all have arrived, each result equal to its solve
+    alone."""
+    print("capped least-work-first lane", flush=True)
+    spec = [(1500, 9, 0.02, 0.03, 11), (600, 5, 0.0, 0.08, 22), (1100, 7, 0.09, 0.05, 33), (800, 4, 0.05, 0.1, 44),
+            (1300, 6, 0.01, 0.06, 55)]
+    reqs = [serving.Request(k, n, st, f, phi, sd, 0.0) for k, (n, st, f, phi, sd) in enumerate(spec)]
+    srv = serving.Server(precision="fp64", chunk_seconds=0.0, max_active=2, sjf=True)
+    srv.trace = []
+    srv.serve(reqs)
+    srv.close()
+    assert max(e[5] for e in srv.trace if e[0] == "chunk") == 2
+    by_work = sorted(reqs, key=lambda q: q.n * q.steps)
+    assert [q.admitted for q in by_work] == sorted(q.admitted for q in reqs)
+    assert by_work[0].admitted == by_work[1].admitted < by_work[2].admitted
+    for q in reqs:
+        alone = copy.deepcopy(q)
+        alone.state = None
+        alone.ewald = {}
+        solo = serving.Server(precision="fp64", chunk_seconds=10.0)
+        solo.serve([alone])
+        solo.close()
+        for a, b in zip(q.state, alone.state):
+            assert np.array_equal(a, b), (q.rid, rel_err(a, b))
+
+
 def test_footprint_estimate_covers_device_bytes():
     srv = serving.Server(precision="fp64")
     reqs = requests()
