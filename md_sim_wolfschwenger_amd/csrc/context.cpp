@@ -79,6 +79,11 @@ void free_retired(int dev) {
         out.assign(keep, g_retired.end());
         g_retired.erase(keep, g_retired.end());
     }
+    if (out.empty()) return;
+    // a queued executable may still be running (the 1024th re-capture
+    // releases its own context's earlier ones): drain the device first
+    (void)hipSetDevice(dev);
+    (void)hipDeviceSynchronize();
     for (const Retired& r : out) {
         if (r.exec) (void)hipGraphExecDestroy(r.exec);
         if (r.mem) (void)hipFree(r.mem);

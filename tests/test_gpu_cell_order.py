@@ -44,10 +44,13 @@ def context(s, precision, reach=0):
     return ctx
 
 
-def test_one_cell_box_equals_the_oracle_and_repeats():
+@pytest.mark.parametrize("n", [1500, 1470])
+def test_one_cell_box_equals_the_oracle_and_repeats(n):
+    """N = 1500 and 1470: an even and an odd number of 64-particle chunks in
+    the ring of windows."""
     import oracle
     from refcfg import derived, parse_ini
-    s = build(1500, 8, 0.45)
+    s = build(n, 8, 0.45)
     print(f"one-cell box: N = {s['n']}, r_c = 0.45 L, n_c = {s['nc']}", flush=True)
     ctx = context(s, "fp64")
     F, B, P, X = ctx.comp_interactions()

@@ -258,6 +258,27 @@ def test_kspace_overlap_changes_nothing(graph):
         assert np.array_equal(a, b)
 
 
+def test_many_recaptures_release_their_graphs():
+    """1040 runs whose dt alternates re-capture the step graph every time;
+    the retired executables are released while the context lives (after
+    1024), and the states equal the eager run of the same steps bit for bit."""
+    d = load("heun_n1000")
+    print("1040 re-captures against eager steps", flush=True)
+    out = []
+    for graph in (True, False):
+        ctx = gpu_ctx(d, fixed=True)
+        ctx.set_state(d["s0_pos_in"], d["s0_mm_in"], d["s0_ea_in"])
+        ctx.use_philox(9, 0)
+        ctx.set_graph(graph)
+        t = 0.0
+        for k in range(1040):
+            t, _, _ = ctx.run(2, t, 1e-12 if k % 2 == 0 else 1.1e-12)
+        out.append(ctx.get_state())
+        ctx.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("n", [1, 2, 5])
 def test_tiny_systems_vs_oracle(n):
     """One, two and five particles (no pair within r_c or two): the k-space
