@@ -445,9 +445,9 @@ mds_status mds_set_graph(mds_ctx* ctx, int32_t on);
 mds_status mds_set_kspace_overlap(mds_ctx* ctx, int32_t on);
 /* Beside the pair kernels, the S(k) build takes at most `waves` wavefronts
  * per SIMD (a grid of blocks striding over its work items; 0: one block per
- * item). A build that takes every CU slot makes the main stream's next
- * kernels wait for one. Results do not depend on it. No counterpart in the
- * reference. */
+ * item; -1, the default: one wave per SIMD for the matrix-core build of the
+ * mixed path, no cap for the FP64 vector build, the faster setting of each
+ * at C3). Results do not depend on it. No counterpart in the reference. */
 mds_status mds_set_kspace_waves(mds_ctx* ctx, int32_t waves);
 /* Restrict the context's work to the compute units set in mask (words x 32
  * bits, bit i = CU i; words = 0 restores all CUs). Contexts that serve

@@ -339,7 +339,8 @@ class Context:
         return {"cell_reach": v[0].value, "ncell": v[1].value, "nstencil": v[2].value}
 
     def set_kspace_waves(self, waves):
-        """Waves per SIMD the S(k) build may hold beside the pair kernels (0: no limit)."""
+        """Waves per SIMD the S(k) build may hold beside the pair kernels (0: no limit;
+        -1, the default: one for the matrix-core build, no limit for the vector build)."""
         self._chk(self._L.mds_set_kspace_waves(self.h, int(waves)), "set_kspace_waves")
 
     def set_pair_list(self, skin, region_scale=1.0):

@@ -287,7 +287,7 @@ struct mds_ctx {
     std::vector<Rep> h_rep;
     int64_t n = 0;  // all particles
     int32_t ncu = 256;          // compute units of the device
-    int32_t kspace_waves = 0;   // waves per SIMD the forked S(k) builds may hold (0: no limit)
+    int32_t kspace_waves = -1;  // waves per SIMD the forked S(k) builds may hold (0: no limit, -1: per kernel)
     int32_t nrep = 0;
     bool state_set = false;
     bool planned = false;
@@ -1174,15 +1174,21 @@ mds_status kspace_chain(mds_ctx* c, bool force_pass, bool& forked) {
             {
                 mds_ctx::Span sp(c, KC_KBUILD);
                 // beside the pair kernels the builds take at most
-                // kspace_waves waves per SIMD (a strided grid); in order, all
+                // kspace_waves waves per SIMD (a strided grid); in order, all.
+                // By default (-1) the matrix-core build holds one wave per
+                // SIMD and the vector build is not capped: C3 mixed step
+                // 4.32 -> 4.10 ms (256 blocks; 64-192 and 384-512 slower),
+                // FP64 step 8.30 -> 8.56 ms if its build were capped too.
                 const int ncu = c->ncu;
-                const bool capped = forked_here && c->kspace_waves > 0;
+                const int wm = c->kspace_waves < 0 ? 1 : c->kspace_waves;
+                const int wv = c->kspace_waves < 0 ? 0 : c->kspace_waves;
                 if (c->any_mfma_build)
                     HIPCHK(launch_kbuild_mfma(ks, n, c->reps.p, c->nrep, c->kbm_items, c->nc_mfma_build, c->rows_m.p,
-                                              c->tab32.p, c->mom.p, c->kpartial.p, capped ? c->kspace_waves * ncu : 0));
+                                              c->tab32.p, c->mom.p, c->kpartial.p,
+                                              forked_here && wm > 0 ? wm * ncu : 0));
                 if (c->any_vec_build)
                     HIPCHK(launch_kbuild(ks, n, c->reps.p, c->nrep, c->kb_items, c->segs.p, c->tab.p, c->mom.p,
-                                         c->kpartial.p, capped ? 4 * c->kspace_waves * ncu : 0));
+                                         c->kpartial.p, forked_here && wv > 0 ? 4 * wv * ncu : 0));
             }
             {
                 mds_ctx::Span sp(c, KC_KREDUCE);
@@ -2618,7 +2624,7 @@ int32_t mds_device_cus(int device) {
 }
 
 mds_status mds_set_kspace_waves(mds_ctx* c, int32_t waves) {
-    if (!c || waves < 0 || waves > 16) return fail(c, MDS_EINVAL, "k-space waves per SIMD must be in [0, 16]");
+    if (!c || waves < -1 || waves > 16) return fail(c, MDS_EINVAL, "k-space waves per SIMD must be in [-1, 16]");
     c->kspace_waves = waves;
     c->drop_graph();
     return MDS_OK;

@@ -258,6 +258,32 @@ def test_kspace_overlap_changes_nothing(graph):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_kspace_build_cap_changes_nothing(precision):
+    """The S(k) build's grid capped at 1 or 3 waves per SIMD (blocks striding
+    over the work items), uncapped, and the default per-kernel cap: the same
+    bits over graph-replayed Philox steps."""
+    cfg, r, rh, L, pos, mm, ea = workload.system("c2", seed=6, n=3000)
+    mu = cfg.sat_mag * 4.0 / 3.0 * cfg.my_pi * r ** 3
+    ew = workload.ewald_for(cfg, mu, rh, L, 0.3)
+    print(f"{precision}: S(k) build caps", flush=True)
+    out = []
+    for waves in (-1, 0, 1, 3):
+        ctx = mds.Context(mds.config_from_string(workload.config_text("c2", seed=6, precision=precision, n=3000)))
+        ctx.set_particles(r, rh, L)
+        ctx.set_ewald(*ew)
+        ctx.set_state(pos, mm, ea)
+        ctx.use_philox(4, 0)
+        ctx.set_kspace_waves(waves)
+        ctx.set_graph(True)
+        _, _, mz = ctx.run(6, 0.0, 1e-12, want_mz=True)
+        out.append((*ctx.get_state(), mz))
+        ctx.close()
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)
+
+
 def test_many_recaptures_release_their_graphs():
     """1040 runs whose dt alternates re-capture the step graph every time;
     the retired executables are released while the context lives (after
