@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -309,6 +310,7 @@ struct mds_ctx {
     double list_skin = 0.005;
     DevBuf<double> pos_build;
     DevBuf<int32_t> list_state, list_mode, lcount, loff, list;
+    DevBuf<int32_t> sort_skip;  // set on the device when every replica keeps its list
     DevBuf<uint64_t> list_maxd2, list_builds;
     DevBuf<int64_t> list_used;
     DevBuf<float4> rec_build32;
@@ -878,6 +880,7 @@ mds_status plan_all(mds_ctx* c) {
     if (c->list_on()) {
         HIPCHK(c->list.ensure(std::max<int64_t>(1, list_words)));
         HIPCHK(c->list_mode.ensure(c->nrep));
+        HIPCHK(c->sort_skip.ensure(1));
         HIPCHK(c->list_maxd2.ensure(c->nrep));
         HIPCHK(c->list_used.ensure(c->nrep));
     }
@@ -918,8 +921,25 @@ mds_status plan_all(mds_ctx* c) {
     return MDS_OK;
 }
 
-// Per-particle scratch of n particles (contents not kept).
+mds_status size_scratch_buffers(mds_ctx* c);
+
+// Per-particle scratch of n particles (contents not kept). Some of it
+// carries state from step to step (the main grid's sort and work items, the
+// pair lists' streams, which a kept list reuses): if any of that moves, the
+// plan is redone, which rebuilds the lists from their build positions.
 mds_status size_scratch(mds_ctx* c) {
+    const void* kept[] = {c->key_main.p, c->sorted_main.p, c->inv_perm.p, c->lcount.p, c->loff.p, c->rec_build32.p};
+    mds_status st = size_scratch_buffers(c);
+    const void* now[] = {c->key_main.p, c->sorted_main.p, c->inv_perm.p, c->lcount.p, c->loff.p, c->rec_build32.p};
+    if (st == MDS_OK && c->planned && !std::equal(std::begin(kept), std::end(kept), std::begin(now))) {
+        c->planned = false;
+        c->geometry_ready = false;
+        c->drop_graph();
+    }
+    return st;
+}
+
+mds_status size_scratch_buffers(mds_ctx* c) {
     const int64_t n = std::max(c->n, c->reserve_n);
     const size_t n3 = 3 * (size_t)n;
     for (auto* b : {&c->pos0, &c->pos_cur, &c->force, &c->field, &c->mom, &c->new_mm, &c->new_ea}) HIPCHK(b->ensure(n3));
@@ -1084,21 +1104,27 @@ mds_status prepare_cells(mds_ctx* c, bool forked = false) {
     if (split && !forked && (st = contact_chain(c, true)) != MDS_OK) return st;
     {
         mds_ctx::Span sp(c, KC_SETUP);
+        // when every replica keeps its list (most steps) the main grid's
+        // keys, sort, order and work items are those of the last step: the
+        // kernels that make them return at once; the records always follow
+        // the current positions
+        const int32_t* skip = nullptr;
         if (list) {
             HIPCHK(launch_list_decide(s, c->nrep, c->reps.p, c->list_state.p, (unsigned long long*)c->list_maxd2.p,
-                                      c->list_mode.p, (unsigned long long*)c->list_builds.p));
+                                      c->list_mode.p, (unsigned long long*)c->list_builds.p, c->sort_skip.p));
+            skip = c->sort_skip.p;
             HIPCHK(launch_list_keys(s, n, c->reps.p, c->rep_of.p, c->list_mode.p, c->pos0.p, c->pos_build.p,
-                                    c->key_main.p));
+                                    c->key_main.p, skip));
         }
         HIPCHK(launch_cell_sort(s, n, c->key_main.p, c->cells_main, c->cnt_main.p, c->start_main.p, c->end_main.p,
-                                c->cur_main.p, c->order_tmp.p, c->scan_tmp.p));
+                                c->cur_main.p, c->order_tmp.p, c->scan_tmp.p, skip));
         HIPCHK(launch_cell_order(s, n, c->cells_main, c->key_main.p, c->start_main.p, c->end_main.p, c->order_tmp.p,
-                                 c->sorted_main.p));
+                                 c->sorted_main.p, skip));
         HIPCHK(launch_build_records(s, n, c->sorted_main.p, c->pos0.p, c->rec_pos.p, c->rec_pos32.p, c->inv_perm.p,
                                     c->reps.p, c->rep_of.p, list ? c->pos_build.p : nullptr, c->list_mode.p,
                                     c->rec_build32.p, c->lcount.p));
         HIPCHK(launch_build_work(s, c->cells_main, c->start_main.p, c->end_main.p, c->work.p, c->nwork.p,
-                                 c->work_cap, c->err_flag.p));
+                                 c->work_cap, c->err_flag.p, skip));
     }
     if (list) {
         mds_ctx::Span sp(c, KC_LIST);

@@ -158,7 +158,9 @@ struct PairParams {
 };
 
 // Launchers: each launches its kernel(s) on stream s and returns the launch status.
-hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v);
+// skip (nullable): a device flag; when it is set the kernels that take it
+// return at once (k_list_decide sets it when every replica keeps its list)
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v, const int32_t* skip = nullptr);
 // peak probe: blocks x 256 threads, each `iters` x 8 FMAs; out holds one
 // value per thread (double or float)
 hipError_t launch_peak_fma(hipStream_t s, bool fp64, int32_t blocks, int32_t iters, void* out);
@@ -176,10 +178,14 @@ hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* p
 // displacement; resets the maximum, marks the state valid), then the build
 // positions of the replicas built afresh (<- pos0) and every particle's
 // main-grid key from its build position.
+// sort_skip: set to 1 when every replica keeps its list (the main grid's
+// keys, sort, order and work items are then those of the last step).
 hipError_t launch_list_decide(hipStream_t s, int32_t nrep, const Rep* reps, int32_t* list_state,
-                              unsigned long long* list_maxd2, int32_t* list_mode, unsigned long long* list_builds);
+                              unsigned long long* list_maxd2, int32_t* list_mode, unsigned long long* list_builds,
+                              int32_t* sort_skip);
 hipError_t launch_list_keys(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of,
-                            const int32_t* list_mode, const double* pos0, double* pos_build, int32_t* key_main);
+                            const int32_t* list_mode, const double* pos0, double* pos_build, int32_t* key_main,
+                            const int32_t* skip = nullptr);
 // Count pass, per-replica offsets, fill pass of the lists of the replicas
 // whose mode is FRESH or RESTORE (grid: max_work blocks). list_used[r]: the
 // replica's words; a list beyond its region sets flag 128, mode BAD and
@@ -190,7 +196,8 @@ hipError_t launch_list_build(hipStream_t s, const Geometry& g, int32_t nrep, int
 // scan_tmp of scan_tiles(cells) ints
 int32_t scan_tiles(int64_t m);
 hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
-                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp);
+                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp,
+                            const int32_t* skip = nullptr);
 // rec_pos[k].w carries a tag: the original index in the low 32 bits. With a
 // pair list (pos_build not null) each record is the step-start position in
 // the periodic image nearest the build position (so a list window's image
@@ -204,7 +211,7 @@ hipError_t launch_build_records(hipStream_t s, int64_t n, const int32_t* sorted_
 hipError_t launch_mark_moved(hipStream_t s, const int32_t* n_moved, const int32_t* moved_list,
                              const int32_t* inv_perm, double4* rec_pos);
 hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end,
-                             int32_t* work, int32_t* nwork, int32_t cap, int32_t* err);
+                             int32_t* work, int32_t* nwork, int32_t cap, int32_t* err, const int32_t* skip = nullptr);
 // contact search + sequential overlap resolution
 // frec: [n] scratch for the positions and radii in fine-cell order;
 // cand_base: [n + 1] (each particle's run in cand_tmp, then the append
@@ -272,7 +279,8 @@ hipError_t launch_observables(hipStream_t s, const Rep* reps, int32_t nrep, cons
 // ordered by value (original index) into sorted_idx, so the pair sums' order
 // does not depend on scheduling; key: each particle's cell (as sorted).
 hipError_t launch_cell_order(hipStream_t s, int64_t n, int32_t ncells, const int32_t* key, const int32_t* start,
-                             const int32_t* end, const int32_t* placed, int32_t* sorted_idx);
+                             const int32_t* end, const int32_t* placed, int32_t* sorted_idx,
+                             const int32_t* skip = nullptr);
 
 // k space
 // FP64 tables for the replicas with a vector k-space kernel, FP32 ones for

@@ -77,7 +77,8 @@ __device__ __forceinline__ T wave_max(T v) {
 }
 
 // ------------------------------------------------------------------ fills
-__global__ void k_fill_i32(int32_t* p, int64_t count, int32_t v) {
+__global__ void k_fill_i32(int32_t* p, int64_t count, int32_t v, const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
@@ -155,20 +156,32 @@ __global__ void k_prepare(int64_t n, const double* __restrict__ pos, double* __r
 }
 
 // Pair list mode of each replica for this step.
-__global__ void k_list_decide(int32_t nrep, const Rep* __restrict__ reps, int32_t* __restrict__ list_state,
-                              unsigned long long* __restrict__ list_maxd2, int32_t* __restrict__ list_mode,
-                              unsigned long long* __restrict__ list_builds) {
-    for (int32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrep; r += gridDim.x * blockDim.x) {
+// One block. sort_skip: 1 when every replica keeps its list, so that the
+// main grid's keys, cell sort, order, work items and lists would come out as
+// the last step left them (the kernels that make them return at once).
+__global__ __launch_bounds__(256) void k_list_decide(int32_t nrep, const Rep* __restrict__ reps,
+                                                     int32_t* __restrict__ list_state,
+                                                     unsigned long long* __restrict__ list_maxd2,
+                                                     int32_t* __restrict__ list_mode,
+                                                     unsigned long long* __restrict__ list_builds,
+                                                     int32_t* __restrict__ sort_skip) {
+    int rebuild = 0;
+    for (int32_t r = threadIdx.x; r < nrep; r += blockDim.x) {
         const int32_t st = list_state[r];
         const double d2 = __longlong_as_double((long long)list_maxd2[r]);
         int32_t m;
         if (st == LIST_INVALID || !(d2 <= reps[r].skin_half2)) m = LIST_FRESH;
         else m = st == LIST_STALE ? LIST_RESTORE : LIST_KEEP;
         list_mode[r] = m;
-        if (m != LIST_KEEP) list_builds[r] += 1;
+        if (m != LIST_KEEP) {
+            list_builds[r] += 1;
+            rebuild = 1;
+        }
         list_state[r] = LIST_VALID;
         list_maxd2[r] = 0ull;
     }
+    rebuild = __syncthreads_or(rebuild);
+    if (threadIdx.x == 0) *sort_skip = rebuild ? 0 : 1;
 }
 
 // Build positions of the replicas built afresh; main-grid keys of every
@@ -176,9 +189,10 @@ __global__ void k_list_decide(int32_t nrep, const Rep* __restrict__ reps, int32_
 // work items come out as they were built).
 __global__ void k_list_keys(int64_t n, const Rep* __restrict__ reps, const int32_t* __restrict__ rep_of,
                             const int32_t* __restrict__ list_mode, const double* __restrict__ pos0,
-                            double* __restrict__ pos_build, int32_t* __restrict__ key_main) {
+                            double* __restrict__ pos_build, int32_t* __restrict__ key_main,
+                            const int32_t* __restrict__ skip) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (skip && *skip)) return;
     const int32_t r = rep_of[i];
     double x, y, z;
     if (list_mode[r] == LIST_FRESH) {
@@ -197,8 +211,10 @@ __global__ void k_list_keys(int64_t n, const Rep* __restrict__ reps, const int32
 }
 
 // Counting sort by key; a negative key leaves the item out of the sort.
-__global__ void k_count(int64_t n, const int32_t* __restrict__ key, int32_t* __restrict__ count) {
+__global__ void k_count(int64_t n, const int32_t* __restrict__ key, int32_t* __restrict__ count,
+                        const int32_t* __restrict__ skip) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (skip && *skip) return;
     if (i < n && key[i] >= 0) atomicAdd(&count[key[i]], 1);
 }
 
@@ -225,8 +241,9 @@ __device__ __forceinline__ int32_t block_excl_scan256(int32_t v, int32_t* sh, in
 }
 
 __global__ __launch_bounds__(256) void k_scan_sums(const int32_t* __restrict__ count, int32_t m,
-                                                   int32_t* __restrict__ tile_sum) {
+                                                   int32_t* __restrict__ tile_sum, const int32_t* __restrict__ skip) {
     __shared__ int32_t sh[256];
+    if (skip && *skip) return;  // uniform over the block
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * 8;
     int32_t v = 0;
     for (int k = 0; k < 8; ++k)
@@ -237,8 +254,9 @@ __global__ __launch_bounds__(256) void k_scan_sums(const int32_t* __restrict__ c
 }
 
 __global__ __launch_bounds__(1024) void k_scan_top(int32_t* __restrict__ tile_sum, int32_t nt,
-                                                   int32_t* __restrict__ total) {
+                                                   int32_t* __restrict__ total, const int32_t* __restrict__ skip) {
     __shared__ int32_t part[1024];
+    if (skip && *skip) return;  // uniform over the block
     const int t = threadIdx.x;
     const int64_t per = (nt + 1023) / 1024;
     const int64_t b = t * per, e = (b + per < nt) ? b + per : nt;
@@ -263,8 +281,10 @@ __global__ __launch_bounds__(1024) void k_scan_top(int32_t* __restrict__ tile_su
 
 __global__ __launch_bounds__(256) void k_scan_apply(const int32_t* __restrict__ count, int32_t m,
                                                     const int32_t* __restrict__ tile_off, int32_t* __restrict__ start,
-                                                    int32_t* __restrict__ end, int32_t* __restrict__ cursor) {
+                                                    int32_t* __restrict__ end, int32_t* __restrict__ cursor,
+                                                    const int32_t* __restrict__ skip) {
     __shared__ int32_t sh[256];
+    if (skip && *skip) return;  // uniform over the block
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * 8;
     int32_t c[8];
     int32_t v = 0;
@@ -286,8 +306,9 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int32_t* __restrict__ 
 }
 
 __global__ void k_place(int64_t n, const int32_t* __restrict__ key, int32_t* __restrict__ cursor,
-                        int32_t* __restrict__ sorted_idx) {
+                        int32_t* __restrict__ sorted_idx, const int32_t* __restrict__ skip) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (skip && *skip) return;
     if (i < n && key[i] >= 0) sorted_idx[atomicAdd(&cursor[key[i]], 1)] = (int32_t)i;
 }
 
@@ -336,8 +357,10 @@ __global__ void k_mark_moved(const int32_t* __restrict__ n_moved, const int32_t*
 // Work items of the pair kernel: one per 64 particles of a cell.
 __global__ __launch_bounds__(1024) void k_work(int32_t ncells, const int32_t* __restrict__ start,
                                                const int32_t* __restrict__ end, int32_t* __restrict__ work,
-                                               int32_t* __restrict__ nwork, int32_t cap, int32_t* __restrict__ err) {
+                                               int32_t* __restrict__ nwork, int32_t cap, int32_t* __restrict__ err,
+                                               const int32_t* __restrict__ skip) {
     __shared__ int32_t part[1024];
+    if (skip && *skip) return;  // uniform over the block
     const int t = threadIdx.x;
     const int64_t per = (ncells + 1023) / 1024;
     const int64_t b = t * per, e = (b + per < ncells) ? b + per : ncells;
@@ -1933,7 +1956,9 @@ __global__ __launch_bounds__(WAVE * ORDER_SPLIT) void k_cell_order(int64_t n, in
                                                                    const int32_t* __restrict__ start,
                                                                    const int32_t* __restrict__ end,
                                                                    const int32_t* __restrict__ placed,
-                                                                   int32_t* __restrict__ sorted_idx) {
+                                                                   int32_t* __restrict__ sorted_idx,
+                                                                   const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;  // uniform over the block
     const int lane = lane_id(), wv = threadIdx.x / WAVE;
     if ((int32_t)blockIdx.x < cell_blocks) {
         for (int32_t c = blockIdx.x * ORDER_SPLIT + wv; c < ncells; c += cell_blocks * ORDER_SPLIT) {
@@ -3059,8 +3084,8 @@ __global__ void k_clear_events(int64_t n, int32_t* __restrict__ head, int32_t* _
 // ------------------------------------------------------------------ launchers
 #define LAUNCH_CHECK() return hipGetLastError()
 
-hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, p, count, v);
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v, const int32_t* skip) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, p, count, v, skip);
     LAUNCH_CHECK();
 }
 // ------------------------------------------------------------ peak probe
@@ -3108,18 +3133,20 @@ hipError_t launch_prepare(hipStream_t s, int64_t n, const double* pos, double* p
 }
 
 hipError_t launch_list_decide(hipStream_t s, int32_t nrep, const Rep* reps, int32_t* list_state,
-                              unsigned long long* list_maxd2, int32_t* list_mode, unsigned long long* list_builds) {
+                              unsigned long long* list_maxd2, int32_t* list_mode, unsigned long long* list_builds,
+                              int32_t* sort_skip) {
     if (nrep <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_list_decide, dim3(grid_for(nrep, 256)), dim3(256), 0, s, nrep, reps, list_state, list_maxd2,
-                       list_mode, list_builds);
+    hipLaunchKernelGGL(k_list_decide, dim3(1), dim3(256), 0, s, nrep, reps, list_state, list_maxd2, list_mode,
+                       list_builds, sort_skip);
     LAUNCH_CHECK();
 }
 
 hipError_t launch_list_keys(hipStream_t s, int64_t n, const Rep* reps, const int32_t* rep_of,
-                            const int32_t* list_mode, const double* pos0, double* pos_build, int32_t* key_main) {
+                            const int32_t* list_mode, const double* pos0, double* pos_build, int32_t* key_main,
+                            const int32_t* skip) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_list_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, n, reps, rep_of, list_mode, pos0,
-                       pos_build, key_main);
+                       pos_build, key_main, skip);
     LAUNCH_CHECK();
 }
 
@@ -3140,21 +3167,23 @@ int32_t scan_tiles(int64_t m) { return (int32_t)((m + SCAN_TILE - 1) / SCAN_TILE
 
 // tmp: scan_tiles(m) ints
 hipError_t launch_scan(hipStream_t s, const int32_t* count, int32_t m, int32_t* start, int32_t* end, int32_t* cursor,
-                       int32_t* total, int32_t* tmp) {
+                       int32_t* total, int32_t* tmp, const int32_t* skip = nullptr) {
     const int32_t nt = std::max(1, scan_tiles(m));
-    hipLaunchKernelGGL(k_scan_sums, dim3(nt), dim3(256), 0, s, count, m, tmp);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, tmp, nt, total);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(256), 0, s, count, m, (const int32_t*)tmp, start, end, cursor);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nt), dim3(256), 0, s, count, m, tmp, skip);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, tmp, nt, total, skip);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(256), 0, s, count, m, (const int32_t*)tmp, start, end, cursor,
+                       skip);
     LAUNCH_CHECK();
 }
 
 hipError_t launch_cell_sort(hipStream_t s, int64_t n, const int32_t* key, int32_t ncells, int32_t* count,
-                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp) {
-    hipError_t e = launch_fill_i32(s, count, ncells, 0);
+                            int32_t* start, int32_t* end, int32_t* cursor, int32_t* sorted_idx, int32_t* scan_tmp,
+                            const int32_t* skip) {
+    hipError_t e = launch_fill_i32(s, count, ncells, 0, skip);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_count, dim3(grid_for(n, 256)), dim3(256), 0, s, n, key, count);
-    if ((e = launch_scan(s, count, ncells, start, end, cursor, nullptr, scan_tmp)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_place, dim3(grid_for(n, 256)), dim3(256), 0, s, n, key, cursor, sorted_idx);
+    hipLaunchKernelGGL(k_count, dim3(grid_for(n, 256)), dim3(256), 0, s, n, key, count, skip);
+    if ((e = launch_scan(s, count, ncells, start, end, cursor, nullptr, scan_tmp, skip)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_place, dim3(grid_for(n, 256)), dim3(256), 0, s, n, key, cursor, sorted_idx, skip);
     LAUNCH_CHECK();
 }
 
@@ -3174,8 +3203,8 @@ hipError_t launch_mark_moved(hipStream_t s, const int32_t* n_moved, const int32_
 }
 
 hipError_t launch_build_work(hipStream_t s, int32_t ncells, const int32_t* start, const int32_t* end, int32_t* work,
-                             int32_t* nwork, int32_t cap, int32_t* err) {
-    hipLaunchKernelGGL(k_work, dim3(1), dim3(1024), 0, s, ncells, start, end, work, nwork, cap, err);
+                             int32_t* nwork, int32_t cap, int32_t* err, const int32_t* skip) {
+    hipLaunchKernelGGL(k_work, dim3(1), dim3(1024), 0, s, ncells, start, end, work, nwork, cap, err, skip);
     LAUNCH_CHECK();
 }
 
@@ -3370,12 +3399,12 @@ hipError_t launch_pairs_out(hipStream_t s, const Geometry& g, const PairOut& out
 }
 
 hipError_t launch_cell_order(hipStream_t s, int64_t n, int32_t ncells, const int32_t* key, const int32_t* start,
-                             const int32_t* end, const int32_t* placed, int32_t* sorted_idx) {
+                             const int32_t* end, const int32_t* placed, int32_t* sorted_idx, const int32_t* skip) {
     if (ncells <= 0 || n <= 0) return hipSuccess;
     const int64_t cb = (ncells + ORDER_SPLIT - 1) / ORDER_SPLIT, pb = (n + WAVE - 1) / WAVE;
     const int32_t cell_blocks = (int32_t)(cb < 4096 ? cb : 4096), chunk_blocks = (int32_t)(pb < 8192 ? pb : 8192);
     hipLaunchKernelGGL(k_cell_order, dim3(cell_blocks + chunk_blocks), dim3(WAVE * ORDER_SPLIT), 0, s, n, ncells,
-                       cell_blocks, key, start, end, placed, sorted_idx);
+                       cell_blocks, key, start, end, placed, sorted_idx, skip);
     LAUNCH_CHECK();
 }
 

@@ -7,7 +7,10 @@ Checks: a run with lists equals the run that searches the cells every
 evaluation (skin 0) to rounding (the same pairs, added in another order)
 while the lists are rebuilt several times along it; a list region too small
 for the list is grown by the checked step that sees the overflow, with the
-same results bit for bit; every build is counted."""
+same results bit for bit; every build is counted; most steps keep every
+list and then skip the main grid's sort, and a reservation that moves the
+buffers in the middle of a run (the kept lists' streams among them) changes
+no bit."""
 import re
 
 import numpy as np
@@ -102,3 +105,23 @@ def test_overflowing_region_grows_and_changes_nothing(c2_small):
         assert np.array_equal(x, y)
     for c in (ref, small, g_ref, g_small):
         c.close()
+
+
+def test_kept_lists_survive_a_reservation_mid_run(c2_small):
+    """Steps that keep every list reuse the last step's sort, order, work
+    items and list streams; mds_reserve of twice the particles between two
+    runs moves those buffers, which must re-plan and rebuild the lists from
+    their build positions: the same bits as the run without it."""
+    print("reserve between two graph-replayed runs", flush=True)
+    a = make(c2_small, "mixed", 0.005, graph=True)
+    b = make(c2_small, "mixed", 0.005, graph=True)
+    ta, dta, _ = a.run(10, 0.0, 1e-12)
+    tb, dtb, _ = b.run(10, 0.0, 1e-12)
+    b.reserve(2 * len(c2_small["r"]))
+    a.run(10, ta, dta)
+    b.run(10, tb, dtb)
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert np.array_equal(x, y)
+    assert b.pair_list_stats(0)["builds"] >= a.pair_list_stats(0)["builds"]
+    a.close()
+    b.close()
