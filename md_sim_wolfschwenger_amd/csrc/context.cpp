@@ -1043,9 +1043,9 @@ mds_status prepare_keys(mds_ctx* c) {
 
 // The sin/cos tables of EvalSinCos: with the keys, everything the k-space
 // chain reads of the geometry.
-mds_status prepare_sincos(mds_ctx* c) {
+mds_status prepare_sincos(mds_ctx* c, hipStream_t s = nullptr) {
     mds_ctx::Span sp(c, KC_SINCOS);
-    HIPCHK(launch_sincos_tables(c->stream, c->n, c->reps.p, c->rep_of.p, c->pos0.p, c->tab.p, c->tab32.p));
+    HIPCHK(launch_sincos_tables(s ? s : c->stream, c->n, c->reps.p, c->rep_of.p, c->pos0.p, c->tab.p, c->tab32.p));
     return MDS_OK;
 }
 
@@ -1191,11 +1191,20 @@ mds_status prepare_geometry(mds_ctx* c, bool check) {
 // The chain reads only the sin/cos tables and the moments, so it is forked
 // onto the side stream (`forked`) and runs beside the cell and pair kernels
 // (in order on one stream when kernels are timed).
-mds_status kspace_chain(mds_ctx* c, bool force_pass, bool& forked) {
+mds_status moments_orig(mds_ctx* c, const double* mm_dev, hipStream_t s = nullptr);
+
+// tables_mm (optional): the chain first computes its own inputs on its
+// stream, the sin/cos tables and the moments of tables_mm.
+mds_status kspace_chain(mds_ctx* c, bool force_pass, bool& forked, const double* tables_mm = nullptr) {
     const int64_t n = c->n;
     hipStream_t s = c->stream;
     const bool forked_here = c->overlap_k && !c->timing;
     auto kspace = [&](hipStream_t ks) -> mds_status {
+        if (tables_mm) {
+            mds_status st = prepare_sincos(c, ks);
+            if (st != MDS_OK) return st;
+            if ((st = moments_orig(c, tables_mm, ks)) != MDS_OK) return st;
+        }
         if (c->nk_all > 0) {
             {
                 mds_ctx::Span sp(c, KC_KBUILD);
@@ -1281,12 +1290,13 @@ mds_status evaluate(mds_ctx* c, bool force_pass) {
 
 // Moments in original order and their per-replica sums (what the k-space
 // chain reads), without the cell-sorted copies.
-mds_status moments_orig(mds_ctx* c, const double* mm_dev) {
+mds_status moments_orig(mds_ctx* c, const double* mm_dev, hipStream_t s) {
     mds_ctx::Span sp(c, KC_MOMENTS);
-    HIPCHK(launch_moments(c->stream, c->n, mm_dev, c->mag_mom.p, nullptr, c->reps.p, c->rep_of.p, c->mom.p, nullptr,
+    if (!s) s = c->stream;
+    HIPCHK(launch_moments(s, c->n, mm_dev, c->mag_mom.p, nullptr, c->reps.p, c->rep_of.p, c->mom.p, nullptr,
                           nullptr));
-    HIPCHK(launch_rep_sum3(c->stream, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), c->n,
-                           c->mom.p, c->red_part.p, c->msum.p));
+    HIPCHK(launch_rep_sum3(s, c->reps.p, c->nrep, c->red_chunks.p, (int32_t)c->h_chunks.size(), c->n, c->mom.p,
+                           c->red_part.p, c->msum.p));
     return MDS_OK;
 }
 
@@ -1380,18 +1390,16 @@ mds_status fixed_step_body(mds_ctx* c, double h, bool check) {
     }
     mds_status st;
     if (!check && c->overlap_k && !c->timing) {
-        // the force pass's k-space chain needs only the step-start tables
-        // and moments: forked before the cell sort, the pair list and the
-        // contact search, so it runs beside them too (the same inputs, so
-        // the same results as after them)
-        // the contact search needs only the keys: forked first, so it ends
-        // before the cell sort and the pair lists (it ran past them)
+        // After the keys, three streams: the contact search (it needs only
+        // the keys, and ends before the cell sort and the pair lists), the
+        // force pass's k-space chain with its own sin/cos tables and
+        // moments (the step-start positions and moments), and the cell sort,
+        // lists and pair kernels. The inputs are the same, so the results
+        // are those of the in-order step.
         if ((st = prepare_keys(c)) != MDS_OK) return st;
         if ((st = contact_chain(c, true)) != MDS_OK) return st;
-        if ((st = prepare_sincos(c)) != MDS_OK) return st;
-        if ((st = moments_orig(c, c->mm.p)) != MDS_OK) return st;
         bool forked = false;
-        if ((st = kspace_chain(c, true, forked)) != MDS_OK) return st;
+        if ((st = kspace_chain(c, true, forked, c->mm.p)) != MDS_OK) return st;
         if ((st = prepare_cells(c, true)) != MDS_OK) return st;
         if ((st = moments_sorted(c, c->mm.p)) != MDS_OK) return st;
         if ((st = pair_pass(c, true, forked)) != MDS_OK) return st;
