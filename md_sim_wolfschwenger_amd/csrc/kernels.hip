@@ -2453,15 +2453,23 @@ __global__ __launch_bounds__(256) void k_kpack(int32_t npk_all, const Rep* __res
         const int nk = R.nk;
         const double* partial = partial_all + R.part_off;
         const int c0 = (R.nchunks * wv) / 4, c1 = (R.nchunks * (wv + 1)) / 4;
+        // both k vectors' (re, im) as one 16-byte load each, unconditional
+        // (an absent one reads slot 0 and is not added), four chunks in
+        // flight; the chunks are still added in order
+        const bool hp = d.kp >= 0, hm = d.km >= 0;
+        const int64_t op = hp ? 2 * (int64_t)d.kp : 0, om = hm ? 2 * (int64_t)d.km : 0;
+#pragma unroll 4
         for (int ch = c0; ch < c1; ++ch) {
             const double* q = partial + (int64_t)ch * nk * 2;
-            if (d.kp >= 0) {
-                cp += q[2 * d.kp];
-                sp += q[2 * d.kp + 1];
+            const double2 vp = *(const double2*)(q + op);
+            const double2 vm = *(const double2*)(q + om);
+            if (hp) {
+                cp += vp.x;
+                sp += vp.y;
             }
-            if (d.km >= 0) {
-                cm += q[2 * d.km];
-                sm += q[2 * d.km + 1];
+            if (hm) {
+                cm += vm.x;
+                sm += vm.y;
             }
         }
     }
