@@ -412,7 +412,8 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=64,
+                    help="timed steps (default: one chunk of the run loop, whose first step is checked eagerly)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rc-frac", type=float, default=0.0, help="r_c / L; 0 = autotune")
     ap.add_argument("--reach", type=int, default=0, help="cell reach of the pair search; 0 = autotune")
