@@ -440,6 +440,9 @@ def main():
                     help="per serving lane, at most this many requests at once (0: no limit)")
     ap.add_argument("--serving-order", default="fifo,fifo,sjf",
                     help="per serving lane, admission order: fifo (arrival) or sjf (least N x steps first)")
+    ap.add_argument("--serving-cu-spans", default="",
+                    help="per serving lane first:last fraction of the CU ids, e.g. 0:0.125,0.125:0.375,0.375:1 "
+                         "(empty: the server's default spans)")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0)
     ap.add_argument("--mz-run", type=int, default=0,
                     help="only run the C3 system for this many steps at r_c = 0.225 L, reach 3, and write its "
@@ -545,7 +548,10 @@ def main():
             serv[prec] = serving_leg(prec, args.requests, args.steps_scale, args.hbm_budget_gib,
                                      loads=tuple(float(x) for x in args.serving_loads.split(",")),
                                      max_active=tuple(int(x) for x in args.serving_max_active.split(",")),
-                                     sjf=tuple(x == "sjf" for x in args.serving_order.split(",")))
+                                     sjf=tuple(x == "sjf" for x in args.serving_order.split(",")),
+                                     cu_spans=(tuple(tuple(float(v) for v in span.split(":"))
+                                                     for span in args.serving_cu_spans.split(","))
+                                               if args.serving_cu_spans else None))
             log(f"  {serv[prec]}")
     cpu = None
     if not args.no_cpu:
