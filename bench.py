@@ -440,6 +440,8 @@ def main():
                     help="per serving lane, at most this many requests at once (0: no limit)")
     ap.add_argument("--serving-order", default="fifo,fifo,sjf",
                     help="per serving lane, admission order: fifo (arrival) or sjf (least N x steps first)")
+    ap.add_argument("--serving-overlap-lanes", default="0,0,0",
+                    help="per serving lane, 1: the k-space chain and contact search on helper streams")
     ap.add_argument("--serving-cu-spans", default="",
                     help="per serving lane first:last fraction of the CU ids, e.g. 0:0.125,0.125:0.375,0.375:1 "
                          "(empty: the server's default spans)")
@@ -549,6 +551,7 @@ def main():
                                      loads=tuple(float(x) for x in args.serving_loads.split(",")),
                                      max_active=tuple(int(x) for x in args.serving_max_active.split(",")),
                                      sjf=tuple(x == "sjf" for x in args.serving_order.split(",")),
+                                     overlap_lanes=tuple(x == "1" for x in args.serving_overlap_lanes.split(",")),
                                      cu_spans=(tuple(tuple(float(v) for v in span.split(":"))
                                                      for span in args.serving_cu_spans.split(","))
                                                if args.serving_cu_spans else None))

@@ -343,7 +343,7 @@ class LaneServer:
                  cu_fractions: tuple | None = None,
                  cu_spans: tuple | None = ((0.0, 0.125), (0.125, 0.375), (0.375, 1.0)), device: int = 0,
                  arbiter_hold: float = 0.0, max_active: tuple = (0, 0, 3), sjf: tuple = (False, False, True),
-                 **kw):
+                 overlap_lanes: tuple | None = None, **kw):
         """cu_fractions: share of the device's compute units given to each
         lane (disjoint ranges of CU ids); cu_spans: per lane a (first, last)
         fraction of the CU id range, which may overlap (a small-request lane
@@ -370,15 +370,19 @@ class LaneServer:
         # each lane reserves room for a few of its largest requests, so
         # admissions and retirements reuse device memory
         tops = list(bounds) + [2e5]
-        # one stream per lane: the lanes already run side by side, and a
-        # second (k-space) stream per lane would exceed the four hardware
-        # queues a process gets, so that streams of different lanes would
-        # share a queue and wait on each other
-        kw.setdefault("kspace_overlap", False)
+        # one stream per lane by default (overlap_lanes): the lanes already
+        # run side by side, and helper streams in every lane would exceed the
+        # four hardware queues a process gets by default, so that streams of
+        # different lanes would share a queue and wait on each other
+        overlap = kw.pop("kspace_overlap", None)
+        if overlap_lanes is None:
+            overlap_lanes = (bool(overlap),) * (len(bounds) + 1)
         assert len(max_active) == len(self.bounds) + 1 and len(sjf) == len(self.bounds) + 1
+        assert len(overlap_lanes) == len(self.bounds) + 1
         self.lanes = [Server(precision=precision, hbm_budget=hbm_budget, chunk_seconds=cs, budget=self.budget,
-                             reserve_particles=int(4 * top), device=device, max_active=int(ma), sjf=bool(sj), **kw)
-                      for cs, top, ma, sj in zip(chunk_seconds, tops, max_active, sjf)]
+                             reserve_particles=int(4 * top), device=device, max_active=int(ma), sjf=bool(sj),
+                             kspace_overlap=bool(ov), **kw)
+                      for cs, top, ma, sj, ov in zip(chunk_seconds, tops, max_active, sjf, overlap_lanes)]
         self.cu_ranges = None
         if cu_fractions:
             assert len(cu_fractions) == len(self.lanes)
