@@ -2692,14 +2692,17 @@ __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __r
     }
     __syncthreads();  // the tables are staged
     // this lane's particles (rows of the products): 16 wv + 4 kk + r
+    // (loads from a valid index for every lane, all issued before the first
+    // wait; lanes past the replica's end take zeros)
     float mxs[4], mys[4], mzs[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int64_t jr = pb + 16 * wv + 4 * kk + r;
-        const int64_t j = R.off + jr;
-        mxs[r] = jr < R.n ? (float)(mom[j] * im) : 0.0f;
-        mys[r] = jr < R.n ? (float)(mom[n + j] * im) : 0.0f;
-        mzs[r] = jr < R.n ? (float)(mom[2 * n + j] * im) : 0.0f;
+        const int64_t j = R.off + (jr < R.n ? jr : 0);
+        const double x = mom[j], y = mom[n + j], z = mom[2 * n + j];
+        mxs[r] = jr < R.n ? (float)(x * im) : 0.0f;
+        mys[r] = jr < R.n ? (float)(y * im) : 0.0f;
+        mzs[r] = jr < R.n ? (float)(z * im) : 0.0f;
     }
     constexpr int NA = FORCE ? 6 : 3;
     double acc[4][NA];
@@ -2721,10 +2724,18 @@ __global__ __launch_bounds__(256) void k_kscatter_mfma(int64_t n, const Rep* __r
 #pragma unroll
         for (int u = 0; u < (NA == 6 ? 6 : 4); ++u) c[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         const bool xs_ = kk & 1;
+        // the next a's entry and phase are loaded before this a's products
+        // (the entry comes from L2: one wait per a otherwise)
+        float4 p_next = tile[(kk >> 1) * 16 + col];
+        float x_next = xrow[0];
         for (int ks = 0; ks < kn; ++ks) {
             const int a = 2 * ks + (kk >> 1);
-            const float xa = xrow[4 * ks];
-            const float4 p = tile[a * 16 + col];
+            const float xa = x_next;
+            const float4 p = p_next;
+            if (ks + 1 < kn) {  // uniform over the wavefront
+                p_next = tile[(a + 2) * 16 + col];
+                x_next = xrow[4 * (ks + 1)];
+            }
             const float fa = (float)a;
             const float e1 = xs_ ? p.w : p.x;          // SU | DV
             const float e2 = xs_ ? -p.y : p.z;         // SV | -DU
