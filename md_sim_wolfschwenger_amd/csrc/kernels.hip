@@ -2358,19 +2358,46 @@ __global__ __launch_bounds__(256) void k_kbuild_mfma(int64_t n_all, const Rep* _
         // stage KBM_SUB particles, four floats at a time: X for a in
         // [a0, a0 + KBM_NX), the y and z tables, the moment; zeros past the
         // chunk's end and past n_c (the tables' own padding up to tw4)
+        // Every load reads a valid address (out-of-range slots read the
+        // chunk's first particle and are replaced by zeros), so a group of
+        // them is issued before the first wait instead of one round trip each.
         __syncthreads();
-        for (int e = tid; e < KBM_SUB * (2 * KBM_NX / 4); e += 256) {
-            const int p = e / (2 * KBM_NX / 4), q = e % (2 * KBM_NX / 4);
-            const int64_t jr = s0 + p;
-            const int o = 2 * a0 + 4 * q;
-            ((float4*)sx[p])[q] = (jr < je && o < tw4) ? *(const float4*)(tab + jr * stride + o)
-                                                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        {
+            constexpr int NXQ = 2 * KBM_NX / 4, TOT = KBM_SUB * NXQ;  // 3 per thread
+            static_assert(TOT % 256 == 0, "whole rounds of the block");
+            float4 v[TOT / 256];
+            bool ok[TOT / 256];
+#pragma unroll
+            for (int u = 0; u < TOT / 256; ++u) {
+                const int e = tid + 256 * u, p = e / NXQ, q = e % NXQ;
+                const int64_t jr = s0 + p;
+                const int o = 2 * a0 + 4 * q;
+                ok[u] = jr < je && o < tw4;
+                v[u] = *(const float4*)(tab + (ok[u] ? jr : jb) * stride + (ok[u] ? o : 0));
+            }
+#pragma unroll
+            for (int u = 0; u < TOT / 256; ++u) {
+                const int e = tid + 256 * u;
+                ((float4*)sx[e / NXQ])[e % NXQ] = ok[u] ? v[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
         }
-        for (int e = tid; e < KBM_SUB * (tw4 / 2); e += 256) {
-            const int p = e / (tw4 / 2), q = e % (tw4 / 2);
-            const int64_t jr = s0 + p;
-            ((float4*)syz)[e] = jr < je ? *(const float4*)(tab + jr * stride + tw4 + 4 * q)
-                                        : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        {
+            const int yq = tw4 / 2, tot = KBM_SUB * yq;
+            for (int e0 = tid; e0 < tot; e0 += 4 * 256) {
+                float4 v[4];
+                bool ok[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = e0 + 256 * u, p = e / yq, q = e % yq;
+                    const int64_t jr = s0 + p;
+                    ok[u] = e < tot && jr < je;
+                    v[u] = *(const float4*)(tab + (ok[u] ? jr : jb) * stride + tw4 + (ok[u] ? 4 * q : 0));
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (e0 + 256 * u < tot)
+                        ((float4*)syz)[e0 + 256 * u] = ok[u] ? v[u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
         }
         if (tid < KBM_SUB * 3) {
             const int p = tid / 3, d = tid % 3;
