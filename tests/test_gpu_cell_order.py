@@ -6,7 +6,8 @@ Their lists are ordered by particle index by the order kernel's large-cell
 path (ranks counted by several wavefronts per chunk of the list). Checks:
 the one-cell box equals the CPU oracle and repeats bit for bit in a fresh
 context; the large cells give the sums of the default grid to rounding, in
-FP64 and mixed precision, over graph-replayed steps too."""
+FP64 and mixed precision, over graph-replayed steps too; and a one-cell
+box stepped with its lists equals the same box on a fine grid."""
 import os
 
 import numpy as np
@@ -101,3 +102,26 @@ def test_large_cells_give_the_default_grid_sums(precision, tol):
     assert np.allclose(mza, mzb, rtol=1e3 * tol, atol=1e-14)
     big.close()
     ref.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("mixed", 1e-5)])
+def test_one_cell_box_steps_match_a_fine_grid(precision, tol):
+    """Graph-replayed Philox steps of the one-cell box (its windows dealt to
+    the wavefronts as a ring, its pair lists rebuilt along the run) against
+    the same box on a fine grid (cell reach 5: twelve cells per side at
+    r_c = 0.4 L)."""
+    s = build(1500, 8, 0.4)
+    print(f"{precision}: one-cell box against reach 5, 30 graph-replayed steps", flush=True)
+    one = context(s, precision)
+    fine = context(s, precision, reach=5)
+    for ctx in (one, fine):
+        ctx.use_philox(3, 0)
+        ctx.set_graph(True)
+    _, _, mza = one.run(30, 0.0, 1e-12, want_mz=True)
+    _, _, mzb = fine.run(30, 0.0, 1e-12, want_mz=True)
+    assert one.tiling()["ncell"] == 1 and fine.tiling()["ncell"] >= 11
+    for x, y in zip(one.get_state(), fine.get_state()):
+        assert rel_err(x, y) < tol
+    assert np.allclose(mza, mzb, rtol=tol, atol=1e-14)
+    one.close()
+    fine.close()
